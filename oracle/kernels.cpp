@@ -1,0 +1,137 @@
+// Oracle kernel registry and program executor — TEST INFRASTRUCTURE ONLY.
+//
+// A program is the reference's replay list (replay.js:85-121 after plan()):
+// one entry per dispatch or copy, each with a kernel name and a 64-word
+// argument slot (A[0..2] thread grid, A[4+n] word offset of parameter n).
+// "_copy" entries copy A[0] words from A[4] to A[5] (replay.js:403-406).
+#include <cstdio>
+#include <stdexcept>
+#include <vector>
+
+#include "oracle.hpp"
+
+namespace orc {
+#include "k_dyn.inc"
+#include "k_adv.inc"
+#include "k_ac.inc"
+#include "k_bc.inc"
+#include "k_turb.inc"
+
+#define KE(n) {#n, &n<float>, &n<double>}
+static const KernelEntry kTable[] = {
+    KE(update_diagnostics_moist), KE(update_diagnostics), KE(stage_tendency_reset), KE(stage_state_reset),
+    KE(massflux_from_state), KE(velocity_to_massflux), KE(div_massflux), KE(divergence3d), KE(buoyancy_apply),
+    KE(rk_update), KE(build_sadv), KE(apply_thterm), KE(turb_add_tendency), KE(turb_zero), KE(tke_advance),
+    KE(ppx_finish), KE(diabatic_unapply), KE(diabatic_rates), KE(pressure_tendency), KE(update_rho_prs),
+    KE(adv_flux_x), KE(adv_flux_y), KE(adv_flux_z), KE(adv_flux_x_weno), KE(adv_flux_y_weno), KE(adv_flux_z_weno),
+    KE(adv_flux_x_stag), KE(adv_flux_y_stag), KE(adv_flux_z_stag), KE(adv_flux_x_weno_w), KE(adv_flux_y_weno_w),
+    KE(adv_flux_z_weno_w), KE(adv_flux_x_weno_uv), KE(adv_flux_y_weno_uv), KE(adv_flux_z_weno_uv), KE(adv_assemble),
+    KE(acoustic_prepare), KE(acoustic_begin), KE(acoustic_uv), KE(acoustic_fpk), KE(acoustic_w_solve), KE(acoustic_pp),
+    KE(acoustic_accumulate), KE(pdef_dum), KE(pdef_apply), KE(bc_periodic_x), KE(bc_periodic_y), KE(bc_open_x),
+    KE(bc_open_y), KE(openbc_uv_tendency), KE(radbc_ew), KE(radbc_ns), KE(turb_openbc_flux),
+    KE(turb_openbc_stress_x), KE(turb_openbc_stress_y), KE(rayleigh_damping), KE(turb_rho_faces),
+    KE(turb_def_centers), KE(turb_def_faces), KE(turb_def_assemble), KE(turb_nm), KE(turb_km_kh),
+    KE(turb_dissipative_heating), KE(turb_gettau), KE(turb_sfcstress), KE(tke_epst), KE(tke_sources),
+    KE(turb_u_tendency), KE(turb_v_tendency), KE(turb_w_tendency), KE(turb_scalar_flux_h),
+    KE(turb_scalar_vert_implicit), KE(tke_flux_h), KE(tke_transport),
+};
+#undef KE
+
+int kernel_count() { return (int)(sizeof(kTable) / sizeof(kTable[0])); }
+const KernelEntry* kernel_at(int i) { return &kTable[i]; }
+const KernelEntry* find_kernel(const std::string& name) {
+  for (const auto& e : kTable)
+    if (name == e.name) return &e;
+  return nullptr;
+}
+
+template <class T>
+static int run_program(uint32_t* w, T* v, const char* const* names, const uint32_t* args, int nops, int first,
+                       int last, char* err, int errlen) {
+  for (int o = first; o < nops && o <= last; ++o) {
+    const uint32_t* A = args + (size_t)o * 64;
+    std::string nm(names[o]);
+    if (nm == "_copy") {
+      const uint32_t n = A[0], src = A[4], dst = A[5];
+#pragma omp parallel for schedule(static)
+      for (int64_t q = 0; q < (int64_t)n; ++q) v[dst + q] = v[src + q];
+      continue;
+    }
+    const KernelEntry* e = find_kernel(nm);
+    if (!e) {
+      std::snprintf(err, errlen, "oracle: no kernel '%s' (op %d)", names[o], o);
+      return -1;
+    }
+    K<T> k{w, v, A};
+    if constexpr (sizeof(T) == 4) e->f32(k);
+    else e->f64(k);
+  }
+  return 0;
+}
+
+// reference region hash (replay.js:408-418): sum over the region of a
+// 32-bit mix of (word ^ index*0x9E3779B1).
+static uint32_t region_hash(const uint32_t* m, size_t len) {
+  uint32_t s = 0;
+  for (size_t i = 0; i < len; ++i) {
+    uint32_t h = (m[i] ^ ((uint32_t)i * 0x9E3779B1u)) * 0x85EBCA6Bu;
+    h ^= h >> 13;
+    h *= 0xC2B2AE35u;
+    h ^= h >> 16;
+    s += h;
+  }
+  return s;
+}
+
+}  // namespace orc
+
+extern "C" {
+
+// Runs ops [first, last] of a program on a float word memory.
+int oracle_run(uint32_t* mem, const char* const* names, const uint32_t* args, int nops, int first, int last, char* err,
+               int errlen) {
+  try {
+    return orc::run_program<float>(mem, reinterpret_cast<float*>(mem), names, args, nops, first, last, err, errlen);
+  } catch (const char* msg) {
+    std::snprintf(err, errlen, "%s", msg);
+    return -2;
+  }
+}
+
+// FP64 build: `mem` keeps the byte parameters, `vals` the double values of
+// every word (same indexing).
+int oracle_run_f64(uint32_t* mem, double* vals, const char* const* names, const uint32_t* args, int nops, int first,
+                   int last, char* err, int errlen) {
+  try {
+    return orc::run_program<double>(mem, vals, names, args, nops, first, last, err, errlen);
+  } catch (const char* msg) {
+    std::snprintf(err, errlen, "%s", msg);
+    return -2;
+  }
+}
+
+uint32_t oracle_region_hash(const uint32_t* mem, uint64_t len) { return orc::region_hash(mem, len); }
+
+uint32_t oracle_fnv1a(const uint32_t* mem, uint64_t len) {
+  uint32_t h = 0x811c9dc5u;
+  for (uint64_t i = 0; i < len; ++i) h = (h ^ mem[i]) * 0x01000193u;
+  return h;
+}
+
+int oracle_kernel_count() { return orc::kernel_count(); }
+const char* oracle_kernel_name(int i) { return orc::kernel_at(i)->name; }
+
+// Elementary functions exposed for the closed-form / brute-force tests.
+float oracle_rr_pow(float x, float p) { return orc::rr_pow(x, p); }
+float oracle_rr_log(float x) { return orc::rr_log(x); }
+float oracle_rr_exp(float x) { return orc::rr_exp(x); }
+float oracle_weno5(float s1, float s2, float s3, float s4, float s5, float weps) {
+  return orc::weno5<float>(s1, s2, s3, s4, s5, weps);
+}
+double oracle_weno5_f64(double s1, double s2, double s3, double s4, double s5, double weps) {
+  return orc::weno5<double>(s1, s2, s3, s4, s5, weps);
+}
+float oracle_flx5(float s1, float s2, float s3, float s4, float s5) { return orc::flx5<float>(s1, s2, s3, s4, s5); }
+float oracle_flx3(float s1, float s2, float s3) { return orc::flx3<float>(s1, s2, s3); }
+float oracle_upstrpd(float s1, float s2, float s3, float weps) { return orc::upstrpd<float>(s1, s2, s3, weps); }
+}
