@@ -1,0 +1,158 @@
+"""Throughput of the CM1 large step on one MI355X (Mpt-steps/s, FP32).
+
+Workload: the 192x192x80 supercell-hour configuration (dx = dy = 1 km,
+dz = 250 m, dt = 6 s, nsound = 6 -> 11 acoustic substeps per large step,
+1.5-order TKE, Rayleigh damping above 15 km, periodic x/y, WENO5 scalars
+th qv qc and 5th-order flux-form momentum, WK82 sounding + quarter-circle
+hodograph + 2 K bubble, synthetic).  The state is resident in HBM before the
+timed region; a step is one full large step (a replay of its captured HIP
+graph).
+
+Prints one JSON line.  `roofline` is for the kernel with the largest device
+time per step, timed live with HIP events on the stream it runs on;
+`cpu_baseline` times the oracle on this host's cores on a bounded sample.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "thunderengine-web_amd"))
+
+import numpy as np  # noqa: E402
+
+import te_host  # noqa: E402
+
+PEAK_HBM_GBS = 8000.0
+# Algorithmic bytes of one large step per interior point for this workload:
+# 4 B x [18 + 3 (12 + 3*3 + 3 + 4) + 23*11 + 8] (fused design, TKE, 3 scalars)
+B_ALG_STEP = 1452.0
+
+WORKLOAD = dict(nx=192, ny=192, nz=80, dx=1000.0, dy=1000.0, dz=250.0, dt=6.0, nsound=6, moist=1, species=3,
+                weno_scalars=1, weno_momentum=0, tke=1, rayleigh=1, rayleigh_zd=15000.0)
+
+
+def kernel_io():
+    """kernel -> [(param index, written)] of its buffer parameters."""
+    p = os.path.join(ROOT, "thunderengine-web_amd", "kernel_io.json")
+    return json.load(open(p))
+
+
+def alg_bytes_per_launch(run, names, A, op):
+    """Distinct arena fields a dispatch reads plus those it writes, over its
+    launch grid, x 4 B (each field touched once, as a fused kernel would)."""
+    io = kernel_io().get(names[op])
+    if io is None:
+        return None
+    a0 = run.field_offset(run.fields[0])
+    end = a0 + run.field_words * len(run.fields)
+    pts = int(A[op, 0]) * int(A[op, 1]) * int(A[op, 2])
+    n = 0
+    for idx, read, written in io:
+        off = int(A[op, 4 + idx])
+        if a0 <= off < end:
+            n += int(read) + int(written)
+    return 4.0 * n * pts
+
+
+def profiles_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC
+    summary (profiles/<kernel>_pmc_192x192x80.txt), or None."""
+    p = os.path.join(ROOT, "profiles", "%s_pmc_192x192x80.txt" % kernel)
+    if not os.path.exists(p):
+        return None
+    for line in open(p):
+        if line.startswith("traffic_bytes_per_launch"):
+            return float(line.split("=")[1])
+    return None
+
+
+def cpu_baseline(cfg, seconds=15.0):
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib
+    import runs
+    ref = runs.OracleRun(cfg)
+    ref.step(1)  # warm-up
+    n, t0 = 0, time.perf_counter()
+    while True:
+        ref.step(1)
+        n += 1
+        dt = time.perf_counter() - t0
+        if dt > seconds or n >= 20:
+            break
+    pts = cfg.nx * cfg.ny * cfg.nz
+    return {"value": round(pts * n / dt / 1e6, 3), "unit": "Mpt-steps/s", "cores": int(oracle_lib.lib().oracle_threads()),
+            "kind": "port",
+            "sample": "%d large steps of the same 192x192x80 workload after 1 warm-up step, oracle (C++ -O2, OpenMP)" % n}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--profile-steps", type=int, default=3)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    cfg = te_host.config(**WORKLOAD)
+    run = te_host.Run(cfg)
+    run.init_state()
+    pts = cfg.nx * cfg.ny * cfg.nz
+    if args.warmup > 0:
+        run.step(args.warmup)
+    run.sync()
+    t0 = time.perf_counter()
+    run.step(args.steps)
+    run.sync()
+    wall = time.perf_counter() - t0
+    dev_ms = run.last_run_ms()
+    ms_per_step = wall * 1e3 / args.steps
+    value = pts * args.steps / wall / 1e6
+
+    # dominant kernel: largest device time per step, per-op HIP events
+    names, A = run.program()
+    op_ms = run.profile_ops(args.profile_steps) / args.profile_steps
+    per_kernel = {}
+    for i, n in enumerate(names):
+        t, c = per_kernel.get(n, (0.0, 0))
+        per_kernel[n] = (t + float(op_ms[i]), c + 1)
+    dom = max(per_kernel, key=lambda k: per_kernel[k][0])
+    dom_ops = [i for i, n in enumerate(names) if n == dom]
+    avg_ms = per_kernel[dom][0] / len(dom_ops)
+    alg = [alg_bytes_per_launch(run, names, A, i) for i in dom_ops]
+    alg_b = float(np.mean(alg)) if all(x is not None for x in alg) else None
+    achieved = alg_b / (avg_ms * 1e-3) / 1e9 if alg_b else None
+    traffic = profiles_traffic(dom)
+    roof = {"bound": "hbm", "kernel": dom, "launches_per_step": len(dom_ops), "avg_launch_ms": round(avg_ms, 5),
+            "alg_bytes_per_launch": alg_b,
+            "achieved": round(achieved, 1) if achieved else None, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+            "frac": round(achieved / PEAK_HBM_GBS, 4) if achieved else None, "traffic": traffic}
+    step_gbs = value * 1e6 * B_ALG_STEP / 1e9
+
+    out = {
+        "metric": "Mpt-steps/s FP32 at 192^2x80 (%HBM peak)",
+        "value": round(value, 2), "unit": "Mpt-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4), "device_ms_per_step": round(dev_ms / args.steps, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "supercell hour 192x192x80 dz=250m dt=6s nsound=6 TKE (synthetic WK82 + bubble)",
+                   "nx": cfg.nx, "ny": cfg.ny, "nz": cfg.nz, "substeps_per_step": 11, "ops_per_step": len(names),
+                   "parallelism": "single GPU"},
+        "step_hbm": {"alg_bytes_per_pt_step": B_ALG_STEP, "achieved_GBs": round(step_gbs, 1),
+                     "frac": round(step_gbs / PEAK_HBM_GBS, 4)},
+        "roofline": roof,
+        "top_kernels_ms_per_step": {k: round(v[0], 4) for k, v in
+                                    sorted(per_kernel.items(), key=lambda kv: -kv[1][0])[:8]},
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(cfg)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,169 @@
+/* ThunderEngine MI355X large step — C ABI.
+ *
+ * The drop-in boundary of the CM1 dynamics large step.  Two ways in:
+ *
+ *  1. Program interface: executes a planned command tape, the job of the
+ *     reference's WebGPU replayer (web/replay.js:36-396, class Replayer).  A
+ *     program is the replayer's op list after plan(): one flat word memory M
+ *     and, per dispatch, a kernel name and a 64-word argument slot
+ *     (A[0..2] thread grid, A[4+n] word offset of parameter n, replay.js:7-11,
+ *     258-288); "_copy" ops move A[0] words from A[4] to A[5]
+ *     (replay.js:403-406).
+ *  2. Config interface: builds the program itself from the namelist flags of
+ *     the hot path (the reference's buildRun()/TimeStep.encode,
+ *     web/core/driver.js:23-75 and web/core/timestep.js:797-2201), lays out the
+ *     FieldArena (web/core/grid.js:322-346) and fills the t=0 state.
+ *
+ * Conventions: every call returns TE_OK (0) or a negative status; the message
+ * of the last failure is te_last_error(ctx).  The caller owns host buffers, the
+ * context owns device memory and one HIP stream; calls are stream-ordered and
+ * not thread-safe on one context.  Nothing here takes torch types.
+ */
+#ifndef TE_H
+#define TE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TE_OK 0
+#define TE_ERR_ARG (-1)
+#define TE_ERR_HIP (-2)
+#define TE_ERR_UNSUPPORTED (-3)
+#define TE_ERR_STATE (-4)
+
+typedef struct te_ctx te_ctx;
+
+/* ---------------------------------------------------------------- programs */
+
+/* Replayer(device, tape, kernels, sources) + plan() + build()
+ * (web/replay.js:36-66,180-209): allocates mem_words words of device memory
+ * (zeroed) and binds the op list.  names[i] is a reference kernel name or
+ * "_copy"; args holds nops*64 words. */
+int te_program_create(uint64_t mem_words, int32_t nops, const char* const* names, const uint32_t* args,
+                      te_ctx** out);
+
+/* Replayer.upload / writeWords (web/replay.js:216-229,301): host -> M. */
+int te_upload_words(te_ctx* ctx, uint64_t word_off, const uint32_t* src, uint64_t nwords);
+
+/* Replayer.read (web/replay.js:368-383): M -> host (synchronous). */
+int te_download_words(te_ctx* ctx, uint64_t word_off, uint32_t* dst, uint64_t nwords);
+
+/* Replayer.encodeSteps(enc, n, false, lastOp) (web/replay.js:335-366): runs ops
+ * [first_op, last_op] of the program nsteps times (last_op < 0: to the end).
+ * Full steps replay a captured HIP graph. */
+int te_run(te_ctx* ctx, int32_t nsteps, int32_t first_op, int32_t last_op);
+
+/* Runs nsteps full steps eagerly with a HIP event after every op on the
+ * context's stream and adds each op's device duration (ms) into op_ms
+ * (te_op_count floats, cleared first).  Profiling aid: no counterpart in the
+ * reference, whose replayer only times whole steps. */
+int te_profile_ops(te_ctx* ctx, int32_t nsteps, float* op_ms);
+
+/* Waits for the context's stream. */
+int te_sync(te_ctx* ctx);
+
+/* region_hash of the reference (web/replay.js:408-418) over nwords of M. */
+int te_region_hash(te_ctx* ctx, uint64_t word_off, uint64_t nwords, uint32_t* out);
+
+/* word-wise FNV-1a (web/view.js:1064-1068) over nwords of M. */
+int te_fnv1a(te_ctx* ctx, uint64_t word_off, uint64_t nwords, uint32_t* out);
+
+/* Total words of M, number of ops, and op i as (name, 64 argument words). */
+uint64_t te_memory_words(const te_ctx* ctx);
+int32_t te_op_count(const te_ctx* ctx);
+int te_op_get(const te_ctx* ctx, int32_t i, const char** name, uint32_t* args64);
+
+/* Kernels the library implements (the pipelines of data/kernels/kernels.json
+ * that belong to the large step, plus "_copy"): count and name i. */
+int32_t te_kernel_count(void);
+const char* te_kernel_name(int32_t i);
+
+/* Device duration of the last te_run in milliseconds (HIP events on the
+ * context's stream). */
+int te_last_run_ms(te_ctx* ctx, float* ms);
+
+const char* te_last_error(const te_ctx* ctx);
+void te_destroy(te_ctx* ctx);
+
+/* ------------------------------------------------------------------ config */
+
+/* Namelist subset of the hot path (web/core/config.js:113-207 field rules;
+ * web/core/scenario.js:138-226 flag wiring).  Scalars are th, or th qv qc
+ * when moist (qv, qc advected without microphysics). */
+typedef struct te_config {
+  int32_t nx, ny, nz;
+  float dx, dy, dz;        /* m; dz is the nominal spacing when zf is given */
+  const float* zf;         /* nz+1 face heights (m), or NULL for uniform dz */
+  float dt;                /* large step (s) */
+  int32_t nsound;          /* acoustic parameter: stage loops n1, n/2, n (timestep.js:73-83) */
+  int32_t moist;           /* imoist */
+  int32_t open_x, open_y;  /* wbc/ebc, sbc/nbc = 2 (open); 0 = periodic */
+  int32_t weno_scalars;    /* advwenos = 2 */
+  int32_t weno_momentum;   /* advwenov = 2 */
+  int32_t tke;             /* sgsmodel = 1 (1.5-order TKE, LES) */
+  int32_t explicit_vdiff;  /* doimpl = 0 */
+  int32_t dissipative_heating; /* idiss = 1 */
+  int32_t rayleigh;        /* irdamp = 1 */
+  float rayleigh_zd;       /* zd (m) */
+  float rayleigh_alpha;    /* rdalpha (1/s) */
+  float alph, kdiv, smeps; /* acoustic parameters (driver.js:28) */
+  float umove, vmove;      /* domain translation (m/s) */
+  /* initial state (web/core/base.js WK82 sounding, quarter-circle hodograph, cos^2 bubble) */
+  float th_sfc, th_trop, t_trop, z_trop, qv_pbl;  /* K, K, K, m, kg/kg */
+  float hodo_radius, hodo_depth, shear_umax, shear_depth; /* m/s, m, m/s, m */
+  float bubble_amp, bubble_rh, bubble_rv, bubble_zc;      /* K, m, m, m */
+  float noise_sigma;       /* K, N(0, sigma) theta' noise */
+  uint64_t seed;
+  int32_t initial_tke;     /* init3d testcase of the t=0 TKE profile: 0 none, 3, 4, 5, 9, 11
+                              (scenario.js:347-369) */
+  /* encoder options used to reproduce recorded reference configurations */
+  int32_t species;         /* moist scalars: 3 = th qv qc, 7 = + qr qi qs qg, 11 = + nci ncs ncr ncg */
+  int32_t microphysics;    /* 0 = none; 5 = Morrison call sites (encoded only, not executable) */
+  int32_t mass_conservation;          /* apmasscon = 1 (encoded only, not executable) */
+  int32_t cache_moisture_turbulence;  /* moisture SGS fluxes computed once per step */
+  int32_t n_arena_fields;             /* > 0: explicit FieldArena field list (sorted by the caller) */
+  const char* const* arena_fields;
+} te_config;
+
+/* Fills a config with the synthetic 64x64x40 case defaults (dx=dy=1 km,
+ * dz=500 m, dt=6 s, nsound=6, moist, periodic, WENO scalars, WK82). */
+void te_config_defaults(te_config* cfg);
+
+/* TimeStep.encode (timestep.js:797-2201) without a device: grid, profiles,
+ * arena layout and the op list of one large step (readable with te_op_get;
+ * ops of kernels outside this path keep their names but cannot run). */
+int te_encode(const te_config* cfg, te_ctx** out);
+
+/* buildRun(conf) (driver.js:23-75): te_encode plus device memory with the
+ * profiles and parameter bytes in place; the arena state is zero. */
+int te_create(const te_config* cfg, te_ctx** out);
+
+/* SupercellScenario.fill (web/core/scenario.js:234-296): t=0 state into the
+ * arena and uploaded (for a te_encode context: into the host image only,
+ * readable with te_download_words). */
+int te_init_state(te_ctx* ctx);
+
+/* run.encodeStep() + encodeSteps (driver.js:53): n large steps. */
+int te_step(te_ctx* ctx, int32_t nsteps);
+
+/* FieldArena.handles (grid.js:329-330): word offset of a field in M, words
+ * per field slot, and the arena words count. */
+int te_field_offset(const te_ctx* ctx, const char* name, uint64_t* word_off);
+uint64_t te_field_stride_words(const te_ctx* ctx);
+int32_t te_field_count(const te_ctx* ctx);
+const char* te_field_name(const te_ctx* ctx, int32_t i);
+
+/* Word offset of a named profile / parameter region of an encoded program
+ * (c1 c2 mh mf zf rho0 rr0 rf0 th0 pi0 rth0 qv0 qc0 qi0 tauh tauf u0 v0
+ * hmetric radbcw radbce radbcs radbcn sfc dims). */
+int te_region_offset(const te_ctx* ctx, const char* name, uint64_t* word_off);
+
+/* Grid of a config context: nx ny nz strideJ strideK. */
+int te_grid(const te_ctx* ctx, int32_t out5[5]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

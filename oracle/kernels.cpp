@@ -4,6 +4,9 @@
 // one entry per dispatch or copy, each with a kernel name and a 64-word
 // argument slot (A[0..2] thread grid, A[4+n] word offset of parameter n).
 // "_copy" entries copy A[0] words from A[4] to A[5] (replay.js:403-406).
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include <cstdio>
 #include <stdexcept>
 #include <vector>
@@ -86,6 +89,15 @@ static uint32_t region_hash(const uint32_t* m, size_t len) {
 }  // namespace orc
 
 extern "C" {
+
+// OpenMP threads the oracle's loops use.
+int oracle_threads() {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
 
 // Runs ops [first, last] of a program on a float word memory.
 int oracle_run(uint32_t* mem, const char* const* names, const uint32_t* args, int nops, int first, int last, char* err,

@@ -1,0 +1,77 @@
+"""Region-hash check of one executor against the supercell recording.
+
+The recording (tests/golden/supercell) holds, for every dispatch of step 1 of
+the reference's 120x120x40 supercell run and for every buffer parameter the
+kernel writes, the region hash of that parameter's word range after the
+dispatch (web/replay.js:100-118 assigns the slots; replay.js:408-418 is the
+hash).  An executor passes when every slot it can reproduce matches.
+
+Slots that cannot be reproduced are skipped by taint tracking: a kernel
+outside this path (microphysics, mass conservation) taints every region it
+writes, and a region read or written by a dispatch that reads a tainted
+region becomes tainted too.  `tend` and `fpk` are skipped as well: those
+regions hold scratch values whose recorded hashes are known to depend on the
+reference's own dispatch ordering of unrelated scratch writes.
+"""
+import numpy as np
+
+import tapeplan
+
+SKIP_REGIONS = ("tend", "fpk")
+
+
+def plan():
+    tape, params = tapeplan.load_supercell()
+    return tapeplan.Plan(tape, params)
+
+
+def check(P, run_op, region_hash, have):
+    """run_op(i) executes dispatch i of P.ops; region_hash(off, n) hashes M.
+
+    Returns (ok, bad, skipped_ops, mismatches[(op, kernel, region, slot)]).
+    """
+    ref = tapeplan.ref_hashes()
+    slots_by_op = {}
+    for s, (oi, idx, off, ln) in enumerate(P.slots):
+        slots_by_op.setdefault(oi, []).append((s, off, ln))
+    tainted = set()
+    ok = bad = skipped = 0
+    mism = []
+    for oi, o in enumerate(P.ops):
+        nm = o["name"]
+        if nm == "_copy":
+            run_op(oi)
+            continue
+        reads = {P.field_of(off) for off in o["bind"].values()}
+        if nm not in have:
+            for s, off, ln in slots_by_op.get(oi, []):
+                tainted.add(P.field_of(off))
+            skipped += 1
+            continue
+        run_op(oi)
+        dirty = bool(reads & tainted)
+        for s, off, ln in slots_by_op.get(oi, []):
+            f = P.field_of(off)
+            if dirty:
+                tainted.add(f)
+            if f in tainted or f in SKIP_REGIONS:
+                continue
+            if region_hash(off, ln) == int(ref[s]):
+                ok += 1
+            else:
+                bad += 1
+                tainted.add(f)
+                mism.append((oi, nm, f, s))
+    return ok, bad, skipped, mism
+
+
+def program_for(P, have):
+    """(names, args) of P with kernels outside `have` replaced by empty copies."""
+    names, A = P.args()
+    names = list(names)
+    A = A.copy()
+    for i, n in enumerate(names):
+        if n not in have and n != "_copy":
+            names[i] = "_copy"
+            A[i, :] = 0
+    return names, A

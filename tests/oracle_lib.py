@@ -36,6 +36,7 @@ def lib():
         L.oracle_weno5_f64.argtypes = [ctypes.c_double] * 6
         L.oracle_weno5_f64.restype = ctypes.c_double
         L.oracle_kernel_name.restype = ctypes.c_char_p
+        L.oracle_threads.restype = ctypes.c_int
         _lib = L
     return _lib
 

@@ -1,0 +1,59 @@
+"""Programs of the encoder run in the oracle (test infrastructure).
+
+A configuration is encoded by libte.so without a device (te_encode), its t=0
+state filled into the host image (te_init_state), and the same program and
+memory image are then run by the oracle; GPU tests run them through te_create
+and compare word for word.
+"""
+import numpy as np
+
+import oracle_lib as ol
+import te_host
+
+FRONT = 1 << 16  # words below M that a stencil at a region's start may touch
+
+
+class OracleRun:
+    def __init__(self, cfg):
+        self.enc = te_host.Run(cfg, encode_only=True)
+        self.enc.init_state()
+        e = self.enc
+        self.names, self.args = e.program()
+        self.prog = ol.Program(self.names, self.args)
+        self.words = e.mem_words
+        back = e.sJ * e.sK * (e.nz + 4) + 4096
+        self._buf = np.zeros(FRONT + self.words + back, dtype=np.uint32)
+        self.M = self._buf[FRONT:]
+        self.M[: self.words] = e.read(0, self.words)
+        self.M0 = self.M[: self.words].copy()
+
+    def step(self, n=1):
+        for _ in range(n):
+            self.prog.run(self.M)
+
+    def field(self, name):
+        e = self.enc
+        o = e.field_offset(name)
+        return self.M[o: o + e.sJ * e.sK * (e.nz + 1)].view(np.float32).reshape(e.nz + 1, e.sK, e.sJ)
+
+    def interior(self, name):
+        e = self.enc
+        return self.field(name)[: e.nz, 3: 3 + e.ny, 3: 3 + e.nx]
+
+
+def first_difference(run, a, b):
+    """Names the first arena field where word arrays a and b differ."""
+    e = run.enc if hasattr(run, "enc") else run
+    diff = np.nonzero(a != b)[0]
+    if diff.size == 0:
+        return None
+    w = int(diff[0])
+    a0 = e.field_offset(e.fields[0])
+    if w >= a0:
+        q = (w - a0) // e.field_words
+        if q < len(e.fields):
+            r = w - a0 - q * e.field_words
+            k, rem = divmod(r, e.sJ * e.sK)
+            j, i = divmod(rem, e.sJ)
+            return "%s[k=%d j=%d i=%d] (%d words differ)" % (e.fields[q], k, j - 3, i - 3, diff.size)
+    return "word %d (%d words differ)" % (w, diff.size)

@@ -1,0 +1,126 @@
+"""The C++ encoder (te_encode) against the reference's recorded tape (CPU).
+
+The supercell recording holds the reference encoder's output for step 1
+(web/core/timestep.js TimeStep.encode).  te_encode with the same options must
+produce the same dispatch sequence, launch grids and parameter bindings
+(arena field + word offset, profile/parameter buffer + offset, and the
+content of every setBytes payload) for every dispatch of the dynamics path,
+i.e. up to the microphysics tail, whose kernels are not on this path.
+"""
+import numpy as np
+import pytest
+
+import golden_check
+import te_host
+
+TERRAIN = range(20, 30)  # terrain bindings (always the zero terrain buffer)
+REGIONS = ["c1", "c2", "mh", "mf", "zf", "rho0", "rr0", "rf0", "th0", "pi0", "rth0", "qv0", "qc0", "qi0",
+           "tauh", "tauf", "u0", "v0", "hmetric", "radbcw", "radbce", "radbcs", "radbcn", "sfc",
+           "mass_rows", "mass_levels", "mass_ref", "mass_out"]
+MASS_LABEL = {"mass_rows": "buf30", "mass_levels": "buf31", "mass_ref": "buf32", "mass_out": "buf33"}
+
+
+def supercell_config(fields, **kw):
+    base = dict(nx=120, ny=120, nz=40, dx=1000., dy=1000., dz=500., dt=6., nsound=0, moist=1, species=11,
+                open_x=1, open_y=1, weno_scalars=1, weno_momentum=0, tke=1, explicit_vdiff=0,
+                dissipative_heating=1, rayleigh=1, rayleigh_zd=15000., umove=12.5, vmove=3.0, microphysics=5,
+                mass_conservation=1, cache_moisture_turbulence=1, arena_fields=fields)
+    base.update(kw)
+    return te_host.config(**base)
+
+
+@pytest.fixture(scope="module")
+def encoded():
+    P = golden_check.plan()
+    fields = [n for n, _ in sorted(P.tape["fields"].items(), key=lambda x: x[1])]
+    r = te_host.Run(supercell_config(fields), encode_only=True)
+    names, A = r.program()
+    return P, fields, r, names, A
+
+
+def test_dispatch_sequence(encoded):
+    P, fields, r, names, A = encoded
+    ref = [o["name"] for o in P.ops]
+    mi = ref.index("mp_pdefq")
+    assert names.index("mp_pdefq") == mi
+    assert names[:mi] == ref[:mi]
+    for i in range(mi):
+        assert list(A[i, 0:3]) == list(P.ops[i]["grid"]), (i, names[i])
+
+
+def test_bindings(encoded):
+    P, fields, r, names, A = encoded
+    arena0, fw = r.field_offset(fields[0]), r.field_words
+    regions = {}
+    for n in REGIONS:
+        try:
+            regions[n] = r.region_offset(n)
+        except te_host.TeError:
+            pass
+    starts = sorted((o, n) for n, o in regions.items())
+    img = r.read(0, r.mem_words)
+
+    def mine(off):
+        if arena0 <= off < arena0 + fw * len(fields):
+            q = (off - arena0) // fw
+            return ("field", fields[q], off - arena0 - q * fw)
+        if off >= arena0 + fw * len(fields):  # setBytes payloads follow the arena
+            return ("bytes", off)
+        cand = [(o, n) for o, n in starts if o <= off]
+        if cand:
+            o, n = cand[-1]
+            if n == "sfc" or n in MASS_LABEL or off == o:
+                return ("buf", MASS_LABEL.get(n, n), off - o)
+        return ("bytes", off)
+
+    def ref(off):
+        if off >= P.buffers_end:
+            return ("bytes", [w for o, w in P.bytes if o == off][0])
+        for b in range(len(P.tape["buffers"]) - 1, -1, -1):
+            if off >= P.base[b]:
+                if P.tape["buffers"][b]["arena"]:
+                    f = P.field_of(off)
+                    return ("field", f, off - P.base[b] - P.tape["fields"][f] // 4)
+                return ("buf", P.tape["buffers"][b]["label"] or "buf%d" % b, off - P.base[b])
+
+    mi = [o["name"] for o in P.ops].index("mp_pdefq")
+    bad = []
+    checked = 0
+    for i in range(mi):
+        o = P.ops[i]
+        if o["name"] == "_copy":
+            continue
+        for idx, kind, write in P.params[o["name"]]:
+            if idx in TERRAIN or idx not in o["bind"]:
+                continue
+            rd, md = ref(o["bind"][idx]), mine(int(A[i, 4 + idx]))
+            if rd[0] == "bytes":
+                w = rd[1]
+                got = img[md[1]: md[1] + len(w)] if md[0] == "bytes" else None
+                if got is None or not np.array_equal(got, w):
+                    bad.append((i, o["name"], idx, "bytes", list(w[:4]), None if got is None else list(got[:4])))
+            elif rd != md:
+                bad.append((i, o["name"], idx, rd, md))
+            checked += 1
+    assert checked > 5000
+    assert not bad, bad[:10]
+
+
+def test_schedule_closed_form():
+    """Acoustic substeps per large step: nsound n -> stages of n1, n/2, n
+    substeps with n1 = max(1, round(n/3)) (timestep.js schedule), i.e. 11 for
+    n = 6; dispatch counts follow it."""
+    for ns, total in ((6, 2 + 3 + 6), (4, 2 + 2 + 4), (8, 3 + 4 + 8)):
+        r = te_host.Run(te_host.config(nx=16, ny=16, nz=12, nsound=ns), encode_only=True)
+        names, _ = r.program()
+        assert names.count("acoustic_uv") == total, (ns, names.count("acoustic_uv"))
+        assert names.count("acoustic_w_solve") == total
+
+
+def test_encoder_rejects_bad_config():
+    with pytest.raises(te_host.TeError):
+        te_host.Run(te_host.config(nx=0), encode_only=True)
+    with pytest.raises(te_host.TeError):
+        te_host.Run(te_host.config(species=5), encode_only=True)
+    with pytest.raises(te_host.TeError):  # automatic schedule beyond 192 substeps
+        te_host.Run(te_host.config(nsound=0, dt=300.0, dx=100.0, dy=100.0), encode_only=True)

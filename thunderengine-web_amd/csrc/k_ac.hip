@@ -1,0 +1,296 @@
+// Acoustic substep kernels (one per reference kernel) and the
+// positive-definite flux correction.
+#include "te_device.hpp"
+#include "te_kernels.hpp"
+
+namespace te {
+
+// acoustic_prepare.wgsl:130-184 (FMA operand order switches with
+// vec_v = nx>3 && i<(nx/4)*4, as in the reference)
+__global__ void k_acoustic_prepare(uint32_t* M, KArg a) {
+  TE_TID3
+  Dims g = dims(M, a.p[0]);
+  int i = gx, j = gy, k = gz;
+  if (i >= g.nx || j >= g.ny || k >= g.nz) return;
+  Acoustic ac = acoustic(M, a.p[1]);
+  const float *ppterm = fp(M, a.p[2]), *piadv = fp(M, a.p[3]), *thv = fp(M, a.p[4]), *rr0 = fp(M, a.p[5]),
+              *rf0 = fp(M, a.p[6]), *mh = fp(M, a.p[7]), *mf = fp(M, a.p[8]), *c1 = fp(M, a.p[9]), *c2 = fp(M, a.p[10]);
+  float *qk = fp(M, a.p[11]), *pk1 = fp(M, a.p[12]), *pk2 = fp(M, a.p[13]), *mm = fp(M, a.p[14]);
+  const int64_t sk = g.sk(), c = g.idx(i, j, k);
+  float tem = (ac.dts * g.rdz) * mh[k];
+  float r2 = (tem * rr0[k]) * rf0[k + 1];
+  float r1 = (tem * rr0[k]) * rf0[k];
+  float tem1 = ((ac.dts * g.rdz) * CP) * mf[k];
+  float c1a = c1[k], c1b = c1[k + 1], c2a = c2[k], c2b = c2[k + 1];
+  bool vec_v = g.nx > 3 && i < (g.nx / 4) * 4;
+  qk[c] = tem * ppterm[c];
+  float p2 = 0.0f, p1 = 0.0f;
+  if (k != g.nz - 1)
+    p2 = vec_v ? fma_(-c2b, piadv[c + sk], (1.0f - c1b) * piadv[c]) : fma_(1.0f - c1b, piadv[c], -(c2b * piadv[c + sk]));
+  if (k != 0)
+    p1 = vec_v ? fma_(c2a - 1.0f, piadv[c], c1a * piadv[c - sk]) : fma_(c1a, piadv[c - sk], (c2a - 1.0f) * piadv[c]);
+  pk2[c] = k == g.nz - 1 ? 0.0f : r2 * p2;
+  pk1[c] = k == 0 ? 0.0f : r1 * p1;
+  mm[c] = tem1 * fma_(c2a, thv[c], c1a * (k == 0 ? thv[c] : thv[c - sk]));
+}
+
+// acoustic_begin.wgsl:128-149
+__global__ void k_acoustic_begin(uint32_t* M, KArg a) {
+  TE_TID3
+  Dims g = dims(M, a.p[0]);
+  int i = gx - 3, j = gy - 3, k = gz;
+  if (i > g.nx + 3 || j > g.ny + 3 || k > g.nz) return;
+  Acoustic ac = acoustic(M, a.p[1]);
+  const int64_t c = g.idx(i, j, k);
+  fp(M, a.p[3])[c] = 0.0f;
+  fp(M, a.p[4])[c] = 0.0f;
+  fp(M, a.p[5])[c] = fp(M, a.p[2])[c] * ac.beta;
+}
+
+// acoustic_uv.wgsl:144-181
+__global__ void k_acoustic_uv(uint32_t* M, KArg a) {
+  TE_TID3
+  Dims g = dims(M, a.p[0]);
+  int i = gx, j = gy, k = gz;
+  if (i > g.nx || j > g.ny || k >= g.nz) return;
+  Acoustic ac = acoustic(M, a.p[1]);
+  const float *ppd = fp(M, a.p[2]), *thv = fp(M, a.p[3]), *uten = fp(M, a.p[4]), *vten = fp(M, a.p[5]),
+              *hm = fp(M, a.p[30]);
+  float *u = fp(M, a.p[6]), *v = fp(M, a.p[7]);
+  int openx = ldi(M, a.p[8]), openy = ldi(M, a.p[8] + 1);
+  const int64_t sj = g.sJ, c = g.idx(i, j, k);
+  float tem1 = (g.rdx * CP) * 0.5f;
+  float tem2 = (g.rdy * CP) * 0.5f;
+  bool uEdge = openx != 0 && (i == 0 || i == g.nx);
+  bool vEdge = openy != 0 && (j == 0 || j == g.ny);
+  if (!uEdge) u[c] = fma_(ac.dts, fma_(-((tem1 * (ppd[c] - ppd[c - 1])) * hm_uf(hm, g, i)), thv[c] + thv[c - 1], uten[c]), u[c]);
+  if (!vEdge) v[c] = fma_(ac.dts, fma_(-((tem2 * (ppd[c] - ppd[c - sj])) * hm_vf(hm, g, j)), thv[c] + thv[c - sj], vten[c]), v[c]);
+}
+
+// acoustic_fpk.wgsl:167-207
+__global__ void k_acoustic_fpk(uint32_t* M, KArg a) {
+  TE_TID3
+  Dims g = dims(M, a.p[0]);
+  int i = gx, j = gy, k = gz;
+  if (i >= g.nx || j >= g.ny || k >= g.nz) return;
+  Acoustic ac = acoustic(M, a.p[1]);
+  const float *u = fp(M, a.p[2]), *v = fp(M, a.p[3]), *w = fp(M, a.p[4]), *piadv = fp(M, a.p[5]),
+              *ppten = fp(M, a.p[6]), *ppterm = fp(M, a.p[7]), *pk1 = fp(M, a.p[8]), *pk2 = fp(M, a.p[9]),
+              *qk = fp(M, a.p[10]), *hm = fp(M, a.p[30]);
+  const int64_t sj = g.sJ, sk = g.sk(), c = g.idx(i, j, k);
+  float temx = (ac.dts * 0.5f) * g.rdx;
+  float temy = (ac.dts * 0.5f) * g.rdy;
+  float div = fma_((u[c + 1] - u[c]) * g.rdx, hm_uh(hm, g, i), ((v[c + sj] - v[c]) * g.rdy) * hm_vh(hm, g, j));
+  if (ab(div) < ac.smeps) div = 0.0f;
+  float Au = fma_(u[c], piadv[c] - piadv[c - 1], u[c + 1] * (piadv[c + 1] - piadv[c]));
+  float Av = fma_(v[c], piadv[c] - piadv[c - sj], v[c + sj] * (piadv[c + sj] - piadv[c]));
+  float advPi = fma_((-Au) * temx, hm_uh(hm, g, i), -((Av * temy) * hm_vh(hm, g, j)));
+  fp(M, a.p[11])[c] = fma_(ac.beta, fma_(w[c], pk1[c] + qk[c], w[c + sk] * (pk2[c] - qk[c])),
+                           fma_(ac.dts, fma_(-ppterm[c], div, ppten[c]), advPi));
+}
+
+// acoustic_w_solve.wgsl:124-212: one lane per (i,j) column, Thomas sweep over
+// k with the elimination coefficients in cA/cB; lanes stay coalesced along i.
+__global__ void k_acoustic_w_solve(uint32_t* M, KArg a) {
+  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x), j = (int)(blockIdx.y * blockDim.y + threadIdx.y);
+  if (i >= (int)a.n[0] || j >= (int)a.n[1]) return;
+  Dims g = dims(M, a.p[0]);
+  if (i >= g.nx || j >= g.ny) return;
+  Acoustic ac = acoustic(M, a.p[1]);
+  const float *pk1 = fp(M, a.p[2]), *pk2 = fp(M, a.p[3]), *qk = fp(M, a.p[4]), *mm = fp(M, a.p[5]),
+              *fpk = fp(M, a.p[6]), *wten = fp(M, a.p[7]), *pp3d = fp(M, a.p[8]);
+  float *w = fp(M, a.p[9]), *cA = fp(M, a.p[10]), *cB = fp(M, a.p[11]);
+  const int64_t sk = g.sk(), base = g.idx(i, j, 0);
+  const float asq = ac.alph * ac.alph;
+  w[base] = 0.0f;
+  w[base + g.nz * sk] = 0.0f;
+  float aa, bb, cc, dd;
+  auto coef = [&](int k) {
+    const int64_t c = base + k * sk, cm = c - sk;
+    float m = mm[c];
+    aa = (-asq) * (m * (pk1[cm] + qk[cm]));
+    cc = (-asq) * ((-m) * (pk2[c] - qk[c]));
+    bb = fma_(-asq, m * ((-(pk1[c] + qk[c])) + (pk2[cm] - qk[cm])), 1.0f);
+    float fw = fma_(ac.dts, wten[c], ac.beta * ((-m) * (pp3d[c] - pp3d[cm])));
+    dd = fma_(ac.alph, (-m) * ((pp3d[c] + fpk[c]) - (pp3d[cm] + fpk[cm])), w[c] + fw);
+  };
+  coef(1);
+  float r = 1.0f / bb;
+  float pa = (-cc) * r, pb = dd * r;
+  cA[base + sk] = pa;
+  cB[base + sk] = pb;
+  for (int k = 2; k < g.nz - 1; ++k) {
+    coef(k);
+    r = 1.0f / fma_(aa, pa, bb);
+    pa = (-cc) * r;
+    pb = fma_(-aa, pb, dd) * r;
+    cA[base + k * sk] = pa;
+    cB[base + k * sk] = pb;
+  }
+  coef(g.nz - 1);
+  r = 1.0f / fma_(aa, pa, bb);
+  float wn = fma_(-aa, pb, dd) * r;
+  w[base + (g.nz - 1) * sk] = wn;
+  for (int k = g.nz - 2; k >= 1; --k) {
+    const int64_t c = base + k * sk;
+    wn = fma_(cA[c], wn, cB[c]);
+    w[c] = wn;
+  }
+}
+
+// acoustic_pp.wgsl:131-167
+__global__ void k_acoustic_pp(uint32_t* M, KArg a) {
+  TE_TID3
+  Dims g = dims(M, a.p[0]);
+  int i = gx, j = gy, k = gz;
+  if (i >= g.nx || j >= g.ny || k >= g.nz) return;
+  Acoustic ac = acoustic(M, a.p[1]);
+  const float *w = fp(M, a.p[2]), *pk1 = fp(M, a.p[3]), *pk2 = fp(M, a.p[4]), *qk = fp(M, a.p[5]), *fpk = fp(M, a.p[6]);
+  float *pp3d = fp(M, a.p[7]), *ppd = fp(M, a.p[8]), *ppx = fp(M, a.p[9]);
+  int saveX = ldi(M, a.p[10]);
+  const int64_t sk = g.sk(), c = g.idx(i, j, k);
+  float prev = pp3d[c];
+  float p = fma_(ac.alph, fma_(w[c], pk1[c] + qk[c], w[c + sk] * (pk2[c] - qk[c])), prev + fpk[c]);
+  if (ab(p) < ac.smeps) p = 0.0f;
+  pp3d[c] = p;
+  float dum1 = ac.kdiv * (p - prev);
+  ppd[c] = p + dum1;
+  if (saveX != 0) ppx[c] = dum1;
+}
+
+// acoustic_accumulate.wgsl:129-158
+__global__ void k_acoustic_accumulate(uint32_t* M, KArg a) {
+  TE_TID3
+  Dims g = dims(M, a.p[0]);
+  int i = gx - 3, j = gy - 3, k = gz;
+  if (i > g.nx + 3 || j > g.ny + 3 || k > g.nz) return;
+  Acoustic ac = acoustic(M, a.p[1]);
+  const float *u = fp(M, a.p[2]), *v = fp(M, a.p[3]), *w = fp(M, a.p[4]);
+  float *rru = fp(M, a.p[5]), *rrv = fp(M, a.p[6]), *rrw = fp(M, a.p[7]);
+  float tavg = ldf(M, a.p[8]);
+  const int64_t c = g.idx(i, j, k);
+  if (tavg > 0.0f) {
+    rru[c] = (rru[c] + u[c]) * tavg;
+    rrv[c] = (rrv[c] + v[c]) * tavg;
+    rrw[c] = fma_(w[c], ac.alph, rrw[c]) * tavg;
+  } else {
+    rru[c] = rru[c] + u[c];
+    rrv[c] = rrv[c] + v[c];
+    rrw[c] = rrw[c] + w[c];
+  }
+}
+
+// ---- positive-definite flux correction
+TE_DEV bool pdef_inflow(const Dims& g, int dir, int i, int j, int64_t c, int64_t step, const float* rr, int openx,
+                        int openy) {
+  if (dir == 0 && openx != 0) {
+    if (i == 0 && rr[c] >= 0.0f) return true;
+    if (i == g.nx - 1 && rr[c + step] <= 0.0f) return true;
+  }
+  if (dir == 1 && openy != 0) {
+    if (j == 0 && rr[c] >= 0.0f) return true;
+    if (j == g.ny - 1 && rr[c + step] <= 0.0f) return true;
+  }
+  return false;
+}
+
+// pdef_dum.wgsl:177-231
+__global__ void k_pdef_dum(uint32_t* M, KArg a) {
+  TE_TID3
+  Dims g = dims(M, a.p[0]);
+  int i = gx, j = gy, k = gz;
+  if (i >= g.nx || j >= g.ny || k >= g.nz) return;
+  int dir = ldi(M, a.p[1]);
+  const float *s = fp(M, a.p[2]), *s0 = fp(M, a.p[3]), *f = fp(M, a.p[4]), *rr = fp(M, a.p[5]), *rho0 = fp(M, a.p[6]),
+              *mh = fp(M, a.p[7]), *hm = fp(M, a.p[30]);
+  float dt = ldf(M, a.p[8]);
+  int openx = ldi(M, a.p[9]), openy = ldi(M, a.p[9] + 1);
+  float* dum = fp(M, a.p[10]);
+  const int64_t c = g.idx(i, j, k);
+  const int64_t step = dir == 0 ? 1 : (dir == 1 ? (int64_t)g.sJ : g.sk());
+  float rd = dir == 0 ? g.rdx : (dir == 1 ? g.rdy : g.rdz);
+  float base = rho0[k] * s0[c];
+  if (pdef_inflow(g, dir, i, j, c, step, rr, openx, openy)) { dum[c] = base; return; }
+  float adv = (-(f[c + step] - f[c])) * rd;
+  float sdiv = (s[c] * (rr[c + step] - rr[c])) * rd;
+  float m = dir == 0 ? hm_uh(hm, g, i) : (dir == 1 ? hm_vh(hm, g, j) : mh[k]);
+  adv = adv * m;
+  sdiv = sdiv * m;
+  dum[c] = fma_(dt, adv + sdiv, base);
+}
+
+struct PdefCell { bool neg; float f1, f2, foo3; };
+TE_DEV PdefCell pdef_cell(float d, float dm, float dp) {
+  PdefCell r{false, 0.0f, 0.0f, 0.0f};
+  if (d < 0.0f) {
+    float f1 = mx(0.0f, dm), f2 = mx(0.0f, dp);
+    if (f1 + f2 > 1e-30f) {
+      r.neg = true; r.f1 = f1; r.f2 = f2;
+      r.foo3 = mx(d, -(f1 + f2)) / (f1 + f2);
+    }
+  }
+  return r;
+}
+TE_DEV float pdef_mass(float dm2, float dm1, float d0, float dp1, float dp2, bool& flag) {
+  PdefCell l = pdef_cell(dm1, dm2, d0), c = pdef_cell(d0, dm1, dp1), r = pdef_cell(dp1, d0, dp2);
+  float mass = 0.0f;
+  if (l.neg) mass = fma_(l.f2, l.foo3, mass);
+  if (c.neg) mass = fma_(-(c.f1 + c.f2), c.foo3, mass);
+  if (r.neg) mass = fma_(r.f1, r.foo3, mass);
+  flag = c.neg || (l.neg && d0 > 1e-30f) || (r.neg && d0 > 1e-30f);
+  return mass;
+}
+
+// pdef_apply.wgsl:207-282
+__global__ void k_pdef_apply(uint32_t* M, KArg a) {
+  TE_TID3
+  Dims g = dims(M, a.p[0]);
+  int i = gx, j = gy, k = gz;
+  if (i >= g.nx || j >= g.ny || k >= g.nz) return;
+  const float *s = fp(M, a.p[1]), *s0 = fp(M, a.p[2]), *fx = fp(M, a.p[3]), *fy = fp(M, a.p[4]), *fz = fp(M, a.p[5]),
+              *rru = fp(M, a.p[6]), *rrv = fp(M, a.p[7]), *rrw = fp(M, a.p[8]), *dx = fp(M, a.p[9]),
+              *dy = fp(M, a.p[10]), *dz = fp(M, a.p[11]), *rho0 = fp(M, a.p[12]), *mh = fp(M, a.p[13]),
+              *hm = fp(M, a.p[30]);
+  float dt = ldf(M, a.p[14]);
+  int openx = ldi(M, a.p[15]), openy = ldi(M, a.p[15] + 1);
+  float* tend = fp(M, a.p[16]);
+  const int64_t sj = g.sJ, sk = g.sk(), c = g.idx(i, j, k);
+  float rdt = 1.0f / dt;
+  float base = rho0[k] * s0[c];
+  float delta = 0.0f;
+  bool flag;
+  {
+    float ms = pdef_mass(dx[c - 2], dx[c - 1], dx[c], dx[c + 1], dx[c + 2], flag);
+    if (flag) {
+      float sdiv = ((s[c] * (rru[c + 1] - rru[c])) * g.rdx) * hm_uh(hm, g, i);
+      float advOld = pdef_inflow(g, 0, i, j, c, 1, rru, openx, openy) ? -sdiv : ((-(fx[c + 1] - fx[c])) * g.rdx) * hm_uh(hm, g, i);
+      float advNew = fma_((dx[c] + ms) - base, rdt, -sdiv);
+      delta = delta + (advNew - advOld);
+    }
+  }
+  {
+    float ms = pdef_mass(dy[c - 2 * sj], dy[c - sj], dy[c], dy[c + sj], dy[c + 2 * sj], flag);
+    if (flag) {
+      float sdiv = ((s[c] * (rrv[c + sj] - rrv[c])) * g.rdy) * hm_vh(hm, g, j);
+      float advOld = pdef_inflow(g, 1, i, j, c, sj, rrv, openx, openy) ? -sdiv : ((-(fy[c + sj] - fy[c])) * g.rdy) * hm_vh(hm, g, j);
+      float advNew = fma_((dy[c] + ms) - base, rdt, -sdiv);
+      delta = delta + (advNew - advOld);
+    }
+  }
+  {
+    float zm2 = k >= 2 ? dz[c - 2 * sk] : 0.0f;
+    float zm1 = k >= 1 ? dz[c - sk] : 0.0f;
+    float zp1 = k + 1 < g.nz ? dz[c + sk] : 0.0f;
+    float zp2 = k + 2 < g.nz ? dz[c + 2 * sk] : 0.0f;
+    float ms = pdef_mass(zm2, zm1, dz[c], zp1, zp2, flag);
+    if (flag) {
+      float sdiv = ((s[c] * (rrw[c + sk] - rrw[c])) * g.rdz) * mh[k];
+      float advOld = ((-(fz[c + sk] - fz[c])) * g.rdz) * mh[k];
+      float advNew = fma_((dz[c] + ms) - base, rdt, -sdiv);
+      delta = delta + (advNew - advOld);
+    }
+  }
+  tend[c] = tend[c] + delta / rho0[k];
+}
+
+}  // namespace te

@@ -1,0 +1,338 @@
+// Advection kernels, one per reference kernel: 5th-order and WENO5 fluxes of
+// scalars and momentum, and their assembly into tendencies.
+#include "te_device.hpp"
+#include "te_kernels.hpp"
+
+namespace te {
+
+// adv_flux_x.wgsl:120-143 / adv_flux_y.wgsl:120-144
+template <int DIR>
+__device__ void flux_h5(uint32_t* M, const KArg& a, int i, int j, int k) {
+  Dims g = dims(M, a.p[0]);
+  if (DIR == 0 ? (i > g.nx || j >= g.ny || k >= g.nz) : (i >= g.nx || j > g.ny || k >= g.nz)) return;
+  const float *s = fp(M, a.p[1]), *rr = fp(M, a.p[2]);
+  const int64_t st = DIR == 0 ? 1 : (int64_t)g.sJ, c = g.idx(i, j, k);
+  float u = rr[c], f;
+  if (u >= 0.0f) f = flx5(s[c - 3 * st], s[c - 2 * st], s[c - st], s[c], s[c + st]);
+  else f = flx5(s[c + 2 * st], s[c + st], s[c], s[c - st], s[c - 2 * st]);
+  fp(M, a.p[3])[c] = u * f;
+}
+__global__ void k_adv_flux_x(uint32_t* M, KArg a) { TE_TID3 flux_h5<0>(M, a, gx, gy, gz); }
+__global__ void k_adv_flux_y(uint32_t* M, KArg a) { TE_TID3 flux_h5<1>(M, a, gx, gy, gz); }
+
+// adv_flux_z.wgsl:127-200
+__global__ void k_adv_flux_z(uint32_t* M, KArg a) {
+  TE_TID3
+  Dims g = dims(M, a.p[0]);
+  int i = gx, j = gy, k = gz;
+  if (i >= g.nx || j >= g.ny || k > g.nz) return;
+  const float *s = fp(M, a.p[1]), *rrw = fp(M, a.p[2]), *c1 = fp(M, a.p[3]), *c2 = fp(M, a.p[4]);
+  float* fz = fp(M, a.p[5]);
+  const int64_t sk = g.sk(), c = g.idx(i, j, k);
+  if (k == 0 || k == g.nz) { fz[c] = 0.0f; return; }
+  float w = rrw[c], f;
+  bool up = w >= 0.0f;
+  if (k == 1) f = up ? fma_(c1[k], s[c - sk], c2[k] * s[c]) : flx3(s[c + sk], s[c], s[c - sk]);
+  else if (k == 2) f = up ? flx3(s[c - 2 * sk], s[c - sk], s[c]) : flx5(s[c + 2 * sk], s[c + sk], s[c], s[c - sk], s[c - 2 * sk]);
+  else if (k == g.nz - 2) f = up ? flx5(s[c - 3 * sk], s[c - 2 * sk], s[c - sk], s[c], s[c + sk]) : flx3(s[c + sk], s[c], s[c - sk]);
+  else if (k == g.nz - 1) f = up ? flx3(s[c - 2 * sk], s[c - sk], s[c]) : fma_(c1[k], s[c - sk], c2[k] * s[c]);
+  else f = up ? flx5(s[c - 3 * sk], s[c - 2 * sk], s[c - sk], s[c], s[c + sk]) : flx5(s[c + 2 * sk], s[c + sk], s[c], s[c - sk], s[c - 2 * sk]);
+  fz[c] = w * f;
+}
+
+// adv_flux_x_weno.wgsl:158-200 / adv_flux_y_weno.wgsl:158-201
+template <int DIR>
+__device__ void flux_h_weno(uint32_t* M, const KArg& a, int i, int j, int k) {
+  Dims g = dims(M, a.p[0]);
+  if (i > g.nx || j > g.ny || k >= g.nz) return;
+  const float *s = fp(M, a.p[1]), *rr = fp(M, a.p[2]);
+  float weps = ldf(M, a.p[3]);
+  bool pdef = ldi(M, a.p[4]) == 1;
+  const int64_t st = DIR == 0 ? 1 : (int64_t)g.sJ, c = g.idx(i, j, k);
+  float s1, s2, s3, s4, s5;
+  if (rr[c] >= 0.0f) { s1 = s[c - 3 * st]; s2 = s[c - 2 * st]; s3 = s[c - st]; s4 = s[c]; s5 = s[c + st]; }
+  else { s1 = s[c + 2 * st]; s2 = s[c + st]; s3 = s[c]; s4 = s[c - st]; s5 = s[c - 2 * st]; }
+  fp(M, a.p[5])[c] = weno_pdef_ok(s1, s2, s3, s4, s5, weps, pdef) ? rr[c] * weno5(s1, s2, s3, s4, s5, weps) : 0.0f;
+}
+__global__ void k_adv_flux_x_weno(uint32_t* M, KArg a) { TE_TID3 flux_h_weno<0>(M, a, gx, gy, gz); }
+__global__ void k_adv_flux_y_weno(uint32_t* M, KArg a) { TE_TID3 flux_h_weno<1>(M, a, gx, gy, gz); }
+
+// adv_flux_z_weno.wgsl:174-279
+__global__ void k_adv_flux_z_weno(uint32_t* M, KArg a) {
+  TE_TID3
+  Dims g = dims(M, a.p[0]);
+  int i = gx, j = gy, kf = gz + 1;
+  if (i >= g.nx || j >= g.ny || kf > g.nz + 1) return;
+  const float *s = fp(M, a.p[1]), *rrw = fp(M, a.p[2]), *c1 = fp(M, a.p[3]), *c2 = fp(M, a.p[4]);
+  float* fz = fp(M, a.p[8]);
+  float weps = ldf(M, a.p[5]);
+  int pdefweno = ldi(M, a.p[7]);
+  int k = kf - 1;
+  const int64_t sk = g.sk(), c = g.idx(i, j, k);
+  if (kf == 1 || kf == g.nz + 1) { fz[c] = 0.0f; return; }
+#define L(n) s[c + (n) * sk]
+  float w = rrw[c];
+  const int nk = g.nz;
+  if (kf == 2) {
+    fz[c] = (w >= 0.0f) ? w * fma_(c1[k], L(-1), c2[k] * L(0)) : w * upstrpd(L(1), L(0), L(-1), weps);
+    return;
+  }
+  if (kf == nk) {
+    fz[c] = (w > 0.0f) ? w * upstrpd(L(-2), L(-1), L(0), weps) : w * fma_(c1[k], L(-1), c2[k] * L(0));
+    return;
+  }
+  float s1, s2, s3, s4, s5;
+  if (w >= 0.0f) { s1 = L(-3); s2 = L(-2); s3 = L(-1); s4 = L(0); s5 = L(1); }
+  else { s1 = L(2); s2 = L(1); s3 = L(0); s4 = L(-1); s5 = L(-2); }
+  if (pdefweno == 1) {
+    bool doit = weno_pdef_ok(s1, s2, s3, s4, s5, weps, true);
+    if (kf == 3 && w > 0.0f) doit = false;
+    if (kf == nk - 1 && w < 0.0f) doit = false;
+    if (doit) { fz[c] = w * weno5(s1, s2, s3, s4, s5, weps); return; }
+    if (kf != 3 && kf != nk - 1) { fz[c] = 0.0f; return; }
+  }
+  if (kf == 3) {
+    fz[c] = (w >= 0.0f) ? w * upstrpd(L(-2), L(-1), L(0), weps) : w * weno5(L(2), L(1), L(0), L(-1), L(-2), weps);
+    return;
+  }
+  if (kf == nk - 1) {
+    fz[c] = (w >= 0.0f) ? w * weno5(L(-3), L(-2), L(-1), L(0), L(1), weps) : w * upstrpd(L(1), L(0), L(-1), weps);
+    return;
+  }
+  fz[c] = w * weno5(s1, s2, s3, s4, s5, weps);
+#undef L
+}
+
+TE_DEV float flx5_upwind(const float* s, int64_t c, int64_t st, bool up, bool shifted) {
+  if (shifted)
+    return up ? flx5(s[c - 2 * st], s[c - st], s[c], s[c + st], s[c + 2 * st])
+              : flx5(s[c + 3 * st], s[c + 2 * st], s[c + st], s[c], s[c - st]);
+  return up ? flx5(s[c - 3 * st], s[c - 2 * st], s[c - st], s[c], s[c + st])
+            : flx5(s[c + 2 * st], s[c + st], s[c], s[c - st], s[c - 2 * st]);
+}
+TE_DEV float weno5_upwind(const float* s, int64_t c, int64_t st, bool up, bool shifted, float weps) {
+  if (shifted)
+    return up ? weno5(s[c - 2 * st], s[c - st], s[c], s[c + st], s[c + 2 * st], weps)
+              : weno5(s[c + 3 * st], s[c + 2 * st], s[c + st], s[c], s[c - st], weps);
+  return up ? weno5(s[c - 3 * st], s[c - 2 * st], s[c - st], s[c], s[c + st], weps)
+            : weno5(s[c + 2 * st], s[c + st], s[c], s[c - st], s[c - 2 * st], weps);
+}
+
+// adv_flux_x_stag.wgsl:144-186 / adv_flux_y_stag.wgsl
+template <int DIR>
+__device__ void flux_h_stag(uint32_t* M, const KArg& a, int i, int j, int k) {
+  Dims g = dims(M, a.p[0]);
+  if (i > g.nx || j > g.ny || k >= g.nz) return;
+  int stag = ldi(M, a.p[1]);
+  const float *s = fp(M, a.p[2]), *rr = fp(M, a.p[DIR == 0 ? 3 : 4]), *c2 = fp(M, a.p[7]);
+  float* fl = fp(M, a.p[8]);
+  const int64_t sj = g.sJ, sk = g.sk(), st = DIR == 0 ? 1 : sj, c = g.idx(i, j, k);
+  if (stag == 4 && k == 0) { fl[c] = 0.0f; return; }
+  float m;
+  if (stag == 2) m = 0.5f * (rr[c] + rr[c + (DIR == 0 ? 1 : -1)]);
+  else if (stag == 3) m = 0.5f * (rr[c] + rr[c + (DIR == 0 ? -sj : sj)]);
+  else if (stag == 4) { float cc2 = c2[k]; m = fma_(cc2, rr[c], (1.0f - cc2) * rr[c - sk]); }
+  else m = rr[c];
+  fl[c] = m * flx5_upwind(s, c, st, m >= 0.0f, stag == (DIR == 0 ? 2 : 3));
+}
+__global__ void k_adv_flux_x_stag(uint32_t* M, KArg a) { TE_TID3 flux_h_stag<0>(M, a, gx - 1, gy - 1, gz); }
+__global__ void k_adv_flux_y_stag(uint32_t* M, KArg a) { TE_TID3 flux_h_stag<1>(M, a, gx - 1, gy - 1, gz); }
+
+// adv_flux_z_stag.wgsl:127-268
+__global__ void k_adv_flux_z_stag(uint32_t* M, KArg a) {
+  TE_TID3
+  Dims g = dims(M, a.p[0]);
+  int i = gx, j = gy, k = gz;
+  if (i > g.nx || j > g.ny || k > g.nz) return;
+  int stag = ldi(M, a.p[1]);
+  const float *s = fp(M, a.p[2]), *rrw = fp(M, a.p[5]), *c1 = fp(M, a.p[6]), *c2 = fp(M, a.p[7]);
+  float* fz = fp(M, a.p[8]);
+  const int64_t sj = g.sJ, sk = g.sk(), c = g.idx(i, j, k);
+  float m;
+  if (stag == 2) m = 0.5f * (rrw[c] + rrw[c - 1]);
+  else if (stag == 3) m = 0.5f * (rrw[c] + rrw[c - sj]);
+  else m = 0.5f * (rrw[c] + rrw[c + sk]);
+  bool up = m >= 0.0f;
+#define A_(n) s[c + (n) * sk]
+  float f;
+  if (stag == 4) {
+    if (k > g.nz - 1) { fz[c] = 0.0f; return; }
+    if (k == 0) f = up ? 0.5f * (A_(0) + A_(1)) : flx3(A_(2), A_(1), A_(0));
+    else if (k == 1) f = up ? flx3(A_(-1), A_(0), A_(1)) : flx5(A_(3), A_(2), A_(1), A_(0), A_(-1));
+    else if (k == g.nz - 2) f = up ? flx5(A_(-2), A_(-1), A_(0), A_(1), A_(2)) : flx3(A_(2), A_(1), A_(0));
+    else if (k == g.nz - 1) f = up ? flx3(A_(-1), A_(0), A_(1)) : 0.5f * (A_(0) + A_(1));
+    else f = up ? flx5(A_(-2), A_(-1), A_(0), A_(1), A_(2)) : flx5(A_(3), A_(2), A_(1), A_(0), A_(-1));
+  } else {
+    if (k == 0 || k == g.nz) { fz[c] = 0.0f; return; }
+    if (k == 1) f = up ? fma_(c1[k], A_(-1), c2[k] * A_(0)) : flx3(A_(1), A_(0), A_(-1));
+    else if (k == 2) f = up ? flx3(A_(-2), A_(-1), A_(0)) : flx5(A_(2), A_(1), A_(0), A_(-1), A_(-2));
+    else if (k == g.nz - 2) f = up ? flx5(A_(-3), A_(-2), A_(-1), A_(0), A_(1)) : flx3(A_(1), A_(0), A_(-1));
+    else if (k == g.nz - 1) f = up ? flx3(A_(-2), A_(-1), A_(0)) : fma_(c1[k], A_(-1), c2[k] * A_(0));
+    else f = up ? flx5(A_(-3), A_(-2), A_(-1), A_(0), A_(1)) : flx5(A_(2), A_(1), A_(0), A_(-1), A_(-2));
+  }
+#undef A_
+  fz[c] = m * f;
+}
+
+// adv_flux_x_weno_w.wgsl:143-175 / adv_flux_y_weno_w.wgsl
+template <int DIR>
+__device__ void flux_h_weno_w(uint32_t* M, const KArg& a, int i, int j, int k) {
+  Dims g = dims(M, a.p[0]);
+  if (i < 0 || j < 0 || i > g.nx || j > g.ny || k >= g.nz) return;
+  const float *s = fp(M, a.p[1]), *rr = fp(M, a.p[2]), *c2 = fp(M, a.p[3]);
+  float weps = ldf(M, a.p[4]);
+  float* fl = fp(M, a.p[5]);
+  const int64_t sk = g.sk(), st = DIR == 0 ? 1 : (int64_t)g.sJ, c = g.idx(i, j, k);
+  if (k == 0) { fl[c] = 0.0f; return; }
+  float cc2 = c2[k], cc1 = 1.0f - cc2;
+  float ub = fma_(cc2, rr[c], cc1 * rr[c - sk]);
+  if (ub >= 0.0f) fl[c] = ub * weno5(s[c - 3 * st], s[c - 2 * st], s[c - st], s[c], s[c + st], weps);
+  else fl[c] = ub * weno5(s[c + 2 * st], s[c + st], s[c], s[c - st], s[c - 2 * st], weps);
+}
+__global__ void k_adv_flux_x_weno_w(uint32_t* M, KArg a) { TE_TID3 flux_h_weno_w<0>(M, a, gx - 1, gy - 1, gz); }
+__global__ void k_adv_flux_y_weno_w(uint32_t* M, KArg a) { TE_TID3 flux_h_weno_w<1>(M, a, gx - 1, gy - 1, gz); }
+
+// adv_flux_z_weno_w.wgsl:159-231
+__global__ void k_adv_flux_z_weno_w(uint32_t* M, KArg a) {
+  TE_TID3
+  Dims g = dims(M, a.p[0]);
+  int i = gx, j = gy, k = gz;
+  if (i > g.nx || j > g.ny || k > g.nz) return;
+  const float *s = fp(M, a.p[1]), *rrw = fp(M, a.p[2]);
+  float weps = ldf(M, a.p[3]);
+  float* fz = fp(M, a.p[4]);
+  const int64_t sk = g.sk(), c = g.idx(i, j, k);
+  if (k > g.nz - 1) { fz[c] = 0.0f; return; }
+#define L(n) s[c + (n) * sk]
+  float wbar = 0.5f * (rrw[c] + rrw[c + sk]);
+  const int nz = g.nz;
+  float f;
+  if (k == 0) f = (wbar >= 0.0f) ? 0.5f * (L(0) + L(1)) : upstrpd(L(2), L(1), L(0), weps);
+  else if (k == nz - 1) f = (wbar > 0.0f) ? upstrpd(L(-1), L(0), L(1), weps) : 0.5f * (L(0) + L(1));
+  else if (k == 1) f = (wbar >= 0.0f) ? upstrpd(L(-1), L(0), L(1), weps) : weno5(L(3), L(2), L(1), L(0), L(-1), weps);
+  else if (k == nz - 2) f = (wbar > 0.0f) ? weno5(L(-2), L(-1), L(0), L(1), L(2), weps) : upstrpd(L(2), L(1), L(0), weps);
+  else f = (wbar >= 0.0f) ? weno5(L(-2), L(-1), L(0), L(1), L(2), weps) : weno5(L(3), L(2), L(1), L(0), L(-1), weps);
+#undef L
+  fz[c] = wbar * f;
+}
+
+// adv_flux_x_weno_uv.wgsl:168-188 / adv_flux_y_weno_uv.wgsl
+template <int DIR>
+__device__ void flux_h_weno_uv(uint32_t* M, const KArg& a, int i, int j, int k) {
+  Dims g = dims(M, a.p[0]);
+  if (i > g.nx || j > g.ny || k >= g.nz) return;
+  int stag = ldi(M, a.p[1]);
+  const float *s = fp(M, a.p[2]), *rr = fp(M, a.p[3]);
+  float weps = ldf(M, a.p[4]);
+  const int64_t sj = g.sJ, st = DIR == 0 ? 1 : sj, c = g.idx(i, j, k);
+  float m;
+  if (DIR == 0) m = (stag == 2) ? 0.5f * (rr[c] + rr[c + 1]) : 0.5f * (rr[c] + rr[c - sj]);
+  else m = (stag == 2) ? 0.5f * (rr[c] + rr[c - 1]) : 0.5f * (rr[c] + rr[c + sj]);
+  fp(M, a.p[5])[c] = m * weno5_upwind(s, c, st, m >= 0.0f, stag == (DIR == 0 ? 2 : 3), weps);
+}
+__global__ void k_adv_flux_x_weno_uv(uint32_t* M, KArg a) { TE_TID3 flux_h_weno_uv<0>(M, a, gx - 1, gy - 1, gz); }
+__global__ void k_adv_flux_y_weno_uv(uint32_t* M, KArg a) { TE_TID3 flux_h_weno_uv<1>(M, a, gx - 1, gy - 1, gz); }
+
+// adv_flux_z_weno_uv.wgsl:159-236
+__global__ void k_adv_flux_z_weno_uv(uint32_t* M, KArg a) {
+  TE_TID3
+  Dims g = dims(M, a.p[0]);
+  int i = gx, j = gy, k = gz;
+  if (i > g.nx || j > g.ny || k > g.nz) return;
+  int stag = ldi(M, a.p[1]);
+  const float *s = fp(M, a.p[2]), *rrw = fp(M, a.p[3]), *c1 = fp(M, a.p[4]), *c2 = fp(M, a.p[5]);
+  float weps = ldf(M, a.p[6]);
+  float* fz = fp(M, a.p[7]);
+  const int64_t sj = g.sJ, sk = g.sk(), c = g.idx(i, j, k);
+  if (k == 0 || k == g.nz) { fz[c] = 0.0f; return; }
+#define L(n) s[c + (n) * sk]
+  float m = (stag == 2) ? 0.5f * (rrw[c] + rrw[c - 1]) : 0.5f * (rrw[c] + rrw[c - sj]);
+  bool up = m >= 0.0f;
+  float f;
+  if (k == 1) f = up ? fma_(c1[k], L(-1), c2[k] * L(0)) : upstrpd(L(1), L(0), L(-1), weps);
+  else if (k == 2) f = up ? upstrpd(L(-2), L(-1), L(0), weps) : weno5(L(2), L(1), L(0), L(-1), L(-2), weps);
+  else if (k == g.nz - 2) f = up ? weno5(L(-3), L(-2), L(-1), L(0), L(1), weps) : upstrpd(L(1), L(0), L(-1), weps);
+  else if (k == g.nz - 1) f = up ? upstrpd(L(-2), L(-1), L(0), weps) : fma_(c1[k], L(-1), c2[k] * L(0));
+  else f = up ? weno5(L(-3), L(-2), L(-1), L(0), L(1), weps) : weno5(L(2), L(1), L(0), L(-1), L(-2), weps);
+#undef L
+  fz[c] = m * f;
+}
+
+// adv_assemble.wgsl:139-279: flux divergence + mass-divergence term, divided
+// by the reference density; open-boundary inflow corrections included.
+__global__ void k_adv_assemble(uint32_t* M, KArg a) {
+  TE_TID3
+  Dims g = dims(M, a.p[0]);
+  int i = gx, j = gy, k = gz;
+  int stag = ldi(M, a.p[1]);
+  int st = stag < 0 ? -stag : stag;
+  int iMax = st == 2 ? g.nx : g.nx - 1, jMax = st == 3 ? g.ny : g.ny - 1, kLo = st == 4 ? 1 : 0, kMax = g.nz - 1;
+  if (i > iMax || j > jMax || k < kLo || k > kMax) return;
+  int openx = ldi(M, a.p[17]), openy = ldi(M, a.p[17] + 1);
+  if (openx != 0 && st == 2 && (i == 0 || i == g.nx)) return;
+  if (openy != 0 && st == 3 && (j == 0 || j == g.ny)) return;
+  const float *s = fp(M, a.p[2]), *fx = fp(M, a.p[3]), *fy = fp(M, a.p[4]), *fz = fp(M, a.p[5]), *divx = fp(M, a.p[6]),
+              *rho0 = fp(M, a.p[7]), *rf0 = fp(M, a.p[8]), *mh = fp(M, a.p[9]), *mf = fp(M, a.p[10]),
+              *rru = fp(M, a.p[13]), *rrv = fp(M, a.p[14]), *c1 = fp(M, a.p[15]), *c2 = fp(M, a.p[16]),
+              *hm = fp(M, a.p[30]);
+  float* tend = fp(M, a.p[11]);
+  int accumulate = ldi(M, a.p[12]);
+  const int64_t sj = g.sJ, sk = g.sk(), c = g.idx(i, j, k);
+  float dFx = st == 2 ? fx[c] - fx[c - 1] : fx[c + 1] - fx[c];
+  float dFy = st == 3 ? fy[c] - fy[c - sj] : fy[c + sj] - fy[c];
+  float dFz = st == 4 ? fz[c] - fz[c - sk] : fz[c + sk] - fz[c];
+  float mz = stag < 0 ? 1.0f : (stag == 4 ? mf[k] : mh[k]);
+  float dv;
+  if (st == 2) dv = 0.5f * (divx[c] + divx[c - 1]);
+  else if (st == 3) dv = 0.5f * (divx[c] + divx[c - sj]);
+  else if (st == 4) dv = fma_(c2[k], divx[c], c1[k] * divx[c - sk]);
+  else dv = divx[c];
+  if (openx != 0 && st != 2) {
+    bool west = i == 0, east = i == g.nx - 1;
+    if (west || east) {
+      int64_t cf = west ? c : c + 1;
+      float un, piece;
+      if (st == 3) {
+        un = rru[cf] + rru[cf - sj];
+        piece = ((0.5f * ((rru[c + 1 - sj] - rru[c - sj]) + (rru[c + 1] - rru[c]))) * g.rdx) * hm_uh(hm, g, i);
+      } else if (st == 4) {
+        float cc2 = 0.5f * (c2[k - 1] + c2[k]), cc1 = 1.0f - cc2;
+        un = fma_(cc2, rru[cf], cc1 * rru[cf - sk]);
+        piece = (fma_(c1[k], rru[c + 1 - sk] - rru[c - sk], c2[k] * (rru[c + 1] - rru[c])) * g.rdx) * hm_uh(hm, g, i);
+      } else {
+        un = rru[cf];
+        piece = ((rru[c + 1] - rru[c]) * g.rdx) * hm_uh(hm, g, i);
+      }
+      bool inflow = west ? un >= 0.0f : un <= 0.0f;
+      if (inflow) { dFx = 0.0f; dv = dv - piece; }
+    }
+  }
+  if (openy != 0 && st != 3) {
+    bool south = j == 0, north = j == g.ny - 1;
+    if (south || north) {
+      int64_t cf = south ? c : c + sj;
+      float vn, piece;
+      if (st == 2) {
+        vn = rrv[cf] + rrv[cf - 1];
+        piece = ((0.5f * ((rrv[c + sj - 1] - rrv[c - 1]) + (rrv[c + sj] - rrv[c]))) * g.rdy) * hm_vh(hm, g, j);
+      } else if (st == 4) {
+        float cc2 = 0.5f * (c2[k - 1] + c2[k]), cc1 = 1.0f - cc2;
+        vn = fma_(cc2, rrv[cf], cc1 * rrv[cf - sk]);
+        piece = (fma_(c1[k], rrv[c + sj - sk] - rrv[c - sk], c2[k] * (rrv[c + sj] - rrv[c])) * g.rdy) * hm_vh(hm, g, j);
+      } else {
+        vn = rrv[cf];
+        piece = ((rrv[c + sj] - rrv[c]) * g.rdy) * hm_vh(hm, g, j);
+      }
+      bool inflow = south ? vn >= 0.0f : vn <= 0.0f;
+      if (inflow) { dFy = 0.0f; dv = dv - piece; }
+    }
+  }
+  float mxm = st == 2 ? hm_uf(hm, g, i) : hm_uh(hm, g, i);
+  float mym = st == 3 ? hm_vf(hm, g, j) : hm_vh(hm, g, j);
+  float divF = fma_(dFz * g.rdz, mz, fma_(dFx * g.rdx, mxm, (dFy * g.rdy) * mym));
+  float rho = st == 4 ? rf0[k] : rho0[k];
+  float t = fma_(s[c], dv, -divF) / rho;
+  tend[c] = accumulate != 0 ? tend[c] + t : t;
+}
+
+}  // namespace te

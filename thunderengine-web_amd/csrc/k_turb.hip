@@ -1,0 +1,802 @@
+// Rayleigh damping and 1.5-order TKE (LES) turbulence kernels, one per
+// reference kernel.  Column kernels run one lane per (i,j) column with the
+// tridiagonal sweep carried in registers.
+#include "te_device.hpp"
+#include "te_kernels.hpp"
+
+namespace te {
+
+// rayleigh_damping.wgsl:111-164
+__global__ void k_rayleigh_damping(uint32_t* M, KArg a) {
+  TE_TID3
+  Dims g = dims(M, a.p[0]);
+  int i = gx, j = gy, k = gz;
+  if (i > g.nx || j > g.ny || k > g.nz) return;
+  const float *u = fp(M, a.p[1]), *v = fp(M, a.p[2]), *w = fp(M, a.p[3]), *th = fp(M, a.p[4]), *u0 = fp(M, a.p[5]),
+              *v0 = fp(M, a.p[6]), *tauh = fp(M, a.p[7]), *tauf = fp(M, a.p[8]), *hdx = fp(M, a.p[14]),
+              *hdy = fp(M, a.p[15]), *avg = fp(M, a.p[18]), *th0 = fp(M, a.p[19]);
+  float *uten = fp(M, a.p[10]), *vten = fp(M, a.p[11]), *wten = fp(M, a.p[12]), *thten = fp(M, a.p[13]);
+  float rdalpha = ldf(M, a.p[9]);
+  int lateral = ldi(M, a.p[16]), toAverage = ldi(M, a.p[17]);
+  const int64_t c = g.idx(i, j, k);
+  if (k < g.nz) {
+    float t = tauh[k];
+    float ur = toAverage != 0 ? avg[k] : u0[k];
+    float vr = toAverage != 0 ? avg[g.nz + k] : v0[k];
+    if (j < g.ny) uten[c] = ((-rdalpha) * t) * (u[c] - ur);
+    if (i < g.nx) vten[c] = ((-rdalpha) * t) * (v[c] - vr);
+    if (i < g.nx && j < g.ny) {
+      float d = toAverage != 0 ? (th0[k] - avg[2 * g.nz + k]) + th[c] : th[c];
+      thten[c] = ((-rdalpha) * t) * d;
+    }
+  }
+  if (i < g.nx && j < g.ny) {
+    float tf = tauf[k];
+    if (lateral != 0 && k < g.nz) tf = mx(mx(tf, hdx[i]), hdy[j]);
+    wten[c] = ((-rdalpha) * tf) * w[c];
+  }
+}
+
+// turb_rho_faces.wgsl:111-151
+__global__ void k_turb_rho_faces(uint32_t* M, KArg a) {
+  TE_TID3
+  Dims g = dims(M, a.p[0]);
+  int i = gx - 2, j = gy - 2, k = gz;
+  if (i > g.nx + 1 || j > g.ny + 1 || k > g.nz) return;
+  const float *rho = fp(M, a.p[1]), *c1 = fp(M, a.p[2]), *c2 = fp(M, a.p[3]);
+  float *rr = fp(M, a.p[4]), *rf = fp(M, a.p[5]);
+  const uint32_t os = a.p[6], ot = a.p[7];
+  const int64_t sk = g.sk(), c = g.idx(i, j, k);
+  if (k < g.nz) rr[c] = 1.0f / rho[c];
+  if (k == 0) {
+    float r = fma_(ldf(M, os), rho[c], ldf(M, os + 1) * rho[c + sk]);
+    if (g.nz >= 3) r = fma_(ldf(M, os + 2), rho[c + 2 * sk], r);
+    rf[c] = r;
+  } else if (k == g.nz) {
+    float r = fma_(ldf(M, ot), rho[c - sk], ldf(M, ot + 1) * rho[c - 2 * sk]);
+    if (g.nz >= 3) r = fma_(ldf(M, ot + 2), rho[c - 3 * sk], r);
+    rf[c] = r;
+  } else {
+    rf[c] = fma_(c1[k], rho[c - sk], c2[k] * rho[c]);
+  }
+}
+
+// turb_def_centers.wgsl:139-168
+__global__ void k_turb_def_centers(uint32_t* M, KArg a) {
+  TE_TID3
+  Dims g = dims(M, a.p[0]);
+  int i = gx - 1, j = gy - 1, k = gz;
+  if (i > g.nx + 1 || j > g.ny + 1 || k >= g.nz) return;
+  const float *u = fp(M, a.p[1]), *v = fp(M, a.p[2]), *w = fp(M, a.p[3]), *rho = fp(M, a.p[4]), *mh = fp(M, a.p[5]),
+              *hm = fp(M, a.p[30]);
+  const int64_t sj = g.sJ, sk = g.sk(), c = g.idx(i, j, k);
+  fp(M, a.p[6])[c] = ((rho[c] * (u[c + 1] - u[c])) * g.rdx) * hm_uh(hm, g, i);
+  fp(M, a.p[7])[c] = ((rho[c] * (v[c + sj] - v[c])) * g.rdy) * hm_vh(hm, g, j);
+  fp(M, a.p[8])[c] = ((rho[c] * (w[c + sk] - w[c])) * g.rdz) * mh[k];
+  float rcorner = 0.25f * ((rho[c - 1 - sj] + rho[c]) + (rho[c - 1] + rho[c - sj]));
+  fp(M, a.p[9])[c] =
+      (0.5f * fma_((u[c] - u[c - sj]) * g.rdy, hm_vf(hm, g, j), ((v[c] - v[c - 1]) * g.rdx) * hm_uf(hm, g, i))) * rcorner;
+}
+
+// turb_def_faces.wgsl:125-151
+__global__ void k_turb_def_faces(uint32_t* M, KArg a) {
+  TE_TID3
+  Dims g = dims(M, a.p[0]);
+  int i = gx, j = gy, k = gz + 1;
+  if (i > g.nx + 1 || j > g.ny + 1 || k > g.nz - 1) return;
+  const float *u = fp(M, a.p[1]), *v = fp(M, a.p[2]), *w = fp(M, a.p[3]), *rf = fp(M, a.p[4]), *mf = fp(M, a.p[5]),
+              *hm = fp(M, a.p[30]);
+  const int64_t sj = g.sJ, sk = g.sk(), c = g.idx(i, j, k);
+  float mfu = 0.5f * (mf[k] + mf[k]);
+  float mfv = 0.5f * (mf[k] + mf[k]);
+  fp(M, a.p[6])[c] =
+      ((0.5f * fma_((w[c] - w[c - 1]) * g.rdx, hm_uf(hm, g, i), ((u[c] - u[c - sk]) * g.rdz) * mfu)) * 0.5f) * (rf[c - 1] + rf[c]);
+  fp(M, a.p[7])[c] =
+      ((0.5f * fma_((w[c] - w[c - sj]) * g.rdy, hm_vf(hm, g, j), ((v[c] - v[c - sk]) * g.rdz) * mfv)) * 0.5f) * (rf[c - sj] + rf[c]);
+}
+
+// turb_def_assemble.wgsl:111-149
+__global__ void k_turb_def_assemble(uint32_t* M, KArg a) {
+  TE_TID3
+  Dims g = dims(M, a.p[0]);
+  int i = gx, j = gy, k = gz + 1;
+  if (i >= g.nx || j >= g.ny || k > g.nz - 1) return;
+  const float *s11 = fp(M, a.p[1]), *s22 = fp(M, a.p[2]), *s33 = fp(M, a.p[3]), *s12 = fp(M, a.p[4]),
+              *s13 = fp(M, a.p[5]), *s23 = fp(M, a.p[6]), *c1 = fp(M, a.p[7]), *c2 = fp(M, a.p[8]), *rf = fp(M, a.p[9]);
+  const int64_t sj = g.sJ, sk = g.sk(), c = g.idx(i, j, k);
+  float A = c1[k], B = c2[k];
+  float t11 = fma_(A * s11[c - sk], s11[c - sk], (B * s11[c]) * s11[c]);
+  float t22 = fma_(A * s22[c - sk], s22[c - sk], (B * s22[c]) * s22[c]);
+  float t33 = fma_(A * s33[c - sk], s33[c - sk], (B * s33[c]) * s33[c]);
+  auto sq2 = [&](int64_t p, int64_t q) { return fma_(s12[p], s12[p], s12[q] * s12[q]); };
+  float lo = sq2(c - sk, c + 1 + sj - sk) + sq2(c + sj - sk, c + 1 - sk);
+  float hi = sq2(c, c + 1 + sj) + sq2(c + sj, c + 1);
+  float t12 = 0.25f * fma_(A, lo, B * hi);
+  float t13 = 0.5f * fma_(s13[c], s13[c], s13[c + 1] * s13[c + 1]);
+  float t23 = 0.5f * fma_(s23[c], s23[c], s23[c + sj] * s23[c + sj]);
+  float rrf = 1.0f / (rf[c] * rf[c]);
+  fp(M, a.p[10])[c] = (4.0f * (t13 + t23)) * rrf;
+  fp(M, a.p[11])[c] = fma_(2.0f, (t11 + t22) + t33, 4.0f * t12) * rrf;
+}
+
+// saturation mixing ratios over liquid / ice (turb_nm.wgsl:204-230)
+TE_DEV float mp_rslf(float p, float t) {
+  float esl = 611.2f * rr_exp((17.67f * (t - 273.15f)) / (t - 29.65f));
+  esl = mn(esl, p * 0.5f);
+  return ((RD / RV) * esl) / (p - esl);
+}
+TE_DEV float mp_rsif(float p, float t) {
+  float esi = 611.2f * rr_exp((21.874557f * (t - 273.15f)) / (t - 7.66f));
+  esi = mn(esi, p * 0.5f);
+  return ((RD / RV) * esi) / (p - esi);
+}
+
+// turb_nm.wgsl:236-355 (moist squared buoyancy frequency)
+__global__ void k_turb_nm(uint32_t* M, KArg a) {
+  TE_TID3
+  Dims g = dims(M, a.p[0]);
+  int i = gx, j = gy, k = gz + 1;
+  if (i >= g.nx || j >= g.ny || k > g.nz - 1) return;
+  const float *thP = fp(M, a.p[1]), *ppP = fp(M, a.p[2]), *th0 = fp(M, a.p[3]), *pi0 = fp(M, a.p[4]),
+              *prs = fp(M, a.p[5]), *qv = fp(M, a.p[6]), *qc = fp(M, a.p[7]), *qr = fp(M, a.p[8]), *qi = fp(M, a.p[9]),
+              *qs = fp(M, a.p[10]), *qg = fp(M, a.p[11]), *c1 = fp(M, a.p[12]), *c2 = fp(M, a.p[13]), *mf = fp(M, a.p[14]);
+  float *nm = fp(M, a.p[15]), *sat = fp(M, a.p[17]);
+  int moist = ldi(M, a.p[16]), wantSat = ldi(M, a.p[18]);
+  const int64_t sk = g.sk(), c = g.idx(i, j, k);
+  if (wantSat != 0) sat[c] = 0.0f;
+  const float nmsat = 1e-10f, eps = RD / RV, reps = RV / RD, gAcc = 9.81f;
+  const float LV1 = 3134708.0f, LV2 = 2.32e+03f, LS1 = 2898463.5f, LS2 = 236.0f;
+  float thA = th0[k - 1] + thP[c - sk];
+  float thB = th0[k] + thP[c];
+  if (moist == 0) {
+    nm[c] = ((rr_log(thB / thA) * gAcc) * g.rdz) * mf[k];
+    return;
+  }
+  float piA = pi0[k - 1] + ppP[c - sk], piB = pi0[k] + ppP[c];
+  float tA = thA * piA, tB = thB * piB;
+  float qtA = ((((qv[c - sk] + qc[c - sk]) + qr[c - sk]) + qi[c - sk]) + qs[c - sk]) + qg[c - sk];
+  float qtB = ((((qv[c] + qc[c]) + qr[c]) + qi[c]) + qs[c]) + qg[c];
+  float qciA = (qv[c - sk] + qc[c - sk]) + qi[c - sk];
+  float qciB = (qv[c] + qc[c]) + qi[c];
+  float thvA = (thA * fma_(reps, qv[c - sk], 1.0f)) / (1.0f + qtA);
+  float thvB = (thB * fma_(reps, qv[c], 1.0f)) / (1.0f + qtB);
+  float rmf = g.rdz * mf[k];
+  float dlnT = rr_log(tB / tA) * rmf;
+  float dlnQt = rr_log((1.0f + qtB) / (1.0f + qtA)) * rmf;
+  float out = (gAcc * rr_log(thvB / thvA)) * rmf;
+  float A = c1[k], B = c2[k];
+  float pavg = fma_(A, prs[c - sk], B * prs[c]);
+  float tavg = fma_(A, tA, B * tB);
+  float qtavg = fma_(A, qtA, B * qtB);
+  float qciavg = fma_(A, qciA, B * qciB);
+  auto lam = [&](float qvs, float lh, float drdt, float cpml) {
+    float gamma = ((gAcc * (1.0f + qtavg)) * (1.0f + (lh * qvs) / (RD * tavg))) / fma_(lh, drdt, cpml);
+    return gAcc * fma_(dlnT + gamma / tavg, 1.0f + (tavg * drdt) / (eps + qvs), -dlnQt);
+  };
+  if (tavg >= 273.15f) {
+    float qvs = mp_rslf(pavg, tavg);
+    float ql = mx(qciavg - qvs, 0.0f);
+    if (ql > nmsat) {
+      if (wantSat != 0) sat[c] = 1.0f;
+      float qvl = mx(qciavg - ql, 0.0f);
+      float cpml = fma_(CPL, ql, fma_(CPV, qvl, CP));
+      float lh = fma_(-LV2, tavg, LV1);
+      float drdt = ((17.67f * (273.15f - 29.65f)) * qvs) / ((tavg - 29.65f) * (tavg - 29.65f));
+      out = lam(qvs, lh, drdt, cpml);
+    }
+  } else if (tavg <= 233.15f) {
+    float qvs = mp_rsif(pavg, tavg);
+    float qice = mx(qciavg - qvs, 0.0f);
+    if (qice > nmsat) {
+      if (wantSat != 0) sat[c] = 1.0f;
+      float qvl = mx(qciavg - qice, 0.0f);
+      float cpml = fma_(CPI, qice, fma_(CPV, qvl, CP));
+      float lh = fma_(-LS2, tavg, LS1);
+      float drdt = ((21.874557f * (273.15f - 7.66f)) * qvs) / ((tavg - 7.66f) * (tavg - 7.66f));
+      out = lam(qvs, lh, drdt, cpml);
+    }
+  } else {
+    float ff = (tavg - 233.15f) / (273.15f - 233.15f);
+    float qvsl = mp_rslf(pavg, tavg), qvsi = mp_rsif(pavg, tavg);
+    float qvs = fma_(ff, qvsl, (1.0f - ff) * qvsi);
+    float qice = mx(0.0f, (1.0f - ff) * (qciavg - qvs));
+    float ql = mx(0.0f, ff * (qciavg - qvs));
+    if (ql + qice > nmsat) {
+      if (wantSat != 0) sat[c] = 1.0f;
+      float qvl = mx(0.0f, (qciavg - ql) - qice);
+      float cpml = fma_(CPI, qice, fma_(CPL, ql, fma_(CPV, qvl, CP)));
+      float lhl = fma_(-LV2, tavg, LV1);
+      float drdtl = ((17.67f * (273.15f - 29.65f)) * qvsl) / ((tavg - 29.65f) * (tavg - 29.65f));
+      float nml = lam(qvsl, lhl, drdtl, cpml);
+      float lhi = fma_(-LS2, tavg, LS1);
+      float drdti = ((21.874557f * (273.15f - 7.66f)) * qvsi) / ((tavg - 7.66f) * (tavg - 7.66f));
+      float nmi = lam(qvsi, lhi, drdti, cpml);
+      out = fma_(ff, nml, (1.0f - ff) * nmi);
+    }
+  }
+  nm[c] = out;
+}
+
+// turb_km_kh.wgsl:125-187
+__global__ void k_turb_km_kh(uint32_t* M, KArg a) {
+  TE_TID3
+  Dims g = dims(M, a.p[0]);
+  int i = gx, j = gy, k = gz + 1;
+  if (i >= g.nx || j >= g.ny || k > g.nz - 1) return;
+  const float *tkea = fp(M, a.p[1]), *nm = fp(M, a.p[2]), *mf = fp(M, a.p[3]), *hm = fp(M, a.p[30]);
+  float *kmh = fp(M, a.p[4]), *kmv = fp(M, a.p[5]), *khh = fp(M, a.p[6]), *khv = fp(M, a.p[7]), *lenscl = fp(M, a.p[8]),
+        *dissten = fp(M, a.p[9]);
+  float kmScale = ldf(M, a.p[10]), kmFixed = ldf(M, a.p[11]), turbfac = ldf(M, a.p[12]);
+  const int64_t c = g.idx(i, j, k);
+  const float c_m = 0.1f, c_l = 0.8165f, ri_c = 0.25f, tke_min = 1e-10f, nm_min = 1e-06f, small_len_frac = 0.001f;
+  float cmz = mx(0.001f, turbfac * c_m);
+  float ce1z = ((cmz * c_l) * c_l) * (1.0f / ri_c - 1.0f);
+  float ce2z = mx(0.0f, fma_(cmz * 3.1415927f, 3.1415927f, -ce1z));
+  float dzk = (1.0f / g.rdz) / mf[k];
+  float grdscl = rr_pow((((1.0f / g.rdx) / hm_uh(hm, g, i)) * ((1.0f / g.rdy) / hm_vh(hm, g, j))) * dzk, 0.33333334f);
+  float rgrdscl = 1.0f / grdscl;
+  float tk, len;
+  if (tkea[c] <= tke_min) {
+    tk = tke_min;
+    len = small_len_frac * grdscl;
+  } else {
+    tk = tkea[c];
+    len = grdscl;
+    if (nm[c] > nm_min) {
+      len = c_l * sqrtf(tk / nm[c]);
+      len = mn(len, grdscl);
+      len = mx(len, small_len_frac * grdscl);
+    }
+  }
+  lenscl[c] = len;
+  dissten[c] = dissten[c] + ((fma_(ce2z * len, rgrdscl, ce1z) * tk) * sqrtf(tk)) / len;
+  float km = kmFixed > 0.0f ? kmFixed : ((cmz * sqrtf(tk)) * len) * kmScale;
+  kmh[c] = km;
+  kmv[c] = km;
+  float prinv = 3.0f;
+  if (nm[c] > nm_min) prinv = mn(fma_(2.0f * len, rgrdscl, 1.0f), 3.0f);
+  khh[c] = km * prinv;
+  khv[c] = km * prinv;
+}
+
+// turb_dissipative_heating.wgsl:139-194
+__global__ void k_turb_dissipative_heating(uint32_t* M, KArg a) {
+  TE_TID3
+  Dims g = dims(M, a.p[0]);
+  int i = gx, j = gy, k = gz;
+  if (i >= g.nx || j >= g.ny || k >= g.nz) return;
+  const float *dissten = fp(M, a.p[1]), *pi0 = fp(M, a.p[2]), *ppi = fp(M, a.p[3]), *qv = fp(M, a.p[4]),
+              *qc = fp(M, a.p[5]), *qr = fp(M, a.p[6]), *qi = fp(M, a.p[7]), *qs = fp(M, a.p[8]), *qg = fp(M, a.p[9]),
+              *sfc = fp(M, a.p[12]);
+  float* thten = fp(M, a.p[10]);
+  const uint32_t op = a.p[11];
+  int moist = ldi(M, op), bbc3 = ldi(M, op + 1), zntField = ldi(M, op + 4);
+  float znt = ldf(M, op + 2), dz1 = ldf(M, op + 3);
+  const int64_t sk = g.sk(), c = g.idx(i, j, k);
+  const float cv = CP - RD, cvv = CPV - RV, cpdcv = CP / (CP - RD);
+  float epsd;
+  if (k == 0 && bbc3 != 0) {
+    float ust = sfc[3 * sk + g.idx(i, j, 0)];
+    float z0 = zntField != 0 ? sfc[4 * sk + g.idx(i, j, 0)] : znt;
+    epsd = (((ust * ust) * ust) * rr_log((dz1 + z0) / z0)) / (0.4f * dz1);
+  } else {
+    epsd = 0.5f * (dissten[c] + dissten[c + sk]);
+  }
+  float pi = pi0[k] + ppi[c];
+  float dheat;
+  if (moist != 0) {
+    float cvm = fma_(2106.0f, (qi[c] + qs[c]) + qg[c], fma_(4.19e+03f, qc[c] + qr[c], fma_(cvv, qv[c], cv)));
+    dheat = epsd / ((cpdcv * cvm) * pi);
+  } else {
+    dheat = epsd / (CP * pi);
+  }
+  thten[c] = thten[c] + dheat;
+}
+
+// turb_gettau.wgsl:111-143
+__global__ void k_turb_gettau(uint32_t* M, KArg a) {
+  TE_TID3
+  Dims g = dims(M, a.p[0]);
+  int i = gx - 1, j = gy - 1, k = gz;
+  if (i > g.nx || j > g.ny || k >= g.nz) return;
+  const float *kmh = fp(M, a.p[1]), *kmv = fp(M, a.p[2]);
+  float *t11 = fp(M, a.p[3]), *t22 = fp(M, a.p[4]), *t33 = fp(M, a.p[5]), *t12 = fp(M, a.p[6]), *t13 = fp(M, a.p[7]),
+        *t23 = fp(M, a.p[8]);
+  const int64_t sj = g.sJ, sk = g.sk(), c = g.idx(i, j, k);
+  float tem = kmh[c] + kmh[c + sk];
+  t11[c] = t11[c] * tem;
+  t22[c] = t22[c] * tem;
+  t33[c] = t33[c] * tem;
+  float q = ((kmh[c - 1 - sj] + kmh[c]) + (kmh[c - 1] + kmh[c - sj])) +
+            ((kmh[c - 1 - sj + sk] + kmh[c + sk]) + (kmh[c - 1 + sk] + kmh[c - sj + sk]));
+  t12[c] = t12[c] * (0.25f * q);
+  if (k >= 1 && k <= g.nz - 1 && i >= 0 && j >= 0) {
+    t13[c] = t13[c] * (kmv[c - 1] + kmv[c]);
+    t23[c] = t23[c] * (kmv[c - sj] + kmv[c]);
+  }
+}
+
+// turb_sfcstress.wgsl:132-207 (free-slip and no-slip bottom; surface-flux
+// modes need the surface-layer planes and are not on this path)
+__global__ void k_turb_sfcstress(uint32_t* M, KArg a) {
+  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x), j = (int)(blockIdx.y * blockDim.y + threadIdx.y);
+  if (i >= (int)a.n[0] || j >= (int)a.n[1]) return;
+  Dims g = dims(M, a.p[0]);
+  if (i > g.nx || j > g.ny) return;
+  const float *rf = fp(M, a.p[2]), *u = fp(M, a.p[6]), *v = fp(M, a.p[7]), *kmv = fp(M, a.p[9]), *mf = fp(M, a.p[11]);
+  float *t13 = fp(M, a.p[3]), *t23 = fp(M, a.p[4]);
+  int mode = ldi(M, a.p[5]), topNoSlip = ldi(M, a.p[10]);
+  float movex = ldf(M, a.p[8]), movey = ldf(M, a.p[8] + 1);
+  const int64_t plane = g.sk(), sjj = g.sJ;
+  {
+    const int64_t ct = g.idx(i, j, g.nz);
+    if (topNoSlip != 0) {
+      const int64_t cu = ct - plane;
+      float mftu = 0.5f * (mf[g.nz] + mf[g.nz]), mftv = 0.5f * (mf[g.nz] + mf[g.nz]);
+      t13[ct] = ((((((-2.0f * (u[cu] + movex)) * g.rdz) * mftu) * 0.5f) * (rf[ct - 1] + rf[ct])) * 0.5f) * (kmv[cu - 1] + kmv[cu]);
+      t23[ct] = ((((((-2.0f * (v[cu] + movey)) * g.rdz) * mftv) * 0.5f) * (rf[ct - sjj] + rf[ct])) * 0.5f) * (kmv[cu - sjj] + kmv[cu]);
+    } else {
+      t13[ct] = t13[ct - plane];
+      t23[ct] = t23[ct - plane];
+    }
+  }
+  const int64_t c = g.idx(i, j, 0);
+  if (mode == 4) {
+    float mfbu = 0.5f * (mf[0] + mf[0]), mfbv = 0.5f * (mf[0] + mf[0]);
+    t13[c] = ((((((2.0f * (u[c] + movex)) * g.rdz) * mfbu) * 0.5f) * (rf[c - 1] + rf[c])) * 0.5f) * (kmv[c - 1 + plane] + kmv[c + plane]);
+    t23[c] = ((((((2.0f * (v[c] + movey)) * g.rdz) * mfbv) * 0.5f) * (rf[c - sjj] + rf[c])) * 0.5f) * (kmv[c - sjj + plane] + kmv[c + plane]);
+    return;
+  }
+  t13[c] = t13[c + plane];
+  t23[c] = t23[c + plane];
+}
+
+// tke_epst.wgsl:170-206 (shear production)
+__global__ void k_tke_epst(uint32_t* M, KArg a) {
+  TE_TID3
+  Dims g = dims(M, a.p[0]);
+  int i = gx, j = gy, k = gz + 1;
+  if (i >= g.nx || j >= g.ny || k > g.nz - 1) return;
+  const float *u = fp(M, a.p[1]), *v = fp(M, a.p[2]), *w = fp(M, a.p[3]), *t11 = fp(M, a.p[4]), *t12 = fp(M, a.p[5]),
+              *t13 = fp(M, a.p[6]), *t22 = fp(M, a.p[7]), *t23 = fp(M, a.p[8]), *t33 = fp(M, a.p[9]), *rf = fp(M, a.p[10]),
+              *mh = fp(M, a.p[11]), *mf = fp(M, a.p[12]), *c1 = fp(M, a.p[13]), *c2 = fp(M, a.p[14]), *hm = fp(M, a.p[30]);
+  const int64_t sj = g.sJ, sk = g.sk(), c = g.idx(i, j, k), cd = c - sk;
+  auto L1 = [&](int64_t cc) {
+    return fma_(((u[cc + 1] - u[cc]) * g.rdx) * hm_uh(hm, g, i), t11[cc], (((v[cc + sj] - v[cc]) * g.rdy) * hm_vh(hm, g, j)) * t22[cc]);
+  };
+  auto L2 = [&](int64_t cc, int ii, int jj) {
+    return t12[cc] * fma_((u[cc] - u[cc - sj]) * g.rdy, hm_vf(hm, g, jj), ((v[cc] - v[cc - 1]) * g.rdx) * hm_uf(hm, g, ii));
+  };
+  auto L3 = [&](int64_t cc, int ii) {
+    return fma_((w[cc] - w[cc - 1]) * g.rdx, hm_uf(hm, g, ii), ((u[cc] - u[cc - sk]) * g.rdz) * mf[k]) * t13[cc];
+  };
+  auto L4 = [&](int64_t cc, int jj) {
+    return fma_((w[cc] - w[cc - sj]) * g.rdy, hm_vf(hm, g, jj), ((v[cc] - v[cc - sk]) * g.rdz) * mf[k]) * t23[cc];
+  };
+  auto L5 = [&](int64_t cc, int lev) { return (((w[cc + sk] - w[cc]) * g.rdz) * mh[lev]) * t33[cc]; };
+  float e = fma_(c1[k], L1(cd), c2[k] * L1(c));
+  float up = (L2(c, i, j) + L2(c + 1 + sj, i + 1, j + 1)) + (L2(c + sj, i, j + 1) + L2(c + 1, i + 1, j));
+  float dn = (L2(cd, i, j) + L2(cd + 1 + sj, i + 1, j + 1)) + (L2(cd + sj, i, j + 1) + L2(cd + 1, i + 1, j));
+  e = fma_(0.25f, fma_(c2[k], up, c1[k] * dn), e);
+  e = fma_(0.5f, (L3(c, i) + L3(c + 1, i + 1)) + (L4(c, j) + L4(c + sj, j + 1)), e);
+  e = e + fma_(c1[k], L5(cd, k - 1), c2[k] * L5(c, k));
+  fp(M, a.p[15])[c] = e / rf[c];
+}
+
+// tke_sources.wgsl:111-129
+__global__ void k_tke_sources(uint32_t* M, KArg a) {
+  TE_TID3
+  Dims g = dims(M, a.p[0]);
+  int i = gx, j = gy, k = gz + 1;
+  if (i >= g.nx || j >= g.ny || k > g.nz - 1) return;
+  const int64_t c = g.idx(i, j, k);
+  const float *khv = fp(M, a.p[1]), *nm = fp(M, a.p[2]), *dissten = fp(M, a.p[3]), *epst = fp(M, a.p[4]);
+  fp(M, a.p[5])[c] = fma_(-khv[c], nm[c], -dissten[c]) + epst[c];
+}
+
+// turb_u_tendency.wgsl:146-294 / turb_v_tendency.wgsl
+template <int DIR>
+__device__ void uv_tendency(uint32_t* M, const KArg& a, int i, int j) {
+  Dims g = dims(M, a.p[0]);
+  if (DIR == 0 ? (i > g.nx || j >= g.ny) : (i >= g.nx || j > g.ny)) return;
+  const float *q = fp(M, a.p[1]), *w = fp(M, a.p[2]), *rho = fp(M, a.p[3]), *rf = fp(M, a.p[4]), *kmv = fp(M, a.p[5]),
+              *tA = fp(M, a.p[6]), *tB = fp(M, a.p[7]), *tC = fp(M, a.p[8]), *mh = fp(M, a.p[9]), *mf = fp(M, a.p[10]),
+              *hm = fp(M, a.p[30]);
+  float dt = ldf(M, a.p[11]);
+  float *cA = fp(M, a.p[12]), *cB = fp(M, a.p[13]), *dwd = fp(M, a.p[14]), *ten = fp(M, a.p[15]);
+  int openx = ldi(M, a.p[16]), openy = ldi(M, a.p[16] + 1), explicitZ = ldi(M, a.p[17]);
+  const int64_t sj = g.sJ, sk = g.sk(), st = DIR == 0 ? 1 : sj, base = g.idx(i, j, 0);
+  bool zeroX = DIR == 0 ? (openx != 0 && (i == 0 || i == g.nx)) : (openx != 0 && (i == 0 || i == g.nx - 1));
+  bool zeroY = DIR == 0 ? (openy != 0 && (j == 0 || j == g.ny - 1)) : (openy != 0 && (j == 0 || j == g.ny));
+  auto turbxy = [&](int64_t c, float& tx, float& ty) {
+    if (DIR == 0) {
+      tx = zeroX ? 0.0f : ((tA[c] - tA[c - 1]) * g.rdx) * hm_uf(hm, g, i);
+      ty = zeroY ? 0.0f : ((tB[c + sj] - tB[c]) * g.rdy) * hm_vh(hm, g, j);
+    } else {
+      tx = zeroX ? 0.0f : ((tA[c + 1] - tA[c]) * g.rdx) * hm_uh(hm, g, i);
+      ty = zeroY ? 0.0f : ((tB[c] - tB[c - sj]) * g.rdy) * hm_vf(hm, g, j);
+    }
+  };
+  auto rinv = [&](int k) { const int64_t c = base + k * sk; return 1.0f / (0.5f * (rho[c - st] + rho[c])); };
+  if (explicitZ != 0) {
+    for (int k = 0; k < g.nz; ++k) {
+      const int64_t c = base + k * sk;
+      float tx, ty;
+      turbxy(c, tx, ty);
+      float tz = (((tC[c + sk] - tC[c]) * g.rdz) * 0.5f) * (mh[k] + mh[k]);
+      ten[c] = ten[c] + ((tx + ty) + tz) / (0.5f * (rho[c - st] + rho[c]));
+    }
+    return;
+  }
+  float tema = ((-0.0625f * dt) * g.rdz) * g.rdz;
+  float temc = (0.5f * dt) * g.rdz;
+  float rdt = 0.5f / dt;
+  const float hmf = DIR == 0 ? hm_uf(hm, g, i) : hm_vf(hm, g, j), rdh = DIR == 0 ? g.rdx : g.rdy;
+  for (int k = 0; k <= g.nz; ++k) {
+    const int64_t c = base + k * sk;
+    dwd[c] = (((((w[c] - w[c - st]) * rdh) * hmf) * 0.25f) * (kmv[c - st] + kmv[c])) * (rf[c - st] + rf[c]);
+  }
+  auto L1 = [&](int k, int lev) {
+    const int64_t cl = base + lev * sk;
+    float tem = (mh[k] + mh[k]) * rinv(k);
+    return (((kmv[cl - st] + kmv[cl]) * (mf[lev] + mf[lev])) * (rf[cl - st] + rf[cl])) * tem;
+  };
+  float pa, pb;
+  {
+    float cfc = tema * L1(0, 1);
+    float cfb = 1.0f - cfc;
+    float cfd = fma_(-((temc * tC[base]) * (mh[0] + mh[0])), rinv(0), q[base]);
+    float t = 1.0f / cfb;
+    pa = (-cfc) * t;
+    pb = cfd * t;
+    cA[base] = pa;
+    cB[base] = pb;
+  }
+  for (int k = 1; k < g.nz - 1; ++k) {
+    const int64_t c = base + k * sk;
+    float cfa = tema * L1(k, k), cfc = tema * L1(k, k + 1);
+    float cfb = (1.0f - cfa) - cfc;
+    float t = 1.0f / fma_(cfa, pa, cfb);
+    pa = (-cfc) * t;
+    pb = fma_(-cfa, pb, q[c]) * t;
+    cA[c] = pa;
+    cB[c] = pb;
+  }
+  const int kt = g.nz - 1;
+  const int64_t ct = base + kt * sk;
+  {
+    float cfa = tema * L1(kt, kt);
+    float cfb = 1.0f - cfa;
+    float cfd = fma_((temc * tC[base + g.nz * sk]) * (mh[kt] + mh[kt]), rinv(kt), q[ct]);
+    float t = 1.0f / fma_(cfa, pa, cfb);
+    pb = fma_(-cfa, pb, cfd) * t;
+    cB[ct] = pb;
+  }
+  for (int k = g.nz - 2; k >= 0; --k) {
+    const int64_t c = base + k * sk;
+    pb = fma_(cA[c], pb, cB[c]);
+    cB[c] = pb;
+  }
+  for (int k = 0; k < g.nz; ++k) {
+    const int64_t c = base + k * sk;
+    float tx, ty;
+    turbxy(c, tx, ty);
+    float tz = ((rho[c - st] + rho[c]) * (cB[c] - q[c])) * rdt;
+    tz = fma_(((dwd[c + sk] - dwd[c]) * g.rdz) * 0.5f, mh[k] + mh[k], tz);
+    ten[c] = fma_((tx + ty) + tz, rinv(k), ten[c]);
+  }
+}
+__global__ void k_turb_u_tendency(uint32_t* M, KArg a) {
+  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x), j = (int)(blockIdx.y * blockDim.y + threadIdx.y);
+  if (i < (int)a.n[0] && j < (int)a.n[1]) uv_tendency<0>(M, a, i, j);
+}
+__global__ void k_turb_v_tendency(uint32_t* M, KArg a) {
+  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x), j = (int)(blockIdx.y * blockDim.y + threadIdx.y);
+  if (i < (int)a.n[0] && j < (int)a.n[1]) uv_tendency<1>(M, a, i, j);
+}
+
+// turb_w_tendency.wgsl:152-287
+__global__ void k_turb_w_tendency(uint32_t* M, KArg a) {
+  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x), j = (int)(blockIdx.y * blockDim.y + threadIdx.y);
+  if (i >= (int)a.n[0] || j >= (int)a.n[1]) return;
+  Dims g = dims(M, a.p[0]);
+  if (i >= g.nx || j >= g.ny) return;
+  const float *w = fp(M, a.p[1]), *rho = fp(M, a.p[2]), *rf = fp(M, a.p[3]), *kmh = fp(M, a.p[4]), *t13 = fp(M, a.p[5]),
+              *t23 = fp(M, a.p[6]), *t33 = fp(M, a.p[7]), *mh = fp(M, a.p[8]), *mf = fp(M, a.p[9]), *hm = fp(M, a.p[30]);
+  float dt = ldf(M, a.p[10]);
+  float *cA = fp(M, a.p[11]), *cB = fp(M, a.p[12]), *wten = fp(M, a.p[13]);
+  int openx = ldi(M, a.p[14]), openy = ldi(M, a.p[14] + 1), explicitZ = ldi(M, a.p[15]);
+  const int64_t sj = g.sJ, sk = g.sk(), base = g.idx(i, j, 0);
+  bool zeroX = openx != 0 && (i == 0 || i == g.nx - 1);
+  bool zeroY = openy != 0 && (j == 0 || j == g.ny - 1);
+  auto turbxy = [&](int64_t c, float& tx, float& ty) {
+    tx = zeroX ? 0.0f : ((t13[c + 1] - t13[c]) * g.rdx) * hm_uh(hm, g, i);
+    ty = zeroY ? 0.0f : ((t23[c + sj] - t23[c]) * g.rdy) * hm_vh(hm, g, j);
+  };
+  if (explicitZ != 0) {
+    for (int k = 1; k < g.nz; ++k) {
+      const int64_t c = base + k * sk;
+      float tx, ty;
+      turbxy(c, tx, ty);
+      float tz = ((t33[c] - t33[c - sk]) * g.rdz) * mf[k];
+      wten[c] = wten[c] + ((tx + ty) + tz) / rf[c];
+    }
+    return;
+  }
+  float tema = ((-1.0f * dt) * g.rdz) * g.rdz;
+  float temc = dt * g.rdz;
+  float rdt = 1.0f / dt;
+  auto mfrf = [&](int k) { return mf[k] / rf[base + k * sk]; };
+  auto L1 = [&](int k) { const int64_t c = base + k * sk; return (((kmh[c] + kmh[c + sk]) * mh[k]) * rho[c]) * mfrf(k); };
+  auto L4 = [&](int k) { const int64_t c = base + k * sk; return (((kmh[c - sk] + kmh[c]) * mh[k - 1]) * rho[c - sk]) * mfrf(k); };
+  float pa, pb;
+  {
+    const int64_t c = base + sk;
+    float cfc = tema * L1(1);
+    float cfb = 1.0f - cfc;
+    float cfd = fma_(-(temc * t33[c - sk]), mfrf(1), w[c]);
+    float t = 1.0f / cfb;
+    pa = (-cfc) * t;
+    pb = cfd * t;
+    cA[c] = pa;
+    cB[c] = pb;
+  }
+  for (int k = 2; k < g.nz - 1; ++k) {
+    const int64_t c = base + k * sk;
+    float cfa = tema * L4(k), cfc = tema * L1(k);
+    float cfb = (1.0f - cfa) - cfc;
+    float t = 1.0f / fma_(cfa, pa, cfb);
+    pa = (-cfc) * t;
+    pb = fma_(-cfa, pb, w[c]) * t;
+    cA[c] = pa;
+    cB[c] = pb;
+  }
+  const int kt = g.nz - 1;
+  const int64_t ct = base + kt * sk;
+  {
+    float cfa = tema * L4(kt);
+    float cfb = 1.0f - cfa;
+    float cfd = fma_(temc * t33[ct], mfrf(kt), w[ct]);
+    float t = 1.0f / fma_(cfa, pa, cfb);
+    pb = fma_(-cfa, pb, cfd) * t;
+    cB[ct] = pb;
+  }
+  for (int k = g.nz - 2; k >= 1; --k) {
+    const int64_t c = base + k * sk;
+    pb = fma_(cA[c], pb, cB[c]);
+    cB[c] = pb;
+  }
+  for (int k = 1; k < g.nz; ++k) {
+    const int64_t c = base + k * sk;
+    float tx, ty;
+    turbxy(c, tx, ty);
+    float tz = (rf[c] * (cB[c] - w[c])) * rdt;
+    wten[c] = wten[c] + ((tx + ty) + tz) / rf[c];
+  }
+}
+
+// turb_scalar_flux_h.wgsl:125-148
+__global__ void k_turb_scalar_flux_h(uint32_t* M, KArg a) {
+  TE_TID3
+  Dims g = dims(M, a.p[0]);
+  int i = gx, j = gy, k = gz;
+  if (i > g.nx || j > g.ny || k >= g.nz) return;
+  const float *s = fp(M, a.p[1]), *rho = fp(M, a.p[2]), *khh = fp(M, a.p[3]), *hm = fp(M, a.p[30]);
+  float leNum = ldf(M, a.p[4]);
+  const int64_t sj = g.sJ, sk = g.sk(), c = g.idx(i, j, k);
+  fp(M, a.p[5])[c] = (((((-0.125f * (rho[c] + rho[c - 1])) * leNum) * ((khh[c] + khh[c - 1]) + (khh[c + sk] + khh[c - 1 + sk]))) *
+                       (s[c] - s[c - 1])) * g.rdx) * hm_uf(hm, g, i);
+  fp(M, a.p[6])[c] = (((((-0.125f * (rho[c] + rho[c - sj])) * leNum) * ((khh[c] + khh[c - sj]) + (khh[c + sk] + khh[c - sj + sk]))) *
+                       (s[c] - s[c - sj])) * g.rdy) * hm_vf(hm, g, j);
+}
+
+// turb_scalar_vert_implicit.wgsl:210-327
+__global__ void k_turb_scalar_vert_implicit(uint32_t* M, KArg a) {
+  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x), j = (int)(blockIdx.y * blockDim.y + threadIdx.y);
+  if (i >= (int)a.n[0] || j >= (int)a.n[1]) return;
+  Dims g = dims(M, a.p[0]);
+  if (i >= g.nx || j >= g.ny) return;
+  const float *s = fp(M, a.p[1]), *rho = fp(M, a.p[2]), *rr = fp(M, a.p[3]), *rf = fp(M, a.p[4]), *khv = fp(M, a.p[5]),
+              *mh = fp(M, a.p[6]), *mf = fp(M, a.p[7]), *fx = fp(M, a.p[8]), *fy = fp(M, a.p[9]), *sflux = fp(M, a.p[14]),
+              *hm = fp(M, a.p[30]);
+  float dt = ldf(M, a.p[10]);
+  float *cA = fp(M, a.p[11]), *cB = fp(M, a.p[12]), *sten = fp(M, a.p[13]);
+  int useFlux = ldi(M, a.p[15]), explicitZ = ldi(M, a.p[16]), bcturbs = ldi(M, a.p[18]);
+  const uint32_t ow = a.p[17];
+  float wall0 = ldf(M, ow), wall1 = ldf(M, ow + 1), wall2 = ldf(M, ow + 2), wall3 = ldf(M, ow + 3);
+  const int64_t sj = g.sJ, sk = g.sk(), base = g.idx(i, j, 0);
+  auto L2 = [&](int k) {
+    const int64_t cc = base + k * sk;
+    return ((((-khv[cc]) * (s[cc] - s[cc - sk])) * g.rdz) * mf[k]) * rf[cc];
+  };
+  auto hdiv = [&](int64_t c) {
+    return ((-(fx[c + 1] - fx[c])) * g.rdx) * hm_uh(hm, g, i) + ((-(fy[c + sj] - fy[c])) * g.rdy) * hm_vh(hm, g, j);
+  };
+  if (explicitZ != 0) {
+    auto L1 = [&](int kf) -> float {
+      const int64_t c = base + kf * sk;
+      if (kf == 0) {
+        if (wall3 == 0.0f) return useFlux != 0 ? sflux[base] * rf[c] : (bcturbs == 2 ? L2(1) : 0.0f);
+        if (wall3 == 2.0f) return (wall0 * wall1) * rf[c];
+        return (((((-wall0) * 2.0f) * (s[c] - wall1)) * g.rdz) * mf[0]) * rf[c];
+      }
+      if (kf == g.nz) {
+        if (wall3 == 0.0f) return bcturbs == 2 ? L2(g.nz - 1) : 0.0f;
+        const int64_t cb = base + (g.nz - 1) * sk;
+        if (wall3 == 2.0f) return (wall0 * wall2) * rf[c];
+        return (((((-wall0) * 2.0f) * (wall2 - s[cb])) * g.rdz) * mf[g.nz]) * rf[c];
+      }
+      return L2(kf);
+    };
+    float below = L1(0);
+    for (int k = 0; k < g.nz; ++k) {
+      const int64_t c = base + k * sk;
+      float above = L1(k + 1);
+      float tz = ((-(above - below)) * g.rdz) * mh[k];
+      sten[c] = fma_(hdiv(c) + tz, rr[c], sten[c]);
+      below = above;
+    }
+    return;
+  }
+  float tema = ((-dt) * g.rdz) * g.rdz;
+  float rdt = 1.0f / dt;
+  float temc = dt * g.rdz;
+  auto L3 = [&](int k) { const int64_t c = base + k * sk; return (((khv[c + sk] * mf[k + 1]) * rf[c + sk]) * mh[k]) * rr[c]; };
+  auto L4 = [&](int k) { const int64_t c = base + k * sk; return (((khv[c] * mf[k]) * rf[c]) * mh[k]) * rr[c]; };
+  float pa, pb;
+  {
+    float cfc = tema * L3(0);
+    float cfb = 1.0f - cfc;
+    float t = 1.0f / cfb;
+    pa = (-cfc) * t;
+    cA[base] = pa;
+    float cfd = s[base];
+    if (useFlux != 0 || bcturbs == 2) {
+      float dumz;
+      if (useFlux != 0) dumz = sflux[base] * rf[base];
+      else {
+        const int64_t c1 = base + sk;
+        dumz = ((((-khv[c1]) * (s[c1] - s[base])) * g.rdz) * mf[1]) * rf[c1];
+      }
+      cfd = fma_((temc * dumz) * mh[0], rr[base], s[base]);
+    }
+    pb = cfd * t;
+    cB[base] = pb;
+  }
+  for (int k = 1; k < g.nz - 1; ++k) {
+    const int64_t c = base + k * sk;
+    float cfa = tema * L4(k), cfc = tema * L3(k);
+    float cfb = (1.0f - cfa) - cfc;
+    float t = 1.0f / fma_(cfa, pa, cfb);
+    pa = (-cfc) * t;
+    pb = fma_(-cfa, pb, s[c]) * t;
+    cA[c] = pa;
+    cB[c] = pb;
+  }
+  const int kt = g.nz - 1;
+  const int64_t ct = base + kt * sk;
+  {
+    float cfa = tema * L4(kt);
+    float cfb = 1.0f - cfa;
+    float t = 1.0f / fma_(cfa, pa, cfb);
+    float cfd = s[ct];
+    if (bcturbs == 2) {
+      float dumz = ((((-khv[ct]) * (s[ct] - s[ct - sk])) * g.rdz) * mf[kt]) * rf[ct];
+      cfd = fma_(-((temc * dumz) * mh[kt]), rr[ct], s[ct]);
+    }
+    pb = fma_(-cfa, pb, cfd) * t;
+    cB[ct] = pb;
+    sten[ct] = fma_(hdiv(ct) + (rho[ct] * (pb - s[ct])) * rdt, rr[ct], sten[ct]);
+  }
+  for (int k = g.nz - 2; k >= 0; --k) {
+    const int64_t c = base + k * sk;
+    pb = fma_(cA[c], pb, cB[c]);
+    cB[c] = pb;
+    sten[c] = fma_(hdiv(c) + (rho[c] * (pb - s[c])) * rdt, rr[c], sten[c]);
+  }
+}
+
+// tke_flux_h.wgsl:125-146
+__global__ void k_tke_flux_h(uint32_t* M, KArg a) {
+  TE_TID3
+  Dims g = dims(M, a.p[0]);
+  int i = gx, j = gy, k = gz + 1;
+  if (i > g.nx || j > g.ny || k > g.nz - 1) return;
+  const float *t = fp(M, a.p[1]), *rf = fp(M, a.p[2]), *kmh = fp(M, a.p[3]), *hm = fp(M, a.p[30]);
+  const int64_t sj = g.sJ, c = g.idx(i, j, k);
+  fp(M, a.p[4])[c] =
+      (((((-0.25f * (rf[c] + rf[c - 1])) * 2.0f) * (kmh[c] + kmh[c - 1])) * (t[c] - t[c - 1])) * g.rdx) * hm_uf(hm, g, i);
+  fp(M, a.p[5])[c] =
+      (((((-0.25f * (rf[c] + rf[c - sj])) * 2.0f) * (kmh[c] + kmh[c - sj])) * (t[c] - t[c - sj])) * g.rdy) * hm_vf(hm, g, j);
+}
+
+// tke_transport.wgsl:158-301
+__global__ void k_tke_transport(uint32_t* M, KArg a) {
+  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x), j = (int)(blockIdx.y * blockDim.y + threadIdx.y);
+  if (i >= (int)a.n[0] || j >= (int)a.n[1]) return;
+  Dims g = dims(M, a.p[0]);
+  if (i >= g.nx || j >= g.ny) return;
+  const float *t = fp(M, a.p[1]), *rho = fp(M, a.p[2]), *rf = fp(M, a.p[3]), *kmh = fp(M, a.p[4]), *kmv = fp(M, a.p[5]),
+              *mh = fp(M, a.p[6]), *mf = fp(M, a.p[7]), *fx = fp(M, a.p[11]), *fy = fp(M, a.p[12]), *hm = fp(M, a.p[30]);
+  float dt = ldf(M, a.p[8]);
+  float *cA = fp(M, a.p[9]), *cB = fp(M, a.p[10]), *tketen = fp(M, a.p[13]);
+  int openx = ldi(M, a.p[14]), openy = ldi(M, a.p[14] + 1), explicitZ = ldi(M, a.p[15]);
+  const int64_t sj = g.sJ, sk = g.sk(), base = g.idx(i, j, 0);
+  bool zeroX = openx != 0 && (i == 0 || i == g.nx - 1);
+  bool zeroY = openy != 0 && (j == 0 || j == g.ny - 1);
+  auto turbxy = [&](int64_t c, float& tx, float& ty) {
+    tx = zeroX ? 0.0f : ((-(fx[c + 1] - fx[c])) * g.rdx) * hm_uh(hm, g, i);
+    ty = zeroY ? 0.0f : ((-(fy[c + sj] - fy[c])) * g.rdy) * hm_vh(hm, g, j);
+  };
+  auto mfrf = [&](int k) { return mf[k] / rf[base + k * sk]; };
+  if (explicitZ != 0) {
+    auto L1 = [&](int k) {
+      const int64_t c = base + k * sk;
+      return ((((-(kmv[c] + kmv[c + sk])) * (t[c + sk] - t[c])) * g.rdz) * mh[k]) * rho[c];
+    };
+    float below = L1(0);
+    for (int k = 1; k < g.nz; ++k) {
+      const int64_t c = base + k * sk;
+      float here = L1(k);
+      float tx, ty;
+      turbxy(c, tx, ty);
+      float tz = ((-(here - below)) * g.rdz) * mf[k];
+      tketen[c] = tketen[c] + ((tx + ty) + tz) / rf[c];
+      below = here;
+    }
+    return;
+  }
+  float tema = ((-1.0f * dt) * g.rdz) * g.rdz;
+  float temc = dt * g.rdz;
+  float rdt = 1.0f / dt;
+  auto L2 = [&](int k) { const int64_t c = base + k * sk; return (((kmh[c] + kmh[c + sk]) * mh[k]) * rho[c]) * mfrf(k); };
+  auto L5 = [&](int k) { const int64_t c = base + k * sk; return (((kmh[c - sk] + kmh[c]) * mh[k - 1]) * rho[c - sk]) * mfrf(k); };
+  float pa, pb;
+  {
+    const int64_t c = base + sk;
+    float cfc = tema * L2(1);
+    float cfb = 1.0f - cfc;
+    float tem = ((((-(kmv[c - sk] + kmv[c])) * (t[c] - t[c - sk])) * g.rdz) * mh[0]) * rho[c - sk];
+    float cfd = fma_(temc * tem, mfrf(1), t[c]);
+    float inv = 1.0f / cfb;
+    pa = (-cfc) * inv;
+    pb = cfd * inv;
+    cA[c] = pa;
+    cB[c] = pb;
+  }
+  for (int k = 2; k < g.nz - 1; ++k) {
+    const int64_t c = base + k * sk;
+    float cfa = tema * L5(k), cfc = tema * L2(k);
+    float cfb = (1.0f - cfa) - cfc;
+    float inv = 1.0f / fma_(cfa, pa, cfb);
+    pa = (-cfc) * inv;
+    pb = fma_(-cfa, pb, t[c]) * inv;
+    cA[c] = pa;
+    cB[c] = pb;
+  }
+  const int kt = g.nz - 1;
+  const int64_t ct = base + kt * sk;
+  {
+    float cfa = tema * L5(kt);
+    float cfb = 1.0f - cfa;
+    float tem = ((((-(kmv[ct] + kmv[ct + sk])) * (t[ct + sk] - t[ct])) * g.rdz) * mh[kt]) * rho[ct];
+    float cfd = fma_(-(temc * tem), mfrf(kt), t[ct]);
+    float inv = 1.0f / fma_(cfa, pa, cfb);
+    pb = fma_(-cfa, pb, cfd) * inv;
+    cB[ct] = pb;
+  }
+  for (int k = g.nz - 2; k >= 1; --k) {
+    const int64_t c = base + k * sk;
+    pb = fma_(cA[c], pb, cB[c]);
+    cB[c] = pb;
+  }
+  for (int k = 1; k < g.nz; ++k) {
+    const int64_t c = base + k * sk;
+    float tx, ty;
+    turbxy(c, tx, ty);
+    float tz = (rf[c] * (cB[c] - t[c])) * rdt;
+    tketen[c] = tketen[c] + ((tx + ty) + tz) / rf[c];
+  }
+}
+
+}  // namespace te

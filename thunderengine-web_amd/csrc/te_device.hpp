@@ -1,0 +1,218 @@
+// Device-side building blocks of the CM1 large step on gfx950.
+//
+// Arithmetic contract (shared by every kernel of this library, and the reason
+// the build flags are what they are):
+//   * one IEEE binary32 operation per source operation: -ffp-contract=off, and
+//     __builtin_fmaf exactly where the reference step fuses (its WGSL writes
+//     fma(), e.g. data/kernels/acoustic_w_solve.wgsl:129-133);
+//   * subnormal operands and results flush to zero with the sign kept
+//     (-fgpu-flush-denormals-to-zero): the reference recording was made that way;
+//   * division and sqrt are correctly rounded (hipcc's default for f32);
+//   * min/max are explicit selects with the reference's tie rule.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace te {
+
+// Per-dispatch argument block: thread grid and the word offset of each
+// parameter inside the flat memory M (parameter n at p[n]).
+struct KArg {
+  uint32_t n[3];
+  uint32_t p[32];
+};
+
+#define TE_DEV __device__ __forceinline__
+
+TE_DEV float ldf(const uint32_t* M, uint32_t o) { return __uint_as_float(M[o]); }
+TE_DEV int32_t ldi(const uint32_t* M, uint32_t o) { return (int32_t)M[o]; }
+TE_DEV float* fp(uint32_t* M, uint32_t o) { return reinterpret_cast<float*>(M + o); }
+
+struct Dims {
+  int nx, ny, nz, sJ, sK;
+  float rdx, rdy, rdz;
+  TE_DEV int64_t sk() const { return (int64_t)sJ * sK; }
+  TE_DEV int64_t idx(int i, int j, int k) const { return (int64_t)(i + 3) + (int64_t)sJ * ((j + 3) + (int64_t)sK * k); }
+};
+// GridDims bytes: nx ny nz strideJ strideK 1/dx 1/dy 1/dz (timestep.js:88-90).
+TE_DEV Dims dims(const uint32_t* M, uint32_t o) {
+  Dims g;
+  g.nx = ldi(M, o); g.ny = ldi(M, o + 1); g.nz = ldi(M, o + 2); g.sJ = ldi(M, o + 3); g.sK = ldi(M, o + 4);
+  g.rdx = ldf(M, o + 5); g.rdy = ldf(M, o + 6); g.rdz = ldf(M, o + 7);
+  return g;
+}
+struct Acoustic { float dts, alph, beta, kdiv, smeps; };
+// AcousticParamsGPU bytes: dts alph beta kdiv smeps (timestep.js:46-52).
+TE_DEV Acoustic acoustic(const uint32_t* M, uint32_t o) {
+  return {ldf(M, o), ldf(M, o + 1), ldf(M, o + 2), ldf(M, o + 3), ldf(M, o + 4)};
+}
+
+TE_DEV float fma_(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+TE_DEV float mx(float a, float b) { return (a < b || a != a) ? b : a; }
+TE_DEV float mn(float a, float b) { return (b < a || a != a) ? b : a; }
+TE_DEV float ab(float a) { return __builtin_fabsf(a); }
+TE_DEV float copysign_mag(float mag, float sgn) {
+  return __uint_as_float((__float_as_uint(mag) & 0x7fffffffu) | (__float_as_uint(sgn) & 0x80000000u));
+}
+
+// Metric buffer (grid.js:200-214): uh | uf | vh | vf ...
+TE_DEV float hm_uh(const float* hm, const Dims& g, int i) { return hm[i + 3]; }
+TE_DEV float hm_uf(const float* hm, const Dims& g, int i) { return hm[g.sJ + i + 3]; }
+TE_DEV float hm_vh(const float* hm, const Dims& g, int j) { return hm[2 * g.sJ + j + 3]; }
+TE_DEV float hm_vf(const float* hm, const Dims& g, int j) { return hm[2 * g.sJ + g.sK + j + 3]; }
+
+constexpr float RD = 287.04f, RV = 461.5f, CP = 1005.7f, CV = 718.66003f, CPV = 1.87e+03f, CVV = 1408.5f,
+                ROVCP = 0.28541315f, CPL = 4.19e+03f, CPI = 2106.0f, P00 = 1e+05f, GRAV = 9.81f;
+
+// ---- advection stencils
+// 5th-order upwind face value, weights (2,-13,47,27,-3)/60 (adv_flux_x.wgsl:106-113)
+TE_DEV float flx5(float s1, float s2, float s3, float s4, float s5) {
+  return fma_(-3.0f / 60.0f, s5, fma_(27.0f / 60.0f, s4, fma_(47.0f / 60.0f, s3, fma_(2.0f / 60.0f, s1, (-13.0f / 60.0f) * s2))));
+}
+// 3rd-order face value, weights (-1,5,2)/6 (adv_flux_z.wgsl:106-111)
+TE_DEV float flx3(float s1, float s2, float s3) {
+  return fma_(2.0f / 6.0f, s3, fma_(-1.0f / 6.0f, s1, (5.0f / 6.0f) * s2));
+}
+// WENO5 with WENO-Z style weights (adv_flux_x_weno.wgsl:121-151)
+TE_DEV float weno5(float s1, float s2, float s3, float s4, float s5, float weps) {
+  const float c1312 = 13.0f / 12.0f;
+  float a1 = fma_(-2.0f, s2, s1) + s3, q1 = fma_(3.0f, s3, fma_(-4.0f, s2, s1));
+  float b1 = fma_(c1312 * a1, a1, (0.25f * q1) * q1);
+  float a2 = fma_(-2.0f, s3, s2) + s4, q2 = s2 - s4;
+  float b2 = fma_(c1312 * a2, a2, (0.25f * q2) * q2);
+  float a3 = fma_(-2.0f, s4, s3) + s5, q3 = fma_(3.0f, s3, -(4.0f * s4)) + s5;
+  float b3 = fma_(c1312 * a3, a3, (0.25f * q3) * q3);
+  float tau = ab(b1 - b3);
+  float r1 = mn(1e+30f, tau / (b1 + weps));
+  float r2 = mn(1e+30f, tau / (b2 + weps));
+  float r3 = mn(1e+30f, tau / (b3 + weps));
+  float m = mx(r1, mx(r2, r3));
+  float one = 1.0f;
+  if (m > 1e+15f) {
+    float im = 1.0f / m;
+    r1 = r1 * im; r2 = r2 * im; r3 = r3 * im;
+    one = im * im;
+  }
+  float w1 = 0.1f * fma_(r1, r1, one);
+  float w2 = 0.6f * fma_(r2, r2, one);
+  float w3 = 0.3f * fma_(r3, r3, one);
+  float p1 = fma_(11.0f / 6.0f, s3, fma_(2.0f / 6.0f, s1, (-7.0f / 6.0f) * s2));
+  float p2 = fma_(2.0f / 6.0f, s4, fma_(-1.0f / 6.0f, s2, (5.0f / 6.0f) * s3));
+  float p3 = fma_(-1.0f / 6.0f, s5, fma_(2.0f / 6.0f, s3, (5.0f / 6.0f) * s4));
+  return fma_(w3, p3, fma_(w1, p1, w2 * p2)) / ((w1 + w2) + w3);
+}
+TE_DEV bool weno_pdef_ok(float s1, float s2, float s3, float s4, float s5, float weps, bool pdef) {
+  if (!pdef) return true;
+  float bmax = mx(mx(mx(ab(s1), ab(s2)), mx(ab(s3), ab(s4))), ab(s5));
+  return bmax > weps;
+}
+// flux-limited 2nd-order upstream value (adv_flux_z_weno.wgsl:111-120)
+TE_DEV float upstrpd(float s1, float s2, float s3, float weps) {
+  float dd = s2 - s1;
+  float rr = (s3 - s2) / (copysign_mag(weps, dd) + dd);
+  float phi = mx(0.0f, mn(rr, mn(fma_(2.0f / 6.0f, rr, 1.0f / 6.0f), 1.0f)));
+  return fma_(phi, s2 - s1, s2);
+}
+
+// ---- double-float log/exp/pow with a final rounding fix
+// (pressure_tendency.wgsl:114-227: the reference's own algorithm, kept so the
+// thermodynamic powers match its recording bit for bit).
+struct FF { float x, y; };
+TE_DEV FF ff_two_sum(float a, float b) {
+  float s = a + b, bb = s - a;
+  return {s, (a - (s - bb)) + (b - bb)};
+}
+TE_DEV FF ff_quick_two_sum(float a, float b) {
+  float s = a + b;
+  return {s, b - (s - a)};
+}
+TE_DEV FF ff_mul(FF a, FF b) {
+  float p = a.x * b.x;
+  float e = fma_(a.x, b.x, -p);
+  e = fma_(a.x, b.y, e);
+  e = fma_(a.y, b.x, e);
+  return ff_quick_two_sum(p, e);
+}
+TE_DEV FF ff_add(FF a, FF b) {
+  FF s = ff_two_sum(a.x, b.x), t = ff_two_sum(a.y, b.y);
+  FF u = ff_quick_two_sum(s.x, s.y + t.x);
+  return ff_quick_two_sum(u.x, u.y + t.y);
+}
+TE_DEV FF ff_log(float x) {
+  int e;
+  float m = frexpf(x, &e);
+  if (m < 0.70710677f) { m = m * 2.0f; e = e - 1; }
+  float f = m - 1.0f;
+  FF d = ff_two_sum(2.0f, f);
+  float sh = f / d.x;
+  float r = fma_(-sh, d.x, f);
+  r = fma_(-sh, d.y, r);
+  FF sf = ff_quick_two_sum(sh, r / d.x);
+  FF s2 = ff_mul(sf, sf), s3 = ff_mul(s2, sf);
+  FF P = {0.11764706f, -4.3826945e-10f};
+  P = ff_add(ff_mul(P, s2), FF{0.13333334f, -6.9538753e-09f});
+  P = ff_add(ff_mul(P, s2), FF{0.15384616f, -5.731216e-09f});
+  P = ff_add(ff_mul(P, s2), FF{0.18181819f, -5.4186042e-09f});
+  P = ff_add(ff_mul(P, s2), FF{0.22222222f, -1.6556846e-09f});
+  P = ff_add(ff_mul(P, s2), FF{0.2857143f, -1.2772424e-08f});
+  P = ff_add(ff_mul(P, s2), FF{0.4f, -5.9604646e-09f});
+  P = ff_add(ff_mul(P, s2), FF{0.6666667f, -1.9868216e-08f});
+  FF l1p = ff_add(FF{2.0f * sf.x, 2.0f * sf.y}, ff_mul(s3, P));
+  const float ln2h = 0.69314575f, ln2m = 1.4285906e-06f, ln2l = 1.6198487e-11f;
+  float fe = (float)e;
+  FF a = ff_add(FF{fe * ln2h, fe * ln2m}, l1p);
+  return ff_add(a, FF{fe * ln2l, 0.0f});
+}
+TE_DEV float ff_exp(FF y) {
+  const float ln2h = 0.69314575f, ln2m = 1.4285906e-06f, ln2l = 1.6198487e-11f;
+  float kq = rintf(y.x / 0.6931472f);
+  FF r = ff_add(y, FF{-kq * ln2h, -kq * ln2m});
+  r = ff_add(r, FF{-kq * ln2l, 0.0f});
+  FF P = {2.5052108e-08f, 4.417623e-16f};
+  P = ff_add(ff_mul(P, r), FF{2.755732e-07f, -7.575112e-15f});
+  P = ff_add(ff_mul(P, r), FF{2.7557319e-06f, 3.7935712e-14f});
+  P = ff_add(ff_mul(P, r), FF{2.4801588e-05f, -3.406996e-13f});
+  P = ff_add(ff_mul(P, r), FF{0.0001984127f, -2.7255969e-12f});
+  P = ff_add(ff_mul(P, r), FF{0.0013888889f, -3.3631094e-11f});
+  P = ff_add(ff_mul(P, r), FF{0.008333334f, -4.346172e-10f});
+  P = ff_add(ff_mul(P, r), FF{0.041666668f, -1.2417635e-09f});
+  P = ff_add(ff_mul(P, r), FF{0.16666667f, -4.967054e-09f});
+  P = ff_add(ff_mul(P, r), FF{0.5f, 0.0f});
+  P = ff_add(ff_mul(P, r), FF{1.0f, 0.0f});
+  FF Q = ff_mul(P, r);
+  float s = 1.0f + Q.x;
+  float t = (1.0f - s) + Q.x;
+  FF ur = ff_two_sum(t, Q.y);
+  float res = s + ur.x;
+  float bb = res - s;
+  float e3 = (s - (res - bb)) + (ur.x - bb);
+  if (e3 != 0.0f && ur.y != 0.0f && ((e3 > 0.0f) == (ur.y > 0.0f))) {
+    float nb = __uint_as_float(__float_as_uint(res) + ((e3 > 0.0f) ? 1u : 0xFFFFFFFFu));
+    if (e3 + e3 == nb - res) res = nb;
+  }
+  return ldexpf(res, (int)kq);
+}
+TE_DEV float rr_pow(float x, float p) {
+  if (!(x > 0.0f) || (__float_as_uint(x) & 0x7fffffffu) == 0x7f800000u) return powf(x, p);
+  FF L = ff_log(x);
+  float yh = L.x * p;
+  return ff_exp(ff_quick_two_sum(yh, fma_(L.y, p, fma_(L.x, p, -yh))));
+}
+TE_DEV float rr_log(float x) {
+  if (!(x > 0.0f) || (__float_as_uint(x) & 0x7fffffffu) == 0x7f800000u) return logf(x);
+  FF a = ff_log(x);
+  return a.x + a.y;
+}
+TE_DEV float rr_exp(float x) {
+  if (!(ab(x) < 87.0f)) return expf(x);
+  return ff_exp(FF{x, 0.0f});
+}
+
+// global thread ids of a 3-D launch
+#define TE_TID3                                                          \
+  const int gx = (int)(blockIdx.x * blockDim.x + threadIdx.x);           \
+  const int gy = (int)(blockIdx.y * blockDim.y + threadIdx.y);           \
+  const int gz = (int)(blockIdx.z * blockDim.z + threadIdx.z);           \
+  if (gx >= (int)a.n[0] || gy >= (int)a.n[1] || gz >= (int)a.n[2]) return;
+
+}  // namespace te

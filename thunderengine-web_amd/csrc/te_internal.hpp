@@ -1,0 +1,71 @@
+// Host-side state of a context (program, device memory, stream, encoder
+// layout).  Internal to the library.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/te.h"
+
+namespace te {
+
+// One program entry: kernel id (index into the registry) and its 64-word
+// argument slot (A[0..2] thread grid, A[4+n] parameter n).
+struct Op {
+  int32_t kid;
+  uint32_t A[64];
+};
+
+int kernel_id(const char* name);
+const char* kernel_name(int kid);
+
+// Word layout of a program built by the encoder: named regions of M.
+struct Layout {
+  uint64_t words = 0;                       // total words of M
+  uint64_t arena = 0, field_words = 0;      // arena base and words per field slot
+  std::vector<std::string> fields;          // arena field names, sorted
+  std::map<std::string, uint64_t> region;   // profile/parameter regions by name
+  int nx = 0, ny = 0, nz = 0, sJ = 0, sK = 0;
+  uint64_t field(const std::string& n) const;
+};
+
+// The encoder's product: the layout, the initial memory image (profiles,
+// parameter bytes, metric buffer; the arena part is zero), and the op list.
+struct Encoded {
+  Layout L;
+  std::vector<uint32_t> image;
+  std::vector<Op> ops;
+  std::vector<std::string> names;  // kernel name of each op (also for kernels outside this path)
+};
+
+Encoded encode_step(const te_config& cfg);
+// Fills the arena part of `image` with the t=0 state of the scenario.
+void fill_initial_state(const te_config& cfg, const Layout& L, std::vector<uint32_t>& image);
+
+}  // namespace te
+
+struct te_ctx {
+  uint32_t* dBase = nullptr;  // allocation (front guard + M + back guard)
+  uint32_t* dM = nullptr;     // word 0 of the program's memory M
+  uint64_t words = 0;       // allocated words (program memory + guard region)
+  uint64_t prog_words = 0;  // words of the program's memory M
+  bool guard_ok = false;
+  uint32_t* dHash = nullptr;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  float last_ms = 0.0f;
+  std::vector<te::Op> ops;
+  std::vector<std::string> op_names;
+  hipGraphExec_t graph = nullptr;
+  bool graph_failed = false;
+  std::string err;
+  // config contexts
+  bool has_config = false;
+  te_config cfg{};
+  std::vector<float> zf;  // owned copy of cfg.zf
+  te::Layout layout;
+  std::vector<uint32_t> image;
+};

@@ -1,0 +1,26 @@
+// Registry of the step kernels: every kernel has the signature
+// (uint32_t* M, KArg a) and is addressed by the reference kernel name.
+#pragma once
+#include "te_device.hpp"
+
+#define TE_KERNEL_LIST(X)                                                                                      \
+  X(update_diagnostics_moist) X(update_diagnostics) X(stage_tendency_reset) X(stage_state_reset)              \
+  X(massflux_from_state) X(velocity_to_massflux) X(div_massflux) X(divergence3d) X(buoyancy_apply) X(rk_update) \
+  X(build_sadv) X(apply_thterm) X(turb_add_tendency) X(turb_zero) X(tke_advance) X(ppx_finish)                 \
+  X(diabatic_unapply) X(diabatic_rates) X(pressure_tendency) X(update_rho_prs) X(adv_flux_x) X(adv_flux_y)     \
+  X(adv_flux_z) X(adv_flux_x_weno) X(adv_flux_y_weno) X(adv_flux_z_weno) X(adv_flux_x_stag) X(adv_flux_y_stag) \
+  X(adv_flux_z_stag) X(adv_flux_x_weno_w) X(adv_flux_y_weno_w) X(adv_flux_z_weno_w) X(adv_flux_x_weno_uv)      \
+  X(adv_flux_y_weno_uv) X(adv_flux_z_weno_uv) X(adv_assemble) X(acoustic_prepare) X(acoustic_begin)            \
+  X(acoustic_uv) X(acoustic_fpk) X(acoustic_w_solve) X(acoustic_pp) X(acoustic_accumulate) X(pdef_dum)         \
+  X(pdef_apply) X(bc_periodic_x) X(bc_periodic_y) X(bc_open_x) X(bc_open_y) X(openbc_uv_tendency) X(radbc_ew)   \
+  X(radbc_ns) X(turb_openbc_flux) X(turb_openbc_stress_x) X(turb_openbc_stress_y) X(rayleigh_damping)          \
+  X(turb_rho_faces) X(turb_def_centers) X(turb_def_faces) X(turb_def_assemble) X(turb_nm) X(turb_km_kh)        \
+  X(turb_dissipative_heating) X(turb_gettau) X(turb_sfcstress) X(tke_epst) X(tke_sources) X(turb_u_tendency)   \
+  X(turb_v_tendency) X(turb_w_tendency) X(turb_scalar_flux_h) X(turb_scalar_vert_implicit) X(tke_flux_h)       \
+  X(tke_transport) X(_copy)
+
+namespace te {
+#define TE_DECL(n) __global__ void k_##n(uint32_t* M, KArg a);
+TE_KERNEL_LIST(TE_DECL)
+#undef TE_DECL
+}  // namespace te

@@ -1,0 +1,453 @@
+// Program executor and the C ABI (include/te.h).
+//
+// The executor owns one flat device memory M (every buffer of a program,
+// word addressed, one hipMalloc) and launches the program's kernels in order on
+// one HIP stream.  A full step is captured once into a HIP graph and replayed.
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+
+#include "te_internal.hpp"
+#include "te_kernels.hpp"
+
+using te::KArg;
+using te::Op;
+
+namespace te {
+
+struct KernelInfo {
+  const char* name;
+  const void* fn;
+};
+#define TE_ROW(n) {#n, (const void*)&k_##n},
+static const KernelInfo kKernels[] = {TE_KERNEL_LIST(TE_ROW)};
+#undef TE_ROW
+static const int kNumKernels = (int)(sizeof(kKernels) / sizeof(kKernels[0]));
+
+int kernel_id(const char* name) {
+  for (int i = 0; i < kNumKernels; ++i)
+    if (std::strcmp(kKernels[i].name, name) == 0) return i;
+  return -1;
+}
+const char* kernel_name(int kid) { return (kid >= 0 && kid < kNumKernels) ? kKernels[kid].name : "?"; }
+
+uint64_t Layout::field(const std::string& n) const {
+  for (size_t i = 0; i < fields.size(); ++i)
+    if (fields[i] == n) return arena + i * field_words;
+  return UINT64_MAX;
+}
+
+// Block shape for a thread grid: 64 lanes along i (one wavefront, coalesced
+// i-fastest accesses), then j, then k, up to 256 threads.
+static dim3 block_for(const uint32_t* n) {
+  auto p2 = [](uint32_t v) { uint32_t r = 1; while (r < v) r <<= 1; return r; };
+  uint32_t bx = std::min<uint32_t>(64, p2(std::max<uint32_t>(n[0], 1)));
+  uint32_t by = std::min<uint32_t>(256 / bx, p2(std::max<uint32_t>(n[1], 1)));
+  uint32_t bz = std::min<uint32_t>(256 / (bx * by), p2(std::max<uint32_t>(n[2], 1)));
+  return dim3(bx, by, bz);
+}
+
+static KArg karg_of(const Op& op) {
+  KArg a;
+  a.n[0] = op.A[0]; a.n[1] = op.A[1]; a.n[2] = op.A[2];
+  for (int q = 0; q < 32; ++q) a.p[q] = op.A[4 + q];
+  return a;
+}
+
+static hipError_t launch(te_ctx* c, const Op& op) {
+  KArg a = karg_of(op);
+  if (a.n[0] == 0 || a.n[1] == 0 || a.n[2] == 0) return hipSuccess;
+  dim3 b = block_for(a.n);
+  dim3 g((a.n[0] + b.x - 1) / b.x, (a.n[1] + b.y - 1) / b.y, (a.n[2] + b.z - 1) / b.z);
+  void* args[] = {(void*)&c->dM, (void*)&a};
+  return hipLaunchKernel(kKernels[op.kid].fn, g, b, args, 0, c->stream);
+}
+
+// region_hash (replay.js:408-418): per-lane sums of a 32-bit mix, reduced
+// with one atomic add per block.
+__global__ void k_region_hash(const uint32_t* M, uint64_t off, uint64_t n, uint32_t* out) {
+  uint32_t s = 0;
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t h = (M[off + i] ^ ((uint32_t)i * 0x9E3779B1u)) * 0x85EBCA6Bu;
+    h ^= h >> 13;
+    h *= 0xC2B2AE35u;
+    h ^= h >> 16;
+    s += h;
+  }
+  for (int d = 32; d > 0; d >>= 1) s += __shfl_down(s, d, 64);
+  __shared__ uint32_t part[4];
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) atomicAdd(out, part[0] + part[1] + part[2] + part[3]);
+}
+__global__ void k_zero_word(uint32_t* p) { *p = 0u; }
+
+}  // namespace te
+
+// ------------------------------------------------------------------ helpers
+static int fail(te_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+static int hip_fail(te_ctx* c, hipError_t e, const char* what) {
+  return fail(c, TE_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+#define TE_HIP(call, what)                       \
+  do {                                           \
+    hipError_t e_ = (call);                      \
+    if (e_ != hipSuccess) return hip_fail(c, e_, what); \
+  } while (0)
+
+static int validate_ops(te_ctx* c) {
+  for (size_t i = 0; i < c->ops.size(); ++i) {
+    const Op& op = c->ops[i];
+    const char* nm = te::kernel_name(op.kid);
+    if (std::strcmp(nm, "_copy") == 0) {
+      if ((uint64_t)op.A[4] + op.A[0] > c->words || (uint64_t)op.A[5] + op.A[0] > c->words)
+        return fail(c, TE_ERR_ARG, "copy op " + std::to_string(i) + " leaves the memory");
+      continue;
+    }
+    for (int q = 0; q < 32; ++q)
+      if (op.A[4 + q] >= c->words) return fail(c, TE_ERR_ARG, "op " + std::to_string(i) + " (" + nm + ") parameter offset outside memory");
+  }
+  return TE_OK;
+}
+
+// Words of zeroed memory kept in front of M: no kernel indexes below a
+// parameter's offset by more than a few rows, so this keeps every access of a
+// validated program inside the allocation.
+static const uint64_t kFrontGuard = 1u << 20;
+
+static int alloc_memory(te_ctx* c, uint64_t words) {
+  uint32_t* base = nullptr;
+  TE_HIP(hipMalloc((void**)&base, (kFrontGuard + words) * 4), "hipMalloc(M)");
+  TE_HIP(hipMemsetAsync(base, 0, (kFrontGuard + words) * 4, c->stream), "hipMemset(M)");
+  if (c->dBase) {
+    TE_HIP(hipMemcpyAsync(base + kFrontGuard, c->dM, c->prog_words * 4, hipMemcpyDeviceToDevice, c->stream), "copy M");
+    TE_HIP(hipStreamSynchronize(c->stream), "copy M sync");
+    (void)hipFree(c->dBase);
+  }
+  c->dBase = base;
+  c->dM = base + kFrontGuard;
+  c->words = words;
+  return TE_OK;
+}
+
+static int ctx_alloc(te_ctx* c, uint64_t words) {
+  c->prog_words = words;
+  TE_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
+  TE_HIP(hipEventCreate(&c->ev0), "hipEventCreate");
+  TE_HIP(hipEventCreate(&c->ev1), "hipEventCreate");
+  int rc = alloc_memory(c, words);
+  if (rc != TE_OK) return rc;
+  TE_HIP(hipMalloc((void**)&c->dHash, 256), "hipMalloc(hash)");
+  return TE_OK;
+}
+
+// Guard region: every kernel indexes at most one field footprint
+// (strideJ*strideK*(nz+1) words, plus one plane of stencil reach) past a
+// parameter's offset, and the offsets were checked to lie inside the program's
+// memory.  Before the first run the grid of every op is read back and the
+// allocation grown so that footprint past the end of the program's memory is
+// still allocated: no launch of a validated program can leave M.
+static int ensure_guard(te_ctx* c) {
+  if (c->guard_ok) return TE_OK;
+  uint64_t need = 0;
+  for (const Op& op : c->ops) {
+    if (std::strcmp(te::kernel_name(op.kid), "_copy") == 0) continue;
+    uint32_t d[5];
+    TE_HIP(hipMemcpy(d, c->dM + op.A[4], sizeof(d), hipMemcpyDeviceToHost), "read grid");
+    const int32_t nz = (int32_t)d[2], sJ = (int32_t)d[3], sK = (int32_t)d[4];
+    if (nz < 1 || sJ < 1 || sK < 1 || nz > 100000 || sJ > 100000 || sK > 100000)
+      return fail(c, TE_ERR_ARG, std::string("op ") + te::kernel_name(op.kid) + ": invalid grid parameters");
+    need = std::max<uint64_t>(need, (uint64_t)sJ * sK * (uint64_t)(nz + 3));
+    if (std::strcmp(te::kernel_name(op.kid), "turb_sfcstress") == 0) {
+      int32_t mode = 0;
+      TE_HIP(hipMemcpy(&mode, c->dM + op.A[4 + 5], 4, hipMemcpyDeviceToHost), "read mode");
+      if (mode != 0 && mode != 4)
+        return fail(c, TE_ERR_UNSUPPORTED, "turb_sfcstress: surface-flux stress modes (bbc=3) are not on this path");
+    }
+  }
+  const uint64_t want = c->prog_words + need;
+  if (want > c->words) {
+    int rc = alloc_memory(c, want);
+    if (rc != TE_OK) return rc;
+  }
+  c->guard_ok = true;
+  return TE_OK;
+}
+
+static int run_ops(te_ctx* c, int first, int last) {
+  for (int i = first; i <= last; ++i) {
+    hipError_t e = te::launch(c, c->ops[i]);
+    if (e != hipSuccess) return hip_fail(c, e, (std::string("launch ") + te::kernel_name(c->ops[i].kid)).c_str());
+  }
+  return TE_OK;
+}
+
+static int ensure_graph(te_ctx* c) {
+  if (c->graph || c->graph_failed) return TE_OK;
+  hipGraph_t g = nullptr;
+  if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+    c->graph_failed = true;
+    return TE_OK;
+  }
+  int rc = run_ops(c, 0, (int)c->ops.size() - 1);
+  hipError_t e = hipStreamEndCapture(c->stream, &g);
+  if (rc != TE_OK || e != hipSuccess || !g) {
+    if (g) (void)hipGraphDestroy(g);
+    c->graph_failed = true;
+    (void)hipGetLastError();
+    return TE_OK;
+  }
+  e = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) {
+    c->graph = nullptr;
+    c->graph_failed = true;
+    (void)hipGetLastError();
+  }
+  return TE_OK;
+}
+
+extern "C" {
+
+int te_program_create(uint64_t mem_words, int32_t nops, const char* const* names, const uint32_t* args, te_ctx** out) {
+  if (!out || (nops > 0 && (!names || !args)) || mem_words == 0) return TE_ERR_ARG;
+  *out = nullptr;
+  te_ctx* c = new te_ctx();
+  c->ops.resize(nops);
+  for (int i = 0; i < nops; ++i) {
+    int kid = te::kernel_id(names[i]);
+    if (kid < 0) {
+      int rc = fail(c, TE_ERR_UNSUPPORTED, std::string("no kernel '") + names[i] + "' on this path");
+      *out = c;
+      return rc;
+    }
+    c->ops[i].kid = kid;
+    c->op_names.push_back(names[i]);
+    std::memcpy(c->ops[i].A, args + (size_t)i * 64, 64 * 4);
+    if (std::strcmp(names[i], "_copy") == 0) {  // copy slot: A[0]=n, A[4]=src, A[5]=dst
+      c->ops[i].A[1] = c->ops[i].A[2] = 1;
+    }
+  }
+  *out = c;
+  c->words = c->prog_words = mem_words;
+  int rc = validate_ops(c);
+  if (rc != TE_OK) return rc;
+  return ctx_alloc(c, mem_words);
+}
+
+int te_upload_words(te_ctx* c, uint64_t off, const uint32_t* src, uint64_t n) {
+  if (!c || !src || off + n > c->prog_words) return fail(c, TE_ERR_ARG, "upload outside memory");
+  TE_HIP(hipMemcpyAsync(c->dM + off, src, n * 4, hipMemcpyHostToDevice, c->stream), "upload");
+  TE_HIP(hipStreamSynchronize(c->stream), "upload sync");
+  return TE_OK;
+}
+
+int te_download_words(te_ctx* c, uint64_t off, uint32_t* dst, uint64_t n) {
+  if (!c || !dst || off + n > c->prog_words) return fail(c, TE_ERR_ARG, "download outside memory");
+  if (!c->dM) {  // encoded program without device memory: read the host image
+    if (off + n > c->image.size()) return fail(c, TE_ERR_STATE, "no memory image");
+    std::memcpy(dst, c->image.data() + off, n * 4);
+    return TE_OK;
+  }
+  TE_HIP(hipMemcpyAsync(dst, c->dM + off, n * 4, hipMemcpyDeviceToHost, c->stream), "download");
+  TE_HIP(hipStreamSynchronize(c->stream), "download sync");
+  return TE_OK;
+}
+
+int te_run(te_ctx* c, int32_t nsteps, int32_t first, int32_t last) {
+  if (!c || !c->dM) return fail(c, TE_ERR_STATE, "context has no memory");
+  const int n = (int)c->ops.size();
+  if (last < 0 || last >= n) last = n - 1;
+  if (first < 0 || first > last || nsteps < 0) return fail(c, TE_ERR_ARG, "bad op range");
+  int rc0 = ensure_guard(c);
+  if (rc0 != TE_OK) return rc0;
+  const bool full = first == 0 && last == n - 1;
+  if (full && nsteps > 1) ensure_graph(c);
+  TE_HIP(hipEventRecord(c->ev0, c->stream), "event");
+  for (int s = 0; s < nsteps; ++s) {
+    if (full && c->graph) {
+      TE_HIP(hipGraphLaunch(c->graph, c->stream), "hipGraphLaunch");
+    } else {
+      int rc = run_ops(c, first, last);
+      if (rc != TE_OK) return rc;
+    }
+  }
+  TE_HIP(hipEventRecord(c->ev1, c->stream), "event");
+  return TE_OK;
+}
+
+int te_profile_ops(te_ctx* c, int32_t nsteps, float* op_ms) {
+  if (!c || !c->dM || !op_ms || nsteps < 1) return fail(c, TE_ERR_ARG, "te_profile_ops: bad arguments");
+  const int n = (int)c->ops.size();
+  for (int i = 0; i < n; ++i)
+    if (c->ops[i].kid < 0) return fail(c, TE_ERR_UNSUPPORTED, "kernel '" + c->op_names[i] + "' is not on this path");
+  int rc0 = ensure_guard(c);
+  if (rc0 != TE_OK) return rc0;
+  std::vector<hipEvent_t> ev(n + 1, nullptr);
+  int rc = TE_OK;
+  for (int i = 0; i <= n && rc == TE_OK; ++i)
+    if (hipEventCreate(&ev[i]) != hipSuccess) rc = fail(c, TE_ERR_HIP, "hipEventCreate");
+  for (int i = 0; i < n; ++i) op_ms[i] = 0.0f;
+  for (int s = 0; s < nsteps && rc == TE_OK; ++s) {
+    if (hipEventRecord(ev[0], c->stream) != hipSuccess) { rc = fail(c, TE_ERR_HIP, "event"); break; }
+    for (int i = 0; i < n && rc == TE_OK; ++i) {
+      hipError_t e = te::launch(c, c->ops[i]);
+      if (e == hipSuccess) e = hipEventRecord(ev[i + 1], c->stream);
+      if (e != hipSuccess) rc = hip_fail(c, e, "profiled launch");
+    }
+    if (rc == TE_OK && hipStreamSynchronize(c->stream) != hipSuccess) rc = fail(c, TE_ERR_HIP, "sync");
+    for (int i = 0; i < n && rc == TE_OK; ++i) {
+      float ms = 0.0f;
+      if (hipEventElapsedTime(&ms, ev[i], ev[i + 1]) != hipSuccess) rc = fail(c, TE_ERR_HIP, "elapsed");
+      op_ms[i] += ms;
+    }
+  }
+  for (hipEvent_t e : ev)
+    if (e) (void)hipEventDestroy(e);
+  return rc;
+}
+
+int te_sync(te_ctx* c) {
+  if (!c) return TE_ERR_ARG;
+  TE_HIP(hipStreamSynchronize(c->stream), "sync");
+  TE_HIP(hipEventElapsedTime(&c->last_ms, c->ev0, c->ev1), "elapsed");
+  return TE_OK;
+}
+
+int te_last_run_ms(te_ctx* c, float* ms) {
+  if (!c || !ms) return TE_ERR_ARG;
+  TE_HIP(hipEventSynchronize(c->ev1), "event sync");
+  TE_HIP(hipEventElapsedTime(&c->last_ms, c->ev0, c->ev1), "elapsed");
+  *ms = c->last_ms;
+  return TE_OK;
+}
+
+int te_region_hash(te_ctx* c, uint64_t off, uint64_t n, uint32_t* out) {
+  if (!c || !out || off + n > c->words) return fail(c, TE_ERR_ARG, "hash outside memory");
+  hipLaunchKernelGGL(te::k_zero_word, dim3(1), dim3(1), 0, c->stream, c->dHash);
+  hipLaunchKernelGGL(te::k_region_hash, dim3(256), dim3(256), 0, c->stream, (const uint32_t*)c->dM, off, n, c->dHash);
+  TE_HIP(hipGetLastError(), "hash launch");
+  TE_HIP(hipMemcpyAsync(out, c->dHash, 4, hipMemcpyDeviceToHost, c->stream), "hash copy");
+  TE_HIP(hipStreamSynchronize(c->stream), "hash sync");
+  return TE_OK;
+}
+
+int te_fnv1a(te_ctx* c, uint64_t off, uint64_t n, uint32_t* out) {
+  if (!c || !out || off + n > c->words) return fail(c, TE_ERR_ARG, "fnv outside memory");
+  std::vector<uint32_t> h(n);
+  int rc = te_download_words(c, off, h.data(), n);
+  if (rc != TE_OK) return rc;
+  uint32_t x = 0x811c9dc5u;
+  for (uint64_t i = 0; i < n; ++i) x = (x ^ h[i]) * 0x01000193u;
+  *out = x;
+  return TE_OK;
+}
+
+uint64_t te_memory_words(const te_ctx* c) { return c ? c->prog_words : 0; }
+int32_t te_op_count(const te_ctx* c) { return c ? (int32_t)c->ops.size() : 0; }
+int te_op_get(const te_ctx* c, int32_t i, const char** name, uint32_t* args64) {
+  if (!c || i < 0 || i >= (int32_t)c->ops.size()) return TE_ERR_ARG;
+  if (name) *name = i < (int32_t)c->op_names.size() ? c->op_names[i].c_str() : te::kernel_name(c->ops[i].kid);
+  if (args64) std::memcpy(args64, c->ops[i].A, 64 * 4);
+  return TE_OK;
+}
+
+int32_t te_kernel_count(void) { return te::kNumKernels; }
+const char* te_kernel_name(int32_t i) { return (i >= 0 && i < te::kNumKernels) ? te::kKernels[i].name : nullptr; }
+
+const char* te_last_error(const te_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+void te_destroy(te_ctx* c) {
+  if (!c) return;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->graph) (void)hipGraphExecDestroy(c->graph);
+  if (c->dBase) (void)hipFree(c->dBase);
+  if (c->dHash) (void)hipFree(c->dHash);
+  if (c->ev0) (void)hipEventDestroy(c->ev0);
+  if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+// ------------------------------------------------------------------ config
+int te_encode(const te_config* cfg, te_ctx** out) {
+  if (!cfg || !out) return TE_ERR_ARG;
+  *out = nullptr;
+  te_ctx* c = new te_ctx();
+  *out = c;
+  c->has_config = true;
+  c->cfg = *cfg;
+  if (cfg->zf) {
+    c->zf.assign(cfg->zf, cfg->zf + cfg->nz + 1);
+    c->cfg.zf = c->zf.data();
+  }
+  c->cfg.arena_fields = nullptr;  // the caller's list is consumed by the encoder only
+  try {
+    te::Encoded e = te::encode_step(*cfg);
+    c->layout = std::move(e.L);
+    c->image = std::move(e.image);
+    c->ops = std::move(e.ops);
+    c->op_names = std::move(e.names);
+  } catch (const std::exception& ex) {
+    return fail(c, TE_ERR_UNSUPPORTED, ex.what());
+  }
+  c->words = c->prog_words = c->layout.words;
+  return TE_OK;
+}
+
+int te_create(const te_config* cfg, te_ctx** out) {
+  int rc = te_encode(cfg, out);
+  if (rc != TE_OK) return rc;
+  te_ctx* c = *out;
+  for (size_t i = 0; i < c->ops.size(); ++i)
+    if (c->ops[i].kid < 0)
+      return fail(c, TE_ERR_UNSUPPORTED, "kernel '" + c->op_names[i] + "' is not on this path (microphysics / mass conservation)");
+  rc = validate_ops(c);
+  if (rc != TE_OK) return rc;
+  rc = ctx_alloc(c, c->layout.words);
+  if (rc != TE_OK) return rc;
+  return te_upload_words(c, 0, c->image.data(), c->image.size());
+}
+
+int te_init_state(te_ctx* c) {
+  if (!c || !c->has_config) return fail(c, TE_ERR_STATE, "not a config context");
+  try {
+    te::fill_initial_state(c->cfg, c->layout, c->image);
+  } catch (const std::exception& ex) {
+    return fail(c, TE_ERR_ARG, ex.what());
+  }
+  if (!c->dM) return TE_OK;  // encoded-only context: the host image holds the state
+  return te_upload_words(c, 0, c->image.data(), c->image.size());
+}
+
+int te_step(te_ctx* c, int32_t n) { return te_run(c, n, 0, -1); }
+
+int te_field_offset(const te_ctx* c, const char* name, uint64_t* off) {
+  if (!c || !name || !off || !c->has_config) return TE_ERR_ARG;
+  uint64_t o = c->layout.field(name);
+  if (o == UINT64_MAX) return TE_ERR_ARG;
+  *off = o;
+  return TE_OK;
+}
+int te_region_offset(const te_ctx* c, const char* name, uint64_t* off) {
+  if (!c || !name || !off || !c->has_config) return TE_ERR_ARG;
+  auto it = c->layout.region.find(name);
+  if (it == c->layout.region.end()) return TE_ERR_ARG;
+  *off = it->second;
+  return TE_OK;
+}
+uint64_t te_field_stride_words(const te_ctx* c) { return c ? c->layout.field_words : 0; }
+int32_t te_field_count(const te_ctx* c) { return c ? (int32_t)c->layout.fields.size() : 0; }
+const char* te_field_name(const te_ctx* c, int32_t i) {
+  return (c && i >= 0 && i < (int32_t)c->layout.fields.size()) ? c->layout.fields[i].c_str() : nullptr;
+}
+int te_grid(const te_ctx* c, int32_t o[5]) {
+  if (!c || !o || !c->has_config) return TE_ERR_ARG;
+  o[0] = c->layout.nx; o[1] = c->layout.ny; o[2] = c->layout.nz; o[3] = c->layout.sJ; o[4] = c->layout.sK;
+  return TE_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,303 @@
+// Grid, base state, step profiles and the t=0 arena of a configuration.
+//
+// Vertical grid and step profiles follow the reference's binary32 recipes
+// (web/core/grid.js:218-293, web/core/scenario.js:122-136) because their values
+// become kernel inputs.  The sounding (web/core/base.js:181-216 Weisman-Klemp,
+// :368-442 winds, :475-489 bubble) is this library's own construction of the
+// same analytic profiles: an initial state, not part of the per-step parity.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+
+#include "te_scenario.hpp"
+
+namespace te {
+
+static inline float f32(double x) { return (float)x; }
+
+// Fornberg weights for 3-point extrapolation to x0 (grid.js:219-235).
+static void fornberg(float x0f, const float af[3], float out[3]) {
+  double x0 = x0f, alpha[3] = {af[0], af[1], af[2]};
+  double d[3][3] = {{1, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+  double b1 = 1.0;
+  for (int n = 1; n <= 2; ++n) {
+    double b2 = 1.0;
+    for (int nu = 0; nu < n; ++nu) {
+      double b3 = alpha[n] - alpha[nu];
+      b2 = b2 * b3;
+      d[n][nu] = ((alpha[n] - x0) * d[n - 1][nu]) / b3;
+    }
+    d[n][n] = (b1 / b2) * (-((alpha[n - 1] - x0) * d[n - 1][n - 1]));
+    b1 = b2;
+  }
+  for (int q = 0; q < 3; ++q) out[q] = f32(d[2][q]);
+}
+static void lagrange(const double* x, int n, double x0, float out[3]) {
+  double w[3] = {0, 0, 0};
+  for (int i = 0; i < n; ++i) {
+    double p = 1.0;
+    for (int j = 0; j < n; ++j)
+      if (j != i) p *= (x0 - x[j]) / (x[i] - x[j]);
+    w[i] = p;
+  }
+  for (int q = 0; q < 3; ++q) out[q] = f32(w[q]);
+}
+
+Vertical make_vertical(const std::vector<double>& F, double nominal) {
+  const int n = (int)F.size() - 1;
+  if (n < 2) throw std::runtime_error("vertical grid needs at least two cells");
+  for (int k = 0; k < n; ++k)
+    if (!(F[k] < F[k + 1])) throw std::runtime_error("face heights must increase strictly");
+  Vertical v;
+  v.nz = n;
+  v.zf = F;
+  v.zh.resize(n);
+  for (int k = 0; k < n; ++k) v.zh[k] = f32(0.5 * (F[k] + F[k + 1]));
+  v.dz_nominal = nominal > 0 ? nominal : (F[n] - F[0]) / n;
+  const float* c = v.zh.data();
+  const float ghostLo = -c[0];
+  const float ghostHi = c[n - 1] + 2.0f * (f32(F[n]) - c[n - 1]);
+  const float dz32 = f32(v.dz_nominal);
+  v.mf.assign(n + 1, 0.0f);
+  v.mf[0] = dz32 / (c[0] - ghostLo);
+  for (int k = 1; k < n; ++k) v.mf[k] = dz32 / (c[k] - c[k - 1]);
+  v.mf[n] = dz32 / (ghostHi - c[n - 1]);
+  v.c2.assign(n + 1, 0.0f);
+  v.c2[0] = (f32(F[0]) - ghostLo) / (c[0] - ghostLo);
+  for (int k = 1; k < n; ++k) v.c2[k] = (f32(F[k]) - c[k - 1]) / (c[k] - c[k - 1]);
+  v.c2[n] = (f32(F[n]) - c[n - 1]) / (ghostHi - c[n - 1]);
+  v.c1.resize(n + 1);
+  for (int k = 0; k <= n; ++k) v.c1[k] = f32(1.0 - (double)v.c2[k]);
+  v.mh.resize(n);
+  for (int k = 0; k < n; ++k) v.mh[k] = dz32 / (f32(F[k + 1]) - f32(F[k]));
+  if (n >= 3) {
+    const float dBot = f32(F[1]) - f32(F[0]), dTop = f32(F[n]) - f32(F[n - 1]);
+    const float a[3] = {c[0] / dBot, c[1] / dBot, c[2] / dBot};
+    fornberg(f32(F[0]) / dBot, a, v.cgs);
+    const float b[3] = {c[n - 1] / dTop, c[n - 2] / dTop, c[n - 3] / dTop};
+    fornberg(f32(F[n]) / dTop, b, v.cgt);
+  } else {
+    double lo[3] = {v.zh[0], n > 1 ? v.zh[1] : 0.0, 0.0}, hi[3] = {v.zh[n - 1], n > 1 ? v.zh[n - 2] : 0.0, 0.0};
+    lagrange(lo, std::min(3, n), F[0], v.cgs);
+    lagrange(hi, std::min(3, n), F[n], v.cgt);
+  }
+  return v;
+}
+
+// ---- thermodynamic base state (constants of web/core/grid.js:13-20)
+static const double G = 9.81, RDd = 287.04, CPd = 1005.7, RVd = 461.5, P00d = 1.0e5;
+
+static float rslf(float p, float t, float eps) {
+  float esl = 611.2f * f32(std::exp((double)((17.67f * (t - 273.15f)) / (t - 29.65f))));
+  esl = std::min(esl, p * 0.5f);
+  return (eps * esl) / (p - esl);
+}
+
+Base make_base(const te_config& cfg, const Vertical& v) {
+  const int nk = v.nz;
+  const float rd = f32(RDd), cp = f32(CPd), rv = f32(RVd), p00 = f32(P00d), g = f32(G);
+  const float reps = rv / rd, eps = rd / rv, cpdrd = cp / rd;
+  Base b;
+  b.th0.assign(nk, 0); b.qv0.assign(nk, 0); b.qc0.assign(nk, 0); b.pi0.assign(nk, 0); b.prs0.assign(nk, 0);
+  b.rho0.assign(nk, 0); b.u0.assign(nk, 0); b.v0.assign(nk, 0);
+  std::vector<float> rh(nk, 0.0f), thv(nk, 0.0f);
+  // Weisman & Klemp (1982) analytic sounding
+  const float zTrop = cfg.z_trop, thTrop = cfg.th_trop, tTrop = cfg.t_trop, thSfc = cfg.th_sfc, prsSfc = 100000.0f;
+  const float piSfc = f32(std::pow((double)(prsSfc / p00), (double)(rd / cp)));
+  const float qvSfc = cfg.moist ? rslf(prsSfc, thSfc * piSfc, eps) : 0.0f;
+  const float thvSfc = (thSfc * std::fma(qvSfc, reps, 1.0f)) / (1.0f + qvSfc);
+  for (int k = 0; k < nk; ++k) {
+    const float z = v.zh[k];
+    if (z < zTrop) {
+      const float fr = f32(std::pow((double)(z / zTrop), 1.25));
+      b.th0[k] = std::fma(thTrop - thSfc, fr, thSfc);
+      rh[k] = cfg.moist ? std::fma(-0.75f, fr, 1.0f) : 0.0f;
+    } else {
+      b.th0[k] = thTrop * f32(std::exp((double)((g / (tTrop * cp)) * (z - zTrop))));
+      rh[k] = cfg.moist ? 0.25f : 0.0f;
+    }
+  }
+  for (int it = 0; it < 20; ++it) {
+    for (int k = 0; k < nk; ++k) thv[k] = (b.th0[k] * std::fma(reps, b.qv0[k], 1.0f)) / (1.0f + b.qv0[k]);
+    b.pi0[0] = piSfc - (g * v.zh[0]) / ((cp * 0.5f) * (thvSfc + thv[0]));
+    for (int k = 1; k < nk; ++k)
+      b.pi0[k] = b.pi0[k - 1] - (g * (v.zh[k] - v.zh[k - 1])) / ((cp * 0.5f) * (thv[k] + thv[k - 1]));
+    for (int k = 0; k < nk; ++k) b.prs0[k] = p00 * f32(std::pow((double)b.pi0[k], (double)cpdrd));
+    for (int k = 0; k < nk; ++k) {
+      b.qv0[k] = cfg.moist ? rh[k] * rslf(b.prs0[k], b.th0[k] * b.pi0[k], eps) : 0.0f;
+      if (b.qv0[k] > cfg.qv_pbl) b.qv0[k] = cfg.qv_pbl;
+    }
+  }
+  for (int k = 0; k < nk; ++k) {
+    b.prs0[k] = p00 * f32(std::pow((double)b.pi0[k], (double)cpdrd));
+    b.rho0[k] = b.prs0[k] / (((rd * b.th0[k]) * b.pi0[k]) * std::fma(b.qv0[k], reps, 1.0f));
+  }
+  // quarter-circle hodograph of radius R over 0..zq, then straight to umax2 at zs
+  const float R = cfg.hodo_radius, zq = cfg.hodo_depth, umax2 = cfg.shear_umax, zs = cfg.shear_depth;
+  const double deg = M_PI / 180.0;
+  for (int k = 0; k < nk; ++k) {
+    const float z = v.zh[k];
+    float u, w;
+    if (z <= zq) {
+      const double ang = 90.0 * (z / zq) * deg;
+      u = f32(R - R * std::cos(ang));
+      w = f32(R * std::sin(ang));
+    } else if (z <= zs) {
+      u = R + ((z - zq) * (umax2 - R)) / (zs - zq);
+      w = R;
+    } else {
+      u = umax2;
+      w = R;
+    }
+    b.u0[k] = u - cfg.umove;
+    b.v0[k] = w - cfg.vmove;
+  }
+  return b;
+}
+
+Profiles make_profiles(const te_config& cfg, const Vertical& v, const Base& b) {
+  const int nz = v.nz;
+  Profiles p;
+  p.rho0 = b.rho0;
+  p.rr0.resize(nz);
+  for (int k = 0; k < nz; ++k) p.rr0[k] = 1.0f / b.rho0[k];
+  // rf0 at faces by extrapolation/interpolation (scenario.js:122-136)
+  p.rf0.resize(nz + 1);
+  const float* r = b.rho0.data();
+  for (int k = 0; k <= nz; ++k) {
+    if (k == 0) p.rf0[k] = std::fma(v.cgs[2], r[2], std::fma(v.cgs[0], r[0], v.cgs[1] * r[1]));
+    else if (k == nz) p.rf0[k] = std::fma(v.cgt[2], r[nz - 3], std::fma(v.cgt[0], r[nz - 1], v.cgt[1] * r[nz - 2]));
+    else p.rf0[k] = std::fma(v.c1[k], r[k - 1], v.c2[k] * r[k]);
+  }
+  p.th0 = b.th0; p.pi0 = b.pi0; p.qv0 = b.qv0; p.qc0 = b.qc0; p.qi0.assign(nz, 0.0f);
+  p.rth0.resize(nz);
+  for (int k = 0; k < nz; ++k) p.rth0[k] = 1.0f / b.th0[k];
+  p.u0 = b.u0; p.v0 = b.v0;
+  // Rayleigh damping weights above zd (timestep.js:657-677)
+  p.tauh.assign(nz, 0.0f);
+  p.tauf.assign(nz + 1, 0.0f);
+  const float zd = cfg.rayleigh_zd, ztop = f32(v.zf[nz]), pi = f32(3.1415926535897932384626433);
+  auto tau = [&](float z) -> float {
+    if (!(z > zd)) return 0.0f;
+    const float arg = (pi * (z - zd)) / (ztop - zd);
+    return 0.5f * (1.0f - f32(std::cos((double)arg)));
+  };
+  for (int k = 0; k < nz; ++k) p.tauh[k] = tau(v.zh[k]);
+  for (int k = 0; k <= nz; ++k) p.tauf[k] = tau(f32(v.zf[k]));
+  return p;
+}
+
+// Metric buffer of a uniform, centred horizontal grid (grid.js:77-85,200-214).
+std::vector<float> make_metric(int nx, int ny, float dx, float dy) {
+  const int sJ = nx + 7, sK = ny + 7;
+  std::vector<float> m((size_t)9 * sJ + 3 * sK, 1.0f);
+  auto xf = [](int n, double d, int i) { return i * d - 0.5 * n * d; };
+  for (int i = 0; i < sJ; ++i) m[2 * sJ + 2 * sK + i] = f32(xf(nx, dx, i - 3));
+  for (int j = 0; j < sK; ++j) m[3 * sJ + 2 * sK + j] = f32(xf(ny, dy, j - 3));
+  const int base = 3 * sJ + 3 * sK;
+  for (int i = 0; i < sJ; ++i) {
+    m[base + 4 * sJ + i] = 0.0f;                                      // rxh (axisymmetric only)
+    m[base + 5 * sJ + i] = f32((i - 3 + 0.5) * dx - 0.5 * nx * dx);   // xh
+  }
+  return m;
+}
+
+// Counter-based normal deviates: SplitMix64 of (seed, cell) -> two uniforms ->
+// Box-Muller.
+static double unif(uint64_t key) {
+  uint64_t z = key + 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z ^= z >> 31;
+  return ((z >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+}
+static double normal(uint64_t seed, uint64_t cell) {
+  const double u1 = unif(seed * 0x100000001B3ull ^ (cell * 2 + 0)), u2 = unif(seed * 0x100000001B3ull ^ (cell * 2 + 1));
+  return std::sqrt(-2.0 * std::log(u1)) * std::cos(2.0 * M_PI * u2);
+}
+
+void fill_state(const te_config& cfg, const Vertical& v, const Base& b, const Layout& L, uint32_t* M) {
+  const int nx = L.nx, ny = L.ny, nz = L.nz, sJ = L.sJ, sK = L.sK;
+  auto field = [&](const char* n) -> float* {
+    uint64_t o = L.field(n);
+    if (o == UINT64_MAX) return nullptr;
+    return reinterpret_cast<float*>(M + o);
+  };
+  auto idx = [&](int i, int j, int k) { return (size_t)(i + 3) + (size_t)sJ * ((j + 3) + (size_t)sK * k); };
+  const float rd = f32(RDd), cp = f32(CPd), rv = f32(RVd), p00 = f32(P00d), cpdrd = cp / rd;
+  float *th = field("th"), *tha = field("th_a"), *u = field("u"), *ua = field("u_a"), *vv = field("v"), *va = field("v_a"),
+        *prs = field("prs"), *rho = field("rho"), *qv = field("qv"), *qva = field("qv_a"), *qc = field("qc"),
+        *qca = field("qc_a");
+  const double lx = nx * (double)cfg.dx, ly = ny * (double)cfg.dy;
+  std::vector<float> thp((size_t)sJ * sK * (nz + 1), 0.0f);
+  for (int k = 0; k < nz; ++k)
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i) {
+        const double x = (i + 0.5) * cfg.dx - 0.5 * lx, y = (j + 0.5) * cfg.dy - 0.5 * ly, z = v.zh[k];
+        const double bx = x / cfg.bubble_rh, by = y / cfg.bubble_rh, bz = (z - cfg.bubble_zc) / cfg.bubble_rv;
+        const double beta = std::sqrt(bx * bx + by * by + bz * bz);
+        double t = 0.0;
+        if (beta < 1.0) {
+          const double cb = std::cos(0.5 * M_PI * beta);
+          t = cfg.bubble_amp * cb * cb;
+        }
+        if (cfg.noise_sigma > 0.0f)
+          t += cfg.noise_sigma * normal(cfg.seed, (uint64_t)i + (uint64_t)nx * ((uint64_t)j + (uint64_t)ny * k));
+        thp[idx(i, j, k)] = f32(t);
+      }
+  // halo: periodic images (open boundaries: nearest interior value)
+  for (int k = 0; k < nz; ++k)
+    for (int j = -3; j < ny + 4; ++j)
+      for (int i = -3; i < nx + 4; ++i) {
+        if (i >= 0 && i < nx && j >= 0 && j < ny) continue;
+        int ii = cfg.open_x ? std::min(std::max(i, 0), nx - 1) : ((i % nx) + nx) % nx;
+        int jj = cfg.open_y ? std::min(std::max(j, 0), ny - 1) : ((j % ny) + ny) % ny;
+        thp[idx(i, j, k)] = thp[idx(ii, jj, k)];
+      }
+  for (int k = 0; k < nz; ++k) {
+    const float pi = b.pi0[k];
+    const float p = p00 * f32(std::pow((double)pi, (double)cpdrd));
+    for (int j = -3; j < ny + 4; ++j)
+      for (int i = -3; i < nx + 4; ++i) {
+        const size_t c = idx(i, j, k);
+        const float t = thp[c];
+        th[c] = t; tha[c] = t;
+        u[c] = b.u0[k]; ua[c] = b.u0[k];
+        vv[c] = b.v0[k]; va[c] = b.v0[k];
+        float q = 0.0f;
+        if (cfg.moist) {
+          q = b.qv0[k];
+          qv[c] = q; qva[c] = q;
+          if (qc) { qc[c] = b.qc0[k]; qca[c] = b.qc0[k]; }
+        }
+        const float rmix = rd + std::max(0.0f, q) * rv;
+        prs[c] = p;
+        rho[c] = p / (((b.th0[k] + t) * pi) * rmix);
+      }
+  }
+  // t=0 TKE profile of init3d testcases 3, 4/5, 9/11 (scenario.js:347-369),
+  // on faces k = 0..nz, zero at the surface and the top
+  float *tke = field("tke"), *tkea = field("tke_a");
+  const int tc = cfg.initial_tke;
+  if (tke && (tc == 3 || tc == 4 || tc == 5 || tc == 9 || tc == 11)) {
+    std::vector<float> prof(nz + 1, 0.0f);
+    for (int k = 1; k < nz; ++k) {
+      const double zf = v.zf[k], zh = v.zh[std::min(k, nz - 1)];
+      if (tc == 3) prof[k] = f32(std::max(0.0, 1.0 - zf / 3000.0));
+      else if (tc == 4 || tc == 5) prof[k] = 1.0f;
+      else if (zh < 255.0) {
+        const double r = 1.0 - zf / 255.0;
+        prof[k] = f32(0.4 * r * r * r);
+      }
+    }
+    for (int k = 0; k <= nz; ++k)
+      for (size_t q = 0; q < (size_t)sJ * sK; ++q) {
+        const size_t c = (size_t)k * sJ * sK + q;
+        tke[c] = prof[k];
+        if (tkea) tkea[c] = prof[k];
+      }
+  }
+}
+
+}  // namespace te

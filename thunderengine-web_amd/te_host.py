@@ -1,0 +1,263 @@
+"""Python host of the MI355X large step (ctypes over include/te.h).
+
+Mirrors the reference's host objects for this path:
+
+* ``Replayer`` - web/replay.js ``Replayer``: executes a planned command tape
+  (flat word memory + per-dispatch 64-word argument slots) on the GPU;
+* ``Run`` - web/core/driver.js ``buildRun``: builds the step program, the
+  FieldArena and the t=0 state from a configuration and steps it.
+
+The library is ``lib/libte.so`` next to this file (built by ``make`` here or
+by ``__graft_entry__.build()``).  Nothing here falls back to the CPU: every
+call fails loudly when the library or a GPU is missing.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libte.so")
+
+TE_OK = 0
+
+
+class TeError(RuntimeError):
+    pass
+
+
+class Config(ctypes.Structure):
+    _fields_ = [
+        ("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32),
+        ("dx", ctypes.c_float), ("dy", ctypes.c_float), ("dz", ctypes.c_float),
+        ("zf", ctypes.POINTER(ctypes.c_float)),
+        ("dt", ctypes.c_float), ("nsound", ctypes.c_int32), ("moist", ctypes.c_int32),
+        ("open_x", ctypes.c_int32), ("open_y", ctypes.c_int32),
+        ("weno_scalars", ctypes.c_int32), ("weno_momentum", ctypes.c_int32),
+        ("tke", ctypes.c_int32), ("explicit_vdiff", ctypes.c_int32), ("dissipative_heating", ctypes.c_int32),
+        ("rayleigh", ctypes.c_int32), ("rayleigh_zd", ctypes.c_float), ("rayleigh_alpha", ctypes.c_float),
+        ("alph", ctypes.c_float), ("kdiv", ctypes.c_float), ("smeps", ctypes.c_float),
+        ("umove", ctypes.c_float), ("vmove", ctypes.c_float),
+        ("th_sfc", ctypes.c_float), ("th_trop", ctypes.c_float), ("t_trop", ctypes.c_float),
+        ("z_trop", ctypes.c_float), ("qv_pbl", ctypes.c_float),
+        ("hodo_radius", ctypes.c_float), ("hodo_depth", ctypes.c_float), ("shear_umax", ctypes.c_float),
+        ("shear_depth", ctypes.c_float),
+        ("bubble_amp", ctypes.c_float), ("bubble_rh", ctypes.c_float), ("bubble_rv", ctypes.c_float),
+        ("bubble_zc", ctypes.c_float),
+        ("noise_sigma", ctypes.c_float), ("seed", ctypes.c_uint64), ("initial_tke", ctypes.c_int32),
+        ("species", ctypes.c_int32), ("microphysics", ctypes.c_int32), ("mass_conservation", ctypes.c_int32),
+        ("cache_moisture_turbulence", ctypes.c_int32), ("n_arena_fields", ctypes.c_int32),
+        ("arena_fields", ctypes.POINTER(ctypes.c_char_p)),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    """Loads libte.so (raises TeError when it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise TeError("libte.so is not built (run __graft_entry__.build() or make -C thunderengine-web_amd)")
+    L = ctypes.CDLL(LIB_PATH)
+    P = ctypes.POINTER
+    u32p, u64 = P(ctypes.c_uint32), ctypes.c_uint64
+    ctx = ctypes.c_void_p
+    sig = {
+        "te_program_create": ([u64, ctypes.c_int32, P(ctypes.c_char_p), u32p, P(ctx)], ctypes.c_int),
+        "te_upload_words": ([ctx, u64, u32p, u64], ctypes.c_int),
+        "te_download_words": ([ctx, u64, u32p, u64], ctypes.c_int),
+        "te_run": ([ctx, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32], ctypes.c_int),
+        "te_sync": ([ctx], ctypes.c_int),
+        "te_profile_ops": ([ctx, ctypes.c_int32, P(ctypes.c_float)], ctypes.c_int),
+        "te_region_hash": ([ctx, u64, u64, u32p], ctypes.c_int),
+        "te_fnv1a": ([ctx, u64, u64, u32p], ctypes.c_int),
+        "te_memory_words": ([ctx], u64),
+        "te_op_count": ([ctx], ctypes.c_int32),
+        "te_op_get": ([ctx, ctypes.c_int32, P(ctypes.c_char_p), u32p], ctypes.c_int),
+        "te_last_run_ms": ([ctx, P(ctypes.c_float)], ctypes.c_int),
+        "te_last_error": ([ctx], ctypes.c_char_p),
+        "te_kernel_count": ([], ctypes.c_int32),
+        "te_kernel_name": ([ctypes.c_int32], ctypes.c_char_p),
+        "te_destroy": ([ctx], None),
+        "te_config_defaults": ([P(Config)], None),
+        "te_encode": ([P(Config), P(ctx)], ctypes.c_int),
+        "te_create": ([P(Config), P(ctx)], ctypes.c_int),
+        "te_init_state": ([ctx], ctypes.c_int),
+        "te_step": ([ctx, ctypes.c_int32], ctypes.c_int),
+        "te_field_offset": ([ctx, ctypes.c_char_p, P(u64)], ctypes.c_int),
+        "te_field_stride_words": ([ctx], u64),
+        "te_region_offset": ([ctx, ctypes.c_char_p, P(u64)], ctypes.c_int),
+        "te_field_count": ([ctx], ctypes.c_int32),
+        "te_field_name": ([ctx, ctypes.c_int32], ctypes.c_char_p),
+        "te_grid": ([ctx, P(ctypes.c_int32)], ctypes.c_int),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = L
+    return L
+
+
+def kernel_names():
+    """Names of the kernels libte.so implements."""
+    L = lib()
+    return [L.te_kernel_name(i).decode() for i in range(L.te_kernel_count())]
+
+
+def config(**kw):
+    """te_config with the library defaults (64x64x40 synthetic case) and overrides."""
+    c = Config()
+    lib().te_config_defaults(ctypes.byref(c))
+    keep = []
+    for k, v in kw.items():
+        if k == "zf":
+            arr = (ctypes.c_float * len(v))(*v)
+            keep.append(arr)
+            c.zf = ctypes.cast(arr, ctypes.POINTER(ctypes.c_float))
+        elif k == "arena_fields":
+            arr = (ctypes.c_char_p * len(v))(*[s.encode() for s in v])
+            keep.append(arr)
+            c.arena_fields = arr
+            c.n_arena_fields = len(v)
+        else:
+            setattr(c, k, v)
+    c._keep = keep
+    return c
+
+
+class _Ctx:
+    def __init__(self):
+        self.h = ctypes.c_void_p()
+
+    def _check(self, rc, what):
+        if rc != TE_OK:
+            msg = lib().te_last_error(self.h)
+            raise TeError("%s failed (%d): %s" % (what, rc, msg.decode() if msg else "?"))
+
+    def close(self):
+        if self.h:
+            lib().te_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- memory
+    @property
+    def mem_words(self):
+        return int(lib().te_memory_words(self.h))
+
+    def upload(self, word_off, words):
+        w = np.ascontiguousarray(words, dtype=np.uint32)
+        self._check(lib().te_upload_words(self.h, word_off, w.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), w.size),
+                    "te_upload_words")
+
+    def read(self, word_off, nwords):
+        out = np.empty(nwords, dtype=np.uint32)
+        self._check(lib().te_download_words(self.h, word_off, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+                                            nwords), "te_download_words")
+        return out
+
+    def region_hash(self, word_off, nwords):
+        out = ctypes.c_uint32()
+        self._check(lib().te_region_hash(self.h, word_off, nwords, ctypes.byref(out)), "te_region_hash")
+        return out.value
+
+    def fnv1a(self, word_off, nwords):
+        out = ctypes.c_uint32()
+        self._check(lib().te_fnv1a(self.h, word_off, nwords, ctypes.byref(out)), "te_fnv1a")
+        return out.value
+
+    # ---- execution
+    def run(self, nsteps=1, first=0, last=-1):
+        self._check(lib().te_run(self.h, nsteps, first, last), "te_run")
+
+    def sync(self):
+        self._check(lib().te_sync(self.h), "te_sync")
+
+    def profile_ops(self, nsteps=1):
+        """Per-op device milliseconds summed over nsteps eager steps (HIP events)."""
+        n = int(lib().te_op_count(self.h))
+        out = np.zeros(n, dtype=np.float32)
+        self._check(lib().te_profile_ops(self.h, nsteps, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))),
+                    "te_profile_ops")
+        return out
+
+    def last_run_ms(self):
+        ms = ctypes.c_float()
+        self._check(lib().te_last_run_ms(self.h, ctypes.byref(ms)), "te_last_run_ms")
+        return ms.value
+
+    def program(self):
+        """(names, (nops, 64) uint32 argument slots) of the context's program."""
+        n = int(lib().te_op_count(self.h))
+        names, A = [], np.zeros((n, 64), dtype=np.uint32)
+        nm = ctypes.c_char_p()
+        for i in range(n):
+            self._check(lib().te_op_get(self.h, i, ctypes.byref(nm), A[i].ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))),
+                        "te_op_get")
+            names.append(nm.value.decode())
+        return names, A
+
+
+class Replayer(_Ctx):
+    """web/replay.js Replayer over the C ABI: plan + build + upload + encodeSteps + read."""
+
+    def __init__(self, mem_words, names, args):
+        super().__init__()
+        n = len(names)
+        cn = (ctypes.c_char_p * n)(*[s.encode() for s in names])
+        A = np.ascontiguousarray(args, dtype=np.uint32)
+        self._check(lib().te_program_create(mem_words, n, cn, A.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+                                            ctypes.byref(self.h)), "te_program_create")
+
+    def encode_steps(self, n=1, last_op=-1, first_op=0):
+        self.run(n, first_op, last_op)
+
+
+class Run(_Ctx):
+    """buildRun(configuration) (web/core/driver.js:23-75): program + arena + state."""
+
+    def __init__(self, cfg=None, encode_only=False, **kw):
+        super().__init__()
+        self.cfg = cfg if cfg is not None else config(**kw)
+        fn = lib().te_encode if encode_only else lib().te_create
+        self._check(fn(ctypes.byref(self.cfg), ctypes.byref(self.h)), "te_encode" if encode_only else "te_create")
+        g = (ctypes.c_int32 * 5)()
+        self._check(lib().te_grid(self.h, g), "te_grid")
+        self.nx, self.ny, self.nz, self.sJ, self.sK = list(g)
+        self.field_words = int(lib().te_field_stride_words(self.h))
+        self.fields = [lib().te_field_name(self.h, i).decode() for i in range(lib().te_field_count(self.h))]
+
+    def init_state(self):
+        self._check(lib().te_init_state(self.h), "te_init_state")
+
+    def step(self, n=1):
+        self._check(lib().te_step(self.h, n), "te_step")
+
+    def field_offset(self, name):
+        o = ctypes.c_uint64()
+        self._check(lib().te_field_offset(self.h, name.encode(), ctypes.byref(o)), "te_field_offset(%s)" % name)
+        return o.value
+
+    def region_offset(self, name):
+        o = ctypes.c_uint64()
+        self._check(lib().te_region_offset(self.h, name.encode(), ctypes.byref(o)), "te_region_offset(%s)" % name)
+        return o.value
+
+    def field(self, name):
+        """FieldArena.field(name): (nz+1, strideK, strideJ) float32 copy of the field."""
+        n = self.sJ * self.sK * (self.nz + 1)
+        w = self.read(self.field_offset(name), n)
+        return w.view(np.float32).reshape(self.nz + 1, self.sK, self.sJ)
+
+    def interior(self, name, stagger=(0, 0, 0)):
+        f = self.field(name)
+        return f[: self.nz + stagger[2], 3: 3 + self.ny + stagger[1], 3: 3 + self.nx + stagger[0]]
