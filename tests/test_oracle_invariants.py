@@ -1,0 +1,63 @@
+"""Whole-step invariants of the oracle on periodic configurations (CPU).
+
+The recorded tape is an open-boundary case, so the periodic halo kernels and
+the periodic index arithmetic of the large step are pinned here instead:
+
+* halo cells hold exact periodic images of the interior after a step;
+* the step commutes with a circular shift of the domain: stepping a shifted
+  initial state gives, bit for bit, the shifted result (nx a multiple of 4,
+  so the FMA-order switch of acoustic_prepare is the same for every column).
+  With TKE this holds for one step only: the reference never refreshes the
+  halo of rho after update_rho_prs (web/core/timestep.js:2132-2141 writes the
+  interior; turb_rho_faces and turb_def_centers, timestep.js:1074-1090, read
+  rho at i, j = -1 and -2), so from step 2 the turbulence at the domain edge
+  sees t=0 halo values, which a shift does not commute with.
+"""
+import numpy as np
+import pytest
+
+import runs
+import te_host
+
+PROGNOSTIC = ["th", "u", "v", "w", "pp3d", "qv", "tke"]
+
+
+def interior_wrap(f, nx, ny):
+    inner = f[:, 3:3 + ny, 3:3 + nx]
+    return np.pad(inner, ((0, 0), (3, 4), (3, 4)), mode="wrap")
+
+
+@pytest.mark.parametrize("kw", [dict(), dict(tke=1, initial_tke=3, rayleigh=1)])
+def test_periodic_halos_after_step(kw):
+    r = runs.OracleRun(te_host.config(nx=24, ny=20, nz=12, **kw))
+    r.step(2)
+    e = r.enc
+    for name in ("th", "u", "v", "w", "qv", "pp3d"):
+        f = r.field(name)[: e.nz]
+        want = interior_wrap(f, e.nx, e.ny)
+        assert np.array_equal(f.view(np.uint32), want.view(np.uint32)), name
+
+
+@pytest.mark.parametrize("kw,steps", [(dict(), 3), (dict(weno_momentum=1, nsound=4), 2),
+                                      (dict(tke=1, initial_tke=3, weno_momentum=1, rayleigh=1), 1)])
+def test_step_commutes_with_circular_shift(kw, steps):
+    cfg = te_host.config(nx=24, ny=20, nz=12, **kw)
+    a = runs.OracleRun(cfg)
+    b = runs.OracleRun(cfg)
+    e = a.enc
+    sx, sy = 7, 5
+    a0 = e.field_offset(e.fields[0])
+    n = e.sJ * e.sK * (e.nz + 1)
+    for q in range(len(e.fields)):
+        o = a0 + q * e.field_words
+        f = b.M[o: o + n].view(np.float32).reshape(e.nz + 1, e.sK, e.sJ)
+        inner = np.roll(f[:, 3:3 + e.ny, 3:3 + e.nx], (sy, sx), axis=(1, 2))
+        f[:] = np.pad(inner, ((0, 0), (3, 4), (3, 4)), mode="wrap")
+    a.step(steps)
+    b.step(steps)
+    for name in PROGNOSTIC:
+        if name not in e.fields:
+            continue
+        fa = np.roll(a.field(name)[:, 3:3 + e.ny, 3:3 + e.nx], (sy, sx), axis=(1, 2))
+        fb = b.field(name)[:, 3:3 + e.ny, 3:3 + e.nx]
+        assert np.array_equal(fa.view(np.uint32), fb.view(np.uint32)), name
