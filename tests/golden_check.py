@@ -25,10 +25,11 @@ def plan():
     return tapeplan.Plan(tape, params)
 
 
-def check(P, run_op, region_hash, have):
+def check(P, run_op, region_hash, have, ok_slots=None):
     """run_op(i) executes dispatch i of P.ops; region_hash(off, n) hashes M.
 
-    Returns (ok, bad, skipped_ops, mismatches[(op, kernel, region, slot)]).
+    Returns (ok, bad, skipped_ops, mismatches[(op, kernel, region, slot)]);
+    appends the index of every matching slot to ok_slots when given.
     """
     ref = tapeplan.ref_hashes()
     slots_by_op = {}
@@ -58,6 +59,8 @@ def check(P, run_op, region_hash, have):
                 continue
             if region_hash(off, ln) == int(ref[s]):
                 ok += 1
+                if ok_slots is not None:
+                    ok_slots.append(s)
             else:
                 bad += 1
                 tainted.add(f)

@@ -1,0 +1,65 @@
+'use strict';
+// Node-side checks of the JavaScript host, driven by tests/test_node_host.py.
+//   node check_host.js plan            -> tape plan summary (CPU)
+//   node check_host.js encode '<json>' -> program summary of a config (CPU)
+//   node check_host.js replay <out>    -> supercell tape replayed on the GPU with hashes
+//   node check_host.js run '<json>' n  -> field FNV-1a after n steps on the GPU
+const path = require('path');
+const fs = require('fs');
+const zlib = require('zlib');
+const te = require(path.join(__dirname, '..', '..', 'thunderengine-web_amd', 'js', 'thunderengine.js'));
+
+const SC = path.join(__dirname, '..', 'golden', 'supercell');
+const loadTape = () => JSON.parse(zlib.gunzipSync(fs.readFileSync(path.join(SC, 'tape_step1.json.gz'))));
+const loadParams = () => JSON.parse(fs.readFileSync(path.join(SC, 'kernel_params.json')));
+
+function fnvWords(w) {
+  let h = 0x811c9dc5;
+  for (let i = 0; i < w.length; i++) h = Math.imul(h ^ w[i], 0x01000193) >>> 0;
+  return h >>> 0;
+}
+
+const mode = process.argv[2];
+if (mode === 'plan') {
+  const p = te.planTape(loadTape(), loadParams());
+  console.log(JSON.stringify({ memWords: p.memWords, ops: p.names.length, slots: p.slots.length,
+    argsFnv: fnvWords(p.args), slotFnv: fnvWords(Uint32Array.from(p.slots.map(s => s.off))) }));
+} else if (mode === 'encode') {
+  const r = te.buildRun(JSON.parse(process.argv[3]), { encodeOnly: true });
+  const prog = r.program();
+  console.log(JSON.stringify({ ops: prog.names.length, names: prog.names, argsFnv: fnvWords(prog.args),
+    fields: r.arena.names.length, memWords: r.memoryWords() }));
+} else if (mode === 'replay') {
+  const tape = loadTape();
+  const rp = new te.Replayer(tape, loadParams());
+  const p = rp.plan();
+  const have = new Set(te.addon().kernelNames());
+  p.names.forEach((n, i) => {   // kernels outside this path become empty copies
+    if (!have.has(n)) { p.names[i] = '_copy'; p.args.fill(0, i * 64, i * 64 + 64); }
+  });
+  rp.build();
+  for (const b of tape.buffers) {
+    const f = path.join(SC, 'state', b.id + '.bin.gz');
+    if (fs.existsSync(f)) rp.upload(b.id, zlib.gunzipSync(fs.readFileSync(f)));
+  }
+  console.log('replaying', p.names.length, 'dispatches');
+  rp.encodeSteps(1, true);
+  fs.writeFileSync(process.argv[3], Buffer.from(rp.readHashes().buffer));
+  rp.destroy();
+  console.log('done');
+} else if (mode === 'run') {
+  const conf = JSON.parse(process.argv[3]);
+  const n = Number(process.argv[4]);
+  const r = te.buildRun(conf);
+  r.fill();
+  r.step(n);
+  const out = {};
+  const L = r.layout;
+  const words = L.strideJ * L.strideK * (L.nz + 1);
+  for (const name of r.arena.names) out[name] = r.fnv1a(r.arena.offsetWords(name), words);
+  console.log(JSON.stringify(out));
+  r.destroy();
+} else {
+  console.error('unknown mode ' + mode);
+  process.exit(2);
+}

@@ -1,0 +1,105 @@
+"""The JavaScript host (Node N-API addon + thunderengine.js).
+
+CPU: its tape planner agrees with the reference's replay plan restated in
+tests/tapeplan.py, and its buildRun encodes the same program as the C ABI.
+GPU: it replays the recorded supercell tape with per-dispatch hashes equal to
+the reference's, and steps configurations bit-exactly against the oracle.
+"""
+import json
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+import golden_check
+import oracle_lib as ol
+import runs
+import tapeplan
+import te_host
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+SCRIPT = os.path.join(ROOT, "tests", "js", "check_host.js")
+ADDON = os.path.join(ROOT, "thunderengine-web_amd", "lib", "te_napi.node")
+
+pytestmark = pytest.mark.skipif(shutil.which("node") is None or not os.path.exists(ADDON),
+                                reason="node or the N-API addon is not available")
+
+
+def node(*args, timeout=300):
+    out = subprocess.run(["node", SCRIPT] + [str(a) for a in args], capture_output=True, text=True, timeout=timeout)
+    assert out.returncode == 0, out.stderr[-2000:]
+    return out.stdout
+
+
+def fnv(words):
+    h = 0x811C9DC5
+    for w in np.asarray(words, np.uint32).tolist():
+        h = ((h ^ w) * 0x01000193) & 0xFFFFFFFF
+    return h
+
+
+def test_tape_plan_matches_reference_plan():
+    got = json.loads(node("plan"))
+    P = golden_check.plan()
+    names, A = P.args()
+    assert got["memWords"] == P.mem_words
+    assert got["ops"] == len(names)
+    assert got["slots"] == len(P.slots) == 1010
+    assert got["argsFnv"] == ol.fnv1a(A.reshape(-1))
+    assert got["slotFnv"] == ol.fnv1a(np.array([s[2] for s in P.slots], np.uint32))
+
+
+def test_build_run_encodes_like_the_c_abi():
+    conf = dict(nx=24, ny=20, nz=12, tke=1, rayleigh=1)
+    got = json.loads(node("encode", json.dumps(conf)))
+    r = te_host.Run(te_host.config(**conf), encode_only=True)
+    names, A = r.program()
+    assert got["names"] == names
+    assert got["argsFnv"] == ol.fnv1a(A.reshape(-1))
+    assert got["memWords"] == r.mem_words
+
+
+def test_preset_mapping():
+    conf = {"param0": {"nx": 24, "ny": 20, "nz": 12}, "param1": {"dx": 1000.0, "dy": 1000.0, "dz": 500.0, "dtl": 6.0},
+            "param2": {"advwenos": 2, "advwenov": 0, "sgsmodel": 1, "doimpl": 1, "irdamp": 1, "wbc": 1, "ebc": 1,
+                       "sbc": 1, "nbc": 1, "imoist": 1, "ptype": 0, "idiss": 1, "imove": 0},
+            "param3": {"kdiv": 0.1, "alph": 0.6, "zd": 15000.0, "rdalpha": 0.0033333333333}}
+    got = json.loads(node("encode", json.dumps(conf)))
+    r = te_host.Run(te_host.config(nx=24, ny=20, nz=12, tke=1, rayleigh=1, dissipative_heating=1), encode_only=True)
+    names, A = r.program()
+    assert got["names"] == names
+    assert got["argsFnv"] == ol.fnv1a(A.reshape(-1))
+
+
+@pytest.mark.gpu
+def test_replay_supercell_tape_on_gpu(gpu, tmp_path):
+    out = tmp_path / "hashes.bin"
+    print(node("replay", out, timeout=600), flush=True)
+    got = np.fromfile(out, dtype=np.uint32)
+    # the slots the oracle reproduces must all match on the GPU through the JS host
+    P = golden_check.plan()
+    have = set(ol.kernel_names()) | {"_copy"}
+    names, A = P.args()
+    prog = ol.Program(names, A)
+    M = P.memory()
+    ok_slots = []
+    ok, bad, skipped, mism = golden_check.check(
+        P, lambda i: prog.run(M, i, i), lambda off, n: ol.region_hash(M, off, n), have, ok_slots)
+    assert bad == 0 and ok >= 800
+    ref = tapeplan.ref_hashes()
+    wrong = [s for s in ok_slots if got[s] != ref[s]]
+    assert not wrong, wrong[:10]
+
+
+@pytest.mark.gpu
+def test_js_run_bit_exact_vs_oracle(gpu):
+    conf = dict(nx=32, ny=24, nz=16, tke=1, initial_tke=3, rayleigh=1)
+    got = json.loads(node("run", json.dumps(conf), 2))
+    ref = runs.OracleRun(te_host.config(**conf))
+    ref.step(2)
+    e = ref.enc
+    n = e.sJ * e.sK * (e.nz + 1)
+    bad = [f for f in e.fields if got[f] != fnv(ref.M[e.field_offset(f): e.field_offset(f) + n])]
+    assert not bad, bad

@@ -1,0 +1,214 @@
+'use strict';
+// JavaScript host of the MI355X large step (Node >= 12, CommonJS).
+//
+// Mirrors the reference's host objects for this path over the N-API addon
+// (lib/te_napi.node -> libte.so):
+//   Replayer          web/replay.js Replayer: plans a recorded command tape
+//                     into one word-addressed memory M plus one 64-word
+//                     argument slot per dispatch, uploads buffers, runs steps
+//                     (optionally hashing every written region), reads back.
+//   buildRun(conf)    web/core/driver.js buildRun: a preset / namelist object
+//                     (or flat options) -> step program, FieldArena, t=0 state.
+//   presetToConfig    the namelist keys of this path (web/core/config.js).
+// Nothing here computes on the CPU: every call goes to the HIP library and
+// throws when the addon or a GPU is missing.
+const path = require('path');
+
+let native = null;
+function addon() {
+  if (!native) native = require(path.join(__dirname, '..', 'lib', 'te_napi.node'));
+  return native;
+}
+
+const ALIGN = 64;  // words: every buffer starts on a 256-byte boundary (replay.js plan)
+
+// ---------------------------------------------------------------- tapes
+// plan (web/replay.js:46-121): buffers at 64-word-aligned bases, setBytes
+// payloads after them (rounded to 4 words + 4 spare), a 64-word argument slot
+// per dispatch (A[0..2] grid, A[4+n] word offset of parameter n), copies as
+// "_copy" slots (A[0] words, A[4] source, A[5] destination), and one hash slot
+// per (dispatch, written parameter) in kernel-signature order.
+function planTape(tape, kernelParams, cmds) {
+  cmds = cmds || tape.steps[0];
+  const base = [];
+  let top = 0;
+  for (const b of tape.buffers) { base.push(top); top += Math.ceil(b.length / (4 * ALIGN)) * ALIGN; }
+  const buffersEnd = top;
+  const bytes = [];
+  const byteOff = new Map();
+  cmds.forEach((c, i) => {
+    if (c.op !== 'bytes') return;
+    const raw = Buffer.from(c.data, 'base64');
+    const words = Math.ceil(raw.length / 4);
+    const w = new Uint32Array(words);
+    Buffer.from(w.buffer).set(raw);
+    byteOff.set(i, top);
+    bytes.push({ off: top, words: w });
+    top += Math.ceil(words / 4) * 4 + 4;
+  });
+  const memWords = Math.ceil(top / ALIGN) * ALIGN;
+  const ops = [], slots = [];
+  let bind = {}, last = {}, pipe = null;
+  const regionLen = off => {
+    for (let b = tape.buffers.length - 1; b >= 0; b--) {
+      if (off >= base[b]) {
+        const end = base[b] + tape.buffers[b].length / 4;
+        return tape.buffers[b].arena ? Math.min(tape.fieldStride / 4, end - off) : end - off;
+      }
+    }
+    return 0;
+  };
+  cmds.forEach((c, i) => {
+    switch (c.op) {
+      case 'pipe': pipe = c.name; break;
+      case 'buf': bind[c.index] = c.buf < 0 ? 0 : base[c.buf] + c.offset / 4; last[c.index] = c.buf; break;
+      case 'bufoff': bind[c.index] = base[last[c.index]] + c.offset / 4; break;
+      case 'bytes': bind[c.index] = byteOff.get(i); break;
+      case 'end': bind = {}; break;
+      case 'copy':
+        ops.push({ name: '_copy', n: c.size / 4, src: base[c.src] + c.srcOffset / 4, dst: base[c.dst] + c.dstOffset / 4 });
+        break;
+      case 'dispatch': {
+        const o = { name: pipe, grid: c.grid.slice(), bind: Object.assign({}, bind) };
+        const sig = kernelParams[pipe];
+        if (!sig) throw new Error('kernel ' + pipe + ' has no signature');
+        for (const [idx, , write] of sig) {
+          if (write) { const off = bind[idx] || 0; slots.push({ op: ops.length, index: idx, off, len: regionLen(off) }); }
+        }
+        ops.push(o);
+        break;
+      }
+      default: break;
+    }
+  });
+  const names = ops.map(o => o.name);
+  const args = new Uint32Array(ops.length * 64);
+  ops.forEach((o, n) => {
+    const A = args.subarray(n * 64, n * 64 + 64);
+    if (o.name === '_copy') { A[0] = o.n; A[1] = 1; A[2] = 1; A[4] = o.src; A[5] = o.dst; return; }
+    A[0] = o.grid[0]; A[1] = o.grid[1]; A[2] = o.grid[2];
+    for (const k of Object.keys(o.bind)) A[4 + Number(k)] = o.bind[k];
+  });
+  return { base, buffersEnd, bytes, memWords, ops, slots, names, args };
+}
+
+class Replayer {
+  // new Replayer(tape, kernelParams): kernelParams maps kernel name ->
+  // [[index, kind, write], ...] (the kernels.json signature table).
+  constructor(tape, kernelParams) {
+    this.tape = tape;
+    this.kernelParams = kernelParams;
+    this.planned = null;
+    this.ctx = null;
+  }
+  plan() { this.planned = planTape(this.tape, this.kernelParams); return this.planned; }
+  build() {
+    if (!this.planned) this.plan();
+    const p = this.planned;
+    this.ctx = addon().programCreate(p.memWords, p.names, p.args);
+    for (const b of p.bytes) addon().upload(this.ctx, b.off, b.words);
+    return this;
+  }
+  // upload(id, bytes): t=0 content of buffer `id` (a typed array or Buffer)
+  upload(id, data) {
+    const u8 = data instanceof Uint8Array ? data : new Uint8Array(data.buffer, data.byteOffset, data.byteLength);
+    const words = new Uint32Array(Math.ceil(u8.length / 4));
+    new Uint8Array(words.buffer).set(u8);
+    addon().upload(this.ctx, this.planned.base[id], words);
+  }
+  // encodeSteps(n, withHashes, lastOp): runs n steps (ops 0..lastOp); with
+  // hashes, one step op by op, hashing every written region after its
+  // dispatch into this.hashes (slot order of replay.js).
+  encodeSteps(n, withHashes, lastOp) {
+    const p = this.planned;
+    const last = lastOp == null ? p.names.length - 1 : lastOp;
+    if (!withHashes) { addon().run(this.ctx, n, 0, last); addon().sync(this.ctx); return; }
+    this.hashes = new Uint32Array(p.slots.length);
+    let s = 0;
+    for (let i = 0; i <= last; i++) {
+      addon().run(this.ctx, 1, i, i);
+      while (s < p.slots.length && p.slots[s].op === i) {
+        this.hashes[s] = addon().regionHash(this.ctx, p.slots[s].off, p.slots[s].len);
+        s++;
+      }
+    }
+  }
+  read(words, offWords) { return addon().download(this.ctx, offWords, words); }
+  readHashes() { return this.hashes; }
+  destroy() { if (this.ctx) addon().destroy(this.ctx); this.ctx = null; }
+}
+
+// ---------------------------------------------------------------- presets
+// The namelist keys of this path (web/core/config.js reads the same groups):
+// grid (param0/param1), dt (dtl), boundary codes (wbc/ebc: 1 periodic,
+// 2 open), WENO (advwenos/advwenov = 2), TKE (sgsmodel 1), implicit vertical
+// diffusion (doimpl), Rayleigh damping (irdamp, zd, rdalpha), idiss, imove
+// (umove, vmove), kdiv, alph, imoist and the explicit nsound (psolver 3).
+function presetToConfig(preset, overrides) {
+  const g = k => { for (const grp of Object.keys(preset || {})) { const v = preset[grp]; if (v && typeof v === 'object' && k in v) return v[k]; } return undefined; };
+  const c = {};
+  const set = (key, v) => { if (v !== undefined && v !== null) c[key] = v; };
+  set('nx', g('nx')); set('ny', g('ny')); set('nz', g('nz'));
+  set('dx', g('dx')); set('dy', g('dy')); set('dz', g('dz'));
+  set('dt', g('dtl'));
+  const wbc = g('wbc'), sbc = g('sbc');
+  if (wbc !== undefined) c.open_x = wbc === 2 ? 1 : 0;
+  if (sbc !== undefined) c.open_y = sbc === 2 ? 1 : 0;
+  if (g('advwenos') !== undefined) c.weno_scalars = g('advwenos') >= 1 ? 1 : 0;
+  if (g('advwenov') !== undefined) c.weno_momentum = g('advwenov') >= 1 ? 1 : 0;
+  if (g('sgsmodel') !== undefined) c.tke = g('sgsmodel') === 1 ? 1 : 0;
+  if (g('doimpl') !== undefined) c.explicit_vdiff = g('doimpl') === 0 ? 1 : 0;
+  if (g('idiss') !== undefined) c.dissipative_heating = g('idiss') === 1 ? 1 : 0;
+  if (g('irdamp') !== undefined) c.rayleigh = g('irdamp') === 1 ? 1 : 0;
+  set('rayleigh_zd', g('zd')); set('rayleigh_alpha', g('rdalpha'));
+  set('alph', g('alph')); set('kdiv', g('kdiv'));
+  if (g('imove') === 1) { set('umove', g('umove')); set('vmove', g('vmove')); }
+  if (g('imoist') !== undefined) c.moist = g('imoist') === 1 ? 1 : 0;
+  if (g('ptype') !== undefined && g('ptype') !== 0) c.microphysics = g('ptype');
+  if (g('apmasscon') !== undefined) c.mass_conservation = g('apmasscon');
+  set('nsound', g('nsound'));
+  return Object.assign(c, overrides || {});
+}
+
+// ---------------------------------------------------------------- runs
+class FieldArena {
+  constructor(run) { this.run = run; this.layout = run.layout; this.fieldStride = run.layout.fieldStrideWords * 4; }
+  get names() { return Object.keys(this.layout.fields); }
+  offsetWords(name) {
+    const o = this.layout.fields[name];
+    if (o === undefined) throw new Error('no field ' + name);
+    return o;
+  }
+  // field(name): Float32Array copy of strideJ*strideK*(nz+1) values
+  field(name) {
+    const L = this.layout;
+    const w = addon().download(this.run.ctx, this.offsetWords(name), L.strideJ * L.strideK * (L.nz + 1));
+    return new Float32Array(w.buffer);
+  }
+  setField(name, values) {
+    const w = new Uint32Array(values.buffer, values.byteOffset, values.length);
+    addon().upload(this.run.ctx, this.offsetWords(name), w);
+  }
+  index(i, j, k) { const L = this.layout; return (i + 3) + L.strideJ * ((j + 3) + L.strideK * k); }
+}
+
+function buildRun(conf, options) {
+  options = options || {};
+  const flat = conf && (conf.param0 || conf.param1 || conf.param2) ? presetToConfig(conf, options.overrides) : Object.assign({}, conf || {}, options.overrides || {});
+  const ctx = addon().create(flat, !!options.encodeOnly);
+  const run = { ctx, config: flat, layout: addon().layout(ctx) };
+  run.arena = new FieldArena(run);
+  run.program = () => addon().program(ctx);
+  run.fill = () => { addon().initState(ctx); return run; };
+  run.encodeStep = () => { addon().step(ctx, 1); return run; };
+  run.step = n => { addon().step(ctx, n == null ? 1 : n); addon().sync(ctx); return run; };
+  run.lastStepMs = () => addon().lastRunMs(ctx);
+  run.read = (words, offWords) => addon().download(ctx, offWords, words);
+  run.fnv1a = (offWords, words) => addon().fnv1a(ctx, offWords, words);
+  run.regionHash = (offWords, words) => addon().regionHash(ctx, offWords, words);
+  run.memoryWords = () => addon().memoryWords(ctx);
+  run.destroy = () => addon().destroy(ctx);
+  return run;
+}
+
+module.exports = { addon, planTape, Replayer, presetToConfig, buildRun, FieldArena };
