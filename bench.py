@@ -30,6 +30,10 @@ PEAK_HBM_GBS = 8000.0
 # 4 B x [18 + 3 (12 + 3*3 + 3 + 4) + 23*11 + 8] (fused design, TKE, 3 scalars)
 B_ALG_STEP = 1452.0
 
+# kernels launched with one lane per (i, j) column that sweep k serially
+COLUMN_KERNELS = {"acoustic_w_solve", "turb_scalar_vert_implicit", "turb_u_tendency", "turb_v_tendency",
+                  "turb_w_tendency", "tke_transport"}
+
 WORKLOAD = dict(nx=192, ny=192, nz=80, dx=1000.0, dy=1000.0, dz=250.0, dt=6.0, nsound=6, moist=1, species=3,
                 weno_scalars=1, weno_momentum=0, tke=1, rayleigh=1, rayleigh_zd=15000.0)
 
@@ -49,6 +53,8 @@ def alg_bytes_per_launch(run, names, A, op):
     a0 = run.field_offset(run.fields[0])
     end = a0 + run.field_words * len(run.fields)
     pts = int(A[op, 0]) * int(A[op, 1]) * int(A[op, 2])
+    if names[op] in COLUMN_KERNELS:  # one lane per column sweeps all levels
+        pts *= run.nz
     n = 0
     for idx, read, written in io:
         off = int(A[op, 4 + idx])
