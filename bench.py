@@ -64,15 +64,28 @@ def alg_bytes_per_launch(run, names, A, op):
 
 
 def profiles_traffic(kernel):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC
-    summary (profiles/<kernel>_pmc_192x192x80.txt), or None."""
-    p = os.path.join(ROOT, "profiles", "%s_pmc_192x192x80.txt" % kernel)
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary
+    (profiles/hbm_traffic_192x192x80.txt: FETCH_SIZE + WRITE_SIZE per dispatch,
+    each scaled by the copy-kernel calibration), or None."""
+    p = os.path.join(ROOT, "profiles", "hbm_traffic_192x192x80.txt")
     if not os.path.exists(p):
         return None
     for line in open(p):
-        if line.startswith("traffic_bytes_per_launch"):
-            return float(line.split("=")[1])
+        f = line.split()
+        if len(f) >= 5 and f[0] == kernel:
+            return float(f[4])
     return None
+
+
+def calibration_copy(words=1 << 26):
+    """One plain copy of `words` 32-bit words between two fresh buffers (the
+    PMC runs read FETCH_SIZE / WRITE_SIZE of this dispatch: 4*words bytes each)."""
+    A = np.zeros((1, 64), np.uint32)
+    A[0, 0], A[0, 4], A[0, 5] = words, 0, words
+    r = te_host.Replayer(2 * words, ["_copy"], A)
+    r.run(1)
+    r.sync()
+    r.close()
 
 
 def cpu_baseline(cfg, seconds=15.0):
@@ -101,11 +114,14 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=3)
+    ap.add_argument("--calib-copy", action="store_true", help="also launch the PMC calibration copy")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
 
+    if args.calib_copy:
+        calibration_copy()
     cfg = te_host.config(**WORKLOAD)
     run = te_host.Run(cfg)
     run.init_state()

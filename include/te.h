@@ -118,6 +118,9 @@ typedef struct te_config {
   uint64_t seed;
   int32_t initial_tke;     /* init3d testcase of the t=0 TKE profile: 0 none, 3, 4, 5, 9, 11
                               (scenario.js:347-369) */
+  int32_t fuse;            /* 1 (default): merged / fused launches where the step allows them
+                              (same state bits; scratch fields may differ); 0: the reference's
+                              dispatch list one for one */
   /* encoder options used to reproduce recorded reference configurations */
   int32_t species;         /* moist scalars: 3 = th qv qc, 7 = + qr qi qs qg, 11 = + nci ncs ncr ncg */
   int32_t microphysics;    /* 0 = none; 5 = Morrison call sites (encoded only, not executable) */

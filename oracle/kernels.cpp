@@ -60,6 +60,27 @@ static int run_program(uint32_t* w, T* v, const char* const* names, const uint32
       for (int64_t q = 0; q < (int64_t)n; ++q) v[dst + q] = v[src + q];
       continue;
     }
+    if (nm == "bc_periodic_xy") {
+      // the library's merged periodic halo launch, checked as what it stands
+      // for: bc_periodic_x then bc_periodic_y on each of its fields, in order
+      const KernelEntry* ex = find_kernel("bc_periodic_x");
+      const KernelEntry* ey = find_kernel("bc_periodic_y");
+      const uint32_t* dm = w + A[4];
+      const uint32_t nx = dm[0], ny = dm[1], nz = dm[2];
+      const uint32_t nf = w[A[4 + 9]];
+      for (uint32_t q = 0; q < nf && q < 8; ++q) {
+        uint32_t Ax[64] = {0}, Ay[64] = {0};
+        Ax[0] = 4; Ax[1] = ny + 7; Ax[2] = nz + 1;
+        Ay[0] = 4; Ay[1] = nx + 7; Ay[2] = nz + 1;
+        Ax[4] = Ay[4] = A[4];
+        Ax[5] = Ay[5] = A[5 + q];
+        Ax[34] = Ay[34] = A[34];
+        K<T> kx{w, v, Ax}, ky{w, v, Ay};
+        if constexpr (sizeof(T) == 4) { ex->f32(kx); ey->f32(ky); }
+        else { ex->f64(kx); ey->f64(ky); }
+      }
+      continue;
+    }
     const KernelEntry* e = find_kernel(nm);
     if (!e) {
       std::snprintf(err, errlen, "oracle: no kernel '%s' (op %d)", names[o], o);

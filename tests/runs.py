@@ -13,8 +13,22 @@ import te_host
 FRONT = 1 << 16  # words below M that a stencil at a region's start may touch
 
 
+def unfused(cfg):
+    """Copy of a te_config that encodes the reference's dispatch list one for one."""
+    c = type(cfg).from_buffer_copy(cfg)
+    c._keep = getattr(cfg, "_keep", None)
+    c.fuse = 0
+    return c
+
+
 class OracleRun:
-    def __init__(self, cfg):
+    """The oracle stepping the reference's dispatch list for `cfg` (fuse=0 unless
+    fused=True, whose merged launches the oracle runs as the dispatches they
+    replace)."""
+
+    def __init__(self, cfg, fused=False):
+        if not fused:
+            cfg = unfused(cfg)
         self.enc = te_host.Run(cfg, encode_only=True)
         self.enc.init_state()
         e = self.enc
@@ -39,6 +53,11 @@ class OracleRun:
     def interior(self, name):
         e = self.enc
         return self.field(name)[: e.nz, 3: 3 + e.ny, 3: 3 + e.nx]
+
+
+def arena_end(e):
+    """Word offset just past the FieldArena (profiles and arena lie below it)."""
+    return e.field_offset(e.fields[0]) + e.field_words * len(e.fields)
 
 
 def first_difference(run, a, b):

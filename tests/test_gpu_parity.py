@@ -52,19 +52,23 @@ CONFIGS = {
 
 @pytest.mark.parametrize("name", sorted(CONFIGS))
 @pytest.mark.parametrize("steps", [1, 3])
-def test_config_bit_exact(gpu, name, steps):
-    cfg = te_host.config(**CONFIGS[name])
-    say("config", name, "steps", steps)
+@pytest.mark.parametrize("fuse", [1, 0])
+def test_config_bit_exact(gpu, name, steps, fuse):
+    """fuse=0 runs the reference's dispatch list (whole memory must match);
+    fuse=1 the library's merged launches (profiles + arena must match)."""
+    cfg = te_host.config(fuse=fuse, **CONFIGS[name])
+    say("config", name, "steps", steps, "fuse", fuse)
     ref = runs.OracleRun(cfg)
     run = te_host.Run(cfg)
     run.init_state()
-    M0 = run.read(0, run.mem_words)
-    assert np.array_equal(M0, ref.M0), runs.first_difference(run, M0, ref.M0)
+    end = runs.arena_end(run) if fuse else run.mem_words
+    M0 = run.read(0, end)
+    assert np.array_equal(M0, ref.M0[:end]), runs.first_difference(run, M0, ref.M0[:end])
     run.step(steps)
     run.sync()
-    got = run.read(0, run.mem_words)
+    got = run.read(0, end)
     ref.step(steps)
-    want = ref.M[: ref.words]
+    want = ref.M[:end]
     assert np.isfinite(ref.interior("th")).all()
     assert np.array_equal(got, want), runs.first_difference(run, got, want)
     run.close()

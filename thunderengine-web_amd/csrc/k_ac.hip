@@ -91,6 +91,12 @@ __global__ void k_acoustic_fpk(uint32_t* M, KArg a) {
 
 // acoustic_w_solve.wgsl:124-212: one lane per (i,j) column, Thomas sweep over
 // k with the elimination coefficients in cA/cB; lanes stay coalesced along i.
+// The sweeps run in chunks of WS_CHUNK levels: every load of a chunk is
+// issued before its serial recurrence, so one wave keeps ~8 levels of loads
+// in flight instead of waiting on each level.  Sums of level k-1 (pk1+qk,
+// pk2-qk, pp3d, pp3d+fpk) carry in registers: they are the same operations on
+// the same values as the reference's re-evaluation at level k.
+#define WS_CHUNK 8
 __global__ void k_acoustic_w_solve(uint32_t* M, KArg a) {
   const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x), j = (int)(blockIdx.y * blockDim.y + threadIdx.y);
   if (i >= (int)a.n[0] || j >= (int)a.n[1]) return;
@@ -100,41 +106,74 @@ __global__ void k_acoustic_w_solve(uint32_t* M, KArg a) {
   const float *pk1 = fp(M, a.p[2]), *pk2 = fp(M, a.p[3]), *qk = fp(M, a.p[4]), *mm = fp(M, a.p[5]),
               *fpk = fp(M, a.p[6]), *wten = fp(M, a.p[7]), *pp3d = fp(M, a.p[8]);
   float *w = fp(M, a.p[9]), *cA = fp(M, a.p[10]), *cB = fp(M, a.p[11]);
+  const int nz = g.nz;
   const int64_t sk = g.sk(), base = g.idx(i, j, 0);
   const float asq = ac.alph * ac.alph;
+  // level 0 values (read before w[0] is cleared: w is not among them)
+  float pq_m = pk1[base] + qk[base], p2q_m = pk2[base] - qk[base], pp_m = pp3d[base];
+  float ppf_m = pp_m + fpk[base];
   w[base] = 0.0f;
-  w[base + g.nz * sk] = 0.0f;
-  float aa, bb, cc, dd;
-  auto coef = [&](int k) {
-    const int64_t c = base + k * sk, cm = c - sk;
-    float m = mm[c];
-    aa = (-asq) * (m * (pk1[cm] + qk[cm]));
-    cc = (-asq) * ((-m) * (pk2[c] - qk[c]));
-    bb = fma_(-asq, m * ((-(pk1[c] + qk[c])) + (pk2[cm] - qk[cm])), 1.0f);
-    float fw = fma_(ac.dts, wten[c], ac.beta * ((-m) * (pp3d[c] - pp3d[cm])));
-    dd = fma_(ac.alph, (-m) * ((pp3d[c] + fpk[c]) - (pp3d[cm] + fpk[cm])), w[c] + fw);
-  };
-  coef(1);
-  float r = 1.0f / bb;
-  float pa = (-cc) * r, pb = dd * r;
-  cA[base + sk] = pa;
-  cB[base + sk] = pb;
-  for (int k = 2; k < g.nz - 1; ++k) {
-    coef(k);
-    r = 1.0f / fma_(aa, pa, bb);
-    pa = (-cc) * r;
-    pb = fma_(-aa, pb, dd) * r;
-    cA[base + k * sk] = pa;
-    cB[base + k * sk] = pb;
+  w[base + nz * sk] = 0.0f;
+  float pa = 0.0f, pb = 0.0f;
+  for (int k0 = 1; k0 <= nz - 1; k0 += WS_CHUNK) {
+    float m[WS_CHUNK], a1[WS_CHUNK], q[WS_CHUNK], a2[WS_CHUNK], wt[WS_CHUNK], p[WS_CHUNK], f[WS_CHUNK], wc[WS_CHUNK];
+#pragma unroll
+    for (int t = 0; t < WS_CHUNK; ++t) {
+      const int k = k0 + t;
+      if (k <= nz - 1) {
+        const int64_t c = base + k * sk;
+        m[t] = mm[c]; a1[t] = pk1[c]; q[t] = qk[c]; a2[t] = pk2[c];
+        wt[t] = wten[c]; p[t] = pp3d[c]; f[t] = fpk[c]; wc[t] = w[c];
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < WS_CHUNK; ++t) {
+      const int k = k0 + t;
+      if (k <= nz - 1) {
+        const int64_t c = base + k * sk;
+        const float pq = a1[t] + q[t], p2q = a2[t] - q[t], ppf = p[t] + f[t];
+        const float aa = (-asq) * (m[t] * pq_m);
+        const float cc = (-asq) * ((-m[t]) * p2q);
+        const float bb = fma_(-asq, m[t] * ((-pq) + p2q_m), 1.0f);
+        const float fw = fma_(ac.dts, wt[t], ac.beta * ((-m[t]) * (p[t] - pp_m)));
+        const float dd = fma_(ac.alph, (-m[t]) * (ppf - ppf_m), wc[t] + fw);
+        if (k == 1) {
+          const float r = 1.0f / bb;
+          pa = (-cc) * r;
+          pb = dd * r;
+          cA[c] = pa;
+          cB[c] = pb;
+        } else if (k < nz - 1) {
+          const float r = 1.0f / fma_(aa, pa, bb);
+          pa = (-cc) * r;
+          pb = fma_(-aa, pb, dd) * r;
+          cA[c] = pa;
+          cB[c] = pb;
+        } else {
+          const float r = 1.0f / fma_(aa, pa, bb);
+          pb = fma_(-aa, pb, dd) * r;  // w at the top interior level
+          w[c] = pb;
+        }
+        pq_m = pq; p2q_m = p2q; pp_m = p[t]; ppf_m = ppf;
+      }
+    }
   }
-  coef(g.nz - 1);
-  r = 1.0f / fma_(aa, pa, bb);
-  float wn = fma_(-aa, pb, dd) * r;
-  w[base + (g.nz - 1) * sk] = wn;
-  for (int k = g.nz - 2; k >= 1; --k) {
-    const int64_t c = base + k * sk;
-    wn = fma_(cA[c], wn, cB[c]);
-    w[c] = wn;
+  float wn = pb;
+  for (int k0 = nz - 2; k0 >= 1; k0 -= WS_CHUNK) {
+    float ca[WS_CHUNK], cb[WS_CHUNK];
+#pragma unroll
+    for (int t = 0; t < WS_CHUNK; ++t) {
+      const int k = k0 - t;
+      if (k >= 1) { ca[t] = cA[base + k * sk]; cb[t] = cB[base + k * sk]; }
+    }
+#pragma unroll
+    for (int t = 0; t < WS_CHUNK; ++t) {
+      const int k = k0 - t;
+      if (k >= 1) {
+        wn = fma_(ca[t], wn, cb[t]);
+        w[base + k * sk] = wn;
+      }
+    }
   }
 }
 

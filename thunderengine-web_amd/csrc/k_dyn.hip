@@ -74,7 +74,9 @@ __global__ void k_stage_state_reset(uint32_t* M, KArg a) {
   int i = gx - 3, j = gy - 3, k = gz;
   if (i > g.nx + 3 || j > g.ny + 3 || k > g.nz) return;
   int64_t c = g.idx(i, j, k);
-  for (int q = 0; q < 5; ++q) fp(M, a.p[5 + q])[c] = fp(M, a.p[1 + q])[c];
+  // u <- u_a, v <- v_a, w <- w_a, pp3d <- ppi, ppd <- ppx
+  const int src[5] = {1, 2, 3, 4, 10};
+  for (int q = 0; q < 5; ++q) fp(M, a.p[5 + q])[c] = fp(M, a.p[src[q]])[c];
 }
 
 // massflux_from_state.wgsl:111-137

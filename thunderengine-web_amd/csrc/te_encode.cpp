@@ -108,7 +108,31 @@ class Encoder {
   }
   Bytes acoustic;  // current stage's acoustic parameter bytes
 
+  // Periodic halo fills queue here when both directions are periodic and
+  // fusion is on; they are written by one bc_periodic_xy launch (up to 8
+  // fields) just before the next op that is not a halo fill.
+  bool mergeHalo = false;
+  std::vector<std::string> pendingHalo;
+  void flushHalo() {
+    std::vector<std::string> q;
+    q.swap(pendingHalo);
+    for (size_t s = 0; s < q.size(); s += 8) {
+      const int n = (int)std::min<size_t>(8, q.size() - s);
+      Op o;
+      std::memset(&o, 0, sizeof(o));
+      o.kid = kernel_id("bc_periodic_xy");
+      o.A[0] = (uint32_t)(7 * (cfg.nx + 7) + 7 * cfg.ny); o.A[1] = (uint32_t)(cfg.nz + 1); o.A[2] = 1;
+      o.A[4 + 0] = (uint32_t)dimsOff;
+      o.A[4 + 30] = (uint32_t)hmOff;
+      for (int q2 = 0; q2 < n; ++q2) o.A[4 + 1 + q2] = (uint32_t)off(q[s + q2]);
+      o.A[4 + 9] = (uint32_t)bytes(I1(n));
+      E.ops.push_back(o);
+      E.names.push_back("bc_periodic_xy");
+    }
+  }
+
   void op(const char* kname, std::array<int, 3> grid, std::initializer_list<Bind> binds, bool withAcoustic = false) {
+    if (!pendingHalo.empty()) flushHalo();
     Op o;
     std::memset(&o, 0, sizeof(o));
     o.kid = kernel_id(kname);
@@ -166,6 +190,7 @@ void Encoder::setup() {
   rayleigh = cfg.rayleigh != 0;
   openX = cfg.open_x != 0; openY = cfg.open_y != 0;
   periodicX = !openX; periodicY = !openY;
+  mergeHalo = cfg.fuse != 0 && periodicX && periodicY;
   wenoS = cfg.weno_scalars != 0;
   wenoM = cfg.weno_momentum != 0;
   explicitZ = cfg.explicit_vdiff != 0;
@@ -268,6 +293,10 @@ void Encoder::openHalo(const std::string& n) {
 
 void Encoder::haloFill(const std::string& n) {
   const int H = 3;
+  if (mergeHalo) {
+    if (std::find(pendingHalo.begin(), pendingHalo.end(), n) == pendingHalo.end()) pendingHalo.push_back(n);
+    return;
+  }
   if (periodicX) op("bc_periodic_x", {H + 1, cfg.ny + 1 + 2 * H, cfg.nz + 1}, {fb(1, n)});
   if (periodicY) op("bc_periodic_y", {H + 1, cfg.nx + 1 + 2 * H, cfg.nz + 1}, {fb(1, n)});
   openHalo(n);
@@ -563,13 +592,21 @@ void Encoder::stages() {
          {fb(2, "w"), fb(3, "pk1"), fb(4, "pk2"), fb(5, "qk"), fb(6, "fpk"), fb(7, "pp3d"), fb(8, "ppd"), fb(9, "ppx"),
           bb(10, I1(nrk == 2 ? 1 : 0))},
          true);
-      haloFill("ppd");
       const float tavg = n == nloop - 1 ? (float)(1.0 / nloop) : 0.0f;
-      op("acoustic_accumulate", whole,
-         {fb(2, "u"), fb(3, "v"), fb(4, "w"), fb(5, "rru"), fb(6, "rrv"), fb(7, "rrw"), bb(8, F1(tavg))}, true);
-      haloFill("u");
-      haloFill("v");
-      haloFill("w");
+      auto accumulate = [&] {
+        op("acoustic_accumulate", whole,
+           {fb(2, "u"), fb(3, "v"), fb(4, "w"), fb(5, "rru"), fb(6, "rrv"), fb(7, "rrw"), bb(8, F1(tavg))}, true);
+      };
+      if (mergeHalo) {  // accumulate reads no ppd: the four fills become one launch
+        accumulate();
+        for (const char* h : {"ppd", "u", "v", "w"}) haloFill(h);
+      } else {
+        haloFill("ppd");
+        accumulate();
+        haloFill("u");
+        haloFill("v");
+        haloFill("w");
+      }
     }
     haloFill("pp3d");
     if (openX)
@@ -627,6 +664,7 @@ void Encoder::end() {
         fb(8, "qvten"), fb(9, "qcten")});
   op("ppx_finish", full, {fb(1, "pp3d"), fb(2, "ppx")});
   haloFill("ppx");
+  flushHalo();
   // state swap (timestep.js:766-774)
   const uint64_t count = (uint64_t)E.L.sJ * E.L.sK * (nz + 1);
   std::vector<std::pair<std::string, std::string>> pairs;
@@ -702,4 +740,5 @@ extern "C" void te_config_defaults(te_config* c) {
   c->hodo_radius = 10.0f; c->hodo_depth = 2000.0f; c->shear_umax = 30.0f; c->shear_depth = 6000.0f;
   c->bubble_amp = 2.0f; c->bubble_rh = 10000.0f; c->bubble_rv = 1400.0f; c->bubble_zc = 1400.0f;
   c->noise_sigma = 0.01f; c->seed = 20260101ull;
+  c->fuse = 1;
 }

@@ -4,6 +4,7 @@
 // word addressed, one hipMalloc) and launches the program's kernels in order on
 // one HIP stream.  A full step is captured once into a HIP graph and replayed.
 #include <algorithm>
+#include <array>
 #include <cstdio>
 #include <cstring>
 
@@ -153,10 +154,17 @@ static int ctx_alloc(te_ctx* c, uint64_t words) {
 static int ensure_guard(te_ctx* c) {
   if (c->guard_ok) return TE_OK;
   uint64_t need = 0;
+  std::map<uint32_t, std::array<uint32_t, 5>> seen;  // grid parameter blocks already read
   for (const Op& op : c->ops) {
     if (std::strcmp(te::kernel_name(op.kid), "_copy") == 0) continue;
     uint32_t d[5];
-    TE_HIP(hipMemcpy(d, c->dM + op.A[4], sizeof(d), hipMemcpyDeviceToHost), "read grid");
+    auto it = seen.find(op.A[4]);
+    if (it != seen.end()) {
+      std::memcpy(d, it->second.data(), sizeof(d));
+    } else {
+      TE_HIP(hipMemcpy(d, c->dM + op.A[4], sizeof(d), hipMemcpyDeviceToHost), "read grid");
+      seen[op.A[4]] = {d[0], d[1], d[2], d[3], d[4]};
+    }
     const int32_t nz = (int32_t)d[2], sJ = (int32_t)d[3], sK = (int32_t)d[4];
     if (nz < 1 || sJ < 1 || sK < 1 || nz > 100000 || sJ > 100000 || sK > 100000)
       return fail(c, TE_ERR_ARG, std::string("op ") + te::kernel_name(op.kid) + ": invalid grid parameters");

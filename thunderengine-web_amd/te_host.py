@@ -45,6 +45,7 @@ class Config(ctypes.Structure):
         ("bubble_amp", ctypes.c_float), ("bubble_rh", ctypes.c_float), ("bubble_rv", ctypes.c_float),
         ("bubble_zc", ctypes.c_float),
         ("noise_sigma", ctypes.c_float), ("seed", ctypes.c_uint64), ("initial_tke", ctypes.c_int32),
+        ("fuse", ctypes.c_int32),
         ("species", ctypes.c_int32), ("microphysics", ctypes.c_int32), ("mass_conservation", ctypes.c_int32),
         ("cache_moisture_turbulence", ctypes.c_int32), ("n_arena_fields", ctypes.c_int32),
         ("arena_fields", ctypes.POINTER(ctypes.c_char_p)),
