@@ -7,7 +7,7 @@ namespace te {
 
 // acoustic_prepare.wgsl:130-184 (FMA operand order switches with
 // vec_v = nx>3 && i<(nx/4)*4, as in the reference)
-__global__ void k_acoustic_prepare(uint32_t* M, KArg a) {
+__global__ __launch_bounds__(256) void k_acoustic_prepare(uint32_t* M, KArg a) {
   TE_TID3
   Dims g = dims(M, a.p[0]);
   int i = gx, j = gy, k = gz;
@@ -35,7 +35,7 @@ __global__ void k_acoustic_prepare(uint32_t* M, KArg a) {
 }
 
 // acoustic_begin.wgsl:128-149
-__global__ void k_acoustic_begin(uint32_t* M, KArg a) {
+__global__ __launch_bounds__(256) void k_acoustic_begin(uint32_t* M, KArg a) {
   TE_TID3
   Dims g = dims(M, a.p[0]);
   int i = gx - 3, j = gy - 3, k = gz;
@@ -48,7 +48,7 @@ __global__ void k_acoustic_begin(uint32_t* M, KArg a) {
 }
 
 // acoustic_uv.wgsl:144-181
-__global__ void k_acoustic_uv(uint32_t* M, KArg a) {
+__global__ __launch_bounds__(256) void k_acoustic_uv(uint32_t* M, KArg a) {
   TE_TID3
   Dims g = dims(M, a.p[0]);
   int i = gx, j = gy, k = gz;
@@ -68,7 +68,7 @@ __global__ void k_acoustic_uv(uint32_t* M, KArg a) {
 }
 
 // acoustic_fpk.wgsl:167-207
-__global__ void k_acoustic_fpk(uint32_t* M, KArg a) {
+__global__ __launch_bounds__(256) void k_acoustic_fpk(uint32_t* M, KArg a) {
   TE_TID3
   Dims g = dims(M, a.p[0]);
   int i = gx, j = gy, k = gz;
@@ -97,7 +97,7 @@ __global__ void k_acoustic_fpk(uint32_t* M, KArg a) {
 // pk2-qk, pp3d, pp3d+fpk) carry in registers: they are the same operations on
 // the same values as the reference's re-evaluation at level k.
 #define WS_CHUNK 8
-__global__ void k_acoustic_w_solve(uint32_t* M, KArg a) {
+__global__ __launch_bounds__(256) void k_acoustic_w_solve(uint32_t* M, KArg a) {
   const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x), j = (int)(blockIdx.y * blockDim.y + threadIdx.y);
   if (i >= (int)a.n[0] || j >= (int)a.n[1]) return;
   Dims g = dims(M, a.p[0]);
@@ -178,7 +178,7 @@ __global__ void k_acoustic_w_solve(uint32_t* M, KArg a) {
 }
 
 // acoustic_pp.wgsl:131-167
-__global__ void k_acoustic_pp(uint32_t* M, KArg a) {
+__global__ __launch_bounds__(256) void k_acoustic_pp(uint32_t* M, KArg a) {
   TE_TID3
   Dims g = dims(M, a.p[0]);
   int i = gx, j = gy, k = gz;
@@ -198,7 +198,7 @@ __global__ void k_acoustic_pp(uint32_t* M, KArg a) {
 }
 
 // acoustic_accumulate.wgsl:129-158
-__global__ void k_acoustic_accumulate(uint32_t* M, KArg a) {
+__global__ __launch_bounds__(256) void k_acoustic_accumulate(uint32_t* M, KArg a) {
   TE_TID3
   Dims g = dims(M, a.p[0]);
   int i = gx - 3, j = gy - 3, k = gz;
@@ -234,7 +234,7 @@ TE_DEV bool pdef_inflow(const Dims& g, int dir, int i, int j, int64_t c, int64_t
 }
 
 // pdef_dum.wgsl:177-231
-__global__ void k_pdef_dum(uint32_t* M, KArg a) {
+__global__ __launch_bounds__(256) void k_pdef_dum(uint32_t* M, KArg a) {
   TE_TID3
   Dims g = dims(M, a.p[0]);
   int i = gx, j = gy, k = gz;
@@ -281,7 +281,7 @@ TE_DEV float pdef_mass(float dm2, float dm1, float d0, float dp1, float dp2, boo
 }
 
 // pdef_apply.wgsl:207-282
-__global__ void k_pdef_apply(uint32_t* M, KArg a) {
+__global__ __launch_bounds__(256) void k_pdef_apply(uint32_t* M, KArg a) {
   TE_TID3
   Dims g = dims(M, a.p[0]);
   int i = gx, j = gy, k = gz;

@@ -17,11 +17,11 @@ __device__ void flux_h5(uint32_t* M, const KArg& a, int i, int j, int k) {
   else f = flx5(s[c + 2 * st], s[c + st], s[c], s[c - st], s[c - 2 * st]);
   fp(M, a.p[3])[c] = u * f;
 }
-__global__ void k_adv_flux_x(uint32_t* M, KArg a) { TE_TID3 flux_h5<0>(M, a, gx, gy, gz); }
-__global__ void k_adv_flux_y(uint32_t* M, KArg a) { TE_TID3 flux_h5<1>(M, a, gx, gy, gz); }
+__global__ __launch_bounds__(256) void k_adv_flux_x(uint32_t* M, KArg a) { TE_TID3 flux_h5<0>(M, a, gx, gy, gz); }
+__global__ __launch_bounds__(256) void k_adv_flux_y(uint32_t* M, KArg a) { TE_TID3 flux_h5<1>(M, a, gx, gy, gz); }
 
 // adv_flux_z.wgsl:127-200
-__global__ void k_adv_flux_z(uint32_t* M, KArg a) {
+__global__ __launch_bounds__(256) void k_adv_flux_z(uint32_t* M, KArg a) {
   TE_TID3
   Dims g = dims(M, a.p[0]);
   int i = gx, j = gy, k = gz;
@@ -54,11 +54,11 @@ __device__ void flux_h_weno(uint32_t* M, const KArg& a, int i, int j, int k) {
   else { s1 = s[c + 2 * st]; s2 = s[c + st]; s3 = s[c]; s4 = s[c - st]; s5 = s[c - 2 * st]; }
   fp(M, a.p[5])[c] = weno_pdef_ok(s1, s2, s3, s4, s5, weps, pdef) ? rr[c] * weno5(s1, s2, s3, s4, s5, weps) : 0.0f;
 }
-__global__ void k_adv_flux_x_weno(uint32_t* M, KArg a) { TE_TID3 flux_h_weno<0>(M, a, gx, gy, gz); }
-__global__ void k_adv_flux_y_weno(uint32_t* M, KArg a) { TE_TID3 flux_h_weno<1>(M, a, gx, gy, gz); }
+__global__ __launch_bounds__(256) void k_adv_flux_x_weno(uint32_t* M, KArg a) { TE_TID3 flux_h_weno<0>(M, a, gx, gy, gz); }
+__global__ __launch_bounds__(256) void k_adv_flux_y_weno(uint32_t* M, KArg a) { TE_TID3 flux_h_weno<1>(M, a, gx, gy, gz); }
 
 // adv_flux_z_weno.wgsl:174-279
-__global__ void k_adv_flux_z_weno(uint32_t* M, KArg a) {
+__global__ __launch_bounds__(256) void k_adv_flux_z_weno(uint32_t* M, KArg a) {
   TE_TID3
   Dims g = dims(M, a.p[0]);
   int i = gx, j = gy, kf = gz + 1;
@@ -135,11 +135,11 @@ __device__ void flux_h_stag(uint32_t* M, const KArg& a, int i, int j, int k) {
   else m = rr[c];
   fl[c] = m * flx5_upwind(s, c, st, m >= 0.0f, stag == (DIR == 0 ? 2 : 3));
 }
-__global__ void k_adv_flux_x_stag(uint32_t* M, KArg a) { TE_TID3 flux_h_stag<0>(M, a, gx - 1, gy - 1, gz); }
-__global__ void k_adv_flux_y_stag(uint32_t* M, KArg a) { TE_TID3 flux_h_stag<1>(M, a, gx - 1, gy - 1, gz); }
+__global__ __launch_bounds__(256) void k_adv_flux_x_stag(uint32_t* M, KArg a) { TE_TID3 flux_h_stag<0>(M, a, gx - 1, gy - 1, gz); }
+__global__ __launch_bounds__(256) void k_adv_flux_y_stag(uint32_t* M, KArg a) { TE_TID3 flux_h_stag<1>(M, a, gx - 1, gy - 1, gz); }
 
 // adv_flux_z_stag.wgsl:127-268
-__global__ void k_adv_flux_z_stag(uint32_t* M, KArg a) {
+__global__ __launch_bounds__(256) void k_adv_flux_z_stag(uint32_t* M, KArg a) {
   TE_TID3
   Dims g = dims(M, a.p[0]);
   int i = gx, j = gy, k = gz;
@@ -189,11 +189,11 @@ __device__ void flux_h_weno_w(uint32_t* M, const KArg& a, int i, int j, int k) {
   if (ub >= 0.0f) fl[c] = ub * weno5(s[c - 3 * st], s[c - 2 * st], s[c - st], s[c], s[c + st], weps);
   else fl[c] = ub * weno5(s[c + 2 * st], s[c + st], s[c], s[c - st], s[c - 2 * st], weps);
 }
-__global__ void k_adv_flux_x_weno_w(uint32_t* M, KArg a) { TE_TID3 flux_h_weno_w<0>(M, a, gx - 1, gy - 1, gz); }
-__global__ void k_adv_flux_y_weno_w(uint32_t* M, KArg a) { TE_TID3 flux_h_weno_w<1>(M, a, gx - 1, gy - 1, gz); }
+__global__ __launch_bounds__(256) void k_adv_flux_x_weno_w(uint32_t* M, KArg a) { TE_TID3 flux_h_weno_w<0>(M, a, gx - 1, gy - 1, gz); }
+__global__ __launch_bounds__(256) void k_adv_flux_y_weno_w(uint32_t* M, KArg a) { TE_TID3 flux_h_weno_w<1>(M, a, gx - 1, gy - 1, gz); }
 
 // adv_flux_z_weno_w.wgsl:159-231
-__global__ void k_adv_flux_z_weno_w(uint32_t* M, KArg a) {
+__global__ __launch_bounds__(256) void k_adv_flux_z_weno_w(uint32_t* M, KArg a) {
   TE_TID3
   Dims g = dims(M, a.p[0]);
   int i = gx, j = gy, k = gz;
@@ -230,11 +230,11 @@ __device__ void flux_h_weno_uv(uint32_t* M, const KArg& a, int i, int j, int k) 
   else m = (stag == 2) ? 0.5f * (rr[c] + rr[c - 1]) : 0.5f * (rr[c] + rr[c + sj]);
   fp(M, a.p[5])[c] = m * weno5_upwind(s, c, st, m >= 0.0f, stag == (DIR == 0 ? 2 : 3), weps);
 }
-__global__ void k_adv_flux_x_weno_uv(uint32_t* M, KArg a) { TE_TID3 flux_h_weno_uv<0>(M, a, gx - 1, gy - 1, gz); }
-__global__ void k_adv_flux_y_weno_uv(uint32_t* M, KArg a) { TE_TID3 flux_h_weno_uv<1>(M, a, gx - 1, gy - 1, gz); }
+__global__ __launch_bounds__(256) void k_adv_flux_x_weno_uv(uint32_t* M, KArg a) { TE_TID3 flux_h_weno_uv<0>(M, a, gx - 1, gy - 1, gz); }
+__global__ __launch_bounds__(256) void k_adv_flux_y_weno_uv(uint32_t* M, KArg a) { TE_TID3 flux_h_weno_uv<1>(M, a, gx - 1, gy - 1, gz); }
 
 // adv_flux_z_weno_uv.wgsl:159-236
-__global__ void k_adv_flux_z_weno_uv(uint32_t* M, KArg a) {
+__global__ __launch_bounds__(256) void k_adv_flux_z_weno_uv(uint32_t* M, KArg a) {
   TE_TID3
   Dims g = dims(M, a.p[0]);
   int i = gx, j = gy, k = gz;
@@ -260,7 +260,7 @@ __global__ void k_adv_flux_z_weno_uv(uint32_t* M, KArg a) {
 
 // adv_assemble.wgsl:139-279: flux divergence + mass-divergence term, divided
 // by the reference density; open-boundary inflow corrections included.
-__global__ void k_adv_assemble(uint32_t* M, KArg a) {
+__global__ __launch_bounds__(256) void k_adv_assemble(uint32_t* M, KArg a) {
   TE_TID3
   Dims g = dims(M, a.p[0]);
   int i = gx, j = gy, k = gz;

@@ -18,8 +18,8 @@ __device__ void periodic(uint32_t* M, const KArg& a, int h, int t, int k) {
   if (h < 3) f[at(-1 - h)] = f[at(lo)];
   f[at(n + h)] = f[at(hi)];
 }
-__global__ void k_bc_periodic_x(uint32_t* M, KArg a) { TE_TID3 periodic<0>(M, a, gx, gy - 3, gz); }
-__global__ void k_bc_periodic_y(uint32_t* M, KArg a) { TE_TID3 periodic<1>(M, a, gx, gy - 3, gz); }
+__global__ __launch_bounds__(256) void k_bc_periodic_x(uint32_t* M, KArg a) { TE_TID3 periodic<0>(M, a, gx, gy - 3, gz); }
+__global__ __launch_bounds__(256) void k_bc_periodic_y(uint32_t* M, KArg a) { TE_TID3 periodic<1>(M, a, gx, gy - 3, gz); }
 
 // bc_open_x.wgsl:111-131 / bc_open_y.wgsl:111-131 (zero-gradient halo)
 template <int DIR>
@@ -33,11 +33,11 @@ __device__ void open_halo(uint32_t* M, const KArg& a, int h, int t, int k) {
   f[at(-1 - h)] = f[at(0)];
   f[at(last + 1 + h)] = f[at(last)];
 }
-__global__ void k_bc_open_x(uint32_t* M, KArg a) { TE_TID3 open_halo<0>(M, a, gx, gy - 3, gz); }
-__global__ void k_bc_open_y(uint32_t* M, KArg a) { TE_TID3 open_halo<1>(M, a, gx, gy - 3, gz); }
+__global__ __launch_bounds__(256) void k_bc_open_x(uint32_t* M, KArg a) { TE_TID3 open_halo<0>(M, a, gx, gy - 3, gz); }
+__global__ __launch_bounds__(256) void k_bc_open_y(uint32_t* M, KArg a) { TE_TID3 open_halo<1>(M, a, gx, gy - 3, gz); }
 
 // openbc_uv_tendency.wgsl:167-208
-__global__ void k_openbc_uv_tendency(uint32_t* M, KArg a) {
+__global__ __launch_bounds__(256) void k_openbc_uv_tendency(uint32_t* M, KArg a) {
   TE_TID3
   Dims g = dims(M, a.p[0]);
   const int n = gx, k = gy;
@@ -89,11 +89,11 @@ __device__ void radbc(uint32_t* M, const KArg& a, int t) {
   lo[t] = avg1 / (float)g.nz;
   hi[t] = avg2 / (float)g.nz;
 }
-__global__ void k_radbc_ew(uint32_t* M, KArg a) { TE_TID3 radbc<0>(M, a, gx); }
-__global__ void k_radbc_ns(uint32_t* M, KArg a) { TE_TID3 radbc<1>(M, a, gx); }
+__global__ __launch_bounds__(256) void k_radbc_ew(uint32_t* M, KArg a) { TE_TID3 radbc<0>(M, a, gx); }
+__global__ __launch_bounds__(256) void k_radbc_ns(uint32_t* M, KArg a) { TE_TID3 radbc<1>(M, a, gx); }
 
 // turb_openbc_flux.wgsl:111-140
-__global__ void k_turb_openbc_flux(uint32_t* M, KArg a) {
+__global__ __launch_bounds__(256) void k_turb_openbc_flux(uint32_t* M, KArg a) {
   TE_TID3
   Dims g = dims(M, a.p[0]);
   const int n = gx, k = gy;
@@ -125,11 +125,11 @@ __device__ void openbc_stress(uint32_t* M, const KArg& a, int t, int k) {
     s12[at(nq)] = s12[at(nq - 1)];
   }
 }
-__global__ void k_turb_openbc_stress_x(uint32_t* M, KArg a) { TE_TID3 openbc_stress<0>(M, a, gx - 1, gy); }
-__global__ void k_turb_openbc_stress_y(uint32_t* M, KArg a) { TE_TID3 openbc_stress<1>(M, a, gx - 1, gy); }
+__global__ __launch_bounds__(256) void k_turb_openbc_stress_x(uint32_t* M, KArg a) { TE_TID3 openbc_stress<0>(M, a, gx - 1, gy); }
+__global__ __launch_bounds__(256) void k_turb_openbc_stress_y(uint32_t* M, KArg a) { TE_TID3 openbc_stress<1>(M, a, gx - 1, gy); }
 
 // copy_words (replay.js:403-406): end-of-step state swap copies.
-__global__ void k__copy(uint32_t* M, KArg a) {
+__global__ __launch_bounds__(256) void k__copy(uint32_t* M, KArg a) {
   const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= a.n[0]) return;
   M[a.p[1] + q] = M[a.p[0] + q];

@@ -8,7 +8,7 @@
 namespace te {
 
 // update_diagnostics_moist.wgsl:128-184
-__global__ void k_update_diagnostics_moist(uint32_t* M, KArg a) {
+__global__ __launch_bounds__(256) void k_update_diagnostics_moist(uint32_t* M, KArg a) {
   TE_TID3
   Dims g = dims(M, a.p[0]);
   int i = gx - 3, j = gy - 3, k = gz;
@@ -38,7 +38,7 @@ __global__ void k_update_diagnostics_moist(uint32_t* M, KArg a) {
 }
 
 // update_diagnostics.wgsl:114-144 (dry)
-__global__ void k_update_diagnostics(uint32_t* M, KArg a) {
+__global__ __launch_bounds__(256) void k_update_diagnostics(uint32_t* M, KArg a) {
   TE_TID3
   Dims g = dims(M, a.p[0]);
   int i = gx - 3, j = gy - 3, k = gz;
@@ -56,7 +56,7 @@ __global__ void k_update_diagnostics(uint32_t* M, KArg a) {
 }
 
 // stage_tendency_reset.wgsl:110-129
-__global__ void k_stage_tendency_reset(uint32_t* M, KArg a) {
+__global__ __launch_bounds__(256) void k_stage_tendency_reset(uint32_t* M, KArg a) {
   TE_TID3
   Dims g = dims(M, a.p[0]);
   int i = gx - 3, j = gy - 3, k = gz;
@@ -68,7 +68,7 @@ __global__ void k_stage_tendency_reset(uint32_t* M, KArg a) {
 }
 
 // stage_state_reset.wgsl:110-138
-__global__ void k_stage_state_reset(uint32_t* M, KArg a) {
+__global__ __launch_bounds__(256) void k_stage_state_reset(uint32_t* M, KArg a) {
   TE_TID3
   Dims g = dims(M, a.p[0]);
   int i = gx - 3, j = gy - 3, k = gz;
@@ -80,7 +80,7 @@ __global__ void k_stage_state_reset(uint32_t* M, KArg a) {
 }
 
 // massflux_from_state.wgsl:111-137
-__global__ void k_massflux_from_state(uint32_t* M, KArg a) {
+__global__ __launch_bounds__(256) void k_massflux_from_state(uint32_t* M, KArg a) {
   TE_TID3
   Dims g = dims(M, a.p[0]);
   int i = gx - 3, j = gy - 3, k = gz;
@@ -96,7 +96,7 @@ __global__ void k_massflux_from_state(uint32_t* M, KArg a) {
 }
 
 // velocity_to_massflux.wgsl:111-134
-__global__ void k_velocity_to_massflux(uint32_t* M, KArg a) {
+__global__ __launch_bounds__(256) void k_velocity_to_massflux(uint32_t* M, KArg a) {
   TE_TID3
   Dims g = dims(M, a.p[0]);
   int i = gx - 3, j = gy - 3, k = gz;
@@ -112,7 +112,7 @@ __global__ void k_velocity_to_massflux(uint32_t* M, KArg a) {
 }
 
 // div_massflux.wgsl:125-147
-__global__ void k_div_massflux(uint32_t* M, KArg a) {
+__global__ __launch_bounds__(256) void k_div_massflux(uint32_t* M, KArg a) {
   TE_TID3
   Dims g = dims(M, a.p[0]);
   int i = gx - 3, j = gy - 3, k = gz;
@@ -126,7 +126,7 @@ __global__ void k_div_massflux(uint32_t* M, KArg a) {
 }
 
 // divergence3d.wgsl:125-147
-__global__ void k_divergence3d(uint32_t* M, KArg a) {
+__global__ __launch_bounds__(256) void k_divergence3d(uint32_t* M, KArg a) {
   TE_TID3
   Dims g = dims(M, a.p[0]);
   int i = gx, j = gy, k = gz;
@@ -140,7 +140,7 @@ __global__ void k_divergence3d(uint32_t* M, KArg a) {
 }
 
 // buoyancy_apply.wgsl:111-129
-__global__ void k_buoyancy_apply(uint32_t* M, KArg a) {
+__global__ __launch_bounds__(256) void k_buoyancy_apply(uint32_t* M, KArg a) {
   TE_TID3
   Dims g = dims(M, a.p[0]);
   int i = gx, j = gy, k = gz + 1;
@@ -152,7 +152,7 @@ __global__ void k_buoyancy_apply(uint32_t* M, KArg a) {
 }
 
 // rk_update.wgsl:111-132
-__global__ void k_rk_update(uint32_t* M, KArg a) {
+__global__ __launch_bounds__(256) void k_rk_update(uint32_t* M, KArg a) {
   TE_TID3
   Dims g = dims(M, a.p[0]);
   int stag = ldi(M, a.p[1]);
@@ -167,7 +167,7 @@ __global__ void k_rk_update(uint32_t* M, KArg a) {
 }
 
 // build_sadv.wgsl:111-127
-__global__ void k_build_sadv(uint32_t* M, KArg a) {
+__global__ __launch_bounds__(256) void k_build_sadv(uint32_t* M, KArg a) {
   TE_TID3
   Dims g = dims(M, a.p[0]);
   int i = gx - 3, j = gy - 3, k = gz;
@@ -178,7 +178,7 @@ __global__ void k_build_sadv(uint32_t* M, KArg a) {
 }
 
 // apply_thterm.wgsl:111-127
-__global__ void k_apply_thterm(uint32_t* M, KArg a) {
+__global__ __launch_bounds__(256) void k_apply_thterm(uint32_t* M, KArg a) {
   TE_TID3
   Dims g = dims(M, a.p[0]);
   if (gx >= g.nx || gy >= g.ny || gz >= g.nz) return;
@@ -189,7 +189,7 @@ __global__ void k_apply_thterm(uint32_t* M, KArg a) {
 }
 
 // turb_add_tendency.wgsl:111-127
-__global__ void k_turb_add_tendency(uint32_t* M, KArg a) {
+__global__ __launch_bounds__(256) void k_turb_add_tendency(uint32_t* M, KArg a) {
   TE_TID3
   Dims g = dims(M, a.p[0]);
   int i = gx - 3, j = gy - 3, k = gz;
@@ -202,7 +202,7 @@ __global__ void k_turb_add_tendency(uint32_t* M, KArg a) {
 }
 
 // turb_zero.wgsl:103-116
-__global__ void k_turb_zero(uint32_t* M, KArg a) {
+__global__ __launch_bounds__(256) void k_turb_zero(uint32_t* M, KArg a) {
   TE_TID3
   Dims g = dims(M, a.p[0]);
   if (gx >= g.sJ || gy >= g.sK || gz > g.nz + 1) return;
@@ -210,7 +210,7 @@ __global__ void k_turb_zero(uint32_t* M, KArg a) {
 }
 
 // tke_advance.wgsl:111-129
-__global__ void k_tke_advance(uint32_t* M, KArg a) {
+__global__ __launch_bounds__(256) void k_tke_advance(uint32_t* M, KArg a) {
   TE_TID3
   Dims g = dims(M, a.p[0]);
   int i = gx, j = gy, k = gz + 1;
@@ -223,7 +223,7 @@ __global__ void k_tke_advance(uint32_t* M, KArg a) {
 }
 
 // ppx_finish.wgsl:111-126
-__global__ void k_ppx_finish(uint32_t* M, KArg a) {
+__global__ __launch_bounds__(256) void k_ppx_finish(uint32_t* M, KArg a) {
   TE_TID3
   Dims g = dims(M, a.p[0]);
   if (gx >= g.nx || gy >= g.ny || gz >= g.nz) return;
@@ -233,7 +233,7 @@ __global__ void k_ppx_finish(uint32_t* M, KArg a) {
 }
 
 // diabatic_unapply.wgsl:111-140
-__global__ void k_diabatic_unapply(uint32_t* M, KArg a) {
+__global__ __launch_bounds__(256) void k_diabatic_unapply(uint32_t* M, KArg a) {
   TE_TID3
   Dims g = dims(M, a.p[0]);
   if (gx >= g.nx || gy >= g.ny || gz >= g.nz) return;
@@ -248,7 +248,7 @@ __global__ void k_diabatic_unapply(uint32_t* M, KArg a) {
 }
 
 // diabatic_rates.wgsl:111-137
-__global__ void k_diabatic_rates(uint32_t* M, KArg a) {
+__global__ __launch_bounds__(256) void k_diabatic_rates(uint32_t* M, KArg a) {
   TE_TID3
   Dims g = dims(M, a.p[0]);
   if (gx >= g.nx || gy >= g.ny || gz >= g.nz) return;
@@ -263,7 +263,7 @@ __global__ void k_diabatic_rates(uint32_t* M, KArg a) {
 }
 
 // pressure_tendency.wgsl:234-291
-__global__ void k_pressure_tendency(uint32_t* M, KArg a) {
+__global__ __launch_bounds__(256) void k_pressure_tendency(uint32_t* M, KArg a) {
   TE_TID3
   Dims g = dims(M, a.p[0]);
   if (gx >= g.nx || gy >= g.ny || gz >= g.nz) return;
@@ -301,7 +301,7 @@ __global__ void k_pressure_tendency(uint32_t* M, KArg a) {
 }
 
 // update_rho_prs.wgsl:234-267
-__global__ void k_update_rho_prs(uint32_t* M, KArg a) {
+__global__ __launch_bounds__(256) void k_update_rho_prs(uint32_t* M, KArg a) {
   TE_TID3
   Dims g = dims(M, a.p[0]);
   if (gx >= g.nx || gy >= g.ny || gz >= g.nz) return;

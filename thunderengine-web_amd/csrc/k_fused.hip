@@ -13,9 +13,9 @@ namespace te {
 // rows (including the fresh i-halo) into the j-halo, so every halo cell ends
 // as the image f[i mod nx, j mod ny, k] of an interior cell; this kernel
 // writes exactly those images.
-// Grid: x = halo cells of one level (7 rows of strideJ, then 7 columns of ny),
+// Grid: x = halo cells of one level (7 rows of strideJ, then ny rows of 7),
 // y = k in [0, nz].  p[1..8] fields, p[9] -> field count.
-__global__ void k_bc_periodic_xy(uint32_t* M, KArg a) {
+__global__ __launch_bounds__(256) void k_bc_periodic_xy(uint32_t* M, KArg a) {
   TE_TID3
   Dims g = dims(M, a.p[0]);
   const int k = gy;
@@ -27,10 +27,10 @@ __global__ void k_bc_periodic_xy(uint32_t* M, KArg a) {
     i = gx - r * g.sJ - 3;
     j = r < 3 ? r - 3 : g.ny + (r - 3);
   } else {
-    const int t = gx - rowCells;       // 7 columns x ny rows
-    const int cidx = t / g.ny;
-    j = t - cidx * g.ny;
-    if (cidx >= 7) return;
+    const int t = gx - rowCells;       // ny rows x 7 halo columns, row by row
+    j = t / 7;
+    const int cidx = t - j * 7;
+    if (j >= g.ny) return;
     i = cidx < 3 ? cidx - 3 : g.nx + (cidx - 3);
   }
   const int si = ((i % g.nx) + g.nx) % g.nx, sj = ((j % g.ny) + g.ny) % g.ny;

@@ -4,10 +4,13 @@
 #include "te_device.hpp"
 #include "te_kernels.hpp"
 
+// Column sweeps load TS_CHUNK levels ahead of their serial arithmetic.
+#define TS_CHUNK 8
+
 namespace te {
 
 // rayleigh_damping.wgsl:111-164
-__global__ void k_rayleigh_damping(uint32_t* M, KArg a) {
+__global__ __launch_bounds__(256) void k_rayleigh_damping(uint32_t* M, KArg a) {
   TE_TID3
   Dims g = dims(M, a.p[0]);
   int i = gx, j = gy, k = gz;
@@ -38,7 +41,7 @@ __global__ void k_rayleigh_damping(uint32_t* M, KArg a) {
 }
 
 // turb_rho_faces.wgsl:111-151
-__global__ void k_turb_rho_faces(uint32_t* M, KArg a) {
+__global__ __launch_bounds__(256) void k_turb_rho_faces(uint32_t* M, KArg a) {
   TE_TID3
   Dims g = dims(M, a.p[0]);
   int i = gx - 2, j = gy - 2, k = gz;
@@ -62,7 +65,7 @@ __global__ void k_turb_rho_faces(uint32_t* M, KArg a) {
 }
 
 // turb_def_centers.wgsl:139-168
-__global__ void k_turb_def_centers(uint32_t* M, KArg a) {
+__global__ __launch_bounds__(256) void k_turb_def_centers(uint32_t* M, KArg a) {
   TE_TID3
   Dims g = dims(M, a.p[0]);
   int i = gx - 1, j = gy - 1, k = gz;
@@ -79,7 +82,7 @@ __global__ void k_turb_def_centers(uint32_t* M, KArg a) {
 }
 
 // turb_def_faces.wgsl:125-151
-__global__ void k_turb_def_faces(uint32_t* M, KArg a) {
+__global__ __launch_bounds__(256) void k_turb_def_faces(uint32_t* M, KArg a) {
   TE_TID3
   Dims g = dims(M, a.p[0]);
   int i = gx, j = gy, k = gz + 1;
@@ -96,7 +99,7 @@ __global__ void k_turb_def_faces(uint32_t* M, KArg a) {
 }
 
 // turb_def_assemble.wgsl:111-149
-__global__ void k_turb_def_assemble(uint32_t* M, KArg a) {
+__global__ __launch_bounds__(256) void k_turb_def_assemble(uint32_t* M, KArg a) {
   TE_TID3
   Dims g = dims(M, a.p[0]);
   int i = gx, j = gy, k = gz + 1;
@@ -132,7 +135,7 @@ TE_DEV float mp_rsif(float p, float t) {
 }
 
 // turb_nm.wgsl:236-355 (moist squared buoyancy frequency)
-__global__ void k_turb_nm(uint32_t* M, KArg a) {
+__global__ __launch_bounds__(256) void k_turb_nm(uint32_t* M, KArg a) {
   TE_TID3
   Dims g = dims(M, a.p[0]);
   int i = gx, j = gy, k = gz + 1;
@@ -218,7 +221,7 @@ __global__ void k_turb_nm(uint32_t* M, KArg a) {
 }
 
 // turb_km_kh.wgsl:125-187
-__global__ void k_turb_km_kh(uint32_t* M, KArg a) {
+__global__ __launch_bounds__(256) void k_turb_km_kh(uint32_t* M, KArg a) {
   TE_TID3
   Dims g = dims(M, a.p[0]);
   int i = gx, j = gy, k = gz + 1;
@@ -260,7 +263,7 @@ __global__ void k_turb_km_kh(uint32_t* M, KArg a) {
 }
 
 // turb_dissipative_heating.wgsl:139-194
-__global__ void k_turb_dissipative_heating(uint32_t* M, KArg a) {
+__global__ __launch_bounds__(256) void k_turb_dissipative_heating(uint32_t* M, KArg a) {
   TE_TID3
   Dims g = dims(M, a.p[0]);
   int i = gx, j = gy, k = gz;
@@ -294,7 +297,7 @@ __global__ void k_turb_dissipative_heating(uint32_t* M, KArg a) {
 }
 
 // turb_gettau.wgsl:111-143
-__global__ void k_turb_gettau(uint32_t* M, KArg a) {
+__global__ __launch_bounds__(256) void k_turb_gettau(uint32_t* M, KArg a) {
   TE_TID3
   Dims g = dims(M, a.p[0]);
   int i = gx - 1, j = gy - 1, k = gz;
@@ -318,7 +321,7 @@ __global__ void k_turb_gettau(uint32_t* M, KArg a) {
 
 // turb_sfcstress.wgsl:132-207 (free-slip and no-slip bottom; surface-flux
 // modes need the surface-layer planes and are not on this path)
-__global__ void k_turb_sfcstress(uint32_t* M, KArg a) {
+__global__ __launch_bounds__(256) void k_turb_sfcstress(uint32_t* M, KArg a) {
   const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x), j = (int)(blockIdx.y * blockDim.y + threadIdx.y);
   if (i >= (int)a.n[0] || j >= (int)a.n[1]) return;
   Dims g = dims(M, a.p[0]);
@@ -352,7 +355,7 @@ __global__ void k_turb_sfcstress(uint32_t* M, KArg a) {
 }
 
 // tke_epst.wgsl:170-206 (shear production)
-__global__ void k_tke_epst(uint32_t* M, KArg a) {
+__global__ __launch_bounds__(256) void k_tke_epst(uint32_t* M, KArg a) {
   TE_TID3
   Dims g = dims(M, a.p[0]);
   int i = gx, j = gy, k = gz + 1;
@@ -384,7 +387,7 @@ __global__ void k_tke_epst(uint32_t* M, KArg a) {
 }
 
 // tke_sources.wgsl:111-129
-__global__ void k_tke_sources(uint32_t* M, KArg a) {
+__global__ __launch_bounds__(256) void k_tke_sources(uint32_t* M, KArg a) {
   TE_TID3
   Dims g = dims(M, a.p[0]);
   int i = gx, j = gy, k = gz + 1;
@@ -432,18 +435,37 @@ __device__ void uv_tendency(uint32_t* M, const KArg& a, int i, int j) {
   float temc = (0.5f * dt) * g.rdz;
   float rdt = 0.5f / dt;
   const float hmf = DIR == 0 ? hm_uf(hm, g, i) : hm_vf(hm, g, j), rdh = DIR == 0 ? g.rdx : g.rdy;
-  for (int k = 0; k <= g.nz; ++k) {
-    const int64_t c = base + k * sk;
-    dwd[c] = (((((w[c] - w[c - st]) * rdh) * hmf) * 0.25f) * (kmv[c - st] + kmv[c])) * (rf[c - st] + rf[c]);
+  const int nz = g.nz;
+  // Every pass runs in chunks of TS_CHUNK levels whose loads are issued before
+  // the chunk's arithmetic.  Y(l) = ((kmv_l + kmv_l') * (mf+mf)) * (rf_l + rf_l')
+  // is the level-l factor of L1(k, l) = Y(l) * tem(k); Y(k+1) carries to the
+  // next level.  The back substitution and the tendency pass share one sweep.
+  for (int k0 = 0; k0 <= nz; k0 += TS_CHUNK) {
+    float w0[TS_CHUNK], w1[TS_CHUNK], k0v[TS_CHUNK], k1v[TS_CHUNK], r0[TS_CHUNK], r1[TS_CHUNK];
+#pragma unroll
+    for (int t = 0; t < TS_CHUNK; ++t) {
+      const int k = k0 + t;
+      if (k <= nz) {
+        const int64_t c = base + k * sk;
+        w0[t] = w[c]; w1[t] = w[c - st]; k0v[t] = kmv[c - st]; k1v[t] = kmv[c]; r0[t] = rf[c - st]; r1[t] = rf[c];
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < TS_CHUNK; ++t) {
+      const int k = k0 + t;
+      if (k <= nz)
+        dwd[base + k * sk] = (((((w0[t] - w1[t]) * rdh) * hmf) * 0.25f) * (k0v[t] + k1v[t])) * (r0[t] + r1[t]);
+    }
   }
-  auto L1 = [&](int k, int lev) {
+  auto Y = [&](int lev) {
     const int64_t cl = base + lev * sk;
-    float tem = (mh[k] + mh[k]) * rinv(k);
-    return (((kmv[cl - st] + kmv[cl]) * (mf[lev] + mf[lev])) * (rf[cl - st] + rf[cl])) * tem;
+    return ((kmv[cl - st] + kmv[cl]) * (mf[lev] + mf[lev])) * (rf[cl - st] + rf[cl]);
   };
   float pa, pb;
+  float Yn = Y(1);
   {
-    float cfc = tema * L1(0, 1);
+    float tem = (mh[0] + mh[0]) * rinv(0);
+    float cfc = tema * (Yn * tem);
     float cfb = 1.0f - cfc;
     float cfd = fma_(-((temc * tC[base]) * (mh[0] + mh[0])), rinv(0), q[base]);
     float t = 1.0f / cfb;
@@ -452,51 +474,223 @@ __device__ void uv_tendency(uint32_t* M, const KArg& a, int i, int j) {
     cA[base] = pa;
     cB[base] = pb;
   }
-  for (int k = 1; k < g.nz - 1; ++k) {
-    const int64_t c = base + k * sk;
-    float cfa = tema * L1(k, k), cfc = tema * L1(k, k + 1);
-    float cfb = (1.0f - cfa) - cfc;
-    float t = 1.0f / fma_(cfa, pa, cfb);
-    pa = (-cfc) * t;
-    pb = fma_(-cfa, pb, q[c]) * t;
-    cA[c] = pa;
-    cB[c] = pb;
+  for (int k0 = 1; k0 < nz - 1; k0 += TS_CHUNK) {
+    float ro0[TS_CHUNK], ro1[TS_CHUNK], mhk[TS_CHUNK], ka[TS_CHUNK], kb[TS_CHUNK], mfn[TS_CHUNK], ra[TS_CHUNK],
+        rb[TS_CHUNK], qv[TS_CHUNK];
+#pragma unroll
+    for (int t = 0; t < TS_CHUNK; ++t) {
+      const int k = k0 + t;
+      if (k < nz - 1) {
+        const int64_t c = base + k * sk, cn = c + sk;
+        ro0[t] = rho[c - st]; ro1[t] = rho[c]; mhk[t] = mh[k];
+        ka[t] = kmv[cn - st]; kb[t] = kmv[cn]; mfn[t] = mf[k + 1]; ra[t] = rf[cn - st]; rb[t] = rf[cn];
+        qv[t] = q[c];
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < TS_CHUNK; ++t) {
+      const int k = k0 + t;
+      if (k < nz - 1) {
+        const int64_t c = base + k * sk;
+        const float rinvk = 1.0f / (0.5f * (ro0[t] + ro1[t]));
+        const float tem = (mhk[t] + mhk[t]) * rinvk;
+        const float Yk = Yn;
+        Yn = ((ka[t] + kb[t]) * (mfn[t] + mfn[t])) * (ra[t] + rb[t]);
+        float cfa = tema * (Yk * tem), cfc = tema * (Yn * tem);
+        float cfb = (1.0f - cfa) - cfc;
+        float tt = 1.0f / fma_(cfa, pa, cfb);
+        pa = (-cfc) * tt;
+        pb = fma_(-cfa, pb, qv[t]) * tt;
+        cA[c] = pa;
+        cB[c] = pb;
+      }
+    }
   }
-  const int kt = g.nz - 1;
+  const int kt = nz - 1;
   const int64_t ct = base + kt * sk;
   {
-    float cfa = tema * L1(kt, kt);
+    float tem = (mh[kt] + mh[kt]) * rinv(kt);
+    float cfa = tema * (Yn * tem);
     float cfb = 1.0f - cfa;
-    float cfd = fma_((temc * tC[base + g.nz * sk]) * (mh[kt] + mh[kt]), rinv(kt), q[ct]);
+    float cfd = fma_((temc * tC[base + nz * sk]) * (mh[kt] + mh[kt]), rinv(kt), q[ct]);
     float t = 1.0f / fma_(cfa, pa, cfb);
     pb = fma_(-cfa, pb, cfd) * t;
     cB[ct] = pb;
   }
-  for (int k = g.nz - 2; k >= 0; --k) {
-    const int64_t c = base + k * sk;
-    pb = fma_(cA[c], pb, cB[c]);
-    cB[c] = pb;
-  }
-  for (int k = 0; k < g.nz; ++k) {
-    const int64_t c = base + k * sk;
+  // level kt's tendency, then back substitution + tendency for k = kt-1 .. 0
+  auto tendency = [&](int k, int64_t c, float cb, float qk, float ro0, float ro1, float d0, float d1, float tx, float ty,
+                      float mhk, float tv) {
+    float tz = ((ro0 + ro1) * (cb - qk)) * rdt;
+    tz = fma_(((d1 - d0) * g.rdz) * 0.5f, mhk + mhk, tz);
+    ten[c] = fma_((tx + ty) + tz, 1.0f / (0.5f * (ro0 + ro1)), tv);
+  };
+  {
     float tx, ty;
-    turbxy(c, tx, ty);
-    float tz = ((rho[c - st] + rho[c]) * (cB[c] - q[c])) * rdt;
-    tz = fma_(((dwd[c + sk] - dwd[c]) * g.rdz) * 0.5f, mh[k] + mh[k], tz);
-    ten[c] = fma_((tx + ty) + tz, rinv(k), ten[c]);
+    turbxy(ct, tx, ty);
+    tendency(kt, ct, pb, q[ct], rho[ct - st], rho[ct], dwd[ct], dwd[ct + sk], tx, ty, mh[kt], ten[ct]);
+  }
+  for (int k0 = nz - 2; k0 >= 0; k0 -= TS_CHUNK) {
+    float ca[TS_CHUNK], cb[TS_CHUNK], qv[TS_CHUNK], ro0[TS_CHUNK], ro1[TS_CHUNK], d0[TS_CHUNK], d1[TS_CHUNK],
+        txv[TS_CHUNK], tyv[TS_CHUNK], mhk[TS_CHUNK], tv[TS_CHUNK];
+#pragma unroll
+    for (int t = 0; t < TS_CHUNK; ++t) {
+      const int k = k0 - t;
+      if (k >= 0) {
+        const int64_t c = base + k * sk;
+        ca[t] = cA[c]; cb[t] = cB[c]; qv[t] = q[c]; ro0[t] = rho[c - st]; ro1[t] = rho[c];
+        d0[t] = dwd[c]; d1[t] = dwd[c + sk]; mhk[t] = mh[k]; tv[t] = ten[c];
+        turbxy(c, txv[t], tyv[t]);
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < TS_CHUNK; ++t) {
+      const int k = k0 - t;
+      if (k >= 0) {
+        const int64_t c = base + k * sk;
+        pb = fma_(ca[t], pb, cb[t]);
+        cB[c] = pb;
+        tendency(k, c, pb, qv[t], ro0[t], ro1[t], d0[t], d1[t], txv[t], tyv[t], mhk[t], tv[t]);
+      }
+    }
   }
 }
-__global__ void k_turb_u_tendency(uint32_t* M, KArg a) {
+__global__ __launch_bounds__(256) void k_turb_u_tendency(uint32_t* M, KArg a) {
   const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x), j = (int)(blockIdx.y * blockDim.y + threadIdx.y);
   if (i < (int)a.n[0] && j < (int)a.n[1]) uv_tendency<0>(M, a, i, j);
 }
-__global__ void k_turb_v_tendency(uint32_t* M, KArg a) {
+__global__ __launch_bounds__(256) void k_turb_v_tendency(uint32_t* M, KArg a) {
   const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x), j = (int)(blockIdx.y * blockDim.y + threadIdx.y);
   if (i < (int)a.n[0] && j < (int)a.n[1]) uv_tendency<1>(M, a, i, j);
 }
 
+// Implicit vertical solve shared by turb_w_tendency and tke_transport (both
+// solve for a face field x on levels 1..nz-1 with the same matrix):
+//   Z(k) = ((kmh[k] + kmh[k+1]) * mh[k]) * rho[k],   mfrf(k) = mf[k] / rf[k],
+//   upper(k) = tema * (Z(k) * mfrf(k)),  lower(k) = tema * (Z(k-1) * mfrf(k)),
+// first / last right-hand sides cfd1 / cfdT given by the caller; the
+// back-substituted solution is handed to out(k, c, x_new, x_old, rf) in one
+// descending sweep (levels nz-1 .. 1).  Loads run TS_CHUNK levels ahead.
+template <class OUT>
+__device__ __forceinline__ void face_implicit(const float* kmh, const float* mh, const float* rho, const float* mf,
+                                              const float* rf, const float* x, float* cA, float* cB, int64_t base,
+                                              int64_t sk, int nz, float tema, float cfd1, float cfdT, OUT out) {
+  float Zp = ((kmh[base] + kmh[base + sk]) * mh[0]) * rho[base];  // Z(0)
+  float Zk = ((kmh[base + sk] + kmh[base + 2 * sk]) * mh[1]) * rho[base + sk];
+  float pa, pb;
+  {
+    const int64_t c = base + sk;
+    const float mfrf1 = mf[1] / rf[c];
+    float cfc = tema * (Zk * mfrf1);
+    float cfb = 1.0f - cfc;
+    float inv = 1.0f / cfb;
+    pa = (-cfc) * inv;
+    pb = cfd1 * inv;
+    cA[c] = pa;
+    cB[c] = pb;
+  }
+  Zp = Zk;  // Z(1)
+  for (int k0 = 2; k0 < nz - 1; k0 += TS_CHUNK) {
+    float ka[TS_CHUNK], kb[TS_CHUNK], mhk[TS_CHUNK], ro[TS_CHUNK], mfk[TS_CHUNK], rfk[TS_CHUNK], xv[TS_CHUNK];
+#pragma unroll
+    for (int t = 0; t < TS_CHUNK; ++t) {
+      const int k = k0 + t;
+      if (k < nz - 1) {
+        const int64_t c = base + k * sk;
+        ka[t] = kmh[c]; kb[t] = kmh[c + sk]; mhk[t] = mh[k]; ro[t] = rho[c]; mfk[t] = mf[k]; rfk[t] = rf[c];
+        xv[t] = x[c];
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < TS_CHUNK; ++t) {
+      const int k = k0 + t;
+      if (k < nz - 1) {
+        const int64_t c = base + k * sk;
+        const float mfrf = mfk[t] / rfk[t];
+        Zk = ((ka[t] + kb[t]) * mhk[t]) * ro[t];
+        float cfa = tema * (Zp * mfrf), cfc = tema * (Zk * mfrf);
+        float cfb = (1.0f - cfa) - cfc;
+        float inv = 1.0f / fma_(cfa, pa, cfb);
+        pa = (-cfc) * inv;
+        pb = fma_(-cfa, pb, xv[t]) * inv;
+        cA[c] = pa;
+        cB[c] = pb;
+        Zp = Zk;
+      }
+    }
+  }
+  const int kt = nz - 1;
+  const int64_t ct = base + kt * sk;
+  {
+    // Z(kt-1): Zp holds it when the middle sweep ran (nz > 3); else Z(1) = Z(kt-1)
+    const float mfrfT = mf[kt] / rf[ct];
+    float cfa = tema * (Zp * mfrfT);
+    float cfb = 1.0f - cfa;
+    float inv = 1.0f / fma_(cfa, pa, cfb);
+    pb = fma_(-cfa, pb, cfdT) * inv;
+    cB[ct] = pb;
+    out(kt, ct, pb, x[ct], rf[ct]);
+  }
+  for (int k0 = nz - 2; k0 >= 1; k0 -= TS_CHUNK) {
+    float ca[TS_CHUNK], cb[TS_CHUNK], xv[TS_CHUNK], rfk[TS_CHUNK];
+#pragma unroll
+    for (int t = 0; t < TS_CHUNK; ++t) {
+      const int k = k0 - t;
+      if (k >= 1) {
+        const int64_t c = base + k * sk;
+        ca[t] = cA[c]; cb[t] = cB[c]; xv[t] = x[c]; rfk[t] = rf[c];
+        out.prefetch(t, c);
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < TS_CHUNK; ++t) {
+      const int k = k0 - t;
+      if (k >= 1) {
+        const int64_t c = base + k * sk;
+        pb = fma_(ca[t], pb, cb[t]);
+        cB[c] = pb;
+        out.at(t, c, pb, xv[t], rfk[t]);
+      }
+    }
+  }
+}
+
+// Tendency of a face field from the implicit solution plus horizontal flux
+// divergences: ten += ((tx + ty) + (rf * (x_new - x_old)) * rdt) / rf.
+struct FaceTendency {
+  const float *hx, *hy, *hm;
+  float* ten;
+  Dims g;
+  int i, j;
+  bool zeroX, zeroY, negate;
+  float rdt;
+  float txv[TS_CHUNK], tyv[TS_CHUNK], tv[TS_CHUNK];
+  __device__ void fluxes(int64_t c, float& tx, float& ty) const {
+    if (negate) {  // tke_transport: -(f[i+1] - f[i])
+      tx = zeroX ? 0.0f : ((-(hx[c + 1] - hx[c])) * g.rdx) * hm_uh(hm, g, i);
+      ty = zeroY ? 0.0f : ((-(hy[c + g.sJ] - hy[c])) * g.rdy) * hm_vh(hm, g, j);
+    } else {       // turb_w_tendency: (t13[i+1] - t13[i])
+      tx = zeroX ? 0.0f : ((hx[c + 1] - hx[c]) * g.rdx) * hm_uh(hm, g, i);
+      ty = zeroY ? 0.0f : ((hy[c + g.sJ] - hy[c]) * g.rdy) * hm_vh(hm, g, j);
+    }
+  }
+  __device__ void apply(int64_t c, float xn, float xo, float rfc, float tx, float ty, float t0) const {
+    float tz = (rfc * (xn - xo)) * rdt;
+    ten[c] = t0 + ((tx + ty) + tz) / rfc;
+  }
+  __device__ void operator()(int, int64_t c, float xn, float xo, float rfc) {
+    float tx, ty;
+    fluxes(c, tx, ty);
+    apply(c, xn, xo, rfc, tx, ty, ten[c]);
+  }
+  __device__ void prefetch(int t, int64_t c) {
+    fluxes(c, txv[t], tyv[t]);
+    tv[t] = ten[c];
+  }
+  __device__ void at(int t, int64_t c, float xn, float xo, float rfc) { apply(c, xn, xo, rfc, txv[t], tyv[t], tv[t]); }
+};
+
 // turb_w_tendency.wgsl:152-287
-__global__ void k_turb_w_tendency(uint32_t* M, KArg a) {
+__global__ __launch_bounds__(256) void k_turb_w_tendency(uint32_t* M, KArg a) {
   const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x), j = (int)(blockIdx.y * blockDim.y + threadIdx.y);
   if (i >= (int)a.n[0] || j >= (int)a.n[1]) return;
   Dims g = dims(M, a.p[0]);
@@ -526,57 +720,16 @@ __global__ void k_turb_w_tendency(uint32_t* M, KArg a) {
   float tema = ((-1.0f * dt) * g.rdz) * g.rdz;
   float temc = dt * g.rdz;
   float rdt = 1.0f / dt;
-  auto mfrf = [&](int k) { return mf[k] / rf[base + k * sk]; };
-  auto L1 = [&](int k) { const int64_t c = base + k * sk; return (((kmh[c] + kmh[c + sk]) * mh[k]) * rho[c]) * mfrf(k); };
-  auto L4 = [&](int k) { const int64_t c = base + k * sk; return (((kmh[c - sk] + kmh[c]) * mh[k - 1]) * rho[c - sk]) * mfrf(k); };
-  float pa, pb;
-  {
-    const int64_t c = base + sk;
-    float cfc = tema * L1(1);
-    float cfb = 1.0f - cfc;
-    float cfd = fma_(-(temc * t33[c - sk]), mfrf(1), w[c]);
-    float t = 1.0f / cfb;
-    pa = (-cfc) * t;
-    pb = cfd * t;
-    cA[c] = pa;
-    cB[c] = pb;
-  }
-  for (int k = 2; k < g.nz - 1; ++k) {
-    const int64_t c = base + k * sk;
-    float cfa = tema * L4(k), cfc = tema * L1(k);
-    float cfb = (1.0f - cfa) - cfc;
-    float t = 1.0f / fma_(cfa, pa, cfb);
-    pa = (-cfc) * t;
-    pb = fma_(-cfa, pb, w[c]) * t;
-    cA[c] = pa;
-    cB[c] = pb;
-  }
-  const int kt = g.nz - 1;
-  const int64_t ct = base + kt * sk;
-  {
-    float cfa = tema * L4(kt);
-    float cfb = 1.0f - cfa;
-    float cfd = fma_(temc * t33[ct], mfrf(kt), w[ct]);
-    float t = 1.0f / fma_(cfa, pa, cfb);
-    pb = fma_(-cfa, pb, cfd) * t;
-    cB[ct] = pb;
-  }
-  for (int k = g.nz - 2; k >= 1; --k) {
-    const int64_t c = base + k * sk;
-    pb = fma_(cA[c], pb, cB[c]);
-    cB[c] = pb;
-  }
-  for (int k = 1; k < g.nz; ++k) {
-    const int64_t c = base + k * sk;
-    float tx, ty;
-    turbxy(c, tx, ty);
-    float tz = (rf[c] * (cB[c] - w[c])) * rdt;
-    wten[c] = wten[c] + ((tx + ty) + tz) / rf[c];
-  }
+  const int nz = g.nz;
+  const int64_t c1 = base + sk, ct = base + (nz - 1) * sk;
+  const float cfd1 = fma_(-(temc * t33[c1 - sk]), mf[1] / rf[c1], w[c1]);
+  const float cfdT = fma_(temc * t33[ct], mf[nz - 1] / rf[ct], w[ct]);
+  FaceTendency out{t13, t23, hm, wten, g, i, j, zeroX, zeroY, false, rdt, {}, {}, {}};
+  face_implicit(kmh, mh, rho, mf, rf, w, cA, cB, base, sk, nz, tema, cfd1, cfdT, out);
 }
 
 // turb_scalar_flux_h.wgsl:125-148
-__global__ void k_turb_scalar_flux_h(uint32_t* M, KArg a) {
+__global__ __launch_bounds__(256) void k_turb_scalar_flux_h(uint32_t* M, KArg a) {
   TE_TID3
   Dims g = dims(M, a.p[0]);
   int i = gx, j = gy, k = gz;
@@ -591,7 +744,7 @@ __global__ void k_turb_scalar_flux_h(uint32_t* M, KArg a) {
 }
 
 // turb_scalar_vert_implicit.wgsl:210-327
-__global__ void k_turb_scalar_vert_implicit(uint32_t* M, KArg a) {
+__global__ __launch_bounds__(256) void k_turb_scalar_vert_implicit(uint32_t* M, KArg a) {
   const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x), j = (int)(blockIdx.y * blockDim.y + threadIdx.y);
   if (i >= (int)a.n[0] || j >= (int)a.n[1]) return;
   Dims g = dims(M, a.p[0]);
@@ -641,11 +794,14 @@ __global__ void k_turb_scalar_vert_implicit(uint32_t* M, KArg a) {
   float tema = ((-dt) * g.rdz) * g.rdz;
   float rdt = 1.0f / dt;
   float temc = dt * g.rdz;
-  auto L3 = [&](int k) { const int64_t c = base + k * sk; return (((khv[c + sk] * mf[k + 1]) * rf[c + sk]) * mh[k]) * rr[c]; };
-  auto L4 = [&](int k) { const int64_t c = base + k * sk; return (((khv[c] * mf[k]) * rf[c]) * mh[k]) * rr[c]; };
+  // L4(k) = ((X(k) * mh[k]) * rr[k]) and L3(k) = ((X(k+1) * mh[k]) * rr[k]) with
+  // X(k) = (khv[k] * mf[k]) * rf[k]: X(k+1) is evaluated once and carried.
+  const int nz = g.nz;
+  auto X = [&](int k) { const int64_t c = base + k * sk; return (khv[c] * mf[k]) * rf[c]; };
   float pa, pb;
+  float Xk1 = X(1);  // X(k+1) for k = 0
   {
-    float cfc = tema * L3(0);
+    float cfc = tema * ((Xk1 * mh[0]) * rr[base]);
     float cfb = 1.0f - cfc;
     float t = 1.0f / cfb;
     pa = (-cfc) * t;
@@ -663,20 +819,37 @@ __global__ void k_turb_scalar_vert_implicit(uint32_t* M, KArg a) {
     pb = cfd * t;
     cB[base] = pb;
   }
-  for (int k = 1; k < g.nz - 1; ++k) {
-    const int64_t c = base + k * sk;
-    float cfa = tema * L4(k), cfc = tema * L3(k);
-    float cfb = (1.0f - cfa) - cfc;
-    float t = 1.0f / fma_(cfa, pa, cfb);
-    pa = (-cfc) * t;
-    pb = fma_(-cfa, pb, s[c]) * t;
-    cA[c] = pa;
-    cB[c] = pb;
+  for (int k0 = 1; k0 < nz - 1; k0 += TS_CHUNK) {
+    float kh[TS_CHUNK], rfu[TS_CHUNK], mfu[TS_CHUNK], mhk[TS_CHUNK], rrk[TS_CHUNK], sk_[TS_CHUNK];
+#pragma unroll
+    for (int t = 0; t < TS_CHUNK; ++t) {
+      const int k = k0 + t;
+      if (k < nz - 1) {
+        const int64_t c = base + k * sk;
+        kh[t] = khv[c + sk]; rfu[t] = rf[c + sk]; mfu[t] = mf[k + 1]; mhk[t] = mh[k]; rrk[t] = rr[c]; sk_[t] = s[c];
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < TS_CHUNK; ++t) {
+      const int k = k0 + t;
+      if (k < nz - 1) {
+        const int64_t c = base + k * sk;
+        const float Xk = Xk1;
+        Xk1 = (kh[t] * mfu[t]) * rfu[t];
+        float cfa = tema * ((Xk * mhk[t]) * rrk[t]), cfc = tema * ((Xk1 * mhk[t]) * rrk[t]);
+        float cfb = (1.0f - cfa) - cfc;
+        float tt = 1.0f / fma_(cfa, pa, cfb);
+        pa = (-cfc) * tt;
+        pb = fma_(-cfa, pb, sk_[t]) * tt;
+        cA[c] = pa;
+        cB[c] = pb;
+      }
+    }
   }
-  const int kt = g.nz - 1;
+  const int kt = nz - 1;
   const int64_t ct = base + kt * sk;
   {
-    float cfa = tema * L4(kt);
+    float cfa = tema * ((Xk1 * mh[kt]) * rr[ct]);
     float cfb = 1.0f - cfa;
     float t = 1.0f / fma_(cfa, pa, cfb);
     float cfd = s[ct];
@@ -688,16 +861,31 @@ __global__ void k_turb_scalar_vert_implicit(uint32_t* M, KArg a) {
     cB[ct] = pb;
     sten[ct] = fma_(hdiv(ct) + (rho[ct] * (pb - s[ct])) * rdt, rr[ct], sten[ct]);
   }
-  for (int k = g.nz - 2; k >= 0; --k) {
-    const int64_t c = base + k * sk;
-    pb = fma_(cA[c], pb, cB[c]);
-    cB[c] = pb;
-    sten[c] = fma_(hdiv(c) + (rho[c] * (pb - s[c])) * rdt, rr[c], sten[c]);
+  for (int k0 = nz - 2; k0 >= 0; k0 -= TS_CHUNK) {
+    float ca[TS_CHUNK], cb[TS_CHUNK], hd[TS_CHUNK], ro[TS_CHUNK], sv[TS_CHUNK], rv[TS_CHUNK], tv[TS_CHUNK];
+#pragma unroll
+    for (int t = 0; t < TS_CHUNK; ++t) {
+      const int k = k0 - t;
+      if (k >= 0) {
+        const int64_t c = base + k * sk;
+        ca[t] = cA[c]; cb[t] = cB[c]; hd[t] = hdiv(c); ro[t] = rho[c]; sv[t] = s[c]; rv[t] = rr[c]; tv[t] = sten[c];
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < TS_CHUNK; ++t) {
+      const int k = k0 - t;
+      if (k >= 0) {
+        const int64_t c = base + k * sk;
+        pb = fma_(ca[t], pb, cb[t]);
+        cB[c] = pb;
+        sten[c] = fma_(hd[t] + (ro[t] * (pb - sv[t])) * rdt, rv[t], tv[t]);
+      }
+    }
   }
 }
 
 // tke_flux_h.wgsl:125-146
-__global__ void k_tke_flux_h(uint32_t* M, KArg a) {
+__global__ __launch_bounds__(256) void k_tke_flux_h(uint32_t* M, KArg a) {
   TE_TID3
   Dims g = dims(M, a.p[0]);
   int i = gx, j = gy, k = gz + 1;
@@ -711,7 +899,7 @@ __global__ void k_tke_flux_h(uint32_t* M, KArg a) {
 }
 
 // tke_transport.wgsl:158-301
-__global__ void k_tke_transport(uint32_t* M, KArg a) {
+__global__ __launch_bounds__(256) void k_tke_transport(uint32_t* M, KArg a) {
   const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x), j = (int)(blockIdx.y * blockDim.y + threadIdx.y);
   if (i >= (int)a.n[0] || j >= (int)a.n[1]) return;
   Dims g = dims(M, a.p[0]);
@@ -749,54 +937,14 @@ __global__ void k_tke_transport(uint32_t* M, KArg a) {
   float tema = ((-1.0f * dt) * g.rdz) * g.rdz;
   float temc = dt * g.rdz;
   float rdt = 1.0f / dt;
-  auto L2 = [&](int k) { const int64_t c = base + k * sk; return (((kmh[c] + kmh[c + sk]) * mh[k]) * rho[c]) * mfrf(k); };
-  auto L5 = [&](int k) { const int64_t c = base + k * sk; return (((kmh[c - sk] + kmh[c]) * mh[k - 1]) * rho[c - sk]) * mfrf(k); };
-  float pa, pb;
-  {
-    const int64_t c = base + sk;
-    float cfc = tema * L2(1);
-    float cfb = 1.0f - cfc;
-    float tem = ((((-(kmv[c - sk] + kmv[c])) * (t[c] - t[c - sk])) * g.rdz) * mh[0]) * rho[c - sk];
-    float cfd = fma_(temc * tem, mfrf(1), t[c]);
-    float inv = 1.0f / cfb;
-    pa = (-cfc) * inv;
-    pb = cfd * inv;
-    cA[c] = pa;
-    cB[c] = pb;
-  }
-  for (int k = 2; k < g.nz - 1; ++k) {
-    const int64_t c = base + k * sk;
-    float cfa = tema * L5(k), cfc = tema * L2(k);
-    float cfb = (1.0f - cfa) - cfc;
-    float inv = 1.0f / fma_(cfa, pa, cfb);
-    pa = (-cfc) * inv;
-    pb = fma_(-cfa, pb, t[c]) * inv;
-    cA[c] = pa;
-    cB[c] = pb;
-  }
-  const int kt = g.nz - 1;
-  const int64_t ct = base + kt * sk;
-  {
-    float cfa = tema * L5(kt);
-    float cfb = 1.0f - cfa;
-    float tem = ((((-(kmv[ct] + kmv[ct + sk])) * (t[ct + sk] - t[ct])) * g.rdz) * mh[kt]) * rho[ct];
-    float cfd = fma_(-(temc * tem), mfrf(kt), t[ct]);
-    float inv = 1.0f / fma_(cfa, pa, cfb);
-    pb = fma_(-cfa, pb, cfd) * inv;
-    cB[ct] = pb;
-  }
-  for (int k = g.nz - 2; k >= 1; --k) {
-    const int64_t c = base + k * sk;
-    pb = fma_(cA[c], pb, cB[c]);
-    cB[c] = pb;
-  }
-  for (int k = 1; k < g.nz; ++k) {
-    const int64_t c = base + k * sk;
-    float tx, ty;
-    turbxy(c, tx, ty);
-    float tz = (rf[c] * (cB[c] - t[c])) * rdt;
-    tketen[c] = tketen[c] + ((tx + ty) + tz) / rf[c];
-  }
+  const int nz = g.nz;
+  const int64_t c1 = base + sk, ct = base + (nz - 1) * sk;
+  const float tem1 = ((((-(kmv[c1 - sk] + kmv[c1])) * (t[c1] - t[c1 - sk])) * g.rdz) * mh[0]) * rho[c1 - sk];
+  const float cfd1 = fma_(temc * tem1, mf[1] / rf[c1], t[c1]);
+  const float temT = ((((-(kmv[ct] + kmv[ct + sk])) * (t[ct + sk] - t[ct])) * g.rdz) * mh[nz - 1]) * rho[ct];
+  const float cfdT = fma_(-(temc * temT), mf[nz - 1] / rf[ct], t[ct]);
+  FaceTendency out{fx, fy, hm, tketen, g, i, j, zeroX, zeroY, true, rdt, {}, {}, {}};
+  face_implicit(kmh, mh, rho, mf, rf, t, cA, cB, base, sk, nz, tema, cfd1, cfdT, out);
 }
 
 }  // namespace te

@@ -25,7 +25,7 @@
 #define TE_FUSED_LIST(X) X(bc_periodic_xy)
 
 namespace te {
-#define TE_DECL(n) __global__ void k_##n(uint32_t* M, KArg a);
+#define TE_DECL(n) __global__ __launch_bounds__(256) void k_##n(uint32_t* M, KArg a);
 TE_KERNEL_LIST(TE_DECL)
 #undef TE_DECL
 }  // namespace te
