@@ -81,6 +81,60 @@ static int run_program(uint32_t* w, T* v, const char* const* names, const uint32
       }
       continue;
     }
+    if (nm == "ac_fpk_solve" || nm == "ac_pp_accumulate") {
+      // the acoustic dispatches each stands for, in the reference's order
+      const uint32_t* dm = w + A[4];
+      const uint32_t nx = dm[0], ny = dm[1], nz = dm[2];
+      uint32_t X[64] = {0}, Y[64] = {0};
+      X[4] = Y[4] = A[4];      // dims
+      X[5] = Y[5] = A[5];      // acoustic parameters
+      X[34] = Y[34] = A[34];   // metric
+      const char *first, *second;
+      if (nm == "ac_fpk_solve") {
+        first = "acoustic_fpk"; second = "acoustic_w_solve";
+        X[0] = nx; X[1] = ny; X[2] = nz;
+        for (int q = 2; q <= 11; ++q) X[4 + q] = A[4 + q];            // u v w piadv ppten ppterm pk1 pk2 qk fpk
+        Y[0] = nx; Y[1] = ny; Y[2] = 1;
+        const int ws[10] = {8, 9, 10, 12, 11, 13, 14, 4, 15, 16};       // pk1 pk2 qk mm fpk wten pp3d w cA cB
+        for (int q = 0; q < 10; ++q) Y[4 + 2 + q] = A[4 + ws[q]];
+      } else {
+        first = "acoustic_pp"; second = "acoustic_accumulate";
+        X[0] = nx; X[1] = ny; X[2] = nz;
+        for (int q = 2; q <= 10; ++q) X[4 + q] = A[4 + q];            // w pk1 pk2 qk fpk pp3d ppd ppx saveX
+        Y[0] = A[0]; Y[1] = A[1]; Y[2] = A[2];
+        const int as[7] = {11, 12, 2, 13, 14, 15, 16};                   // u v w rru rrv rrw tavg
+        for (int q = 0; q < 7; ++q) Y[4 + 2 + q] = A[4 + as[q]];
+      }
+      const KernelEntry *e1 = find_kernel(first), *e2 = find_kernel(second);
+      K<T> k1{w, v, X}, k2{w, v, Y};
+      if constexpr (sizeof(T) == 4) { e1->f32(k1); e2->f32(k2); }
+      else { e1->f64(k1); e2->f64(k2); }
+      continue;
+    }
+    if (nm == "zero_fields") {  // turb_zero on each of its fields
+      const KernelEntry* ez = find_kernel("turb_zero");
+      const uint32_t nf = w[A[4 + 9]];
+      for (uint32_t q = 0; q < nf && q < 8; ++q) {
+        uint32_t Az[64] = {0};
+        Az[0] = A[0]; Az[1] = A[1]; Az[2] = A[2];
+        Az[4] = A[4];
+        Az[5] = A[5 + q];
+        Az[34] = A[34];
+        K<T> kz{w, v, Az};
+        if constexpr (sizeof(T) == 4) ez->f32(kz);
+        else ez->f64(kz);
+      }
+      continue;
+    }
+    if (nm == "copy_fields") {  // the buffer copies it stands for, in order
+      const uint32_t nf = w[A[4 + 9]], n = A[0];
+      for (uint32_t f = 0; f < nf && f < 4; ++f) {
+        const uint32_t src = A[5 + f], dst = A[9 + f];
+#pragma omp parallel for schedule(static)
+        for (int64_t q = 0; q < (int64_t)n; ++q) v[dst + q] = v[src + q];
+      }
+      continue;
+    }
     const KernelEntry* e = find_kernel(nm);
     if (!e) {
       std::snprintf(err, errlen, "oracle: no kernel '%s' (op %d)", names[o], o);

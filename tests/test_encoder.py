@@ -24,7 +24,7 @@ def supercell_config(fields, **kw):
     base = dict(nx=120, ny=120, nz=40, dx=1000., dy=1000., dz=500., dt=6., nsound=0, moist=1, species=11,
                 open_x=1, open_y=1, weno_scalars=1, weno_momentum=0, tke=1, explicit_vdiff=0,
                 dissipative_heating=1, rayleigh=1, rayleigh_zd=15000., umove=12.5, vmove=3.0, microphysics=5,
-                mass_conservation=1, cache_moisture_turbulence=1, arena_fields=fields)
+                mass_conservation=1, cache_moisture_turbulence=1, arena_fields=fields, fuse=0)
     base.update(kw)
     return te_host.config(**base)
 
@@ -111,10 +111,13 @@ def test_schedule_closed_form():
     substeps with n1 = max(1, round(n/3)) (timestep.js schedule), i.e. 11 for
     n = 6; dispatch counts follow it."""
     for ns, total in ((6, 2 + 3 + 6), (4, 2 + 2 + 4), (8, 3 + 4 + 8)):
-        r = te_host.Run(te_host.config(nx=16, ny=16, nz=12, nsound=ns), encode_only=True)
+        r = te_host.Run(te_host.config(nx=16, ny=16, nz=12, nsound=ns, fuse=0), encode_only=True)
         names, _ = r.program()
         assert names.count("acoustic_uv") == total, (ns, names.count("acoustic_uv"))
         assert names.count("acoustic_w_solve") == total
+        r = te_host.Run(te_host.config(nx=16, ny=16, nz=12, nsound=ns, fuse=1), encode_only=True)
+        names, _ = r.program()
+        assert names.count("acoustic_uv") == names.count("ac_fpk_solve") == total
 
 
 def test_encoder_rejects_bad_config():

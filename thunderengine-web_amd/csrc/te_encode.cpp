@@ -92,6 +92,12 @@ class Encoder {
     return o;
   }
   bool has(const std::string& n) const { return E.L.field(n) != UINT64_MAX; }
+  std::string fieldName(uint64_t o) const {
+    if (o < E.L.arena || E.L.field_words == 0) throw std::runtime_error("encoder: offset outside the arena");
+    const uint64_t q = (o - E.L.arena) / E.L.field_words;
+    if (q >= E.L.fields.size() || E.L.arena + q * E.L.field_words != o) throw std::runtime_error("encoder: not a field start");
+    return E.L.fields[q];
+  }
   uint64_t bytes(const Bytes& w) {
     auto it = bytesAt.find(w);
     if (it != bytesAt.end()) return it->second;
@@ -112,7 +118,25 @@ class Encoder {
   // fusion is on; they are written by one bc_periodic_xy launch (up to 8
   // fields) just before the next op that is not a halo fill.
   bool mergeHalo = false;
-  std::vector<std::string> pendingHalo;
+  std::vector<std::string> pendingHalo, pendingZero;
+  // turb_zero runs queue likewise (fusion on): one zero_fields launch per 8.
+  void flushZero() {
+    std::vector<std::string> q;
+    q.swap(pendingZero);
+    for (size_t s = 0; s < q.size(); s += 8) {
+      const int n = (int)std::min<size_t>(8, q.size() - s);
+      Op o;
+      std::memset(&o, 0, sizeof(o));
+      o.kid = kernel_id("zero_fields");
+      o.A[0] = (uint32_t)(cfg.nx + 7); o.A[1] = (uint32_t)(cfg.ny + 7); o.A[2] = (uint32_t)(cfg.nz + 1);
+      o.A[4 + 0] = (uint32_t)dimsOff;
+      o.A[4 + 30] = (uint32_t)hmOff;
+      for (int q2 = 0; q2 < n; ++q2) o.A[4 + 1 + q2] = (uint32_t)off(q[s + q2]);
+      o.A[4 + 9] = (uint32_t)bytes(I1(n));
+      E.ops.push_back(o);
+      E.names.push_back("zero_fields");
+    }
+  }
   void flushHalo() {
     std::vector<std::string> q;
     q.swap(pendingHalo);
@@ -132,7 +156,14 @@ class Encoder {
   }
 
   void op(const char* kname, std::array<int, 3> grid, std::initializer_list<Bind> binds, bool withAcoustic = false) {
+    if (cfg.fuse != 0 && std::strcmp(kname, "turb_zero") == 0 && binds.size() == 1) {
+      if (!pendingHalo.empty()) flushHalo();
+      const std::string f = fieldName(binds.begin()->off);
+      if (std::find(pendingZero.begin(), pendingZero.end(), f) == pendingZero.end()) pendingZero.push_back(f);
+      return;
+    }
     if (!pendingHalo.empty()) flushHalo();
+    if (!pendingZero.empty()) flushZero();
     Op o;
     std::memset(&o, 0, sizeof(o));
     o.kid = kernel_id(kname);
@@ -294,6 +325,7 @@ void Encoder::openHalo(const std::string& n) {
 void Encoder::haloFill(const std::string& n) {
   const int H = 3;
   if (mergeHalo) {
+    if (!pendingZero.empty()) flushZero();
     if (std::find(pendingHalo.begin(), pendingHalo.end(), n) == pendingHalo.end()) pendingHalo.push_back(n);
     return;
   }
@@ -580,6 +612,22 @@ void Encoder::stages() {
            true);
       op("acoustic_uv", stag,
          {fb(2, "ppd"), fb(3, "thv"), fb(4, "uten"), fb(5, "vten"), fb(6, "u"), fb(7, "v"), bb(8, open)}, true);
+      if (cfg.fuse != 0) {  // fpk + w solve in one column sweep, pp + accumulate in one pass;
+                            // accumulate reads no ppd, so ppd's halo joins u, v, w's
+        op("ac_fpk_solve", cols,
+           {fb(2, "u"), fb(3, "v"), fb(4, "w"), fb(5, "piadv"), fb(6, "ppten"), fb(7, "ppterm"), fb(8, "pk1"),
+            fb(9, "pk2"), fb(10, "qk"), fb(11, "fpk"), fb(12, "mm"), fb(13, "wten"), fb(14, "pp3d"), fb(15, "cA"),
+            fb(16, "cB")},
+           true);
+        const float tavg = n == nloop - 1 ? (float)(1.0 / nloop) : 0.0f;
+        op("ac_pp_accumulate", whole,
+           {fb(2, "w"), fb(3, "pk1"), fb(4, "pk2"), fb(5, "qk"), fb(6, "fpk"), fb(7, "pp3d"), fb(8, "ppd"),
+            fb(9, "ppx"), bb(10, I1(nrk == 2 ? 1 : 0)), fb(11, "u"), fb(12, "v"), fb(13, "rru"), fb(14, "rrv"),
+            fb(15, "rrw"), bb(16, F1(tavg))},
+           true);
+        for (const char* h : {"ppd", "u", "v", "w"}) haloFill(h);
+        continue;
+      }
       op("acoustic_fpk", full,
          {fb(2, "u"), fb(3, "v"), fb(4, "w"), fb(5, "piadv"), fb(6, "ppten"), fb(7, "ppterm"), fb(8, "pk1"),
           fb(9, "pk2"), fb(10, "qk"), fb(11, "fpk"), rb(15, "rho0"), rb(16, "rr0"), rb(17, "rf0")},
@@ -592,21 +640,13 @@ void Encoder::stages() {
          {fb(2, "w"), fb(3, "pk1"), fb(4, "pk2"), fb(5, "qk"), fb(6, "fpk"), fb(7, "pp3d"), fb(8, "ppd"), fb(9, "ppx"),
           bb(10, I1(nrk == 2 ? 1 : 0))},
          true);
+      haloFill("ppd");
       const float tavg = n == nloop - 1 ? (float)(1.0 / nloop) : 0.0f;
-      auto accumulate = [&] {
-        op("acoustic_accumulate", whole,
-           {fb(2, "u"), fb(3, "v"), fb(4, "w"), fb(5, "rru"), fb(6, "rrv"), fb(7, "rrw"), bb(8, F1(tavg))}, true);
-      };
-      if (mergeHalo) {  // accumulate reads no ppd: the four fills become one launch
-        accumulate();
-        for (const char* h : {"ppd", "u", "v", "w"}) haloFill(h);
-      } else {
-        haloFill("ppd");
-        accumulate();
-        haloFill("u");
-        haloFill("v");
-        haloFill("w");
-      }
+      op("acoustic_accumulate", whole,
+         {fb(2, "u"), fb(3, "v"), fb(4, "w"), fb(5, "rru"), fb(6, "rrv"), fb(7, "rrw"), bb(8, F1(tavg))}, true);
+      haloFill("u");
+      haloFill("v");
+      haloFill("w");
     }
     haloFill("pp3d");
     if (openX)
@@ -665,6 +705,7 @@ void Encoder::end() {
   op("ppx_finish", full, {fb(1, "pp3d"), fb(2, "ppx")});
   haloFill("ppx");
   flushHalo();
+  flushZero();
   // state swap (timestep.js:766-774)
   const uint64_t count = (uint64_t)E.L.sJ * E.L.sK * (nz + 1);
   std::vector<std::pair<std::string, std::string>> pairs;
@@ -672,6 +713,24 @@ void Encoder::end() {
   for (auto& s : momentum) pairs.push_back({s, s + "_a"});
   pairs.push_back({"pp3d", "ppi"});
   if (tke) pairs.push_back({"tke", "tke_a"});
+  if (cfg.fuse != 0) {  // up to 4 copies of equal length per copy_fields launch
+    for (size_t s = 0; s < pairs.size(); s += 4) {
+      const int n = (int)std::min<size_t>(4, pairs.size() - s);
+      Op o;
+      std::memset(&o, 0, sizeof(o));
+      o.kid = kernel_id("copy_fields");
+      o.A[0] = (uint32_t)count; o.A[1] = 1; o.A[2] = 1;
+      o.A[4 + 0] = (uint32_t)dimsOff;
+      for (int q = 0; q < n; ++q) {
+        o.A[4 + 1 + q] = (uint32_t)off(pairs[s + q].first);
+        o.A[4 + 5 + q] = (uint32_t)off(pairs[s + q].second);
+      }
+      o.A[4 + 9] = (uint32_t)bytes(I1(n));
+      E.ops.push_back(o);
+      E.names.push_back("copy_fields");
+    }
+    pairs.clear();
+  }
   for (auto& p : pairs) {
     Op o;
     std::memset(&o, 0, sizeof(o));

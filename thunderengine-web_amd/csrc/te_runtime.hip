@@ -108,6 +108,10 @@ static int validate_ops(te_ctx* c) {
         return fail(c, TE_ERR_ARG, "copy op " + std::to_string(i) + " leaves the memory");
       continue;
     }
+    if (std::strcmp(nm, "copy_fields") == 0)
+      for (int q = 1; q <= 8; ++q)
+        if ((uint64_t)op.A[4 + q] + op.A[0] > c->words)
+          return fail(c, TE_ERR_ARG, "copy_fields op " + std::to_string(i) + " leaves the memory");
     for (int q = 0; q < 32; ++q)
       if (op.A[4 + q] >= c->words) return fail(c, TE_ERR_ARG, "op " + std::to_string(i) + " (" + nm + ") parameter offset outside memory");
   }
