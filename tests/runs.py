@@ -55,6 +55,25 @@ class OracleRun:
         return self.field(name)[: e.nz, 3: 3 + e.ny, 3: 3 + e.nx]
 
 
+# Scratch fields the fused launches may leave with other values than the
+# reference's dispatch list (fluxes, elimination coefficients, fpk, tend,
+# sadv).  tests/test_oracle_invariants.py checks that each is dead at the
+# start of a step: its contents then never reach another field.
+FUSED_SCRATCH = ("fx", "fy", "fz", "cA", "cB", "fpk", "tend", "sadv")
+
+
+def state_mask(e):
+    """Boolean mask over [0, arena_end) of the words compared in fused runs:
+    profiles, parameter regions and every arena field but FUSED_SCRATCH."""
+    end = arena_end(e)
+    m = np.ones(end, dtype=bool)
+    for f in FUSED_SCRATCH:
+        if f in e.fields:
+            o = e.field_offset(f)
+            m[o: o + e.field_words] = False
+    return m
+
+
 def arena_end(e):
     """Word offset just past the FieldArena (profiles and arena lie below it)."""
     return e.field_offset(e.fields[0]) + e.field_words * len(e.fields)

@@ -101,5 +101,6 @@ def test_js_run_bit_exact_vs_oracle(gpu):
     ref.step(2)
     e = ref.enc
     n = e.sJ * e.sK * (e.nz + 1)
-    bad = [f for f in e.fields if got[f] != fnv(ref.M[e.field_offset(f): e.field_offset(f) + n])]
+    bad = [f for f in e.fields if f not in runs.FUSED_SCRATCH
+           and got[f] != fnv(ref.M[e.field_offset(f): e.field_offset(f) + n])]
     assert not bad, bad

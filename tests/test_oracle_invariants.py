@@ -77,3 +77,27 @@ def test_fused_program_equals_reference_program(kw):
     assert runs.arena_end(b.enc) == end  # parameter bytes follow the arena and may differ
     wa, wb = a.M[:end], b.M[:end]
     assert np.array_equal(wa, wb), runs.first_difference(a, wa, wb)
+
+
+@pytest.mark.parametrize("kw", [dict(tke=1, initial_tke=3, rayleigh=1), dict(open_x=1, open_y=1, tke=1, initial_tke=3)])
+def test_fused_scratch_fields_are_dead(kw):
+    """Garbage in any FUSED_SCRATCH field at the start of a step changes no
+    other field at its end: fused launches may leave them different."""
+    cfg = te_host.config(nx=20, ny=16, nz=10, **kw)
+    base = runs.OracleRun(cfg)
+    base.step(1)
+    e = base.enc
+    n = e.sJ * e.sK * (e.nz + 1)
+    junk = np.frombuffer(np.full(n, 123.456, np.float32).tobytes(), np.uint32)
+    for f in runs.FUSED_SCRATCH:
+        if f not in e.fields:
+            continue
+        r = runs.OracleRun(cfg)
+        o = e.field_offset(f)
+        r.M[o: o + n] = junk
+        r.step(1)
+        for g in e.fields:
+            if g == f:
+                continue
+            og = e.field_offset(g)
+            assert np.array_equal(r.M[og: og + n], base.M[og: og + n]), (f, g)
