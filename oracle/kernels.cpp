@@ -8,6 +8,7 @@
 #include <omp.h>
 #endif
 #include <cstdio>
+#include <cstring>
 #include <stdexcept>
 #include <vector>
 
@@ -109,6 +110,55 @@ static int run_program(uint32_t* w, T* v, const char* const* names, const uint32
       K<T> k1{w, v, X}, k2{w, v, Y};
       if constexpr (sizeof(T) == 4) { e1->f32(k1); e2->f32(k2); }
       else { e1->f64(k1); e2->f64(k2); }
+      continue;
+    }
+    if (nm == "adv_fused") {
+      // the three flux dispatches it stands for, then adv_assemble
+      const uint32_t* dm = w + A[4];
+      const uint32_t nx = dm[0], ny = dm[1], nz = dm[2];
+      const uint32_t kind = w[A[4 + 3]];
+      const uint32_t S = A[4 + 2], RU = A[4 + 13], RV = A[4 + 14], RW = A[4 + 4], C1 = A[4 + 15], C2 = A[4 + 16];
+      const uint32_t ST = A[4 + 1], WE = A[4 + 5], PD = A[4 + 20], PW = A[4 + 21];
+      const uint32_t FX = A[4 + 22], FY = A[4 + 23], FZ = A[4 + 24];
+      struct Slot { const char* k; uint32_t g[3]; std::vector<std::pair<int, uint32_t>> p; };
+      std::vector<Slot> fl;
+      if (kind == 0) {
+        fl = {{"adv_flux_x", {nx + 1, ny, nz}, {{1, S}, {2, RU}, {3, FX}}},
+              {"adv_flux_y", {nx, ny + 1, nz}, {{1, S}, {2, RV}, {3, FY}}},
+              {"adv_flux_z", {nx, ny, nz + 1}, {{1, S}, {2, RW}, {3, C1}, {4, C2}, {5, FZ}}}};
+      } else if (kind == 1) {
+        fl = {{"adv_flux_x_weno", {nx + 1, ny, nz}, {{1, S}, {2, RU}, {3, WE}, {4, PD}, {5, FX}}},
+              {"adv_flux_y_weno", {nx, ny + 1, nz}, {{1, S}, {2, RV}, {3, WE}, {4, PD}, {5, FY}}},
+              {"adv_flux_z_weno", {nx, ny, nz + 1}, {{1, S}, {2, RW}, {3, C1}, {4, C2}, {5, WE}, {6, PD}, {7, PW}, {8, FZ}}}};
+      } else if (kind == 2) {
+        std::vector<std::pair<int, uint32_t>> base = {{1, ST}, {2, S}, {3, RU}, {4, RV}, {5, RW}, {6, C1}, {7, C2}};
+        auto with = [&](uint32_t f) { auto p = base; p.push_back({8, f}); return p; };
+        fl = {{"adv_flux_x_stag", {nx + 2, ny + 2, nz}, with(FX)},
+              {"adv_flux_y_stag", {nx + 2, ny + 2, nz}, with(FY)},
+              {"adv_flux_z_stag", {nx + 1, ny + 1, nz + 1}, with(FZ)}};
+      } else {
+        fl = {{"adv_flux_x_weno_w", {nx + 2, ny + 2, nz}, {{1, S}, {2, RU}, {3, C2}, {4, WE}, {5, FX}}},
+              {"adv_flux_y_weno_w", {nx + 2, ny + 2, nz}, {{1, S}, {2, RV}, {3, C2}, {4, WE}, {5, FY}}},
+              {"adv_flux_z_weno_w", {nx + 1, ny + 1, nz + 1}, {{1, S}, {2, RW}, {3, WE}, {4, FZ}}}};
+      }
+      for (const Slot& sl : fl) {
+        uint32_t X[64] = {0};
+        X[0] = sl.g[0]; X[1] = sl.g[1]; X[2] = sl.g[2];
+        X[4] = A[4];
+        X[34] = A[34];
+        for (auto& pr : sl.p) X[4 + pr.first] = pr.second;
+        const KernelEntry* e1 = find_kernel(sl.k);
+        K<T> k1{w, v, X};
+        if constexpr (sizeof(T) == 4) e1->f32(k1);
+        else e1->f64(k1);
+      }
+      uint32_t Y[64];
+      std::memcpy(Y, A, sizeof(Y));
+      Y[4 + 3] = FX; Y[4 + 4] = FY; Y[4 + 5] = FZ;
+      const KernelEntry* ea = find_kernel("adv_assemble");
+      K<T> ka{w, v, Y};
+      if constexpr (sizeof(T) == 4) ea->f32(ka);
+      else ea->f64(ka);
       continue;
     }
     if (nm == "zero_fields") {  // turb_zero on each of its fields

@@ -5,31 +5,37 @@
 
 namespace te {
 
+// Every flux below is a value function fv_*(..., i, j, k, out): it returns
+// false where the reference kernel writes nothing, else the face value the
+// kernel stores.  The flux kernels store it; the fused advection kernel
+// (k_fused.hip) evaluates the two faces of a cell directly.
+
 // adv_flux_x.wgsl:120-143 / adv_flux_y.wgsl:120-144
 template <int DIR>
-__device__ void flux_h5(uint32_t* M, const KArg& a, int i, int j, int k) {
-  Dims g = dims(M, a.p[0]);
-  if (DIR == 0 ? (i > g.nx || j >= g.ny || k >= g.nz) : (i >= g.nx || j > g.ny || k >= g.nz)) return;
-  const float *s = fp(M, a.p[1]), *rr = fp(M, a.p[2]);
+TE_DEV bool fv_h5(const Dims& g, const float* s, const float* rr, int i, int j, int k, float& out) {
+  if (DIR == 0 ? (i > g.nx || j >= g.ny || k >= g.nz) : (i >= g.nx || j > g.ny || k >= g.nz)) return false;
   const int64_t st = DIR == 0 ? 1 : (int64_t)g.sJ, c = g.idx(i, j, k);
   float u = rr[c], f;
   if (u >= 0.0f) f = flx5(s[c - 3 * st], s[c - 2 * st], s[c - st], s[c], s[c + st]);
   else f = flx5(s[c + 2 * st], s[c + st], s[c], s[c - st], s[c - 2 * st]);
-  fp(M, a.p[3])[c] = u * f;
+  out = u * f;
+  return true;
+}
+template <int DIR>
+__device__ void flux_h5(uint32_t* M, const KArg& a, int i, int j, int k) {
+  Dims g = dims(M, a.p[0]);
+  float f;
+  if (fv_h5<DIR>(g, fp(M, a.p[1]), fp(M, a.p[2]), i, j, k, f)) fp(M, a.p[3])[g.idx(i, j, k)] = f;
 }
 __global__ __launch_bounds__(256) void k_adv_flux_x(uint32_t* M, KArg a) { TE_TID3 flux_h5<0>(M, a, gx, gy, gz); }
 __global__ __launch_bounds__(256) void k_adv_flux_y(uint32_t* M, KArg a) { TE_TID3 flux_h5<1>(M, a, gx, gy, gz); }
 
 // adv_flux_z.wgsl:127-200
-__global__ __launch_bounds__(256) void k_adv_flux_z(uint32_t* M, KArg a) {
-  TE_TID3
-  Dims g = dims(M, a.p[0]);
-  int i = gx, j = gy, k = gz;
-  if (i >= g.nx || j >= g.ny || k > g.nz) return;
-  const float *s = fp(M, a.p[1]), *rrw = fp(M, a.p[2]), *c1 = fp(M, a.p[3]), *c2 = fp(M, a.p[4]);
-  float* fz = fp(M, a.p[5]);
+TE_DEV bool fv_z5(const Dims& g, const float* s, const float* rrw, const float* c1, const float* c2, int i, int j, int k,
+                  float& out) {
+  if (i >= g.nx || j >= g.ny || k > g.nz) return false;
   const int64_t sk = g.sk(), c = g.idx(i, j, k);
-  if (k == 0 || k == g.nz) { fz[c] = 0.0f; return; }
+  if (k == 0 || k == g.nz) { out = 0.0f; return true; }
   float w = rrw[c], f;
   bool up = w >= 0.0f;
   if (k == 1) f = up ? fma_(c1[k], s[c - sk], c2[k] * s[c]) : flx3(s[c + sk], s[c], s[c - sk]);
@@ -37,49 +43,56 @@ __global__ __launch_bounds__(256) void k_adv_flux_z(uint32_t* M, KArg a) {
   else if (k == g.nz - 2) f = up ? flx5(s[c - 3 * sk], s[c - 2 * sk], s[c - sk], s[c], s[c + sk]) : flx3(s[c + sk], s[c], s[c - sk]);
   else if (k == g.nz - 1) f = up ? flx3(s[c - 2 * sk], s[c - sk], s[c]) : fma_(c1[k], s[c - sk], c2[k] * s[c]);
   else f = up ? flx5(s[c - 3 * sk], s[c - 2 * sk], s[c - sk], s[c], s[c + sk]) : flx5(s[c + 2 * sk], s[c + sk], s[c], s[c - sk], s[c - 2 * sk]);
-  fz[c] = w * f;
+  out = w * f;
+  return true;
+}
+__global__ __launch_bounds__(256) void k_adv_flux_z(uint32_t* M, KArg a) {
+  TE_TID3
+  Dims g = dims(M, a.p[0]);
+  float f;
+  if (fv_z5(g, fp(M, a.p[1]), fp(M, a.p[2]), fp(M, a.p[3]), fp(M, a.p[4]), gx, gy, gz, f))
+    fp(M, a.p[5])[g.idx(gx, gy, gz)] = f;
 }
 
 // adv_flux_x_weno.wgsl:158-200 / adv_flux_y_weno.wgsl:158-201
 template <int DIR>
-__device__ void flux_h_weno(uint32_t* M, const KArg& a, int i, int j, int k) {
-  Dims g = dims(M, a.p[0]);
-  if (i > g.nx || j > g.ny || k >= g.nz) return;
-  const float *s = fp(M, a.p[1]), *rr = fp(M, a.p[2]);
-  float weps = ldf(M, a.p[3]);
-  bool pdef = ldi(M, a.p[4]) == 1;
+TE_DEV bool fv_h_weno(const Dims& g, const float* s, const float* rr, float weps, bool pdef, int i, int j, int k,
+                      float& out) {
+  if (i > g.nx || j > g.ny || k >= g.nz) return false;
   const int64_t st = DIR == 0 ? 1 : (int64_t)g.sJ, c = g.idx(i, j, k);
   float s1, s2, s3, s4, s5;
   if (rr[c] >= 0.0f) { s1 = s[c - 3 * st]; s2 = s[c - 2 * st]; s3 = s[c - st]; s4 = s[c]; s5 = s[c + st]; }
   else { s1 = s[c + 2 * st]; s2 = s[c + st]; s3 = s[c]; s4 = s[c - st]; s5 = s[c - 2 * st]; }
-  fp(M, a.p[5])[c] = weno_pdef_ok(s1, s2, s3, s4, s5, weps, pdef) ? rr[c] * weno5(s1, s2, s3, s4, s5, weps) : 0.0f;
+  out = weno_pdef_ok(s1, s2, s3, s4, s5, weps, pdef) ? rr[c] * weno5(s1, s2, s3, s4, s5, weps) : 0.0f;
+  return true;
+}
+template <int DIR>
+__device__ void flux_h_weno(uint32_t* M, const KArg& a, int i, int j, int k) {
+  Dims g = dims(M, a.p[0]);
+  float f;
+  if (fv_h_weno<DIR>(g, fp(M, a.p[1]), fp(M, a.p[2]), ldf(M, a.p[3]), ldi(M, a.p[4]) == 1, i, j, k, f))
+    fp(M, a.p[5])[g.idx(i, j, k)] = f;
 }
 __global__ __launch_bounds__(256) void k_adv_flux_x_weno(uint32_t* M, KArg a) { TE_TID3 flux_h_weno<0>(M, a, gx, gy, gz); }
 __global__ __launch_bounds__(256) void k_adv_flux_y_weno(uint32_t* M, KArg a) { TE_TID3 flux_h_weno<1>(M, a, gx, gy, gz); }
 
-// adv_flux_z_weno.wgsl:174-279
-__global__ __launch_bounds__(256) void k_adv_flux_z_weno(uint32_t* M, KArg a) {
-  TE_TID3
-  Dims g = dims(M, a.p[0]);
-  int i = gx, j = gy, kf = gz + 1;
-  if (i >= g.nx || j >= g.ny || kf > g.nz + 1) return;
-  const float *s = fp(M, a.p[1]), *rrw = fp(M, a.p[2]), *c1 = fp(M, a.p[3]), *c2 = fp(M, a.p[4]);
-  float* fz = fp(M, a.p[8]);
-  float weps = ldf(M, a.p[5]);
-  int pdefweno = ldi(M, a.p[7]);
-  int k = kf - 1;
+// adv_flux_z_weno.wgsl:174-279 (face level k = kf - 1)
+TE_DEV bool fv_z_weno(const Dims& g, const float* s, const float* rrw, const float* c1, const float* c2, float weps,
+                      int pdefweno, int i, int j, int k, float& out) {
+  const int kf = k + 1;
+  if (i >= g.nx || j >= g.ny || k < 0 || kf > g.nz + 1) return false;
   const int64_t sk = g.sk(), c = g.idx(i, j, k);
-  if (kf == 1 || kf == g.nz + 1) { fz[c] = 0.0f; return; }
+  if (kf == 1 || kf == g.nz + 1) { out = 0.0f; return true; }
 #define L(n) s[c + (n) * sk]
   float w = rrw[c];
   const int nk = g.nz;
   if (kf == 2) {
-    fz[c] = (w >= 0.0f) ? w * fma_(c1[k], L(-1), c2[k] * L(0)) : w * upstrpd(L(1), L(0), L(-1), weps);
-    return;
+    out = (w >= 0.0f) ? w * fma_(c1[k], L(-1), c2[k] * L(0)) : w * upstrpd(L(1), L(0), L(-1), weps);
+    return true;
   }
   if (kf == nk) {
-    fz[c] = (w > 0.0f) ? w * upstrpd(L(-2), L(-1), L(0), weps) : w * fma_(c1[k], L(-1), c2[k] * L(0));
-    return;
+    out = (w > 0.0f) ? w * upstrpd(L(-2), L(-1), L(0), weps) : w * fma_(c1[k], L(-1), c2[k] * L(0));
+    return true;
   }
   float s1, s2, s3, s4, s5;
   if (w >= 0.0f) { s1 = L(-3); s2 = L(-2); s3 = L(-1); s4 = L(0); s5 = L(1); }
@@ -88,19 +101,28 @@ __global__ __launch_bounds__(256) void k_adv_flux_z_weno(uint32_t* M, KArg a) {
     bool doit = weno_pdef_ok(s1, s2, s3, s4, s5, weps, true);
     if (kf == 3 && w > 0.0f) doit = false;
     if (kf == nk - 1 && w < 0.0f) doit = false;
-    if (doit) { fz[c] = w * weno5(s1, s2, s3, s4, s5, weps); return; }
-    if (kf != 3 && kf != nk - 1) { fz[c] = 0.0f; return; }
+    if (doit) { out = w * weno5(s1, s2, s3, s4, s5, weps); return true; }
+    if (kf != 3 && kf != nk - 1) { out = 0.0f; return true; }
   }
   if (kf == 3) {
-    fz[c] = (w >= 0.0f) ? w * upstrpd(L(-2), L(-1), L(0), weps) : w * weno5(L(2), L(1), L(0), L(-1), L(-2), weps);
-    return;
+    out = (w >= 0.0f) ? w * upstrpd(L(-2), L(-1), L(0), weps) : w * weno5(L(2), L(1), L(0), L(-1), L(-2), weps);
+    return true;
   }
   if (kf == nk - 1) {
-    fz[c] = (w >= 0.0f) ? w * weno5(L(-3), L(-2), L(-1), L(0), L(1), weps) : w * upstrpd(L(1), L(0), L(-1), weps);
-    return;
+    out = (w >= 0.0f) ? w * weno5(L(-3), L(-2), L(-1), L(0), L(1), weps) : w * upstrpd(L(1), L(0), L(-1), weps);
+    return true;
   }
-  fz[c] = w * weno5(s1, s2, s3, s4, s5, weps);
+  out = w * weno5(s1, s2, s3, s4, s5, weps);
+  return true;
 #undef L
+}
+__global__ __launch_bounds__(256) void k_adv_flux_z_weno(uint32_t* M, KArg a) {
+  TE_TID3
+  Dims g = dims(M, a.p[0]);
+  float f;
+  if (fv_z_weno(g, fp(M, a.p[1]), fp(M, a.p[2]), fp(M, a.p[3]), fp(M, a.p[4]), ldf(M, a.p[5]), ldi(M, a.p[7]), gx, gy,
+                gz, f))
+    fp(M, a.p[8])[g.idx(gx, gy, gz)] = f;
 }
 
 TE_DEV float flx5_upwind(const float* s, int64_t c, int64_t st, bool up, bool shifted) {
@@ -118,35 +140,35 @@ TE_DEV float weno5_upwind(const float* s, int64_t c, int64_t st, bool up, bool s
             : weno5(s[c + 2 * st], s[c + st], s[c], s[c - st], s[c - 2 * st], weps);
 }
 
-// adv_flux_x_stag.wgsl:144-186 / adv_flux_y_stag.wgsl
+// adv_flux_x_stag.wgsl:144-186 / adv_flux_y_stag.wgsl (rr = rru for x, rrv for y)
 template <int DIR>
-__device__ void flux_h_stag(uint32_t* M, const KArg& a, int i, int j, int k) {
-  Dims g = dims(M, a.p[0]);
-  if (i > g.nx || j > g.ny || k >= g.nz) return;
-  int stag = ldi(M, a.p[1]);
-  const float *s = fp(M, a.p[2]), *rr = fp(M, a.p[DIR == 0 ? 3 : 4]), *c2 = fp(M, a.p[7]);
-  float* fl = fp(M, a.p[8]);
+TE_DEV bool fv_h_stag(const Dims& g, int stag, const float* s, const float* rr, const float* c2, int i, int j, int k,
+                      float& out) {
+  if (i > g.nx || j > g.ny || k >= g.nz) return false;
   const int64_t sj = g.sJ, sk = g.sk(), st = DIR == 0 ? 1 : sj, c = g.idx(i, j, k);
-  if (stag == 4 && k == 0) { fl[c] = 0.0f; return; }
+  if (stag == 4 && k == 0) { out = 0.0f; return true; }
   float m;
   if (stag == 2) m = 0.5f * (rr[c] + rr[c + (DIR == 0 ? 1 : -1)]);
   else if (stag == 3) m = 0.5f * (rr[c] + rr[c + (DIR == 0 ? -sj : sj)]);
   else if (stag == 4) { float cc2 = c2[k]; m = fma_(cc2, rr[c], (1.0f - cc2) * rr[c - sk]); }
   else m = rr[c];
-  fl[c] = m * flx5_upwind(s, c, st, m >= 0.0f, stag == (DIR == 0 ? 2 : 3));
+  out = m * flx5_upwind(s, c, st, m >= 0.0f, stag == (DIR == 0 ? 2 : 3));
+  return true;
+}
+template <int DIR>
+__device__ void flux_h_stag(uint32_t* M, const KArg& a, int i, int j, int k) {
+  Dims g = dims(M, a.p[0]);
+  float f;
+  if (fv_h_stag<DIR>(g, ldi(M, a.p[1]), fp(M, a.p[2]), fp(M, a.p[DIR == 0 ? 3 : 4]), fp(M, a.p[7]), i, j, k, f))
+    fp(M, a.p[8])[g.idx(i, j, k)] = f;
 }
 __global__ __launch_bounds__(256) void k_adv_flux_x_stag(uint32_t* M, KArg a) { TE_TID3 flux_h_stag<0>(M, a, gx - 1, gy - 1, gz); }
 __global__ __launch_bounds__(256) void k_adv_flux_y_stag(uint32_t* M, KArg a) { TE_TID3 flux_h_stag<1>(M, a, gx - 1, gy - 1, gz); }
 
 // adv_flux_z_stag.wgsl:127-268
-__global__ __launch_bounds__(256) void k_adv_flux_z_stag(uint32_t* M, KArg a) {
-  TE_TID3
-  Dims g = dims(M, a.p[0]);
-  int i = gx, j = gy, k = gz;
-  if (i > g.nx || j > g.ny || k > g.nz) return;
-  int stag = ldi(M, a.p[1]);
-  const float *s = fp(M, a.p[2]), *rrw = fp(M, a.p[5]), *c1 = fp(M, a.p[6]), *c2 = fp(M, a.p[7]);
-  float* fz = fp(M, a.p[8]);
+TE_DEV bool fv_z_stag(const Dims& g, int stag, const float* s, const float* rrw, const float* c1, const float* c2, int i,
+                      int j, int k, float& out) {
+  if (i > g.nx || j > g.ny || k > g.nz) return false;
   const int64_t sj = g.sJ, sk = g.sk(), c = g.idx(i, j, k);
   float m;
   if (stag == 2) m = 0.5f * (rrw[c] + rrw[c - 1]);
@@ -156,14 +178,14 @@ __global__ __launch_bounds__(256) void k_adv_flux_z_stag(uint32_t* M, KArg a) {
 #define A_(n) s[c + (n) * sk]
   float f;
   if (stag == 4) {
-    if (k > g.nz - 1) { fz[c] = 0.0f; return; }
+    if (k > g.nz - 1) { out = 0.0f; return true; }
     if (k == 0) f = up ? 0.5f * (A_(0) + A_(1)) : flx3(A_(2), A_(1), A_(0));
     else if (k == 1) f = up ? flx3(A_(-1), A_(0), A_(1)) : flx5(A_(3), A_(2), A_(1), A_(0), A_(-1));
     else if (k == g.nz - 2) f = up ? flx5(A_(-2), A_(-1), A_(0), A_(1), A_(2)) : flx3(A_(2), A_(1), A_(0));
     else if (k == g.nz - 1) f = up ? flx3(A_(-1), A_(0), A_(1)) : 0.5f * (A_(0) + A_(1));
     else f = up ? flx5(A_(-2), A_(-1), A_(0), A_(1), A_(2)) : flx5(A_(3), A_(2), A_(1), A_(0), A_(-1));
   } else {
-    if (k == 0 || k == g.nz) { fz[c] = 0.0f; return; }
+    if (k == 0 || k == g.nz) { out = 0.0f; return true; }
     if (k == 1) f = up ? fma_(c1[k], A_(-1), c2[k] * A_(0)) : flx3(A_(1), A_(0), A_(-1));
     else if (k == 2) f = up ? flx3(A_(-2), A_(-1), A_(0)) : flx5(A_(2), A_(1), A_(0), A_(-1), A_(-2));
     else if (k == g.nz - 2) f = up ? flx5(A_(-3), A_(-2), A_(-1), A_(0), A_(1)) : flx3(A_(1), A_(0), A_(-1));
@@ -171,38 +193,45 @@ __global__ __launch_bounds__(256) void k_adv_flux_z_stag(uint32_t* M, KArg a) {
     else f = up ? flx5(A_(-3), A_(-2), A_(-1), A_(0), A_(1)) : flx5(A_(2), A_(1), A_(0), A_(-1), A_(-2));
   }
 #undef A_
-  fz[c] = m * f;
+  out = m * f;
+  return true;
+}
+__global__ __launch_bounds__(256) void k_adv_flux_z_stag(uint32_t* M, KArg a) {
+  TE_TID3
+  Dims g = dims(M, a.p[0]);
+  float f;
+  if (fv_z_stag(g, ldi(M, a.p[1]), fp(M, a.p[2]), fp(M, a.p[5]), fp(M, a.p[6]), fp(M, a.p[7]), gx, gy, gz, f))
+    fp(M, a.p[8])[g.idx(gx, gy, gz)] = f;
 }
 
 // adv_flux_x_weno_w.wgsl:143-175 / adv_flux_y_weno_w.wgsl
 template <int DIR>
-__device__ void flux_h_weno_w(uint32_t* M, const KArg& a, int i, int j, int k) {
-  Dims g = dims(M, a.p[0]);
-  if (i < 0 || j < 0 || i > g.nx || j > g.ny || k >= g.nz) return;
-  const float *s = fp(M, a.p[1]), *rr = fp(M, a.p[2]), *c2 = fp(M, a.p[3]);
-  float weps = ldf(M, a.p[4]);
-  float* fl = fp(M, a.p[5]);
+TE_DEV bool fv_h_weno_w(const Dims& g, const float* s, const float* rr, const float* c2, float weps, int i, int j, int k,
+                        float& out) {
+  if (i < 0 || j < 0 || i > g.nx || j > g.ny || k >= g.nz) return false;
   const int64_t sk = g.sk(), st = DIR == 0 ? 1 : (int64_t)g.sJ, c = g.idx(i, j, k);
-  if (k == 0) { fl[c] = 0.0f; return; }
+  if (k == 0) { out = 0.0f; return true; }
   float cc2 = c2[k], cc1 = 1.0f - cc2;
   float ub = fma_(cc2, rr[c], cc1 * rr[c - sk]);
-  if (ub >= 0.0f) fl[c] = ub * weno5(s[c - 3 * st], s[c - 2 * st], s[c - st], s[c], s[c + st], weps);
-  else fl[c] = ub * weno5(s[c + 2 * st], s[c + st], s[c], s[c - st], s[c - 2 * st], weps);
+  if (ub >= 0.0f) out = ub * weno5(s[c - 3 * st], s[c - 2 * st], s[c - st], s[c], s[c + st], weps);
+  else out = ub * weno5(s[c + 2 * st], s[c + st], s[c], s[c - st], s[c - 2 * st], weps);
+  return true;
+}
+template <int DIR>
+__device__ void flux_h_weno_w(uint32_t* M, const KArg& a, int i, int j, int k) {
+  Dims g = dims(M, a.p[0]);
+  float f;
+  if (fv_h_weno_w<DIR>(g, fp(M, a.p[1]), fp(M, a.p[2]), fp(M, a.p[3]), ldf(M, a.p[4]), i, j, k, f))
+    fp(M, a.p[5])[g.idx(i, j, k)] = f;
 }
 __global__ __launch_bounds__(256) void k_adv_flux_x_weno_w(uint32_t* M, KArg a) { TE_TID3 flux_h_weno_w<0>(M, a, gx - 1, gy - 1, gz); }
 __global__ __launch_bounds__(256) void k_adv_flux_y_weno_w(uint32_t* M, KArg a) { TE_TID3 flux_h_weno_w<1>(M, a, gx - 1, gy - 1, gz); }
 
 // adv_flux_z_weno_w.wgsl:159-231
-__global__ __launch_bounds__(256) void k_adv_flux_z_weno_w(uint32_t* M, KArg a) {
-  TE_TID3
-  Dims g = dims(M, a.p[0]);
-  int i = gx, j = gy, k = gz;
-  if (i > g.nx || j > g.ny || k > g.nz) return;
-  const float *s = fp(M, a.p[1]), *rrw = fp(M, a.p[2]);
-  float weps = ldf(M, a.p[3]);
-  float* fz = fp(M, a.p[4]);
+TE_DEV bool fv_z_weno_w(const Dims& g, const float* s, const float* rrw, float weps, int i, int j, int k, float& out) {
+  if (i > g.nx || j > g.ny || k > g.nz) return false;
   const int64_t sk = g.sk(), c = g.idx(i, j, k);
-  if (k > g.nz - 1) { fz[c] = 0.0f; return; }
+  if (k > g.nz - 1) { out = 0.0f; return true; }
 #define L(n) s[c + (n) * sk]
   float wbar = 0.5f * (rrw[c] + rrw[c + sk]);
   const int nz = g.nz;
@@ -213,7 +242,14 @@ __global__ __launch_bounds__(256) void k_adv_flux_z_weno_w(uint32_t* M, KArg a) 
   else if (k == nz - 2) f = (wbar > 0.0f) ? weno5(L(-2), L(-1), L(0), L(1), L(2), weps) : upstrpd(L(2), L(1), L(0), weps);
   else f = (wbar >= 0.0f) ? weno5(L(-2), L(-1), L(0), L(1), L(2), weps) : weno5(L(3), L(2), L(1), L(0), L(-1), weps);
 #undef L
-  fz[c] = wbar * f;
+  out = wbar * f;
+  return true;
+}
+__global__ __launch_bounds__(256) void k_adv_flux_z_weno_w(uint32_t* M, KArg a) {
+  TE_TID3
+  Dims g = dims(M, a.p[0]);
+  float f;
+  if (fv_z_weno_w(g, fp(M, a.p[1]), fp(M, a.p[2]), ldf(M, a.p[3]), gx, gy, gz, f)) fp(M, a.p[4])[g.idx(gx, gy, gz)] = f;
 }
 
 // adv_flux_x_weno_uv.wgsl:168-188 / adv_flux_y_weno_uv.wgsl
@@ -260,10 +296,8 @@ __global__ __launch_bounds__(256) void k_adv_flux_z_weno_uv(uint32_t* M, KArg a)
 
 // adv_assemble.wgsl:139-279: flux divergence + mass-divergence term, divided
 // by the reference density; open-boundary inflow corrections included.
-__global__ __launch_bounds__(256) void k_adv_assemble(uint32_t* M, KArg a) {
-  TE_TID3
-  Dims g = dims(M, a.p[0]);
-  int i = gx, j = gy, k = gz;
+template <class FLUX>
+TE_DEV void assemble_at(uint32_t* M, const KArg& a, const Dims& g, int i, int j, int k, FLUX F) {
   int stag = ldi(M, a.p[1]);
   int st = stag < 0 ? -stag : stag;
   int iMax = st == 2 ? g.nx : g.nx - 1, jMax = st == 3 ? g.ny : g.ny - 1, kLo = st == 4 ? 1 : 0, kMax = g.nz - 1;
@@ -271,16 +305,16 @@ __global__ __launch_bounds__(256) void k_adv_assemble(uint32_t* M, KArg a) {
   int openx = ldi(M, a.p[17]), openy = ldi(M, a.p[17] + 1);
   if (openx != 0 && st == 2 && (i == 0 || i == g.nx)) return;
   if (openy != 0 && st == 3 && (j == 0 || j == g.ny)) return;
-  const float *s = fp(M, a.p[2]), *fx = fp(M, a.p[3]), *fy = fp(M, a.p[4]), *fz = fp(M, a.p[5]), *divx = fp(M, a.p[6]),
+  const float *s = fp(M, a.p[2]), *divx = fp(M, a.p[6]),
               *rho0 = fp(M, a.p[7]), *rf0 = fp(M, a.p[8]), *mh = fp(M, a.p[9]), *mf = fp(M, a.p[10]),
               *rru = fp(M, a.p[13]), *rrv = fp(M, a.p[14]), *c1 = fp(M, a.p[15]), *c2 = fp(M, a.p[16]),
               *hm = fp(M, a.p[30]);
   float* tend = fp(M, a.p[11]);
   int accumulate = ldi(M, a.p[12]);
   const int64_t sj = g.sJ, sk = g.sk(), c = g.idx(i, j, k);
-  float dFx = st == 2 ? fx[c] - fx[c - 1] : fx[c + 1] - fx[c];
-  float dFy = st == 3 ? fy[c] - fy[c - sj] : fy[c + sj] - fy[c];
-  float dFz = st == 4 ? fz[c] - fz[c - sk] : fz[c + sk] - fz[c];
+  float dFx = st == 2 ? F(0, i, j, k) - F(0, i - 1, j, k) : F(0, i + 1, j, k) - F(0, i, j, k);
+  float dFy = st == 3 ? F(1, i, j, k) - F(1, i, j - 1, k) : F(1, i, j + 1, k) - F(1, i, j, k);
+  float dFz = st == 4 ? F(2, i, j, k) - F(2, i, j, k - 1) : F(2, i, j, k + 1) - F(2, i, j, k);
   float mz = stag < 0 ? 1.0f : (stag == 4 ? mf[k] : mh[k]);
   float dv;
   if (st == 2) dv = 0.5f * (divx[c] + divx[c - 1]);
@@ -333,6 +367,51 @@ __global__ __launch_bounds__(256) void k_adv_assemble(uint32_t* M, KArg a) {
   float rho = st == 4 ? rf0[k] : rho0[k];
   float t = fma_(s[c], dv, -divF) / rho;
   tend[c] = accumulate != 0 ? tend[c] + t : t;
+}
+
+__global__ __launch_bounds__(256) void k_adv_assemble(uint32_t* M, KArg a) {
+  TE_TID3
+  Dims g = dims(M, a.p[0]);
+  const float *fx = fp(M, a.p[3]), *fy = fp(M, a.p[4]), *fz = fp(M, a.p[5]);
+  assemble_at(M, a, g, gx, gy, gz, [&](int d, int i, int j, int k) {
+    const int64_t c = g.idx(i, j, k);
+    return d == 0 ? fx[c] : (d == 1 ? fy[c] : fz[c]);
+  });
+}
+
+// The three flux dispatches of one advected field and adv_assemble in one
+// launch: each cell evaluates the face fluxes its divergence needs with the
+// flux kernels' own value functions instead of reading them back from
+// fx/fy/fz (which are then left unwritten; they are dead scratch).  Two
+// neighbouring cells evaluate a shared face identically.
+// p: adv_assemble's parameters, with 3 -> flux kind (0 flx5, 1 WENO5,
+// 2 staggered flx5, 3 WENO5 of w-staggered fields), 4 rrw, 5 -> weps,
+// 20 -> pdef, 21 -> pdefweno (22-24: fx, fy, fz of the unfused dispatches).
+__global__ __launch_bounds__(256) void k_adv_fused(uint32_t* M, KArg a) {
+  TE_TID3
+  Dims g = dims(M, a.p[0]);
+  const int kind = ldi(M, a.p[3]), stag = ldi(M, a.p[1]);
+  const float *s = fp(M, a.p[2]), *rru = fp(M, a.p[13]), *rrv = fp(M, a.p[14]), *rrw = fp(M, a.p[4]),
+              *c1 = fp(M, a.p[15]), *c2 = fp(M, a.p[16]);
+  const float weps = ldf(M, a.p[5]);
+  const int pdef = ldi(M, a.p[20]), pdw = ldi(M, a.p[21]);
+  assemble_at(M, a, g, gx, gy, gz, [&](int d, int i, int j, int k) {
+    float f = __builtin_nanf("");
+    bool ok;
+    if (kind == 0) ok = d == 0 ? fv_h5<0>(g, s, rru, i, j, k, f) : d == 1 ? fv_h5<1>(g, s, rrv, i, j, k, f)
+                                                                        : fv_z5(g, s, rrw, c1, c2, i, j, k, f);
+    else if (kind == 1)
+      ok = d == 0 ? fv_h_weno<0>(g, s, rru, weps, pdef == 1, i, j, k, f)
+                  : d == 1 ? fv_h_weno<1>(g, s, rrv, weps, pdef == 1, i, j, k, f)
+                           : fv_z_weno(g, s, rrw, c1, c2, weps, pdw, i, j, k, f);
+    else if (kind == 2)
+      ok = d == 0 ? fv_h_stag<0>(g, stag, s, rru, c2, i, j, k, f)
+                  : d == 1 ? fv_h_stag<1>(g, stag, s, rrv, c2, i, j, k, f) : fv_z_stag(g, stag, s, rrw, c1, c2, i, j, k, f);
+    else
+      ok = d == 0 ? fv_h_weno_w<0>(g, s, rru, c2, weps, i, j, k, f)
+                  : d == 1 ? fv_h_weno_w<1>(g, s, rrv, c2, weps, i, j, k, f) : fv_z_weno_w(g, s, rrw, weps, i, j, k, f);
+    return ok ? f : __builtin_nanf("");  // a face no flux dispatch writes: poisons the result visibly
+  });
 }
 
 }  // namespace te

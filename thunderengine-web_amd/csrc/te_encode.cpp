@@ -200,6 +200,10 @@ class Encoder {
   void turbulence();
   void advect(int nrk, const std::string& s, int stagger, const std::string& into, bool accumulate,
               const std::string& state);
+  void advectFluxes(int nrk, const std::string& s, int stagger, const std::string& target);
+  void afterAssemble(int nrk, const std::string& s, int stagger, const std::string& into, bool accumulate,
+                     const std::string& target, const std::array<int, 3>& asmSize, const Bytes& dttmp,
+                     const Bytes& open);
   void stages();
   void end();
 };
@@ -460,6 +464,56 @@ void Encoder::advect(int nrk, const std::string& s, int stagger, const std::stri
   const Bytes open = I2(openX ? 1 : 0, openY ? 1 : 0);
   const Bytes dttmp = F1((float)((double)dtf / (3 - nrk)));
   const std::string target = state.empty() ? s : state;
+  const std::array<int, 3> asmSize{nx + (stagger == 2 ? 1 : 0), ny + (stagger == 3 ? 1 : 0), nz};
+  const bool pdefAfter = !accumulate && stagger == 1 && moist && nrk == 2 && target != scalars[0];
+  const bool wenoUVm = (stagger == 2 || stagger == 3) && wenoM && nrk == 2;
+  if (cfg.fuse != 0 && !pdefAfter && !wenoUVm) {
+    // fluxes + assembly in one launch (pdef needs the fluxes in memory; the
+    // WENO momentum fluxes of u and v keep their own dispatches)
+    int kind;
+    Bytes weps = F1(0.0f), pdef = I1(0), pdw = I1(0);
+    if (stagger == 1) {
+      const bool isTheta = target == scalars[0];
+      if (wenoS && nrk == 2) {
+        kind = 1;
+        weps = F1(wenoEps(target));
+        pdef = I1(isTheta ? 0 : 1);
+        pdw = I1((isTheta || target == "qv") ? 0 : 1);
+      } else {
+        kind = 0;
+      }
+    } else {
+      const bool wenoTke = stagger == 4 && s == "tke" && wenoS && nrk == 2;
+      const bool wenoW = stagger == 4 && s == "w" && wenoM && nrk == 2;
+      if (wenoTke || wenoW) {
+        kind = 3;
+        weps = F1(wenoW ? (float)(100.0 * 1.0e-18) : wenoEps("tke"));
+      } else {
+        kind = 2;
+      }
+    }
+    std::vector<Bind> bs = {bb(1, I1(stagger)), fb(2, s), bb(3, I1(kind)), fb(4, "rrw"), bb(5, weps), fb(6, "divx"),
+                            rb(7, "rho0"), rb(8, "rf0"), rb(9, "mh"), rb(10, "mf"), fb(11, into),
+                            bb(12, I1(accumulate ? 1 : 0)), fb(13, "rru"), fb(14, "rrv"), rb(15, "c1"), rb(16, "c2"),
+                            bb(17, open), rb(18, "rr0"), fb(19, "tend"), bb(20, pdef), bb(21, pdw), fb(22, "fx"),
+                            fb(23, "fy"), fb(24, "fz"), bb(27, F1(0.0f))};
+    op("adv_fused", asmSize, {});
+    for (const Bind& bd : bs) E.ops.back().A[4 + bd.idx] = (uint32_t)bd.off;
+  } else {
+    advectFluxes(nrk, s, stagger, target);
+    std::vector<Bind> bs = {bb(1, I1(stagger)), fb(2, s), fb(3, "fx"), fb(4, "fy"), fb(5, "fz"), fb(6, "divx"),
+                            rb(7, "rho0"), rb(8, "rf0"), rb(9, "mh"), rb(10, "mf"), fb(11, into),
+                            bb(12, I1(accumulate ? 1 : 0)), fb(13, "rru"), fb(14, "rrv"), rb(15, "c1"), rb(16, "c2"),
+                            bb(17, open), rb(18, "rr0"), fb(19, "tend"), bb(27, F1(0.0f))};
+    op("adv_assemble", asmSize, {});
+    for (const Bind& bd : bs) E.ops.back().A[4 + bd.idx] = (uint32_t)bd.off;
+  }
+  afterAssemble(nrk, s, stagger, into, accumulate, target, asmSize, dttmp, open);
+}
+
+void Encoder::advectFluxes(int nrk, const std::string& s, int stagger, const std::string& target) {
+  const int nx = cfg.nx, ny = cfg.ny, nz = cfg.nz;
+  const std::array<int, 3> fullU{nx + 1, ny, nz}, fullV{nx, ny + 1, nz}, fullW{nx, ny, nz + 1};
   if (stagger == 1) {
     const bool isTheta = target == scalars[0];
     if (wenoS && nrk == 2) {
@@ -499,15 +553,13 @@ void Encoder::advect(int nrk, const std::string& s, int stagger, const std::stri
       }
     }
   }
-  const std::array<int, 3> asmSize{nx + (stagger == 2 ? 1 : 0), ny + (stagger == 3 ? 1 : 0), nz};
-  {
-    std::vector<Bind> bs = {bb(1, I1(stagger)), fb(2, s), fb(3, "fx"), fb(4, "fy"), fb(5, "fz"), fb(6, "divx"),
-                            rb(7, "rho0"), rb(8, "rf0"), rb(9, "mh"), rb(10, "mf"), fb(11, into),
-                            bb(12, I1(accumulate ? 1 : 0)), fb(13, "rru"), fb(14, "rrv"), rb(15, "c1"), rb(16, "c2"),
-                            bb(17, open), rb(18, "rr0"), fb(19, "tend"), bb(27, F1(0.0f))};
-    op("adv_assemble", asmSize, {});
-    for (const Bind& bd : bs) E.ops.back().A[4 + bd.idx] = (uint32_t)bd.off;
-  }
+}
+
+void Encoder::afterAssemble(int nrk, const std::string& s, int stagger, const std::string& into, bool accumulate,
+                            const std::string& target, const std::array<int, 3>& asmSize, const Bytes& dttmp,
+                            const Bytes& open) {
+  const int nx = cfg.nx, ny = cfg.ny, nz = cfg.nz;
+  const std::array<int, 3> full{nx, ny, nz}, whole{nx + 7, ny + 7, nz + 1}, stag{nx + 1, ny + 1, nz}, cols{nx, ny, 1};
   if (accumulate) return;
   if (stagger == 1 && moist && nrk == 2 && target != scalars[0]) {
     const std::string s0 = target + "_a";
