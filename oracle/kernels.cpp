@@ -161,6 +161,21 @@ static int run_program(uint32_t* w, T* v, const char* const* names, const uint32
       else ea->f64(ka);
       continue;
     }
+    if (nm == "add_fields") {  // turb_add_tendency on each triple, in order
+      const KernelEntry* ea = find_kernel("turb_add_tendency");
+      const uint32_t nf = w[A[4 + 13]];
+      for (uint32_t q = 0; q < nf && q < 4; ++q) {
+        uint32_t X[64] = {0};
+        X[0] = A[0]; X[1] = A[1]; X[2] = A[2];
+        X[4] = A[4];
+        X[5] = A[5 + q]; X[6] = A[9 + q]; X[7] = A[13 + q];
+        X[34] = A[34];
+        K<T> kx{w, v, X};
+        if constexpr (sizeof(T) == 4) ea->f32(kx);
+        else ea->f64(kx);
+      }
+      continue;
+    }
     if (nm == "zero_fields") {  // turb_zero on each of its fields
       const KernelEntry* ez = find_kernel("turb_zero");
       const uint32_t nf = w[A[4 + 9]];

@@ -58,6 +58,23 @@ __global__ __launch_bounds__(256) void k_zero_fields(uint32_t* M, KArg a) {
   }
 }
 
+// turb_add_tendency (dst = fma(scale, src, dst) over the whole grid) for up
+// to 4 independent triples: p[1..4] sources, p[5..8] destinations,
+// p[9..12] -> scales, p[13] -> count.
+__global__ __launch_bounds__(256) void k_add_fields(uint32_t* M, KArg a) {
+  TE_TID3
+  Dims g = dims(M, a.p[0]);
+  int i = gx - 3, j = gy - 3, k = gz;
+  if (i > g.nx + 3 || j > g.ny + 3 || k > g.nz) return;
+  const int64_t c = g.idx(i, j, k);
+  const int n = ldi(M, a.p[13]);
+  for (int q = 0; q < 4; ++q) {
+    if (q >= n) break;
+    float* dst = fp(M, a.p[5 + q]);
+    dst[c] = fma_(ldf(M, a.p[9 + q]), fp(M, a.p[1 + q])[c], dst[c]);
+  }
+}
+
 // Up to 4 buffer copies of a.n[0] words each (the state swap of
 // timestep.js:766-774): p[1..4] sources, p[5..8] destinations, p[9] -> count.
 __global__ __launch_bounds__(256) void k_copy_fields(uint32_t* M, KArg a) {

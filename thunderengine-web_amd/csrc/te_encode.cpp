@@ -137,6 +137,31 @@ class Encoder {
       E.names.push_back("zero_fields");
     }
   }
+  // turb_add_tendency runs queue likewise: up to 4 independent (src, dst,
+  // scale) triples per add_fields launch.
+  struct AddT { uint64_t src, dst, scale; };
+  std::vector<AddT> pendingAdd;
+  void flushAdd() {
+    std::vector<AddT> q;
+    q.swap(pendingAdd);
+    for (size_t s = 0; s < q.size(); s += 4) {
+      const int n = (int)std::min<size_t>(4, q.size() - s);
+      Op o;
+      std::memset(&o, 0, sizeof(o));
+      o.kid = kernel_id("add_fields");
+      o.A[0] = (uint32_t)(cfg.nx + 7); o.A[1] = (uint32_t)(cfg.ny + 7); o.A[2] = (uint32_t)(cfg.nz + 1);
+      o.A[4 + 0] = (uint32_t)dimsOff;
+      o.A[4 + 30] = (uint32_t)hmOff;
+      for (int q2 = 0; q2 < n; ++q2) {
+        o.A[4 + 1 + q2] = (uint32_t)q[s + q2].src;
+        o.A[4 + 5 + q2] = (uint32_t)q[s + q2].dst;
+        o.A[4 + 9 + q2] = (uint32_t)q[s + q2].scale;
+      }
+      o.A[4 + 13] = (uint32_t)bytes(I1(n));
+      E.ops.push_back(o);
+      E.names.push_back("add_fields");
+    }
+  }
   void flushHalo() {
     std::vector<std::string> q;
     q.swap(pendingHalo);
@@ -156,6 +181,17 @@ class Encoder {
   }
 
   void op(const char* kname, std::array<int, 3> grid, std::initializer_list<Bind> binds, bool withAcoustic = false) {
+    if (cfg.fuse != 0 && std::strcmp(kname, "turb_add_tendency") == 0 && binds.size() == 3) {
+      if (!pendingHalo.empty()) flushHalo();
+      if (!pendingZero.empty()) flushZero();
+      uint64_t v[4] = {0, 0, 0, 0};
+      for (const Bind& bd : binds) v[bd.idx] = bd.off;
+      for (const AddT& t : pendingAdd)  // a dependent triple starts a new launch
+        if (t.dst == v[1] || t.src == v[2] || t.dst == v[2]) { flushAdd(); break; }
+      pendingAdd.push_back({v[1], v[2], v[3]});
+      return;
+    }
+    if (!pendingAdd.empty()) flushAdd();
     if (cfg.fuse != 0 && std::strcmp(kname, "turb_zero") == 0 && binds.size() == 1) {
       if (!pendingHalo.empty()) flushHalo();
       const std::string f = fieldName(binds.begin()->off);
@@ -329,6 +365,7 @@ void Encoder::openHalo(const std::string& n) {
 void Encoder::haloFill(const std::string& n) {
   const int H = 3;
   if (mergeHalo) {
+    if (!pendingAdd.empty()) flushAdd();
     if (!pendingZero.empty()) flushZero();
     if (std::find(pendingHalo.begin(), pendingHalo.end(), n) == pendingHalo.end()) pendingHalo.push_back(n);
     return;
@@ -756,6 +793,7 @@ void Encoder::end() {
         fb(8, "qvten"), fb(9, "qcten")});
   op("ppx_finish", full, {fb(1, "pp3d"), fb(2, "ppx")});
   haloFill("ppx");
+  flushAdd();
   flushHalo();
   flushZero();
   // state swap (timestep.js:766-774)

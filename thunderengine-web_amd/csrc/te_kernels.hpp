@@ -23,7 +23,7 @@
 // each replaces a run of reference dispatches with the same result on the
 // fields the step carries.
 #define TE_FUSED_LIST(X) X(bc_periodic_xy) X(zero_fields) X(copy_fields) X(ac_fpk_solve) X(ac_pp_accumulate) \
-  X(adv_fused)
+  X(adv_fused) X(add_fields)
 
 namespace te {
 #define TE_DECL(n) __global__ __launch_bounds__(256) void k_##n(uint32_t* M, KArg a);
