@@ -101,3 +101,14 @@ def test_fused_scratch_fields_are_dead(kw):
                 continue
             og = e.field_offset(g)
             assert np.array_equal(r.M[og: og + n], base.M[og: og + n]), (f, g)
+
+
+def test_fp64_build_is_the_same_step():
+    """The FP64 instantiation of the oracle follows the FP32 one to FP32
+    rounding over a few steps (a kernel whose float and double forms
+    differed would show as an O(1) difference)."""
+    import fp64_error
+    res = fp64_error.run(te_host.config(nx=24, ny=20, nz=12, tke=1, initial_tke=3, rayleigh=1), [1, 3])
+    for steps, row in res.items():
+        for label, e in row.items():
+            assert e["max_abs"] <= 1e-3 * max(1.0, e["max_abs_fp64"]), (steps, label, e)
