@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <array>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <initializer_list>
 #include <map>
@@ -701,13 +702,25 @@ void Encoder::stages() {
            true);
       op("acoustic_uv", stag,
          {fb(2, "ppd"), fb(3, "thv"), fb(4, "uten"), fb(5, "vten"), fb(6, "u"), fb(7, "v"), bb(8, open)}, true);
-      if (cfg.fuse != 0) {  // fpk + w solve in one column sweep, pp + accumulate in one pass;
-                            // accumulate reads no ppd, so ppd's halo joins u, v, w's
-        op("ac_fpk_solve", cols,
-           {fb(2, "u"), fb(3, "v"), fb(4, "w"), fb(5, "piadv"), fb(6, "ppten"), fb(7, "ppterm"), fb(8, "pk1"),
-            fb(9, "pk2"), fb(10, "qk"), fb(11, "fpk"), fb(12, "mm"), fb(13, "wten"), fb(14, "pp3d"), fb(15, "cA"),
-            fb(16, "cB")},
-           true);
+      if (cfg.fuse != 0) {  // pp + accumulate in one pass; accumulate reads no ppd, so ppd's
+                            // halo joins u, v, w's.  fpk + w solve: one column sweep or two launches
+        const char* fs = std::getenv("TE_FPK_SOLVE");
+        if (fs == nullptr || std::atoi(fs) != 0) {
+          op("ac_fpk_solve", cols,
+             {fb(2, "u"), fb(3, "v"), fb(4, "w"), fb(5, "piadv"), fb(6, "ppten"), fb(7, "ppterm"), fb(8, "pk1"),
+              fb(9, "pk2"), fb(10, "qk"), fb(11, "fpk"), fb(12, "mm"), fb(13, "wten"), fb(14, "pp3d"), fb(15, "cA"),
+              fb(16, "cB")},
+             true);
+        } else {
+          op("acoustic_fpk", full,
+             {fb(2, "u"), fb(3, "v"), fb(4, "w"), fb(5, "piadv"), fb(6, "ppten"), fb(7, "ppterm"), fb(8, "pk1"),
+              fb(9, "pk2"), fb(10, "qk"), fb(11, "fpk"), rb(15, "rho0"), rb(16, "rr0"), rb(17, "rf0")},
+             true);
+          op("acoustic_w_solve", cols,
+             {fb(2, "pk1"), fb(3, "pk2"), fb(4, "qk"), fb(5, "mm"), fb(6, "fpk"), fb(7, "wten"), fb(8, "pp3d"),
+              fb(9, "w"), fb(10, "cA"), fb(11, "cB")},
+             true);
+        }
         const float tavg = n == nloop - 1 ? (float)(1.0 / nloop) : 0.0f;
         op("ac_pp_accumulate", whole,
            {fb(2, "w"), fb(3, "pk1"), fb(4, "pk2"), fb(5, "qk"), fb(6, "fpk"), fb(7, "pp3d"), fb(8, "ppd"),

@@ -4,6 +4,7 @@
 // word addressed, one hipMalloc) and launches the program's kernels in order on
 // one HIP stream.  A full step is captured once into a HIP graph and replayed.
 #include <algorithm>
+#include <cstdlib>
 #include <array>
 #include <cstdio>
 #include <cstring>
@@ -55,10 +56,31 @@ static KArg karg_of(const Op& op) {
   return a;
 }
 
+// Column kernels: one lane per (i, j) column sweeping k serially.  A launch
+// has only nx*ny lanes (36 864 at 192 x 192, fewer waves than the chip has
+// SIMDs), so each wave carries TE_COLUMN_LANES columns (default 16) and the
+// chip holds several waves per SIMD to hide the sweeps' memory latency.
+static bool is_column_kernel(int kid) {
+  static const char* const cols[] = {"acoustic_w_solve", "ac_fpk_solve", "turb_scalar_vert_implicit",
+                                     "turb_u_tendency", "turb_v_tendency", "turb_w_tendency", "tke_transport"};
+  for (const char* n : cols)
+    if (std::strcmp(kKernels[kid].name, n) == 0) return true;
+  return false;
+}
+static uint32_t column_lanes() {
+  static uint32_t lanes = [] {
+    const char* e = std::getenv("TE_COLUMN_LANES");
+    int v = e ? std::atoi(e) : 16;
+    return (uint32_t)((v == 8 || v == 16 || v == 32 || v == 64) ? v : 16);
+  }();
+  return lanes;
+}
+
 static hipError_t launch(te_ctx* c, const Op& op) {
   KArg a = karg_of(op);
   if (a.n[0] == 0 || a.n[1] == 0 || a.n[2] == 0) return hipSuccess;
   dim3 b = block_for(a.n);
+  if (a.n[2] == 1 && is_column_kernel(op.kid)) b = dim3(column_lanes(), 1, 1);
   dim3 g((a.n[0] + b.x - 1) / b.x, (a.n[1] + b.y - 1) / b.y, (a.n[2] + b.z - 1) / b.z);
   void* args[] = {(void*)&c->dM, (void*)&a};
   return hipLaunchKernel(kKernels[op.kid].fn, g, b, args, 0, c->stream);
