@@ -117,7 +117,7 @@ def test_schedule_closed_form():
         assert names.count("acoustic_w_solve") == total
         r = te_host.Run(te_host.config(nx=16, ny=16, nz=12, nsound=ns, fuse=1), encode_only=True)
         names, _ = r.program()
-        assert names.count("acoustic_uv") == names.count("ac_fpk_solve") == total
+        assert names.count("acoustic_uv") == names.count("acoustic_w_solve_lds") == names.count("ac_pp_accumulate") == total
 
 
 def test_encoder_rejects_bad_config():

@@ -97,7 +97,8 @@ __global__ __launch_bounds__(256) void k_acoustic_fpk(uint32_t* M, KArg a) {
 // pk2-qk, pp3d, pp3d+fpk) carry in registers: they are the same operations on
 // the same values as the reference's re-evaluation at level k.
 #define WS_CHUNK 8
-__global__ __launch_bounds__(256) void k_acoustic_w_solve(uint32_t* M, KArg a) {
+template <bool LDS>
+__device__ void w_solve(uint32_t* M, const KArg& a) {
   const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x), j = (int)(blockIdx.y * blockDim.y + threadIdx.y);
   if (i >= (int)a.n[0] || j >= (int)a.n[1]) return;
   Dims g = dims(M, a.p[0]);
@@ -109,6 +110,7 @@ __global__ __launch_bounds__(256) void k_acoustic_w_solve(uint32_t* M, KArg a) {
   const int nz = g.nz;
   const int64_t sk = g.sk(), base = g.idx(i, j, 0);
   const float asq = ac.alph * ac.alph;
+  const ColStore<LDS> CA{cA, base, sk, TE_LANE, TE_NLANES, 0}, CB{cB, base, sk, TE_LANE, TE_NLANES, nz};
   // level 0 values (read before w[0] is cleared: w is not among them)
   float pq_m = pk1[base] + qk[base], p2q_m = pk2[base] - qk[base], pp_m = pp3d[base];
   float ppf_m = pp_m + fpk[base];
@@ -141,14 +143,14 @@ __global__ __launch_bounds__(256) void k_acoustic_w_solve(uint32_t* M, KArg a) {
           const float r = 1.0f / bb;
           pa = (-cc) * r;
           pb = dd * r;
-          cA[c] = pa;
-          cB[c] = pb;
+          CA.at(k) = pa;
+          CB.at(k) = pb;
         } else if (k < nz - 1) {
           const float r = 1.0f / fma_(aa, pa, bb);
           pa = (-cc) * r;
           pb = fma_(-aa, pb, dd) * r;
-          cA[c] = pa;
-          cB[c] = pb;
+          CA.at(k) = pa;
+          CB.at(k) = pb;
         } else {
           const float r = 1.0f / fma_(aa, pa, bb);
           pb = fma_(-aa, pb, dd) * r;  // w at the top interior level
@@ -164,7 +166,7 @@ __global__ __launch_bounds__(256) void k_acoustic_w_solve(uint32_t* M, KArg a) {
 #pragma unroll
     for (int t = 0; t < WS_CHUNK; ++t) {
       const int k = k0 - t;
-      if (k >= 1) { ca[t] = cA[base + k * sk]; cb[t] = cB[base + k * sk]; }
+      if (k >= 1) { ca[t] = CA.at(k); cb[t] = CB.at(k); }
     }
 #pragma unroll
     for (int t = 0; t < WS_CHUNK; ++t) {
@@ -176,6 +178,10 @@ __global__ __launch_bounds__(256) void k_acoustic_w_solve(uint32_t* M, KArg a) {
     }
   }
 }
+
+__global__ __launch_bounds__(256) void k_acoustic_w_solve(uint32_t* M, KArg a) { w_solve<false>(M, a); }
+// Same sweep with cA / cB in LDS (2 nz floats per lane), for fused step programs.
+__global__ __launch_bounds__(256) void k_acoustic_w_solve_lds(uint32_t* M, KArg a) { w_solve<true>(M, a); }
 
 // acoustic_pp.wgsl:131-167
 __global__ __launch_bounds__(256) void k_acoustic_pp(uint32_t* M, KArg a) {

@@ -380,22 +380,22 @@ __global__ __launch_bounds__(256) void k_adv_assemble(uint32_t* M, KArg a) {
 }
 
 // The three flux dispatches of one advected field and adv_assemble in one
-// launch: each cell evaluates the face fluxes its divergence needs with the
-// flux kernels' own value functions instead of reading them back from
-// fx/fy/fz (which are then left unwritten; they are dead scratch).  Two
-// neighbouring cells evaluate a shared face identically.
+// launch: each cell evaluates face fluxes with the flux kernels' own value
+// functions instead of reading them back from fx/fy/fz (which are then left
+// unwritten; they are dead scratch).  Every thread publishes its low x and y
+// faces in LDS; the high face is the neighbour's low face, evaluated by the
+// thread itself only at the block's edge.  Both z faces are evaluated per
+// thread.  A face is the same value whichever thread evaluates it.
 // p: adv_assemble's parameters, with 3 -> flux kind (0 flx5, 1 WENO5,
 // 2 staggered flx5, 3 WENO5 of w-staggered fields), 4 rrw, 5 -> weps,
 // 20 -> pdef, 21 -> pdefweno (22-24: fx, fy, fz of the unfused dispatches).
-__global__ __launch_bounds__(256) void k_adv_fused(uint32_t* M, KArg a) {
-  TE_TID3
-  Dims g = dims(M, a.p[0]);
-  const int kind = ldi(M, a.p[3]), stag = ldi(M, a.p[1]);
-  const float *s = fp(M, a.p[2]), *rru = fp(M, a.p[13]), *rrv = fp(M, a.p[14]), *rrw = fp(M, a.p[4]),
-              *c1 = fp(M, a.p[15]), *c2 = fp(M, a.p[16]);
-  const float weps = ldf(M, a.p[5]);
-  const int pdef = ldi(M, a.p[20]), pdw = ldi(M, a.p[21]);
-  assemble_at(M, a, g, gx, gy, gz, [&](int d, int i, int j, int k) {
+struct FusedFlux {
+  Dims g;
+  int kind, stag, pdef, pdw;
+  const float *s, *rru, *rrv, *rrw, *c1, *c2;
+  float weps;
+  // face flux of direction d at (i, j, k); NaN where no flux dispatch writes
+  __device__ float operator()(int d, int i, int j, int k) const {
     float f = __builtin_nanf("");
     bool ok;
     if (kind == 0) ok = d == 0 ? fv_h5<0>(g, s, rru, i, j, k, f) : d == 1 ? fv_h5<1>(g, s, rrv, i, j, k, f)
@@ -411,6 +411,39 @@ __global__ __launch_bounds__(256) void k_adv_fused(uint32_t* M, KArg a) {
       ok = d == 0 ? fv_h_weno_w<0>(g, s, rru, c2, weps, i, j, k, f)
                   : d == 1 ? fv_h_weno_w<1>(g, s, rrv, c2, weps, i, j, k, f) : fv_z_weno_w(g, s, rrw, weps, i, j, k, f);
     return ok ? f : __builtin_nanf("");  // a face no flux dispatch writes: poisons the result visibly
+  }
+};
+
+__global__ __launch_bounds__(256) void k_adv_fused(uint32_t* M, KArg a) {
+  __shared__ float lx[256], ly[256];
+  const int tx = threadIdx.x, ty = threadIdx.y, tz = threadIdx.z;
+  const int i = (int)(blockIdx.x * blockDim.x) + tx, j = (int)(blockIdx.y * blockDim.y) + ty,
+            k = (int)(blockIdx.z * blockDim.z) + tz;
+  const bool inGrid = i < (int)a.n[0] && j < (int)a.n[1] && k < (int)a.n[2];
+  const int t = tx + (int)blockDim.x * (ty + (int)blockDim.y * tz);
+  Dims g = dims(M, a.p[0]);
+  const int stag = ldi(M, a.p[1]), st = stag < 0 ? -stag : stag;
+  const FusedFlux F{g, ldi(M, a.p[3]), stag, ldi(M, a.p[20]), ldi(M, a.p[21]), fp(M, a.p[2]), fp(M, a.p[13]),
+                    fp(M, a.p[14]), fp(M, a.p[4]), fp(M, a.p[15]), fp(M, a.p[16]), ldf(M, a.p[5])};
+  // low faces of this cell's x and y divergence
+  const int xlo = st == 2 ? i - 1 : i, ylo = st == 3 ? j - 1 : j;
+  float fxl = 0.0f, fyl = 0.0f;
+  if (inGrid) {
+    fxl = F(0, xlo, j, k);
+    fyl = F(1, i, ylo, k);
+  }
+  lx[t] = fxl;
+  ly[t] = fyl;
+  __syncthreads();
+  if (!inGrid) return;
+  const bool xNb = tx + 1 < (int)blockDim.x && i + 1 < (int)a.n[0];
+  const bool yNb = ty + 1 < (int)blockDim.y && j + 1 < (int)a.n[1];
+  const float fxh = xNb ? lx[t + 1] : F(0, xlo + 1, j, k);
+  const float fyh = yNb ? ly[t + (int)blockDim.x] : F(1, i, ylo + 1, k);
+  assemble_at(M, a, g, i, j, k, [&](int d, int fi, int fj, int fk) {
+    if (d == 0) return fi == xlo ? fxl : fxh;
+    if (d == 1) return fj == ylo ? fyl : fyh;
+    return F(2, fi, fj, fk);
   });
 }
 

@@ -398,7 +398,7 @@ __global__ __launch_bounds__(256) void k_tke_sources(uint32_t* M, KArg a) {
 }
 
 // turb_u_tendency.wgsl:146-294 / turb_v_tendency.wgsl
-template <int DIR>
+template <int DIR, bool LDS>
 __device__ void uv_tendency(uint32_t* M, const KArg& a, int i, int j) {
   Dims g = dims(M, a.p[0]);
   if (DIR == 0 ? (i > g.nx || j >= g.ny) : (i >= g.nx || j > g.ny)) return;
@@ -436,6 +436,7 @@ __device__ void uv_tendency(uint32_t* M, const KArg& a, int i, int j) {
   float rdt = 0.5f / dt;
   const float hmf = DIR == 0 ? hm_uf(hm, g, i) : hm_vf(hm, g, j), rdh = DIR == 0 ? g.rdx : g.rdy;
   const int nz = g.nz;
+  const ColStore<LDS> CA{cA, base, sk, TE_LANE, TE_NLANES, 0}, CB{cB, base, sk, TE_LANE, TE_NLANES, nz};
   // Every pass runs in chunks of TS_CHUNK levels whose loads are issued before
   // the chunk's arithmetic.  Y(l) = ((kmv_l + kmv_l') * (mf+mf)) * (rf_l + rf_l')
   // is the level-l factor of L1(k, l) = Y(l) * tem(k); Y(k+1) carries to the
@@ -471,8 +472,8 @@ __device__ void uv_tendency(uint32_t* M, const KArg& a, int i, int j) {
     float t = 1.0f / cfb;
     pa = (-cfc) * t;
     pb = cfd * t;
-    cA[base] = pa;
-    cB[base] = pb;
+    CA.at(0) = pa;
+    CB.at(0) = pb;
   }
   for (int k0 = 1; k0 < nz - 1; k0 += TS_CHUNK) {
     float ro0[TS_CHUNK], ro1[TS_CHUNK], mhk[TS_CHUNK], ka[TS_CHUNK], kb[TS_CHUNK], mfn[TS_CHUNK], ra[TS_CHUNK],
@@ -501,8 +502,8 @@ __device__ void uv_tendency(uint32_t* M, const KArg& a, int i, int j) {
         float tt = 1.0f / fma_(cfa, pa, cfb);
         pa = (-cfc) * tt;
         pb = fma_(-cfa, pb, qv[t]) * tt;
-        cA[c] = pa;
-        cB[c] = pb;
+        CA.at(k) = pa;
+        CB.at(k) = pb;
       }
     }
   }
@@ -515,7 +516,7 @@ __device__ void uv_tendency(uint32_t* M, const KArg& a, int i, int j) {
     float cfd = fma_((temc * tC[base + nz * sk]) * (mh[kt] + mh[kt]), rinv(kt), q[ct]);
     float t = 1.0f / fma_(cfa, pa, cfb);
     pb = fma_(-cfa, pb, cfd) * t;
-    cB[ct] = pb;
+    CB.at(kt) = pb;
   }
   // level kt's tendency, then back substitution + tendency for k = kt-1 .. 0
   auto tendency = [&](int k, int64_t c, float cb, float qk, float ro0, float ro1, float d0, float d1, float tx, float ty,
@@ -537,7 +538,7 @@ __device__ void uv_tendency(uint32_t* M, const KArg& a, int i, int j) {
       const int k = k0 - t;
       if (k >= 0) {
         const int64_t c = base + k * sk;
-        ca[t] = cA[c]; cb[t] = cB[c]; qv[t] = q[c]; ro0[t] = rho[c - st]; ro1[t] = rho[c];
+        ca[t] = CA.at(k); cb[t] = CB.at(k); qv[t] = q[c]; ro0[t] = rho[c - st]; ro1[t] = rho[c];
         d0[t] = dwd[c]; d1[t] = dwd[c + sk]; mhk[t] = mh[k]; tv[t] = ten[c];
         turbxy(c, txv[t], tyv[t]);
       }
@@ -548,7 +549,7 @@ __device__ void uv_tendency(uint32_t* M, const KArg& a, int i, int j) {
       if (k >= 0) {
         const int64_t c = base + k * sk;
         pb = fma_(ca[t], pb, cb[t]);
-        cB[c] = pb;
+        CB.at(k) = pb;
         tendency(k, c, pb, qv[t], ro0[t], ro1[t], d0[t], d1[t], txv[t], tyv[t], mhk[t], tv[t]);
       }
     }
@@ -556,11 +557,19 @@ __device__ void uv_tendency(uint32_t* M, const KArg& a, int i, int j) {
 }
 __global__ __launch_bounds__(256) void k_turb_u_tendency(uint32_t* M, KArg a) {
   const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x), j = (int)(blockIdx.y * blockDim.y + threadIdx.y);
-  if (i < (int)a.n[0] && j < (int)a.n[1]) uv_tendency<0>(M, a, i, j);
+  if (i < (int)a.n[0] && j < (int)a.n[1]) uv_tendency<0, false>(M, a, i, j);
+}
+__global__ __launch_bounds__(256) void k_turb_u_tendency_lds(uint32_t* M, KArg a) {
+  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x), j = (int)(blockIdx.y * blockDim.y + threadIdx.y);
+  if (i < (int)a.n[0] && j < (int)a.n[1]) uv_tendency<0, true>(M, a, i, j);
 }
 __global__ __launch_bounds__(256) void k_turb_v_tendency(uint32_t* M, KArg a) {
   const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x), j = (int)(blockIdx.y * blockDim.y + threadIdx.y);
-  if (i < (int)a.n[0] && j < (int)a.n[1]) uv_tendency<1>(M, a, i, j);
+  if (i < (int)a.n[0] && j < (int)a.n[1]) uv_tendency<1, false>(M, a, i, j);
+}
+__global__ __launch_bounds__(256) void k_turb_v_tendency_lds(uint32_t* M, KArg a) {
+  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x), j = (int)(blockIdx.y * blockDim.y + threadIdx.y);
+  if (i < (int)a.n[0] && j < (int)a.n[1]) uv_tendency<1, true>(M, a, i, j);
 }
 
 // Implicit vertical solve shared by turb_w_tendency and tke_transport (both
@@ -570,10 +579,11 @@ __global__ __launch_bounds__(256) void k_turb_v_tendency(uint32_t* M, KArg a) {
 // first / last right-hand sides cfd1 / cfdT given by the caller; the
 // back-substituted solution is handed to out(k, c, x_new, x_old, rf) in one
 // descending sweep (levels nz-1 .. 1).  Loads run TS_CHUNK levels ahead.
-template <class OUT>
+template <class OUT, bool LDS>
 __device__ __forceinline__ void face_implicit(const float* kmh, const float* mh, const float* rho, const float* mf,
-                                              const float* rf, const float* x, float* cA, float* cB, int64_t base,
-                                              int64_t sk, int nz, float tema, float cfd1, float cfdT, OUT out) {
+                                              const float* rf, const float* x, const ColStore<LDS>& CA,
+                                              const ColStore<LDS>& CB, int64_t base, int64_t sk, int nz, float tema,
+                                              float cfd1, float cfdT, OUT out) {
   float Zp = ((kmh[base] + kmh[base + sk]) * mh[0]) * rho[base];  // Z(0)
   float Zk = ((kmh[base + sk] + kmh[base + 2 * sk]) * mh[1]) * rho[base + sk];
   float pa, pb;
@@ -585,8 +595,8 @@ __device__ __forceinline__ void face_implicit(const float* kmh, const float* mh,
     float inv = 1.0f / cfb;
     pa = (-cfc) * inv;
     pb = cfd1 * inv;
-    cA[c] = pa;
-    cB[c] = pb;
+    CA.at(1) = pa;
+    CB.at(1) = pb;
   }
   Zp = Zk;  // Z(1)
   for (int k0 = 2; k0 < nz - 1; k0 += TS_CHUNK) {
@@ -612,8 +622,8 @@ __device__ __forceinline__ void face_implicit(const float* kmh, const float* mh,
         float inv = 1.0f / fma_(cfa, pa, cfb);
         pa = (-cfc) * inv;
         pb = fma_(-cfa, pb, xv[t]) * inv;
-        cA[c] = pa;
-        cB[c] = pb;
+        CA.at(k) = pa;
+        CB.at(k) = pb;
         Zp = Zk;
       }
     }
@@ -627,7 +637,7 @@ __device__ __forceinline__ void face_implicit(const float* kmh, const float* mh,
     float cfb = 1.0f - cfa;
     float inv = 1.0f / fma_(cfa, pa, cfb);
     pb = fma_(-cfa, pb, cfdT) * inv;
-    cB[ct] = pb;
+    CB.at(kt) = pb;
     out(kt, ct, pb, x[ct], rf[ct]);
   }
   for (int k0 = nz - 2; k0 >= 1; k0 -= TS_CHUNK) {
@@ -637,7 +647,7 @@ __device__ __forceinline__ void face_implicit(const float* kmh, const float* mh,
       const int k = k0 - t;
       if (k >= 1) {
         const int64_t c = base + k * sk;
-        ca[t] = cA[c]; cb[t] = cB[c]; xv[t] = x[c]; rfk[t] = rf[c];
+        ca[t] = CA.at(k); cb[t] = CB.at(k); xv[t] = x[c]; rfk[t] = rf[c];
         out.prefetch(t, c);
       }
     }
@@ -647,7 +657,7 @@ __device__ __forceinline__ void face_implicit(const float* kmh, const float* mh,
       if (k >= 1) {
         const int64_t c = base + k * sk;
         pb = fma_(ca[t], pb, cb[t]);
-        cB[c] = pb;
+        CB.at(k) = pb;
         out.at(t, c, pb, xv[t], rfk[t]);
       }
     }
@@ -690,7 +700,8 @@ struct FaceTendency {
 };
 
 // turb_w_tendency.wgsl:152-287
-__global__ __launch_bounds__(256) void k_turb_w_tendency(uint32_t* M, KArg a) {
+template <bool LDS>
+__device__ void w_tendency(uint32_t* M, const KArg& a) {
   const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x), j = (int)(blockIdx.y * blockDim.y + threadIdx.y);
   if (i >= (int)a.n[0] || j >= (int)a.n[1]) return;
   Dims g = dims(M, a.p[0]);
@@ -725,8 +736,11 @@ __global__ __launch_bounds__(256) void k_turb_w_tendency(uint32_t* M, KArg a) {
   const float cfd1 = fma_(-(temc * t33[c1 - sk]), mf[1] / rf[c1], w[c1]);
   const float cfdT = fma_(temc * t33[ct], mf[nz - 1] / rf[ct], w[ct]);
   FaceTendency out{t13, t23, hm, wten, g, i, j, zeroX, zeroY, false, rdt, {}, {}, {}};
-  face_implicit(kmh, mh, rho, mf, rf, w, cA, cB, base, sk, nz, tema, cfd1, cfdT, out);
+  const ColStore<LDS> CA{cA, base, sk, TE_LANE, TE_NLANES, 0}, CB{cB, base, sk, TE_LANE, TE_NLANES, nz};
+  face_implicit(kmh, mh, rho, mf, rf, w, CA, CB, base, sk, nz, tema, cfd1, cfdT, out);
 }
+__global__ __launch_bounds__(256) void k_turb_w_tendency(uint32_t* M, KArg a) { w_tendency<false>(M, a); }
+__global__ __launch_bounds__(256) void k_turb_w_tendency_lds(uint32_t* M, KArg a) { w_tendency<true>(M, a); }
 
 // turb_scalar_flux_h.wgsl:125-148
 __global__ __launch_bounds__(256) void k_turb_scalar_flux_h(uint32_t* M, KArg a) {
@@ -744,7 +758,8 @@ __global__ __launch_bounds__(256) void k_turb_scalar_flux_h(uint32_t* M, KArg a)
 }
 
 // turb_scalar_vert_implicit.wgsl:210-327
-__global__ __launch_bounds__(256) void k_turb_scalar_vert_implicit(uint32_t* M, KArg a) {
+template <bool LDS>
+__device__ void scalar_vert_implicit(uint32_t* M, const KArg& a) {
   const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x), j = (int)(blockIdx.y * blockDim.y + threadIdx.y);
   if (i >= (int)a.n[0] || j >= (int)a.n[1]) return;
   Dims g = dims(M, a.p[0]);
@@ -797,6 +812,7 @@ __global__ __launch_bounds__(256) void k_turb_scalar_vert_implicit(uint32_t* M, 
   // L4(k) = ((X(k) * mh[k]) * rr[k]) and L3(k) = ((X(k+1) * mh[k]) * rr[k]) with
   // X(k) = (khv[k] * mf[k]) * rf[k]: X(k+1) is evaluated once and carried.
   const int nz = g.nz;
+  const ColStore<LDS> CA{cA, base, sk, TE_LANE, TE_NLANES, 0}, CB{cB, base, sk, TE_LANE, TE_NLANES, nz};
   auto X = [&](int k) { const int64_t c = base + k * sk; return (khv[c] * mf[k]) * rf[c]; };
   float pa, pb;
   float Xk1 = X(1);  // X(k+1) for k = 0
@@ -805,7 +821,7 @@ __global__ __launch_bounds__(256) void k_turb_scalar_vert_implicit(uint32_t* M, 
     float cfb = 1.0f - cfc;
     float t = 1.0f / cfb;
     pa = (-cfc) * t;
-    cA[base] = pa;
+    CA.at(0) = pa;
     float cfd = s[base];
     if (useFlux != 0 || bcturbs == 2) {
       float dumz;
@@ -817,7 +833,7 @@ __global__ __launch_bounds__(256) void k_turb_scalar_vert_implicit(uint32_t* M, 
       cfd = fma_((temc * dumz) * mh[0], rr[base], s[base]);
     }
     pb = cfd * t;
-    cB[base] = pb;
+    CB.at(0) = pb;
   }
   for (int k0 = 1; k0 < nz - 1; k0 += TS_CHUNK) {
     float kh[TS_CHUNK], rfu[TS_CHUNK], mfu[TS_CHUNK], mhk[TS_CHUNK], rrk[TS_CHUNK], sk_[TS_CHUNK];
@@ -841,8 +857,8 @@ __global__ __launch_bounds__(256) void k_turb_scalar_vert_implicit(uint32_t* M, 
         float tt = 1.0f / fma_(cfa, pa, cfb);
         pa = (-cfc) * tt;
         pb = fma_(-cfa, pb, sk_[t]) * tt;
-        cA[c] = pa;
-        cB[c] = pb;
+        CA.at(k) = pa;
+        CB.at(k) = pb;
       }
     }
   }
@@ -858,7 +874,7 @@ __global__ __launch_bounds__(256) void k_turb_scalar_vert_implicit(uint32_t* M, 
       cfd = fma_(-((temc * dumz) * mh[kt]), rr[ct], s[ct]);
     }
     pb = fma_(-cfa, pb, cfd) * t;
-    cB[ct] = pb;
+    CB.at(kt) = pb;
     sten[ct] = fma_(hdiv(ct) + (rho[ct] * (pb - s[ct])) * rdt, rr[ct], sten[ct]);
   }
   for (int k0 = nz - 2; k0 >= 0; k0 -= TS_CHUNK) {
@@ -868,7 +884,7 @@ __global__ __launch_bounds__(256) void k_turb_scalar_vert_implicit(uint32_t* M, 
       const int k = k0 - t;
       if (k >= 0) {
         const int64_t c = base + k * sk;
-        ca[t] = cA[c]; cb[t] = cB[c]; hd[t] = hdiv(c); ro[t] = rho[c]; sv[t] = s[c]; rv[t] = rr[c]; tv[t] = sten[c];
+        ca[t] = CA.at(k); cb[t] = CB.at(k); hd[t] = hdiv(c); ro[t] = rho[c]; sv[t] = s[c]; rv[t] = rr[c]; tv[t] = sten[c];
       }
     }
 #pragma unroll
@@ -877,11 +893,16 @@ __global__ __launch_bounds__(256) void k_turb_scalar_vert_implicit(uint32_t* M, 
       if (k >= 0) {
         const int64_t c = base + k * sk;
         pb = fma_(ca[t], pb, cb[t]);
-        cB[c] = pb;
+        CB.at(k) = pb;
         sten[c] = fma_(hd[t] + (ro[t] * (pb - sv[t])) * rdt, rv[t], tv[t]);
       }
     }
   }
+}
+
+__global__ __launch_bounds__(256) void k_turb_scalar_vert_implicit(uint32_t* M, KArg a) { scalar_vert_implicit<false>(M, a); }
+__global__ __launch_bounds__(256) void k_turb_scalar_vert_implicit_lds(uint32_t* M, KArg a) {
+  scalar_vert_implicit<true>(M, a);
 }
 
 // tke_flux_h.wgsl:125-146
@@ -899,7 +920,8 @@ __global__ __launch_bounds__(256) void k_tke_flux_h(uint32_t* M, KArg a) {
 }
 
 // tke_transport.wgsl:158-301
-__global__ __launch_bounds__(256) void k_tke_transport(uint32_t* M, KArg a) {
+template <bool LDS>
+__device__ void tke_transport(uint32_t* M, const KArg& a) {
   const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x), j = (int)(blockIdx.y * blockDim.y + threadIdx.y);
   if (i >= (int)a.n[0] || j >= (int)a.n[1]) return;
   Dims g = dims(M, a.p[0]);
@@ -944,7 +966,10 @@ __global__ __launch_bounds__(256) void k_tke_transport(uint32_t* M, KArg a) {
   const float temT = ((((-(kmv[ct] + kmv[ct + sk])) * (t[ct + sk] - t[ct])) * g.rdz) * mh[nz - 1]) * rho[ct];
   const float cfdT = fma_(-(temc * temT), mf[nz - 1] / rf[ct], t[ct]);
   FaceTendency out{fx, fy, hm, tketen, g, i, j, zeroX, zeroY, true, rdt, {}, {}, {}};
-  face_implicit(kmh, mh, rho, mf, rf, t, cA, cB, base, sk, nz, tema, cfd1, cfdT, out);
+  const ColStore<LDS> CA{cA, base, sk, TE_LANE, TE_NLANES, 0}, CB{cB, base, sk, TE_LANE, TE_NLANES, nz};
+  face_implicit(kmh, mh, rho, mf, rf, t, CA, CB, base, sk, nz, tema, cfd1, cfdT, out);
 }
+__global__ __launch_bounds__(256) void k_tke_transport(uint32_t* M, KArg a) { tke_transport<false>(M, a); }
+__global__ __launch_bounds__(256) void k_tke_transport_lds(uint32_t* M, KArg a) { tke_transport<true>(M, a); }
 
 }  // namespace te

@@ -209,6 +209,23 @@ TE_DEV float rr_exp(float x) {
 }
 
 // global thread ids of a 3-D launch
+// Elimination coefficients of a column sweep: level k of this lane's column,
+// either in the field at p (cA / cB of the reference) or, for the fused
+// launches, in dynamic LDS (slot k + off of nl lanes; they are dead scratch).
+extern __shared__ float te_lds[];
+template <bool LDS>
+struct ColStore {
+  float* p;
+  int64_t base, sk;
+  int lane, nl, off;
+  __device__ __forceinline__ float& at(int k) const {
+    if (LDS) return te_lds[(k + off) * nl + lane];
+    return p[base + k * sk];
+  }
+};
+#define TE_LANE ((int)(threadIdx.x + blockDim.x * threadIdx.y))
+#define TE_NLANES ((int)(blockDim.x * blockDim.y))
+
 #define TE_TID3                                                          \
   const int gx = (int)(blockIdx.x * blockDim.x + threadIdx.x);           \
   const int gy = (int)(blockIdx.y * blockDim.y + threadIdx.y);           \

@@ -201,16 +201,23 @@ class Encoder {
     }
     if (!pendingHalo.empty()) flushHalo();
     if (!pendingZero.empty()) flushZero();
+    // column sweeps keep their elimination coefficients (dead scratch) in LDS
+    static const char* const ldsCols[] = {"acoustic_w_solve", "turb_scalar_vert_implicit", "turb_u_tendency",
+                                          "turb_v_tendency", "turb_w_tendency", "tke_transport"};
+    std::string kn = kname;
+    if (cfg.fuse != 0)
+      for (const char* c : ldsCols)
+        if (kn == c) kn += "_lds";
     Op o;
     std::memset(&o, 0, sizeof(o));
-    o.kid = kernel_id(kname);
+    o.kid = kernel_id(kn.c_str());
     o.A[0] = (uint32_t)grid[0]; o.A[1] = (uint32_t)grid[1]; o.A[2] = (uint32_t)grid[2];
     o.A[4 + 0] = (uint32_t)dimsOff;
     o.A[4 + 30] = (uint32_t)hmOff;
     if (withAcoustic) o.A[4 + 1] = (uint32_t)bytes(acoustic);
     for (const Bind& bd : binds) o.A[4 + bd.idx] = (uint32_t)bd.off;
     E.ops.push_back(o);
-    E.names.push_back(kname);
+    E.names.push_back(kn);
   }
   Bind fb(int i, const std::string& n) const { return {i, off(n)}; }
   Bind rb(int i, const std::string& n) const { return {i, region(n)}; }
@@ -703,24 +710,15 @@ void Encoder::stages() {
       op("acoustic_uv", stag,
          {fb(2, "ppd"), fb(3, "thv"), fb(4, "uten"), fb(5, "vten"), fb(6, "u"), fb(7, "v"), bb(8, open)}, true);
       if (cfg.fuse != 0) {  // pp + accumulate in one pass; accumulate reads no ppd, so ppd's
-                            // halo joins u, v, w's.  fpk + w solve: one column sweep or two launches
-        const char* fs = std::getenv("TE_FPK_SOLVE");
-        if (fs == nullptr || std::atoi(fs) != 0) {
-          op("ac_fpk_solve", cols,
-             {fb(2, "u"), fb(3, "v"), fb(4, "w"), fb(5, "piadv"), fb(6, "ppten"), fb(7, "ppterm"), fb(8, "pk1"),
-              fb(9, "pk2"), fb(10, "qk"), fb(11, "fpk"), fb(12, "mm"), fb(13, "wten"), fb(14, "pp3d"), fb(15, "cA"),
-              fb(16, "cB")},
-             true);
-        } else {
-          op("acoustic_fpk", full,
-             {fb(2, "u"), fb(3, "v"), fb(4, "w"), fb(5, "piadv"), fb(6, "ppten"), fb(7, "ppterm"), fb(8, "pk1"),
-              fb(9, "pk2"), fb(10, "qk"), fb(11, "fpk"), rb(15, "rho0"), rb(16, "rr0"), rb(17, "rf0")},
-             true);
-          op("acoustic_w_solve", cols,
-             {fb(2, "pk1"), fb(3, "pk2"), fb(4, "qk"), fb(5, "mm"), fb(6, "fpk"), fb(7, "wten"), fb(8, "pp3d"),
-              fb(9, "w"), fb(10, "cA"), fb(11, "cB")},
-             true);
-        }
+                            // halo joins u, v, w's
+        op("acoustic_fpk", full,
+           {fb(2, "u"), fb(3, "v"), fb(4, "w"), fb(5, "piadv"), fb(6, "ppten"), fb(7, "ppterm"), fb(8, "pk1"),
+            fb(9, "pk2"), fb(10, "qk"), fb(11, "fpk"), rb(15, "rho0"), rb(16, "rr0"), rb(17, "rf0")},
+           true);
+        op("acoustic_w_solve", cols,
+           {fb(2, "pk1"), fb(3, "pk2"), fb(4, "qk"), fb(5, "mm"), fb(6, "fpk"), fb(7, "wten"), fb(8, "pp3d"),
+            fb(9, "w"), fb(10, "cA"), fb(11, "cB")},
+           true);
         const float tavg = n == nloop - 1 ? (float)(1.0 / nloop) : 0.0f;
         op("ac_pp_accumulate", whole,
            {fb(2, "w"), fb(3, "pk1"), fb(4, "pk2"), fb(5, "qk"), fb(6, "fpk"), fb(7, "pp3d"), fb(8, "ppd"),

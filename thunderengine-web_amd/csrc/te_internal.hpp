@@ -17,6 +17,7 @@ namespace te {
 struct Op {
   int32_t kid;
   uint32_t A[64];
+  int32_t nz = 0;  // levels of the op's grid (set before the first launch)
 };
 
 int kernel_id(const char* name);

@@ -56,22 +56,32 @@ static KArg karg_of(const Op& op) {
   return a;
 }
 
-// Column kernels: one lane per (i, j) column sweeping k serially.  A launch
-// has only nx*ny lanes (36 864 at 192 x 192, fewer waves than the chip has
-// SIMDs), so each wave carries TE_COLUMN_LANES columns (default 16) and the
-// chip holds several waves per SIMD to hide the sweeps' memory latency.
+// Column kernels: one lane per (i, j) column sweeping k serially, a wave of
+// TE_COLUMN_LANES columns along i (64 by default: measured fastest at
+// 192 x 192 x 80 against 32, 16 and 8, which split rows into shorter
+// segments without adding resident waves at these kernels' register use).
 static bool is_column_kernel(int kid) {
-  static const char* const cols[] = {"acoustic_w_solve", "ac_fpk_solve", "turb_scalar_vert_implicit",
+  static const char* const cols[] = {"acoustic_w_solve", "turb_scalar_vert_implicit",
                                      "turb_u_tendency", "turb_v_tendency", "turb_w_tendency", "tke_transport"};
-  for (const char* n : cols)
-    if (std::strcmp(kKernels[kid].name, n) == 0) return true;
+  const char* nm = kKernels[kid].name;
+  for (const char* n : cols) {
+    const size_t l = std::strlen(n);
+    if (std::strncmp(nm, n, l) == 0 && (nm[l] == 0 || std::strcmp(nm + l, "_lds") == 0)) return true;
+  }
   return false;
+}
+// Column kernels whose elimination coefficients live in dynamic LDS:
+// 2 * nz floats per lane.
+static bool uses_column_lds(int kid) {
+  const char* nm = kKernels[kid].name;
+  const size_t l = std::strlen(nm);
+  return l > 4 && std::strcmp(nm + l - 4, "_lds") == 0;
 }
 static uint32_t column_lanes() {
   static uint32_t lanes = [] {
     const char* e = std::getenv("TE_COLUMN_LANES");
-    int v = e ? std::atoi(e) : 16;
-    return (uint32_t)((v == 8 || v == 16 || v == 32 || v == 64) ? v : 16);
+    int v = e ? std::atoi(e) : 64;
+    return (uint32_t)((v == 8 || v == 16 || v == 32 || v == 64) ? v : 64);
   }();
   return lanes;
 }
@@ -80,10 +90,14 @@ static hipError_t launch(te_ctx* c, const Op& op) {
   KArg a = karg_of(op);
   if (a.n[0] == 0 || a.n[1] == 0 || a.n[2] == 0) return hipSuccess;
   dim3 b = block_for(a.n);
-  if (a.n[2] == 1 && is_column_kernel(op.kid)) b = dim3(column_lanes(), 1, 1);
+  size_t lds = 0;
+  if (a.n[2] == 1 && is_column_kernel(op.kid)) {
+    b = dim3(column_lanes(), 1, 1);
+    if (uses_column_lds(op.kid)) lds = (size_t)2 * op.nz * b.x * sizeof(float);
+  }
   dim3 g((a.n[0] + b.x - 1) / b.x, (a.n[1] + b.y - 1) / b.y, (a.n[2] + b.z - 1) / b.z);
   void* args[] = {(void*)&c->dM, (void*)&a};
-  return hipLaunchKernel(kKernels[op.kid].fn, g, b, args, 0, c->stream);
+  return hipLaunchKernel(kKernels[op.kid].fn, g, b, args, lds, c->stream);
 }
 
 // region_hash (replay.js:408-418): per-lane sums of a 32-bit mix, reduced
@@ -181,7 +195,7 @@ static int ensure_guard(te_ctx* c) {
   if (c->guard_ok) return TE_OK;
   uint64_t need = 0;
   std::map<uint32_t, std::array<uint32_t, 5>> seen;  // grid parameter blocks already read
-  for (const Op& op : c->ops) {
+  for (Op& op : c->ops) {
     if (std::strcmp(te::kernel_name(op.kid), "_copy") == 0) continue;
     uint32_t d[5];
     auto it = seen.find(op.A[4]);
@@ -194,6 +208,9 @@ static int ensure_guard(te_ctx* c) {
     const int32_t nz = (int32_t)d[2], sJ = (int32_t)d[3], sK = (int32_t)d[4];
     if (nz < 1 || sJ < 1 || sK < 1 || nz > 100000 || sJ > 100000 || sK > 100000)
       return fail(c, TE_ERR_ARG, std::string("op ") + te::kernel_name(op.kid) + ": invalid grid parameters");
+    op.nz = nz;
+    if (te::uses_column_lds(op.kid) && (size_t)2 * nz * te::column_lanes() * sizeof(float) > 65536)
+      return fail(c, TE_ERR_UNSUPPORTED, std::string(te::kernel_name(op.kid)) + ": nz too large for LDS coefficients");
     need = std::max<uint64_t>(need, (uint64_t)sJ * sK * (uint64_t)(nz + 3));
     if (std::strcmp(te::kernel_name(op.kid), "turb_sfcstress") == 0) {
       int32_t mode = 0;
