@@ -20,9 +20,11 @@ import tapeplan
 SKIP_REGIONS = ("tend", "fpk")
 
 
-def plan():
-    tape, params = tapeplan.load_supercell()
-    return tapeplan.Plan(tape, params)
+def plan(name="supercell"):
+    tape, params = tapeplan.load_preset(name)
+    P = tapeplan.Plan(tape, params)
+    P.preset = name
+    return P
 
 
 def check(P, run_op, region_hash, have, ok_slots=None):
@@ -31,7 +33,7 @@ def check(P, run_op, region_hash, have, ok_slots=None):
     Returns (ok, bad, skipped_ops, mismatches[(op, kernel, region, slot)]);
     appends the index of every matching slot to ok_slots when given.
     """
-    ref = tapeplan.ref_hashes()
+    ref = tapeplan.ref_hashes(getattr(P, "preset", "supercell"))
     slots_by_op = {}
     for s, (oi, idx, off, ln) in enumerate(P.slots):
         slots_by_op.setdefault(oi, []).append((s, off, ln))

@@ -17,14 +17,23 @@ import numpy as np
 
 ALIGN = 64
 HERE = os.path.dirname(os.path.abspath(__file__))
-SUPERCELL = os.path.join(HERE, "golden", "supercell")
+GOLDEN = os.path.join(HERE, "golden")
+SUPERCELL = os.path.join(GOLDEN, "supercell")
+# recorded presets with a fixture under tests/golden/<name>/
+PRESETS = ("supercell", "squall_line", "supercell_stretched")
+
+
+def load_preset(name):
+    d = os.path.join(GOLDEN, name)
+    with gzip.open(os.path.join(d, "tape_step1.json.gz"), "rt") as fh:
+        tape = json.load(fh)
+    params = json.load(open(os.path.join(d, "kernel_params.json")))
+    tape["_dir"] = d
+    return tape, params
 
 
 def load_supercell():
-    with gzip.open(os.path.join(SUPERCELL, "tape_step1.json.gz"), "rt") as fh:
-        tape = json.load(fh)
-    params = json.load(open(os.path.join(SUPERCELL, "kernel_params.json")))
-    return tape, params
+    return load_preset("supercell")
 
 
 class Plan:
@@ -106,7 +115,7 @@ class Plan:
     def memory(self, state_dir=None):
         """Flat u32 memory with the t=0 buffers and the parameter bytes in place."""
         M = np.zeros(self.mem_words, dtype=np.uint32)
-        state_dir = state_dir or os.path.join(SUPERCELL, "state")
+        state_dir = state_dir or os.path.join(self.tape.get("_dir", SUPERCELL), "state")
         for b in self.tape["buffers"]:
             p = os.path.join(state_dir, "%d.bin.gz" % b["id"])
             if os.path.exists(p):
@@ -133,5 +142,5 @@ class Plan:
         return None
 
 
-def ref_hashes():
-    return np.fromfile(os.path.join(SUPERCELL, "hashes.ref.bin"), dtype=np.uint32)
+def ref_hashes(name="supercell"):
+    return np.fromfile(os.path.join(GOLDEN, name, "hashes.ref.bin"), dtype=np.uint32)

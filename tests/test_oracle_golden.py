@@ -1,18 +1,21 @@
 """The oracle against the reference's own golden vectors (CPU).
 
-Runs step 1 of the recorded 120x120x40 supercell tape through the oracle and
-compares every reproducible region-hash slot with the hashes the reference
-recorded on its GPU (tests/golden/supercell/hashes.ref.bin).
+Runs step 1 of each recorded preset tape (the 120x120x40 supercell, the
+300x60x40 squall line, the 144x144x40 supercell on a stretched vertical grid)
+through the oracle and compares every reproducible region-hash slot with the
+hashes the reference recorded on its GPU (tests/golden/<preset>/hashes.ref.bin).
 """
 import numpy as np
+import pytest
 
 import golden_check
 import oracle_lib as ol
 import tapeplan
 
 
-def test_supercell_step1_region_hashes():
-    P = golden_check.plan()
+@pytest.mark.parametrize("preset", tapeplan.PRESETS)
+def test_preset_step1_region_hashes(preset):
+    P = golden_check.plan(preset)
     have = set(ol.kernel_names()) | {"_copy"}
     names, A = P.args()
     prog = ol.Program(names, A)
