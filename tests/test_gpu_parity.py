@@ -1,7 +1,7 @@
 """HIP path parity (MI355X): bit-exact against the oracle and the reference's
 recorded hashes, through the C ABI (libte.so).
 
-* the recorded supercell tape replayed on the GPU, dispatch by dispatch,
+* the recorded preset tapes (supercell, squall line, stretched grid) replayed on the GPU, dispatch by dispatch,
   against the region hashes the reference recorded on its own GPU;
 * programs of the encoder on synthetic configurations (the 64x64x40 case,
   TKE, open boundaries, dry, WENO momentum, explicit vertical diffusion) run
@@ -15,6 +15,7 @@ import pytest
 
 import golden_check
 import runs
+import tapeplan
 import te_host
 
 pytestmark = pytest.mark.gpu
@@ -25,18 +26,19 @@ def say(*a):
     sys.stdout.flush()
 
 
-def test_supercell_tape_region_hashes(gpu):
-    P = golden_check.plan()
+@pytest.mark.parametrize("preset", tapeplan.PRESETS)
+def test_preset_tape_region_hashes(gpu, preset):
+    P = golden_check.plan(preset)
     have = set(te_host.kernel_names())
     names, A = golden_check.program_for(P, have)
-    say("supercell tape on GPU:", len(names), "dispatches, memory words", P.mem_words)
+    say(preset, "tape on GPU:", len(names), "dispatches, memory words", P.mem_words)
     r = te_host.Replayer(P.mem_words, names, A)
     r.upload(0, P.memory())
     ok, bad, skipped, mism = golden_check.check(P, lambda i: r.run(1, i, i), r.region_hash, have)
     r.close()
     say("ok", ok, "bad", bad, "skipped", skipped)
     assert bad == 0, mism[:10]
-    assert ok >= 800
+    assert ok >= {"supercell": 816, "squall_line": 806, "supercell_stretched": 52}[preset]
 
 
 CONFIGS = {
