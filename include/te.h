@@ -124,7 +124,7 @@ typedef struct te_config {
   /* encoder options used to reproduce recorded reference configurations */
   int32_t species;         /* moist scalars: 3 = th qv qc, 7 = + qr qi qs qg, 11 = + nci ncs ncr ncg */
   int32_t microphysics;    /* 0 = none; 5 = Morrison call sites (encoded only, not executable) */
-  int32_t mass_conservation;          /* apmasscon = 1 (encoded only, not executable) */
+  int32_t mass_conservation;          /* apmasscon = 1: dry-mass correction at the end of the step */
   int32_t cache_moisture_turbulence;  /* moisture SGS fluxes computed once per step */
   int32_t n_arena_fields;             /* > 0: explicit FieldArena field list (sorted by the caller) */
   const char* const* arena_fields;

@@ -20,6 +20,7 @@ namespace orc {
 #include "k_ac.inc"
 #include "k_bc.inc"
 #include "k_turb.inc"
+#include "k_mass.inc"
 
 #define KE(n) {#n, &n<float>, &n<double>}
 static const KernelEntry kTable[] = {
@@ -37,7 +38,8 @@ static const KernelEntry kTable[] = {
     KE(turb_def_centers), KE(turb_def_faces), KE(turb_def_assemble), KE(turb_nm), KE(turb_km_kh),
     KE(turb_dissipative_heating), KE(turb_gettau), KE(turb_sfcstress), KE(tke_epst), KE(tke_sources),
     KE(turb_u_tendency), KE(turb_v_tendency), KE(turb_w_tendency), KE(turb_scalar_flux_h),
-    KE(turb_scalar_vert_implicit), KE(tke_flux_h), KE(tke_transport),
+    KE(turb_scalar_vert_implicit), KE(tke_flux_h), KE(tke_transport), KE(mass_row_sums), KE(mass_level_sums),
+    KE(mass_reference_init), KE(mass_correction_apply),
 };
 #undef KE
 

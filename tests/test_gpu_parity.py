@@ -38,7 +38,7 @@ def test_preset_tape_region_hashes(gpu, preset):
     r.close()
     say("ok", ok, "bad", bad, "skipped", skipped)
     assert bad == 0, mism[:10]
-    assert ok >= {"supercell": 816, "squall_line": 806, "supercell_stretched": 52}[preset]
+    assert ok >= {"supercell": 825, "squall_line": 815, "supercell_stretched": 52}[preset]
 
 
 CONFIGS = {
@@ -49,6 +49,8 @@ CONFIGS = {
     "dry_explicit_vdiff": dict(nx=36, ny=36, nz=20, moist=0, tke=1, initial_tke=3, explicit_vdiff=1),
     "weno_momentum": dict(nx=40, ny=36, nz=24, weno_momentum=1),
     "species7_ragged": dict(nx=37, ny=29, nz=17, species=7, tke=1, initial_tke=4, nsound=4, dt=4.0),
+    "open_mass_conservation": dict(nx=30, ny=26, nz=16, open_x=1, open_y=1, tke=1, initial_tke=3, umove=5.0,
+                                   mass_conservation=1, rayleigh=1),
 }
 
 

@@ -13,8 +13,8 @@ import oracle_lib as ol
 import tapeplan
 
 
-# reproducible slots per preset (supercell 816, squall line 806, stretched 52)
-MIN_SLOTS = {"supercell": 816, "squall_line": 806, "supercell_stretched": 52}
+# reproducible slots per preset (supercell 825, squall line 815, stretched 52)
+MIN_SLOTS = {"supercell": 825, "squall_line": 815, "supercell_stretched": 52}
 
 
 @pytest.mark.parametrize("preset", tapeplan.PRESETS)

@@ -455,7 +455,7 @@ int te_create(const te_config* cfg, te_ctx** out) {
   te_ctx* c = *out;
   for (size_t i = 0; i < c->ops.size(); ++i)
     if (c->ops[i].kid < 0)
-      return fail(c, TE_ERR_UNSUPPORTED, "kernel '" + c->op_names[i] + "' is not on this path (microphysics / mass conservation)");
+      return fail(c, TE_ERR_UNSUPPORTED, "kernel '" + c->op_names[i] + "' is not on this path (microphysics)");
   rc = validate_ops(c);
   if (rc != TE_OK) return rc;
   rc = ctx_alloc(c, c->layout.words);
