@@ -88,6 +88,38 @@ def calibration_copy(words=1 << 26):
     r.close()
 
 
+def ensemble_configs(n=32, seed=20260102):
+    """Seeded uneven ensemble: nx, ny in {96, 128, 160, 192}, nz in {40, 60, 80}
+    with dz = 20 km / nz; theta' N(0, 0.5 K) smooth in z, qv x (1 + N(0, 0.05)),
+    0-6 km shear U(10, 40) m/s, bubble U(1, 3) K; TKE as the main workload."""
+    rng = np.random.default_rng(seed)
+    cfgs = []
+    for m in range(n):
+        nx, ny = (int(v) for v in rng.choice([96, 128, 160, 192], 2))
+        nz = int(rng.choice([40, 60, 80]))
+        cfgs.append(te_host.config(nx=nx, ny=ny, nz=nz, dz=20000.0 / nz, dt=6.0, nsound=6, tke=1, rayleigh=1,
+                                   rayleigh_zd=15000.0, sounding_seed=seed + m, th_pert_sigma=0.5,
+                                   qv_pert_sigma=0.05, shear_umax=float(rng.uniform(10, 40)),
+                                   bubble_amp=float(rng.uniform(1, 3)), seed=seed + 1000 + m))
+    return cfgs
+
+
+def ensemble_bench(steps, warmup):
+    cfgs = ensemble_configs()
+    ens = te_host.Ensemble(cfgs)
+    ens.step(warmup)
+    ens.sync()
+    t0 = time.perf_counter()
+    ens.step(steps)
+    ens.sync()
+    wall = time.perf_counter() - t0
+    out = {"members": len(cfgs), "points": ens.points, "steps": steps,
+           "value": round(ens.points * steps / wall / 1e6, 2), "unit": "Mpt-steps/s",
+           "grids": sorted({"%dx%dx%d" % (c.nx, c.ny, c.nz) for c in cfgs})}
+    ens.close()
+    return out
+
+
 def cpu_baseline(cfg, seconds=15.0):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib
@@ -115,6 +147,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=3)
     ap.add_argument("--calib-copy", action="store_true", help="also launch the PMC calibration copy")
+    ap.add_argument("--ensemble", action="store_true",
+                    help="also time the 32-member uneven ensemble (members stepped together)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -171,6 +205,11 @@ def main():
         "top_kernels_ms_per_step": {k: round(v[0], 4) for k, v in
                                     sorted(per_kernel.items(), key=lambda kv: -kv[1][0])[:8]},
     }
+    if args.ensemble:
+        run.close()
+        ens = ensemble_bench(max(5, args.steps // 5), 2)
+        ens["per_point_vs_plain"] = round(ens["value"] / value, 3)
+        out["ensemble"] = ens
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(cfg)
     print(json.dumps(out), flush=True)

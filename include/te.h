@@ -121,6 +121,9 @@ typedef struct te_config {
   int32_t fuse;            /* 1 (default): merged / fused launches where the step allows them
                               (same state bits; scratch fields may differ); 0: the reference's
                               dispatch list one for one */
+  uint64_t sounding_seed;  /* perturbed soundings (ensemble members): seed of the draws below */
+  float th_pert_sigma;     /* K: theta(z) += sigma * N(0,1) noise smoothed over ~1 km in z */
+  float qv_pert_sigma;     /* qv(z) *= 1 + sigma * N(0,1), one draw per sounding */
   /* encoder options used to reproduce recorded reference configurations */
   int32_t species;         /* moist scalars: 3 = th qv qc, 7 = + qr qi qs qg, 11 = + nci ncs ncr ncg */
   int32_t microphysics;    /* 0 = none; 5 = Morrison call sites (encoded only, not executable) */
@@ -165,6 +168,21 @@ int te_region_offset(const te_ctx* ctx, const char* name, uint64_t* word_off);
 
 /* Grid of a config context: nx ny nz strideJ strideK. */
 int te_grid(const te_ctx* ctx, int32_t out5[5]);
+
+/* ---- ensembles: members of uneven grids and perturbed soundings stepped
+ * together.  No counterpart in the reference (one run per page there).  Each
+ * member is a te_create context with its t=0 state; te_ensemble_step
+ * replays every member's step graph on the member's own stream, step by
+ * step, so the device runs members concurrently. */
+typedef struct te_ensemble te_ensemble;
+int te_ensemble_create(const te_config* members, int32_t n, te_ensemble** out);
+int te_ensemble_step(te_ensemble* ens, int32_t nsteps);
+int te_ensemble_sync(te_ensemble* ens);
+int32_t te_ensemble_size(const te_ensemble* ens);
+te_ctx* te_ensemble_member(te_ensemble* ens, int32_t m);  /* borrowed: valid until te_ensemble_destroy */
+uint64_t te_ensemble_points(const te_ensemble* ens);        /* interior points summed over members */
+const char* te_ensemble_last_error(const te_ensemble* ens);
+void te_ensemble_destroy(te_ensemble* ens);
 
 #ifdef __cplusplus
 }

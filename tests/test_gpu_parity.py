@@ -83,6 +83,40 @@ def test_config_bit_exact(gpu, name, steps, fuse):
     run.close()
 
 
+def test_ensemble_members_match_single_runs(gpu):
+    """Members of uneven grids and perturbed soundings stepped together on
+    their own streams equal the same configurations stepped alone, and one
+    perturbed member equals the oracle."""
+    kws = [dict(nx=32, ny=24, nz=16, dz=1250.0, tke=1, initial_tke=3, sounding_seed=11, th_pert_sigma=0.5,
+                qv_pert_sigma=0.05, shear_umax=17.0, bubble_amp=1.5),
+           dict(nx=40, ny=32, nz=20, dz=1000.0, sounding_seed=12, th_pert_sigma=0.5, qv_pert_sigma=0.05,
+                shear_umax=35.0, bubble_amp=2.5),
+           dict(nx=24, ny=40, nz=12, dz=1666.6667, tke=1, initial_tke=4, sounding_seed=13, th_pert_sigma=0.5)]
+    cfgs = [te_host.config(**kw) for kw in kws]
+    ens = te_host.Ensemble(cfgs)
+    assert len(ens) == 3 and ens.points == sum(k["nx"] * k["ny"] * k["nz"] for k in kws)
+    ens.step(2)
+    ens.sync()
+    for m, cfg in enumerate(cfgs):
+        single = te_host.Run(cfg)
+        single.init_state()
+        single.step(2)
+        single.sync()
+        end = runs.arena_end(single)
+        a, b = ens.member_words(m, 0, end), single.read(0, end)
+        assert np.array_equal(a, b), (m, runs.first_difference(single, a, b))
+        if m == 0:
+            ref = runs.OracleRun(cfg)
+            ref.step(2)
+            keep = runs.state_mask(single)
+            want = ref.M[:end].copy()
+            got = a.copy()
+            got[~keep] = 0
+            want[~keep] = 0
+            assert np.array_equal(got, want), runs.first_difference(single, got, want)
+    ens.close()
+
+
 def test_graph_replays_match_eager(gpu):
     """Steps replayed from the captured graph equal eager single steps."""
     cfg = te_host.config(nx=32, ny=32, nz=16, tke=1, initial_tke=3)

@@ -112,3 +112,23 @@ def test_fp64_build_is_the_same_step():
     for steps, row in res.items():
         for label, e in row.items():
             assert e["max_abs"] <= 1e-3 * max(1.0, e["max_abs_fp64"]), (steps, label, e)
+
+
+def test_perturbed_sounding_is_seeded_and_smooth():
+    """Sounding perturbations (ensemble members) are reproducible from the
+    seed, leave the default sounding untouched, and theta' is smooth in z."""
+    base = runs.OracleRun(te_host.config(nx=16, ny=16, nz=40, dz=500.0))
+    a = runs.OracleRun(te_host.config(nx=16, ny=16, nz=40, dz=500.0, sounding_seed=5, th_pert_sigma=0.5,
+                                      qv_pert_sigma=0.05))
+    b = runs.OracleRun(te_host.config(nx=16, ny=16, nz=40, dz=500.0, sounding_seed=5, th_pert_sigma=0.5,
+                                      qv_pert_sigma=0.05))
+    c = runs.OracleRun(te_host.config(nx=16, ny=16, nz=40, dz=500.0, sounding_seed=6, th_pert_sigma=0.5))
+    e = base.enc
+    th0 = lambda r: r.M[e.region_offset("th0"): e.region_offset("th0") + e.nz].view(np.float32).astype(np.float64)
+    assert np.array_equal(a.M0, b.M0)
+    d = th0(a) - th0(base)
+    assert 0.1 < d.std() < 1.5 and np.abs(d).max() < 3.0
+    assert np.abs(np.diff(d)).max() < 0.5 * np.abs(d).max() + 1e-9   # smooth over ~1 km (2 levels)
+    assert not np.array_equal(th0(a), th0(c))
+    default = runs.OracleRun(te_host.config(nx=16, ny=16, nz=40, dz=500.0, th_pert_sigma=0.0))
+    assert np.array_equal(default.M0, base.M0)

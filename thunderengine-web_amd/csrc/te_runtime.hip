@@ -501,4 +501,70 @@ int te_grid(const te_ctx* c, int32_t o[5]) {
   return TE_OK;
 }
 
+// ------------------------------------------------------------------ ensembles
+struct te_ensemble {
+  std::vector<te_ctx*> members;
+  std::string err;
+  uint64_t points = 0;
+};
+
+int te_ensemble_create(const te_config* cfgs, int32_t n, te_ensemble** out) {
+  if (!cfgs || n < 1 || !out) return TE_ERR_ARG;
+  te_ensemble* e = new te_ensemble();
+  *out = e;
+  for (int32_t m = 0; m < n; ++m) {
+    te_ctx* c = nullptr;
+    int rc = te_create(&cfgs[m], &c);
+    if (rc == TE_OK) rc = te_init_state(c);
+    if (rc == TE_OK) rc = ensure_guard(c);
+    if (rc == TE_OK) rc = ensure_graph(c);
+    if (rc != TE_OK) {
+      e->err = "member " + std::to_string(m) + ": " + (c ? c->err : std::string("create failed"));
+      te_destroy(c);
+      return rc;
+    }
+    e->members.push_back(c);
+    e->points += (uint64_t)cfgs[m].nx * cfgs[m].ny * cfgs[m].nz;
+  }
+  return TE_OK;
+}
+
+int te_ensemble_step(te_ensemble* e, int32_t nsteps) {
+  if (!e || nsteps < 0) return TE_ERR_ARG;
+  // step s of every member before step s+1 of any: members advance together,
+  // each on its own stream, so the device runs them concurrently
+  for (int32_t s = 0; s < nsteps; ++s)
+    for (te_ctx* c : e->members) {
+      hipError_t r = c->graph ? hipGraphLaunch(c->graph, c->stream) : hipSuccess;
+      int rc = r == hipSuccess ? (c->graph ? TE_OK : run_ops(c, 0, (int)c->ops.size() - 1)) : hip_fail(c, r, "hipGraphLaunch");
+      if (rc != TE_OK) {
+        e->err = c->err;
+        return rc;
+      }
+    }
+  return TE_OK;
+}
+
+int te_ensemble_sync(te_ensemble* e) {
+  if (!e) return TE_ERR_ARG;
+  for (te_ctx* c : e->members)
+    if (hipStreamSynchronize(c->stream) != hipSuccess) {
+      e->err = "ensemble sync failed";
+      return TE_ERR_HIP;
+    }
+  return TE_OK;
+}
+
+int32_t te_ensemble_size(const te_ensemble* e) { return e ? (int32_t)e->members.size() : 0; }
+te_ctx* te_ensemble_member(te_ensemble* e, int32_t m) {
+  return (e && m >= 0 && m < (int32_t)e->members.size()) ? e->members[m] : nullptr;
+}
+uint64_t te_ensemble_points(const te_ensemble* e) { return e ? e->points : 0; }
+const char* te_ensemble_last_error(const te_ensemble* e) { return e ? e->err.c_str() : "null ensemble"; }
+void te_ensemble_destroy(te_ensemble* e) {
+  if (!e) return;
+  for (te_ctx* c : e->members) te_destroy(c);
+  delete e;
+}
+
 }  // extern "C"

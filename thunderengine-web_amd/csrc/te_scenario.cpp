@@ -94,6 +94,32 @@ static float rslf(float p, float t, float eps) {
   return (eps * esl) / (p - esl);
 }
 
+static double normal(uint64_t seed, uint64_t cell);
+
+// Perturbed sounding (ensemble members): theta'(z) = sigma * N(0,1) white
+// noise over the levels, smoothed by a Gaussian of 1 km width and rescaled
+// to unit variance; one N(0,1) draw scales qv.  Draws are keyed by
+// sounding_seed (cells 0..nz-1 for theta, nz for qv).
+static std::vector<double> theta_perturbation(const te_config& cfg, const Vertical& v) {
+  const int nk = v.nz;
+  std::vector<double> wn(nk), out(nk, 0.0);
+  if (!(cfg.th_pert_sigma > 0.0f)) return out;
+  // unit variance: each smoothed value is a unit-norm combination of N(0,1)
+  for (int k = 0; k < nk; ++k) wn[k] = normal(cfg.sounding_seed, (uint64_t)k);
+  for (int k = 0; k < nk; ++k) {
+    double s = 0.0, wsum = 0.0;
+    for (int q = 0; q < nk; ++q) {
+      const double d = (v.zh[k] - v.zh[q]) / 1000.0;
+      const double wt = std::exp(-0.5 * d * d);
+      s += wt * wn[q];
+      wsum += wt * wt;
+    }
+    out[k] = s / std::sqrt(wsum);
+  }
+  for (int k = 0; k < nk; ++k) out[k] *= cfg.th_pert_sigma;
+  return out;
+}
+
 Base make_base(const te_config& cfg, const Vertical& v) {
   const int nk = v.nz;
   const float rd = f32(RDd), cp = f32(CPd), rv = f32(RVd), p00 = f32(P00d), g = f32(G);
@@ -118,6 +144,10 @@ Base make_base(const te_config& cfg, const Vertical& v) {
       rh[k] = cfg.moist ? 0.25f : 0.0f;
     }
   }
+  const std::vector<double> dth = theta_perturbation(cfg, v);
+  for (int k = 0; k < nk; ++k) b.th0[k] = f32(b.th0[k] + dth[k]);
+  const float qvScale = cfg.qv_pert_sigma > 0.0f
+                            ? f32(1.0 + cfg.qv_pert_sigma * normal(cfg.sounding_seed, (uint64_t)nk)) : 1.0f;
   for (int it = 0; it < 20; ++it) {
     for (int k = 0; k < nk; ++k) thv[k] = (b.th0[k] * std::fma(reps, b.qv0[k], 1.0f)) / (1.0f + b.qv0[k]);
     b.pi0[0] = piSfc - (g * v.zh[0]) / ((cp * 0.5f) * (thvSfc + thv[0]));
@@ -125,7 +155,7 @@ Base make_base(const te_config& cfg, const Vertical& v) {
       b.pi0[k] = b.pi0[k - 1] - (g * (v.zh[k] - v.zh[k - 1])) / ((cp * 0.5f) * (thv[k] + thv[k - 1]));
     for (int k = 0; k < nk; ++k) b.prs0[k] = p00 * f32(std::pow((double)b.pi0[k], (double)cpdrd));
     for (int k = 0; k < nk; ++k) {
-      b.qv0[k] = cfg.moist ? rh[k] * rslf(b.prs0[k], b.th0[k] * b.pi0[k], eps) : 0.0f;
+      b.qv0[k] = cfg.moist ? (rh[k] * rslf(b.prs0[k], b.th0[k] * b.pi0[k], eps)) * qvScale : 0.0f;
       if (b.qv0[k] > cfg.qv_pbl) b.qv0[k] = cfg.qv_pbl;
     }
   }

@@ -112,7 +112,7 @@ const FieldDesc kFields[] = {
     FD(rayleigh_alpha, F32), FD(alph, F32), FD(kdiv, F32), FD(smeps, F32), FD(umove, F32), FD(vmove, F32),
     FD(th_sfc, F32), FD(th_trop, F32), FD(t_trop, F32), FD(z_trop, F32), FD(qv_pbl, F32), FD(hodo_radius, F32),
     FD(hodo_depth, F32), FD(shear_umax, F32), FD(shear_depth, F32), FD(bubble_amp, F32), FD(bubble_rh, F32),
-    FD(bubble_rv, F32), FD(bubble_zc, F32), FD(noise_sigma, F32), FD(seed, U64), FD(initial_tke, I32), FD(fuse, I32),
+    FD(bubble_rv, F32), FD(bubble_zc, F32), FD(noise_sigma, F32), FD(seed, U64), FD(initial_tke, I32), FD(fuse, I32), FD(sounding_seed, U64), FD(th_pert_sigma, F32), FD(qv_pert_sigma, F32),
     FD(species, I32), FD(microphysics, I32), FD(mass_conservation, I32), FD(cache_moisture_turbulence, I32),
 };
 #undef FD

@@ -45,7 +45,8 @@ class Config(ctypes.Structure):
         ("bubble_amp", ctypes.c_float), ("bubble_rh", ctypes.c_float), ("bubble_rv", ctypes.c_float),
         ("bubble_zc", ctypes.c_float),
         ("noise_sigma", ctypes.c_float), ("seed", ctypes.c_uint64), ("initial_tke", ctypes.c_int32),
-        ("fuse", ctypes.c_int32),
+        ("fuse", ctypes.c_int32), ("sounding_seed", ctypes.c_uint64), ("th_pert_sigma", ctypes.c_float),
+        ("qv_pert_sigma", ctypes.c_float),
         ("species", ctypes.c_int32), ("microphysics", ctypes.c_int32), ("mass_conservation", ctypes.c_int32),
         ("cache_moisture_turbulence", ctypes.c_int32), ("n_arena_fields", ctypes.c_int32),
         ("arena_fields", ctypes.POINTER(ctypes.c_char_p)),
@@ -94,6 +95,14 @@ def lib():
         "te_field_count": ([ctx], ctypes.c_int32),
         "te_field_name": ([ctx, ctypes.c_int32], ctypes.c_char_p),
         "te_grid": ([ctx, P(ctypes.c_int32)], ctypes.c_int),
+        "te_ensemble_create": ([P(Config), ctypes.c_int32, P(ctx)], ctypes.c_int),
+        "te_ensemble_step": ([ctx, ctypes.c_int32], ctypes.c_int),
+        "te_ensemble_sync": ([ctx], ctypes.c_int),
+        "te_ensemble_size": ([ctx], ctypes.c_int32),
+        "te_ensemble_member": ([ctx, ctypes.c_int32], ctx),
+        "te_ensemble_points": ([ctx], u64),
+        "te_ensemble_last_error": ([ctx], ctypes.c_char_p),
+        "te_ensemble_destroy": ([ctx], None),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -262,3 +271,53 @@ class Run(_Ctx):
     def interior(self, name, stagger=(0, 0, 0)):
         f = self.field(name)
         return f[: self.nz + stagger[2], 3: 3 + self.ny + stagger[1], 3: 3 + self.nx + stagger[0]]
+
+
+class Ensemble:
+    """Members of uneven grids / perturbed soundings stepped together
+    (te_ensemble_*): each member on its own stream, graph replays."""
+
+    def __init__(self, cfgs):
+        arr = (Config * len(cfgs))(*cfgs)
+        self._cfgs = cfgs
+        self.h = ctypes.c_void_p()
+        rc = lib().te_ensemble_create(arr, len(cfgs), ctypes.byref(self.h))
+        if rc != TE_OK:
+            msg = lib().te_ensemble_last_error(self.h)
+            self.close()
+            raise TeError("te_ensemble_create failed (%d): %s" % (rc, msg.decode() if msg else "?"))
+
+    def step(self, n=1):
+        rc = lib().te_ensemble_step(self.h, n)
+        if rc != TE_OK:
+            raise TeError("te_ensemble_step failed (%d): %s" % (rc, lib().te_ensemble_last_error(self.h).decode()))
+
+    def sync(self):
+        if lib().te_ensemble_sync(self.h) != TE_OK:
+            raise TeError("te_ensemble_sync failed")
+
+    @property
+    def points(self):
+        return int(lib().te_ensemble_points(self.h))
+
+    def __len__(self):
+        return int(lib().te_ensemble_size(self.h))
+
+    def member_words(self, m, word_off, nwords):
+        """Words of member m's memory (a borrowed context)."""
+        c = lib().te_ensemble_member(self.h, m)
+        out = np.empty(nwords, dtype=np.uint32)
+        if lib().te_download_words(c, word_off, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), nwords) != TE_OK:
+            raise TeError("te_download_words failed")
+        return out
+
+    def close(self):
+        if self.h:
+            lib().te_ensemble_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
