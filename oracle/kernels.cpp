@@ -152,6 +152,24 @@ static int run_program(uint32_t* w, T* v, const char* const* names, const uint32
       else ea->f64(ka);
       continue;
     }
+    if (nm == "tendency_reset_add") {  // stage_tendency_reset, then turb_add_tendency x3
+      const KernelEntry *er = find_kernel("stage_tendency_reset"), *ea = find_kernel("turb_add_tendency");
+      uint32_t R[64] = {0};
+      R[0] = A[0]; R[1] = A[1]; R[2] = A[2]; R[4] = A[4]; R[5] = A[5]; R[6] = A[6]; R[7] = A[7]; R[34] = A[34];
+      K<T> kr{w, v, R};
+      if constexpr (sizeof(T) == 4) er->f32(kr);
+      else er->f64(kr);
+      for (int q = 0; q < 3; ++q) {
+        uint32_t X[64] = {0};
+        X[0] = A[0]; X[1] = A[1]; X[2] = A[2]; X[4] = A[4];
+        X[5] = A[4 + 4 + q]; X[6] = A[4 + 1 + q]; X[7] = A[4 + 7];
+        X[34] = A[34];
+        K<T> kx{w, v, X};
+        if constexpr (sizeof(T) == 4) ea->f32(kx);
+        else ea->f64(kx);
+      }
+      continue;
+    }
     if (nm == "add_fields") {  // turb_add_tendency on each triple, in order
       const KernelEntry* ea = find_kernel("turb_add_tendency");
       const uint32_t nf = w[A[4 + 13]];

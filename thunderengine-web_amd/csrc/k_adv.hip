@@ -389,32 +389,36 @@ __global__ __launch_bounds__(256) void k_adv_assemble(uint32_t* M, KArg a) {
 // p: adv_assemble's parameters, with 3 -> flux kind (0 flx5, 1 WENO5,
 // 2 staggered flx5, 3 WENO5 of w-staggered fields), 4 rrw, 5 -> weps,
 // 20 -> pdef, 21 -> pdefweno (22-24: fx, fy, fz of the unfused dispatches).
+template <int KIND>
 struct FusedFlux {
   Dims g;
-  int kind, stag, pdef, pdw;
+  int stag, pdef, pdw;
   const float *s, *rru, *rrv, *rrw, *c1, *c2;
   float weps;
-  // face flux of direction d at (i, j, k); NaN where no flux dispatch writes
-  __device__ float operator()(int d, int i, int j, int k) const {
+  // face flux of direction D at (i, j, k); NaN where no flux dispatch writes
+  template <int D>
+  __device__ __forceinline__ float at(int i, int j, int k) const {
     float f = __builtin_nanf("");
     bool ok;
-    if (kind == 0) ok = d == 0 ? fv_h5<0>(g, s, rru, i, j, k, f) : d == 1 ? fv_h5<1>(g, s, rrv, i, j, k, f)
-                                                                        : fv_z5(g, s, rrw, c1, c2, i, j, k, f);
-    else if (kind == 1)
-      ok = d == 0 ? fv_h_weno<0>(g, s, rru, weps, pdef == 1, i, j, k, f)
-                  : d == 1 ? fv_h_weno<1>(g, s, rrv, weps, pdef == 1, i, j, k, f)
+    if (KIND == 0)
+      ok = D == 0 ? fv_h5<0>(g, s, rru, i, j, k, f) : D == 1 ? fv_h5<1>(g, s, rrv, i, j, k, f)
+                                                             : fv_z5(g, s, rrw, c1, c2, i, j, k, f);
+    else if (KIND == 1)
+      ok = D == 0 ? fv_h_weno<0>(g, s, rru, weps, pdef == 1, i, j, k, f)
+                  : D == 1 ? fv_h_weno<1>(g, s, rrv, weps, pdef == 1, i, j, k, f)
                            : fv_z_weno(g, s, rrw, c1, c2, weps, pdw, i, j, k, f);
-    else if (kind == 2)
-      ok = d == 0 ? fv_h_stag<0>(g, stag, s, rru, c2, i, j, k, f)
-                  : d == 1 ? fv_h_stag<1>(g, stag, s, rrv, c2, i, j, k, f) : fv_z_stag(g, stag, s, rrw, c1, c2, i, j, k, f);
+    else if (KIND == 2)
+      ok = D == 0 ? fv_h_stag<0>(g, stag, s, rru, c2, i, j, k, f)
+                  : D == 1 ? fv_h_stag<1>(g, stag, s, rrv, c2, i, j, k, f) : fv_z_stag(g, stag, s, rrw, c1, c2, i, j, k, f);
     else
-      ok = d == 0 ? fv_h_weno_w<0>(g, s, rru, c2, weps, i, j, k, f)
-                  : d == 1 ? fv_h_weno_w<1>(g, s, rrv, c2, weps, i, j, k, f) : fv_z_weno_w(g, s, rrw, weps, i, j, k, f);
+      ok = D == 0 ? fv_h_weno_w<0>(g, s, rru, c2, weps, i, j, k, f)
+                  : D == 1 ? fv_h_weno_w<1>(g, s, rrv, c2, weps, i, j, k, f) : fv_z_weno_w(g, s, rrw, weps, i, j, k, f);
     return ok ? f : __builtin_nanf("");  // a face no flux dispatch writes: poisons the result visibly
   }
 };
 
-__global__ __launch_bounds__(256) void k_adv_fused(uint32_t* M, KArg a) {
+template <int KIND>
+__device__ __forceinline__ void adv_fused_body(uint32_t* M, const KArg& a) {
   __shared__ float lx[256], ly[256];
   const int tx = threadIdx.x, ty = threadIdx.y, tz = threadIdx.z;
   const int i = (int)(blockIdx.x * blockDim.x) + tx, j = (int)(blockIdx.y * blockDim.y) + ty,
@@ -423,14 +427,14 @@ __global__ __launch_bounds__(256) void k_adv_fused(uint32_t* M, KArg a) {
   const int t = tx + (int)blockDim.x * (ty + (int)blockDim.y * tz);
   Dims g = dims(M, a.p[0]);
   const int stag = ldi(M, a.p[1]), st = stag < 0 ? -stag : stag;
-  const FusedFlux F{g, ldi(M, a.p[3]), stag, ldi(M, a.p[20]), ldi(M, a.p[21]), fp(M, a.p[2]), fp(M, a.p[13]),
-                    fp(M, a.p[14]), fp(M, a.p[4]), fp(M, a.p[15]), fp(M, a.p[16]), ldf(M, a.p[5])};
-  // low faces of this cell's x and y divergence
+  const FusedFlux<KIND> F{g, stag, ldi(M, a.p[20]), ldi(M, a.p[21]), fp(M, a.p[2]), fp(M, a.p[13]), fp(M, a.p[14]),
+                          fp(M, a.p[4]), fp(M, a.p[15]), fp(M, a.p[16]), ldf(M, a.p[5])};
+  // low faces of this cell's x and y divergence, published for the neighbours
   const int xlo = st == 2 ? i - 1 : i, ylo = st == 3 ? j - 1 : j;
   float fxl = 0.0f, fyl = 0.0f;
   if (inGrid) {
-    fxl = F(0, xlo, j, k);
-    fyl = F(1, i, ylo, k);
+    fxl = F.template at<0>(xlo, j, k);
+    fyl = F.template at<1>(i, ylo, k);
   }
   lx[t] = fxl;
   ly[t] = fyl;
@@ -438,13 +442,26 @@ __global__ __launch_bounds__(256) void k_adv_fused(uint32_t* M, KArg a) {
   if (!inGrid) return;
   const bool xNb = tx + 1 < (int)blockDim.x && i + 1 < (int)a.n[0];
   const bool yNb = ty + 1 < (int)blockDim.y && j + 1 < (int)a.n[1];
-  const float fxh = xNb ? lx[t + 1] : F(0, xlo + 1, j, k);
-  const float fyh = yNb ? ly[t + (int)blockDim.x] : F(1, i, ylo + 1, k);
+  const float fxh = xNb ? lx[t + 1] : F.template at<0>(xlo + 1, j, k);
+  const float fyh = yNb ? ly[t + (int)blockDim.x] : F.template at<1>(i, ylo + 1, k);
+  const int zlo = st == 4 ? k - 1 : k;
+  const float fzl = F.template at<2>(i, j, zlo), fzh = F.template at<2>(i, j, zlo + 1);
   assemble_at(M, a, g, i, j, k, [&](int d, int fi, int fj, int fk) {
     if (d == 0) return fi == xlo ? fxl : fxh;
     if (d == 1) return fj == ylo ? fyl : fyh;
-    return F(2, fi, fj, fk);
+    return fk == zlo ? fzl : fzh;
   });
+}
+
+// The three flux dispatches of one advected field and adv_assemble in one
+// launch (see above); the flux kind selects one specialised body.
+__global__ __launch_bounds__(256) void k_adv_fused(uint32_t* M, KArg a) {
+  switch (ldi(M, a.p[3])) {
+    case 0: adv_fused_body<0>(M, a); break;
+    case 1: adv_fused_body<1>(M, a); break;
+    case 2: adv_fused_body<2>(M, a); break;
+    default: adv_fused_body<3>(M, a); break;
+  }
 }
 
 }  // namespace te

@@ -75,6 +75,20 @@ __global__ __launch_bounds__(256) void k_add_fields(uint32_t* M, KArg a) {
   }
 }
 
+// stage_tendency_reset (stage_tendency_reset.wgsl:111-129) followed by
+// turb_add_tendency of the three SGS / damping momentum tendencies:
+// dst = fma(scale, src, 0), the bits of adding src*scale to the cleared field.
+// p: 1-3 uten vten wten, 4-6 their sources, 7 -> scale.  Grid: whole.
+__global__ __launch_bounds__(256) void k_tendency_reset_add(uint32_t* M, KArg a) {
+  TE_TID3
+  Dims g = dims(M, a.p[0]);
+  int i = gx - 3, j = gy - 3, k = gz;
+  if (i > g.nx + 3 || j > g.ny + 3 || k > g.nz) return;
+  const int64_t c = g.idx(i, j, k);
+  const float scale = ldf(M, a.p[7]);
+  for (int q = 0; q < 3; ++q) fp(M, a.p[1 + q])[c] = fma_(scale, fp(M, a.p[4 + q])[c], 0.0f);
+}
+
 // Up to 4 buffer copies of a.n[0] words each (the state swap of
 // timestep.js:766-774): p[1..4] sources, p[5..8] destinations, p[9] -> count.
 __global__ __launch_bounds__(256) void k_copy_fields(uint32_t* M, KArg a) {

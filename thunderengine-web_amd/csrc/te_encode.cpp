@@ -669,11 +669,17 @@ void Encoder::stages() {
          {fb(1, scalars[0]), fb(2, "pp3d"), rb(3, "th0"), rb(4, "pi0"), fb(5, "thv"), fb(6, "ppterm"), fb(7, "piadv"),
           rb(8, "rth0"), fb(9, "buoy")});
     }
-    op("stage_tendency_reset", whole, {fb(1, "uten"), fb(2, "vten"), fb(3, "wten")});
-    if (tke || rayleigh)
-      for (const char* p : {"u", "v", "w"})
-        op("turb_add_tendency", whole,
-           {fb(1, std::string(p) + "ten_t"), fb(2, std::string(p) + "ten"), bb(3, F1(1.0f))});
+    if (cfg.fuse != 0 && (tke || rayleigh)) {  // reset + add of the SGS / damping tendencies in one pass
+      op("tendency_reset_add", whole,
+         {fb(1, "uten"), fb(2, "vten"), fb(3, "wten"), fb(4, "uten_t"), fb(5, "vten_t"), fb(6, "wten_t"),
+          bb(7, F1(1.0f))});
+    } else {
+      op("stage_tendency_reset", whole, {fb(1, "uten"), fb(2, "vten"), fb(3, "wten")});
+      if (tke || rayleigh)
+        for (const char* p : {"u", "v", "w"})
+          op("turb_add_tendency", whole,
+             {fb(1, std::string(p) + "ten_t"), fb(2, std::string(p) + "ten"), bb(3, F1(1.0f))});
+    }
     op("massflux_from_state", whole,
        {fb(1, "u"), fb(2, "v"), fb(3, "w"), rb(4, "rho0"), rb(5, "rf0"), fb(6, "rru"), fb(7, "rrv"), fb(8, "rrw")});
     auto computeDiv = [&] {

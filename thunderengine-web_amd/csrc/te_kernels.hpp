@@ -24,7 +24,7 @@
 // each replaces a run of reference dispatches with the same result on the
 // fields the step carries.
 #define TE_FUSED_LIST(X) X(bc_periodic_xy) X(zero_fields) X(copy_fields) X(ac_pp_accumulate) \
-  X(adv_fused) X(add_fields) X(acoustic_w_solve_lds) X(turb_scalar_vert_implicit_lds) X(turb_u_tendency_lds) \
+  X(adv_fused) X(add_fields) X(tendency_reset_add) X(acoustic_w_solve_lds) X(turb_scalar_vert_implicit_lds) X(turb_u_tendency_lds) \
   X(turb_v_tendency_lds) X(turb_w_tendency_lds) X(tke_transport_lds)
 
 namespace te {
