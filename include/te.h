@@ -134,7 +134,9 @@ typedef struct te_config {
 } te_config;
 
 /* Fills a config with the synthetic 64x64x40 case defaults (dx=dy=1 km,
- * dz=500 m, dt=6 s, nsound=6, moist, periodic, WENO scalars, WK82). */
+ * dz=500 m, dt=6 s, nsound=6, moist, periodic, WENO scalars, WK82), with the
+ * reference scenario's Rayleigh damping above 15 km and once-per-step
+ * moisture SGS fluxes. */
 void te_config_defaults(te_config* cfg);
 
 /* TimeStep.encode (timestep.js:797-2201) without a device: grid, profiles,

@@ -150,6 +150,24 @@ static int run_program(uint32_t* w, T* v, const char* const* names, const uint32
       K<T> ka{w, v, Y};
       if constexpr (sizeof(T) == 4) ea->f32(ka);
       else ea->f64(ka);
+      // the scalar follow-ups chained into the launch, as their own dispatches
+      const uint32_t chain = w[A[4 + 32]], TEND = A[4 + 11];
+      auto run1 = [&](const char* kn, const uint32_t* g3, std::vector<std::pair<int, uint32_t>> ps) {
+        uint32_t X[64] = {0};
+        X[0] = g3[0]; X[1] = g3[1]; X[2] = g3[2];
+        X[4] = A[4];
+        X[34] = A[34];
+        for (auto& pr : ps) X[4 + pr.first] = pr.second;
+        const KernelEntry* e1 = find_kernel(kn);
+        K<T> k1{w, v, X};
+        if constexpr (sizeof(T) == 4) e1->f32(k1);
+        else e1->f64(k1);
+      };
+      const uint32_t full[3] = {nx, ny, nz}, whole[3] = {nx + 7, ny + 7, nz + 1}, asz[3] = {A[0], A[1], A[2]};
+      if (chain & 1) run1("apply_thterm", full, {{1, A[4 + 33]}, {2, A[4 + 34]}, {3, TEND}});
+      if (chain & 2) run1("turb_add_tendency", whole, {{1, A[4 + 35]}, {2, TEND}, {3, A[4 + 36]}});
+      if (chain & 4) run1("turb_add_tendency", whole, {{1, A[4 + 37]}, {2, TEND}, {3, A[4 + 38]}});
+      if (chain & 8) run1("rk_update", asz, {{1, A[4 + 1]}, {2, A[4 + 39]}, {3, TEND}, {4, A[4 + 40]}, {5, A[4 + 41]}});
       continue;
     }
     if (nm == "tendency_reset_add") {  // stage_tendency_reset, then turb_add_tendency x3

@@ -296,8 +296,8 @@ __global__ __launch_bounds__(256) void k_adv_flux_z_weno_uv(uint32_t* M, KArg a)
 
 // adv_assemble.wgsl:139-279: flux divergence + mass-divergence term, divided
 // by the reference density; open-boundary inflow corrections included.
-template <class FLUX>
-TE_DEV void assemble_at(uint32_t* M, const KArg& a, const Dims& g, int i, int j, int k, FLUX F) {
+template <class FLUX, class STORE>
+TE_DEV void assemble_at(uint32_t* M, const KArg& a, const Dims& g, int i, int j, int k, FLUX F, STORE store) {
   int stag = ldi(M, a.p[1]);
   int st = stag < 0 ? -stag : stag;
   int iMax = st == 2 ? g.nx : g.nx - 1, jMax = st == 3 ? g.ny : g.ny - 1, kLo = st == 4 ? 1 : 0, kMax = g.nz - 1;
@@ -366,17 +366,21 @@ TE_DEV void assemble_at(uint32_t* M, const KArg& a, const Dims& g, int i, int j,
   float divF = fma_(dFz * g.rdz, mz, fma_(dFx * g.rdx, mxm, (dFy * g.rdy) * mym));
   float rho = st == 4 ? rf0[k] : rho0[k];
   float t = fma_(s[c], dv, -divF) / rho;
-  tend[c] = accumulate != 0 ? tend[c] + t : t;
+  store(c, accumulate != 0 ? tend[c] + t : t);
 }
 
 __global__ __launch_bounds__(256) void k_adv_assemble(uint32_t* M, KArg a) {
   TE_TID3
   Dims g = dims(M, a.p[0]);
   const float *fx = fp(M, a.p[3]), *fy = fp(M, a.p[4]), *fz = fp(M, a.p[5]);
-  assemble_at(M, a, g, gx, gy, gz, [&](int d, int i, int j, int k) {
-    const int64_t c = g.idx(i, j, k);
-    return d == 0 ? fx[c] : (d == 1 ? fy[c] : fz[c]);
-  });
+  float* tend = fp(M, a.p[11]);
+  assemble_at(
+      M, a, g, gx, gy, gz,
+      [&](int d, int i, int j, int k) {
+        const int64_t c = g.idx(i, j, k);
+        return d == 0 ? fx[c] : (d == 1 ? fy[c] : fz[c]);
+      },
+      [&](int64_t c, float v) { tend[c] = v; });
 }
 
 // The three flux dispatches of one advected field and adv_assemble in one
@@ -446,11 +450,28 @@ __device__ __forceinline__ void adv_fused_body(uint32_t* M, const KArg& a) {
   const float fyh = yNb ? ly[t + (int)blockDim.x] : F.template at<1>(i, ylo + 1, k);
   const int zlo = st == 4 ? k - 1 : k;
   const float fzl = F.template at<2>(i, j, zlo), fzh = F.template at<2>(i, j, zlo + 1);
-  assemble_at(M, a, g, i, j, k, [&](int d, int fi, int fj, int fk) {
-    if (d == 0) return fi == xlo ? fxl : fxh;
-    if (d == 1) return fj == ylo ? fyl : fyh;
-    return fk == zlo ? fzl : fzh;
-  });
+  // p[32] -> chain flags: the pointwise dispatches that follow a scalar's
+  // assembly in the reference, applied to the tendency in registers:
+  // 1 apply_thterm (tend = fma(-div3d, thterm, tend); p33 div3d, p34 thterm),
+  // 2 / 4 turb_add_tendency (tend = fma(scale, src, tend); p35/p36, p37/p38),
+  // 8 rk_update (phi = fma(dttmp, tend, phi_a); p39 phi_a, p40 phi, p41 dttmp).
+  // With rk_update in the chain the tendency itself is not stored.
+  const int chain = ldi(M, a.p[32]);
+  float* tend = fp(M, a.p[11]);
+  assemble_at(
+      M, a, g, i, j, k,
+      [&](int d, int fi, int fj, int fk) {
+        if (d == 0) return fi == xlo ? fxl : fxh;
+        if (d == 1) return fj == ylo ? fyl : fyh;
+        return fk == zlo ? fzl : fzh;
+      },
+      [&](int64_t c, float v) {
+        if (chain & 1) v = fma_(-fp(M, a.p[33])[c], fp(M, a.p[34])[c], v);
+        if (chain & 2) v = fma_(ldf(M, a.p[36]), fp(M, a.p[35])[c], v);
+        if (chain & 4) v = fma_(ldf(M, a.p[38]), fp(M, a.p[37])[c], v);
+        if (chain & 8) fp(M, a.p[40])[c] = fma_(ldf(M, a.p[41]), v, fp(M, a.p[39])[c]);
+        else tend[c] = v;
+      });
 }
 
 // The three flux dispatches of one advected field and adv_assemble in one

@@ -19,7 +19,7 @@ namespace te {
 // parameter inside the flat memory M (parameter n at p[n]).
 struct KArg {
   uint32_t n[3];
-  uint32_t p[32];
+  uint32_t p[48];  // parameters 0..31 as in the reference's slots; 32..47 for fused launches
 };
 
 #define TE_DEV __device__ __forceinline__

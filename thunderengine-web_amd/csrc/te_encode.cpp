@@ -245,6 +245,7 @@ class Encoder {
   void advect(int nrk, const std::string& s, int stagger, const std::string& into, bool accumulate,
               const std::string& state);
   void advectFluxes(int nrk, const std::string& s, int stagger, const std::string& target);
+  void moistureTurbulence(int nrk, const std::string& target);
   void afterAssemble(int nrk, const std::string& s, int stagger, const std::string& into, bool accumulate,
                      const std::string& target, const std::array<int, 3>& asmSize, const Bytes& dttmp,
                      const Bytes& open);
@@ -542,8 +543,41 @@ void Encoder::advect(int nrk, const std::string& s, int stagger, const std::stri
                             bb(12, I1(accumulate ? 1 : 0)), fb(13, "rru"), fb(14, "rrv"), rb(15, "c1"), rb(16, "c2"),
                             bb(17, open), rb(18, "rr0"), fb(19, "tend"), bb(20, pdef), bb(21, pdw), fb(22, "fx"),
                             fb(23, "fy"), fb(24, "fz"), bb(27, F1(0.0f))};
+    // A scalar's pointwise follow-ups (apply_thterm, the SGS / damping adds,
+    // rk_update) join the launch when nothing else writes its tendency in
+    // between: moisture SGS fluxes then go to their cache first (they read
+    // no tendency), which needs the cached mode.
+    const bool moistSgs = moist && tke && target != scalars[0];
+    const bool chained = stagger == 1 && !accumulate && (!moistSgs || cacheMT) && !(moist && hasMicro && usePT);
+    if (chained) {
+      if (moistSgs) moistureTurbulence(nrk, target);
+      int flags = 8;
+      if (moist && target == scalars[0]) {
+        flags |= 1;
+        bs.push_back(fb(33, "div3d"));
+        bs.push_back(fb(34, "thterm"));
+      }
+      std::string add;
+      if ((tke || rayleigh) && target == scalars[0]) add = "thten_t";
+      if (moistSgs) add = "qturb_" + target;
+      if (!add.empty()) {
+        flags |= 2;
+        bs.push_back(fb(35, add));
+        bs.push_back(bb(36, F1(1.0f)));
+      }
+      bs.push_back(bb(32, I1(flags)));
+      bs.push_back(fb(39, target + "_a"));
+      bs.push_back(fb(40, target));
+      bs.push_back(bb(41, dttmp));
+    } else {
+      bs.push_back(bb(32, I1(0)));
+    }
     op("adv_fused", asmSize, {});
     for (const Bind& bd : bs) E.ops.back().A[4 + bd.idx] = (uint32_t)bd.off;
+    if (chained) {
+      haloFill(target);
+      return;
+    }
   } else {
     advectFluxes(nrk, s, stagger, target);
     std::vector<Bind> bs = {bb(1, I1(stagger)), fb(2, s), fb(3, "fx"), fb(4, "fy"), fb(5, "fz"), fb(6, "divx"),
@@ -600,6 +634,26 @@ void Encoder::advectFluxes(int nrk, const std::string& s, int stagger, const std
   }
 }
 
+// Moisture SGS fluxes of one scalar (timestep.js:1826-1860): into its cache
+// (computed at the first stage only) or straight into tend.
+void Encoder::moistureTurbulence(int nrk, const std::string& target) {
+  const int nx = cfg.nx, ny = cfg.ny, nz = cfg.nz;
+  const std::array<int, 3> whole{nx + 7, ny + 7, nz + 1}, stag{nx + 1, ny + 1, nz}, cols{nx, ny, 1};
+  const std::string qa = target + "_a";
+  const std::string sink = cacheMT ? "qturb_" + target : "tend";
+  if (cacheMT && nrk == 0) op("turb_zero", whole, {fb(1, sink)});
+  if (!cacheMT || nrk == 0) {
+    op("turb_scalar_flux_h", stag, {fb(1, qa), fb(2, "rho"), fb(3, "khh"), bb(4, F1(1.0f)), fb(5, "fx"), fb(6, "fy")});
+    openTurbFlux();
+    std::vector<Bind> bs = {fb(1, qa), fb(2, "rho"), fb(3, "rr"), fb(4, "rf"), fb(5, "khv"), rb(6, "mh"),
+                            rb(7, "mf"), fb(8, "fx"), fb(9, "fy"), bb(10, F1(dtf)), fb(11, "cA"), fb(12, "cB"),
+                            fb(13, sink)};
+    bindSurfaceFlux(bs, qa == "qv_a" ? 10 : -1);
+    op("turb_scalar_vert_implicit", cols, {});
+    for (const Bind& bd : bs) E.ops.back().A[4 + bd.idx] = (uint32_t)bd.off;
+  }
+}
+
 void Encoder::afterAssemble(int nrk, const std::string& s, int stagger, const std::string& into, bool accumulate,
                             const std::string& target, const std::array<int, 3>& asmSize, const Bytes& dttmp,
                             const Bytes& open) {
@@ -624,20 +678,8 @@ void Encoder::afterAssemble(int nrk, const std::string& s, int stagger, const st
   if ((tke || rayleigh) && target == scalars[0])
     op("turb_add_tendency", whole, {fb(1, "thten_t"), fb(2, "tend"), bb(3, F1(1.0f))});
   if (moist && tke && target != scalars[0]) {
-    const std::string qa = target + "_a";
-    const std::string sink = cacheMT ? "qturb_" + target : "tend";
-    if (cacheMT && nrk == 0) op("turb_zero", whole, {fb(1, sink)});
-    if (!cacheMT || nrk == 0) {
-      op("turb_scalar_flux_h", stag, {fb(1, qa), fb(2, "rho"), fb(3, "khh"), bb(4, F1(1.0f)), fb(5, "fx"), fb(6, "fy")});
-      openTurbFlux();
-      std::vector<Bind> bs = {fb(1, qa), fb(2, "rho"), fb(3, "rr"), fb(4, "rf"), fb(5, "khv"), rb(6, "mh"),
-                              rb(7, "mf"), fb(8, "fx"), fb(9, "fy"), bb(10, F1(dtf)), fb(11, "cA"), fb(12, "cB"),
-                              fb(13, sink)};
-      bindSurfaceFlux(bs, qa == "qv_a" ? 10 : -1);
-      op("turb_scalar_vert_implicit", cols, {});
-      for (const Bind& bd : bs) E.ops.back().A[4 + bd.idx] = (uint32_t)bd.off;
-    }
-    if (cacheMT) op("turb_add_tendency", whole, {fb(1, sink), fb(2, "tend"), bb(3, F1(1.0f))});
+    moistureTurbulence(nrk, target);
+    if (cacheMT) op("turb_add_tendency", whole, {fb(1, "qturb_" + target), fb(2, "tend"), bb(3, F1(1.0f))});
   }
   if (moist && hasMicro && usePT) {
     const char* src = target == "th" ? "qtten" : (target == "qv" ? "qvten" : (target == "qc" ? "qcten" : nullptr));
@@ -907,4 +949,8 @@ extern "C" void te_config_defaults(te_config* c) {
   c->bubble_amp = 2.0f; c->bubble_rh = 10000.0f; c->bubble_rv = 1400.0f; c->bubble_zc = 1400.0f;
   c->noise_sigma = 0.01f; c->seed = 20260101ull;
   c->fuse = 1;
+  // SupercellScenario defaults (scenario.js:67-71): Rayleigh damping above
+  // 15 km and moisture SGS fluxes computed once per step
+  c->rayleigh = 1;
+  c->cache_moisture_turbulence = 1;
 }

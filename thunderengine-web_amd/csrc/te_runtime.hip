@@ -52,7 +52,7 @@ static dim3 block_for(const uint32_t* n) {
 static KArg karg_of(const Op& op) {
   KArg a;
   a.n[0] = op.A[0]; a.n[1] = op.A[1]; a.n[2] = op.A[2];
-  for (int q = 0; q < 32; ++q) a.p[q] = op.A[4 + q];
+  for (int q = 0; q < 48; ++q) a.p[q] = op.A[4 + q];
   return a;
 }
 
@@ -148,7 +148,7 @@ static int validate_ops(te_ctx* c) {
       for (int q = 1; q <= 8; ++q)
         if ((uint64_t)op.A[4 + q] + op.A[0] > c->words)
           return fail(c, TE_ERR_ARG, "copy_fields op " + std::to_string(i) + " leaves the memory");
-    for (int q = 0; q < 32; ++q)
+    for (int q = 0; q < 48; ++q)
       if (op.A[4 + q] >= c->words) return fail(c, TE_ERR_ARG, "op " + std::to_string(i) + " (" + nm + ") parameter offset outside memory");
   }
   return TE_OK;
