@@ -143,6 +143,15 @@ static int run_program(uint32_t* w, T* v, const char* const* names, const uint32
         if constexpr (sizeof(T) == 4) e1->f32(k1);
         else e1->f64(k1);
       }
+      if (w[A[4 + 32]] & 16u) {  // turb_zero of the accumulating sink
+        uint32_t Z[64] = {0};
+        Z[0] = nx + 7; Z[1] = ny + 7; Z[2] = nz + 1;
+        Z[4] = A[4]; Z[5] = A[4 + 11]; Z[34] = A[34];
+        const KernelEntry* ez = find_kernel("turb_zero");
+        K<T> kz{w, v, Z};
+        if constexpr (sizeof(T) == 4) ez->f32(kz);
+        else ez->f64(kz);
+      }
       uint32_t Y[64];
       std::memcpy(Y, A, sizeof(Y));
       Y[4 + 3] = FX; Y[4 + 4] = FY; Y[4 + 5] = FZ;
@@ -168,6 +177,10 @@ static int run_program(uint32_t* w, T* v, const char* const* names, const uint32
       if (chain & 2) run1("turb_add_tendency", whole, {{1, A[4 + 35]}, {2, TEND}, {3, A[4 + 36]}});
       if (chain & 4) run1("turb_add_tendency", whole, {{1, A[4 + 37]}, {2, TEND}, {3, A[4 + 38]}});
       if (chain & 8) run1("rk_update", asz, {{1, A[4 + 1]}, {2, A[4 + 39]}, {3, TEND}, {4, A[4 + 40]}, {5, A[4 + 41]}});
+      if (chain & 32) {
+        const uint32_t fg[3] = {nx, ny, nz};
+        run1("tke_advance", fg, {{1, A[4 + 39]}, {2, TEND}, {3, A[4 + 41]}, {4, A[4 + 40]}});
+      }
       continue;
     }
     if (nm == "tendency_reset_add") {  // stage_tendency_reset, then turb_add_tendency x3

@@ -59,7 +59,7 @@ class OracleRun:
 # reference's dispatch list (fluxes, elimination coefficients, fpk, tend,
 # sadv).  tests/test_oracle_invariants.py checks that each is dead at the
 # start of a step: its contents then never reach another field.
-FUSED_SCRATCH = ("fx", "fy", "fz", "cA", "cB", "fpk", "tend", "sadv")
+FUSED_SCRATCH = ("fx", "fy", "fz", "cA", "cB", "fpk", "tend", "sadv", "tketen_s")
 
 
 def state_mask(e):

@@ -366,7 +366,7 @@ TE_DEV void assemble_at(uint32_t* M, const KArg& a, const Dims& g, int i, int j,
   float divF = fma_(dFz * g.rdz, mz, fma_(dFx * g.rdx, mxm, (dFy * g.rdy) * mym));
   float rho = st == 4 ? rf0[k] : rho0[k];
   float t = fma_(s[c], dv, -divF) / rho;
-  store(c, accumulate != 0 ? tend[c] + t : t);
+  store(c, t, accumulate);
 }
 
 __global__ __launch_bounds__(256) void k_adv_assemble(uint32_t* M, KArg a) {
@@ -380,7 +380,7 @@ __global__ __launch_bounds__(256) void k_adv_assemble(uint32_t* M, KArg a) {
         const int64_t c = g.idx(i, j, k);
         return d == 0 ? fx[c] : (d == 1 ? fy[c] : fz[c]);
       },
-      [&](int64_t c, float v) { tend[c] = v; });
+      [&](int64_t c, float t, int acc) { tend[c] = acc != 0 ? tend[c] + t : t; });
 }
 
 // The three flux dispatches of one advected field and adv_assemble in one
@@ -450,12 +450,14 @@ __device__ __forceinline__ void adv_fused_body(uint32_t* M, const KArg& a) {
   const float fyh = yNb ? ly[t + (int)blockDim.x] : F.template at<1>(i, ylo + 1, k);
   const int zlo = st == 4 ? k - 1 : k;
   const float fzl = F.template at<2>(i, j, zlo), fzh = F.template at<2>(i, j, zlo + 1);
-  // p[32] -> chain flags: the pointwise dispatches that follow a scalar's
-  // assembly in the reference, applied to the tendency in registers:
+  // p[32] -> chain flags: the pointwise dispatches around a field's assembly
+  // in the reference, applied to the tendency in registers:
+  // 16 turb_zero of the accumulating sink before it (tend = 0 + t),
   // 1 apply_thterm (tend = fma(-div3d, thterm, tend); p33 div3d, p34 thterm),
   // 2 / 4 turb_add_tendency (tend = fma(scale, src, tend); p35/p36, p37/p38),
-  // 8 rk_update (phi = fma(dttmp, tend, phi_a); p39 phi_a, p40 phi, p41 dttmp).
-  // With rk_update in the chain the tendency itself is not stored.
+  // 8 rk_update (phi = fma(dttmp, tend, phi_a); p39 phi_a, p40 phi, p41 dttmp)
+  // or 32 tke_advance (the same, 0 below 1e-10).
+  // With an update in the chain the tendency itself is not stored.
   const int chain = ldi(M, a.p[32]);
   float* tend = fp(M, a.p[11]);
   assemble_at(
@@ -465,12 +467,19 @@ __device__ __forceinline__ void adv_fused_body(uint32_t* M, const KArg& a) {
         if (d == 1) return fj == ylo ? fyl : fyh;
         return fk == zlo ? fzl : fzh;
       },
-      [&](int64_t c, float v) {
+      [&](int64_t c, float t, int acc) {
+        float v = (chain & 16) ? 0.0f + t : (acc != 0 ? tend[c] + t : t);
         if (chain & 1) v = fma_(-fp(M, a.p[33])[c], fp(M, a.p[34])[c], v);
         if (chain & 2) v = fma_(ldf(M, a.p[36]), fp(M, a.p[35])[c], v);
         if (chain & 4) v = fma_(ldf(M, a.p[38]), fp(M, a.p[37])[c], v);
-        if (chain & 8) fp(M, a.p[40])[c] = fma_(ldf(M, a.p[41]), v, fp(M, a.p[39])[c]);
-        else tend[c] = v;
+        if (chain & 8) {
+          fp(M, a.p[40])[c] = fma_(ldf(M, a.p[41]), v, fp(M, a.p[39])[c]);
+        } else if (chain & 32) {
+          const float u = fma_(ldf(M, a.p[41]), v, fp(M, a.p[39])[c]);
+          fp(M, a.p[40])[c] = (u < 1e-10f) ? 0.0f : u;
+        } else {
+          tend[c] = v;
+        }
       });
 }
 

@@ -549,7 +549,17 @@ void Encoder::advect(int nrk, const std::string& s, int stagger, const std::stri
     // no tendency), which needs the cached mode.
     const bool moistSgs = moist && tke && target != scalars[0];
     const bool chained = stagger == 1 && !accumulate && (!moistSgs || cacheMT) && !(moist && hasMicro && usePT);
-    if (chained) {
+    const bool tkeChained = stagger == 4 && s == "tke" && accumulate && into == "tketen_s";
+    if (tkeChained) {
+      // turb_zero(tketen_s), this assembly, turb_add_tendency(tketen) and
+      // tke_advance (timestep.js TKE stage update) in one launch
+      bs.push_back(bb(32, I1(16 | 2 | 32)));
+      bs.push_back(fb(35, "tketen"));
+      bs.push_back(bb(36, F1(1.0f)));
+      bs.push_back(fb(39, "tke_a"));
+      bs.push_back(fb(40, "tke"));
+      bs.push_back(bb(41, dttmp));
+    } else if (chained) {
       if (moistSgs) moistureTurbulence(nrk, target);
       int flags = 8;
       if (moist && target == scalars[0]) {
@@ -578,6 +588,7 @@ void Encoder::advect(int nrk, const std::string& s, int stagger, const std::stri
       haloFill(target);
       return;
     }
+    if (tkeChained) return;
   } else {
     advectFluxes(nrk, s, stagger, target);
     std::vector<Bind> bs = {bb(1, I1(stagger)), fb(2, s), fb(3, "fx"), fb(4, "fy"), fb(5, "fz"), fb(6, "divx"),
@@ -732,10 +743,14 @@ void Encoder::stages() {
     advect(nrk, "v", 3, "vten", true, "");
     advect(nrk, "w", 4, "wten", true, "");
     if (tke) {
-      op("turb_zero", whole, {fb(1, "tketen_s")});
-      advect(nrk, "tke", 4, "tketen_s", true, "");
-      op("turb_add_tendency", whole, {fb(1, "tketen"), fb(2, "tketen_s"), bb(3, F1(1.0f))});
-      op("tke_advance", full, {fb(1, "tke_a"), fb(2, "tketen_s"), bb(3, dttmp), fb(4, "tke")});
+      if (cfg.fuse != 0) {  // zero, assembly, add and advance in one launch
+        advect(nrk, "tke", 4, "tketen_s", true, "");
+      } else {
+        op("turb_zero", whole, {fb(1, "tketen_s")});
+        advect(nrk, "tke", 4, "tketen_s", true, "");
+        op("turb_add_tendency", whole, {fb(1, "tketen"), fb(2, "tketen_s"), bb(3, F1(1.0f))});
+        op("tke_advance", full, {fb(1, "tke_a"), fb(2, "tketen_s"), bb(3, dttmp), fb(4, "tke")});
+      }
       haloFill("tke");
     }
     op("buoyancy_apply", {nx, ny, std::max(nz - 1, 1)}, {fb(1, "buoy"), rb(2, "c1"), rb(3, "c2"), fb(4, "wten")});
