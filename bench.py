@@ -44,23 +44,78 @@ def kernel_io():
     return json.load(open(p))
 
 
+def _flag(run, A, op, idx):
+    """Integer word that parameter `idx` of dispatch `op` points at."""
+    return int(run.read(int(A[op, 4 + idx]), 1).view(np.int32)[0])
+
+
+def adv_fused_io(run, A, op):
+    """(param index, read, written) of one fused advection launch (k_adv_fused
+    in csrc/k_adv.hip): the advected field, the three mass fluxes, divergence,
+    base-state density and map factors, then either the tendency or the fields
+    the chained follow-ups (thterm, SGS adds, RK update) read and write."""
+    chain, acc = _flag(run, A, op, 32), _flag(run, A, op, 12)
+    io = [(i, True, False) for i in (2, 4, 6, 7, 8, 9, 10, 13, 14, 15, 16, 30)]
+    if chain & 1:
+        io += [(33, True, False), (34, True, False)]
+    if chain & 2:
+        io.append((35, True, False))
+    if chain & 4:
+        io.append((37, True, False))
+    if chain & (8 | 32):
+        io += [(39, True, False), (40, False, True)]
+    else:
+        io.append((11, bool(acc) and not chain & 16, True))
+    return io
+
+
 def alg_bytes_per_launch(run, names, A, op):
     """Distinct arena fields a dispatch reads plus those it writes, over its
     launch grid, x 4 B (each field touched once, as a fused kernel would)."""
-    io = kernel_io().get(names[op])
-    if io is None:
-        return None
+    name = names[op][:-4] if names[op].endswith("_lds") else names[op]
     a0 = run.field_offset(run.fields[0])
     end = a0 + run.field_words * len(run.fields)
     pts = int(A[op, 0]) * int(A[op, 1]) * int(A[op, 2])
-    if names[op] in COLUMN_KERNELS:  # one lane per column sweeps all levels
+    if name in COLUMN_KERNELS:  # one lane per column sweeps all levels
         pts *= run.nz
-    n = 0
-    for idx, read, written in io:
-        off = int(A[op, 4 + idx])
-        if a0 <= off < end:
-            n += int(read) + int(written)
-    return 4.0 * n * pts
+
+    def count(io):
+        rd, wr = set(), set()
+        for idx, read, written in io:
+            off = int(A[op, 4 + idx])
+            if a0 <= off < end:
+                if read:
+                    rd.add(off)
+                if written:
+                    wr.add(off)
+        return len(rd) + len(wr)
+
+    if name == "ac_pp_accumulate":  # pi' update on the interior, then the time average on the whole grid
+        inner = run.nx * run.ny * run.nz
+        save_x = _flag(run, A, op, 10) != 0
+        pp = count([(3, 1, 0), (4, 1, 0), (5, 1, 0), (6, 1, 0), (7, 1, 1), (8, 0, 1)] + ([(9, 0, 1)] if save_x else []))
+        avg = count([(2, 1, 0), (11, 1, 0), (12, 1, 0), (13, 1, 1), (14, 1, 1), (15, 1, 1)])
+        return 4.0 * (pp * inner + avg * pts)
+    if name == "bc_periodic_xy":  # each halo cell reads its interior image and writes itself
+        n = _flag(run, A, op, 9)
+        io = [(1 + q, True, True) for q in range(n)]
+    elif name == "zero_fields":
+        io = [(1 + q, False, True) for q in range(_flag(run, A, op, 9))]
+    elif name == "add_fields":
+        n = _flag(run, A, op, 13)
+        io = [(1 + q, True, False) for q in range(n)] + [(5 + q, True, True) for q in range(n)]
+    elif name == "copy_fields":
+        n = _flag(run, A, op, 9)
+        io = [(1 + q, True, False) for q in range(n)] + [(5 + q, False, True) for q in range(n)]
+    elif name == "tendency_reset_add":
+        io = [(1 + q, False, True) for q in range(3)] + [(4 + q, True, False) for q in range(3)]
+    elif name == "adv_fused":
+        io = adv_fused_io(run, A, op)
+    else:
+        io = kernel_io().get(name)
+    if io is None:
+        return None
+    return 4.0 * count(io) * pts
 
 
 def profiles_traffic(kernel):
@@ -181,8 +236,10 @@ def main():
     dom = max(per_kernel, key=lambda k: per_kernel[k][0])
     dom_ops = [i for i, n in enumerate(names) if n == dom]
     avg_ms = per_kernel[dom][0] / len(dom_ops)
-    alg = [alg_bytes_per_launch(run, names, A, i) for i in dom_ops]
+    alg_all = [alg_bytes_per_launch(run, names, A, i) for i in range(len(names))]
+    alg = [alg_all[i] for i in dom_ops]
     alg_b = float(np.mean(alg)) if all(x is not None for x in alg) else None
+    kernels_alg_pt = sum(x for x in alg_all if x) / pts
     achieved = alg_b / (avg_ms * 1e-3) / 1e9 if alg_b else None
     traffic = profiles_traffic(dom)
     roof = {"bound": "hbm", "kernel": dom, "launches_per_step": len(dom_ops), "avg_launch_ms": round(avg_ms, 5),
@@ -190,6 +247,11 @@ def main():
             "achieved": round(achieved, 1) if achieved else None, "peak": PEAK_HBM_GBS, "unit": "GB/s",
             "frac": round(achieved / PEAK_HBM_GBS, 4) if achieved else None, "traffic": traffic}
     step_gbs = value * 1e6 * B_ALG_STEP / 1e9
+    top_frac = {}
+    for k, (t, _) in sorted(per_kernel.items(), key=lambda kv: -kv[1][0])[:8]:
+        b = [alg_all[i] for i, n in enumerate(names) if n == k]
+        if t > 0 and all(x is not None for x in b):
+            top_frac[k] = round(sum(b) / (t * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)
 
     out = {
         "metric": "Mpt-steps/s FP32 at 192^2x80 (%HBM peak)",
@@ -200,10 +262,12 @@ def main():
                    "nx": cfg.nx, "ny": cfg.ny, "nz": cfg.nz, "substeps_per_step": 11, "ops_per_step": len(names),
                    "parallelism": "single GPU"},
         "step_hbm": {"alg_bytes_per_pt_step": B_ALG_STEP, "achieved_GBs": round(step_gbs, 1),
-                     "frac": round(step_gbs / PEAK_HBM_GBS, 4)},
+                     "frac": round(step_gbs / PEAK_HBM_GBS, 4),
+                     "launch_alg_bytes_per_pt_step": round(kernels_alg_pt, 1)},
         "roofline": roof,
         "top_kernels_ms_per_step": {k: round(v[0], 4) for k, v in
                                     sorted(per_kernel.items(), key=lambda kv: -kv[1][0])[:8]},
+        "top_kernels_hbm_frac": top_frac,
     }
     if args.ensemble:
         run.close()

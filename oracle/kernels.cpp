@@ -177,10 +177,6 @@ static int run_program(uint32_t* w, T* v, const char* const* names, const uint32
       if (chain & 2) run1("turb_add_tendency", whole, {{1, A[4 + 35]}, {2, TEND}, {3, A[4 + 36]}});
       if (chain & 4) run1("turb_add_tendency", whole, {{1, A[4 + 37]}, {2, TEND}, {3, A[4 + 38]}});
       if (chain & 8) run1("rk_update", asz, {{1, A[4 + 1]}, {2, A[4 + 39]}, {3, TEND}, {4, A[4 + 40]}, {5, A[4 + 41]}});
-      if (chain & 32) {
-        const uint32_t fg[3] = {nx, ny, nz};
-        run1("tke_advance", fg, {{1, A[4 + 39]}, {2, TEND}, {3, A[4 + 41]}, {4, A[4 + 40]}});
-      }
       continue;
     }
     if (nm == "tendency_reset_add") {  // stage_tendency_reset, then turb_add_tendency x3

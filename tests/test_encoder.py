@@ -127,3 +127,52 @@ def test_encoder_rejects_bad_config():
         te_host.Run(te_host.config(species=5), encode_only=True)
     with pytest.raises(te_host.TeError):  # automatic schedule beyond 192 substeps
         te_host.Run(te_host.config(nsound=0, dt=300.0, dx=100.0, dy=100.0), encode_only=True)
+
+
+HAZARD_CONFIGS = [
+    dict(nx=24, ny=20, nz=12, tke=1, rayleigh=1),
+    dict(nx=24, ny=20, nz=12, tke=1, species=7, weno_momentum=1),
+    dict(nx=24, ny=20, nz=12, tke=0, moist=0, rayleigh=1),
+    dict(nx=24, ny=20, nz=12, tke=1, open_x=1, open_y=1, mass_conservation=1),
+]
+
+
+@pytest.mark.parametrize("kw", HAZARD_CONFIGS, ids=lambda kw: "-".join(f"{k}{v}" for k, v in kw.items()))
+def test_fused_launches_write_no_field_their_neighbours_read(kw):
+    """A launch whose workgroups read a field at neighbouring points must not
+    write that field: workgroups run in no fixed order, so the reads could see
+    the new values.  The oracle runs a fused launch as the reference dispatches
+    it replaces, one after another, and cannot show such a race; this checks
+    the bindings of every fused launch instead."""
+    r = te_host.Run(te_host.config(fuse=1, **kw), encode_only=True)
+    names, A = r.program()
+
+    def word(op, idx):
+        return int(r.read(int(A[op, 4 + idx]), 1).view(np.int32)[0])
+
+    seen = set()
+    for op, name in enumerate(names):
+        off = lambda idx: int(A[op, 4 + idx])  # noqa: E731
+        if name == "adv_fused":
+            stencil = {off(2), off(4), off(13), off(14)}
+            chain = word(op, 32)
+            written = {off(40)} if chain & 8 else {off(11)}
+            assert not written & stencil, (op, name, chain)
+            if chain & 8:  # the chained update writes the target, reads its start-of-step copy
+                assert off(39) != off(40)
+            seen.add(name)
+        elif name == "ac_pp_accumulate":  # w is read at k and k+1
+            written = {off(q) for q in (7, 8, 9, 13, 14, 15)}
+            assert off(2) not in written, (op, name)
+            seen.add(name)
+        elif name == "add_fields":  # independent triples
+            n = word(op, 13)
+            srcs, dsts = {off(1 + q) for q in range(n)}, [off(5 + q) for q in range(n)]
+            assert len(set(dsts)) == n and not srcs & set(dsts), (op, name)
+            seen.add(name)
+        elif name == "copy_fields":
+            n = word(op, 9)
+            srcs, dsts = {off(1 + q) for q in range(n)}, [off(5 + q) for q in range(n)]
+            assert len(set(dsts)) == n and not srcs & set(dsts), (op, name)
+            seen.add(name)
+    assert "adv_fused" in seen and "ac_pp_accumulate" in seen

@@ -99,7 +99,8 @@ __global__ __launch_bounds__(256) void k_acoustic_fpk(uint32_t* M, KArg a) {
 #define WS_CHUNK 8
 template <bool LDS>
 __device__ void w_solve(uint32_t* M, const KArg& a) {
-  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x), j = (int)(blockIdx.y * blockDim.y + threadIdx.y);
+  const Blk b_ = te_block();
+  const int i = (int)(b_.x * blockDim.x + threadIdx.x), j = (int)(b_.y * blockDim.y + threadIdx.y);
   if (i >= (int)a.n[0] || j >= (int)a.n[1]) return;
   Dims g = dims(M, a.p[0]);
   if (i >= g.nx || j >= g.ny) return;

@@ -296,9 +296,10 @@ __global__ __launch_bounds__(256) void k_adv_flux_z_weno_uv(uint32_t* M, KArg a)
 
 // adv_assemble.wgsl:139-279: flux divergence + mass-divergence term, divided
 // by the reference density; open-boundary inflow corrections included.
-template <class FLUX, class STORE>
+// STC != 0: the stagger is known at compile time (specialised launches).
+template <int STC = 0, class FLUX, class STORE>
 TE_DEV void assemble_at(uint32_t* M, const KArg& a, const Dims& g, int i, int j, int k, FLUX F, STORE store) {
-  int stag = ldi(M, a.p[1]);
+  const int stag = STC != 0 ? STC : ldi(M, a.p[1]);
   int st = stag < 0 ? -stag : stag;
   int iMax = st == 2 ? g.nx : g.nx - 1, jMax = st == 3 ? g.ny : g.ny - 1, kLo = st == 4 ? 1 : 0, kMax = g.nz - 1;
   if (i > iMax || j > jMax || k < kLo || k > kMax) return;
@@ -421,16 +422,16 @@ struct FusedFlux {
   }
 };
 
-template <int KIND>
+template <int KIND, int STC>
 __device__ __forceinline__ void adv_fused_body(uint32_t* M, const KArg& a) {
   __shared__ float lx[256], ly[256];
   const int tx = threadIdx.x, ty = threadIdx.y, tz = threadIdx.z;
-  const int i = (int)(blockIdx.x * blockDim.x) + tx, j = (int)(blockIdx.y * blockDim.y) + ty,
-            k = (int)(blockIdx.z * blockDim.z) + tz;
+  const Blk b_ = te_block();
+  const int i = (int)(b_.x * blockDim.x) + tx, j = (int)(b_.y * blockDim.y) + ty, k = (int)(b_.z * blockDim.z) + tz;
   const bool inGrid = i < (int)a.n[0] && j < (int)a.n[1] && k < (int)a.n[2];
   const int t = tx + (int)blockDim.x * (ty + (int)blockDim.y * tz);
   Dims g = dims(M, a.p[0]);
-  const int stag = ldi(M, a.p[1]), st = stag < 0 ? -stag : stag;
+  const int stag = STC != 0 ? STC : ldi(M, a.p[1]), st = stag < 0 ? -stag : stag;
   const FusedFlux<KIND> F{g, stag, ldi(M, a.p[20]), ldi(M, a.p[21]), fp(M, a.p[2]), fp(M, a.p[13]), fp(M, a.p[14]),
                           fp(M, a.p[4]), fp(M, a.p[15]), fp(M, a.p[16]), ldf(M, a.p[5])};
   // low faces of this cell's x and y divergence, published for the neighbours
@@ -455,12 +456,12 @@ __device__ __forceinline__ void adv_fused_body(uint32_t* M, const KArg& a) {
   // 16 turb_zero of the accumulating sink before it (tend = 0 + t),
   // 1 apply_thterm (tend = fma(-div3d, thterm, tend); p33 div3d, p34 thterm),
   // 2 / 4 turb_add_tendency (tend = fma(scale, src, tend); p35/p36, p37/p38),
-  // 8 rk_update (phi = fma(dttmp, tend, phi_a); p39 phi_a, p40 phi, p41 dttmp)
-  // or 32 tke_advance (the same, 0 below 1e-10).
+  // 8 rk_update (phi = fma(dttmp, tend, phi_a); p39 phi_a, p40 phi, p41 dttmp;
+  // phi must not be the advected field, which other workgroups still read).
   // With an update in the chain the tendency itself is not stored.
   const int chain = ldi(M, a.p[32]);
   float* tend = fp(M, a.p[11]);
-  assemble_at(
+  assemble_at<STC>(
       M, a, g, i, j, k,
       [&](int d, int fi, int fj, int fk) {
         if (d == 0) return fi == xlo ? fxl : fxh;
@@ -474,9 +475,6 @@ __device__ __forceinline__ void adv_fused_body(uint32_t* M, const KArg& a) {
         if (chain & 4) v = fma_(ldf(M, a.p[38]), fp(M, a.p[37])[c], v);
         if (chain & 8) {
           fp(M, a.p[40])[c] = fma_(ldf(M, a.p[41]), v, fp(M, a.p[39])[c]);
-        } else if (chain & 32) {
-          const float u = fma_(ldf(M, a.p[41]), v, fp(M, a.p[39])[c]);
-          fp(M, a.p[40])[c] = (u < 1e-10f) ? 0.0f : u;
         } else {
           tend[c] = v;
         }
@@ -486,12 +484,18 @@ __device__ __forceinline__ void adv_fused_body(uint32_t* M, const KArg& a) {
 // The three flux dispatches of one advected field and adv_assemble in one
 // launch (see above); the flux kind selects one specialised body.
 __global__ __launch_bounds__(256) void k_adv_fused(uint32_t* M, KArg a) {
-  switch (ldi(M, a.p[3])) {
-    case 0: adv_fused_body<0>(M, a); break;
-    case 1: adv_fused_body<1>(M, a); break;
-    case 2: adv_fused_body<2>(M, a); break;
-    default: adv_fused_body<3>(M, a); break;
-  }
+  const int kind = ldi(M, a.p[3]), stag = ldi(M, a.p[1]);
+  // the (kind, stagger) pairs the encoder emits get bodies specialised on both
+  if (kind == 0 && stag == 1) adv_fused_body<0, 1>(M, a);
+  else if (kind == 1 && stag == 1) adv_fused_body<1, 1>(M, a);
+  else if (kind == 2 && stag == 2) adv_fused_body<2, 2>(M, a);
+  else if (kind == 2 && stag == 3) adv_fused_body<2, 3>(M, a);
+  else if (kind == 2 && stag == 4) adv_fused_body<2, 4>(M, a);
+  else if (kind == 3 && stag == 4) adv_fused_body<3, 4>(M, a);
+  else if (kind == 0) adv_fused_body<0, 0>(M, a);
+  else if (kind == 1) adv_fused_body<1, 0>(M, a);
+  else if (kind == 2) adv_fused_body<2, 0>(M, a);
+  else adv_fused_body<3, 0>(M, a);
 }
 
 }  // namespace te

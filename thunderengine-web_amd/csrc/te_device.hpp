@@ -226,10 +226,41 @@ struct ColStore {
 #define TE_LANE ((int)(threadIdx.x + blockDim.x * threadIdx.y))
 #define TE_NLANES ((int)(blockDim.x * blockDim.y))
 
+// Workgroup coordinates, renumbered for the 8 XCDs.  The dispatcher hands
+// workgroups to the XCDs round-robin in dispatch order (workgroup w, counted
+// i-fastest over the grid, runs on XCD w mod 8), and each XCD has its own L2.
+// Neighbouring workgroups share cache lines (row ends of 199-word rows,
+// stencil halos), so numbering the grid such that XCD x runs the x-th
+// contiguous eighth of it keeps those lines in one L2 instead of fetching
+// (and partially writing) them from two.  A bijection of the workgroups:
+// XCD x holds q + (x < r) of the n = 8q + r of them.
+#ifndef TE_XCD_REMAP
+#define TE_XCD_REMAP 1
+#endif
+struct Blk {
+  int x, y, z;
+};
+__device__ __forceinline__ Blk te_block() {
+  const uint32_t nbx = gridDim.x, nby = gridDim.y;
+#if TE_XCD_REMAP
+  const uint32_t n = nbx * nby * gridDim.z;
+  const uint32_t w = blockIdx.x + nbx * (blockIdx.y + nby * blockIdx.z);
+  const uint32_t q = n >> 3, r = n & 7u, x = w & 7u;
+  const uint32_t l = x * q + (x < r ? x : r) + (w >> 3);
+  const uint32_t yz = l / nbx;
+  return Blk{(int)(l - yz * nbx), (int)(yz % nby), (int)(yz / nby)};
+#else
+  (void)nbx;
+  (void)nby;
+  return Blk{(int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z};
+#endif
+}
+
 #define TE_TID3                                                          \
-  const int gx = (int)(blockIdx.x * blockDim.x + threadIdx.x);           \
-  const int gy = (int)(blockIdx.y * blockDim.y + threadIdx.y);           \
-  const int gz = (int)(blockIdx.z * blockDim.z + threadIdx.z);           \
+  const Blk te_b_ = te_block();                                          \
+  const int gx = (int)(te_b_.x * blockDim.x + threadIdx.x);              \
+  const int gy = (int)(te_b_.y * blockDim.y + threadIdx.y);              \
+  const int gz = (int)(te_b_.z * blockDim.z + threadIdx.z);              \
   if (gx >= (int)a.n[0] || gy >= (int)a.n[1] || gz >= (int)a.n[2]) return;
 
 }  // namespace te

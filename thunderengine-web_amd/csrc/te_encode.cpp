@@ -546,22 +546,22 @@ void Encoder::advect(int nrk, const std::string& s, int stagger, const std::stri
     // A scalar's pointwise follow-ups (apply_thterm, the SGS / damping adds,
     // rk_update) join the launch when nothing else writes its tendency in
     // between: moisture SGS fluxes then go to their cache first (they read
-    // no tendency), which needs the cached mode.
+    // no tendency), which needs the cached mode.  rk_update joins only when
+    // the advected field is not the one it writes (theta advects sadv): other
+    // workgroups of the launch still read the advected field's neighbours.
     const bool moistSgs = moist && tke && target != scalars[0];
     const bool chained = stagger == 1 && !accumulate && (!moistSgs || cacheMT) && !(moist && hasMicro && usePT);
+    const bool update = chained && s != target;
     const bool tkeChained = stagger == 4 && s == "tke" && accumulate && into == "tketen_s";
     if (tkeChained) {
-      // turb_zero(tketen_s), this assembly, turb_add_tendency(tketen) and
-      // tke_advance (timestep.js TKE stage update) in one launch
-      bs.push_back(bb(32, I1(16 | 2 | 32)));
+      // turb_zero(tketen_s), this assembly and turb_add_tendency(tketen) in
+      // one launch; tke_advance (which writes the advected tke) follows
+      bs.push_back(bb(32, I1(16 | 2)));
       bs.push_back(fb(35, "tketen"));
       bs.push_back(bb(36, F1(1.0f)));
-      bs.push_back(fb(39, "tke_a"));
-      bs.push_back(fb(40, "tke"));
-      bs.push_back(bb(41, dttmp));
     } else if (chained) {
       if (moistSgs) moistureTurbulence(nrk, target);
-      int flags = 8;
+      int flags = 0;
       if (moist && target == scalars[0]) {
         flags |= 1;
         bs.push_back(fb(33, "div3d"));
@@ -575,16 +575,21 @@ void Encoder::advect(int nrk, const std::string& s, int stagger, const std::stri
         bs.push_back(fb(35, add));
         bs.push_back(bb(36, F1(1.0f)));
       }
+      if (update) {
+        flags |= 8;
+        bs.push_back(fb(39, target + "_a"));
+        bs.push_back(fb(40, target));
+        bs.push_back(bb(41, dttmp));
+      }
       bs.push_back(bb(32, I1(flags)));
-      bs.push_back(fb(39, target + "_a"));
-      bs.push_back(fb(40, target));
-      bs.push_back(bb(41, dttmp));
     } else {
       bs.push_back(bb(32, I1(0)));
     }
     op("adv_fused", asmSize, {});
     for (const Bind& bd : bs) E.ops.back().A[4 + bd.idx] = (uint32_t)bd.off;
     if (chained) {
+      if (!update)
+        op("rk_update", asmSize, {bb(1, I1(stagger)), fb(2, target + "_a"), fb(3, "tend"), fb(4, target), bb(5, dttmp)});
       haloFill(target);
       return;
     }
@@ -743,8 +748,9 @@ void Encoder::stages() {
     advect(nrk, "v", 3, "vten", true, "");
     advect(nrk, "w", 4, "wten", true, "");
     if (tke) {
-      if (cfg.fuse != 0) {  // zero, assembly, add and advance in one launch
+      if (cfg.fuse != 0) {  // zero, assembly and add in one launch
         advect(nrk, "tke", 4, "tketen_s", true, "");
+        op("tke_advance", full, {fb(1, "tke_a"), fb(2, "tketen_s"), bb(3, dttmp), fb(4, "tke")});
       } else {
         op("turb_zero", whole, {fb(1, "tketen_s")});
         advect(nrk, "tke", 4, "tketen_s", true, "");

@@ -150,6 +150,14 @@ static int validate_ops(te_ctx* c) {
           return fail(c, TE_ERR_ARG, "copy_fields op " + std::to_string(i) + " leaves the memory");
     for (int q = 0; q < 48; ++q)
       if (op.A[4 + q] >= c->words) return fail(c, TE_ERR_ARG, "op " + std::to_string(i) + " (" + nm + ") parameter offset outside memory");
+    if (std::strcmp(nm, "adv_fused") == 0) {
+      // the fields a fused advection writes (tendency, chained update) must
+      // not be ones its workgroups read at neighbouring points
+      for (int w : {11, 40})
+        for (int r : {2, 4, 13, 14})
+          if (op.A[4 + w] != 0 && op.A[4 + w] == op.A[4 + r])
+            return fail(c, TE_ERR_ARG, "adv_fused op " + std::to_string(i) + " writes a field its stencil reads");
+    }
   }
   return TE_OK;
 }
