@@ -9,16 +9,59 @@ namespace te {
 // false where the reference kernel writes nothing, else the face value the
 // kernel stores.  The flux kernels store it; the fused advection kernel
 // (k_fused.hip) evaluates the two faces of a cell directly.
+//
+// Each function first loads its whole stencil window unconditionally and
+// then picks the upwind operands with selects: the arithmetic is the
+// reference's, on the same values, but no load waits behind a lane-divergent
+// branch, so a wave issues all its loads together.  Vertical windows clamp the
+// level into [0, nz] (a clamped value is never one the result uses), which
+// keeps every load inside the field.
+
+// s at level k + n of the column whose level-k cell is c, level clamped
+TE_DEV float zwin(const float* s, int64_t c, int64_t sk, int k, int n, int nz) {
+  int kk = k + n;
+  kk = kk < 0 ? 0 : (kk > nz ? nz : kk);
+  return s[c + (int64_t)(kk - k) * sk];
+}
+// the window w[0..6] = s at offsets -3..3 along stride st
+struct Win7 {
+  float v[7];
+  TE_DEV float operator()(int n) const { return v[n + 3]; }
+};
+TE_DEV Win7 hwin(const float* s, int64_t c, int64_t st) {
+  Win7 w;
+#pragma unroll
+  for (int n = -3; n <= 3; ++n) w.v[n + 3] = s[c + n * st];
+  return w;
+}
+TE_DEV Win7 vwin(const float* s, int64_t c, int64_t sk, int k, int nz) {
+  Win7 w;
+#pragma unroll
+  for (int n = -3; n <= 3; ++n) w.v[n + 3] = zwin(s, c, sk, k, n, nz);
+  return w;
+}
+TE_DEV float sel(bool up, float a, float b) { return up ? a : b; }
+// upwind-biased 5-point operands of the face between offsets SH-1 and SH:
+// (-3, -2, -1, 0, 1) + SH when up, else (2, 1, 0, -1, -2) + SH
+struct Up5 {
+  float s1, s2, s3, s4, s5;
+};
+template <int SH>
+TE_DEV Up5 up5(const Win7& W, bool up) {
+  return Up5{sel(up, W(-3 + SH), W(2 + SH)), sel(up, W(-2 + SH), W(1 + SH)), sel(up, W(-1 + SH), W(SH)),
+             sel(up, W(SH), W(-1 + SH)), sel(up, W(1 + SH), W(-2 + SH))};
+}
+TE_DEV float flx5(const Up5& o) { return flx5(o.s1, o.s2, o.s3, o.s4, o.s5); }
+TE_DEV float weno5(const Up5& o, float weps) { return weno5(o.s1, o.s2, o.s3, o.s4, o.s5, weps); }
 
 // adv_flux_x.wgsl:120-143 / adv_flux_y.wgsl:120-144
 template <int DIR>
 TE_DEV bool fv_h5(const Dims& g, const float* s, const float* rr, int i, int j, int k, float& out) {
   if (DIR == 0 ? (i > g.nx || j >= g.ny || k >= g.nz) : (i >= g.nx || j > g.ny || k >= g.nz)) return false;
   const int64_t st = DIR == 0 ? 1 : (int64_t)g.sJ, c = g.idx(i, j, k);
-  float u = rr[c], f;
-  if (u >= 0.0f) f = flx5(s[c - 3 * st], s[c - 2 * st], s[c - st], s[c], s[c + st]);
-  else f = flx5(s[c + 2 * st], s[c + st], s[c], s[c - st], s[c - 2 * st]);
-  out = u * f;
+  const float u = rr[c];
+  const Win7 W = hwin(s, c, st);
+  out = u * flx5(up5<0>(W, u >= 0.0f));
   return true;
 }
 template <int DIR>
@@ -36,13 +79,15 @@ TE_DEV bool fv_z5(const Dims& g, const float* s, const float* rrw, const float* 
   if (i >= g.nx || j >= g.ny || k > g.nz) return false;
   const int64_t sk = g.sk(), c = g.idx(i, j, k);
   if (k == 0 || k == g.nz) { out = 0.0f; return true; }
-  float w = rrw[c], f;
-  bool up = w >= 0.0f;
-  if (k == 1) f = up ? fma_(c1[k], s[c - sk], c2[k] * s[c]) : flx3(s[c + sk], s[c], s[c - sk]);
-  else if (k == 2) f = up ? flx3(s[c - 2 * sk], s[c - sk], s[c]) : flx5(s[c + 2 * sk], s[c + sk], s[c], s[c - sk], s[c - 2 * sk]);
-  else if (k == g.nz - 2) f = up ? flx5(s[c - 3 * sk], s[c - 2 * sk], s[c - sk], s[c], s[c + sk]) : flx3(s[c + sk], s[c], s[c - sk]);
-  else if (k == g.nz - 1) f = up ? flx3(s[c - 2 * sk], s[c - sk], s[c]) : fma_(c1[k], s[c - sk], c2[k] * s[c]);
-  else f = up ? flx5(s[c - 3 * sk], s[c - 2 * sk], s[c - sk], s[c], s[c + sk]) : flx5(s[c + 2 * sk], s[c + sk], s[c], s[c - sk], s[c - 2 * sk]);
+  const float w = rrw[c], cc1 = c1[k], cc2 = c2[k];
+  const Win7 W = vwin(s, c, sk, k, g.nz);
+  const bool up = w >= 0.0f;
+  float f;
+  if (k == 1) f = up ? fma_(cc1, W(-1), cc2 * W(0)) : flx3(W(1), W(0), W(-1));
+  else if (k == 2) f = up ? flx3(W(-2), W(-1), W(0)) : flx5(W(2), W(1), W(0), W(-1), W(-2));
+  else if (k == g.nz - 2) f = up ? flx5(W(-3), W(-2), W(-1), W(0), W(1)) : flx3(W(1), W(0), W(-1));
+  else if (k == g.nz - 1) f = up ? flx3(W(-2), W(-1), W(0)) : fma_(cc1, W(-1), cc2 * W(0));
+  else f = flx5(up5<0>(W, up));
   out = w * f;
   return true;
 }
@@ -60,10 +105,10 @@ TE_DEV bool fv_h_weno(const Dims& g, const float* s, const float* rr, float weps
                       float& out) {
   if (i > g.nx || j > g.ny || k >= g.nz) return false;
   const int64_t st = DIR == 0 ? 1 : (int64_t)g.sJ, c = g.idx(i, j, k);
-  float s1, s2, s3, s4, s5;
-  if (rr[c] >= 0.0f) { s1 = s[c - 3 * st]; s2 = s[c - 2 * st]; s3 = s[c - st]; s4 = s[c]; s5 = s[c + st]; }
-  else { s1 = s[c + 2 * st]; s2 = s[c + st]; s3 = s[c]; s4 = s[c - st]; s5 = s[c - 2 * st]; }
-  out = weno_pdef_ok(s1, s2, s3, s4, s5, weps, pdef) ? rr[c] * weno5(s1, s2, s3, s4, s5, weps) : 0.0f;
+  const float u = rr[c];
+  const Win7 W = hwin(s, c, st);
+  const Up5 o = up5<0>(W, u >= 0.0f);
+  out = weno_pdef_ok(o.s1, o.s2, o.s3, o.s4, o.s5, weps, pdef) ? u * weno5(o, weps) : 0.0f;
   return true;
 }
 template <int DIR>
@@ -83,8 +128,9 @@ TE_DEV bool fv_z_weno(const Dims& g, const float* s, const float* rrw, const flo
   if (i >= g.nx || j >= g.ny || k < 0 || kf > g.nz + 1) return false;
   const int64_t sk = g.sk(), c = g.idx(i, j, k);
   if (kf == 1 || kf == g.nz + 1) { out = 0.0f; return true; }
-#define L(n) s[c + (n) * sk]
-  float w = rrw[c];
+  const Win7 W = vwin(s, c, sk, k, g.nz);
+#define L(n) W(n)
+  const float w = rrw[c];
   const int nk = g.nz;
   if (kf == 2) {
     out = (w >= 0.0f) ? w * fma_(c1[k], L(-1), c2[k] * L(0)) : w * upstrpd(L(1), L(0), L(-1), weps);
@@ -94,9 +140,8 @@ TE_DEV bool fv_z_weno(const Dims& g, const float* s, const float* rrw, const flo
     out = (w > 0.0f) ? w * upstrpd(L(-2), L(-1), L(0), weps) : w * fma_(c1[k], L(-1), c2[k] * L(0));
     return true;
   }
-  float s1, s2, s3, s4, s5;
-  if (w >= 0.0f) { s1 = L(-3); s2 = L(-2); s3 = L(-1); s4 = L(0); s5 = L(1); }
-  else { s1 = L(2); s2 = L(1); s3 = L(0); s4 = L(-1); s5 = L(-2); }
+  const Up5 o = up5<0>(W, w >= 0.0f);
+  const float s1 = o.s1, s2 = o.s2, s3 = o.s3, s4 = o.s4, s5 = o.s5;
   if (pdefweno == 1) {
     bool doit = weno_pdef_ok(s1, s2, s3, s4, s5, weps, true);
     if (kf == 3 && w > 0.0f) doit = false;
@@ -126,18 +171,12 @@ __global__ __launch_bounds__(256) void k_adv_flux_z_weno(uint32_t* M, KArg a) {
 }
 
 TE_DEV float flx5_upwind(const float* s, int64_t c, int64_t st, bool up, bool shifted) {
-  if (shifted)
-    return up ? flx5(s[c - 2 * st], s[c - st], s[c], s[c + st], s[c + 2 * st])
-              : flx5(s[c + 3 * st], s[c + 2 * st], s[c + st], s[c], s[c - st]);
-  return up ? flx5(s[c - 3 * st], s[c - 2 * st], s[c - st], s[c], s[c + st])
-            : flx5(s[c + 2 * st], s[c + st], s[c], s[c - st], s[c - 2 * st]);
+  const Win7 W = hwin(s, c, st);
+  return shifted ? flx5(up5<1>(W, up)) : flx5(up5<0>(W, up));
 }
 TE_DEV float weno5_upwind(const float* s, int64_t c, int64_t st, bool up, bool shifted, float weps) {
-  if (shifted)
-    return up ? weno5(s[c - 2 * st], s[c - st], s[c], s[c + st], s[c + 2 * st], weps)
-              : weno5(s[c + 3 * st], s[c + 2 * st], s[c + st], s[c], s[c - st], weps);
-  return up ? weno5(s[c - 3 * st], s[c - 2 * st], s[c - st], s[c], s[c + st], weps)
-            : weno5(s[c + 2 * st], s[c + st], s[c], s[c - st], s[c - 2 * st], weps);
+  const Win7 W = hwin(s, c, st);
+  return shifted ? weno5(up5<1>(W, up), weps) : weno5(up5<0>(W, up), weps);
 }
 
 // adv_flux_x_stag.wgsl:144-186 / adv_flux_y_stag.wgsl (rr = rru for x, rrv for y)
@@ -174,8 +213,10 @@ TE_DEV bool fv_z_stag(const Dims& g, int stag, const float* s, const float* rrw,
   if (stag == 2) m = 0.5f * (rrw[c] + rrw[c - 1]);
   else if (stag == 3) m = 0.5f * (rrw[c] + rrw[c - sj]);
   else m = 0.5f * (rrw[c] + rrw[c + sk]);
-  bool up = m >= 0.0f;
-#define A_(n) s[c + (n) * sk]
+  const bool up = m >= 0.0f;
+  const float cc1 = c1[k], cc2 = c2[k];
+  const Win7 W = vwin(s, c, sk, k, g.nz);
+#define A_(n) W(n)
   float f;
   if (stag == 4) {
     if (k > g.nz - 1) { out = 0.0f; return true; }
@@ -186,10 +227,10 @@ TE_DEV bool fv_z_stag(const Dims& g, int stag, const float* s, const float* rrw,
     else f = up ? flx5(A_(-2), A_(-1), A_(0), A_(1), A_(2)) : flx5(A_(3), A_(2), A_(1), A_(0), A_(-1));
   } else {
     if (k == 0 || k == g.nz) { out = 0.0f; return true; }
-    if (k == 1) f = up ? fma_(c1[k], A_(-1), c2[k] * A_(0)) : flx3(A_(1), A_(0), A_(-1));
+    if (k == 1) f = up ? fma_(cc1, A_(-1), cc2 * A_(0)) : flx3(A_(1), A_(0), A_(-1));
     else if (k == 2) f = up ? flx3(A_(-2), A_(-1), A_(0)) : flx5(A_(2), A_(1), A_(0), A_(-1), A_(-2));
     else if (k == g.nz - 2) f = up ? flx5(A_(-3), A_(-2), A_(-1), A_(0), A_(1)) : flx3(A_(1), A_(0), A_(-1));
-    else if (k == g.nz - 1) f = up ? flx3(A_(-2), A_(-1), A_(0)) : fma_(c1[k], A_(-1), c2[k] * A_(0));
+    else if (k == g.nz - 1) f = up ? flx3(A_(-2), A_(-1), A_(0)) : fma_(cc1, A_(-1), cc2 * A_(0));
     else f = up ? flx5(A_(-3), A_(-2), A_(-1), A_(0), A_(1)) : flx5(A_(2), A_(1), A_(0), A_(-1), A_(-2));
   }
 #undef A_
@@ -213,8 +254,8 @@ TE_DEV bool fv_h_weno_w(const Dims& g, const float* s, const float* rr, const fl
   if (k == 0) { out = 0.0f; return true; }
   float cc2 = c2[k], cc1 = 1.0f - cc2;
   float ub = fma_(cc2, rr[c], cc1 * rr[c - sk]);
-  if (ub >= 0.0f) out = ub * weno5(s[c - 3 * st], s[c - 2 * st], s[c - st], s[c], s[c + st], weps);
-  else out = ub * weno5(s[c + 2 * st], s[c + st], s[c], s[c - st], s[c - 2 * st], weps);
+  const Win7 W = hwin(s, c, st);
+  out = ub * weno5(up5<0>(W, ub >= 0.0f), weps);
   return true;
 }
 template <int DIR>
@@ -232,7 +273,8 @@ TE_DEV bool fv_z_weno_w(const Dims& g, const float* s, const float* rrw, float w
   if (i > g.nx || j > g.ny || k > g.nz) return false;
   const int64_t sk = g.sk(), c = g.idx(i, j, k);
   if (k > g.nz - 1) { out = 0.0f; return true; }
-#define L(n) s[c + (n) * sk]
+  const Win7 W = vwin(s, c, sk, k, g.nz);
+#define L(n) W(n)
   float wbar = 0.5f * (rrw[c] + rrw[c + sk]);
   const int nz = g.nz;
   float f;
@@ -385,12 +427,13 @@ __global__ __launch_bounds__(256) void k_adv_assemble(uint32_t* M, KArg a) {
 }
 
 // The three flux dispatches of one advected field and adv_assemble in one
-// launch: each cell evaluates face fluxes with the flux kernels' own value
-// functions instead of reading them back from fx/fy/fz (which are then left
-// unwritten; they are dead scratch).  Every thread publishes its low x and y
-// faces in LDS; the high face is the neighbour's low face, evaluated by the
-// thread itself only at the block's edge.  Both z faces are evaluated per
-// thread.  A face is the same value whichever thread evaluates it.
+// launch: each cell evaluates its six face fluxes with the flux kernels' own
+// value functions instead of reading them back from fx/fy/fz (which are then
+// left unwritten; they are dead scratch).  A face is the same value whichever
+// cell evaluates it; the two faces of a direction share one stencil window,
+// so evaluating both costs arithmetic, not loads (measured faster than
+// publishing one face per cell through LDS, which needs a barrier between
+// the loads of the two halves).
 // p: adv_assemble's parameters, with 3 -> flux kind (0 flx5, 1 WENO5,
 // 2 staggered flx5, 3 WENO5 of w-staggered fields), 4 rrw, 5 -> weps,
 // 20 -> pdef, 21 -> pdefweno (22-24: fx, fy, fz of the unfused dispatches).
@@ -424,31 +467,18 @@ struct FusedFlux {
 
 template <int KIND, int STC>
 __device__ __forceinline__ void adv_fused_body(uint32_t* M, const KArg& a) {
-  __shared__ float lx[256], ly[256];
-  const int tx = threadIdx.x, ty = threadIdx.y, tz = threadIdx.z;
   const Blk b_ = te_block();
-  const int i = (int)(b_.x * blockDim.x) + tx, j = (int)(b_.y * blockDim.y) + ty, k = (int)(b_.z * blockDim.z) + tz;
-  const bool inGrid = i < (int)a.n[0] && j < (int)a.n[1] && k < (int)a.n[2];
-  const int t = tx + (int)blockDim.x * (ty + (int)blockDim.y * tz);
+  const int i = (int)(b_.x * blockDim.x + threadIdx.x), j = (int)(b_.y * blockDim.y + threadIdx.y),
+            k = (int)(b_.z * blockDim.z + threadIdx.z);
+  if (i >= (int)a.n[0] || j >= (int)a.n[1] || k >= (int)a.n[2]) return;
   Dims g = dims(M, a.p[0]);
   const int stag = STC != 0 ? STC : ldi(M, a.p[1]), st = stag < 0 ? -stag : stag;
   const FusedFlux<KIND> F{g, stag, ldi(M, a.p[20]), ldi(M, a.p[21]), fp(M, a.p[2]), fp(M, a.p[13]), fp(M, a.p[14]),
                           fp(M, a.p[4]), fp(M, a.p[15]), fp(M, a.p[16]), ldf(M, a.p[5])};
-  // low faces of this cell's x and y divergence, published for the neighbours
+  // faces of this cell's divergence: low and high in x, y and z
   const int xlo = st == 2 ? i - 1 : i, ylo = st == 3 ? j - 1 : j;
-  float fxl = 0.0f, fyl = 0.0f;
-  if (inGrid) {
-    fxl = F.template at<0>(xlo, j, k);
-    fyl = F.template at<1>(i, ylo, k);
-  }
-  lx[t] = fxl;
-  ly[t] = fyl;
-  __syncthreads();
-  if (!inGrid) return;
-  const bool xNb = tx + 1 < (int)blockDim.x && i + 1 < (int)a.n[0];
-  const bool yNb = ty + 1 < (int)blockDim.y && j + 1 < (int)a.n[1];
-  const float fxh = xNb ? lx[t + 1] : F.template at<0>(xlo + 1, j, k);
-  const float fyh = yNb ? ly[t + (int)blockDim.x] : F.template at<1>(i, ylo + 1, k);
+  const float fxl = F.template at<0>(xlo, j, k), fxh = F.template at<0>(xlo + 1, j, k);
+  const float fyl = F.template at<1>(i, ylo, k), fyh = F.template at<1>(i, ylo + 1, k);
   const int zlo = st == 4 ? k - 1 : k;
   const float fzl = F.template at<2>(i, j, zlo), fzh = F.template at<2>(i, j, zlo + 1);
   // p[32] -> chain flags: the pointwise dispatches around a field's assembly
