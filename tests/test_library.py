@@ -4,6 +4,8 @@ import ctypes
 import os
 import re
 
+import numpy as np
+
 import te_host
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -44,3 +46,29 @@ def test_program_create_needs_a_gpu_or_rejects_unknown_kernels():
     import pytest
     with pytest.raises(te_host.TeError, match="no kernel"):
         te_host.Replayer(1024, ["not_a_kernel"], np.zeros((1, 64), np.uint32))
+
+
+def test_workgroup_renumbering_division_is_exact():
+    """te_block() (csrc/te_device.hpp) divides workgroup numbers l < 2^24 by
+    grid extents d <= 2^16 as (l * ceil(2^40 / d)) >> 40.  Checked by brute
+    force against integer division on every d up to 1024 and on the largest
+    extents, at l = 0, the ends of the range and every multiple of d +- 1
+    near them."""
+    def magic(d):
+        return -(-(1 << 40) // d)
+
+    ds = list(range(1, 1025)) + [4095, 4096, 4097, 40000, 65535, 65536]
+    for d in ds:
+        m = magic(d)
+        assert m < 1 << 41
+        ls = {0, 1, (1 << 24) - 1, (1 << 24) - 2}
+        for base in (d, 2 * d, ((1 << 24) - 1) // d * d, ((1 << 24) - 1) // d * d - d):
+            ls.update({base - 1, base, base + 1})
+        for l in ls:
+            if 0 <= l < 1 << 24:
+                assert (l * m) >> 40 == l // d, (d, l)
+    # exhaustive over l for a few extents the bench grids use
+    for d in (3, 4, 48, 50, 192):
+        m = magic(d)
+        l = np.arange(1 << 24, dtype=np.uint64)
+        assert np.array_equal((l * np.uint64(m)) >> np.uint64(40), l // np.uint64(d))

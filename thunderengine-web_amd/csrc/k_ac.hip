@@ -99,8 +99,8 @@ __global__ __launch_bounds__(256) void k_acoustic_fpk(uint32_t* M, KArg a) {
 #define WS_CHUNK 8
 template <bool LDS>
 __device__ void w_solve(uint32_t* M, const KArg& a) {
-  const Blk b_ = te_block();
-  const int i = (int)(b_.x * blockDim.x + threadIdx.x), j = (int)(b_.y * blockDim.y + threadIdx.y);
+  const Blk b_ = te_block(a);
+  const int i = b_.x * TE_BX + (int)threadIdx.x, j = b_.y * TE_BY + (int)threadIdx.y;
   if (i >= (int)a.n[0] || j >= (int)a.n[1]) return;
   Dims g = dims(M, a.p[0]);
   if (i >= g.nx || j >= g.ny) return;

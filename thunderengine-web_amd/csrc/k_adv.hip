@@ -13,16 +13,9 @@ namespace te {
 // Each function first loads its whole stencil window unconditionally and
 // then picks the upwind operands with selects: the arithmetic is the
 // reference's, on the same values, but no load waits behind a lane-divergent
-// branch, so a wave issues all its loads together.  Vertical windows clamp the
-// level into [0, nz] (a clamped value is never one the result uses), which
-// keeps every load inside the field.
-
-// s at level k + n of the column whose level-k cell is c, level clamped
-TE_DEV float zwin(const float* s, int64_t c, int64_t sk, int k, int n, int nz) {
-  int kk = k + n;
-  kk = kk < 0 ? 0 : (kk > nz ? nz : kk);
-  return s[c + (int64_t)(kk - k) * sk];
-}
+// branch, so a wave issues all its loads together.  A window reaches at most
+// 3 levels (or 3 halo-padded rows) past the field; the words there belong to
+// a neighbouring field or the runtime's guard zones, and no result uses them.
 // the window w[0..6] = s at offsets -3..3 along stride st
 struct Win7 {
   float v[7];
@@ -34,12 +27,7 @@ TE_DEV Win7 hwin(const float* s, int64_t c, int64_t st) {
   for (int n = -3; n <= 3; ++n) w.v[n + 3] = s[c + n * st];
   return w;
 }
-TE_DEV Win7 vwin(const float* s, int64_t c, int64_t sk, int k, int nz) {
-  Win7 w;
-#pragma unroll
-  for (int n = -3; n <= 3; ++n) w.v[n + 3] = zwin(s, c, sk, k, n, nz);
-  return w;
-}
+TE_DEV Win7 vwin(const float* s, int64_t c, int64_t sk) { return hwin(s, c, sk); }
 TE_DEV float sel(bool up, float a, float b) { return up ? a : b; }
 // upwind-biased 5-point operands of the face between offsets SH-1 and SH:
 // (-3, -2, -1, 0, 1) + SH when up, else (2, 1, 0, -1, -2) + SH
@@ -56,9 +44,9 @@ TE_DEV float weno5(const Up5& o, float weps) { return weno5(o.s1, o.s2, o.s3, o.
 
 // adv_flux_x.wgsl:120-143 / adv_flux_y.wgsl:120-144
 template <int DIR>
-TE_DEV bool fv_h5(const Dims& g, const float* s, const float* rr, int i, int j, int k, float& out) {
+TE_DEV bool fv_h5(const Dims& g, const float* s, const float* rr, int i, int j, int k, int64_t c, float& out) {
   if (DIR == 0 ? (i > g.nx || j >= g.ny || k >= g.nz) : (i >= g.nx || j > g.ny || k >= g.nz)) return false;
-  const int64_t st = DIR == 0 ? 1 : (int64_t)g.sJ, c = g.idx(i, j, k);
+  const int64_t st = DIR == 0 ? 1 : (int64_t)g.sJ;
   const float u = rr[c];
   const Win7 W = hwin(s, c, st);
   out = u * flx5(up5<0>(W, u >= 0.0f));
@@ -68,19 +56,19 @@ template <int DIR>
 __device__ void flux_h5(uint32_t* M, const KArg& a, int i, int j, int k) {
   Dims g = dims(M, a.p[0]);
   float f;
-  if (fv_h5<DIR>(g, fp(M, a.p[1]), fp(M, a.p[2]), i, j, k, f)) fp(M, a.p[3])[g.idx(i, j, k)] = f;
+  if (fv_h5<DIR>(g, fp(M, a.p[1]), fp(M, a.p[2]), i, j, k, g.idx(i, j, k), f)) fp(M, a.p[3])[g.idx(i, j, k)] = f;
 }
 __global__ __launch_bounds__(256) void k_adv_flux_x(uint32_t* M, KArg a) { TE_TID3 flux_h5<0>(M, a, gx, gy, gz); }
 __global__ __launch_bounds__(256) void k_adv_flux_y(uint32_t* M, KArg a) { TE_TID3 flux_h5<1>(M, a, gx, gy, gz); }
 
 // adv_flux_z.wgsl:127-200
-TE_DEV bool fv_z5(const Dims& g, const float* s, const float* rrw, const float* c1, const float* c2, int i, int j, int k,
+TE_DEV bool fv_z5(const Dims& g, const float* s, const float* rrw, const float* c1, const float* c2, int i, int j, int k, int64_t c,
                   float& out) {
   if (i >= g.nx || j >= g.ny || k > g.nz) return false;
-  const int64_t sk = g.sk(), c = g.idx(i, j, k);
+  const int64_t sk = g.sk();
   if (k == 0 || k == g.nz) { out = 0.0f; return true; }
   const float w = rrw[c], cc1 = c1[k], cc2 = c2[k];
-  const Win7 W = vwin(s, c, sk, k, g.nz);
+  const Win7 W = vwin(s, c, sk);
   const bool up = w >= 0.0f;
   float f;
   if (k == 1) f = up ? fma_(cc1, W(-1), cc2 * W(0)) : flx3(W(1), W(0), W(-1));
@@ -95,16 +83,16 @@ __global__ __launch_bounds__(256) void k_adv_flux_z(uint32_t* M, KArg a) {
   TE_TID3
   Dims g = dims(M, a.p[0]);
   float f;
-  if (fv_z5(g, fp(M, a.p[1]), fp(M, a.p[2]), fp(M, a.p[3]), fp(M, a.p[4]), gx, gy, gz, f))
+  if (fv_z5(g, fp(M, a.p[1]), fp(M, a.p[2]), fp(M, a.p[3]), fp(M, a.p[4]), gx, gy, gz, g.idx(gx, gy, gz), f))
     fp(M, a.p[5])[g.idx(gx, gy, gz)] = f;
 }
 
 // adv_flux_x_weno.wgsl:158-200 / adv_flux_y_weno.wgsl:158-201
 template <int DIR>
-TE_DEV bool fv_h_weno(const Dims& g, const float* s, const float* rr, float weps, bool pdef, int i, int j, int k,
+TE_DEV bool fv_h_weno(const Dims& g, const float* s, const float* rr, float weps, bool pdef, int i, int j, int k, int64_t c,
                       float& out) {
   if (i > g.nx || j > g.ny || k >= g.nz) return false;
-  const int64_t st = DIR == 0 ? 1 : (int64_t)g.sJ, c = g.idx(i, j, k);
+  const int64_t st = DIR == 0 ? 1 : (int64_t)g.sJ;
   const float u = rr[c];
   const Win7 W = hwin(s, c, st);
   const Up5 o = up5<0>(W, u >= 0.0f);
@@ -115,7 +103,7 @@ template <int DIR>
 __device__ void flux_h_weno(uint32_t* M, const KArg& a, int i, int j, int k) {
   Dims g = dims(M, a.p[0]);
   float f;
-  if (fv_h_weno<DIR>(g, fp(M, a.p[1]), fp(M, a.p[2]), ldf(M, a.p[3]), ldi(M, a.p[4]) == 1, i, j, k, f))
+  if (fv_h_weno<DIR>(g, fp(M, a.p[1]), fp(M, a.p[2]), ldf(M, a.p[3]), ldi(M, a.p[4]) == 1, i, j, k, g.idx(i, j, k), f))
     fp(M, a.p[5])[g.idx(i, j, k)] = f;
 }
 __global__ __launch_bounds__(256) void k_adv_flux_x_weno(uint32_t* M, KArg a) { TE_TID3 flux_h_weno<0>(M, a, gx, gy, gz); }
@@ -123,12 +111,12 @@ __global__ __launch_bounds__(256) void k_adv_flux_y_weno(uint32_t* M, KArg a) { 
 
 // adv_flux_z_weno.wgsl:174-279 (face level k = kf - 1)
 TE_DEV bool fv_z_weno(const Dims& g, const float* s, const float* rrw, const float* c1, const float* c2, float weps,
-                      int pdefweno, int i, int j, int k, float& out) {
+                      int pdefweno, int i, int j, int k, int64_t c, float& out) {
   const int kf = k + 1;
   if (i >= g.nx || j >= g.ny || k < 0 || kf > g.nz + 1) return false;
-  const int64_t sk = g.sk(), c = g.idx(i, j, k);
+  const int64_t sk = g.sk();
   if (kf == 1 || kf == g.nz + 1) { out = 0.0f; return true; }
-  const Win7 W = vwin(s, c, sk, k, g.nz);
+  const Win7 W = vwin(s, c, sk);
 #define L(n) W(n)
   const float w = rrw[c];
   const int nk = g.nz;
@@ -166,7 +154,7 @@ __global__ __launch_bounds__(256) void k_adv_flux_z_weno(uint32_t* M, KArg a) {
   Dims g = dims(M, a.p[0]);
   float f;
   if (fv_z_weno(g, fp(M, a.p[1]), fp(M, a.p[2]), fp(M, a.p[3]), fp(M, a.p[4]), ldf(M, a.p[5]), ldi(M, a.p[7]), gx, gy,
-                gz, f))
+                gz, g.idx(gx, gy, gz), f))
     fp(M, a.p[8])[g.idx(gx, gy, gz)] = f;
 }
 
@@ -181,10 +169,10 @@ TE_DEV float weno5_upwind(const float* s, int64_t c, int64_t st, bool up, bool s
 
 // adv_flux_x_stag.wgsl:144-186 / adv_flux_y_stag.wgsl (rr = rru for x, rrv for y)
 template <int DIR>
-TE_DEV bool fv_h_stag(const Dims& g, int stag, const float* s, const float* rr, const float* c2, int i, int j, int k,
+TE_DEV bool fv_h_stag(const Dims& g, int stag, const float* s, const float* rr, const float* c2, int i, int j, int k, int64_t c,
                       float& out) {
   if (i > g.nx || j > g.ny || k >= g.nz) return false;
-  const int64_t sj = g.sJ, sk = g.sk(), st = DIR == 0 ? 1 : sj, c = g.idx(i, j, k);
+  const int64_t sj = g.sJ, sk = g.sk(), st = DIR == 0 ? 1 : sj;
   if (stag == 4 && k == 0) { out = 0.0f; return true; }
   float m;
   if (stag == 2) m = 0.5f * (rr[c] + rr[c + (DIR == 0 ? 1 : -1)]);
@@ -198,7 +186,7 @@ template <int DIR>
 __device__ void flux_h_stag(uint32_t* M, const KArg& a, int i, int j, int k) {
   Dims g = dims(M, a.p[0]);
   float f;
-  if (fv_h_stag<DIR>(g, ldi(M, a.p[1]), fp(M, a.p[2]), fp(M, a.p[DIR == 0 ? 3 : 4]), fp(M, a.p[7]), i, j, k, f))
+  if (fv_h_stag<DIR>(g, ldi(M, a.p[1]), fp(M, a.p[2]), fp(M, a.p[DIR == 0 ? 3 : 4]), fp(M, a.p[7]), i, j, k, g.idx(i, j, k), f))
     fp(M, a.p[8])[g.idx(i, j, k)] = f;
 }
 __global__ __launch_bounds__(256) void k_adv_flux_x_stag(uint32_t* M, KArg a) { TE_TID3 flux_h_stag<0>(M, a, gx - 1, gy - 1, gz); }
@@ -206,16 +194,16 @@ __global__ __launch_bounds__(256) void k_adv_flux_y_stag(uint32_t* M, KArg a) { 
 
 // adv_flux_z_stag.wgsl:127-268
 TE_DEV bool fv_z_stag(const Dims& g, int stag, const float* s, const float* rrw, const float* c1, const float* c2, int i,
-                      int j, int k, float& out) {
+                      int j, int k, int64_t c, float& out) {
   if (i > g.nx || j > g.ny || k > g.nz) return false;
-  const int64_t sj = g.sJ, sk = g.sk(), c = g.idx(i, j, k);
+  const int64_t sj = g.sJ, sk = g.sk();
   float m;
   if (stag == 2) m = 0.5f * (rrw[c] + rrw[c - 1]);
   else if (stag == 3) m = 0.5f * (rrw[c] + rrw[c - sj]);
   else m = 0.5f * (rrw[c] + rrw[c + sk]);
   const bool up = m >= 0.0f;
   const float cc1 = c1[k], cc2 = c2[k];
-  const Win7 W = vwin(s, c, sk, k, g.nz);
+  const Win7 W = vwin(s, c, sk);
 #define A_(n) W(n)
   float f;
   if (stag == 4) {
@@ -241,16 +229,16 @@ __global__ __launch_bounds__(256) void k_adv_flux_z_stag(uint32_t* M, KArg a) {
   TE_TID3
   Dims g = dims(M, a.p[0]);
   float f;
-  if (fv_z_stag(g, ldi(M, a.p[1]), fp(M, a.p[2]), fp(M, a.p[5]), fp(M, a.p[6]), fp(M, a.p[7]), gx, gy, gz, f))
+  if (fv_z_stag(g, ldi(M, a.p[1]), fp(M, a.p[2]), fp(M, a.p[5]), fp(M, a.p[6]), fp(M, a.p[7]), gx, gy, gz, g.idx(gx, gy, gz), f))
     fp(M, a.p[8])[g.idx(gx, gy, gz)] = f;
 }
 
 // adv_flux_x_weno_w.wgsl:143-175 / adv_flux_y_weno_w.wgsl
 template <int DIR>
-TE_DEV bool fv_h_weno_w(const Dims& g, const float* s, const float* rr, const float* c2, float weps, int i, int j, int k,
+TE_DEV bool fv_h_weno_w(const Dims& g, const float* s, const float* rr, const float* c2, float weps, int i, int j, int k, int64_t c,
                         float& out) {
   if (i < 0 || j < 0 || i > g.nx || j > g.ny || k >= g.nz) return false;
-  const int64_t sk = g.sk(), st = DIR == 0 ? 1 : (int64_t)g.sJ, c = g.idx(i, j, k);
+  const int64_t sk = g.sk(), st = DIR == 0 ? 1 : (int64_t)g.sJ;
   if (k == 0) { out = 0.0f; return true; }
   float cc2 = c2[k], cc1 = 1.0f - cc2;
   float ub = fma_(cc2, rr[c], cc1 * rr[c - sk]);
@@ -262,18 +250,18 @@ template <int DIR>
 __device__ void flux_h_weno_w(uint32_t* M, const KArg& a, int i, int j, int k) {
   Dims g = dims(M, a.p[0]);
   float f;
-  if (fv_h_weno_w<DIR>(g, fp(M, a.p[1]), fp(M, a.p[2]), fp(M, a.p[3]), ldf(M, a.p[4]), i, j, k, f))
+  if (fv_h_weno_w<DIR>(g, fp(M, a.p[1]), fp(M, a.p[2]), fp(M, a.p[3]), ldf(M, a.p[4]), i, j, k, g.idx(i, j, k), f))
     fp(M, a.p[5])[g.idx(i, j, k)] = f;
 }
 __global__ __launch_bounds__(256) void k_adv_flux_x_weno_w(uint32_t* M, KArg a) { TE_TID3 flux_h_weno_w<0>(M, a, gx - 1, gy - 1, gz); }
 __global__ __launch_bounds__(256) void k_adv_flux_y_weno_w(uint32_t* M, KArg a) { TE_TID3 flux_h_weno_w<1>(M, a, gx - 1, gy - 1, gz); }
 
 // adv_flux_z_weno_w.wgsl:159-231
-TE_DEV bool fv_z_weno_w(const Dims& g, const float* s, const float* rrw, float weps, int i, int j, int k, float& out) {
+TE_DEV bool fv_z_weno_w(const Dims& g, const float* s, const float* rrw, float weps, int i, int j, int k, int64_t c, float& out) {
   if (i > g.nx || j > g.ny || k > g.nz) return false;
-  const int64_t sk = g.sk(), c = g.idx(i, j, k);
+  const int64_t sk = g.sk();
   if (k > g.nz - 1) { out = 0.0f; return true; }
-  const Win7 W = vwin(s, c, sk, k, g.nz);
+  const Win7 W = vwin(s, c, sk);
 #define L(n) W(n)
   float wbar = 0.5f * (rrw[c] + rrw[c + sk]);
   const int nz = g.nz;
@@ -291,7 +279,7 @@ __global__ __launch_bounds__(256) void k_adv_flux_z_weno_w(uint32_t* M, KArg a) 
   TE_TID3
   Dims g = dims(M, a.p[0]);
   float f;
-  if (fv_z_weno_w(g, fp(M, a.p[1]), fp(M, a.p[2]), ldf(M, a.p[3]), gx, gy, gz, f)) fp(M, a.p[4])[g.idx(gx, gy, gz)] = f;
+  if (fv_z_weno_w(g, fp(M, a.p[1]), fp(M, a.p[2]), ldf(M, a.p[3]), gx, gy, gz, g.idx(gx, gy, gz), f)) fp(M, a.p[4])[g.idx(gx, gy, gz)] = f;
 }
 
 // adv_flux_x_weno_uv.wgsl:168-188 / adv_flux_y_weno_uv.wgsl
@@ -339,8 +327,10 @@ __global__ __launch_bounds__(256) void k_adv_flux_z_weno_uv(uint32_t* M, KArg a)
 // adv_assemble.wgsl:139-279: flux divergence + mass-divergence term, divided
 // by the reference density; open-boundary inflow corrections included.
 // STC != 0: the stagger is known at compile time (specialised launches).
+// c: index of (i, j, k).
 template <int STC = 0, class FLUX, class STORE>
-TE_DEV void assemble_at(uint32_t* M, const KArg& a, const Dims& g, int i, int j, int k, FLUX F, STORE store) {
+TE_DEV void assemble_at(uint32_t* M, const KArg& a, const Dims& g, int i, int j, int k, int64_t c, FLUX F,
+                        STORE store) {
   const int stag = STC != 0 ? STC : ldi(M, a.p[1]);
   int st = stag < 0 ? -stag : stag;
   int iMax = st == 2 ? g.nx : g.nx - 1, jMax = st == 3 ? g.ny : g.ny - 1, kLo = st == 4 ? 1 : 0, kMax = g.nz - 1;
@@ -354,7 +344,7 @@ TE_DEV void assemble_at(uint32_t* M, const KArg& a, const Dims& g, int i, int j,
               *hm = fp(M, a.p[30]);
   float* tend = fp(M, a.p[11]);
   int accumulate = ldi(M, a.p[12]);
-  const int64_t sj = g.sJ, sk = g.sk(), c = g.idx(i, j, k);
+  const int64_t sj = g.sJ, sk = g.sk();
   float dFx = st == 2 ? F(0, i, j, k) - F(0, i - 1, j, k) : F(0, i + 1, j, k) - F(0, i, j, k);
   float dFy = st == 3 ? F(1, i, j, k) - F(1, i, j - 1, k) : F(1, i, j + 1, k) - F(1, i, j, k);
   float dFz = st == 4 ? F(2, i, j, k) - F(2, i, j, k - 1) : F(2, i, j, k + 1) - F(2, i, j, k);
@@ -418,7 +408,7 @@ __global__ __launch_bounds__(256) void k_adv_assemble(uint32_t* M, KArg a) {
   const float *fx = fp(M, a.p[3]), *fy = fp(M, a.p[4]), *fz = fp(M, a.p[5]);
   float* tend = fp(M, a.p[11]);
   assemble_at(
-      M, a, g, gx, gy, gz,
+      M, a, g, gx, gy, gz, g.idx(gx, gy, gz),
       [&](int d, int i, int j, int k) {
         const int64_t c = g.idx(i, j, k);
         return d == 0 ? fx[c] : (d == 1 ? fy[c] : fz[c]);
@@ -445,42 +435,43 @@ struct FusedFlux {
   float weps;
   // face flux of direction D at (i, j, k); NaN where no flux dispatch writes
   template <int D>
-  __device__ __forceinline__ float at(int i, int j, int k) const {
+  __device__ __forceinline__ float at(int i, int j, int k, int64_t c) const {
     float f = __builtin_nanf("");
     bool ok;
     if (KIND == 0)
-      ok = D == 0 ? fv_h5<0>(g, s, rru, i, j, k, f) : D == 1 ? fv_h5<1>(g, s, rrv, i, j, k, f)
-                                                             : fv_z5(g, s, rrw, c1, c2, i, j, k, f);
+      ok = D == 0 ? fv_h5<0>(g, s, rru, i, j, k, c, f) : D == 1 ? fv_h5<1>(g, s, rrv, i, j, k, c, f)
+                                                             : fv_z5(g, s, rrw, c1, c2, i, j, k, c, f);
     else if (KIND == 1)
-      ok = D == 0 ? fv_h_weno<0>(g, s, rru, weps, pdef == 1, i, j, k, f)
-                  : D == 1 ? fv_h_weno<1>(g, s, rrv, weps, pdef == 1, i, j, k, f)
-                           : fv_z_weno(g, s, rrw, c1, c2, weps, pdw, i, j, k, f);
+      ok = D == 0 ? fv_h_weno<0>(g, s, rru, weps, pdef == 1, i, j, k, c, f)
+                  : D == 1 ? fv_h_weno<1>(g, s, rrv, weps, pdef == 1, i, j, k, c, f)
+                           : fv_z_weno(g, s, rrw, c1, c2, weps, pdw, i, j, k, c, f);
     else if (KIND == 2)
-      ok = D == 0 ? fv_h_stag<0>(g, stag, s, rru, c2, i, j, k, f)
-                  : D == 1 ? fv_h_stag<1>(g, stag, s, rrv, c2, i, j, k, f) : fv_z_stag(g, stag, s, rrw, c1, c2, i, j, k, f);
+      ok = D == 0 ? fv_h_stag<0>(g, stag, s, rru, c2, i, j, k, c, f)
+                  : D == 1 ? fv_h_stag<1>(g, stag, s, rrv, c2, i, j, k, c, f) : fv_z_stag(g, stag, s, rrw, c1, c2, i, j, k, c, f);
     else
-      ok = D == 0 ? fv_h_weno_w<0>(g, s, rru, c2, weps, i, j, k, f)
-                  : D == 1 ? fv_h_weno_w<1>(g, s, rrv, c2, weps, i, j, k, f) : fv_z_weno_w(g, s, rrw, weps, i, j, k, f);
+      ok = D == 0 ? fv_h_weno_w<0>(g, s, rru, c2, weps, i, j, k, c, f)
+                  : D == 1 ? fv_h_weno_w<1>(g, s, rrv, c2, weps, i, j, k, c, f) : fv_z_weno_w(g, s, rrw, weps, i, j, k, c, f);
     return ok ? f : __builtin_nanf("");  // a face no flux dispatch writes: poisons the result visibly
   }
 };
 
 template <int KIND, int STC>
 __device__ __forceinline__ void adv_fused_body(uint32_t* M, const KArg& a) {
-  const Blk b_ = te_block();
-  const int i = (int)(b_.x * blockDim.x + threadIdx.x), j = (int)(b_.y * blockDim.y + threadIdx.y),
-            k = (int)(b_.z * blockDim.z + threadIdx.z);
+  const Blk b_ = te_block(a);
+  const int i = b_.x * TE_BX + (int)threadIdx.x, j = b_.y * TE_BY + (int)threadIdx.y, k = b_.z * TE_BZ + (int)threadIdx.z;
   if (i >= (int)a.n[0] || j >= (int)a.n[1] || k >= (int)a.n[2]) return;
   Dims g = dims(M, a.p[0]);
   const int stag = STC != 0 ? STC : ldi(M, a.p[1]), st = stag < 0 ? -stag : stag;
   const FusedFlux<KIND> F{g, stag, ldi(M, a.p[20]), ldi(M, a.p[21]), fp(M, a.p[2]), fp(M, a.p[13]), fp(M, a.p[14]),
                           fp(M, a.p[4]), fp(M, a.p[15]), fp(M, a.p[16]), ldf(M, a.p[5])};
-  // faces of this cell's divergence: low and high in x, y and z
-  const int xlo = st == 2 ? i - 1 : i, ylo = st == 3 ? j - 1 : j;
-  const float fxl = F.template at<0>(xlo, j, k), fxh = F.template at<0>(xlo + 1, j, k);
-  const float fyl = F.template at<1>(i, ylo, k), fyh = F.template at<1>(i, ylo + 1, k);
-  const int zlo = st == 4 ? k - 1 : k;
-  const float fzl = F.template at<2>(i, j, zlo), fzh = F.template at<2>(i, j, zlo + 1);
+  // faces of this cell's divergence: low and high in x, y and z, indexed
+  // from the cell's own index (one index computation per cell)
+  const int64_t c = g.idx(i, j, k), sj = g.sJ, sk = g.sk();
+  const int xlo = st == 2 ? i - 1 : i, ylo = st == 3 ? j - 1 : j, zlo = st == 4 ? k - 1 : k;
+  const int64_t cx = st == 2 ? c - 1 : c, cy = st == 3 ? c - sj : c, cz = st == 4 ? c - sk : c;
+  const float fxl = F.template at<0>(xlo, j, k, cx), fxh = F.template at<0>(xlo + 1, j, k, cx + 1);
+  const float fyl = F.template at<1>(i, ylo, k, cy), fyh = F.template at<1>(i, ylo + 1, k, cy + sj);
+  const float fzl = F.template at<2>(i, j, zlo, cz), fzh = F.template at<2>(i, j, zlo + 1, cz + sk);
   // p[32] -> chain flags: the pointwise dispatches around a field's assembly
   // in the reference, applied to the tendency in registers:
   // 16 turb_zero of the accumulating sink before it (tend = 0 + t),
@@ -492,7 +483,7 @@ __device__ __forceinline__ void adv_fused_body(uint32_t* M, const KArg& a) {
   const int chain = ldi(M, a.p[32]);
   float* tend = fp(M, a.p[11]);
   assemble_at<STC>(
-      M, a, g, i, j, k,
+      M, a, g, i, j, k, c,
       [&](int d, int fi, int fj, int fk) {
         if (d == 0) return fi == xlo ? fxl : fxh;
         if (d == 1) return fj == ylo ? fyl : fyh;

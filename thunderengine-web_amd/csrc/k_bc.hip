@@ -130,7 +130,7 @@ __global__ __launch_bounds__(256) void k_turb_openbc_stress_y(uint32_t* M, KArg 
 
 // copy_words (replay.js:403-406): end-of-step state swap copies.
 __global__ __launch_bounds__(256) void k__copy(uint32_t* M, KArg a) {
-  const uint32_t q = (uint32_t)te_block().x * blockDim.x + threadIdx.x;
+  const uint32_t q = (uint32_t)(te_block(a).x * TE_BX) + threadIdx.x;
   if (q >= a.n[0]) return;
   M[a.p[1] + q] = M[a.p[0] + q];
 }

@@ -92,7 +92,7 @@ __global__ __launch_bounds__(256) void k_tendency_reset_add(uint32_t* M, KArg a)
 // Up to 4 buffer copies of a.n[0] words each (the state swap of
 // timestep.js:766-774): p[1..4] sources, p[5..8] destinations, p[9] -> count.
 __global__ __launch_bounds__(256) void k_copy_fields(uint32_t* M, KArg a) {
-  const uint32_t q = (uint32_t)te_block().x * blockDim.x + threadIdx.x;
+  const uint32_t q = (uint32_t)(te_block(a).x * TE_BX) + threadIdx.x;
   if (q >= a.n[0]) return;
   const int n = ldi(M, a.p[9]);
   for (int f = 0; f < 4; ++f) {

@@ -322,8 +322,8 @@ __global__ __launch_bounds__(256) void k_turb_gettau(uint32_t* M, KArg a) {
 // turb_sfcstress.wgsl:132-207 (free-slip and no-slip bottom; surface-flux
 // modes need the surface-layer planes and are not on this path)
 __global__ __launch_bounds__(256) void k_turb_sfcstress(uint32_t* M, KArg a) {
-  const Blk b_ = te_block();
-  const int i = (int)(b_.x * blockDim.x + threadIdx.x), j = (int)(b_.y * blockDim.y + threadIdx.y);
+  const Blk b_ = te_block(a);
+  const int i = b_.x * TE_BX + (int)threadIdx.x, j = b_.y * TE_BY + (int)threadIdx.y;
   if (i >= (int)a.n[0] || j >= (int)a.n[1]) return;
   Dims g = dims(M, a.p[0]);
   if (i > g.nx || j > g.ny) return;
@@ -557,23 +557,23 @@ __device__ void uv_tendency(uint32_t* M, const KArg& a, int i, int j) {
   }
 }
 __global__ __launch_bounds__(256) void k_turb_u_tendency(uint32_t* M, KArg a) {
-  const Blk b_ = te_block();
-  const int i = (int)(b_.x * blockDim.x + threadIdx.x), j = (int)(b_.y * blockDim.y + threadIdx.y);
+  const Blk b_ = te_block(a);
+  const int i = b_.x * TE_BX + (int)threadIdx.x, j = b_.y * TE_BY + (int)threadIdx.y;
   if (i < (int)a.n[0] && j < (int)a.n[1]) uv_tendency<0, false>(M, a, i, j);
 }
 __global__ __launch_bounds__(256) void k_turb_u_tendency_lds(uint32_t* M, KArg a) {
-  const Blk b_ = te_block();
-  const int i = (int)(b_.x * blockDim.x + threadIdx.x), j = (int)(b_.y * blockDim.y + threadIdx.y);
+  const Blk b_ = te_block(a);
+  const int i = b_.x * TE_BX + (int)threadIdx.x, j = b_.y * TE_BY + (int)threadIdx.y;
   if (i < (int)a.n[0] && j < (int)a.n[1]) uv_tendency<0, true>(M, a, i, j);
 }
 __global__ __launch_bounds__(256) void k_turb_v_tendency(uint32_t* M, KArg a) {
-  const Blk b_ = te_block();
-  const int i = (int)(b_.x * blockDim.x + threadIdx.x), j = (int)(b_.y * blockDim.y + threadIdx.y);
+  const Blk b_ = te_block(a);
+  const int i = b_.x * TE_BX + (int)threadIdx.x, j = b_.y * TE_BY + (int)threadIdx.y;
   if (i < (int)a.n[0] && j < (int)a.n[1]) uv_tendency<1, false>(M, a, i, j);
 }
 __global__ __launch_bounds__(256) void k_turb_v_tendency_lds(uint32_t* M, KArg a) {
-  const Blk b_ = te_block();
-  const int i = (int)(b_.x * blockDim.x + threadIdx.x), j = (int)(b_.y * blockDim.y + threadIdx.y);
+  const Blk b_ = te_block(a);
+  const int i = b_.x * TE_BX + (int)threadIdx.x, j = b_.y * TE_BY + (int)threadIdx.y;
   if (i < (int)a.n[0] && j < (int)a.n[1]) uv_tendency<1, true>(M, a, i, j);
 }
 
@@ -707,8 +707,8 @@ struct FaceTendency {
 // turb_w_tendency.wgsl:152-287
 template <bool LDS>
 __device__ void w_tendency(uint32_t* M, const KArg& a) {
-  const Blk b_ = te_block();
-  const int i = (int)(b_.x * blockDim.x + threadIdx.x), j = (int)(b_.y * blockDim.y + threadIdx.y);
+  const Blk b_ = te_block(a);
+  const int i = b_.x * TE_BX + (int)threadIdx.x, j = b_.y * TE_BY + (int)threadIdx.y;
   if (i >= (int)a.n[0] || j >= (int)a.n[1]) return;
   Dims g = dims(M, a.p[0]);
   if (i >= g.nx || j >= g.ny) return;
@@ -766,8 +766,8 @@ __global__ __launch_bounds__(256) void k_turb_scalar_flux_h(uint32_t* M, KArg a)
 // turb_scalar_vert_implicit.wgsl:210-327
 template <bool LDS>
 __device__ void scalar_vert_implicit(uint32_t* M, const KArg& a) {
-  const Blk b_ = te_block();
-  const int i = (int)(b_.x * blockDim.x + threadIdx.x), j = (int)(b_.y * blockDim.y + threadIdx.y);
+  const Blk b_ = te_block(a);
+  const int i = b_.x * TE_BX + (int)threadIdx.x, j = b_.y * TE_BY + (int)threadIdx.y;
   if (i >= (int)a.n[0] || j >= (int)a.n[1]) return;
   Dims g = dims(M, a.p[0]);
   if (i >= g.nx || j >= g.ny) return;
@@ -929,8 +929,8 @@ __global__ __launch_bounds__(256) void k_tke_flux_h(uint32_t* M, KArg a) {
 // tke_transport.wgsl:158-301
 template <bool LDS>
 __device__ void tke_transport(uint32_t* M, const KArg& a) {
-  const Blk b_ = te_block();
-  const int i = (int)(b_.x * blockDim.x + threadIdx.x), j = (int)(b_.y * blockDim.y + threadIdx.y);
+  const Blk b_ = te_block(a);
+  const int i = b_.x * TE_BX + (int)threadIdx.x, j = b_.y * TE_BY + (int)threadIdx.y;
   if (i >= (int)a.n[0] || j >= (int)a.n[1]) return;
   Dims g = dims(M, a.p[0]);
   if (i >= g.nx || j >= g.ny) return;

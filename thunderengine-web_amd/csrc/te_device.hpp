@@ -20,6 +20,11 @@ namespace te {
 struct KArg {
   uint32_t n[3];
   uint32_t p[48];  // parameters 0..31 as in the reference's slots; 32..47 for fused launches
+  // launch shape, filled by the runtime: block dims (3), grid blocks in x and
+  // y (2), m = ceil(2^40 / d) as (low, high) words for d = grid blocks in x
+  // and in y (4; floor(l / d) = (l * m) >> 40 exactly for l < 2^24 and
+  // d <= 2^16: Granlund & Montgomery, PLDI 1994, thm. 4.2), grid blocks (1)
+  uint32_t lg[10];
 };
 
 #define TE_DEV __device__ __forceinline__
@@ -223,8 +228,8 @@ struct ColStore {
     return p[base + k * sk];
   }
 };
-#define TE_LANE ((int)(threadIdx.x + blockDim.x * threadIdx.y))
-#define TE_NLANES ((int)(blockDim.x * blockDim.y))
+#define TE_LANE ((int)(threadIdx.x + a.lg[0] * threadIdx.y))
+#define TE_NLANES ((int)(a.lg[0] * a.lg[1]))
 
 // Workgroup coordinates, renumbered for the 8 XCDs.  The dispatcher hands
 // workgroups to the XCDs round-robin in dispatch order (workgroup w, counted
@@ -240,27 +245,35 @@ struct ColStore {
 struct Blk {
   int x, y, z;
 };
-__device__ __forceinline__ Blk te_block() {
-  const uint32_t nbx = gridDim.x, nby = gridDim.y;
+TE_DEV uint32_t te_div(uint32_t l, uint32_t mlo, uint32_t mhi) {
+  return (uint32_t)((((uint64_t)mhi << 32 | mlo) * (uint64_t)l) >> 40);
+}
+__device__ __forceinline__ Blk te_block(const KArg& a) {
+  const uint32_t nbx = a.lg[3], nby = a.lg[4];
 #if TE_XCD_REMAP
-  const uint32_t n = nbx * nby * gridDim.z;
+  const uint32_t n = a.lg[9];
   const uint32_t w = blockIdx.x + nbx * (blockIdx.y + nby * blockIdx.z);
   const uint32_t q = n >> 3, r = n & 7u, x = w & 7u;
   const uint32_t l = x * q + (x < r ? x : r) + (w >> 3);
-  const uint32_t yz = l / nbx;
-  return Blk{(int)(l - yz * nbx), (int)(yz % nby), (int)(yz / nby)};
+  const uint32_t yz = te_div(l, a.lg[5], a.lg[6]), z = te_div(yz, a.lg[7], a.lg[8]);
+  return Blk{(int)(l - yz * nbx), (int)(yz - z * nby), (int)z};
 #else
   (void)nbx;
   (void)nby;
   return Blk{(int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z};
 #endif
 }
+// block dims as the runtime launched them (kernel arguments, not the
+// dispatch packet)
+#define TE_BX ((int)a.lg[0])
+#define TE_BY ((int)a.lg[1])
+#define TE_BZ ((int)a.lg[2])
 
 #define TE_TID3                                                          \
-  const Blk te_b_ = te_block();                                          \
-  const int gx = (int)(te_b_.x * blockDim.x + threadIdx.x);              \
-  const int gy = (int)(te_b_.y * blockDim.y + threadIdx.y);              \
-  const int gz = (int)(te_b_.z * blockDim.z + threadIdx.z);              \
+  const Blk te_b_ = te_block(a);                                         \
+  const int gx = te_b_.x * TE_BX + (int)threadIdx.x;                     \
+  const int gy = te_b_.y * TE_BY + (int)threadIdx.y;                     \
+  const int gz = te_b_.z * TE_BZ + (int)threadIdx.z;                     \
   if (gx >= (int)a.n[0] || gy >= (int)a.n[1] || gz >= (int)a.n[2]) return;
 
 }  // namespace te

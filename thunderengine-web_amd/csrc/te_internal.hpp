@@ -50,6 +50,7 @@ void fill_initial_state(const te_config& cfg, const Layout& L, std::vector<uint3
 
 struct te_ctx {
   uint32_t* dBase = nullptr;  // allocation (front guard + M + back guard)
+  uint64_t front = 0;         // words of the front guard (dM = dBase + front)
   uint32_t* dM = nullptr;     // word 0 of the program's memory M
   uint64_t words = 0;       // allocated words (program memory + guard region)
   uint64_t prog_words = 0;  // words of the program's memory M

@@ -49,6 +49,23 @@ static dim3 block_for(const uint32_t* n) {
   return dim3(bx, by, bz);
 }
 
+// Block dims, grid blocks and the division constants te_block() uses to
+// renumber workgroups for the XCDs (te_device.hpp).
+static bool launch_shape(KArg& a, const dim3& b, const dim3& g) {
+  if ((uint64_t)g.x * g.y * g.z >= (1u << 24) || g.x > (1u << 16) || g.y > (1u << 16)) return false;
+  auto magic = [](uint32_t d, uint32_t* w) {
+    const uint64_t m = ((uint64_t)1 << 40) / d + (((uint64_t)1 << 40) % d != 0);
+    w[0] = (uint32_t)m;
+    w[1] = (uint32_t)(m >> 32);
+  };
+  a.lg[0] = b.x; a.lg[1] = b.y; a.lg[2] = b.z;
+  a.lg[3] = g.x; a.lg[4] = g.y;
+  magic(g.x, &a.lg[5]);
+  magic(g.y, &a.lg[7]);
+  a.lg[9] = g.x * g.y * g.z;
+  return true;
+}
+
 static KArg karg_of(const Op& op) {
   KArg a;
   a.n[0] = op.A[0]; a.n[1] = op.A[1]; a.n[2] = op.A[2];
@@ -96,6 +113,7 @@ static hipError_t launch(te_ctx* c, const Op& op) {
     if (uses_column_lds(op.kid)) lds = (size_t)2 * op.nz * b.x * sizeof(float);
   }
   dim3 g((a.n[0] + b.x - 1) / b.x, (a.n[1] + b.y - 1) / b.y, (a.n[2] + b.z - 1) / b.z);
+  if (!launch_shape(a, b, g)) return hipErrorInvalidConfiguration;  // beyond te_block()'s exact range
   void* args[] = {(void*)&c->dM, (void*)&a};
   return hipLaunchKernel(kKernels[op.kid].fn, g, b, args, lds, c->stream);
 }
@@ -162,22 +180,26 @@ static int validate_ops(te_ctx* c) {
   return TE_OK;
 }
 
-// Words of zeroed memory kept in front of M: no kernel indexes below a
-// parameter's offset by more than a few rows, so this keeps every access of a
-// validated program inside the allocation.
+// Words of zeroed memory kept in front of M (at least; the guard grows to the
+// back guard's size once the program's grids are known): kernels read
+// stencil windows up to 3 levels beyond a field's ends, so both guards cover
+// a whole field plus 2 levels and every access of a validated program stays
+// inside the allocation.
 static const uint64_t kFrontGuard = 1u << 20;
 
-static int alloc_memory(te_ctx* c, uint64_t words) {
+static int alloc_memory(te_ctx* c, uint64_t words, uint64_t front = kFrontGuard) {
+  front = std::max(front, std::max<uint64_t>(c->front, kFrontGuard));
   uint32_t* base = nullptr;
-  TE_HIP(hipMalloc((void**)&base, (kFrontGuard + words) * 4), "hipMalloc(M)");
-  TE_HIP(hipMemsetAsync(base, 0, (kFrontGuard + words) * 4, c->stream), "hipMemset(M)");
+  TE_HIP(hipMalloc((void**)&base, (front + words) * 4), "hipMalloc(M)");
+  TE_HIP(hipMemsetAsync(base, 0, (front + words) * 4, c->stream), "hipMemset(M)");
   if (c->dBase) {
-    TE_HIP(hipMemcpyAsync(base + kFrontGuard, c->dM, c->prog_words * 4, hipMemcpyDeviceToDevice, c->stream), "copy M");
+    TE_HIP(hipMemcpyAsync(base + front, c->dM, c->prog_words * 4, hipMemcpyDeviceToDevice, c->stream), "copy M");
     TE_HIP(hipStreamSynchronize(c->stream), "copy M sync");
     (void)hipFree(c->dBase);
   }
   c->dBase = base;
-  c->dM = base + kFrontGuard;
+  c->dM = base + front;
+  c->front = front;
   c->words = words;
   return TE_OK;
 }
@@ -228,8 +250,8 @@ static int ensure_guard(te_ctx* c) {
     }
   }
   const uint64_t want = c->prog_words + need;
-  if (want > c->words) {
-    int rc = alloc_memory(c, want);
+  if (want > c->words || need > c->front) {
+    int rc = alloc_memory(c, std::max(want, c->words), need);
     if (rc != TE_OK) return rc;
   }
   c->guard_ok = true;
