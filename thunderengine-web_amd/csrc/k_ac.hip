@@ -9,13 +9,13 @@ namespace te {
 // vec_v = nx>3 && i<(nx/4)*4, as in the reference)
 __global__ __launch_bounds__(256) void k_acoustic_prepare(uint32_t* M, KArg a) {
   TE_TID3
-  Dims g = dims(M, a.p[0]);
+  Dims g = dims(a.q[0], 0);
   int i = gx, j = gy, k = gz;
   if (i >= g.nx || j >= g.ny || k >= g.nz) return;
-  Acoustic ac = acoustic(M, a.p[1]);
-  const float *ppterm = fp(M, a.p[2]), *piadv = fp(M, a.p[3]), *thv = fp(M, a.p[4]), *rr0 = fp(M, a.p[5]),
-              *rf0 = fp(M, a.p[6]), *mh = fp(M, a.p[7]), *mf = fp(M, a.p[8]), *c1 = fp(M, a.p[9]), *c2 = fp(M, a.p[10]);
-  float *qk = fp(M, a.p[11]), *pk1 = fp(M, a.p[12]), *pk2 = fp(M, a.p[13]), *mm = fp(M, a.p[14]);
+  Acoustic ac = acoustic(a.q[1], 0);
+  const float *ppterm = fq(a.q[2]), *piadv = fq(a.q[3]), *thv = fq(a.q[4]), *rr0 = fq(a.q[5]),
+              *rf0 = fq(a.q[6]), *mh = fq(a.q[7]), *mf = fq(a.q[8]), *c1 = fq(a.q[9]), *c2 = fq(a.q[10]);
+  float *qk = fq(a.q[11]), *pk1 = fq(a.q[12]), *pk2 = fq(a.q[13]), *mm = fq(a.q[14]);
   const int64_t sk = g.sk(), c = g.idx(i, j, k);
   float tem = (ac.dts * g.rdz) * mh[k];
   float r2 = (tem * rr0[k]) * rf0[k + 1];
@@ -37,27 +37,27 @@ __global__ __launch_bounds__(256) void k_acoustic_prepare(uint32_t* M, KArg a) {
 // acoustic_begin.wgsl:128-149
 __global__ __launch_bounds__(256) void k_acoustic_begin(uint32_t* M, KArg a) {
   TE_TID3
-  Dims g = dims(M, a.p[0]);
+  Dims g = dims(a.q[0], 0);
   int i = gx - 3, j = gy - 3, k = gz;
   if (i > g.nx + 3 || j > g.ny + 3 || k > g.nz) return;
-  Acoustic ac = acoustic(M, a.p[1]);
+  Acoustic ac = acoustic(a.q[1], 0);
   const int64_t c = g.idx(i, j, k);
-  fp(M, a.p[3])[c] = 0.0f;
-  fp(M, a.p[4])[c] = 0.0f;
-  fp(M, a.p[5])[c] = fp(M, a.p[2])[c] * ac.beta;
+  fq(a.q[3])[c] = 0.0f;
+  fq(a.q[4])[c] = 0.0f;
+  fq(a.q[5])[c] = fq(a.q[2])[c] * ac.beta;
 }
 
 // acoustic_uv.wgsl:144-181
 __global__ __launch_bounds__(256) void k_acoustic_uv(uint32_t* M, KArg a) {
   TE_TID3
-  Dims g = dims(M, a.p[0]);
+  Dims g = dims(a.q[0], 0);
   int i = gx, j = gy, k = gz;
   if (i > g.nx || j > g.ny || k >= g.nz) return;
-  Acoustic ac = acoustic(M, a.p[1]);
-  const float *ppd = fp(M, a.p[2]), *thv = fp(M, a.p[3]), *uten = fp(M, a.p[4]), *vten = fp(M, a.p[5]),
-              *hm = fp(M, a.p[30]);
-  float *u = fp(M, a.p[6]), *v = fp(M, a.p[7]);
-  int openx = ldi(M, a.p[8]), openy = ldi(M, a.p[8] + 1);
+  Acoustic ac = acoustic(a.q[1], 0);
+  const float *ppd = fq(a.q[2]), *thv = fq(a.q[3]), *uten = fq(a.q[4]), *vten = fq(a.q[5]),
+              *hm = fq(a.q[30]);
+  float *u = fq(a.q[6]), *v = fq(a.q[7]);
+  int openx = ldi(a.q[8], 0), openy = ldi(a.q[8], 1);
   const int64_t sj = g.sJ, c = g.idx(i, j, k);
   float tem1 = (g.rdx * CP) * 0.5f;
   float tem2 = (g.rdy * CP) * 0.5f;
@@ -70,13 +70,13 @@ __global__ __launch_bounds__(256) void k_acoustic_uv(uint32_t* M, KArg a) {
 // acoustic_fpk.wgsl:167-207
 __global__ __launch_bounds__(256) void k_acoustic_fpk(uint32_t* M, KArg a) {
   TE_TID3
-  Dims g = dims(M, a.p[0]);
+  Dims g = dims(a.q[0], 0);
   int i = gx, j = gy, k = gz;
   if (i >= g.nx || j >= g.ny || k >= g.nz) return;
-  Acoustic ac = acoustic(M, a.p[1]);
-  const float *u = fp(M, a.p[2]), *v = fp(M, a.p[3]), *w = fp(M, a.p[4]), *piadv = fp(M, a.p[5]),
-              *ppten = fp(M, a.p[6]), *ppterm = fp(M, a.p[7]), *pk1 = fp(M, a.p[8]), *pk2 = fp(M, a.p[9]),
-              *qk = fp(M, a.p[10]), *hm = fp(M, a.p[30]);
+  Acoustic ac = acoustic(a.q[1], 0);
+  const float *u = fq(a.q[2]), *v = fq(a.q[3]), *w = fq(a.q[4]), *piadv = fq(a.q[5]),
+              *ppten = fq(a.q[6]), *ppterm = fq(a.q[7]), *pk1 = fq(a.q[8]), *pk2 = fq(a.q[9]),
+              *qk = fq(a.q[10]), *hm = fq(a.q[30]);
   const int64_t sj = g.sJ, sk = g.sk(), c = g.idx(i, j, k);
   float temx = (ac.dts * 0.5f) * g.rdx;
   float temy = (ac.dts * 0.5f) * g.rdy;
@@ -85,7 +85,7 @@ __global__ __launch_bounds__(256) void k_acoustic_fpk(uint32_t* M, KArg a) {
   float Au = fma_(u[c], piadv[c] - piadv[c - 1], u[c + 1] * (piadv[c + 1] - piadv[c]));
   float Av = fma_(v[c], piadv[c] - piadv[c - sj], v[c + sj] * (piadv[c + sj] - piadv[c]));
   float advPi = fma_((-Au) * temx, hm_uh(hm, g, i), -((Av * temy) * hm_vh(hm, g, j)));
-  fp(M, a.p[11])[c] = fma_(ac.beta, fma_(w[c], pk1[c] + qk[c], w[c + sk] * (pk2[c] - qk[c])),
+  fq(a.q[11])[c] = fma_(ac.beta, fma_(w[c], pk1[c] + qk[c], w[c + sk] * (pk2[c] - qk[c])),
                            fma_(ac.dts, fma_(-ppterm[c], div, ppten[c]), advPi));
 }
 
@@ -102,12 +102,12 @@ __device__ void w_solve(uint32_t* M, const KArg& a) {
   const Blk b_ = te_block(a);
   const int i = b_.x * TE_BX + (int)threadIdx.x, j = b_.y * TE_BY + (int)threadIdx.y;
   if (i >= (int)a.n[0] || j >= (int)a.n[1]) return;
-  Dims g = dims(M, a.p[0]);
+  Dims g = dims(a.q[0], 0);
   if (i >= g.nx || j >= g.ny) return;
-  Acoustic ac = acoustic(M, a.p[1]);
-  const float *pk1 = fp(M, a.p[2]), *pk2 = fp(M, a.p[3]), *qk = fp(M, a.p[4]), *mm = fp(M, a.p[5]),
-              *fpk = fp(M, a.p[6]), *wten = fp(M, a.p[7]), *pp3d = fp(M, a.p[8]);
-  float *w = fp(M, a.p[9]), *cA = fp(M, a.p[10]), *cB = fp(M, a.p[11]);
+  Acoustic ac = acoustic(a.q[1], 0);
+  const float *pk1 = fq(a.q[2]), *pk2 = fq(a.q[3]), *qk = fq(a.q[4]), *mm = fq(a.q[5]),
+              *fpk = fq(a.q[6]), *wten = fq(a.q[7]), *pp3d = fq(a.q[8]);
+  float *w = fq(a.q[9]), *cA = fq(a.q[10]), *cB = fq(a.q[11]);
   const int nz = g.nz;
   const int64_t sk = g.sk(), base = g.idx(i, j, 0);
   const float asq = ac.alph * ac.alph;
@@ -187,13 +187,13 @@ __global__ __launch_bounds__(256) void k_acoustic_w_solve_lds(uint32_t* M, KArg 
 // acoustic_pp.wgsl:131-167
 __global__ __launch_bounds__(256) void k_acoustic_pp(uint32_t* M, KArg a) {
   TE_TID3
-  Dims g = dims(M, a.p[0]);
+  Dims g = dims(a.q[0], 0);
   int i = gx, j = gy, k = gz;
   if (i >= g.nx || j >= g.ny || k >= g.nz) return;
-  Acoustic ac = acoustic(M, a.p[1]);
-  const float *w = fp(M, a.p[2]), *pk1 = fp(M, a.p[3]), *pk2 = fp(M, a.p[4]), *qk = fp(M, a.p[5]), *fpk = fp(M, a.p[6]);
-  float *pp3d = fp(M, a.p[7]), *ppd = fp(M, a.p[8]), *ppx = fp(M, a.p[9]);
-  int saveX = ldi(M, a.p[10]);
+  Acoustic ac = acoustic(a.q[1], 0);
+  const float *w = fq(a.q[2]), *pk1 = fq(a.q[3]), *pk2 = fq(a.q[4]), *qk = fq(a.q[5]), *fpk = fq(a.q[6]);
+  float *pp3d = fq(a.q[7]), *ppd = fq(a.q[8]), *ppx = fq(a.q[9]);
+  int saveX = ldi(a.q[10], 0);
   const int64_t sk = g.sk(), c = g.idx(i, j, k);
   float prev = pp3d[c];
   float p = fma_(ac.alph, fma_(w[c], pk1[c] + qk[c], w[c + sk] * (pk2[c] - qk[c])), prev + fpk[c]);
@@ -207,13 +207,13 @@ __global__ __launch_bounds__(256) void k_acoustic_pp(uint32_t* M, KArg a) {
 // acoustic_accumulate.wgsl:129-158
 __global__ __launch_bounds__(256) void k_acoustic_accumulate(uint32_t* M, KArg a) {
   TE_TID3
-  Dims g = dims(M, a.p[0]);
+  Dims g = dims(a.q[0], 0);
   int i = gx - 3, j = gy - 3, k = gz;
   if (i > g.nx + 3 || j > g.ny + 3 || k > g.nz) return;
-  Acoustic ac = acoustic(M, a.p[1]);
-  const float *u = fp(M, a.p[2]), *v = fp(M, a.p[3]), *w = fp(M, a.p[4]);
-  float *rru = fp(M, a.p[5]), *rrv = fp(M, a.p[6]), *rrw = fp(M, a.p[7]);
-  float tavg = ldf(M, a.p[8]);
+  Acoustic ac = acoustic(a.q[1], 0);
+  const float *u = fq(a.q[2]), *v = fq(a.q[3]), *w = fq(a.q[4]);
+  float *rru = fq(a.q[5]), *rrv = fq(a.q[6]), *rrw = fq(a.q[7]);
+  float tavg = ldf(a.q[8], 0);
   const int64_t c = g.idx(i, j, k);
   if (tavg > 0.0f) {
     rru[c] = (rru[c] + u[c]) * tavg;
@@ -243,15 +243,15 @@ TE_DEV bool pdef_inflow(const Dims& g, int dir, int i, int j, int64_t c, int64_t
 // pdef_dum.wgsl:177-231
 __global__ __launch_bounds__(256) void k_pdef_dum(uint32_t* M, KArg a) {
   TE_TID3
-  Dims g = dims(M, a.p[0]);
+  Dims g = dims(a.q[0], 0);
   int i = gx, j = gy, k = gz;
   if (i >= g.nx || j >= g.ny || k >= g.nz) return;
-  int dir = ldi(M, a.p[1]);
-  const float *s = fp(M, a.p[2]), *s0 = fp(M, a.p[3]), *f = fp(M, a.p[4]), *rr = fp(M, a.p[5]), *rho0 = fp(M, a.p[6]),
-              *mh = fp(M, a.p[7]), *hm = fp(M, a.p[30]);
-  float dt = ldf(M, a.p[8]);
-  int openx = ldi(M, a.p[9]), openy = ldi(M, a.p[9] + 1);
-  float* dum = fp(M, a.p[10]);
+  int dir = ldi(a.q[1], 0);
+  const float *s = fq(a.q[2]), *s0 = fq(a.q[3]), *f = fq(a.q[4]), *rr = fq(a.q[5]), *rho0 = fq(a.q[6]),
+              *mh = fq(a.q[7]), *hm = fq(a.q[30]);
+  float dt = ldf(a.q[8], 0);
+  int openx = ldi(a.q[9], 0), openy = ldi(a.q[9], 1);
+  float* dum = fq(a.q[10]);
   const int64_t c = g.idx(i, j, k);
   const int64_t step = dir == 0 ? 1 : (dir == 1 ? (int64_t)g.sJ : g.sk());
   float rd = dir == 0 ? g.rdx : (dir == 1 ? g.rdy : g.rdz);
@@ -290,16 +290,16 @@ TE_DEV float pdef_mass(float dm2, float dm1, float d0, float dp1, float dp2, boo
 // pdef_apply.wgsl:207-282
 __global__ __launch_bounds__(256) void k_pdef_apply(uint32_t* M, KArg a) {
   TE_TID3
-  Dims g = dims(M, a.p[0]);
+  Dims g = dims(a.q[0], 0);
   int i = gx, j = gy, k = gz;
   if (i >= g.nx || j >= g.ny || k >= g.nz) return;
-  const float *s = fp(M, a.p[1]), *s0 = fp(M, a.p[2]), *fx = fp(M, a.p[3]), *fy = fp(M, a.p[4]), *fz = fp(M, a.p[5]),
-              *rru = fp(M, a.p[6]), *rrv = fp(M, a.p[7]), *rrw = fp(M, a.p[8]), *dx = fp(M, a.p[9]),
-              *dy = fp(M, a.p[10]), *dz = fp(M, a.p[11]), *rho0 = fp(M, a.p[12]), *mh = fp(M, a.p[13]),
-              *hm = fp(M, a.p[30]);
-  float dt = ldf(M, a.p[14]);
-  int openx = ldi(M, a.p[15]), openy = ldi(M, a.p[15] + 1);
-  float* tend = fp(M, a.p[16]);
+  const float *s = fq(a.q[1]), *s0 = fq(a.q[2]), *fx = fq(a.q[3]), *fy = fq(a.q[4]), *fz = fq(a.q[5]),
+              *rru = fq(a.q[6]), *rrv = fq(a.q[7]), *rrw = fq(a.q[8]), *dx = fq(a.q[9]),
+              *dy = fq(a.q[10]), *dz = fq(a.q[11]), *rho0 = fq(a.q[12]), *mh = fq(a.q[13]),
+              *hm = fq(a.q[30]);
+  float dt = ldf(a.q[14], 0);
+  int openx = ldi(a.q[15], 0), openy = ldi(a.q[15], 1);
+  float* tend = fq(a.q[16]);
   const int64_t sj = g.sJ, sk = g.sk(), c = g.idx(i, j, k);
   float rdt = 1.0f / dt;
   float base = rho0[k] * s0[c];

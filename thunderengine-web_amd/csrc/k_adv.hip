@@ -54,9 +54,9 @@ TE_DEV bool fv_h5(const Dims& g, const float* s, const float* rr, int i, int j, 
 }
 template <int DIR>
 __device__ void flux_h5(uint32_t* M, const KArg& a, int i, int j, int k) {
-  Dims g = dims(M, a.p[0]);
+  Dims g = dims(a.q[0], 0);
   float f;
-  if (fv_h5<DIR>(g, fp(M, a.p[1]), fp(M, a.p[2]), i, j, k, g.idx(i, j, k), f)) fp(M, a.p[3])[g.idx(i, j, k)] = f;
+  if (fv_h5<DIR>(g, fq(a.q[1]), fq(a.q[2]), i, j, k, g.idx(i, j, k), f)) fq(a.q[3])[g.idx(i, j, k)] = f;
 }
 __global__ __launch_bounds__(256) void k_adv_flux_x(uint32_t* M, KArg a) { TE_TID3 flux_h5<0>(M, a, gx, gy, gz); }
 __global__ __launch_bounds__(256) void k_adv_flux_y(uint32_t* M, KArg a) { TE_TID3 flux_h5<1>(M, a, gx, gy, gz); }
@@ -81,10 +81,10 @@ TE_DEV bool fv_z5(const Dims& g, const float* s, const float* rrw, const float* 
 }
 __global__ __launch_bounds__(256) void k_adv_flux_z(uint32_t* M, KArg a) {
   TE_TID3
-  Dims g = dims(M, a.p[0]);
+  Dims g = dims(a.q[0], 0);
   float f;
-  if (fv_z5(g, fp(M, a.p[1]), fp(M, a.p[2]), fp(M, a.p[3]), fp(M, a.p[4]), gx, gy, gz, g.idx(gx, gy, gz), f))
-    fp(M, a.p[5])[g.idx(gx, gy, gz)] = f;
+  if (fv_z5(g, fq(a.q[1]), fq(a.q[2]), fq(a.q[3]), fq(a.q[4]), gx, gy, gz, g.idx(gx, gy, gz), f))
+    fq(a.q[5])[g.idx(gx, gy, gz)] = f;
 }
 
 // adv_flux_x_weno.wgsl:158-200 / adv_flux_y_weno.wgsl:158-201
@@ -101,10 +101,10 @@ TE_DEV bool fv_h_weno(const Dims& g, const float* s, const float* rr, float weps
 }
 template <int DIR>
 __device__ void flux_h_weno(uint32_t* M, const KArg& a, int i, int j, int k) {
-  Dims g = dims(M, a.p[0]);
+  Dims g = dims(a.q[0], 0);
   float f;
-  if (fv_h_weno<DIR>(g, fp(M, a.p[1]), fp(M, a.p[2]), ldf(M, a.p[3]), ldi(M, a.p[4]) == 1, i, j, k, g.idx(i, j, k), f))
-    fp(M, a.p[5])[g.idx(i, j, k)] = f;
+  if (fv_h_weno<DIR>(g, fq(a.q[1]), fq(a.q[2]), ldf(a.q[3], 0), ldi(a.q[4], 0) == 1, i, j, k, g.idx(i, j, k), f))
+    fq(a.q[5])[g.idx(i, j, k)] = f;
 }
 __global__ __launch_bounds__(256) void k_adv_flux_x_weno(uint32_t* M, KArg a) { TE_TID3 flux_h_weno<0>(M, a, gx, gy, gz); }
 __global__ __launch_bounds__(256) void k_adv_flux_y_weno(uint32_t* M, KArg a) { TE_TID3 flux_h_weno<1>(M, a, gx, gy, gz); }
@@ -151,11 +151,11 @@ TE_DEV bool fv_z_weno(const Dims& g, const float* s, const float* rrw, const flo
 }
 __global__ __launch_bounds__(256) void k_adv_flux_z_weno(uint32_t* M, KArg a) {
   TE_TID3
-  Dims g = dims(M, a.p[0]);
+  Dims g = dims(a.q[0], 0);
   float f;
-  if (fv_z_weno(g, fp(M, a.p[1]), fp(M, a.p[2]), fp(M, a.p[3]), fp(M, a.p[4]), ldf(M, a.p[5]), ldi(M, a.p[7]), gx, gy,
+  if (fv_z_weno(g, fq(a.q[1]), fq(a.q[2]), fq(a.q[3]), fq(a.q[4]), ldf(a.q[5], 0), ldi(a.q[7], 0), gx, gy,
                 gz, g.idx(gx, gy, gz), f))
-    fp(M, a.p[8])[g.idx(gx, gy, gz)] = f;
+    fq(a.q[8])[g.idx(gx, gy, gz)] = f;
 }
 
 TE_DEV float flx5_upwind(const float* s, int64_t c, int64_t st, bool up, bool shifted) {
@@ -184,10 +184,10 @@ TE_DEV bool fv_h_stag(const Dims& g, int stag, const float* s, const float* rr, 
 }
 template <int DIR>
 __device__ void flux_h_stag(uint32_t* M, const KArg& a, int i, int j, int k) {
-  Dims g = dims(M, a.p[0]);
+  Dims g = dims(a.q[0], 0);
   float f;
-  if (fv_h_stag<DIR>(g, ldi(M, a.p[1]), fp(M, a.p[2]), fp(M, a.p[DIR == 0 ? 3 : 4]), fp(M, a.p[7]), i, j, k, g.idx(i, j, k), f))
-    fp(M, a.p[8])[g.idx(i, j, k)] = f;
+  if (fv_h_stag<DIR>(g, ldi(a.q[1], 0), fq(a.q[2]), fq(a.q[DIR == 0 ? 3 : 4]), fq(a.q[7]), i, j, k, g.idx(i, j, k), f))
+    fq(a.q[8])[g.idx(i, j, k)] = f;
 }
 __global__ __launch_bounds__(256) void k_adv_flux_x_stag(uint32_t* M, KArg a) { TE_TID3 flux_h_stag<0>(M, a, gx - 1, gy - 1, gz); }
 __global__ __launch_bounds__(256) void k_adv_flux_y_stag(uint32_t* M, KArg a) { TE_TID3 flux_h_stag<1>(M, a, gx - 1, gy - 1, gz); }
@@ -227,10 +227,10 @@ TE_DEV bool fv_z_stag(const Dims& g, int stag, const float* s, const float* rrw,
 }
 __global__ __launch_bounds__(256) void k_adv_flux_z_stag(uint32_t* M, KArg a) {
   TE_TID3
-  Dims g = dims(M, a.p[0]);
+  Dims g = dims(a.q[0], 0);
   float f;
-  if (fv_z_stag(g, ldi(M, a.p[1]), fp(M, a.p[2]), fp(M, a.p[5]), fp(M, a.p[6]), fp(M, a.p[7]), gx, gy, gz, g.idx(gx, gy, gz), f))
-    fp(M, a.p[8])[g.idx(gx, gy, gz)] = f;
+  if (fv_z_stag(g, ldi(a.q[1], 0), fq(a.q[2]), fq(a.q[5]), fq(a.q[6]), fq(a.q[7]), gx, gy, gz, g.idx(gx, gy, gz), f))
+    fq(a.q[8])[g.idx(gx, gy, gz)] = f;
 }
 
 // adv_flux_x_weno_w.wgsl:143-175 / adv_flux_y_weno_w.wgsl
@@ -248,10 +248,10 @@ TE_DEV bool fv_h_weno_w(const Dims& g, const float* s, const float* rr, const fl
 }
 template <int DIR>
 __device__ void flux_h_weno_w(uint32_t* M, const KArg& a, int i, int j, int k) {
-  Dims g = dims(M, a.p[0]);
+  Dims g = dims(a.q[0], 0);
   float f;
-  if (fv_h_weno_w<DIR>(g, fp(M, a.p[1]), fp(M, a.p[2]), fp(M, a.p[3]), ldf(M, a.p[4]), i, j, k, g.idx(i, j, k), f))
-    fp(M, a.p[5])[g.idx(i, j, k)] = f;
+  if (fv_h_weno_w<DIR>(g, fq(a.q[1]), fq(a.q[2]), fq(a.q[3]), ldf(a.q[4], 0), i, j, k, g.idx(i, j, k), f))
+    fq(a.q[5])[g.idx(i, j, k)] = f;
 }
 __global__ __launch_bounds__(256) void k_adv_flux_x_weno_w(uint32_t* M, KArg a) { TE_TID3 flux_h_weno_w<0>(M, a, gx - 1, gy - 1, gz); }
 __global__ __launch_bounds__(256) void k_adv_flux_y_weno_w(uint32_t* M, KArg a) { TE_TID3 flux_h_weno_w<1>(M, a, gx - 1, gy - 1, gz); }
@@ -277,24 +277,24 @@ TE_DEV bool fv_z_weno_w(const Dims& g, const float* s, const float* rrw, float w
 }
 __global__ __launch_bounds__(256) void k_adv_flux_z_weno_w(uint32_t* M, KArg a) {
   TE_TID3
-  Dims g = dims(M, a.p[0]);
+  Dims g = dims(a.q[0], 0);
   float f;
-  if (fv_z_weno_w(g, fp(M, a.p[1]), fp(M, a.p[2]), ldf(M, a.p[3]), gx, gy, gz, g.idx(gx, gy, gz), f)) fp(M, a.p[4])[g.idx(gx, gy, gz)] = f;
+  if (fv_z_weno_w(g, fq(a.q[1]), fq(a.q[2]), ldf(a.q[3], 0), gx, gy, gz, g.idx(gx, gy, gz), f)) fq(a.q[4])[g.idx(gx, gy, gz)] = f;
 }
 
 // adv_flux_x_weno_uv.wgsl:168-188 / adv_flux_y_weno_uv.wgsl
 template <int DIR>
 __device__ void flux_h_weno_uv(uint32_t* M, const KArg& a, int i, int j, int k) {
-  Dims g = dims(M, a.p[0]);
+  Dims g = dims(a.q[0], 0);
   if (i > g.nx || j > g.ny || k >= g.nz) return;
-  int stag = ldi(M, a.p[1]);
-  const float *s = fp(M, a.p[2]), *rr = fp(M, a.p[3]);
-  float weps = ldf(M, a.p[4]);
+  int stag = ldi(a.q[1], 0);
+  const float *s = fq(a.q[2]), *rr = fq(a.q[3]);
+  float weps = ldf(a.q[4], 0);
   const int64_t sj = g.sJ, st = DIR == 0 ? 1 : sj, c = g.idx(i, j, k);
   float m;
   if (DIR == 0) m = (stag == 2) ? 0.5f * (rr[c] + rr[c + 1]) : 0.5f * (rr[c] + rr[c - sj]);
   else m = (stag == 2) ? 0.5f * (rr[c] + rr[c - 1]) : 0.5f * (rr[c] + rr[c + sj]);
-  fp(M, a.p[5])[c] = m * weno5_upwind(s, c, st, m >= 0.0f, stag == (DIR == 0 ? 2 : 3), weps);
+  fq(a.q[5])[c] = m * weno5_upwind(s, c, st, m >= 0.0f, stag == (DIR == 0 ? 2 : 3), weps);
 }
 __global__ __launch_bounds__(256) void k_adv_flux_x_weno_uv(uint32_t* M, KArg a) { TE_TID3 flux_h_weno_uv<0>(M, a, gx - 1, gy - 1, gz); }
 __global__ __launch_bounds__(256) void k_adv_flux_y_weno_uv(uint32_t* M, KArg a) { TE_TID3 flux_h_weno_uv<1>(M, a, gx - 1, gy - 1, gz); }
@@ -302,13 +302,13 @@ __global__ __launch_bounds__(256) void k_adv_flux_y_weno_uv(uint32_t* M, KArg a)
 // adv_flux_z_weno_uv.wgsl:159-236
 __global__ __launch_bounds__(256) void k_adv_flux_z_weno_uv(uint32_t* M, KArg a) {
   TE_TID3
-  Dims g = dims(M, a.p[0]);
+  Dims g = dims(a.q[0], 0);
   int i = gx, j = gy, k = gz;
   if (i > g.nx || j > g.ny || k > g.nz) return;
-  int stag = ldi(M, a.p[1]);
-  const float *s = fp(M, a.p[2]), *rrw = fp(M, a.p[3]), *c1 = fp(M, a.p[4]), *c2 = fp(M, a.p[5]);
-  float weps = ldf(M, a.p[6]);
-  float* fz = fp(M, a.p[7]);
+  int stag = ldi(a.q[1], 0);
+  const float *s = fq(a.q[2]), *rrw = fq(a.q[3]), *c1 = fq(a.q[4]), *c2 = fq(a.q[5]);
+  float weps = ldf(a.q[6], 0);
+  float* fz = fq(a.q[7]);
   const int64_t sj = g.sJ, sk = g.sk(), c = g.idx(i, j, k);
   if (k == 0 || k == g.nz) { fz[c] = 0.0f; return; }
 #define L(n) s[c + (n) * sk]
@@ -331,19 +331,19 @@ __global__ __launch_bounds__(256) void k_adv_flux_z_weno_uv(uint32_t* M, KArg a)
 template <int STC = 0, class FLUX, class STORE>
 TE_DEV void assemble_at(uint32_t* M, const KArg& a, const Dims& g, int i, int j, int k, int64_t c, FLUX F,
                         STORE store) {
-  const int stag = STC != 0 ? STC : ldi(M, a.p[1]);
+  const int stag = STC != 0 ? STC : ldi(a.q[1], 0);
   int st = stag < 0 ? -stag : stag;
   int iMax = st == 2 ? g.nx : g.nx - 1, jMax = st == 3 ? g.ny : g.ny - 1, kLo = st == 4 ? 1 : 0, kMax = g.nz - 1;
   if (i > iMax || j > jMax || k < kLo || k > kMax) return;
-  int openx = ldi(M, a.p[17]), openy = ldi(M, a.p[17] + 1);
+  int openx = ldi(a.q[17], 0), openy = ldi(a.q[17], 1);
   if (openx != 0 && st == 2 && (i == 0 || i == g.nx)) return;
   if (openy != 0 && st == 3 && (j == 0 || j == g.ny)) return;
-  const float *s = fp(M, a.p[2]), *divx = fp(M, a.p[6]),
-              *rho0 = fp(M, a.p[7]), *rf0 = fp(M, a.p[8]), *mh = fp(M, a.p[9]), *mf = fp(M, a.p[10]),
-              *rru = fp(M, a.p[13]), *rrv = fp(M, a.p[14]), *c1 = fp(M, a.p[15]), *c2 = fp(M, a.p[16]),
-              *hm = fp(M, a.p[30]);
-  float* tend = fp(M, a.p[11]);
-  int accumulate = ldi(M, a.p[12]);
+  const float *s = fq(a.q[2]), *divx = fq(a.q[6]),
+              *rho0 = fq(a.q[7]), *rf0 = fq(a.q[8]), *mh = fq(a.q[9]), *mf = fq(a.q[10]),
+              *rru = fq(a.q[13]), *rrv = fq(a.q[14]), *c1 = fq(a.q[15]), *c2 = fq(a.q[16]),
+              *hm = fq(a.q[30]);
+  float* tend = fq(a.q[11]);
+  int accumulate = ldi(a.q[12], 0);
   const int64_t sj = g.sJ, sk = g.sk();
   float dFx = st == 2 ? F(0, i, j, k) - F(0, i - 1, j, k) : F(0, i + 1, j, k) - F(0, i, j, k);
   float dFy = st == 3 ? F(1, i, j, k) - F(1, i, j - 1, k) : F(1, i, j + 1, k) - F(1, i, j, k);
@@ -404,9 +404,9 @@ TE_DEV void assemble_at(uint32_t* M, const KArg& a, const Dims& g, int i, int j,
 
 __global__ __launch_bounds__(256) void k_adv_assemble(uint32_t* M, KArg a) {
   TE_TID3
-  Dims g = dims(M, a.p[0]);
-  const float *fx = fp(M, a.p[3]), *fy = fp(M, a.p[4]), *fz = fp(M, a.p[5]);
-  float* tend = fp(M, a.p[11]);
+  Dims g = dims(a.q[0], 0);
+  const float *fx = fq(a.q[3]), *fy = fq(a.q[4]), *fz = fq(a.q[5]);
+  float* tend = fq(a.q[11]);
   assemble_at(
       M, a, g, gx, gy, gz, g.idx(gx, gy, gz),
       [&](int d, int i, int j, int k) {
@@ -457,13 +457,16 @@ struct FusedFlux {
 
 template <int KIND, int STC>
 __device__ __forceinline__ void adv_fused_body(uint32_t* M, const KArg& a) {
+  // launched as 64 x 4 x 1 blocks (runtime launch()): a wave is one row of
+  // cells, so j and k are wave-uniform and branches on them are scalar
   const Blk b_ = te_block(a);
-  const int i = b_.x * TE_BX + (int)threadIdx.x, j = b_.y * TE_BY + (int)threadIdx.y, k = b_.z * TE_BZ + (int)threadIdx.z;
-  if (i >= (int)a.n[0] || j >= (int)a.n[1] || k >= (int)a.n[2]) return;
-  Dims g = dims(M, a.p[0]);
-  const int stag = STC != 0 ? STC : ldi(M, a.p[1]), st = stag < 0 ? -stag : stag;
-  const FusedFlux<KIND> F{g, stag, ldi(M, a.p[20]), ldi(M, a.p[21]), fp(M, a.p[2]), fp(M, a.p[13]), fp(M, a.p[14]),
-                          fp(M, a.p[4]), fp(M, a.p[15]), fp(M, a.p[16]), ldf(M, a.p[5])};
+  const int i = b_.x * 64 + (int)threadIdx.x, j = b_.y * 4 + (int)__builtin_amdgcn_readfirstlane(threadIdx.y),
+            k = b_.z;
+  if (j >= (int)a.n[1] || k >= (int)a.n[2] || i >= (int)a.n[0]) return;
+  Dims g = dims(a.q[0], 0);
+  const int stag = STC != 0 ? STC : ldi(a.q[1], 0), st = stag < 0 ? -stag : stag;
+  const FusedFlux<KIND> F{g, stag, ldi(a.q[20], 0), ldi(a.q[21], 0), fq(a.q[2]), fq(a.q[13]), fq(a.q[14]),
+                          fq(a.q[4]), fq(a.q[15]), fq(a.q[16]), ldf(a.q[5], 0)};
   // faces of this cell's divergence: low and high in x, y and z, indexed
   // from the cell's own index (one index computation per cell)
   const int64_t c = g.idx(i, j, k), sj = g.sJ, sk = g.sk();
@@ -480,8 +483,8 @@ __device__ __forceinline__ void adv_fused_body(uint32_t* M, const KArg& a) {
   // 8 rk_update (phi = fma(dttmp, tend, phi_a); p39 phi_a, p40 phi, p41 dttmp;
   // phi must not be the advected field, which other workgroups still read).
   // With an update in the chain the tendency itself is not stored.
-  const int chain = ldi(M, a.p[32]);
-  float* tend = fp(M, a.p[11]);
+  const int chain = ldi(a.q[32], 0);
+  float* tend = fq(a.q[11]);
   assemble_at<STC>(
       M, a, g, i, j, k, c,
       [&](int d, int fi, int fj, int fk) {
@@ -491,11 +494,11 @@ __device__ __forceinline__ void adv_fused_body(uint32_t* M, const KArg& a) {
       },
       [&](int64_t c, float t, int acc) {
         float v = (chain & 16) ? 0.0f + t : (acc != 0 ? tend[c] + t : t);
-        if (chain & 1) v = fma_(-fp(M, a.p[33])[c], fp(M, a.p[34])[c], v);
-        if (chain & 2) v = fma_(ldf(M, a.p[36]), fp(M, a.p[35])[c], v);
-        if (chain & 4) v = fma_(ldf(M, a.p[38]), fp(M, a.p[37])[c], v);
+        if (chain & 1) v = fma_(-fq(a.q[33])[c], fq(a.q[34])[c], v);
+        if (chain & 2) v = fma_(ldf(a.q[36], 0), fq(a.q[35])[c], v);
+        if (chain & 4) v = fma_(ldf(a.q[38], 0), fq(a.q[37])[c], v);
         if (chain & 8) {
-          fp(M, a.p[40])[c] = fma_(ldf(M, a.p[41]), v, fp(M, a.p[39])[c]);
+          fq(a.q[40])[c] = fma_(ldf(a.q[41], 0), v, fq(a.q[39])[c]);
         } else {
           tend[c] = v;
         }
@@ -505,7 +508,7 @@ __device__ __forceinline__ void adv_fused_body(uint32_t* M, const KArg& a) {
 // The three flux dispatches of one advected field and adv_assemble in one
 // launch (see above); the flux kind selects one specialised body.
 __global__ __launch_bounds__(256) void k_adv_fused(uint32_t* M, KArg a) {
-  const int kind = ldi(M, a.p[3]), stag = ldi(M, a.p[1]);
+  const int kind = ldi(a.q[3], 0), stag = ldi(a.q[1], 0);
   // the (kind, stagger) pairs the encoder emits get bodies specialised on both
   if (kind == 0 && stag == 1) adv_fused_body<0, 1>(M, a);
   else if (kind == 1 && stag == 1) adv_fused_body<1, 1>(M, a);

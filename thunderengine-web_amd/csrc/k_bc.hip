@@ -8,10 +8,10 @@ namespace te {
 // bc_periodic_x.wgsl:111-133 / bc_periodic_y.wgsl:111-133
 template <int DIR>
 __device__ void periodic(uint32_t* M, const KArg& a, int h, int t, int k) {
-  Dims g = dims(M, a.p[0]);
+  Dims g = dims(a.q[0], 0);
   const int n = DIR == 0 ? g.nx : g.ny, other = DIR == 0 ? g.ny : g.nx;
   if (h > 3 || t > other + 3 || k > g.nz) return;
-  float* f = fp(M, a.p[1]);
+  float* f = fq(a.q[1]);
   int lo = (((-1 - h) % n) + n) % n;
   int hi = (((n + h) % n) + n) % n;
   auto at = [&](int q) { return DIR == 0 ? g.idx(q, t, k) : g.idx(t, q, k); };
@@ -24,11 +24,11 @@ __global__ __launch_bounds__(256) void k_bc_periodic_y(uint32_t* M, KArg a) { TE
 // bc_open_x.wgsl:111-131 / bc_open_y.wgsl:111-131 (zero-gradient halo)
 template <int DIR>
 __device__ void open_halo(uint32_t* M, const KArg& a, int h, int t, int k) {
-  Dims g = dims(M, a.p[0]);
+  Dims g = dims(a.q[0], 0);
   const int n = DIR == 0 ? g.nx : g.ny, other = DIR == 0 ? g.ny : g.nx;
   if (h >= 3 || t > other + 3 || k > g.nz) return;
-  float* f = fp(M, a.p[1]);
-  int last = n + ldi(M, a.p[2]) - 1;
+  float* f = fq(a.q[1]);
+  int last = n + ldi(a.q[2], 0) - 1;
   auto at = [&](int q) { return DIR == 0 ? g.idx(q, t, k) : g.idx(t, q, k); };
   f[at(-1 - h)] = f[at(0)];
   f[at(last + 1 + h)] = f[at(last)];
@@ -39,14 +39,14 @@ __global__ __launch_bounds__(256) void k_bc_open_y(uint32_t* M, KArg a) { TE_TID
 // openbc_uv_tendency.wgsl:167-208
 __global__ __launch_bounds__(256) void k_openbc_uv_tendency(uint32_t* M, KArg a) {
   TE_TID3
-  Dims g = dims(M, a.p[0]);
+  Dims g = dims(a.q[0], 0);
   const int n = gx, k = gy;
   if (k >= g.nz) return;
-  Acoustic ac = acoustic(M, a.p[1]);
-  const float *rw = fp(M, a.p[2]), *re = fp(M, a.p[3]), *rs = fp(M, a.p[4]), *rn = fp(M, a.p[5]), *uten = fp(M, a.p[6]),
-              *vten = fp(M, a.p[7]), *hm = fp(M, a.p[30]);
-  float *u = fp(M, a.p[8]), *v = fp(M, a.p[9]);
-  int openx = ldi(M, a.p[10]), openy = ldi(M, a.p[10] + 1);
+  Acoustic ac = acoustic(a.q[1], 0);
+  const float *rw = fq(a.q[2]), *re = fq(a.q[3]), *rs = fq(a.q[4]), *rn = fq(a.q[5]), *uten = fq(a.q[6]),
+              *vten = fq(a.q[7]), *hm = fq(a.q[30]);
+  float *u = fq(a.q[8]), *v = fq(a.q[9]);
+  int openx = ldi(a.q[10], 0), openy = ldi(a.q[10], 1);
   const int64_t sj = g.sJ;
   if (openx != 0 && n < g.ny) {
     int64_t cw = g.idx(0, n, k);
@@ -65,11 +65,11 @@ __global__ __launch_bounds__(256) void k_openbc_uv_tendency(uint32_t* M, KArg a)
 // radbc_ew.wgsl:118-161 / radbc_ns.wgsl:118-161
 template <int DIR>
 __device__ void radbc(uint32_t* M, const KArg& a, int t) {
-  Dims g = dims(M, a.p[0]);
+  Dims g = dims(a.q[0], 0);
   if (t >= (DIR == 0 ? g.ny : g.nx)) return;
-  const float *ua = fp(M, a.p[1]), *u = fp(M, a.p[2]), *hm = fp(M, a.p[30]);
-  float dt = ldf(M, a.p[3]);
-  float *lo = fp(M, a.p[4]), *hi = fp(M, a.p[5]);
+  const float *ua = fq(a.q[1]), *u = fq(a.q[2]), *hm = fq(a.q[30]);
+  float dt = ldf(a.q[3], 0);
+  float *lo = fq(a.q[4]), *hi = fq(a.q[5]);
   const float umax = 3.5e+02f;
   const int nq = DIR == 0 ? g.nx : g.ny;
   float d1 = DIR == 0 ? (1.0f / g.rdx) / hm_uf(hm, g, 1) : (1.0f / g.rdy) / hm_vf(hm, g, 1);
@@ -95,11 +95,11 @@ __global__ __launch_bounds__(256) void k_radbc_ns(uint32_t* M, KArg a) { TE_TID3
 // turb_openbc_flux.wgsl:111-140
 __global__ __launch_bounds__(256) void k_turb_openbc_flux(uint32_t* M, KArg a) {
   TE_TID3
-  Dims g = dims(M, a.p[0]);
+  Dims g = dims(a.q[0], 0);
   const int n = gx, k = gy;
   if (k > g.nz) return;
-  float *fx = fp(M, a.p[1]), *fy = fp(M, a.p[2]);
-  int openx = ldi(M, a.p[3]), openy = ldi(M, a.p[3] + 1);
+  float *fx = fq(a.q[1]), *fy = fq(a.q[2]);
+  int openx = ldi(a.q[3], 0), openy = ldi(a.q[3], 1);
   if (openx != 0 && n <= g.ny) {
     fx[g.idx(0, n, k)] = fx[g.idx(1, n, k)];
     fx[g.idx(g.nx, n, k)] = fx[g.idx(g.nx - 1, n, k)];
@@ -113,9 +113,9 @@ __global__ __launch_bounds__(256) void k_turb_openbc_flux(uint32_t* M, KArg a) {
 // turb_openbc_stress_x.wgsl:111-137 / turb_openbc_stress_y.wgsl
 template <int DIR>
 __device__ void openbc_stress(uint32_t* M, const KArg& a, int t, int k) {
-  Dims g = dims(M, a.p[0]);
+  Dims g = dims(a.q[0], 0);
   if (t > (DIR == 0 ? g.ny : g.nx) || k >= g.nz) return;
-  float *sd = fp(M, a.p[1]), *s12 = fp(M, a.p[2]);
+  float *sd = fq(a.q[1]), *s12 = fq(a.q[2]);
   const int nq = DIR == 0 ? g.nx : g.ny;
   auto at = [&](int q) { return DIR == 0 ? g.idx(q, t, k) : g.idx(t, q, k); };
   sd[at(-1)] = sd[at(0)];
@@ -132,7 +132,7 @@ __global__ __launch_bounds__(256) void k_turb_openbc_stress_y(uint32_t* M, KArg 
 __global__ __launch_bounds__(256) void k__copy(uint32_t* M, KArg a) {
   const uint32_t q = (uint32_t)(te_block(a).x * TE_BX) + threadIdx.x;
   if (q >= a.n[0]) return;
-  M[a.p[1] + q] = M[a.p[0] + q];
+  a.q[1][q] = a.q[0][q];
 }
 
 }  // namespace te

@@ -10,15 +10,15 @@ namespace te {
 // update_diagnostics_moist.wgsl:128-184
 __global__ __launch_bounds__(256) void k_update_diagnostics_moist(uint32_t* M, KArg a) {
   TE_TID3
-  Dims g = dims(M, a.p[0]);
+  Dims g = dims(a.q[0], 0);
   int i = gx - 3, j = gy - 3, k = gz;
   if (i > g.nx + 3 || j > g.ny + 3 || k >= g.nz) return;
-  const float *thP = fp(M, a.p[1]), *ppP = fp(M, a.p[2]), *th0 = fp(M, a.p[3]), *pi0 = fp(M, a.p[4]),
-              *qv0 = fp(M, a.p[5]), *qc0 = fp(M, a.p[6]), *qi0 = fp(M, a.p[7]), *rth0 = fp(M, a.p[8]),
-              *qv = fp(M, a.p[9]), *qc = fp(M, a.p[10]), *qr = fp(M, a.p[11]), *qi = fp(M, a.p[12]),
-              *qs = fp(M, a.p[13]), *qg = fp(M, a.p[14]);
-  float *thv = fp(M, a.p[15]), *ppterm = fp(M, a.p[16]), *piadv = fp(M, a.p[17]), *thterm = fp(M, a.p[18]),
-        *buoy = fp(M, a.p[19]);
+  const float *thP = fq(a.q[1]), *ppP = fq(a.q[2]), *th0 = fq(a.q[3]), *pi0 = fq(a.q[4]),
+              *qv0 = fq(a.q[5]), *qc0 = fq(a.q[6]), *qi0 = fq(a.q[7]), *rth0 = fq(a.q[8]),
+              *qv = fq(a.q[9]), *qc = fq(a.q[10]), *qr = fq(a.q[11]), *qi = fq(a.q[12]),
+              *qs = fq(a.q[13]), *qg = fq(a.q[14]);
+  float *thv = fq(a.q[15]), *ppterm = fq(a.q[16]), *piadv = fq(a.q[17]), *thterm = fq(a.q[18]),
+        *buoy = fq(a.q[19]);
   int64_t c = g.idx(i, j, k);
   const float reps = RV / RD, repsm1 = reps - 1.0f;
   float qvv = mx(qv[c], 0.0f);
@@ -40,12 +40,12 @@ __global__ __launch_bounds__(256) void k_update_diagnostics_moist(uint32_t* M, K
 // update_diagnostics.wgsl:114-144 (dry)
 __global__ __launch_bounds__(256) void k_update_diagnostics(uint32_t* M, KArg a) {
   TE_TID3
-  Dims g = dims(M, a.p[0]);
+  Dims g = dims(a.q[0], 0);
   int i = gx - 3, j = gy - 3, k = gz;
   if (i > g.nx + 3 || j > g.ny + 3 || k >= g.nz) return;
-  const float *thP = fp(M, a.p[1]), *ppP = fp(M, a.p[2]), *th0 = fp(M, a.p[3]), *pi0 = fp(M, a.p[4]),
-              *rth0 = fp(M, a.p[8]);
-  float *thv = fp(M, a.p[5]), *ppterm = fp(M, a.p[6]), *piadv = fp(M, a.p[7]), *buoy = fp(M, a.p[9]);
+  const float *thP = fq(a.q[1]), *ppP = fq(a.q[2]), *th0 = fq(a.q[3]), *pi0 = fq(a.q[4]),
+              *rth0 = fq(a.q[8]);
+  float *thv = fq(a.q[5]), *ppterm = fq(a.q[6]), *piadv = fq(a.q[7]), *buoy = fq(a.q[9]);
   const float rddcv = RD / (CP - RD);
   int64_t c = g.idx(i, j, k);
   float pi = pi0[k] + ppP[c];
@@ -58,35 +58,35 @@ __global__ __launch_bounds__(256) void k_update_diagnostics(uint32_t* M, KArg a)
 // stage_tendency_reset.wgsl:110-129
 __global__ __launch_bounds__(256) void k_stage_tendency_reset(uint32_t* M, KArg a) {
   TE_TID3
-  Dims g = dims(M, a.p[0]);
+  Dims g = dims(a.q[0], 0);
   int i = gx - 3, j = gy - 3, k = gz;
   if (i > g.nx + 3 || j > g.ny + 3 || k > g.nz) return;
   int64_t c = g.idx(i, j, k);
-  fp(M, a.p[1])[c] = 0.0f;
-  fp(M, a.p[2])[c] = 0.0f;
-  fp(M, a.p[3])[c] = 0.0f;
+  fq(a.q[1])[c] = 0.0f;
+  fq(a.q[2])[c] = 0.0f;
+  fq(a.q[3])[c] = 0.0f;
 }
 
 // stage_state_reset.wgsl:110-138
 __global__ __launch_bounds__(256) void k_stage_state_reset(uint32_t* M, KArg a) {
   TE_TID3
-  Dims g = dims(M, a.p[0]);
+  Dims g = dims(a.q[0], 0);
   int i = gx - 3, j = gy - 3, k = gz;
   if (i > g.nx + 3 || j > g.ny + 3 || k > g.nz) return;
   int64_t c = g.idx(i, j, k);
   // u <- u_a, v <- v_a, w <- w_a, pp3d <- ppi, ppd <- ppx
   const int src[5] = {1, 2, 3, 4, 10};
-  for (int q = 0; q < 5; ++q) fp(M, a.p[5 + q])[c] = fp(M, a.p[src[q]])[c];
+  for (int q = 0; q < 5; ++q) fq(a.q[5 + q])[c] = fq(a.q[src[q]])[c];
 }
 
 // massflux_from_state.wgsl:111-137
 __global__ __launch_bounds__(256) void k_massflux_from_state(uint32_t* M, KArg a) {
   TE_TID3
-  Dims g = dims(M, a.p[0]);
+  Dims g = dims(a.q[0], 0);
   int i = gx - 3, j = gy - 3, k = gz;
   if (i > g.nx + 3 || j > g.ny + 3 || k > g.nz) return;
-  const float *u = fp(M, a.p[1]), *v = fp(M, a.p[2]), *w = fp(M, a.p[3]), *rho0 = fp(M, a.p[4]), *rf0 = fp(M, a.p[5]);
-  float *rru = fp(M, a.p[6]), *rrv = fp(M, a.p[7]), *rrw = fp(M, a.p[8]);
+  const float *u = fq(a.q[1]), *v = fq(a.q[2]), *w = fq(a.q[3]), *rho0 = fq(a.q[4]), *rf0 = fq(a.q[5]);
+  float *rru = fq(a.q[6]), *rrv = fq(a.q[7]), *rrw = fq(a.q[8]);
   int64_t c = g.idx(i, j, k);
   if (k < g.nz) {
     rru[c] = rho0[k] * u[c];
@@ -98,11 +98,11 @@ __global__ __launch_bounds__(256) void k_massflux_from_state(uint32_t* M, KArg a
 // velocity_to_massflux.wgsl:111-134
 __global__ __launch_bounds__(256) void k_velocity_to_massflux(uint32_t* M, KArg a) {
   TE_TID3
-  Dims g = dims(M, a.p[0]);
+  Dims g = dims(a.q[0], 0);
   int i = gx - 3, j = gy - 3, k = gz;
   if (i > g.nx + 3 || j > g.ny + 3 || k > g.nz) return;
-  const float *rho0 = fp(M, a.p[1]), *rf0 = fp(M, a.p[2]);
-  float *rru = fp(M, a.p[3]), *rrv = fp(M, a.p[4]), *rrw = fp(M, a.p[5]);
+  const float *rho0 = fq(a.q[1]), *rf0 = fq(a.q[2]);
+  float *rru = fq(a.q[3]), *rrv = fq(a.q[4]), *rrw = fq(a.q[5]);
   int64_t c = g.idx(i, j, k);
   if (k < g.nz) {
     rru[c] = rru[c] * rho0[k];
@@ -114,12 +114,12 @@ __global__ __launch_bounds__(256) void k_velocity_to_massflux(uint32_t* M, KArg 
 // div_massflux.wgsl:125-147
 __global__ __launch_bounds__(256) void k_div_massflux(uint32_t* M, KArg a) {
   TE_TID3
-  Dims g = dims(M, a.p[0]);
+  Dims g = dims(a.q[0], 0);
   int i = gx - 3, j = gy - 3, k = gz;
   if (i > g.nx + 2 || j > g.ny + 2 || k >= g.nz) return;
-  const float *rru = fp(M, a.p[1]), *rrv = fp(M, a.p[2]), *rrw = fp(M, a.p[3]), *mh = fp(M, a.p[4]),
-              *hm = fp(M, a.p[30]);
-  float* divx = fp(M, a.p[5]);
+  const float *rru = fq(a.q[1]), *rrv = fq(a.q[2]), *rrw = fq(a.q[3]), *mh = fq(a.q[4]),
+              *hm = fq(a.q[30]);
+  float* divx = fq(a.q[5]);
   const int64_t sj = g.sJ, sk = g.sk(), c = g.idx(i, j, k);
   divx[c] = fma_((rrw[c + sk] - rrw[c]) * g.rdz, mh[k],
                  fma_((rru[c + 1] - rru[c]) * g.rdx, hm_uh(hm, g, i), ((rrv[c + sj] - rrv[c]) * g.rdy) * hm_vh(hm, g, j)));
@@ -128,11 +128,11 @@ __global__ __launch_bounds__(256) void k_div_massflux(uint32_t* M, KArg a) {
 // divergence3d.wgsl:125-147
 __global__ __launch_bounds__(256) void k_divergence3d(uint32_t* M, KArg a) {
   TE_TID3
-  Dims g = dims(M, a.p[0]);
+  Dims g = dims(a.q[0], 0);
   int i = gx, j = gy, k = gz;
   if (i >= g.nx || j >= g.ny || k >= g.nz) return;
-  const float *u = fp(M, a.p[1]), *v = fp(M, a.p[2]), *w = fp(M, a.p[3]), *mh = fp(M, a.p[4]), *hm = fp(M, a.p[30]);
-  float* dv = fp(M, a.p[5]);
+  const float *u = fq(a.q[1]), *v = fq(a.q[2]), *w = fq(a.q[3]), *mh = fq(a.q[4]), *hm = fq(a.q[30]);
+  float* dv = fq(a.q[5]);
   const int64_t sj = g.sJ, sk = g.sk(), c = g.idx(i, j, k);
   float d = fma_((w[c + sk] - w[c]) * g.rdz, mh[k],
                  fma_((u[c + 1] - u[c]) * g.rdx, hm_uh(hm, g, i), ((v[c + sj] - v[c]) * g.rdy) * hm_vh(hm, g, j)));
@@ -142,11 +142,11 @@ __global__ __launch_bounds__(256) void k_divergence3d(uint32_t* M, KArg a) {
 // buoyancy_apply.wgsl:111-129
 __global__ __launch_bounds__(256) void k_buoyancy_apply(uint32_t* M, KArg a) {
   TE_TID3
-  Dims g = dims(M, a.p[0]);
+  Dims g = dims(a.q[0], 0);
   int i = gx, j = gy, k = gz + 1;
   if (i >= g.nx || j >= g.ny || k > g.nz - 1) return;
-  const float *buoy = fp(M, a.p[1]), *c1 = fp(M, a.p[2]), *c2 = fp(M, a.p[3]);
-  float* wten = fp(M, a.p[4]);
+  const float *buoy = fq(a.q[1]), *c1 = fq(a.q[2]), *c2 = fq(a.q[3]);
+  float* wten = fq(a.q[4]);
   const int64_t sk = g.sk(), c = g.idx(i, j, k);
   wten[c] = wten[c] + fma_(c1[k], buoy[c - sk], c2[k] * buoy[c]);
 }
@@ -154,14 +154,14 @@ __global__ __launch_bounds__(256) void k_buoyancy_apply(uint32_t* M, KArg a) {
 // rk_update.wgsl:111-132
 __global__ __launch_bounds__(256) void k_rk_update(uint32_t* M, KArg a) {
   TE_TID3
-  Dims g = dims(M, a.p[0]);
-  int stag = ldi(M, a.p[1]);
+  Dims g = dims(a.q[0], 0);
+  int stag = ldi(a.q[1], 0);
   int i = gx, j = gy, k = gz;
   int iMax = stag == 2 ? g.nx : g.nx - 1, jMax = stag == 3 ? g.ny : g.ny - 1, kLo = stag == 4 ? 1 : 0;
   if (i > iMax || j > jMax || k < kLo || k >= g.nz) return;
-  const float *phiA = fp(M, a.p[2]), *tend = fp(M, a.p[3]);
-  float* phi = fp(M, a.p[4]);
-  float dttmp = ldf(M, a.p[5]);
+  const float *phiA = fq(a.q[2]), *tend = fq(a.q[3]);
+  float* phi = fq(a.q[4]);
+  float dttmp = ldf(a.q[5], 0);
   int64_t c = g.idx(i, j, k);
   phi[c] = fma_(dttmp, tend[c], phiA[c]);
 }
@@ -169,21 +169,21 @@ __global__ __launch_bounds__(256) void k_rk_update(uint32_t* M, KArg a) {
 // build_sadv.wgsl:111-127
 __global__ __launch_bounds__(256) void k_build_sadv(uint32_t* M, KArg a) {
   TE_TID3
-  Dims g = dims(M, a.p[0]);
+  Dims g = dims(a.q[0], 0);
   int i = gx - 3, j = gy - 3, k = gz;
   if (i > g.nx + 3 || j > g.ny + 3 || k >= g.nz) return;
-  const float *th0 = fp(M, a.p[1]), *thP = fp(M, a.p[2]);
+  const float *th0 = fq(a.q[1]), *thP = fq(a.q[2]);
   int64_t c = g.idx(i, j, k);
-  fp(M, a.p[3])[c] = (th0[k] - 3e+02f) + thP[c];
+  fq(a.q[3])[c] = (th0[k] - 3e+02f) + thP[c];
 }
 
 // apply_thterm.wgsl:111-127
 __global__ __launch_bounds__(256) void k_apply_thterm(uint32_t* M, KArg a) {
   TE_TID3
-  Dims g = dims(M, a.p[0]);
+  Dims g = dims(a.q[0], 0);
   if (gx >= g.nx || gy >= g.ny || gz >= g.nz) return;
-  const float *dv = fp(M, a.p[1]), *thterm = fp(M, a.p[2]);
-  float* tend = fp(M, a.p[3]);
+  const float *dv = fq(a.q[1]), *thterm = fq(a.q[2]);
+  float* tend = fq(a.q[3]);
   int64_t c = g.idx(gx, gy, gz);
   tend[c] = fma_(-dv[c], thterm[c], tend[c]);
 }
@@ -191,12 +191,12 @@ __global__ __launch_bounds__(256) void k_apply_thterm(uint32_t* M, KArg a) {
 // turb_add_tendency.wgsl:111-127
 __global__ __launch_bounds__(256) void k_turb_add_tendency(uint32_t* M, KArg a) {
   TE_TID3
-  Dims g = dims(M, a.p[0]);
+  Dims g = dims(a.q[0], 0);
   int i = gx - 3, j = gy - 3, k = gz;
   if (i > g.nx + 3 || j > g.ny + 3 || k > g.nz) return;
-  const float* src = fp(M, a.p[1]);
-  float* dst = fp(M, a.p[2]);
-  float scale = ldf(M, a.p[3]);
+  const float* src = fq(a.q[1]);
+  float* dst = fq(a.q[2]);
+  float scale = ldf(a.q[3], 0);
   int64_t c = g.idx(i, j, k);
   dst[c] = fma_(scale, src[c], dst[c]);
 }
@@ -204,44 +204,44 @@ __global__ __launch_bounds__(256) void k_turb_add_tendency(uint32_t* M, KArg a) 
 // turb_zero.wgsl:103-116
 __global__ __launch_bounds__(256) void k_turb_zero(uint32_t* M, KArg a) {
   TE_TID3
-  Dims g = dims(M, a.p[0]);
+  Dims g = dims(a.q[0], 0);
   if (gx >= g.sJ || gy >= g.sK || gz > g.nz + 1) return;
-  fp(M, a.p[1])[gx + (int64_t)g.sJ * (gy + (int64_t)g.sK * gz)] = 0.0f;
+  fq(a.q[1])[gx + (int64_t)g.sJ * (gy + (int64_t)g.sK * gz)] = 0.0f;
 }
 
 // tke_advance.wgsl:111-129
 __global__ __launch_bounds__(256) void k_tke_advance(uint32_t* M, KArg a) {
   TE_TID3
-  Dims g = dims(M, a.p[0]);
+  Dims g = dims(a.q[0], 0);
   int i = gx, j = gy, k = gz + 1;
   if (i >= g.nx || j >= g.ny || k > g.nz - 1) return;
-  const float *tkea = fp(M, a.p[1]), *tketen = fp(M, a.p[2]);
-  float dttmp = ldf(M, a.p[3]);
+  const float *tkea = fq(a.q[1]), *tketen = fq(a.q[2]);
+  float dttmp = ldf(a.q[3], 0);
   int64_t c = g.idx(i, j, k);
   float v = fma_(dttmp, tketen[c], tkea[c]);
-  fp(M, a.p[4])[c] = (v < 1e-10f) ? 0.0f : v;
+  fq(a.q[4])[c] = (v < 1e-10f) ? 0.0f : v;
 }
 
 // ppx_finish.wgsl:111-126
 __global__ __launch_bounds__(256) void k_ppx_finish(uint32_t* M, KArg a) {
   TE_TID3
-  Dims g = dims(M, a.p[0]);
+  Dims g = dims(a.q[0], 0);
   if (gx >= g.nx || gy >= g.ny || gz >= g.nz) return;
   int64_t c = g.idx(gx, gy, gz);
-  float* ppx = fp(M, a.p[2]);
-  ppx[c] = fp(M, a.p[1])[c] + ppx[c];
+  float* ppx = fq(a.q[2]);
+  ppx[c] = fq(a.q[1])[c] + ppx[c];
 }
 
 // diabatic_unapply.wgsl:111-140
 __global__ __launch_bounds__(256) void k_diabatic_unapply(uint32_t* M, KArg a) {
   TE_TID3
-  Dims g = dims(M, a.p[0]);
+  Dims g = dims(a.q[0], 0);
   if (gx >= g.nx || gy >= g.ny || gz >= g.nz) return;
-  float dt = ldf(M, a.p[1]);
+  float dt = ldf(a.q[1], 0);
   int64_t c = g.idx(gx, gy, gz);
   for (int q = 0; q < 4; ++q) {
-    float* s = fp(M, a.p[2 + q]);
-    float* r = fp(M, a.p[6 + q]);
+    float* s = fq(a.q[2 + q]);
+    float* r = fq(a.q[6 + q]);
     s[c] = fma_(-dt, r[c], s[c]);
     r[c] = s[c];
   }
@@ -250,14 +250,14 @@ __global__ __launch_bounds__(256) void k_diabatic_unapply(uint32_t* M, KArg a) {
 // diabatic_rates.wgsl:111-137
 __global__ __launch_bounds__(256) void k_diabatic_rates(uint32_t* M, KArg a) {
   TE_TID3
-  Dims g = dims(M, a.p[0]);
+  Dims g = dims(a.q[0], 0);
   if (gx >= g.nx || gy >= g.ny || gz >= g.nz) return;
-  float dt = ldf(M, a.p[1]);
+  float dt = ldf(a.q[1], 0);
   float rdt = 1.0f / dt;
   int64_t c = g.idx(gx, gy, gz);
   for (int q = 0; q < 4; ++q) {
-    const float* s = fp(M, a.p[2 + q]);
-    float* r = fp(M, a.p[6 + q]);
+    const float* s = fq(a.q[2 + q]);
+    float* r = fq(a.q[6 + q]);
     r[c] = (s[c] - r[c]) * rdt;
   }
 }
@@ -265,14 +265,14 @@ __global__ __launch_bounds__(256) void k_diabatic_rates(uint32_t* M, KArg a) {
 // pressure_tendency.wgsl:234-291
 __global__ __launch_bounds__(256) void k_pressure_tendency(uint32_t* M, KArg a) {
   TE_TID3
-  Dims g = dims(M, a.p[0]);
+  Dims g = dims(a.q[0], 0);
   if (gx >= g.nx || gy >= g.ny || gz >= g.nz) return;
-  const float *th = fp(M, a.p[1]), *qv = fp(M, a.p[2]), *rho = fp(M, a.p[3]), *ppi = fp(M, a.p[4]),
-              *thten = fp(M, a.p[5]), *qpten = fp(M, a.p[6]), *th0 = fp(M, a.p[7]), *pi0 = fp(M, a.p[8]),
-              *qvt = fp(M, a.p[13]);
-  float* ppten = fp(M, a.p[11]);
-  float dt = ldf(M, a.p[9]);
-  int hasDiabatic = ldi(M, a.p[10]), hasStepTend = ldi(M, a.p[12]), moist = ldi(M, a.p[14]);
+  const float *th = fq(a.q[1]), *qv = fq(a.q[2]), *rho = fq(a.q[3]), *ppi = fq(a.q[4]),
+              *thten = fq(a.q[5]), *qpten = fq(a.q[6]), *th0 = fq(a.q[7]), *pi0 = fq(a.q[8]),
+              *qvt = fq(a.q[13]);
+  float* ppten = fq(a.q[11]);
+  float dt = ldf(a.q[9], 0);
+  int hasDiabatic = ldi(a.q[10], 0), hasStepTend = ldi(a.q[12], 0), moist = ldi(a.q[14], 0);
   const int k = gz;
   int64_t c = g.idx(gx, gy, gz);
   const float rp00 = 1.0f / P00, rddcv = RD / (CP - RD);
@@ -303,12 +303,12 @@ __global__ __launch_bounds__(256) void k_pressure_tendency(uint32_t* M, KArg a) 
 // update_rho_prs.wgsl:234-267
 __global__ __launch_bounds__(256) void k_update_rho_prs(uint32_t* M, KArg a) {
   TE_TID3
-  Dims g = dims(M, a.p[0]);
+  Dims g = dims(a.q[0], 0);
   if (gx >= g.nx || gy >= g.ny || gz >= g.nz) return;
-  int moist = ldi(M, a.p[1]);
-  const float *thP = fp(M, a.p[2]), *ppP = fp(M, a.p[3]), *th0 = fp(M, a.p[4]), *pi0 = fp(M, a.p[5]),
-              *qv = fp(M, a.p[6]);
-  float *prs = fp(M, a.p[7]), *rho = fp(M, a.p[8]);
+  int moist = ldi(a.q[1], 0);
+  const float *thP = fq(a.q[2]), *ppP = fq(a.q[3]), *th0 = fq(a.q[4]), *pi0 = fq(a.q[5]),
+              *qv = fq(a.q[6]);
+  float *prs = fq(a.q[7]), *rho = fq(a.q[8]);
   const int k = gz;
   int64_t c = g.idx(gx, gy, gz);
   const float cpdrd = CP / RD;

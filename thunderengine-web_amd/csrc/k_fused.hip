@@ -17,7 +17,7 @@ namespace te {
 // y = k in [0, nz].  p[1..8] fields, p[9] -> field count.
 __global__ __launch_bounds__(256) void k_bc_periodic_xy(uint32_t* M, KArg a) {
   TE_TID3
-  Dims g = dims(M, a.p[0]);
+  Dims g = dims(a.q[0], 0);
   const int k = gy;
   if (k > g.nz) return;
   const int rowCells = 7 * g.sJ;
@@ -35,10 +35,10 @@ __global__ __launch_bounds__(256) void k_bc_periodic_xy(uint32_t* M, KArg a) {
   }
   const int si = ((i % g.nx) + g.nx) % g.nx, sj = ((j % g.ny) + g.ny) % g.ny;
   const int64_t dst = g.idx(i, j, k), src = g.idx(si, sj, k);
-  const int n = ldi(M, a.p[9]);
+  const int n = ldi(a.q[9], 0);
   for (int q = 0; q < 8; ++q) {
     if (q >= n) break;
-    float* f = fp(M, a.p[1 + q]);
+    float* f = fq(a.q[1 + q]);
     f[dst] = f[src];
   }
 }
@@ -48,13 +48,13 @@ __global__ __launch_bounds__(256) void k_bc_periodic_xy(uint32_t* M, KArg a) {
 // p[9] -> field count.
 __global__ __launch_bounds__(256) void k_zero_fields(uint32_t* M, KArg a) {
   TE_TID3
-  Dims g = dims(M, a.p[0]);
+  Dims g = dims(a.q[0], 0);
   if (gx >= g.sJ || gy >= g.sK || gz > g.nz) return;
   const int64_t c = gx + (int64_t)g.sJ * (gy + (int64_t)g.sK * gz);
-  const int n = ldi(M, a.p[9]);
+  const int n = ldi(a.q[9], 0);
   for (int q = 0; q < 8; ++q) {
     if (q >= n) break;
-    fp(M, a.p[1 + q])[c] = 0.0f;
+    fq(a.q[1 + q])[c] = 0.0f;
   }
 }
 
@@ -63,15 +63,15 @@ __global__ __launch_bounds__(256) void k_zero_fields(uint32_t* M, KArg a) {
 // p[9..12] -> scales, p[13] -> count.
 __global__ __launch_bounds__(256) void k_add_fields(uint32_t* M, KArg a) {
   TE_TID3
-  Dims g = dims(M, a.p[0]);
+  Dims g = dims(a.q[0], 0);
   int i = gx - 3, j = gy - 3, k = gz;
   if (i > g.nx + 3 || j > g.ny + 3 || k > g.nz) return;
   const int64_t c = g.idx(i, j, k);
-  const int n = ldi(M, a.p[13]);
+  const int n = ldi(a.q[13], 0);
   for (int q = 0; q < 4; ++q) {
     if (q >= n) break;
-    float* dst = fp(M, a.p[5 + q]);
-    dst[c] = fma_(ldf(M, a.p[9 + q]), fp(M, a.p[1 + q])[c], dst[c]);
+    float* dst = fq(a.q[5 + q]);
+    dst[c] = fma_(ldf(a.q[9 + q], 0), fq(a.q[1 + q])[c], dst[c]);
   }
 }
 
@@ -81,12 +81,12 @@ __global__ __launch_bounds__(256) void k_add_fields(uint32_t* M, KArg a) {
 // p: 1-3 uten vten wten, 4-6 their sources, 7 -> scale.  Grid: whole.
 __global__ __launch_bounds__(256) void k_tendency_reset_add(uint32_t* M, KArg a) {
   TE_TID3
-  Dims g = dims(M, a.p[0]);
+  Dims g = dims(a.q[0], 0);
   int i = gx - 3, j = gy - 3, k = gz;
   if (i > g.nx + 3 || j > g.ny + 3 || k > g.nz) return;
   const int64_t c = g.idx(i, j, k);
-  const float scale = ldf(M, a.p[7]);
-  for (int q = 0; q < 3; ++q) fp(M, a.p[1 + q])[c] = fma_(scale, fp(M, a.p[4 + q])[c], 0.0f);
+  const float scale = ldf(a.q[7], 0);
+  for (int q = 0; q < 3; ++q) fq(a.q[1 + q])[c] = fma_(scale, fq(a.q[4 + q])[c], 0.0f);
 }
 
 // Up to 4 buffer copies of a.n[0] words each (the state swap of
@@ -94,10 +94,10 @@ __global__ __launch_bounds__(256) void k_tendency_reset_add(uint32_t* M, KArg a)
 __global__ __launch_bounds__(256) void k_copy_fields(uint32_t* M, KArg a) {
   const uint32_t q = (uint32_t)(te_block(a).x * TE_BX) + threadIdx.x;
   if (q >= a.n[0]) return;
-  const int n = ldi(M, a.p[9]);
+  const int n = ldi(a.q[9], 0);
   for (int f = 0; f < 4; ++f) {
     if (f >= n) break;
-    M[a.p[5 + f] + q] = M[a.p[1 + f] + q];
+    a.q[5 + f][q] = a.q[1 + f][q];
   }
 }
 
@@ -109,27 +109,27 @@ __global__ __launch_bounds__(256) void k_copy_fields(uint32_t* M, KArg a) {
 // 9 ppx, 10 -> saveX, 11 u, 12 v, 13 rru, 14 rrv, 15 rrw, 16 -> tavg.
 __global__ __launch_bounds__(256) void k_ac_pp_accumulate(uint32_t* M, KArg a) {
   TE_TID3
-  Dims g = dims(M, a.p[0]);
+  Dims g = dims(a.q[0], 0);
   int i = gx - 3, j = gy - 3, k = gz;
   if (i > g.nx + 3 || j > g.ny + 3 || k > g.nz) return;
-  Acoustic ac = acoustic(M, a.p[1]);
-  const float* w = fp(M, a.p[2]);
+  Acoustic ac = acoustic(a.q[1], 0);
+  const float* w = fq(a.q[2]);
   const int64_t sk = g.sk(), c = g.idx(i, j, k);
   const float wc = w[c];
   if (i >= 0 && i < g.nx && j >= 0 && j < g.ny && k < g.nz) {
-    const float *pk1 = fp(M, a.p[3]), *pk2 = fp(M, a.p[4]), *qk = fp(M, a.p[5]), *fpk = fp(M, a.p[6]);
-    float *pp3d = fp(M, a.p[7]), *ppd = fp(M, a.p[8]), *ppx = fp(M, a.p[9]);
+    const float *pk1 = fq(a.q[3]), *pk2 = fq(a.q[4]), *qk = fq(a.q[5]), *fpk = fq(a.q[6]);
+    float *pp3d = fq(a.q[7]), *ppd = fq(a.q[8]), *ppx = fq(a.q[9]);
     const float prev = pp3d[c];
     float p = fma_(ac.alph, fma_(wc, pk1[c] + qk[c], w[c + sk] * (pk2[c] - qk[c])), prev + fpk[c]);
     if (ab(p) < ac.smeps) p = 0.0f;
     pp3d[c] = p;
     const float dum1 = ac.kdiv * (p - prev);
     ppd[c] = p + dum1;
-    if (ldi(M, a.p[10]) != 0) ppx[c] = dum1;
+    if (ldi(a.q[10], 0) != 0) ppx[c] = dum1;
   }
-  const float *u = fp(M, a.p[11]), *v = fp(M, a.p[12]);
-  float *rru = fp(M, a.p[13]), *rrv = fp(M, a.p[14]), *rrw = fp(M, a.p[15]);
-  const float tavg = ldf(M, a.p[16]);
+  const float *u = fq(a.q[11]), *v = fq(a.q[12]);
+  float *rru = fq(a.q[13]), *rrv = fq(a.q[14]), *rrw = fq(a.q[15]);
+  const float tavg = ldf(a.q[16], 0);
   if (tavg > 0.0f) {
     rru[c] = (rru[c] + u[c]) * tavg;
     rrv[c] = (rrv[c] + v[c]) * tavg;

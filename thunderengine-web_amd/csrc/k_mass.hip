@@ -27,12 +27,12 @@ TE_DEV void df_add_pair(DFs& acc, float vx, float vy) {
 // mass_row_sums.wgsl:134-170, one thread per (j, k) row.  Grid [ny, nz].
 __global__ __launch_bounds__(256) void k_mass_row_sums(uint32_t* M, KArg a) {
   TE_TID3
-  Dims g = dims(M, a.p[0]);
+  Dims g = dims(a.q[0], 0);
   const int j = gx, k = gy;
   if (j >= g.ny || k >= g.nz) return;
-  const float *rho = fp(M, a.p[1]), *pi0 = fp(M, a.p[2]), *pp3d = fp(M, a.p[3]), *mh = fp(M, a.p[4]),
-              *hm = fp(M, a.p[30]);
-  float *rows1 = fp(M, a.p[5]), *rows2 = fp(M, a.p[6]);
+  const float *rho = fq(a.q[1]), *pi0 = fq(a.q[2]), *pp3d = fq(a.q[3]), *mh = fq(a.q[4]),
+              *hm = fq(a.q[30]);
+  float *rows1 = fq(a.q[5]), *rows2 = fq(a.q[6]);
   DFs s1{0.0f, 0.0f}, s2{0.0f, 0.0f};
   const float rvh = 1.0f / hm_vh(hm, g, j);
   for (int i = 0; i < g.nx; ++i) {
@@ -50,11 +50,11 @@ __global__ __launch_bounds__(256) void k_mass_row_sums(uint32_t* M, KArg a) {
 // mass_level_sums.wgsl:118-146, one thread per level.  Grid [nz].
 __global__ __launch_bounds__(256) void k_mass_level_sums(uint32_t* M, KArg a) {
   TE_TID3
-  Dims g = dims(M, a.p[0]);
+  Dims g = dims(a.q[0], 0);
   const int k = gx;
   if (k >= g.nz) return;
-  const float *rows1 = fp(M, a.p[1]), *rows2 = fp(M, a.p[2]);
-  float *lev1 = fp(M, a.p[3]), *lev2 = fp(M, a.p[4]);
+  const float *rows1 = fq(a.q[1]), *rows2 = fq(a.q[2]);
+  float *lev1 = fq(a.q[3]), *lev2 = fq(a.q[4]);
   DFs s1{0.0f, 0.0f}, s2{0.0f, 0.0f};
   for (int j = 0; j < g.ny; ++j) {
     const int64_t r = ((int64_t)k * g.sK + j) * 2;
@@ -69,9 +69,9 @@ __global__ __launch_bounds__(256) void k_mass_level_sums(uint32_t* M, KArg a) {
 __global__ __launch_bounds__(256) void k_mass_reference_init(uint32_t* M, KArg a) {
   TE_TID3
   if (gx != 0) return;
-  Dims g = dims(M, a.p[0]);
-  const float* lev1 = fp(M, a.p[1]);
-  float* ref = fp(M, a.p[2]);
+  Dims g = dims(a.q[0], 0);
+  const float* lev1 = fq(a.q[1]);
+  float* ref = fq(a.q[2]);
   if (ref[2] != 0.0f) return;
   DFs s{0.0f, 0.0f};
   for (int k = 0; k < g.nz; ++k) df_add_pair(s, lev1[k * 2], lev1[k * 2 + 1]);
@@ -90,13 +90,13 @@ TE_DEV float powm1(float r, float a) {
 // mass_correction_apply.wgsl:258-321.  Grid [nx, ny, nz].
 __global__ __launch_bounds__(256) void k_mass_correction_apply(uint32_t* M, KArg a) {
   TE_TID3
-  Dims g = dims(M, a.p[0]);
+  Dims g = dims(a.q[0], 0);
   const int i = gx, j = gy, k = gz;
   if (i >= g.nx || j >= g.ny || k >= g.nz) return;
-  const int moist = ldi(M, a.p[1]);
-  const float *lev1 = fp(M, a.p[2]), *lev2 = fp(M, a.p[3]), *ref = fp(M, a.p[4]), *thP = fp(M, a.p[5]),
-              *th0 = fp(M, a.p[7]), *pi0 = fp(M, a.p[8]), *qv = fp(M, a.p[9]);
-  float *pp3d = fp(M, a.p[6]), *prs = fp(M, a.p[10]), *rho = fp(M, a.p[11]), *out = fp(M, a.p[12]);
+  const int moist = ldi(a.q[1], 0);
+  const float *lev1 = fq(a.q[2]), *lev2 = fq(a.q[3]), *ref = fq(a.q[4]), *thP = fq(a.q[5]),
+              *th0 = fq(a.q[7]), *pi0 = fq(a.q[8]), *qv = fq(a.q[9]);
+  float *pp3d = fq(a.q[6]), *prs = fq(a.q[10]), *rho = fq(a.q[11]), *out = fq(a.q[12]);
   DFs m2{0.0f, 0.0f}, pp2{0.0f, 0.0f};
   for (int kk = 0; kk < g.nz; ++kk) {
     df_add_pair(m2, lev1[kk * 2], lev1[kk * 2 + 1]);

@@ -12,15 +12,15 @@ namespace te {
 // rayleigh_damping.wgsl:111-164
 __global__ __launch_bounds__(256) void k_rayleigh_damping(uint32_t* M, KArg a) {
   TE_TID3
-  Dims g = dims(M, a.p[0]);
+  Dims g = dims(a.q[0], 0);
   int i = gx, j = gy, k = gz;
   if (i > g.nx || j > g.ny || k > g.nz) return;
-  const float *u = fp(M, a.p[1]), *v = fp(M, a.p[2]), *w = fp(M, a.p[3]), *th = fp(M, a.p[4]), *u0 = fp(M, a.p[5]),
-              *v0 = fp(M, a.p[6]), *tauh = fp(M, a.p[7]), *tauf = fp(M, a.p[8]), *hdx = fp(M, a.p[14]),
-              *hdy = fp(M, a.p[15]), *avg = fp(M, a.p[18]), *th0 = fp(M, a.p[19]);
-  float *uten = fp(M, a.p[10]), *vten = fp(M, a.p[11]), *wten = fp(M, a.p[12]), *thten = fp(M, a.p[13]);
-  float rdalpha = ldf(M, a.p[9]);
-  int lateral = ldi(M, a.p[16]), toAverage = ldi(M, a.p[17]);
+  const float *u = fq(a.q[1]), *v = fq(a.q[2]), *w = fq(a.q[3]), *th = fq(a.q[4]), *u0 = fq(a.q[5]),
+              *v0 = fq(a.q[6]), *tauh = fq(a.q[7]), *tauf = fq(a.q[8]), *hdx = fq(a.q[14]),
+              *hdy = fq(a.q[15]), *avg = fq(a.q[18]), *th0 = fq(a.q[19]);
+  float *uten = fq(a.q[10]), *vten = fq(a.q[11]), *wten = fq(a.q[12]), *thten = fq(a.q[13]);
+  float rdalpha = ldf(a.q[9], 0);
+  int lateral = ldi(a.q[16], 0), toAverage = ldi(a.q[17], 0);
   const int64_t c = g.idx(i, j, k);
   if (k < g.nz) {
     float t = tauh[k];
@@ -43,21 +43,21 @@ __global__ __launch_bounds__(256) void k_rayleigh_damping(uint32_t* M, KArg a) {
 // turb_rho_faces.wgsl:111-151
 __global__ __launch_bounds__(256) void k_turb_rho_faces(uint32_t* M, KArg a) {
   TE_TID3
-  Dims g = dims(M, a.p[0]);
+  Dims g = dims(a.q[0], 0);
   int i = gx - 2, j = gy - 2, k = gz;
   if (i > g.nx + 1 || j > g.ny + 1 || k > g.nz) return;
-  const float *rho = fp(M, a.p[1]), *c1 = fp(M, a.p[2]), *c2 = fp(M, a.p[3]);
-  float *rr = fp(M, a.p[4]), *rf = fp(M, a.p[5]);
-  const uint32_t os = a.p[6], ot = a.p[7];
+  const float *rho = fq(a.q[1]), *c1 = fq(a.q[2]), *c2 = fq(a.q[3]);
+  float *rr = fq(a.q[4]), *rf = fq(a.q[5]);
+  const uint32_t *os = a.q[6], *ot = a.q[7];
   const int64_t sk = g.sk(), c = g.idx(i, j, k);
   if (k < g.nz) rr[c] = 1.0f / rho[c];
   if (k == 0) {
-    float r = fma_(ldf(M, os), rho[c], ldf(M, os + 1) * rho[c + sk]);
-    if (g.nz >= 3) r = fma_(ldf(M, os + 2), rho[c + 2 * sk], r);
+    float r = fma_(ldf(os, 0), rho[c], ldf(os, 1) * rho[c + sk]);
+    if (g.nz >= 3) r = fma_(ldf(os, 2), rho[c + 2 * sk], r);
     rf[c] = r;
   } else if (k == g.nz) {
-    float r = fma_(ldf(M, ot), rho[c - sk], ldf(M, ot + 1) * rho[c - 2 * sk]);
-    if (g.nz >= 3) r = fma_(ldf(M, ot + 2), rho[c - 3 * sk], r);
+    float r = fma_(ldf(ot, 0), rho[c - sk], ldf(ot, 1) * rho[c - 2 * sk]);
+    if (g.nz >= 3) r = fma_(ldf(ot, 2), rho[c - 3 * sk], r);
     rf[c] = r;
   } else {
     rf[c] = fma_(c1[k], rho[c - sk], c2[k] * rho[c]);
@@ -67,45 +67,45 @@ __global__ __launch_bounds__(256) void k_turb_rho_faces(uint32_t* M, KArg a) {
 // turb_def_centers.wgsl:139-168
 __global__ __launch_bounds__(256) void k_turb_def_centers(uint32_t* M, KArg a) {
   TE_TID3
-  Dims g = dims(M, a.p[0]);
+  Dims g = dims(a.q[0], 0);
   int i = gx - 1, j = gy - 1, k = gz;
   if (i > g.nx + 1 || j > g.ny + 1 || k >= g.nz) return;
-  const float *u = fp(M, a.p[1]), *v = fp(M, a.p[2]), *w = fp(M, a.p[3]), *rho = fp(M, a.p[4]), *mh = fp(M, a.p[5]),
-              *hm = fp(M, a.p[30]);
+  const float *u = fq(a.q[1]), *v = fq(a.q[2]), *w = fq(a.q[3]), *rho = fq(a.q[4]), *mh = fq(a.q[5]),
+              *hm = fq(a.q[30]);
   const int64_t sj = g.sJ, sk = g.sk(), c = g.idx(i, j, k);
-  fp(M, a.p[6])[c] = ((rho[c] * (u[c + 1] - u[c])) * g.rdx) * hm_uh(hm, g, i);
-  fp(M, a.p[7])[c] = ((rho[c] * (v[c + sj] - v[c])) * g.rdy) * hm_vh(hm, g, j);
-  fp(M, a.p[8])[c] = ((rho[c] * (w[c + sk] - w[c])) * g.rdz) * mh[k];
+  fq(a.q[6])[c] = ((rho[c] * (u[c + 1] - u[c])) * g.rdx) * hm_uh(hm, g, i);
+  fq(a.q[7])[c] = ((rho[c] * (v[c + sj] - v[c])) * g.rdy) * hm_vh(hm, g, j);
+  fq(a.q[8])[c] = ((rho[c] * (w[c + sk] - w[c])) * g.rdz) * mh[k];
   float rcorner = 0.25f * ((rho[c - 1 - sj] + rho[c]) + (rho[c - 1] + rho[c - sj]));
-  fp(M, a.p[9])[c] =
+  fq(a.q[9])[c] =
       (0.5f * fma_((u[c] - u[c - sj]) * g.rdy, hm_vf(hm, g, j), ((v[c] - v[c - 1]) * g.rdx) * hm_uf(hm, g, i))) * rcorner;
 }
 
 // turb_def_faces.wgsl:125-151
 __global__ __launch_bounds__(256) void k_turb_def_faces(uint32_t* M, KArg a) {
   TE_TID3
-  Dims g = dims(M, a.p[0]);
+  Dims g = dims(a.q[0], 0);
   int i = gx, j = gy, k = gz + 1;
   if (i > g.nx + 1 || j > g.ny + 1 || k > g.nz - 1) return;
-  const float *u = fp(M, a.p[1]), *v = fp(M, a.p[2]), *w = fp(M, a.p[3]), *rf = fp(M, a.p[4]), *mf = fp(M, a.p[5]),
-              *hm = fp(M, a.p[30]);
+  const float *u = fq(a.q[1]), *v = fq(a.q[2]), *w = fq(a.q[3]), *rf = fq(a.q[4]), *mf = fq(a.q[5]),
+              *hm = fq(a.q[30]);
   const int64_t sj = g.sJ, sk = g.sk(), c = g.idx(i, j, k);
   float mfu = 0.5f * (mf[k] + mf[k]);
   float mfv = 0.5f * (mf[k] + mf[k]);
-  fp(M, a.p[6])[c] =
+  fq(a.q[6])[c] =
       ((0.5f * fma_((w[c] - w[c - 1]) * g.rdx, hm_uf(hm, g, i), ((u[c] - u[c - sk]) * g.rdz) * mfu)) * 0.5f) * (rf[c - 1] + rf[c]);
-  fp(M, a.p[7])[c] =
+  fq(a.q[7])[c] =
       ((0.5f * fma_((w[c] - w[c - sj]) * g.rdy, hm_vf(hm, g, j), ((v[c] - v[c - sk]) * g.rdz) * mfv)) * 0.5f) * (rf[c - sj] + rf[c]);
 }
 
 // turb_def_assemble.wgsl:111-149
 __global__ __launch_bounds__(256) void k_turb_def_assemble(uint32_t* M, KArg a) {
   TE_TID3
-  Dims g = dims(M, a.p[0]);
+  Dims g = dims(a.q[0], 0);
   int i = gx, j = gy, k = gz + 1;
   if (i >= g.nx || j >= g.ny || k > g.nz - 1) return;
-  const float *s11 = fp(M, a.p[1]), *s22 = fp(M, a.p[2]), *s33 = fp(M, a.p[3]), *s12 = fp(M, a.p[4]),
-              *s13 = fp(M, a.p[5]), *s23 = fp(M, a.p[6]), *c1 = fp(M, a.p[7]), *c2 = fp(M, a.p[8]), *rf = fp(M, a.p[9]);
+  const float *s11 = fq(a.q[1]), *s22 = fq(a.q[2]), *s33 = fq(a.q[3]), *s12 = fq(a.q[4]),
+              *s13 = fq(a.q[5]), *s23 = fq(a.q[6]), *c1 = fq(a.q[7]), *c2 = fq(a.q[8]), *rf = fq(a.q[9]);
   const int64_t sj = g.sJ, sk = g.sk(), c = g.idx(i, j, k);
   float A = c1[k], B = c2[k];
   float t11 = fma_(A * s11[c - sk], s11[c - sk], (B * s11[c]) * s11[c]);
@@ -118,8 +118,8 @@ __global__ __launch_bounds__(256) void k_turb_def_assemble(uint32_t* M, KArg a) 
   float t13 = 0.5f * fma_(s13[c], s13[c], s13[c + 1] * s13[c + 1]);
   float t23 = 0.5f * fma_(s23[c], s23[c], s23[c + sj] * s23[c + sj]);
   float rrf = 1.0f / (rf[c] * rf[c]);
-  fp(M, a.p[10])[c] = (4.0f * (t13 + t23)) * rrf;
-  fp(M, a.p[11])[c] = fma_(2.0f, (t11 + t22) + t33, 4.0f * t12) * rrf;
+  fq(a.q[10])[c] = (4.0f * (t13 + t23)) * rrf;
+  fq(a.q[11])[c] = fma_(2.0f, (t11 + t22) + t33, 4.0f * t12) * rrf;
 }
 
 // saturation mixing ratios over liquid / ice (turb_nm.wgsl:204-230)
@@ -137,14 +137,14 @@ TE_DEV float mp_rsif(float p, float t) {
 // turb_nm.wgsl:236-355 (moist squared buoyancy frequency)
 __global__ __launch_bounds__(256) void k_turb_nm(uint32_t* M, KArg a) {
   TE_TID3
-  Dims g = dims(M, a.p[0]);
+  Dims g = dims(a.q[0], 0);
   int i = gx, j = gy, k = gz + 1;
   if (i >= g.nx || j >= g.ny || k > g.nz - 1) return;
-  const float *thP = fp(M, a.p[1]), *ppP = fp(M, a.p[2]), *th0 = fp(M, a.p[3]), *pi0 = fp(M, a.p[4]),
-              *prs = fp(M, a.p[5]), *qv = fp(M, a.p[6]), *qc = fp(M, a.p[7]), *qr = fp(M, a.p[8]), *qi = fp(M, a.p[9]),
-              *qs = fp(M, a.p[10]), *qg = fp(M, a.p[11]), *c1 = fp(M, a.p[12]), *c2 = fp(M, a.p[13]), *mf = fp(M, a.p[14]);
-  float *nm = fp(M, a.p[15]), *sat = fp(M, a.p[17]);
-  int moist = ldi(M, a.p[16]), wantSat = ldi(M, a.p[18]);
+  const float *thP = fq(a.q[1]), *ppP = fq(a.q[2]), *th0 = fq(a.q[3]), *pi0 = fq(a.q[4]),
+              *prs = fq(a.q[5]), *qv = fq(a.q[6]), *qc = fq(a.q[7]), *qr = fq(a.q[8]), *qi = fq(a.q[9]),
+              *qs = fq(a.q[10]), *qg = fq(a.q[11]), *c1 = fq(a.q[12]), *c2 = fq(a.q[13]), *mf = fq(a.q[14]);
+  float *nm = fq(a.q[15]), *sat = fq(a.q[17]);
+  int moist = ldi(a.q[16], 0), wantSat = ldi(a.q[18], 0);
   const int64_t sk = g.sk(), c = g.idx(i, j, k);
   if (wantSat != 0) sat[c] = 0.0f;
   const float nmsat = 1e-10f, eps = RD / RV, reps = RV / RD, gAcc = 9.81f;
@@ -223,13 +223,13 @@ __global__ __launch_bounds__(256) void k_turb_nm(uint32_t* M, KArg a) {
 // turb_km_kh.wgsl:125-187
 __global__ __launch_bounds__(256) void k_turb_km_kh(uint32_t* M, KArg a) {
   TE_TID3
-  Dims g = dims(M, a.p[0]);
+  Dims g = dims(a.q[0], 0);
   int i = gx, j = gy, k = gz + 1;
   if (i >= g.nx || j >= g.ny || k > g.nz - 1) return;
-  const float *tkea = fp(M, a.p[1]), *nm = fp(M, a.p[2]), *mf = fp(M, a.p[3]), *hm = fp(M, a.p[30]);
-  float *kmh = fp(M, a.p[4]), *kmv = fp(M, a.p[5]), *khh = fp(M, a.p[6]), *khv = fp(M, a.p[7]), *lenscl = fp(M, a.p[8]),
-        *dissten = fp(M, a.p[9]);
-  float kmScale = ldf(M, a.p[10]), kmFixed = ldf(M, a.p[11]), turbfac = ldf(M, a.p[12]);
+  const float *tkea = fq(a.q[1]), *nm = fq(a.q[2]), *mf = fq(a.q[3]), *hm = fq(a.q[30]);
+  float *kmh = fq(a.q[4]), *kmv = fq(a.q[5]), *khh = fq(a.q[6]), *khv = fq(a.q[7]), *lenscl = fq(a.q[8]),
+        *dissten = fq(a.q[9]);
+  float kmScale = ldf(a.q[10], 0), kmFixed = ldf(a.q[11], 0), turbfac = ldf(a.q[12], 0);
   const int64_t c = g.idx(i, j, k);
   const float c_m = 0.1f, c_l = 0.8165f, ri_c = 0.25f, tke_min = 1e-10f, nm_min = 1e-06f, small_len_frac = 0.001f;
   float cmz = mx(0.001f, turbfac * c_m);
@@ -265,16 +265,16 @@ __global__ __launch_bounds__(256) void k_turb_km_kh(uint32_t* M, KArg a) {
 // turb_dissipative_heating.wgsl:139-194
 __global__ __launch_bounds__(256) void k_turb_dissipative_heating(uint32_t* M, KArg a) {
   TE_TID3
-  Dims g = dims(M, a.p[0]);
+  Dims g = dims(a.q[0], 0);
   int i = gx, j = gy, k = gz;
   if (i >= g.nx || j >= g.ny || k >= g.nz) return;
-  const float *dissten = fp(M, a.p[1]), *pi0 = fp(M, a.p[2]), *ppi = fp(M, a.p[3]), *qv = fp(M, a.p[4]),
-              *qc = fp(M, a.p[5]), *qr = fp(M, a.p[6]), *qi = fp(M, a.p[7]), *qs = fp(M, a.p[8]), *qg = fp(M, a.p[9]),
-              *sfc = fp(M, a.p[12]);
-  float* thten = fp(M, a.p[10]);
-  const uint32_t op = a.p[11];
-  int moist = ldi(M, op), bbc3 = ldi(M, op + 1), zntField = ldi(M, op + 4);
-  float znt = ldf(M, op + 2), dz1 = ldf(M, op + 3);
+  const float *dissten = fq(a.q[1]), *pi0 = fq(a.q[2]), *ppi = fq(a.q[3]), *qv = fq(a.q[4]),
+              *qc = fq(a.q[5]), *qr = fq(a.q[6]), *qi = fq(a.q[7]), *qs = fq(a.q[8]), *qg = fq(a.q[9]),
+              *sfc = fq(a.q[12]);
+  float* thten = fq(a.q[10]);
+  const uint32_t* op = a.q[11];
+  int moist = ldi(op, 0), bbc3 = ldi(op, 1), zntField = ldi(op, 4);
+  float znt = ldf(op, 2), dz1 = ldf(op, 3);
   const int64_t sk = g.sk(), c = g.idx(i, j, k);
   const float cv = CP - RD, cvv = CPV - RV, cpdcv = CP / (CP - RD);
   float epsd;
@@ -299,12 +299,12 @@ __global__ __launch_bounds__(256) void k_turb_dissipative_heating(uint32_t* M, K
 // turb_gettau.wgsl:111-143
 __global__ __launch_bounds__(256) void k_turb_gettau(uint32_t* M, KArg a) {
   TE_TID3
-  Dims g = dims(M, a.p[0]);
+  Dims g = dims(a.q[0], 0);
   int i = gx - 1, j = gy - 1, k = gz;
   if (i > g.nx || j > g.ny || k >= g.nz) return;
-  const float *kmh = fp(M, a.p[1]), *kmv = fp(M, a.p[2]);
-  float *t11 = fp(M, a.p[3]), *t22 = fp(M, a.p[4]), *t33 = fp(M, a.p[5]), *t12 = fp(M, a.p[6]), *t13 = fp(M, a.p[7]),
-        *t23 = fp(M, a.p[8]);
+  const float *kmh = fq(a.q[1]), *kmv = fq(a.q[2]);
+  float *t11 = fq(a.q[3]), *t22 = fq(a.q[4]), *t33 = fq(a.q[5]), *t12 = fq(a.q[6]), *t13 = fq(a.q[7]),
+        *t23 = fq(a.q[8]);
   const int64_t sj = g.sJ, sk = g.sk(), c = g.idx(i, j, k);
   float tem = kmh[c] + kmh[c + sk];
   t11[c] = t11[c] * tem;
@@ -325,12 +325,12 @@ __global__ __launch_bounds__(256) void k_turb_sfcstress(uint32_t* M, KArg a) {
   const Blk b_ = te_block(a);
   const int i = b_.x * TE_BX + (int)threadIdx.x, j = b_.y * TE_BY + (int)threadIdx.y;
   if (i >= (int)a.n[0] || j >= (int)a.n[1]) return;
-  Dims g = dims(M, a.p[0]);
+  Dims g = dims(a.q[0], 0);
   if (i > g.nx || j > g.ny) return;
-  const float *rf = fp(M, a.p[2]), *u = fp(M, a.p[6]), *v = fp(M, a.p[7]), *kmv = fp(M, a.p[9]), *mf = fp(M, a.p[11]);
-  float *t13 = fp(M, a.p[3]), *t23 = fp(M, a.p[4]);
-  int mode = ldi(M, a.p[5]), topNoSlip = ldi(M, a.p[10]);
-  float movex = ldf(M, a.p[8]), movey = ldf(M, a.p[8] + 1);
+  const float *rf = fq(a.q[2]), *u = fq(a.q[6]), *v = fq(a.q[7]), *kmv = fq(a.q[9]), *mf = fq(a.q[11]);
+  float *t13 = fq(a.q[3]), *t23 = fq(a.q[4]);
+  int mode = ldi(a.q[5], 0), topNoSlip = ldi(a.q[10], 0);
+  float movex = ldf(a.q[8], 0), movey = ldf(a.q[8], 1);
   const int64_t plane = g.sk(), sjj = g.sJ;
   {
     const int64_t ct = g.idx(i, j, g.nz);
@@ -358,12 +358,12 @@ __global__ __launch_bounds__(256) void k_turb_sfcstress(uint32_t* M, KArg a) {
 // tke_epst.wgsl:170-206 (shear production)
 __global__ __launch_bounds__(256) void k_tke_epst(uint32_t* M, KArg a) {
   TE_TID3
-  Dims g = dims(M, a.p[0]);
+  Dims g = dims(a.q[0], 0);
   int i = gx, j = gy, k = gz + 1;
   if (i >= g.nx || j >= g.ny || k > g.nz - 1) return;
-  const float *u = fp(M, a.p[1]), *v = fp(M, a.p[2]), *w = fp(M, a.p[3]), *t11 = fp(M, a.p[4]), *t12 = fp(M, a.p[5]),
-              *t13 = fp(M, a.p[6]), *t22 = fp(M, a.p[7]), *t23 = fp(M, a.p[8]), *t33 = fp(M, a.p[9]), *rf = fp(M, a.p[10]),
-              *mh = fp(M, a.p[11]), *mf = fp(M, a.p[12]), *c1 = fp(M, a.p[13]), *c2 = fp(M, a.p[14]), *hm = fp(M, a.p[30]);
+  const float *u = fq(a.q[1]), *v = fq(a.q[2]), *w = fq(a.q[3]), *t11 = fq(a.q[4]), *t12 = fq(a.q[5]),
+              *t13 = fq(a.q[6]), *t22 = fq(a.q[7]), *t23 = fq(a.q[8]), *t33 = fq(a.q[9]), *rf = fq(a.q[10]),
+              *mh = fq(a.q[11]), *mf = fq(a.q[12]), *c1 = fq(a.q[13]), *c2 = fq(a.q[14]), *hm = fq(a.q[30]);
   const int64_t sj = g.sJ, sk = g.sk(), c = g.idx(i, j, k), cd = c - sk;
   auto L1 = [&](int64_t cc) {
     return fma_(((u[cc + 1] - u[cc]) * g.rdx) * hm_uh(hm, g, i), t11[cc], (((v[cc + sj] - v[cc]) * g.rdy) * hm_vh(hm, g, j)) * t22[cc]);
@@ -384,31 +384,31 @@ __global__ __launch_bounds__(256) void k_tke_epst(uint32_t* M, KArg a) {
   e = fma_(0.25f, fma_(c2[k], up, c1[k] * dn), e);
   e = fma_(0.5f, (L3(c, i) + L3(c + 1, i + 1)) + (L4(c, j) + L4(c + sj, j + 1)), e);
   e = e + fma_(c1[k], L5(cd, k - 1), c2[k] * L5(c, k));
-  fp(M, a.p[15])[c] = e / rf[c];
+  fq(a.q[15])[c] = e / rf[c];
 }
 
 // tke_sources.wgsl:111-129
 __global__ __launch_bounds__(256) void k_tke_sources(uint32_t* M, KArg a) {
   TE_TID3
-  Dims g = dims(M, a.p[0]);
+  Dims g = dims(a.q[0], 0);
   int i = gx, j = gy, k = gz + 1;
   if (i >= g.nx || j >= g.ny || k > g.nz - 1) return;
   const int64_t c = g.idx(i, j, k);
-  const float *khv = fp(M, a.p[1]), *nm = fp(M, a.p[2]), *dissten = fp(M, a.p[3]), *epst = fp(M, a.p[4]);
-  fp(M, a.p[5])[c] = fma_(-khv[c], nm[c], -dissten[c]) + epst[c];
+  const float *khv = fq(a.q[1]), *nm = fq(a.q[2]), *dissten = fq(a.q[3]), *epst = fq(a.q[4]);
+  fq(a.q[5])[c] = fma_(-khv[c], nm[c], -dissten[c]) + epst[c];
 }
 
 // turb_u_tendency.wgsl:146-294 / turb_v_tendency.wgsl
 template <int DIR, bool LDS>
 __device__ void uv_tendency(uint32_t* M, const KArg& a, int i, int j) {
-  Dims g = dims(M, a.p[0]);
+  Dims g = dims(a.q[0], 0);
   if (DIR == 0 ? (i > g.nx || j >= g.ny) : (i >= g.nx || j > g.ny)) return;
-  const float *q = fp(M, a.p[1]), *w = fp(M, a.p[2]), *rho = fp(M, a.p[3]), *rf = fp(M, a.p[4]), *kmv = fp(M, a.p[5]),
-              *tA = fp(M, a.p[6]), *tB = fp(M, a.p[7]), *tC = fp(M, a.p[8]), *mh = fp(M, a.p[9]), *mf = fp(M, a.p[10]),
-              *hm = fp(M, a.p[30]);
-  float dt = ldf(M, a.p[11]);
-  float *cA = fp(M, a.p[12]), *cB = fp(M, a.p[13]), *dwd = fp(M, a.p[14]), *ten = fp(M, a.p[15]);
-  int openx = ldi(M, a.p[16]), openy = ldi(M, a.p[16] + 1), explicitZ = ldi(M, a.p[17]);
+  const float *q = fq(a.q[1]), *w = fq(a.q[2]), *rho = fq(a.q[3]), *rf = fq(a.q[4]), *kmv = fq(a.q[5]),
+              *tA = fq(a.q[6]), *tB = fq(a.q[7]), *tC = fq(a.q[8]), *mh = fq(a.q[9]), *mf = fq(a.q[10]),
+              *hm = fq(a.q[30]);
+  float dt = ldf(a.q[11], 0);
+  float *cA = fq(a.q[12]), *cB = fq(a.q[13]), *dwd = fq(a.q[14]), *ten = fq(a.q[15]);
+  int openx = ldi(a.q[16], 0), openy = ldi(a.q[16], 1), explicitZ = ldi(a.q[17], 0);
   const int64_t sj = g.sJ, sk = g.sk(), st = DIR == 0 ? 1 : sj, base = g.idx(i, j, 0);
   bool zeroX = DIR == 0 ? (openx != 0 && (i == 0 || i == g.nx)) : (openx != 0 && (i == 0 || i == g.nx - 1));
   bool zeroY = DIR == 0 ? (openy != 0 && (j == 0 || j == g.ny - 1)) : (openy != 0 && (j == 0 || j == g.ny));
@@ -710,13 +710,13 @@ __device__ void w_tendency(uint32_t* M, const KArg& a) {
   const Blk b_ = te_block(a);
   const int i = b_.x * TE_BX + (int)threadIdx.x, j = b_.y * TE_BY + (int)threadIdx.y;
   if (i >= (int)a.n[0] || j >= (int)a.n[1]) return;
-  Dims g = dims(M, a.p[0]);
+  Dims g = dims(a.q[0], 0);
   if (i >= g.nx || j >= g.ny) return;
-  const float *w = fp(M, a.p[1]), *rho = fp(M, a.p[2]), *rf = fp(M, a.p[3]), *kmh = fp(M, a.p[4]), *t13 = fp(M, a.p[5]),
-              *t23 = fp(M, a.p[6]), *t33 = fp(M, a.p[7]), *mh = fp(M, a.p[8]), *mf = fp(M, a.p[9]), *hm = fp(M, a.p[30]);
-  float dt = ldf(M, a.p[10]);
-  float *cA = fp(M, a.p[11]), *cB = fp(M, a.p[12]), *wten = fp(M, a.p[13]);
-  int openx = ldi(M, a.p[14]), openy = ldi(M, a.p[14] + 1), explicitZ = ldi(M, a.p[15]);
+  const float *w = fq(a.q[1]), *rho = fq(a.q[2]), *rf = fq(a.q[3]), *kmh = fq(a.q[4]), *t13 = fq(a.q[5]),
+              *t23 = fq(a.q[6]), *t33 = fq(a.q[7]), *mh = fq(a.q[8]), *mf = fq(a.q[9]), *hm = fq(a.q[30]);
+  float dt = ldf(a.q[10], 0);
+  float *cA = fq(a.q[11]), *cB = fq(a.q[12]), *wten = fq(a.q[13]);
+  int openx = ldi(a.q[14], 0), openy = ldi(a.q[14], 1), explicitZ = ldi(a.q[15], 0);
   const int64_t sj = g.sJ, sk = g.sk(), base = g.idx(i, j, 0);
   bool zeroX = openx != 0 && (i == 0 || i == g.nx - 1);
   bool zeroY = openy != 0 && (j == 0 || j == g.ny - 1);
@@ -751,15 +751,15 @@ __global__ __launch_bounds__(256) void k_turb_w_tendency_lds(uint32_t* M, KArg a
 // turb_scalar_flux_h.wgsl:125-148
 __global__ __launch_bounds__(256) void k_turb_scalar_flux_h(uint32_t* M, KArg a) {
   TE_TID3
-  Dims g = dims(M, a.p[0]);
+  Dims g = dims(a.q[0], 0);
   int i = gx, j = gy, k = gz;
   if (i > g.nx || j > g.ny || k >= g.nz) return;
-  const float *s = fp(M, a.p[1]), *rho = fp(M, a.p[2]), *khh = fp(M, a.p[3]), *hm = fp(M, a.p[30]);
-  float leNum = ldf(M, a.p[4]);
+  const float *s = fq(a.q[1]), *rho = fq(a.q[2]), *khh = fq(a.q[3]), *hm = fq(a.q[30]);
+  float leNum = ldf(a.q[4], 0);
   const int64_t sj = g.sJ, sk = g.sk(), c = g.idx(i, j, k);
-  fp(M, a.p[5])[c] = (((((-0.125f * (rho[c] + rho[c - 1])) * leNum) * ((khh[c] + khh[c - 1]) + (khh[c + sk] + khh[c - 1 + sk]))) *
+  fq(a.q[5])[c] = (((((-0.125f * (rho[c] + rho[c - 1])) * leNum) * ((khh[c] + khh[c - 1]) + (khh[c + sk] + khh[c - 1 + sk]))) *
                        (s[c] - s[c - 1])) * g.rdx) * hm_uf(hm, g, i);
-  fp(M, a.p[6])[c] = (((((-0.125f * (rho[c] + rho[c - sj])) * leNum) * ((khh[c] + khh[c - sj]) + (khh[c + sk] + khh[c - sj + sk]))) *
+  fq(a.q[6])[c] = (((((-0.125f * (rho[c] + rho[c - sj])) * leNum) * ((khh[c] + khh[c - sj]) + (khh[c + sk] + khh[c - sj + sk]))) *
                        (s[c] - s[c - sj])) * g.rdy) * hm_vf(hm, g, j);
 }
 
@@ -769,16 +769,16 @@ __device__ void scalar_vert_implicit(uint32_t* M, const KArg& a) {
   const Blk b_ = te_block(a);
   const int i = b_.x * TE_BX + (int)threadIdx.x, j = b_.y * TE_BY + (int)threadIdx.y;
   if (i >= (int)a.n[0] || j >= (int)a.n[1]) return;
-  Dims g = dims(M, a.p[0]);
+  Dims g = dims(a.q[0], 0);
   if (i >= g.nx || j >= g.ny) return;
-  const float *s = fp(M, a.p[1]), *rho = fp(M, a.p[2]), *rr = fp(M, a.p[3]), *rf = fp(M, a.p[4]), *khv = fp(M, a.p[5]),
-              *mh = fp(M, a.p[6]), *mf = fp(M, a.p[7]), *fx = fp(M, a.p[8]), *fy = fp(M, a.p[9]), *sflux = fp(M, a.p[14]),
-              *hm = fp(M, a.p[30]);
-  float dt = ldf(M, a.p[10]);
-  float *cA = fp(M, a.p[11]), *cB = fp(M, a.p[12]), *sten = fp(M, a.p[13]);
-  int useFlux = ldi(M, a.p[15]), explicitZ = ldi(M, a.p[16]), bcturbs = ldi(M, a.p[18]);
-  const uint32_t ow = a.p[17];
-  float wall0 = ldf(M, ow), wall1 = ldf(M, ow + 1), wall2 = ldf(M, ow + 2), wall3 = ldf(M, ow + 3);
+  const float *s = fq(a.q[1]), *rho = fq(a.q[2]), *rr = fq(a.q[3]), *rf = fq(a.q[4]), *khv = fq(a.q[5]),
+              *mh = fq(a.q[6]), *mf = fq(a.q[7]), *fx = fq(a.q[8]), *fy = fq(a.q[9]), *sflux = fq(a.q[14]),
+              *hm = fq(a.q[30]);
+  float dt = ldf(a.q[10], 0);
+  float *cA = fq(a.q[11]), *cB = fq(a.q[12]), *sten = fq(a.q[13]);
+  int useFlux = ldi(a.q[15], 0), explicitZ = ldi(a.q[16], 0), bcturbs = ldi(a.q[18], 0);
+  const uint32_t* ow = a.q[17];
+  float wall0 = ldf(ow, 0), wall1 = ldf(ow, 1), wall2 = ldf(ow, 2), wall3 = ldf(ow, 3);
   const int64_t sj = g.sJ, sk = g.sk(), base = g.idx(i, j, 0);
   auto L2 = [&](int k) {
     const int64_t cc = base + k * sk;
@@ -915,14 +915,14 @@ __global__ __launch_bounds__(256) void k_turb_scalar_vert_implicit_lds(uint32_t*
 // tke_flux_h.wgsl:125-146
 __global__ __launch_bounds__(256) void k_tke_flux_h(uint32_t* M, KArg a) {
   TE_TID3
-  Dims g = dims(M, a.p[0]);
+  Dims g = dims(a.q[0], 0);
   int i = gx, j = gy, k = gz + 1;
   if (i > g.nx || j > g.ny || k > g.nz - 1) return;
-  const float *t = fp(M, a.p[1]), *rf = fp(M, a.p[2]), *kmh = fp(M, a.p[3]), *hm = fp(M, a.p[30]);
+  const float *t = fq(a.q[1]), *rf = fq(a.q[2]), *kmh = fq(a.q[3]), *hm = fq(a.q[30]);
   const int64_t sj = g.sJ, c = g.idx(i, j, k);
-  fp(M, a.p[4])[c] =
+  fq(a.q[4])[c] =
       (((((-0.25f * (rf[c] + rf[c - 1])) * 2.0f) * (kmh[c] + kmh[c - 1])) * (t[c] - t[c - 1])) * g.rdx) * hm_uf(hm, g, i);
-  fp(M, a.p[5])[c] =
+  fq(a.q[5])[c] =
       (((((-0.25f * (rf[c] + rf[c - sj])) * 2.0f) * (kmh[c] + kmh[c - sj])) * (t[c] - t[c - sj])) * g.rdy) * hm_vf(hm, g, j);
 }
 
@@ -932,13 +932,13 @@ __device__ void tke_transport(uint32_t* M, const KArg& a) {
   const Blk b_ = te_block(a);
   const int i = b_.x * TE_BX + (int)threadIdx.x, j = b_.y * TE_BY + (int)threadIdx.y;
   if (i >= (int)a.n[0] || j >= (int)a.n[1]) return;
-  Dims g = dims(M, a.p[0]);
+  Dims g = dims(a.q[0], 0);
   if (i >= g.nx || j >= g.ny) return;
-  const float *t = fp(M, a.p[1]), *rho = fp(M, a.p[2]), *rf = fp(M, a.p[3]), *kmh = fp(M, a.p[4]), *kmv = fp(M, a.p[5]),
-              *mh = fp(M, a.p[6]), *mf = fp(M, a.p[7]), *fx = fp(M, a.p[11]), *fy = fp(M, a.p[12]), *hm = fp(M, a.p[30]);
-  float dt = ldf(M, a.p[8]);
-  float *cA = fp(M, a.p[9]), *cB = fp(M, a.p[10]), *tketen = fp(M, a.p[13]);
-  int openx = ldi(M, a.p[14]), openy = ldi(M, a.p[14] + 1), explicitZ = ldi(M, a.p[15]);
+  const float *t = fq(a.q[1]), *rho = fq(a.q[2]), *rf = fq(a.q[3]), *kmh = fq(a.q[4]), *kmv = fq(a.q[5]),
+              *mh = fq(a.q[6]), *mf = fq(a.q[7]), *fx = fq(a.q[11]), *fy = fq(a.q[12]), *hm = fq(a.q[30]);
+  float dt = ldf(a.q[8], 0);
+  float *cA = fq(a.q[9]), *cB = fq(a.q[10]), *tketen = fq(a.q[13]);
+  int openx = ldi(a.q[14], 0), openy = ldi(a.q[14], 1), explicitZ = ldi(a.q[15], 0);
   const int64_t sj = g.sJ, sk = g.sk(), base = g.idx(i, j, 0);
   bool zeroX = openx != 0 && (i == 0 || i == g.nx - 1);
   bool zeroY = openy != 0 && (j == 0 || j == g.ny - 1);

@@ -25,6 +25,9 @@ struct KArg {
   // and in y (4; floor(l / d) = (l * m) >> 40 exactly for l < 2^24 and
   // d <= 2^16: Granlund & Montgomery, PLDI 1994, thm. 4.2), grid blocks (1)
   uint32_t lg[10];
+  // the parameters as device pointers (the memory's base + p[n] words),
+  // filled by the runtime so that kernels do no per-wave address arithmetic
+  uint32_t* q[48];
 };
 
 #define TE_DEV __device__ __forceinline__
@@ -32,6 +35,7 @@ struct KArg {
 TE_DEV float ldf(const uint32_t* M, uint32_t o) { return __uint_as_float(M[o]); }
 TE_DEV int32_t ldi(const uint32_t* M, uint32_t o) { return (int32_t)M[o]; }
 TE_DEV float* fp(uint32_t* M, uint32_t o) { return reinterpret_cast<float*>(M + o); }
+TE_DEV float* fq(uint32_t* p) { return reinterpret_cast<float*>(p); }
 
 struct Dims {
   int nx, ny, nz, sJ, sK;

@@ -105,8 +105,11 @@ static uint32_t column_lanes() {
 
 static hipError_t launch(te_ctx* c, const Op& op) {
   KArg a = karg_of(op);
+  for (int q = 0; q < 48; ++q) a.q[q] = c->dM + a.p[q];
   if (a.n[0] == 0 || a.n[1] == 0 || a.n[2] == 0) return hipSuccess;
   dim3 b = block_for(a.n);
+  static const int kAdvFused = kernel_id("adv_fused");
+  if (op.kid == kAdvFused) b = dim3(64, 4, 1);  // one row of cells per wave (k_adv_fused)
   size_t lds = 0;
   if (a.n[2] == 1 && is_column_kernel(op.kid)) {
     b = dim3(column_lanes(), 1, 1);
