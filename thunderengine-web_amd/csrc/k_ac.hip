@@ -96,7 +96,9 @@ __global__ __launch_bounds__(256) void k_acoustic_fpk(uint32_t* M, KArg a) {
 // in flight instead of waiting on each level.  Sums of level k-1 (pk1+qk,
 // pk2-qk, pp3d, pp3d+fpk) carry in registers: they are the same operations on
 // the same values as the reference's re-evaluation at level k.
+#ifndef WS_CHUNK
 #define WS_CHUNK 8
+#endif
 template <bool LDS>
 __device__ void w_solve(uint32_t* M, const KArg& a) {
   const Blk b_ = te_block(a);

@@ -130,6 +130,7 @@ __global__ __launch_bounds__(256) void k_turb_openbc_stress_y(uint32_t* M, KArg 
 
 // copy_words (replay.js:403-406): end-of-step state swap copies.
 __global__ __launch_bounds__(256) void k__copy(uint32_t* M, KArg a) {
+  if (te_flat(a, [&](uint32_t w, int width) { st4(a.q[1], w, width, ld4(a.q[0], w, width)); })) return;
   const uint32_t q = (uint32_t)(te_block(a).x * TE_BX) + threadIdx.x;
   if (q >= a.n[0]) return;
   a.q[1][q] = a.q[0][q];

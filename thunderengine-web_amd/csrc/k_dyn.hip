@@ -69,6 +69,11 @@ __global__ __launch_bounds__(256) void k_stage_tendency_reset(uint32_t* M, KArg 
 
 // stage_state_reset.wgsl:110-138
 __global__ __launch_bounds__(256) void k_stage_state_reset(uint32_t* M, KArg a) {
+  if (te_flat(a, [&](uint32_t w, int width) {
+        const int src[5] = {1, 2, 3, 4, 10};
+        for (int q = 0; q < 5; ++q) st4(a.q[5 + q], w, width, ld4(a.q[src[q]], w, width));
+      }))
+    return;
   TE_TID3
   Dims g = dims(a.q[0], 0);
   int i = gx - 3, j = gy - 3, k = gz;

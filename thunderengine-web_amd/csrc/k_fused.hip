@@ -47,6 +47,16 @@ __global__ __launch_bounds__(256) void k_bc_periodic_xy(uint32_t* M, KArg a) {
 // launch over the whole grid [strideJ, strideK, nz+1].  p[1..8] fields,
 // p[9] -> field count.
 __global__ __launch_bounds__(256) void k_zero_fields(uint32_t* M, KArg a) {
+  if (te_flat(a, [&](uint32_t w, int width) {
+        const int n = ldi(a.q[9], 0);
+        F4 z;
+        z.v[0] = z.v[1] = z.v[2] = z.v[3] = 0.0f;
+        for (int q = 0; q < 8; ++q) {
+          if (q >= n) break;
+          st4(a.q[1 + q], w, width, z);
+        }
+      }))
+    return;
   TE_TID3
   Dims g = dims(a.q[0], 0);
   if (gx >= g.sJ || gy >= g.sK || gz > g.nz) return;
@@ -62,6 +72,18 @@ __global__ __launch_bounds__(256) void k_zero_fields(uint32_t* M, KArg a) {
 // to 4 independent triples: p[1..4] sources, p[5..8] destinations,
 // p[9..12] -> scales, p[13] -> count.
 __global__ __launch_bounds__(256) void k_add_fields(uint32_t* M, KArg a) {
+  if (te_flat(a, [&](uint32_t w, int width) {
+        const int n = ldi(a.q[13], 0);
+        for (int q = 0; q < 4; ++q) {
+          if (q >= n) break;
+          const float sc = ldf(a.q[9 + q], 0);
+          const F4 s = ld4(a.q[1 + q], w, width);
+          F4 d = ld4(a.q[5 + q], w, width);
+          for (int e = 0; e < width; ++e) d.v[e] = fma_(sc, s.v[e], d.v[e]);
+          st4(a.q[5 + q], w, width, d);
+        }
+      }))
+    return;
   TE_TID3
   Dims g = dims(a.q[0], 0);
   int i = gx - 3, j = gy - 3, k = gz;
@@ -80,6 +102,15 @@ __global__ __launch_bounds__(256) void k_add_fields(uint32_t* M, KArg a) {
 // dst = fma(scale, src, 0), the bits of adding src*scale to the cleared field.
 // p: 1-3 uten vten wten, 4-6 their sources, 7 -> scale.  Grid: whole.
 __global__ __launch_bounds__(256) void k_tendency_reset_add(uint32_t* M, KArg a) {
+  if (te_flat(a, [&](uint32_t w, int width) {
+        const float sc = ldf(a.q[7], 0);
+        for (int q = 0; q < 3; ++q) {
+          F4 d = ld4(a.q[4 + q], w, width);
+          for (int e = 0; e < width; ++e) d.v[e] = fma_(sc, d.v[e], 0.0f);
+          st4(a.q[1 + q], w, width, d);
+        }
+      }))
+    return;
   TE_TID3
   Dims g = dims(a.q[0], 0);
   int i = gx - 3, j = gy - 3, k = gz;
@@ -92,6 +123,14 @@ __global__ __launch_bounds__(256) void k_tendency_reset_add(uint32_t* M, KArg a)
 // Up to 4 buffer copies of a.n[0] words each (the state swap of
 // timestep.js:766-774): p[1..4] sources, p[5..8] destinations, p[9] -> count.
 __global__ __launch_bounds__(256) void k_copy_fields(uint32_t* M, KArg a) {
+  if (te_flat(a, [&](uint32_t w, int width) {
+        const int n = ldi(a.q[9], 0);
+        for (int f = 0; f < 4; ++f) {
+          if (f >= n) break;
+          st4(a.q[5 + f], w, width, ld4(a.q[1 + f], w, width));
+        }
+      }))
+    return;
   const uint32_t q = (uint32_t)(te_block(a).x * TE_BX) + threadIdx.x;
   if (q >= a.n[0]) return;
   const int n = ldi(a.q[9], 0);

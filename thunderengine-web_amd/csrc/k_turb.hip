@@ -5,7 +5,9 @@
 #include "te_kernels.hpp"
 
 // Column sweeps load TS_CHUNK levels ahead of their serial arithmetic.
+#ifndef TS_CHUNK
 #define TS_CHUNK 8
+#endif
 
 namespace te {
 

@@ -23,8 +23,9 @@ struct KArg {
   // launch shape, filled by the runtime: block dims (3), grid blocks in x and
   // y (2), m = ceil(2^40 / d) as (low, high) words for d = grid blocks in x
   // and in y (4; floor(l / d) = (l * m) >> 40 exactly for l < 2^24 and
-  // d <= 2^16: Granlund & Montgomery, PLDI 1994, thm. 4.2), grid blocks (1)
-  uint32_t lg[10];
+  // d <= 2^16: Granlund & Montgomery, PLDI 1994, thm. 4.2), grid blocks (1),
+  // flat launch (1: see te_flat)
+  uint32_t lg[11];
   // the parameters as device pointers (the memory's base + p[n] words),
   // filled by the runtime so that kernels do no per-wave address arithmetic
   uint32_t* q[48];
@@ -272,6 +273,44 @@ __device__ __forceinline__ Blk te_block(const KArg& a) {
 #define TE_BX ((int)a.lg[0])
 #define TE_BY ((int)a.lg[1])
 #define TE_BZ ((int)a.lg[2])
+
+// Flat launches.  A pointwise op over a whole-field grid [strideJ, strideK,
+// nz+1] (or over a word range) touches every word of its fields once, in
+// memory order, with no dependence on the position: the runtime then
+// launches it over the n0*n1*n2 words in runs of 4 per thread (16-byte
+// vector accesses; it checks every field is 16-byte aligned).  te_flat calls
+// op(word, width) for this thread's run: width 4 when whole, else 1 per word
+// of the tail, and reports whether the launch was flat.
+template <class F>
+TE_DEV bool te_flat(const KArg& a, F op) {
+  if (a.lg[10] == 0) return false;
+  const uint32_t W = a.n[0] * a.n[1] * a.n[2];
+  const uint32_t f = 4u * ((uint32_t)te_block(a).x * (uint32_t)a.lg[0] + threadIdx.x);
+  if (f + 4 <= W) {
+    op(f, 4);
+  } else {
+    for (uint32_t w = f; w < W; ++w) op(w, 1);
+  }
+  return true;
+}
+// 4 or 1 floats at word w of p
+struct F4 {
+  float v[4];
+};
+TE_DEV F4 ld4(const uint32_t* p, uint32_t w, int width) {
+  F4 r;
+  if (width == 4) {
+    const float4 t = *reinterpret_cast<const float4*>(p + w);
+    r.v[0] = t.x; r.v[1] = t.y; r.v[2] = t.z; r.v[3] = t.w;
+  } else {
+    r.v[0] = __uint_as_float(p[w]);
+  }
+  return r;
+}
+TE_DEV void st4(uint32_t* p, uint32_t w, int width, const F4& r) {
+  if (width == 4) *reinterpret_cast<float4*>(p + w) = make_float4(r.v[0], r.v[1], r.v[2], r.v[3]);
+  else p[w] = __float_as_uint(r.v[0]);
+}
 
 #define TE_TID3                                                          \
   const Blk te_b_ = te_block(a);                                         \

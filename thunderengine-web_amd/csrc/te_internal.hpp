@@ -18,6 +18,7 @@ struct Op {
   int32_t kid;
   uint32_t A[64];
   int32_t nz = 0;  // levels of the op's grid (set before the first launch)
+  bool flat = false;  // launched flat over its words (set with nz)
 };
 
 int kernel_id(const char* name);

@@ -67,7 +67,7 @@ static bool launch_shape(KArg& a, const dim3& b, const dim3& g) {
 }
 
 static KArg karg_of(const Op& op) {
-  KArg a;
+  KArg a{};
   a.n[0] = op.A[0]; a.n[1] = op.A[1]; a.n[2] = op.A[2];
   for (int q = 0; q < 48; ++q) a.p[q] = op.A[4 + q];
   return a;
@@ -111,6 +111,15 @@ static hipError_t launch(te_ctx* c, const Op& op) {
   static const int kAdvFused = kernel_id("adv_fused");
   if (op.kid == kAdvFused) b = dim3(64, 4, 1);  // one row of cells per wave (k_adv_fused)
   size_t lds = 0;
+  if (op.flat) {  // runs of 4 words per thread over the op's n0*n1*n2 words (te_flat)
+    const uint64_t words = (uint64_t)a.n[0] * a.n[1] * a.n[2], runs = (words + 3) / 4;
+    b = dim3(256, 1, 1);
+    const dim3 g((uint32_t)((runs + 255) / 256), 1, 1);
+    if (!launch_shape(a, b, g)) return hipErrorInvalidConfiguration;
+    a.lg[10] = 1;
+    void* args[] = {(void*)&c->dM, (void*)&a};
+    return hipLaunchKernel(kKernels[op.kid].fn, g, b, args, 0, c->stream);
+  }
   if (a.n[2] == 1 && is_column_kernel(op.kid)) {
     b = dim3(column_lanes(), 1, 1);
     if (uses_column_lds(op.kid)) lds = (size_t)2 * op.nz * b.x * sizeof(float);
@@ -228,8 +237,22 @@ static int ensure_guard(te_ctx* c) {
   if (c->guard_ok) return TE_OK;
   uint64_t need = 0;
   std::map<uint32_t, std::array<uint32_t, 5>> seen;  // grid parameter blocks already read
+  // pointwise ops that may run flat (te_flat): kernel -> last field parameter
+  static const std::map<std::string, int> flatOps = {{"zero_fields", 8}, {"add_fields", 8},
+                                                     {"tendency_reset_add", 6}, {"stage_state_reset", 10},
+                                                     {"copy_fields", 8}, {"_copy", 1}};
+  auto aligned = [](const Op& op, int first, int last) {
+    for (int q = first; q <= last; ++q)
+      if (op.A[4 + q] % 4 != 0) return false;
+    return true;
+  };
   for (Op& op : c->ops) {
-    if (std::strcmp(te::kernel_name(op.kid), "_copy") == 0) continue;
+    const char* kn = te::kernel_name(op.kid);
+    const auto fl = flatOps.find(kn);
+    if (std::strcmp(kn, "_copy") == 0) {
+      op.flat = aligned(op, 0, 1);
+      continue;
+    }
     uint32_t d[5];
     auto it = seen.find(op.A[4]);
     if (it != seen.end()) {
@@ -242,6 +265,11 @@ static int ensure_guard(te_ctx* c) {
     if (nz < 1 || sJ < 1 || sK < 1 || nz > 100000 || sJ > 100000 || sK > 100000)
       return fail(c, TE_ERR_ARG, std::string("op ") + te::kernel_name(op.kid) + ": invalid grid parameters");
     op.nz = nz;
+    if (fl != flatOps.end()) {
+      const bool wholeGrid = op.A[0] == (uint32_t)sJ && op.A[1] == (uint32_t)sK && op.A[2] == (uint32_t)(nz + 1);
+      const bool words1d = std::strcmp(kn, "copy_fields") == 0 && op.A[1] == 1 && op.A[2] == 1;
+      op.flat = (wholeGrid || words1d) && aligned(op, 1, fl->second);
+    }
     if (te::uses_column_lds(op.kid) && (size_t)2 * nz * te::column_lanes() * sizeof(float) > 65536)
       return fail(c, TE_ERR_UNSUPPORTED, std::string(te::kernel_name(op.kid)) + ": nz too large for LDS coefficients");
     need = std::max<uint64_t>(need, (uint64_t)sJ * sK * (uint64_t)(nz + 3));
