@@ -33,6 +33,9 @@ B_ALG_STEP = 1452.0
 # kernels launched with one lane per (i, j) column that sweep k serially
 COLUMN_KERNELS = {"acoustic_w_solve", "turb_scalar_vert_implicit", "turb_u_tendency", "turb_v_tendency",
                   "turb_w_tendency", "tke_transport"}
+# parameters (elimination coefficients cA, cB) the *_lds launches keep in LDS
+LDS_RESIDENT = {"acoustic_w_solve": (10, 11), "turb_scalar_vert_implicit": (11, 12), "turb_u_tendency": (12, 13),
+                "turb_v_tendency": (12, 13), "turb_w_tendency": (11, 12), "tke_transport": (9, 10)}
 
 WORKLOAD = dict(nx=192, ny=192, nz=80, dx=1000.0, dy=1000.0, dz=250.0, dt=6.0, nsound=6, moist=1, species=3,
                 weno_scalars=1, weno_momentum=0, tke=1, rayleigh=1, rayleigh_zd=15000.0)
@@ -115,6 +118,8 @@ def alg_bytes_per_launch(run, names, A, op):
         io = kernel_io().get(name)
     if io is None:
         return None
+    if names[op].endswith("_lds"):
+        io = [r for r in io if r[0] not in LDS_RESIDENT.get(name, ())]
     return 4.0 * count(io) * pts
 
 
