@@ -558,26 +558,180 @@ __device__ void uv_tendency(uint32_t* M, const KArg& a, int i, int j) {
     }
   }
 }
+// The same sweeps with 4 waves per 64 columns (the _lds launches when the
+// runtime gives them 64 x 4 blocks): every level's dwd and tridiagonal
+// coefficients are formed by all waves at once, the recurrence -- the only
+// serial part -- runs in wave 0 out of LDS, and all waves then form the
+// tendencies.  Level by level these are uv_tendency's operations: its first
+// and last levels' special forms are the general recurrence with cfa = 0 at
+// the bottom and cfc = 0 at the top (fma(0, 0, b) = b, fma(-0, 0, d) = d,
+// (1 - 0) - c = 1 - c, x - 0 = x).  LDS: 3 nz floats per column (cfa, cfc,
+// rhs; then pa, -, pb; then the solution in the rhs slots).
+template <int DIR>
+__device__ void uv_tendency_mw(uint32_t* M, const KArg& a, int i, int j) {
+  Dims g = dims(a.q[0], 0);
+  const int wv = (int)threadIdx.y, lane = (int)threadIdx.x;
+  // lanes past the row's columns touch no memory; they only meet the barriers
+  const bool live = i < (int)a.n[0] && (DIR == 0 ? (i <= g.nx && j < g.ny) : (i < g.nx && j <= g.ny));
+  const float *q = fq(a.q[1]), *w = fq(a.q[2]), *rho = fq(a.q[3]), *rf = fq(a.q[4]), *kmv = fq(a.q[5]),
+              *tA = fq(a.q[6]), *tB = fq(a.q[7]), *tC = fq(a.q[8]), *mh = fq(a.q[9]), *mf = fq(a.q[10]),
+              *hm = fq(a.q[30]);
+  const float dt = ldf(a.q[11], 0);
+  float *dwd = fq(a.q[14]), *ten = fq(a.q[15]);
+  const int openx = ldi(a.q[16], 0), openy = ldi(a.q[16], 1), explicitZ = ldi(a.q[17], 0);
+  const int nz = g.nz;
+  const int64_t sj = g.sJ, sk = g.sk(), st = DIR == 0 ? 1 : sj, base = g.idx(i, j, 0);
+  const bool zeroX = DIR == 0 ? (openx != 0 && (i == 0 || i == g.nx)) : (openx != 0 && (i == 0 || i == g.nx - 1));
+  const bool zeroY = DIR == 0 ? (openy != 0 && (j == 0 || j == g.ny - 1)) : (openy != 0 && (j == 0 || j == g.ny));
+  const float tema = ((-0.0625f * dt) * g.rdz) * g.rdz;
+  const float temc = (0.5f * dt) * g.rdz;
+  const float rdt = 0.5f / dt;
+  const float rdh = DIR == 0 ? g.rdx : g.rdy;
+  float* LA = te_lds + lane;                  // cfa, then pa
+  float* LC = te_lds + (size_t)nz * 64 + lane;      // cfc
+  float* LR = te_lds + (size_t)2 * nz * 64 + lane;  // rhs, then pb, then the solution
+  auto Y = [&](int lev) {
+    const int64_t cl = base + lev * sk;
+    return ((kmv[cl - st] + kmv[cl]) * (mf[lev] + mf[lev])) * (rf[cl - st] + rf[cl]);
+  };
+  auto turbxy = [&](int64_t c, float& tx, float& ty) {
+    if (DIR == 0) {
+      tx = zeroX ? 0.0f : ((tA[c] - tA[c - 1]) * g.rdx) * hm_uf(hm, g, i);
+      ty = zeroY ? 0.0f : ((tB[c + sj] - tB[c]) * g.rdy) * hm_vh(hm, g, j);
+    } else {
+      tx = zeroX ? 0.0f : ((tA[c + 1] - tA[c]) * g.rdx) * hm_uh(hm, g, i);
+      ty = zeroY ? 0.0f : ((tB[c] - tB[c - sj]) * g.rdy) * hm_vf(hm, g, j);
+    }
+  };
+  if (explicitZ != 0) {  // uniform: no barrier is reached
+    if (live)
+      for (int k = wv; k < nz; k += 4) {
+        const int64_t c = base + k * sk;
+        float tx, ty;
+        turbxy(c, tx, ty);
+        float tz = (((tC[c + sk] - tC[c]) * g.rdz) * 0.5f) * (mh[k] + mh[k]);
+        ten[c] = ten[c] + ((tx + ty) + tz) / (0.5f * (rho[c - st] + rho[c]));
+      }
+    return;
+  }
+  // levels k = wv + 4 t; MW_CH of them per pass, all loads of a pass issued
+  // before its arithmetic (level nz + 1 and beyond are never touched)
+  constexpr int MW_CH = 4;
+  if (live) {
+    const float hmf = DIR == 0 ? hm_uf(hm, g, i) : hm_vf(hm, g, j);
+    const float tCb = tC[base], tCt = tC[base + nz * sk];
+    for (int k0 = wv; k0 <= nz; k0 += 4 * MW_CH) {
+      float w0[MW_CH], w1[MW_CH], ka[MW_CH], kb[MW_CH], ra[MW_CH], rb[MW_CH], mfk[MW_CH], ro0[MW_CH], ro1[MW_CH],
+          mhk[MW_CH], qk[MW_CH], kna[MW_CH], knb[MW_CH], rna[MW_CH], rnb[MW_CH], mfn[MW_CH];
+#pragma unroll
+      for (int t = 0; t < MW_CH; ++t) {
+        const int k = k0 + 4 * t;
+        if (k <= nz) {
+          const int64_t c = base + k * sk;
+          w0[t] = w[c]; w1[t] = w[c - st]; ka[t] = kmv[c - st]; kb[t] = kmv[c]; ra[t] = rf[c - st]; rb[t] = rf[c];
+          mfk[t] = mf[k];
+          if (k < nz) {
+            ro0[t] = rho[c - st]; ro1[t] = rho[c]; mhk[t] = mh[k]; qk[t] = q[c];
+            if (k < nz - 1) {
+              kna[t] = kmv[c + sk - st]; knb[t] = kmv[c + sk]; rna[t] = rf[c + sk - st]; rnb[t] = rf[c + sk];
+              mfn[t] = mf[k + 1];
+            }
+          }
+        }
+      }
+#pragma unroll
+      for (int t = 0; t < MW_CH; ++t) {
+        const int k = k0 + 4 * t;
+        if (k <= nz) {
+          const int64_t c = base + k * sk;
+          dwd[c] = (((((w0[t] - w1[t]) * rdh) * hmf) * 0.25f) * (ka[t] + kb[t])) * (ra[t] + rb[t]);
+          if (k < nz) {
+            const float rinvk = 1.0f / (0.5f * (ro0[t] + ro1[t]));
+            const float tem = (mhk[t] + mhk[t]) * rinvk;
+            const float Yk = ((ka[t] + kb[t]) * (mfk[t] + mfk[t])) * (ra[t] + rb[t]);
+            const float cfa = k == 0 ? 0.0f : tema * (Yk * tem);
+            float cfc = 0.0f;
+            if (k < nz - 1) cfc = tema * ((((kna[t] + knb[t]) * (mfn[t] + mfn[t])) * (rna[t] + rnb[t])) * tem);
+            float rhs;
+            if (k == 0) rhs = fma_(-((temc * tCb) * (mhk[t] + mhk[t])), rinvk, qk[t]);
+            else if (k == nz - 1) rhs = fma_((temc * tCt) * (mhk[t] + mhk[t]), rinvk, qk[t]);
+            else rhs = qk[t];
+            LA[k * 64] = cfa;
+            LC[k * 64] = cfc;
+            LR[k * 64] = rhs;
+          }
+        }
+      }
+    }
+  }
+  __syncthreads();
+  if (live && wv == 0) {
+    float pa = 0.0f, pb = 0.0f;
+    for (int k = 0; k < nz; ++k) {
+      const float cfa = LA[k * 64], cfc = LC[k * 64];
+      const float cfb = (1.0f - cfa) - cfc;
+      const float tt = 1.0f / fma_(cfa, pa, cfb);
+      pa = (-cfc) * tt;
+      pb = fma_(-cfa, pb, LR[k * 64]) * tt;
+      LA[k * 64] = pa;
+      LR[k * 64] = pb;
+    }
+    for (int k = nz - 2; k >= 0; --k) {
+      pb = fma_(LA[k * 64], pb, LR[k * 64]);
+      LR[k * 64] = pb;
+    }
+  }
+  __syncthreads();
+  if (!live) return;
+  for (int k0 = wv; k0 < nz; k0 += 4 * MW_CH) {
+    float tx[MW_CH], ty[MW_CH], ro0[MW_CH], ro1[MW_CH], qk[MW_CH], d0[MW_CH], d1[MW_CH], mhk[MW_CH], tv[MW_CH];
+#pragma unroll
+    for (int t = 0; t < MW_CH; ++t) {
+      const int k = k0 + 4 * t;
+      if (k < nz) {
+        const int64_t c = base + k * sk;
+        turbxy(c, tx[t], ty[t]);
+        ro0[t] = rho[c - st]; ro1[t] = rho[c]; qk[t] = q[c]; d0[t] = dwd[c]; d1[t] = dwd[c + sk]; mhk[t] = mh[k];
+        tv[t] = ten[c];
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < MW_CH; ++t) {
+      const int k = k0 + 4 * t;
+      if (k < nz) {
+        const int64_t c = base + k * sk;
+        float tz = ((ro0[t] + ro1[t]) * (LR[k * 64] - qk[t])) * rdt;
+        tz = fma_(((d1[t] - d0[t]) * g.rdz) * 0.5f, mhk[t] + mhk[t], tz);
+        ten[c] = fma_((tx[t] + ty[t]) + tz, 1.0f / (0.5f * (ro0[t] + ro1[t])), tv[t]);
+      }
+    }
+  }
+}
+// 64 x 4 blocks (x: columns, y: waves; grid y: rows) when the runtime chose
+// them, else one wave per block as uv_tendency<DIR, true>.
+template <int DIR>
+__device__ void uv_tendency_lds(uint32_t* M, const KArg& a) {
+  const Blk b_ = te_block(a);
+  if (TE_BY == 4) {  // grid (columns / 64, rows)
+    uv_tendency_mw<DIR>(M, a, b_.x * 64 + (int)threadIdx.x, b_.y);
+    return;
+  }
+  const int i = b_.x * TE_BX + (int)threadIdx.x, j = b_.y * TE_BY + (int)threadIdx.y;
+  if (i < (int)a.n[0] && j < (int)a.n[1]) uv_tendency<DIR, true>(M, a, i, j);
+}
+
 __global__ __launch_bounds__(256) void k_turb_u_tendency(uint32_t* M, KArg a) {
   const Blk b_ = te_block(a);
   const int i = b_.x * TE_BX + (int)threadIdx.x, j = b_.y * TE_BY + (int)threadIdx.y;
   if (i < (int)a.n[0] && j < (int)a.n[1]) uv_tendency<0, false>(M, a, i, j);
 }
-__global__ __launch_bounds__(256) void k_turb_u_tendency_lds(uint32_t* M, KArg a) {
-  const Blk b_ = te_block(a);
-  const int i = b_.x * TE_BX + (int)threadIdx.x, j = b_.y * TE_BY + (int)threadIdx.y;
-  if (i < (int)a.n[0] && j < (int)a.n[1]) uv_tendency<0, true>(M, a, i, j);
-}
+__global__ __launch_bounds__(256) void k_turb_u_tendency_lds(uint32_t* M, KArg a) { uv_tendency_lds<0>(M, a); }
 __global__ __launch_bounds__(256) void k_turb_v_tendency(uint32_t* M, KArg a) {
   const Blk b_ = te_block(a);
   const int i = b_.x * TE_BX + (int)threadIdx.x, j = b_.y * TE_BY + (int)threadIdx.y;
   if (i < (int)a.n[0] && j < (int)a.n[1]) uv_tendency<1, false>(M, a, i, j);
 }
-__global__ __launch_bounds__(256) void k_turb_v_tendency_lds(uint32_t* M, KArg a) {
-  const Blk b_ = te_block(a);
-  const int i = b_.x * TE_BX + (int)threadIdx.x, j = b_.y * TE_BY + (int)threadIdx.y;
-  if (i < (int)a.n[0] && j < (int)a.n[1]) uv_tendency<1, true>(M, a, i, j);
-}
+__global__ __launch_bounds__(256) void k_turb_v_tendency_lds(uint32_t* M, KArg a) { uv_tendency_lds<1>(M, a); }
 
 // Implicit vertical solve shared by turb_w_tendency and tke_transport (both
 // solve for a face field x on levels 1..nz-1 with the same matrix):

@@ -123,6 +123,17 @@ static hipError_t launch(te_ctx* c, const Op& op) {
   if (a.n[2] == 1 && is_column_kernel(op.kid)) {
     b = dim3(column_lanes(), 1, 1);
     if (uses_column_lds(op.kid)) lds = (size_t)2 * op.nz * b.x * sizeof(float);
+    // u / v tendencies: 4 waves per 64 columns, one row per block (uv_tendency_mw)
+    static const int kU = kernel_id("turb_u_tendency_lds"), kV = kernel_id("turb_v_tendency_lds");
+    const size_t mwLds = (size_t)3 * op.nz * 64 * sizeof(float);
+    if ((op.kid == kU || op.kid == kV) && op.nz >= 3 && mwLds <= 65536) {
+      b = dim3(64, 4, 1);
+      lds = mwLds;
+      const dim3 g((a.n[0] + 63) / 64, a.n[1], 1);
+      if (!launch_shape(a, b, g)) return hipErrorInvalidConfiguration;
+      void* args[] = {(void*)&c->dM, (void*)&a};
+      return hipLaunchKernel(kKernels[op.kid].fn, g, b, args, lds, c->stream);
+    }
   }
   dim3 g((a.n[0] + b.x - 1) / b.x, (a.n[1] + b.y - 1) / b.y, (a.n[2] + b.z - 1) / b.z);
   if (!launch_shape(a, b, g)) return hipErrorInvalidConfiguration;  // beyond te_block()'s exact range
