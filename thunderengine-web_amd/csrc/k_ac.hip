@@ -36,6 +36,16 @@ __global__ __launch_bounds__(256) void k_acoustic_prepare(uint32_t* M, KArg a) {
 
 // acoustic_begin.wgsl:128-149
 __global__ __launch_bounds__(256) void k_acoustic_begin(uint32_t* M, KArg a) {
+  if (te_flat(a, [&](uint32_t w, int width) {
+        const float beta = acoustic(a.q[1], 0).beta;
+        F4 z, p = ld4(a.q[2], w, width);
+        for (int e = 0; e < 4; ++e) z.v[e] = 0.0f;
+        for (int e = 0; e < width; ++e) p.v[e] = p.v[e] * beta;
+        st4(a.q[3], w, width, z);
+        st4(a.q[4], w, width, z);
+        st4(a.q[5], w, width, p);
+      }))
+    return;
   TE_TID3
   Dims g = dims(a.q[0], 0);
   int i = gx - 3, j = gy - 3, k = gz;

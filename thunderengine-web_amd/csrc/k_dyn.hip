@@ -86,6 +86,27 @@ __global__ __launch_bounds__(256) void k_stage_state_reset(uint32_t* M, KArg a) 
 
 // massflux_from_state.wgsl:111-137
 __global__ __launch_bounds__(256) void k_massflux_from_state(uint32_t* M, KArg a) {
+  {
+    const Dims g = dims(a.q[0], 0);
+    if (te_flat_levels(a, (uint32_t)(g.sJ * g.sK), [&](uint32_t w, int width, int k) {
+          if (k < g.nz) {
+            const float r = fq(a.q[4])[k];
+            F4 u = ld4(a.q[1], w, width), v = ld4(a.q[2], w, width);
+            for (int e = 0; e < width; ++e) {
+              u.v[e] = r * u.v[e];
+              v.v[e] = r * v.v[e];
+            }
+            st4(a.q[6], w, width, u);
+            st4(a.q[7], w, width, v);
+          }
+          F4 m = ld4(a.q[3], w, width);
+          const bool edge = k == 0 || k == g.nz;
+          const float rf = edge ? 0.0f : fq(a.q[5])[k];
+          for (int e = 0; e < width; ++e) m.v[e] = edge ? 0.0f : rf * m.v[e];
+          st4(a.q[8], w, width, m);
+        }))
+      return;
+  }
   TE_TID3
   Dims g = dims(a.q[0], 0);
   int i = gx - 3, j = gy - 3, k = gz;
@@ -102,6 +123,27 @@ __global__ __launch_bounds__(256) void k_massflux_from_state(uint32_t* M, KArg a
 
 // velocity_to_massflux.wgsl:111-134
 __global__ __launch_bounds__(256) void k_velocity_to_massflux(uint32_t* M, KArg a) {
+  {
+    const Dims g = dims(a.q[0], 0);
+    if (te_flat_levels(a, (uint32_t)(g.sJ * g.sK), [&](uint32_t w, int width, int k) {
+          if (k < g.nz) {
+            const float r = fq(a.q[1])[k];
+            F4 u = ld4(a.q[3], w, width), v = ld4(a.q[4], w, width);
+            for (int e = 0; e < width; ++e) {
+              u.v[e] = u.v[e] * r;
+              v.v[e] = v.v[e] * r;
+            }
+            st4(a.q[3], w, width, u);
+            st4(a.q[4], w, width, v);
+          }
+          const bool edge = k == 0 || k == g.nz;
+          F4 m = ld4(a.q[5], w, width);
+          const float rf = edge ? 0.0f : fq(a.q[2])[k];
+          for (int e = 0; e < width; ++e) m.v[e] = edge ? 0.0f : m.v[e] * rf;
+          st4(a.q[5], w, width, m);
+        }))
+      return;
+  }
   TE_TID3
   Dims g = dims(a.q[0], 0);
   int i = gx - 3, j = gy - 3, k = gz;
@@ -173,6 +215,17 @@ __global__ __launch_bounds__(256) void k_rk_update(uint32_t* M, KArg a) {
 
 // build_sadv.wgsl:111-127
 __global__ __launch_bounds__(256) void k_build_sadv(uint32_t* M, KArg a) {
+  {
+    const Dims g = dims(a.q[0], 0);
+    if (te_flat_levels(a, (uint32_t)(g.sJ * g.sK), [&](uint32_t w, int width, int k) {
+          if (k >= g.nz) return;
+          const float t0 = fq(a.q[1])[k] - 3e+02f;
+          F4 s = ld4(a.q[2], w, width);
+          for (int e = 0; e < width; ++e) s.v[e] = t0 + s.v[e];
+          st4(a.q[3], w, width, s);
+        }))
+      return;
+  }
   TE_TID3
   Dims g = dims(a.q[0], 0);
   int i = gx - 3, j = gy - 3, k = gz;

@@ -293,6 +293,23 @@ TE_DEV bool te_flat(const KArg& a, F op) {
   }
   return true;
 }
+// te_flat for ops whose per-word work depends on the level only: op(word,
+// width, k) with k the word's level (L words per level); a run of 4 that
+// straddles two levels is done word by word.
+template <class F>
+TE_DEV bool te_flat_levels(const KArg& a, uint32_t L, F op) {
+  if (a.lg[10] == 0) return false;
+  const uint32_t W = a.n[0] * a.n[1] * a.n[2];
+  const uint32_t f = 4u * ((uint32_t)te_block(a).x * (uint32_t)a.lg[0] + threadIdx.x);
+  if (f >= W) return true;
+  const uint32_t k0 = f / L, next = (k0 + 1) * L;
+  if (f + 4 <= W && f + 4 <= next) {
+    op(f, 4, (int)k0);
+  } else {
+    for (uint32_t w = f; w < W && w < f + 4; ++w) op(w, 1, (int)(w < next ? k0 : k0 + 1));
+  }
+  return true;
+}
 // 4 or 1 floats at word w of p
 struct F4 {
   float v[4];

@@ -248,12 +248,21 @@ static int ensure_guard(te_ctx* c) {
   if (c->guard_ok) return TE_OK;
   uint64_t need = 0;
   std::map<uint32_t, std::array<uint32_t, 5>> seen;  // grid parameter blocks already read
-  // pointwise ops that may run flat (te_flat): kernel -> last field parameter
-  static const std::map<std::string, int> flatOps = {{"zero_fields", 8}, {"add_fields", 8},
-                                                     {"tendency_reset_add", 6}, {"stage_state_reset", 10},
-                                                     {"copy_fields", 8}, {"_copy", 1}};
-  auto aligned = [](const Op& op, int first, int last) {
-    for (int q = first; q <= last; ++q)
+  // pointwise ops that may run flat (te_flat): kernel -> the parameters it
+  // accesses in 16-byte runs (fields; profiles are read per level)
+  static const std::map<std::string, std::vector<int>> flatOps = {
+      {"zero_fields", {1, 2, 3, 4, 5, 6, 7, 8}},
+      {"add_fields", {1, 2, 3, 4, 5, 6, 7, 8}},
+      {"tendency_reset_add", {1, 2, 3, 4, 5, 6}},
+      {"stage_state_reset", {1, 2, 3, 4, 5, 6, 7, 8, 9, 10}},
+      {"copy_fields", {1, 2, 3, 4, 5, 6, 7, 8}},
+      {"acoustic_begin", {2, 3, 4, 5}},
+      {"massflux_from_state", {1, 2, 3, 6, 7, 8}},
+      {"velocity_to_massflux", {3, 4, 5}},
+      {"build_sadv", {2, 3}},
+      {"_copy", {0, 1}}};
+  auto aligned = [](const Op& op, const std::vector<int>& ps) {
+    for (int q : ps)
       if (op.A[4 + q] % 4 != 0) return false;
     return true;
   };
@@ -261,7 +270,7 @@ static int ensure_guard(te_ctx* c) {
     const char* kn = te::kernel_name(op.kid);
     const auto fl = flatOps.find(kn);
     if (std::strcmp(kn, "_copy") == 0) {
-      op.flat = aligned(op, 0, 1);
+      op.flat = aligned(op, fl->second);
       continue;
     }
     uint32_t d[5];
@@ -279,7 +288,7 @@ static int ensure_guard(te_ctx* c) {
     if (fl != flatOps.end()) {
       const bool wholeGrid = op.A[0] == (uint32_t)sJ && op.A[1] == (uint32_t)sK && op.A[2] == (uint32_t)(nz + 1);
       const bool words1d = std::strcmp(kn, "copy_fields") == 0 && op.A[1] == 1 && op.A[2] == 1;
-      op.flat = (wholeGrid || words1d) && aligned(op, 1, fl->second);
+      op.flat = (wholeGrid || words1d) && aligned(op, fl->second);
     }
     if (te::uses_column_lds(op.kid) && (size_t)2 * nz * te::column_lanes() * sizeof(float) > 65536)
       return fail(c, TE_ERR_UNSUPPORTED, std::string(te::kernel_name(op.kid)) + ": nz too large for LDS coefficients");
