@@ -24,8 +24,9 @@ struct KArg {
   // y (2), m = ceil(2^40 / d) as (low, high) words for d = grid blocks in x
   // and in y (4; floor(l / d) = (l * m) >> 40 exactly for l < 2^24 and
   // d <= 2^16: Granlund & Montgomery, PLDI 1994, thm. 4.2), grid blocks (1),
-  // flat launch (1: see te_flat)
-  uint32_t lg[11];
+  // flat launch (1: see te_flat), plane launch (1: see TE_TID3) with the
+  // division constant of the plane's row length n0 (2)
+  uint32_t lg[14];
   // the parameters as device pointers (the memory's base + p[n] words),
   // filled by the runtime so that kernels do no per-wave address arithmetic
   uint32_t* q[48];
@@ -329,11 +330,27 @@ TE_DEV void st4(uint32_t* p, uint32_t w, int width, const F4& r) {
   else p[w] = __float_as_uint(r.v[0]);
 }
 
+// Thread coordinates of a 3-D launch.  Plane launches (rows whose length is
+// not a multiple of 64: whole-field rows of nx+7, staggered rows of nx+1)
+// number the (i, j) plane's points consecutively, 256 per workgroup, with
+// the level on the grid's y axis, so no wave idles at a row's end.
+TE_DEV void te_tid3(const KArg& a, int& gx, int& gy, int& gz) {
+  const Blk b = te_block(a);
+  if (a.lg[11] != 0) {
+    const uint32_t t = (uint32_t)b.x * 256u + threadIdx.x;
+    const uint32_t row = te_div(t, a.lg[12], a.lg[13]);
+    gx = (int)(t - row * a.n[0]);
+    gy = (int)row;
+    gz = b.y;
+    return;
+  }
+  gx = b.x * TE_BX + (int)threadIdx.x;
+  gy = b.y * TE_BY + (int)threadIdx.y;
+  gz = b.z * TE_BZ + (int)threadIdx.z;
+}
 #define TE_TID3                                                          \
-  const Blk te_b_ = te_block(a);                                         \
-  const int gx = te_b_.x * TE_BX + (int)threadIdx.x;                     \
-  const int gy = te_b_.y * TE_BY + (int)threadIdx.y;                     \
-  const int gz = te_b_.z * TE_BZ + (int)threadIdx.z;                     \
+  int gx, gy, gz;                                                        \
+  te_tid3(a, gx, gy, gz);                                                \
   if (gx >= (int)a.n[0] || gy >= (int)a.n[1] || gz >= (int)a.n[2]) return;
 
 }  // namespace te

@@ -19,6 +19,7 @@ struct Op {
   uint32_t A[64];
   int32_t nz = 0;  // levels of the op's grid (set before the first launch)
   bool flat = false;  // launched flat over its words (set with nz)
+  bool plane = false;  // launched over its (i, j) plane's points (set with nz)
 };
 
 int kernel_id(const char* name);

@@ -8,6 +8,8 @@
 #include <array>
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <set>
 
 #include "te_internal.hpp"
 #include "te_kernels.hpp"
@@ -134,6 +136,18 @@ static hipError_t launch(te_ctx* c, const Op& op) {
       void* args[] = {(void*)&c->dM, (void*)&a};
       return hipLaunchKernel(kKernels[op.kid].fn, g, b, args, lds, c->stream);
     }
+  }
+  if (op.plane) {  // (i, j) points numbered consecutively, level on grid y (te_tid3)
+    const uint64_t pts = (uint64_t)a.n[0] * a.n[1];
+    b = dim3(256, 1, 1);
+    const dim3 g((uint32_t)((pts + 255) / 256), a.n[2], 1);
+    if (pts + 256 > (1u << 24) || a.n[0] > (1u << 16) || !launch_shape(a, b, g)) return hipErrorInvalidConfiguration;
+    const uint64_t m = ((uint64_t)1 << 40) / a.n[0] + (((uint64_t)1 << 40) % a.n[0] != 0);
+    a.lg[11] = 1;
+    a.lg[12] = (uint32_t)m;
+    a.lg[13] = (uint32_t)(m >> 32);
+    void* args[] = {(void*)&c->dM, (void*)&a};
+    return hipLaunchKernel(kKernels[op.kid].fn, g, b, args, 0, c->stream);
   }
   dim3 g((a.n[0] + b.x - 1) / b.x, (a.n[1] + b.y - 1) / b.y, (a.n[2] + b.z - 1) / b.z);
   if (!launch_shape(a, b, g)) return hipErrorInvalidConfiguration;  // beyond te_block()'s exact range
@@ -290,6 +304,15 @@ static int ensure_guard(te_ctx* c) {
       const bool words1d = std::strcmp(kn, "copy_fields") == 0 && op.A[1] == 1 && op.A[2] == 1;
       op.flat = (wholeGrid || words1d) && aligned(op, fl->second);
     }
+    // kernels that map threads themselves (columns, fused advection, copies)
+    static const std::set<std::string> ownMapping = {
+        "acoustic_w_solve", "acoustic_w_solve_lds", "adv_fused", "_copy", "copy_fields", "turb_sfcstress",
+        "turb_u_tendency", "turb_u_tendency_lds", "turb_v_tendency", "turb_v_tendency_lds", "turb_w_tendency",
+        "turb_w_tendency_lds", "turb_scalar_vert_implicit", "turb_scalar_vert_implicit_lds", "tke_transport",
+        "tke_transport_lds"};
+    // (te_div is exact for numerators below 2^24, divisors up to 2^16)
+    op.plane = !op.flat && !ownMapping.count(kn) && op.A[0] % 64 != 0 && op.A[1] > 1 && op.A[0] <= (1u << 16) &&
+               (uint64_t)op.A[0] * op.A[1] + 256 <= (1u << 24);
     if (te::uses_column_lds(op.kid) && (size_t)2 * nz * te::column_lanes() * sizeof(float) > 65536)
       return fail(c, TE_ERR_UNSUPPORTED, std::string(te::kernel_name(op.kid)) + ": nz too large for LDS coefficients");
     need = std::max<uint64_t>(need, (uint64_t)sJ * sK * (uint64_t)(nz + 3));
