@@ -558,6 +558,27 @@ __device__ void uv_tendency(uint32_t* M, const KArg& a, int i, int j) {
     }
   }
 }
+// Serial phase of the 4-wave column sweeps (wave 0): the tridiagonal
+// recurrence over levels kb..ke from the coefficients in LDS (cfa, cfc, rhs at
+// LA / LC / LR + 64 k), leaving the solution in LR.  With cfa = 0 at kb and
+// cfc = 0 at ke it is every sweep's first / last level form as well.
+TE_DEV void mw_recurrence(float* LA, const float* LC, float* LR, int kb, int ke) {
+  float pa = 0.0f, pb = 0.0f;
+  for (int k = kb; k <= ke; ++k) {
+    const float cfa = LA[k * 64], cfc = LC[k * 64];
+    const float cfb = (1.0f - cfa) - cfc;
+    const float tt = 1.0f / fma_(cfa, pa, cfb);
+    pa = (-cfc) * tt;
+    pb = fma_(-cfa, pb, LR[k * 64]) * tt;
+    LA[k * 64] = pa;
+    LR[k * 64] = pb;
+  }
+  for (int k = ke - 1; k >= kb; --k) {
+    pb = fma_(LA[k * 64], pb, LR[k * 64]);
+    LR[k * 64] = pb;
+  }
+}
+
 // The same sweeps with 4 waves per 64 columns (the _lds launches when the
 // runtime gives them 64 x 4 blocks): every level's dwd and tridiagonal
 // coefficients are formed by all waves at once, the recurrence -- the only
@@ -665,22 +686,7 @@ __device__ void uv_tendency_mw(uint32_t* M, const KArg& a, int i, int j) {
     }
   }
   __syncthreads();
-  if (live && wv == 0) {
-    float pa = 0.0f, pb = 0.0f;
-    for (int k = 0; k < nz; ++k) {
-      const float cfa = LA[k * 64], cfc = LC[k * 64];
-      const float cfb = (1.0f - cfa) - cfc;
-      const float tt = 1.0f / fma_(cfa, pa, cfb);
-      pa = (-cfc) * tt;
-      pb = fma_(-cfa, pb, LR[k * 64]) * tt;
-      LA[k * 64] = pa;
-      LR[k * 64] = pb;
-    }
-    for (int k = nz - 2; k >= 0; --k) {
-      pb = fma_(LA[k * 64], pb, LR[k * 64]);
-      LR[k * 64] = pb;
-    }
-  }
+  if (live && wv == 0) mw_recurrence(LA, LC, LR, 0, nz - 1);
   __syncthreads();
   if (!live) return;
   for (int k0 = wv; k0 < nz; k0 += 4 * MW_CH) {
@@ -860,14 +866,79 @@ struct FaceTendency {
   __device__ void at(int t, int64_t c, float xn, float xo, float rfc) { apply(c, xn, xo, rfc, txv[t], tyv[t], tv[t]); }
 };
 
+// face_implicit with 4 waves per 64 columns (grid: columns / 64, rows):
+// every level's coefficients by all waves into LDS, the recurrence in wave 0
+// (mw_recurrence; cfa = 0 at level 1, cfc = 0 at level nz-1 give the first /
+// last forms), then the tendencies by all waves.  Lanes with live = false
+// touch no memory and only meet the barriers.
+constexpr int MW_CH = 4;
+template <class OUT>
+__device__ void face_implicit_mw(bool live, int wv, int lane, const float* kmh, const float* mh, const float* rho,
+                                 const float* mf, const float* rf, const float* x, int64_t base, int64_t sk, int nz,
+                                 float tema, float cfd1, float cfdT, OUT& out) {
+  float* LA = te_lds + lane;
+  float* LC = te_lds + (size_t)nz * 64 + lane;
+  float* LR = te_lds + (size_t)2 * nz * 64 + lane;
+  if (live) {
+    for (int k0 = 1 + wv; k0 <= nz - 1; k0 += 4 * MW_CH) {
+      float ka[MW_CH], kb[MW_CH], kc[MW_CH], mhp[MW_CH], mhk[MW_CH], rop[MW_CH], rok[MW_CH], mfk[MW_CH], rfk[MW_CH],
+          xv[MW_CH];
+#pragma unroll
+      for (int t = 0; t < MW_CH; ++t) {
+        const int k = k0 + 4 * t;
+        if (k <= nz - 1) {
+          const int64_t c = base + k * sk;
+          ka[t] = kmh[c - sk]; kb[t] = kmh[c]; kc[t] = kmh[c + sk]; mhp[t] = mh[k - 1]; mhk[t] = mh[k];
+          rop[t] = rho[c - sk]; rok[t] = rho[c]; mfk[t] = mf[k]; rfk[t] = rf[c]; xv[t] = x[c];
+        }
+      }
+#pragma unroll
+      for (int t = 0; t < MW_CH; ++t) {
+        const int k = k0 + 4 * t;
+        if (k <= nz - 1) {
+          const float mfrf = mfk[t] / rfk[t];
+          const float Zp = ((ka[t] + kb[t]) * mhp[t]) * rop[t];  // Z(k-1)
+          const float Zk = ((kb[t] + kc[t]) * mhk[t]) * rok[t];  // Z(k)
+          LA[k * 64] = k == 1 ? 0.0f : tema * (Zp * mfrf);
+          LC[k * 64] = k == nz - 1 ? 0.0f : tema * (Zk * mfrf);
+          LR[k * 64] = k == 1 ? cfd1 : (k == nz - 1 ? cfdT : xv[t]);
+        }
+      }
+    }
+  }
+  __syncthreads();
+  if (live && wv == 0) mw_recurrence(LA, LC, LR, 1, nz - 1);
+  __syncthreads();
+  if (!live) return;
+  for (int k0 = 1 + wv; k0 <= nz - 1; k0 += 4 * MW_CH) {
+    float xo[MW_CH], rfk[MW_CH];
+#pragma unroll
+    for (int t = 0; t < MW_CH; ++t) {
+      const int k = k0 + 4 * t;
+      if (k <= nz - 1) {
+        const int64_t c = base + k * sk;
+        xo[t] = x[c]; rfk[t] = rf[c];
+        out.prefetch(t, c);
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < MW_CH; ++t) {
+      const int k = k0 + 4 * t;
+      if (k <= nz - 1) out.at(t, base + k * sk, LR[k * 64], xo[t], rfk[t]);
+    }
+  }
+}
+
 // turb_w_tendency.wgsl:152-287
 template <bool LDS>
 __device__ void w_tendency(uint32_t* M, const KArg& a) {
   const Blk b_ = te_block(a);
-  const int i = b_.x * TE_BX + (int)threadIdx.x, j = b_.y * TE_BY + (int)threadIdx.y;
-  if (i >= (int)a.n[0] || j >= (int)a.n[1]) return;
+  const bool mw = LDS && TE_BY == 4;  // 64 x 4 blocks, grid (columns / 64, rows): face_implicit_mw
+  const int i = mw ? b_.x * 64 + (int)threadIdx.x : b_.x * TE_BX + (int)threadIdx.x;
+  const int j = mw ? b_.y : b_.y * TE_BY + (int)threadIdx.y;
   Dims g = dims(a.q[0], 0);
-  if (i >= g.nx || j >= g.ny) return;
+  const bool live = i < (int)a.n[0] && j < (int)a.n[1] && i < g.nx && j < g.ny;
+  if (!live && !mw) return;
   const float *w = fq(a.q[1]), *rho = fq(a.q[2]), *rf = fq(a.q[3]), *kmh = fq(a.q[4]), *t13 = fq(a.q[5]),
               *t23 = fq(a.q[6]), *t33 = fq(a.q[7]), *mh = fq(a.q[8]), *mf = fq(a.q[9]), *hm = fq(a.q[30]);
   float dt = ldf(a.q[10], 0);
@@ -880,7 +951,8 @@ __device__ void w_tendency(uint32_t* M, const KArg& a) {
     tx = zeroX ? 0.0f : ((t13[c + 1] - t13[c]) * g.rdx) * hm_uh(hm, g, i);
     ty = zeroY ? 0.0f : ((t23[c + sj] - t23[c]) * g.rdy) * hm_vh(hm, g, j);
   };
-  if (explicitZ != 0) {
+  if (explicitZ != 0) {  // uniform: no barrier is reached; in 4-wave blocks wave 0 does the columns
+    if (!live || (mw && threadIdx.y != 0)) return;
     for (int k = 1; k < g.nz; ++k) {
       const int64_t c = base + k * sk;
       float tx, ty;
@@ -895,9 +967,14 @@ __device__ void w_tendency(uint32_t* M, const KArg& a) {
   float rdt = 1.0f / dt;
   const int nz = g.nz;
   const int64_t c1 = base + sk, ct = base + (nz - 1) * sk;
-  const float cfd1 = fma_(-(temc * t33[c1 - sk]), mf[1] / rf[c1], w[c1]);
-  const float cfdT = fma_(temc * t33[ct], mf[nz - 1] / rf[ct], w[ct]);
+  const float cfd1 = live ? fma_(-(temc * t33[c1 - sk]), mf[1] / rf[c1], w[c1]) : 0.0f;
+  const float cfdT = live ? fma_(temc * t33[ct], mf[nz - 1] / rf[ct], w[ct]) : 0.0f;
   FaceTendency out{t13, t23, hm, wten, g, i, j, zeroX, zeroY, false, rdt, {}, {}, {}};
+  if (mw) {
+    face_implicit_mw(live, (int)threadIdx.y, (int)threadIdx.x, kmh, mh, rho, mf, rf, w, base, sk, nz, tema, cfd1, cfdT,
+                     out);
+    return;
+  }
   const ColStore<LDS> CA{cA, base, sk, TE_LANE, TE_NLANES, 0}, CB{cB, base, sk, TE_LANE, TE_NLANES, nz};
   face_implicit(kmh, mh, rho, mf, rf, w, CA, CB, base, sk, nz, tema, cfd1, cfdT, out);
 }
@@ -1086,10 +1163,12 @@ __global__ __launch_bounds__(256) void k_tke_flux_h(uint32_t* M, KArg a) {
 template <bool LDS>
 __device__ void tke_transport(uint32_t* M, const KArg& a) {
   const Blk b_ = te_block(a);
-  const int i = b_.x * TE_BX + (int)threadIdx.x, j = b_.y * TE_BY + (int)threadIdx.y;
-  if (i >= (int)a.n[0] || j >= (int)a.n[1]) return;
+  const bool mw = LDS && TE_BY == 4;  // 64 x 4 blocks, grid (columns / 64, rows): face_implicit_mw
+  const int i = mw ? b_.x * 64 + (int)threadIdx.x : b_.x * TE_BX + (int)threadIdx.x;
+  const int j = mw ? b_.y : b_.y * TE_BY + (int)threadIdx.y;
   Dims g = dims(a.q[0], 0);
-  if (i >= g.nx || j >= g.ny) return;
+  const bool live = i < (int)a.n[0] && j < (int)a.n[1] && i < g.nx && j < g.ny;
+  if (!live && !mw) return;
   const float *t = fq(a.q[1]), *rho = fq(a.q[2]), *rf = fq(a.q[3]), *kmh = fq(a.q[4]), *kmv = fq(a.q[5]),
               *mh = fq(a.q[6]), *mf = fq(a.q[7]), *fx = fq(a.q[11]), *fy = fq(a.q[12]), *hm = fq(a.q[30]);
   float dt = ldf(a.q[8], 0);
@@ -1103,7 +1182,8 @@ __device__ void tke_transport(uint32_t* M, const KArg& a) {
     ty = zeroY ? 0.0f : ((-(fy[c + sj] - fy[c])) * g.rdy) * hm_vh(hm, g, j);
   };
   auto mfrf = [&](int k) { return mf[k] / rf[base + k * sk]; };
-  if (explicitZ != 0) {
+  if (explicitZ != 0) {  // uniform: no barrier is reached; in 4-wave blocks wave 0 does the columns
+    if (!live || (mw && threadIdx.y != 0)) return;
     auto L1 = [&](int k) {
       const int64_t c = base + k * sk;
       return ((((-(kmv[c] + kmv[c + sk])) * (t[c + sk] - t[c])) * g.rdz) * mh[k]) * rho[c];
@@ -1125,11 +1205,19 @@ __device__ void tke_transport(uint32_t* M, const KArg& a) {
   float rdt = 1.0f / dt;
   const int nz = g.nz;
   const int64_t c1 = base + sk, ct = base + (nz - 1) * sk;
-  const float tem1 = ((((-(kmv[c1 - sk] + kmv[c1])) * (t[c1] - t[c1 - sk])) * g.rdz) * mh[0]) * rho[c1 - sk];
-  const float cfd1 = fma_(temc * tem1, mf[1] / rf[c1], t[c1]);
-  const float temT = ((((-(kmv[ct] + kmv[ct + sk])) * (t[ct + sk] - t[ct])) * g.rdz) * mh[nz - 1]) * rho[ct];
-  const float cfdT = fma_(-(temc * temT), mf[nz - 1] / rf[ct], t[ct]);
+  float cfd1 = 0.0f, cfdT = 0.0f;
+  if (live) {
+    const float tem1 = ((((-(kmv[c1 - sk] + kmv[c1])) * (t[c1] - t[c1 - sk])) * g.rdz) * mh[0]) * rho[c1 - sk];
+    cfd1 = fma_(temc * tem1, mf[1] / rf[c1], t[c1]);
+    const float temT = ((((-(kmv[ct] + kmv[ct + sk])) * (t[ct + sk] - t[ct])) * g.rdz) * mh[nz - 1]) * rho[ct];
+    cfdT = fma_(-(temc * temT), mf[nz - 1] / rf[ct], t[ct]);
+  }
   FaceTendency out{fx, fy, hm, tketen, g, i, j, zeroX, zeroY, true, rdt, {}, {}, {}};
+  if (mw) {
+    face_implicit_mw(live, (int)threadIdx.y, (int)threadIdx.x, kmh, mh, rho, mf, rf, t, base, sk, nz, tema, cfd1, cfdT,
+                     out);
+    return;
+  }
   const ColStore<LDS> CA{cA, base, sk, TE_LANE, TE_NLANES, 0}, CB{cB, base, sk, TE_LANE, TE_NLANES, nz};
   face_implicit(kmh, mh, rho, mf, rf, t, CA, CB, base, sk, nz, tema, cfd1, cfdT, out);
 }

@@ -125,10 +125,15 @@ static hipError_t launch(te_ctx* c, const Op& op) {
   if (a.n[2] == 1 && is_column_kernel(op.kid)) {
     b = dim3(column_lanes(), 1, 1);
     if (uses_column_lds(op.kid)) lds = (size_t)2 * op.nz * b.x * sizeof(float);
-    // u / v tendencies: 4 waves per 64 columns, one row per block (uv_tendency_mw)
-    static const int kU = kernel_id("turb_u_tendency_lds"), kV = kernel_id("turb_v_tendency_lds");
+    // turbulence sweeps: 4 waves per 64 columns, one row per block (the
+    // kernels' 4-wave forms, 3 nz floats of LDS per column)
+    // (turb_scalar_vert_implicit measured slower in that form and keeps one
+    // wave per 64 columns)
+    static const int kU = kernel_id("turb_u_tendency_lds"), kV = kernel_id("turb_v_tendency_lds"),
+                     kW = kernel_id("turb_w_tendency_lds"), kT = kernel_id("tke_transport_lds");
     const size_t mwLds = (size_t)3 * op.nz * 64 * sizeof(float);
-    if ((op.kid == kU || op.kid == kV) && op.nz >= 3 && mwLds <= 65536) {
+    const bool mwKernel = op.kid == kU || op.kid == kV || op.kid == kW || op.kid == kT;
+    if (mwKernel && op.nz >= 3 && mwLds <= 65536) {
       b = dim3(64, 4, 1);
       lds = mwLds;
       const dim3 g((a.n[0] + 63) / 64, a.n[1], 1);
