@@ -9,6 +9,41 @@ namespace te {
 
 // update_diagnostics_moist.wgsl:128-184
 __global__ __launch_bounds__(256) void k_update_diagnostics_moist(uint32_t* M, KArg a) {
+  {
+    const Dims g = dims(a.q[0], 0);
+    if (te_flat_levels(a, (uint32_t)(g.sJ * g.sK), [&](uint32_t w, int width, int k) {
+          if (k >= g.nz) return;
+          const float reps = RV / RD, repsm1 = reps - 1.0f;
+          const float t0 = fq(a.q[3])[k], p0 = fq(a.q[4])[k], qv0k = fq(a.q[5])[k], qc0k = fq(a.q[6])[k],
+                      qi0k = fq(a.q[7])[k], rt0 = fq(a.q[8])[k];
+          const F4 thP = ld4(a.q[1], w, width), ppP = ld4(a.q[2], w, width), qv = ld4(a.q[9], w, width),
+                   qc = ld4(a.q[10], w, width), qr = ld4(a.q[11], w, width), qi = ld4(a.q[12], w, width),
+                   qs = ld4(a.q[13], w, width), qg = ld4(a.q[14], w, width);
+          F4 thv, ppterm, piadv, thterm, buoy;
+          for (int e = 0; e < width; ++e) {
+            const float qvv = mx(qv.v[e], 0.0f);
+            const float liq = qc.v[e] + qr.v[e];
+            const float ice = (qi.v[e] + qs.v[e]) + qg.v[e];
+            const float qli = mx(0.0f, liq + ice);
+            const float cpli = fma_(CPL, mx(0.0f, liq), CPI * mx(0.0f, ice));
+            const float th = t0 + thP.v[e];
+            const float pi = p0 + ppP.v[e];
+            const float cpm = fma_(CPV, qvv, CP) + cpli;
+            const float cvm = 1.0f / (fma_(CVV, qvv, CV) + cpli);
+            thv.v[e] = (th * fma_(reps, qvv, 1.0f)) / ((1.0f + qvv) + qli);
+            piadv.v[e] = pi;
+            ppterm.v[e] = ((pi * ROVCP) * cpm) * cvm;
+            thterm.v[e] = (th * fma_(-ROVCP, cpm, fma_(RV, qvv, RD))) * cvm;
+            buoy.v[e] = GRAV * (fma_(thP.v[e], rt0, repsm1 * (qvv - qv0k)) - ((qli - qc0k) - qi0k));
+          }
+          st4(a.q[15], w, width, thv);
+          st4(a.q[16], w, width, ppterm);
+          st4(a.q[17], w, width, piadv);
+          st4(a.q[18], w, width, thterm);
+          st4(a.q[19], w, width, buoy);
+        }))
+      return;
+  }
   TE_TID3
   Dims g = dims(a.q[0], 0);
   int i = gx - 3, j = gy - 3, k = gz;

@@ -279,6 +279,7 @@ static int ensure_guard(te_ctx* c) {
       {"massflux_from_state", {1, 2, 3, 6, 7, 8}},
       {"velocity_to_massflux", {3, 4, 5}},
       {"build_sadv", {2, 3}},
+      {"update_diagnostics_moist", {1, 2, 9, 10, 11, 12, 13, 14, 15, 16, 17, 18, 19}},
       {"_copy", {0, 1}}};
   auto aligned = [](const Op& op, const std::vector<int>& ps) {
     for (int q : ps)
