@@ -186,6 +186,46 @@ uint64_t te_ensemble_points(const te_ensemble* ens);        /* interior points s
 const char* te_ensemble_last_error(const te_ensemble* ens);
 void te_ensemble_destroy(te_ensemble* ens);
 
+/* ---- adjoint of the large step.  No counterpart in the reference (its step
+ * is forward only).  The adjoint of the reference's dispatch list: every
+ * dispatch of the step has an adjoint launch over its thread grid that adds
+ * J^T ybar of that dispatch to an adjoint memory laid out like M (one float
+ * per word); acoustic_w_solve's adjoint solves the transposed tridiagonal
+ * system per column.
+ *
+ * te_adjoint_create(ctx, every): ctx is a config context (fused or not); the
+ *   adjoint builds the same configuration's unfused dispatch list on memory of
+ *   its own (same profiles and FieldArena offsets) and a zeroed adjoint memory.
+ *   Unsupported kernels (open boundaries, turbulence, microphysics) make it
+ *   fail with TE_ERR_UNSUPPORTED naming the kernel.
+ * te_fwd_ckpt(adj, n): runs n large steps of ctx (its fast path), copying the
+ *   state (profiles + arena) to a checkpoint before every `every`-th step.
+ * te_adj_upload / te_adj_download / te_adj_zero: the adjoint memory (seed it
+ *   with dJ/d(state after the n steps), read dJ/d(state before them)).
+ * te_bwd(adj): the reverse sweep over the n recorded steps: per checkpoint
+ *   segment, the step-start states are recomputed from the checkpoint, and
+ *   each step, last first, is rerun dispatch by dispatch with the regions each
+ *   dispatch writes saved; then, dispatch by dispatch from the last, those
+ *   regions are restored and the dispatch's adjoint launched.  The adjoint
+ *   memory then holds dJ/d(state at the first recorded step).
+ * te_adj_ops(adj, first, last): adjoint launches of dispatches [first, last]
+ *   of the unfused list, last first, on the adjoint's memory as it stands (no
+ *   restores): one dispatch's J^T for tests.
+ * te_adjoint_program(adj): the unfused context (borrowed) whose memory the
+ *   adjoint launches read. */
+typedef struct te_adjoint te_adjoint;
+int te_adjoint_create(te_ctx* ctx, int32_t every, te_adjoint** out);
+te_ctx* te_adjoint_program(te_adjoint* adj);
+int te_fwd_ckpt(te_adjoint* adj, int32_t nsteps);
+int te_bwd(te_adjoint* adj);
+int te_adj_ops(te_adjoint* adj, int32_t first_op, int32_t last_op);
+int te_adj_upload(te_adjoint* adj, uint64_t word_off, const float* src, uint64_t nwords);
+int te_adj_download(te_adjoint* adj, uint64_t word_off, float* dst, uint64_t nwords);
+int te_adj_zero(te_adjoint* adj);
+int te_adj_last_ms(te_adjoint* adj, float* fwd_ms, float* bwd_ms);
+const char* te_adjoint_last_error(const te_adjoint* adj);
+void te_adjoint_destroy(te_adjoint* adj);
+
 #ifdef __cplusplus
 }
 #endif

@@ -22,19 +22,20 @@ namespace orc {
 #include "k_turb.inc"
 #include "k_mass.inc"
 
-#define KE(n) {#n, &n<float>, &n<double>}
+#define KE(n) {#n, &n<float>, &n<double>, nullptr}
+#define KT(n) {#n, &n<float>, &n<double>, &n<TL>}
 static const KernelEntry kTable[] = {
-    KE(update_diagnostics_moist), KE(update_diagnostics), KE(stage_tendency_reset), KE(stage_state_reset),
-    KE(massflux_from_state), KE(velocity_to_massflux), KE(div_massflux), KE(divergence3d), KE(buoyancy_apply),
-    KE(rk_update), KE(build_sadv), KE(apply_thterm), KE(turb_add_tendency), KE(turb_zero), KE(tke_advance),
-    KE(ppx_finish), KE(diabatic_unapply), KE(diabatic_rates), KE(pressure_tendency), KE(update_rho_prs),
-    KE(adv_flux_x), KE(adv_flux_y), KE(adv_flux_z), KE(adv_flux_x_weno), KE(adv_flux_y_weno), KE(adv_flux_z_weno),
-    KE(adv_flux_x_stag), KE(adv_flux_y_stag), KE(adv_flux_z_stag), KE(adv_flux_x_weno_w), KE(adv_flux_y_weno_w),
-    KE(adv_flux_z_weno_w), KE(adv_flux_x_weno_uv), KE(adv_flux_y_weno_uv), KE(adv_flux_z_weno_uv), KE(adv_assemble),
-    KE(acoustic_prepare), KE(acoustic_begin), KE(acoustic_uv), KE(acoustic_fpk), KE(acoustic_w_solve), KE(acoustic_pp),
-    KE(acoustic_accumulate), KE(pdef_dum), KE(pdef_apply), KE(bc_periodic_x), KE(bc_periodic_y), KE(bc_open_x),
+    KT(update_diagnostics_moist), KT(update_diagnostics), KT(stage_tendency_reset), KT(stage_state_reset),
+    KT(massflux_from_state), KT(velocity_to_massflux), KT(div_massflux), KT(divergence3d), KT(buoyancy_apply),
+    KT(rk_update), KT(build_sadv), KT(apply_thterm), KT(turb_add_tendency), KT(turb_zero), KE(tke_advance),
+    KT(ppx_finish), KT(diabatic_unapply), KT(diabatic_rates), KT(pressure_tendency), KT(update_rho_prs),
+    KT(adv_flux_x), KT(adv_flux_y), KT(adv_flux_z), KT(adv_flux_x_weno), KT(adv_flux_y_weno), KT(adv_flux_z_weno),
+    KT(adv_flux_x_stag), KT(adv_flux_y_stag), KT(adv_flux_z_stag), KT(adv_flux_x_weno_w), KT(adv_flux_y_weno_w),
+    KT(adv_flux_z_weno_w), KT(adv_flux_x_weno_uv), KT(adv_flux_y_weno_uv), KT(adv_flux_z_weno_uv), KT(adv_assemble),
+    KT(acoustic_prepare), KT(acoustic_begin), KT(acoustic_uv), KT(acoustic_fpk), KT(acoustic_w_solve), KT(acoustic_pp),
+    KT(acoustic_accumulate), KT(pdef_dum), KT(pdef_apply), KT(bc_periodic_x), KT(bc_periodic_y), KE(bc_open_x),
     KE(bc_open_y), KE(openbc_uv_tendency), KE(radbc_ew), KE(radbc_ns), KE(turb_openbc_flux),
-    KE(turb_openbc_stress_x), KE(turb_openbc_stress_y), KE(rayleigh_damping), KE(turb_rho_faces),
+    KE(turb_openbc_stress_x), KE(turb_openbc_stress_y), KT(rayleigh_damping), KE(turb_rho_faces),
     KE(turb_def_centers), KE(turb_def_faces), KE(turb_def_assemble), KE(turb_nm), KE(turb_km_kh),
     KE(turb_dissipative_heating), KE(turb_gettau), KE(turb_sfcstress), KE(tke_epst), KE(tke_sources),
     KE(turb_u_tendency), KE(turb_v_tendency), KE(turb_w_tendency), KE(turb_scalar_flux_h),
@@ -42,6 +43,7 @@ static const KernelEntry kTable[] = {
     KE(mass_reference_init), KE(mass_correction_apply),
 };
 #undef KE
+#undef KT
 
 int kernel_count() { return (int)(sizeof(kTable) / sizeof(kTable[0])); }
 const KernelEntry* kernel_at(int i) { return &kTable[i]; }
@@ -49,6 +51,17 @@ const KernelEntry* find_kernel(const std::string& name) {
   for (const auto& e : kTable)
     if (name == e.name) return &e;
   return nullptr;
+}
+
+inline void call(const KernelEntry* e, const K<float>& k) { e->f32(k); }
+inline void call(const KernelEntry* e, const K<double>& k) { e->f64(k); }
+inline void call(const KernelEntry* e, const K<TL>& k) {
+  static thread_local std::string msg;
+  if (!e->ftl) {
+    msg = std::string("oracle: no tangent-linear build of kernel '") + e->name + "'";
+    throw msg.c_str();
+  }
+  e->ftl(k);
 }
 
 template <class T>
@@ -79,8 +92,7 @@ static int run_program(uint32_t* w, T* v, const char* const* names, const uint32
         Ax[5] = Ay[5] = A[5 + q];
         Ax[34] = Ay[34] = A[34];
         K<T> kx{w, v, Ax}, ky{w, v, Ay};
-        if constexpr (sizeof(T) == 4) { ex->f32(kx); ey->f32(ky); }
-        else { ex->f64(kx); ey->f64(ky); }
+        call(ex, kx); call(ey, ky);
       }
       continue;
     }
@@ -99,8 +111,7 @@ static int run_program(uint32_t* w, T* v, const char* const* names, const uint32
       for (int q = 0; q < 7; ++q) Y[4 + 2 + q] = A[4 + as[q]];
       const KernelEntry *e1 = find_kernel("acoustic_pp"), *e2 = find_kernel("acoustic_accumulate");
       K<T> k1{w, v, X}, k2{w, v, Y};
-      if constexpr (sizeof(T) == 4) { e1->f32(k1); e2->f32(k2); }
-      else { e1->f64(k1); e2->f64(k2); }
+      call(e1, k1); call(e2, k2);
       continue;
     }
     if (nm == "adv_fused") {
@@ -140,8 +151,7 @@ static int run_program(uint32_t* w, T* v, const char* const* names, const uint32
         for (auto& pr : sl.p) X[4 + pr.first] = pr.second;
         const KernelEntry* e1 = find_kernel(sl.k);
         K<T> k1{w, v, X};
-        if constexpr (sizeof(T) == 4) e1->f32(k1);
-        else e1->f64(k1);
+        call(e1, k1);
       }
       if (w[A[4 + 32]] & 16u) {  // turb_zero of the accumulating sink
         uint32_t Z[64] = {0};
@@ -149,16 +159,14 @@ static int run_program(uint32_t* w, T* v, const char* const* names, const uint32
         Z[4] = A[4]; Z[5] = A[4 + 11]; Z[34] = A[34];
         const KernelEntry* ez = find_kernel("turb_zero");
         K<T> kz{w, v, Z};
-        if constexpr (sizeof(T) == 4) ez->f32(kz);
-        else ez->f64(kz);
+        call(ez, kz);
       }
       uint32_t Y[64];
       std::memcpy(Y, A, sizeof(Y));
       Y[4 + 3] = FX; Y[4 + 4] = FY; Y[4 + 5] = FZ;
       const KernelEntry* ea = find_kernel("adv_assemble");
       K<T> ka{w, v, Y};
-      if constexpr (sizeof(T) == 4) ea->f32(ka);
-      else ea->f64(ka);
+      call(ea, ka);
       // the scalar follow-ups chained into the launch, as their own dispatches
       const uint32_t chain = w[A[4 + 32]], TEND = A[4 + 11];
       auto run1 = [&](const char* kn, const uint32_t* g3, std::vector<std::pair<int, uint32_t>> ps) {
@@ -169,8 +177,7 @@ static int run_program(uint32_t* w, T* v, const char* const* names, const uint32
         for (auto& pr : ps) X[4 + pr.first] = pr.second;
         const KernelEntry* e1 = find_kernel(kn);
         K<T> k1{w, v, X};
-        if constexpr (sizeof(T) == 4) e1->f32(k1);
-        else e1->f64(k1);
+        call(e1, k1);
       };
       const uint32_t full[3] = {nx, ny, nz}, whole[3] = {nx + 7, ny + 7, nz + 1}, asz[3] = {A[0], A[1], A[2]};
       if (chain & 1) run1("apply_thterm", full, {{1, A[4 + 33]}, {2, A[4 + 34]}, {3, TEND}});
@@ -184,16 +191,14 @@ static int run_program(uint32_t* w, T* v, const char* const* names, const uint32
       uint32_t R[64] = {0};
       R[0] = A[0]; R[1] = A[1]; R[2] = A[2]; R[4] = A[4]; R[5] = A[5]; R[6] = A[6]; R[7] = A[7]; R[34] = A[34];
       K<T> kr{w, v, R};
-      if constexpr (sizeof(T) == 4) er->f32(kr);
-      else er->f64(kr);
+      call(er, kr);
       for (int q = 0; q < 3; ++q) {
         uint32_t X[64] = {0};
         X[0] = A[0]; X[1] = A[1]; X[2] = A[2]; X[4] = A[4];
         X[5] = A[4 + 4 + q]; X[6] = A[4 + 1 + q]; X[7] = A[4 + 7];
         X[34] = A[34];
         K<T> kx{w, v, X};
-        if constexpr (sizeof(T) == 4) ea->f32(kx);
-        else ea->f64(kx);
+        call(ea, kx);
       }
       continue;
     }
@@ -207,8 +212,7 @@ static int run_program(uint32_t* w, T* v, const char* const* names, const uint32
         X[5] = A[5 + q]; X[6] = A[9 + q]; X[7] = A[13 + q];
         X[34] = A[34];
         K<T> kx{w, v, X};
-        if constexpr (sizeof(T) == 4) ea->f32(kx);
-        else ea->f64(kx);
+        call(ea, kx);
       }
       continue;
     }
@@ -222,8 +226,7 @@ static int run_program(uint32_t* w, T* v, const char* const* names, const uint32
         Az[5] = A[5 + q];
         Az[34] = A[34];
         K<T> kz{w, v, Az};
-        if constexpr (sizeof(T) == 4) ez->f32(kz);
-        else ez->f64(kz);
+        call(ez, kz);
       }
       continue;
     }
@@ -245,8 +248,7 @@ static int run_program(uint32_t* w, T* v, const char* const* names, const uint32
       return -1;
     }
     K<T> k{w, v, A};
-    if constexpr (sizeof(T) == 4) e->f32(k);
-    else e->f64(k);
+    call(e, k);
   }
   return 0;
 }
@@ -295,6 +297,20 @@ int oracle_run_f64(uint32_t* mem, double* vals, const char* const* names, const 
                    int last, char* err, int errlen) {
   try {
     return orc::run_program<double>(mem, vals, names, args, nops, first, last, err, errlen);
+  } catch (const char* msg) {
+    std::snprintf(err, errlen, "%s", msg);
+    return -2;
+  }
+}
+
+// FP64 tangent-linear build: `vals` holds (value, tangent) double pairs of
+// every word (same indexing); the tangents after the ops are J times those
+// before.
+int oracle_run_tl(uint32_t* mem, double* vals, const char* const* names, const uint32_t* args, int nops, int first,
+                  int last, char* err, int errlen) {
+  try {
+    return orc::run_program<orc::TL>(mem, reinterpret_cast<orc::TL*>(vals), names, args, nops, first, last, err,
+                                      errlen);
   } catch (const char* msg) {
     std::snprintf(err, errlen, "%s", msg);
     return -2;

@@ -77,6 +77,41 @@ template <class T> inline T mx(T a, T b) { return (a < b || a != a) ? b : a; }  
 template <class T> inline T mn(T a, T b) { return (b < a || a != a) ? b : a; }   // WGSL min
 template <class T> inline T ab(T a) { return std::fabs(a); }
 
+// ---- FP64 tangent-linear build: T = TL carries (value, tangent) through
+// every operation of a kernel; branch decisions use the double values.  The
+// tangent of an output is the kernel's Jacobian applied to the input tangents
+// (the reference for the adjoint's transposed Jacobians).
+struct TL {
+  double v, t;
+  TL() : v(0.0), t(0.0) {}
+  TL(double x) : v(x), t(0.0) {}
+  TL(double x, double dx) : v(x), t(dx) {}
+};
+inline TL operator+(TL a, TL b) { return TL(a.v + b.v, a.t + b.t); }
+inline TL operator-(TL a, TL b) { return TL(a.v - b.v, a.t - b.t); }
+inline TL operator-(TL a) { return TL(-a.v, -a.t); }
+inline TL operator*(TL a, TL b) { return TL(a.v * b.v, a.t * b.v + a.v * b.t); }
+inline TL operator/(TL a, TL b) {
+  const double q = a.v / b.v;
+  return TL(q, (a.t - q * b.t) / b.v);
+}
+inline bool operator<(TL a, TL b) { return a.v < b.v; }
+inline bool operator>(TL a, TL b) { return a.v > b.v; }
+inline bool operator<=(TL a, TL b) { return a.v <= b.v; }
+inline bool operator>=(TL a, TL b) { return a.v >= b.v; }
+inline bool operator==(TL a, TL b) { return a.v == b.v; }
+inline bool operator!=(TL a, TL b) { return a.v != b.v; }
+inline TL fma_(TL a, TL b, TL c) { return TL(std::fma(a.v, b.v, c.v), a.t * b.v + a.v * b.t + c.t); }
+inline TL ab(TL a) { return a.v < 0.0 ? TL(-a.v, -a.t) : a; }
+inline TL copysign_mag(TL mag, TL sgn) {
+  const TL m = ab(mag);
+  return std::signbit(sgn.v) ? TL(-m.v, -m.t) : m;
+}
+inline TL rr_pow(TL x, TL p) {
+  const double y = std::pow(x.v, p.v);
+  return TL(y, x.t * (p.v * y / x.v));
+}
+
 // Metric buffer hmetric (grid.js:200-214): uh | uf | vh | vf | xf | yf | axi...
 template <class T> inline T hm_uh(const T* hm, const Dims<T>& g, int i) { return hm[i + 3]; }
 template <class T> inline T hm_uf(const T* hm, const Dims<T>& g, int i) { return hm[g.sJ + i + 3]; }
@@ -128,6 +163,7 @@ struct KernelEntry {
   const char* name;
   KernelFn<float> f32;
   KernelFn<double> f64;
+  KernelFn<TL> ftl;  // null: no tangent-linear build of this kernel
 };
 const KernelEntry* find_kernel(const std::string& name);
 int kernel_count();

@@ -1,0 +1,76 @@
+// Host build of the library's adjoint bodies — TEST INFRASTRUCTURE ONLY.
+//
+// Compiles thunderengine-web_amd/csrc/adj_bodies.hpp (the same per-point and
+// per-column adjoint bodies the gfx950 kernels of k_adj.hip run) for the CPU
+// and runs one dispatch's adjoint serially over its thread grid, so the CPU
+// suite can check every adjoint against the oracle's FP64 tangent-linear
+// build without a GPU.  The GPU tests check the device launches the same way.
+#include <cstring>
+#include <xmmintrin.h>
+
+#include "../thunderengine-web_amd/csrc/adj_bodies.hpp"
+
+using namespace te::adj;
+
+namespace {
+
+template <class B>
+void grid(const float* M, float* Mb, const AdjArg& a) {
+  for (uint32_t z = 0; z < a.n[2]; ++z)
+    for (uint32_t y = 0; y < a.n[1]; ++y)
+      for (uint32_t x = 0; x < a.n[0]; ++x) adj_point<B>(M, Mb, a, (int)x, (int)y, (int)z);
+}
+
+void columns(const float* M, float* Mb, const AdjArg& a) {
+  for (uint32_t y = 0; y < a.n[1]; ++y)
+    for (uint32_t x = 0; x < a.n[0]; ++x) w_solve_column(M, Mb, a, (int)x, (int)y);
+}
+
+struct HostEntry {
+  const char* name;
+  void (*fn)(const float*, float*, const AdjArg&);
+  int nw;
+  int writes[8];
+};
+
+#define HK(n, B, nw, ...) {n, &grid<B>, nw, __VA_ARGS__},
+const HostEntry kHost[] = {
+    TE_ADJ_POINT_KERNELS(HK)
+    {"acoustic_w_solve", &columns, 3, {9, 10, 11}},
+};
+#undef HK
+
+}  // namespace
+
+extern "C" {
+
+// Adjoint of one dispatch (kernel `name`, 64-word slot A) on value memory M
+// and adjoint memory Mb (word 0 of the program memory).  Returns -1 for a
+// kernel without an adjoint.
+int adjhost_op(const char* name, const uint32_t* A, const float* M, float* Mb) {
+  _mm_setcsr(_mm_getcsr() | 0x8040u);  // flush subnormals, as the GPU build
+  for (const HostEntry& e : kHost)
+    if (std::strcmp(e.name, name) == 0) {
+      AdjArg a;
+      a.n[0] = A[0]; a.n[1] = A[1]; a.n[2] = A[2];
+      for (int q = 0; q < 32; ++q) a.p[q] = A[4 + q];
+      if (std::strcmp(name, "_copy") == 0) a.n[1] = a.n[2] = 1;
+      e.fn(M, Mb, a);
+      return 0;
+    }
+  return -1;
+}
+
+// Parameters kernel `name` writes (at most 8), or -1.
+int adjhost_writes(const char* name, int* out) {
+  for (const HostEntry& e : kHost)
+    if (std::strcmp(e.name, name) == 0) {
+      for (int q = 0; q < e.nw; ++q) out[q] = e.writes[q];
+      return e.nw;
+    }
+  return -1;
+}
+
+int adjhost_count() { return (int)(sizeof(kHost) / sizeof(kHost[0])); }
+const char* adjhost_name(int i) { return kHost[i].name; }
+}

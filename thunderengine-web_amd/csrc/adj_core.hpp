@@ -1,0 +1,202 @@
+// Reverse-mode building blocks for the adjoint of the large step.
+//
+// The adjoint of a reference kernel is built from a per-point body that
+// restates the kernel's arithmetic on dual numbers D1 (value, derivative).
+// For one point of the kernel's thread grid, adj_point() runs the body
+//   * once to find the point's outputs (their adjoints are read and then
+//     cleared: the kernel overwrites them) and to count its active loads, then
+//   * once per active load s, with that load seeded (derivative 1): each
+//     output y then carries dy/dx_s, and sum_o ybar_o * dy_o/dx_s is added to
+//     the adjoint of the loaded word x_s.
+// This is the transposed local Jacobian of the point, so a launch over the
+// kernel's grid adds J^T ybar to the adjoint memory.  Loads of profiles, the
+// metric buffer and parameter words are passive (no derivative).  Branch
+// decisions are taken on the body's own intermediates, formed from the forward
+// state's words, so the derivative is that of the branch the forward step took
+// (up to the rounding of those intermediates right at a switch).
+//
+// The header compiles for gfx950 (atomic adds into the adjoint memory) and for
+// the host (plain adds), so the same bodies can be exercised on a CPU.
+#pragma once
+#include <stdint.h>
+#ifdef __HIPCC__
+#include <hip/hip_runtime.h>
+#define TE_HD __host__ __device__ __forceinline__
+#else
+#include <cmath>
+#define TE_HD inline
+#endif
+
+namespace te {
+namespace adj {
+
+// ---- dual numbers.  Values and derivatives are carried in binary64: the
+// local Jacobians of the WENO weights divide by smoothness indicators formed
+// from differences of nearly equal values, where binary32 intermediates would
+// put the linearisation point off by ~1e-3 relative.  The inputs are the
+// binary32 words of the forward state, so branch decisions follow the values
+// the forward step computed from, up to rounding of the intermediates.
+struct D1 {
+  double v, d;
+  TE_HD D1() : v(0.0), d(0.0) {}
+  TE_HD D1(double x) : v(x), d(0.0) {}
+  TE_HD D1(double x, double dx) : v(x), d(dx) {}
+};
+TE_HD double fmad_(double a, double b, double c) { return __builtin_fma(a, b, c); }
+TE_HD float fmaf_(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+TE_HD D1 operator+(D1 a, D1 b) { return D1(a.v + b.v, a.d + b.d); }
+TE_HD D1 operator-(D1 a, D1 b) { return D1(a.v - b.v, a.d - b.d); }
+TE_HD D1 operator-(D1 a) { return D1(-a.v, -a.d); }
+TE_HD D1 operator*(D1 a, D1 b) { return D1(a.v * b.v, fmad_(a.d, b.v, a.v * b.d)); }
+TE_HD D1 operator/(D1 a, D1 b) {
+  const double q = a.v / b.v;
+  return D1(q, (a.d - q * b.d) / b.v);
+}
+TE_HD bool operator<(D1 a, D1 b) { return a.v < b.v; }
+TE_HD bool operator>(D1 a, D1 b) { return a.v > b.v; }
+TE_HD bool operator<=(D1 a, D1 b) { return a.v <= b.v; }
+TE_HD bool operator>=(D1 a, D1 b) { return a.v >= b.v; }
+TE_HD bool operator!=(D1 a, D1 b) { return a.v != b.v; }
+TE_HD bool operator==(D1 a, D1 b) { return a.v == b.v; }
+// fma(a, b, c): product rule in the derivative
+TE_HD D1 fma_(D1 a, D1 b, D1 c) { return D1(fmad_(a.v, b.v, c.v), fmad_(a.d, b.v, fmad_(a.v, b.d, c.d))); }
+// WGSL max / min as selects (ties keep the first operand), as the forward kernels
+TE_HD D1 mx(D1 a, D1 b) { return (a.v < b.v || a.v != a.v) ? b : a; }
+TE_HD D1 mn(D1 a, D1 b) { return (b.v < a.v || a.v != a.v) ? b : a; }
+TE_HD D1 ab(D1 a) { return a.v < 0.0 ? D1(-a.v, -a.d) : a; }
+TE_HD bool sign_bit(double x) {
+  union { double f; uint64_t u; } q{x};
+  return (q.u >> 63) != 0u;
+}
+// |mag| with the sign of sgn
+TE_HD D1 copysign_mag(D1 mag, D1 sgn) {
+  const bool neg = sign_bit(sgn.v);
+  const D1 m = ab(mag);
+  return neg ? D1(-m.v, -m.d) : m;
+}
+// x^p for a constant p
+TE_HD D1 powc(D1 x, double p) {
+#ifdef __HIP_DEVICE_COMPILE__
+  const double y = pow(x.v, p);
+#else
+  const double y = std::pow(x.v, p);
+#endif
+  return D1(y, x.d * ((p * y) / x.v));
+}
+
+// ---- dispatch argument: thread grid and word offsets of parameters 0..31
+// (the reference's 64-word argument slot: A[0..2] grid, A[4 + n] parameter n)
+struct AdjArg {
+  uint32_t n[3];
+  uint32_t p[32];
+};
+
+// GridDims bytes (timestep.js:88-90): nx ny nz strideJ strideK 1/dx 1/dy 1/dz.
+struct ADims {
+  int nx, ny, nz, sJ, sK;
+  float rdx, rdy, rdz;
+  TE_HD int64_t sk() const { return (int64_t)sJ * sK; }
+  TE_HD int64_t idx(int i, int j, int k) const {
+    return (int64_t)(i + 3) + (int64_t)sJ * ((j + 3) + (int64_t)sK * k);
+  }
+};
+
+constexpr int MAXO = 8;  // outputs per point
+
+// Accessor handed to the bodies: memory, adjoint memory and the pass state.
+struct Acc {
+  const float* M;  // values (word 0 of the program memory)
+  float* Mb;       // adjoints (same word indexing)
+  const AdjArg* a;
+  int mode;  // 0: discovery pass, 1: seeded pass
+  int cnt, seed;
+  int64_t rec;
+  double acc;
+  double yb[MAXO];
+  int64_t oa[MAXO];
+  bool has[MAXO];
+
+  TE_HD Acc(const float* M_, float* Mb_, const AdjArg* a_)
+      : M(M_), Mb(Mb_), a(a_), mode(0), cnt(0), seed(-1), rec(-1), acc(0.0) {
+    for (int o = 0; o < MAXO; ++o) { yb[o] = 0.0; oa[o] = 0; has[o] = false; }
+  }
+  // parameter n as a value pointer / its words
+  TE_HD const float* f(int n) const { return M + a->p[n]; }
+  TE_HD uint32_t word(int n, int k = 0) const {
+    union { float f; uint32_t u; } q{M[a->p[n] + k]};
+    return q.u;
+  }
+  TE_HD int32_t I(int n, int k = 0) const { return (int32_t)word(n, k); }
+  TE_HD float F(int n, int k = 0) const { return M[a->p[n] + k]; }
+  TE_HD ADims dims() const {
+    ADims g;
+    g.nx = I(0, 0); g.ny = I(0, 1); g.nz = I(0, 2); g.sJ = I(0, 3); g.sK = I(0, 4);
+    g.rdx = F(0, 5); g.rdy = F(0, 6); g.rdz = F(0, 7);
+    return g;
+  }
+  // active load of p[c]
+  TE_HD D1 ld(const float* p, int64_t c) {
+    D1 r((double)p[c]);
+    if (mode != 0 && cnt == seed) {
+      r.d = 1.0;
+      rec = (int64_t)(p - M) + c;
+    }
+    ++cnt;
+    return r;
+  }
+  // output y to p[c]
+  template <int O>
+  TE_HD void out(const float* p, int64_t c, D1 y) {
+    static_assert(O < MAXO, "too many outputs");
+    if (mode == 0) {
+      const int64_t w = (int64_t)(p - M) + c;
+      oa[O] = w;
+      yb[O] = (double)Mb[w];
+      has[O] = true;
+    } else {
+      acc = fmad_(yb[O], y.d, acc);
+    }
+  }
+};
+
+TE_HD void adj_add(float* p, float v) {
+#ifdef __HIP_DEVICE_COMPILE__
+  unsafeAtomicAdd(p, v);
+#else
+  *p += v;
+#endif
+}
+
+// Adjoint of one point of a kernel whose per-point body is B::run(Acc&, x, y, z).
+template <class B>
+TE_HD void adj_point(const float* M, float* Mb, const AdjArg& a, int x, int y, int z) {
+  Acc r(M, Mb, &a);
+  B::run(r, x, y, z);
+  bool any = false;
+  for (int o = 0; o < MAXO; ++o) any = any || r.has[o];
+  if (!any) return;
+  const int n = r.cnt;
+  for (int o = 0; o < MAXO; ++o)
+    if (r.has[o]) Mb[r.oa[o]] = 0.0f;
+  r.mode = 1;
+  for (int s = 0; s < n; ++s) {
+    r.cnt = 0;
+    r.seed = s;
+    r.acc = 0.0;
+    r.rec = -1;
+    B::run(r, x, y, z);
+    if (r.acc != 0.0 && r.rec >= 0) adj_add(Mb + r.rec, (float)r.acc);
+  }
+}
+
+// Metric buffer (grid.js:200-214): uh | uf | vh | vf ...
+TE_HD float hm_uh(const float* hm, const ADims& g, int i) { return hm[i + 3]; }
+TE_HD float hm_uf(const float* hm, const ADims& g, int i) { return hm[g.sJ + i + 3]; }
+TE_HD float hm_vh(const float* hm, const ADims& g, int j) { return hm[2 * g.sJ + j + 3]; }
+TE_HD float hm_vf(const float* hm, const ADims& g, int j) { return hm[2 * g.sJ + g.sK + j + 3]; }
+
+constexpr float RD = 287.04f, RV = 461.5f, CP = 1005.7f, CV = 718.66003f, CPV = 1.87e+03f, CVV = 1408.5f,
+                ROVCP = 0.28541315f, CPL = 4.19e+03f, CPI = 2106.0f, P00 = 1e+05f, GRAV = 9.81f;
+
+}  // namespace adj
+}  // namespace te

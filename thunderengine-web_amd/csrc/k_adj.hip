@@ -1,0 +1,81 @@
+// Adjoint kernels of the large step for gfx950.
+//
+// Each reference kernel on the path has an adjoint launch over the same
+// thread grid: k_adj<B> runs adj_point<B> (adj_core.hpp) for every point of
+// the grid, adding the transposed local Jacobian times the output adjoints to
+// the adjoint memory with float atomics; acoustic_w_solve has a hand-written
+// column adjoint (one thread per column).  The registry lists, for every
+// kernel, the parameters it writes: the reverse sweep saves those regions
+// before the forward dispatch and restores them before the adjoint one.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+
+#include "adj_bodies.hpp"
+#include "te_adjoint.hpp"
+
+namespace te {
+namespace adj {
+
+template <class B>
+__global__ __launch_bounds__(256) void k_adj(const float* M, float* Mb, AdjArg a) {
+  const uint64_t t = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+  const uint64_t n0 = a.n[0], n01 = n0 * a.n[1];
+  if (t >= n01 * a.n[2]) return;
+  const uint64_t z = t / n01, rem = t - z * n01, y = rem / n0, x = rem - y * n0;
+  adj_point<B>(M, Mb, a, (int)x, (int)y, (int)z);
+}
+
+__global__ __launch_bounds__(64) void k_adj_w_solve(const float* M, float* Mb, AdjArg a) {
+  const uint64_t t = (uint64_t)blockIdx.x * 64u + threadIdx.x;
+  const uint64_t n0 = a.n[0];
+  if (t >= n0 * a.n[1]) return;
+  const uint64_t y = t / n0, x = t - y * n0;
+  w_solve_column(M, Mb, a, (int)x, (int)y);
+}
+
+struct Entry {
+  const char* name;
+  const void* fn;
+  int nw;
+  int writes[8];
+  bool column;
+};
+
+#define AK(n, B, nw, ...) {n, reinterpret_cast<const void*>(&k_adj<B>), nw, __VA_ARGS__, false},
+static const Entry kAdj[] = {
+    TE_ADJ_POINT_KERNELS(AK)
+    {"acoustic_w_solve", reinterpret_cast<const void*>(&k_adj_w_solve), 3, {9, 10, 11}, true},
+};
+#undef AK
+
+int find(const char* name) {
+  for (int i = 0; i < (int)(sizeof(kAdj) / sizeof(kAdj[0])); ++i)
+    if (std::strcmp(kAdj[i].name, name) == 0) return i;
+  return -1;
+}
+
+int writes(int id, int out[8]) {
+  for (int q = 0; q < kAdj[id].nw; ++q) out[q] = kAdj[id].writes[q];
+  return kAdj[id].nw;
+}
+
+hipError_t launch(int id, const uint32_t* A, const float* M, float* Mb, hipStream_t stream) {
+  const Entry& e = kAdj[id];
+  AdjArg a;
+  a.n[0] = A[0]; a.n[1] = A[1]; a.n[2] = A[2];
+  for (int q = 0; q < 32; ++q) a.p[q] = A[4 + q];
+  void* args[] = {(void*)&M, (void*)&Mb, (void*)&a};
+  if (e.column) {
+    const uint64_t cols = (uint64_t)a.n[0] * a.n[1];
+    if (cols == 0) return hipSuccess;
+    return hipLaunchKernel(e.fn, dim3((unsigned)((cols + 63) / 64)), dim3(64), args, 0, stream);
+  }
+  const uint64_t n = (uint64_t)a.n[0] * a.n[1] * a.n[2];
+  if (n == 0) return hipSuccess;
+  if ((n + 255) / 256 > 0x7fffffffull) return hipErrorInvalidConfiguration;
+  return hipLaunchKernel(e.fn, dim3((unsigned)((n + 255) / 256)), dim3(256), args, 0, stream);
+}
+
+}  // namespace adj
+}  // namespace te

@@ -1,0 +1,264 @@
+// Adjoint of the large step: checkpointed forward, recomputing reverse sweep.
+//
+// Memory (one MI355X holds all of it for the grids of this path):
+//   * checkpoints: the state words [0, arena end) before every `every`-th step;
+//   * step states: the states at the starts of one segment's steps;
+//   * the write log of one step: every region each dispatch writes, saved
+//     before the dispatch (a whole FieldArena slot per written field);
+//   * the adjoint memory: one float per word of M, with M's guards.
+// The reverse of one step restores, dispatch by dispatch from the last, the
+// regions that dispatch wrote, so the memory holds exactly the dispatch's
+// inputs when its adjoint runs.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/te.h"
+#include "te_adjoint.hpp"
+#include "te_internal.hpp"
+
+struct te_adjoint {
+  te_ctx* fwd = nullptr;  // the caller's context
+  te_ctx* rec = nullptr;  // unfused dispatch list on memory of its own (owned)
+  int every = 1;
+  uint64_t state_words = 0;  // [0, arena end)
+  float* dMbBase = nullptr;
+  float* dMb = nullptr;
+  std::vector<uint32_t*> ckpt;
+  int nsteps = 0;
+  uint32_t* dSteps = nullptr;  // `every` step-start states
+  uint32_t* dLog = nullptr;
+  uint64_t log_words = 0;
+  std::vector<int> adj_id;                                        // per dispatch
+  std::vector<std::vector<std::pair<uint64_t, uint64_t>>> wreg;  // (offset, words) per dispatch
+  hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr, e3 = nullptr;
+  float fwd_ms = 0.0f, bwd_ms = 0.0f;
+  std::string err;
+};
+
+static int afail(te_adjoint* a, int code, const std::string& msg) {
+  if (a) a->err = msg;
+  return code;
+}
+#define TA_HIP(call, what)                                                                     \
+  do {                                                                                         \
+    hipError_t e_ = (call);                                                                    \
+    if (e_ != hipSuccess) return afail(a, TE_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+static int run_rec_ops(te_adjoint* a, int first, int last) {
+  int rc = te_run(a->rec, 1, first, last);
+  if (rc != TE_OK) return afail(a, rc, std::string("forward dispatches: ") + te_last_error(a->rec));
+  return TE_OK;
+}
+
+// One logged forward step on rec's memory, then its reverse.
+static int reverse_step(te_adjoint* a) {
+  te_ctx* r = a->rec;
+  const int n = (int)r->ops.size();
+  std::vector<uint64_t> at(n + 1, 0);
+  uint64_t pos = 0;
+  for (int i = 0; i < n; ++i) {
+    at[i] = pos;
+    for (auto& w : a->wreg[i]) {
+      TA_HIP(hipMemcpyAsync(a->dLog + pos, r->dM + w.first, w.second * 4, hipMemcpyDeviceToDevice, r->stream), "log");
+      pos += w.second;
+    }
+    int rc = run_rec_ops(a, i, i);
+    if (rc != TE_OK) return rc;
+  }
+  for (int i = n - 1; i >= 0; --i) {
+    uint64_t p = at[i];
+    for (auto& w : a->wreg[i]) {
+      TA_HIP(hipMemcpyAsync(r->dM + w.first, a->dLog + p, w.second * 4, hipMemcpyDeviceToDevice, r->stream), "restore");
+      p += w.second;
+    }
+    TA_HIP(te::adj::launch(a->adj_id[i], r->ops[i].A, reinterpret_cast<const float*>(r->dM), a->dMb, r->stream),
+           std::string("adjoint of ") + r->op_names[i]);
+  }
+  return TE_OK;
+}
+
+extern "C" {
+
+int te_adjoint_create(te_ctx* ctx, int32_t every, te_adjoint** out) {
+  if (!out) return TE_ERR_ARG;
+  *out = nullptr;
+  te_adjoint* a = new te_adjoint();
+  *out = a;
+  if (!ctx || !ctx->has_config || !ctx->dM) return afail(a, TE_ERR_ARG, "needs a config context with device memory");
+  if (every < 1) return afail(a, TE_ERR_ARG, "checkpoint interval must be at least 1");
+  a->fwd = ctx;
+  a->every = every;
+  te_config cfg = ctx->cfg;
+  cfg.fuse = 0;
+  int rc = te_create(&cfg, &a->rec);
+  if (rc != TE_OK) return afail(a, rc, std::string("unfused program: ") + (a->rec ? te_last_error(a->rec) : ""));
+  te_ctx* r = a->rec;
+  const te::Layout &L = r->layout, &F = ctx->layout;
+  if (L.arena != F.arena || L.field_words != F.field_words || L.fields != F.fields)
+    return afail(a, TE_ERR_STATE, "fused and unfused layouts differ");
+  a->state_words = L.arena + L.field_words * L.fields.size();
+  const uint64_t arena_end = a->state_words;
+  // per dispatch: its adjoint and the regions it writes
+  const int n = (int)r->ops.size();
+  a->adj_id.resize(n);
+  a->wreg.resize(n);
+  for (int i = 0; i < n; ++i) {
+    const std::string& nm = r->op_names[i];
+    const int id = te::adj::find(nm.c_str());
+    if (id < 0) return afail(a, TE_ERR_UNSUPPORTED, "no adjoint for kernel '" + nm + "'");
+    a->adj_id[i] = id;
+    const uint32_t* A = r->ops[i].A;
+    if (nm == "rayleigh_damping" && (r->image[A[4 + 17]] != 0u || r->image[A[4 + 16]] != 0u))
+      return afail(a, TE_ERR_UNSUPPORTED, "no adjoint for Rayleigh damping toward horizontal means or at lateral edges");
+    if (nm == "_copy") {
+      a->wreg[i].push_back({A[5], A[0]});
+      continue;
+    }
+    int wp[8];
+    const int nw = te::adj::writes(id, wp);
+    for (int q = 0; q < nw; ++q) {
+      const uint64_t off = A[4 + wp[q]];
+      if (nm == "acoustic_pp" && wp[q] == 9 && r->image[A[4 + 10]] == 0u) continue;  // ppx unused
+      if (off < L.arena || off >= arena_end || (off - L.arena) % L.field_words != 0)
+        return afail(a, TE_ERR_UNSUPPORTED, "dispatch " + std::to_string(i) + " (" + nm + ") writes outside the FieldArena");
+      bool dup = false;
+      for (auto& w : a->wreg[i]) dup = dup || w.first == off;
+      if (!dup) a->wreg[i].push_back({off, L.field_words});
+    }
+  }
+  for (auto& v : a->wreg)
+    for (auto& w : v) a->log_words += w.second;
+  hipStream_t s = r->stream;
+  const uint64_t mbw = r->front + r->words;
+  TA_HIP(hipMalloc((void**)&a->dMbBase, mbw * 4), "hipMalloc(adjoint memory)");
+  TA_HIP(hipMemsetAsync(a->dMbBase, 0, mbw * 4, s), "clear adjoint memory");
+  a->dMb = a->dMbBase + r->front;
+  TA_HIP(hipMalloc((void**)&a->dLog, std::max<uint64_t>(a->log_words, 1) * 4), "hipMalloc(write log)");
+  TA_HIP(hipMalloc((void**)&a->dSteps, (uint64_t)every * a->state_words * 4), "hipMalloc(step states)");
+  TA_HIP(hipEventCreate(&a->e0), "event");
+  TA_HIP(hipEventCreate(&a->e1), "event");
+  TA_HIP(hipEventCreate(&a->e2), "event");
+  TA_HIP(hipEventCreate(&a->e3), "event");
+  TA_HIP(hipStreamSynchronize(s), "sync");
+  return TE_OK;
+}
+
+te_ctx* te_adjoint_program(te_adjoint* a) { return a ? a->rec : nullptr; }
+
+int te_fwd_ckpt(te_adjoint* a, int32_t nsteps) {
+  if (!a || !a->rec || nsteps < 0) return afail(a, TE_ERR_ARG, "bad arguments");
+  te_ctx* f = a->fwd;
+  const int nseg = (nsteps + a->every - 1) / a->every;
+  while ((int)a->ckpt.size() < nseg) {
+    uint32_t* p = nullptr;
+    TA_HIP(hipMalloc((void**)&p, a->state_words * 4), "hipMalloc(checkpoint)");
+    a->ckpt.push_back(p);
+  }
+  TA_HIP(hipEventRecord(a->e0, f->stream), "event");
+  for (int sg = 0; sg < nseg; ++sg) {
+    TA_HIP(hipMemcpyAsync(a->ckpt[sg], f->dM, a->state_words * 4, hipMemcpyDeviceToDevice, f->stream), "checkpoint");
+    const int k = std::min(a->every, nsteps - sg * a->every);
+    int rc = te_run(f, k, 0, -1);
+    if (rc != TE_OK) return afail(a, rc, std::string("forward: ") + te_last_error(f));
+  }
+  TA_HIP(hipEventRecord(a->e1, f->stream), "event");
+  TA_HIP(hipEventSynchronize(a->e1), "sync");
+  TA_HIP(hipEventElapsedTime(&a->fwd_ms, a->e0, a->e1), "elapsed");
+  a->nsteps = nsteps;
+  return TE_OK;
+}
+
+int te_bwd(te_adjoint* a) {
+  if (!a || !a->rec) return afail(a, TE_ERR_ARG, "bad arguments");
+  if (a->nsteps <= 0) return afail(a, TE_ERR_STATE, "no recorded steps (te_fwd_ckpt)");
+  te_ctx* r = a->rec;
+  TA_HIP(hipStreamSynchronize(a->fwd->stream), "sync forward");
+  TA_HIP(hipEventRecord(a->e2, r->stream), "event");
+  const int nseg = (a->nsteps + a->every - 1) / a->every;
+  const uint64_t sw = a->state_words;
+  for (int sg = nseg - 1; sg >= 0; --sg) {
+    const int s0 = sg * a->every, s1 = std::min(a->nsteps, s0 + a->every);
+    TA_HIP(hipMemcpyAsync(r->dM, a->ckpt[sg], sw * 4, hipMemcpyDeviceToDevice, r->stream), "load checkpoint");
+    for (int s = s0; s < s1; ++s) {
+      TA_HIP(hipMemcpyAsync(a->dSteps + (uint64_t)(s - s0) * sw, r->dM, sw * 4, hipMemcpyDeviceToDevice, r->stream),
+             "step state");
+      if (s < s1 - 1) {
+        int rc = run_rec_ops(a, 0, (int)r->ops.size() - 1);
+        if (rc != TE_OK) return rc;
+      }
+    }
+    for (int s = s1 - 1; s >= s0; --s) {
+      TA_HIP(hipMemcpyAsync(r->dM, a->dSteps + (uint64_t)(s - s0) * sw, sw * 4, hipMemcpyDeviceToDevice, r->stream),
+             "load step state");
+      int rc = reverse_step(a);
+      if (rc != TE_OK) return rc;
+    }
+  }
+  TA_HIP(hipEventRecord(a->e3, r->stream), "event");
+  TA_HIP(hipEventSynchronize(a->e3), "sync");
+  TA_HIP(hipEventElapsedTime(&a->bwd_ms, a->e2, a->e3), "elapsed");
+  return TE_OK;
+}
+
+int te_adj_ops(te_adjoint* a, int32_t first, int32_t last) {
+  if (!a || !a->rec) return afail(a, TE_ERR_ARG, "bad arguments");
+  te_ctx* r = a->rec;
+  const int n = (int)r->ops.size();
+  if (first < 0 || last >= n || first > last) return afail(a, TE_ERR_ARG, "bad dispatch range");
+  for (int i = last; i >= first; --i)
+    TA_HIP(te::adj::launch(a->adj_id[i], r->ops[i].A, reinterpret_cast<const float*>(r->dM), a->dMb, r->stream),
+           std::string("adjoint of ") + r->op_names[i]);
+  TA_HIP(hipStreamSynchronize(r->stream), "sync");
+  return TE_OK;
+}
+
+int te_adj_upload(te_adjoint* a, uint64_t off, const float* src, uint64_t n) {
+  if (!a || !a->rec || !src || off + n > a->rec->prog_words) return afail(a, TE_ERR_ARG, "upload outside memory");
+  TA_HIP(hipMemcpyAsync(a->dMb + off, src, n * 4, hipMemcpyHostToDevice, a->rec->stream), "upload");
+  TA_HIP(hipStreamSynchronize(a->rec->stream), "sync");
+  return TE_OK;
+}
+
+int te_adj_download(te_adjoint* a, uint64_t off, float* dst, uint64_t n) {
+  if (!a || !a->rec || !dst || off + n > a->rec->prog_words) return afail(a, TE_ERR_ARG, "download outside memory");
+  TA_HIP(hipMemcpyAsync(dst, a->dMb + off, n * 4, hipMemcpyDeviceToHost, a->rec->stream), "download");
+  TA_HIP(hipStreamSynchronize(a->rec->stream), "sync");
+  return TE_OK;
+}
+
+int te_adj_zero(te_adjoint* a) {
+  if (!a || !a->rec) return afail(a, TE_ERR_ARG, "bad arguments");
+  const uint64_t mbw = a->rec->front + a->rec->words;
+  TA_HIP(hipMemsetAsync(a->dMbBase, 0, mbw * 4, a->rec->stream), "clear");
+  TA_HIP(hipStreamSynchronize(a->rec->stream), "sync");
+  return TE_OK;
+}
+
+int te_adj_last_ms(te_adjoint* a, float* fwd_ms, float* bwd_ms) {
+  if (!a) return TE_ERR_ARG;
+  if (fwd_ms) *fwd_ms = a->fwd_ms;
+  if (bwd_ms) *bwd_ms = a->bwd_ms;
+  return TE_OK;
+}
+
+const char* te_adjoint_last_error(const te_adjoint* a) { return a ? a->err.c_str() : "null adjoint"; }
+
+void te_adjoint_destroy(te_adjoint* a) {
+  if (!a) return;
+  if (a->rec && a->rec->stream) (void)hipStreamSynchronize(a->rec->stream);
+  for (uint32_t* p : a->ckpt) (void)hipFree(p);
+  if (a->dMbBase) (void)hipFree(a->dMbBase);
+  if (a->dLog) (void)hipFree(a->dLog);
+  if (a->dSteps) (void)hipFree(a->dSteps);
+  for (hipEvent_t e : {a->e0, a->e1, a->e2, a->e3})
+    if (e) (void)hipEventDestroy(e);
+  if (a->rec) te_destroy(a->rec);
+  delete a;
+}
+
+}  // extern "C"

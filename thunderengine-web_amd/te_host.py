@@ -103,6 +103,17 @@ def lib():
         "te_ensemble_points": ([ctx], u64),
         "te_ensemble_last_error": ([ctx], ctypes.c_char_p),
         "te_ensemble_destroy": ([ctx], None),
+        "te_adjoint_create": ([ctx, ctypes.c_int32, P(ctx)], ctypes.c_int),
+        "te_adjoint_program": ([ctx], ctx),
+        "te_fwd_ckpt": ([ctx, ctypes.c_int32], ctypes.c_int),
+        "te_bwd": ([ctx], ctypes.c_int),
+        "te_adj_ops": ([ctx, ctypes.c_int32, ctypes.c_int32], ctypes.c_int),
+        "te_adj_upload": ([ctx, u64, P(ctypes.c_float), u64], ctypes.c_int),
+        "te_adj_download": ([ctx, u64, P(ctypes.c_float), u64], ctypes.c_int),
+        "te_adj_zero": ([ctx], ctypes.c_int),
+        "te_adj_last_ms": ([ctx, P(ctypes.c_float), P(ctypes.c_float)], ctypes.c_int),
+        "te_adjoint_last_error": ([ctx], ctypes.c_char_p),
+        "te_adjoint_destroy": ([ctx], None),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -321,3 +332,76 @@ class Ensemble:
             self.close()
         except Exception:
             pass
+
+
+class Adjoint:
+    """Adjoint of the large step over a config context (te_adjoint_*): a
+    checkpointed forward (te_fwd_ckpt) and the reverse sweep (te_bwd) over the
+    recorded steps.  The adjoint memory has M's word layout, one float per
+    word: seed it with dJ/d(final state), read dJ/d(initial state)."""
+
+    def __init__(self, run, every=10):
+        self.run = run
+        self.h = ctypes.c_void_p()
+        rc = lib().te_adjoint_create(run.h, every, ctypes.byref(self.h))
+        if rc != TE_OK:
+            msg = lib().te_adjoint_last_error(self.h)
+            self.close()
+            raise TeError("te_adjoint_create failed (%d): %s" % (rc, msg.decode() if msg else "?"))
+        # the unfused dispatch list on the adjoint's own memory (borrowed)
+        self.program = _Borrowed(lib().te_adjoint_program(self.h))
+
+    def _check(self, rc, what):
+        if rc != TE_OK:
+            msg = lib().te_adjoint_last_error(self.h)
+            raise TeError("%s failed (%d): %s" % (what, rc, msg.decode() if msg else "?"))
+
+    def forward(self, nsteps):
+        """nsteps large steps of the context, checkpointed."""
+        self._check(lib().te_fwd_ckpt(self.h, nsteps), "te_fwd_ckpt")
+
+    def backward(self):
+        self._check(lib().te_bwd(self.h), "te_bwd")
+
+    def dispatch_adjoints(self, first, last):
+        self._check(lib().te_adj_ops(self.h, first, last), "te_adj_ops")
+
+    def upload(self, word_off, values):
+        v = np.ascontiguousarray(values, dtype=np.float32)
+        self._check(lib().te_adj_upload(self.h, word_off, v.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), v.size),
+                    "te_adj_upload")
+
+    def read(self, word_off, nwords):
+        out = np.empty(nwords, dtype=np.float32)
+        self._check(lib().te_adj_download(self.h, word_off, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), nwords),
+                    "te_adj_download")
+        return out
+
+    def zero(self):
+        self._check(lib().te_adj_zero(self.h), "te_adj_zero")
+
+    def last_ms(self):
+        f, b = ctypes.c_float(), ctypes.c_float()
+        self._check(lib().te_adj_last_ms(self.h, ctypes.byref(f), ctypes.byref(b)), "te_adj_last_ms")
+        return f.value, b.value
+
+    def close(self):
+        if self.h:
+            lib().te_adjoint_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class _Borrowed(_Ctx):
+    """A context owned by another object (not destroyed here)."""
+
+    def __init__(self, h):
+        self.h = ctypes.c_void_p(h)
+
+    def close(self):
+        self.h = ctypes.c_void_p()
