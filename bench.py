@@ -180,6 +180,57 @@ def ensemble_bench(steps, warmup):
     return out
 
 
+# Adjoint workload: 128 x 128 x 60, dx = dy = 1 km, dz = 333 m, dt = 6 s,
+# nsound 6, WENO scalars, theta qv qc, Rayleigh damping, no turbulence;
+# 60 large steps with a checkpoint every 10.
+ADJOINT = dict(nx=128, ny=128, nz=60, dz=20000.0 / 60, dt=6.0, nsound=6)
+
+
+def adjoint_bench(steps=60, every=10):
+    """Cost of the gradient of J = sum of w^2 over 0-8 km in the 20 x 20 km
+    column block around the bubble after `steps` steps, with respect to the
+    initial theta', u, v and qv: the checkpointed forward plus the reverse
+    sweep, against the plain forward of the same steps (graph replays)."""
+    cfg = te_host.config(**ADJOINT)
+    run = te_host.Run(cfg)
+    run.init_state()
+    run.step(2)
+    run.sync()
+    run.init_state()
+    t0 = time.perf_counter()
+    run.step(steps)
+    run.sync()
+    plain = time.perf_counter() - t0
+    run.init_state()
+    adj = te_host.Adjoint(run, every=every)
+    t0 = time.perf_counter()
+    adj.forward(steps)
+    fwd = time.perf_counter() - t0
+    nwords = run.field_offset(run.fields[0]) + run.field_words * len(run.fields)  # profiles + arena
+    c = cfg.nx // 2
+    top = int(8000.0 / cfg.dz)
+    box = te_host.Box(run, i=(c - 10, c + 10), j=(c - 10, c + 10), k=(1, top + 1))
+    state = run.read(0, nwords).view(np.float32)
+    seed = np.zeros(nwords, np.float32)
+    seed[box.words] = 2.0 * state[box.words]
+    adj.upload(0, seed)
+    t0 = time.perf_counter()
+    adj.backward()
+    bwd = time.perf_counter() - t0
+    ctl = te_host.Controls(run, ("th", "u", "v", "qv"))
+    g = ctl.pull(adj.read(0, nwords)).reshape(4, -1)
+    fwd_ms, bwd_ms = adj.last_ms()
+    adj.close()
+    run.close()
+    return {"workload": "128x128x60 dz=333m dt=6s nsound=6, WENO scalars, no turbulence, %d steps, "
+                        "checkpoint every %d" % (steps, every),
+            "plain_fwd_s": round(plain, 4), "fwd_ckpt_s": round(fwd, 4), "bwd_s": round(bwd, 4),
+            "cost_vs_plain_fwd": round((fwd + bwd) / plain, 2),
+            "J": float(np.sum(state[box.words].astype(np.float64) ** 2)),
+            "grad_norms": {k: float(np.linalg.norm(g[q])) for q, k in enumerate(("theta'", "u", "v", "qv"))},
+            "device_ms": {"fwd_ckpt": round(fwd_ms, 2), "bwd": round(bwd_ms, 2)}}
+
+
 def cpu_baseline(cfg, seconds=15.0):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib
@@ -207,8 +258,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=3)
     ap.add_argument("--calib-copy", action="store_true", help="also launch the PMC calibration copy")
-    ap.add_argument("--ensemble", action="store_true",
-                    help="also time the 32-member uneven ensemble (members stepped together)")
+    ap.add_argument("--no-ensemble", action="store_true",
+                    help="skip the 32-member uneven ensemble (members stepped together)")
+    ap.add_argument("--no-adjoint", action="store_true",
+                    help="skip the adjoint cost (128x128x60, 60 steps, checkpoint every 10)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -274,11 +327,13 @@ def main():
                                     sorted(per_kernel.items(), key=lambda kv: -kv[1][0])[:8]},
         "top_kernels_hbm_frac": top_frac,
     }
-    if args.ensemble:
-        run.close()
+    run.close()
+    if not args.no_ensemble:
         ens = ensemble_bench(max(5, args.steps // 5), 2)
         ens["per_point_vs_plain"] = round(ens["value"] / value, 3)
         out["ensemble"] = ens
+    if not args.no_adjoint:
+        out["adjoint"] = adjoint_bench()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(cfg)
     print(json.dumps(out), flush=True)

@@ -23,7 +23,14 @@ void grid(const float* M, float* Mb, const AdjArg& a) {
 
 void columns(const float* M, float* Mb, const AdjArg& a) {
   for (uint32_t y = 0; y < a.n[1]; ++y)
-    for (uint32_t x = 0; x < a.n[0]; ++x) w_solve_column(M, Mb, a, (int)x, (int)y);
+    for (uint32_t x = 0; x < a.n[0]; ++x) w_solve_sweep(M, Mb, a, (int)x, (int)y);
+  Acc r(M, Mb, &a);
+  const ADims g = r.dims();
+  for (int z = 0; z < g.nz; ++z)
+    for (uint32_t y = 0; y < a.n[1]; ++y)
+      for (uint32_t x = 0; x < a.n[0]; ++x) w_solve_level(M, Mb, a, (int)x, (int)y, z);
+  for (uint32_t y = 0; y < a.n[1]; ++y)
+    for (uint32_t x = 0; x < a.n[0]; ++x) w_solve_clear(Mb, a, g, (int)x, (int)y);
 }
 
 struct HostEntry {

@@ -75,10 +75,14 @@ def test_gpu_dispatch_adjoints_transpose_the_tangent_linear_build(gpu, kw):
     assert not bad, bad
 
 
-@pytest.mark.parametrize("every", [1, 2])
-def test_gpu_reverse_sweep_matches_the_host_sweep(gpu, every):
+@pytest.mark.parametrize("every,segment_log", [(1, True), (2, True), (2, False)])
+def test_gpu_reverse_sweep_matches_the_host_sweep(gpu, every, segment_log, monkeypatch):
+    """segment_log False: a zero log budget forces one step's log at a time
+    with the segment's step-start states recomputed first."""
     steps = 3
-    say("reverse sweep, steps", steps, "checkpoint every", every)
+    say("reverse sweep, steps", steps, "checkpoint every", every, "segment log", segment_log)
+    if not segment_log:
+        monkeypatch.setenv("TE_ADJOINT_LOG_GIB", "0")
     run = te_host.Run(te_host.config(**GRID))
     run.init_state()
     adj = te_host.Adjoint(run, every=every)

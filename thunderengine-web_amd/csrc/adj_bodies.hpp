@@ -1063,7 +1063,10 @@ TE_HD WCoef w_coef(Acc& r, int64_t c, int64_t sk, float asq, float dts, float al
   return q;
 }
 
-TE_HD void w_solve_column(const float* M, float* Mb, const AdjArg& a, int i, int j) {
+// Column part: forward factors, the new w, U^T z = wbar and L^T dbar = z.
+// Leaves dbar_k in the adjoint words of cA and the new w in those of cB
+// (levels 1..nz-1) and clears wbar for the inputs' adjoints.
+TE_HD void w_solve_sweep(const float* M, float* Mb, const AdjArg& a, int i, int j) {
   Acc r(M, Mb, &a);
   const ADims g = r.dims();
   if (i >= g.nx || j >= g.ny) return;
@@ -1104,26 +1107,44 @@ TE_HD void w_solve_column(const float* M, float* Mb, const AdjArg& a, int i, int
     aNext = q.aa.v;
     yNext = y;
   }
-  // coefficient adjoints pushed back to the inputs at levels k and k-1
+}
+
+// Level part, one (i, j, k), k = 1..nz-1, after every column's sweep:
+// abar, bbar, cbar, dbar of row k pushed back to its loads at levels k and
+// k-1 (rows k and k+1 both touch level k: adj_add).
+TE_HD void w_solve_level(const float* M, float* Mb, const AdjArg& a, int i, int j, int k) {
+  Acc r(M, Mb, &a);
+  const ADims g = r.dims();
+  const int nz = g.nz;
+  if (i >= g.nx || j >= g.ny || k < 1 || k > nz - 1) return;
+  const float dts = r.F(1, 0), alph = r.F(1, 1), beta = r.F(1, 2);
+  const float asq = alph * alph;
+  const int64_t sk = g.sk(), c0 = g.idx(i, j, 0);
+  const float* S1 = Mb + a.p[10] + c0;
+  const float* S2 = Mb + a.p[11] + c0;
+  const double y = S1[k * sk];
+  if (y == 0.0) return;
+  const double wm = k == 1 ? 0.0 : S2[(k - 1) * sk], w0 = S2[k * sk], wp = k == nz - 1 ? 0.0 : S2[(k + 1) * sk];
+  const double ab_ = k == 1 ? 0.0 : -y * wm, bb_ = -y * w0, cb_ = k == nz - 1 ? 0.0 : -y * wp;
   r.mode = 1;
-  for (int k = 1; k <= nz - 1; ++k) {
-    const double y = S1[k * sk];
-    if (y == 0.0) continue;
-    const double wm = k == 1 ? 0.0 : S2[(k - 1) * sk], w0 = S2[k * sk], wp = k == nz - 1 ? 0.0 : S2[(k + 1) * sk];
-    const double ab_ = k == 1 ? 0.0 : -y * wm, bb_ = -y * w0, cb_ = k == nz - 1 ? 0.0 : -y * wp;
-    for (int s = 0; s < 32; ++s) {  // 17 active loads
-      r.cnt = 0;
-      r.seed = s;
-      r.rec = -1;
-      const WCoef q = w_coef(r, c0 + k * sk, sk, asq, dts, alph, beta);
-      if (s >= r.cnt) break;
-      const double acc = fmad_(ab_, q.aa.d, fmad_(bb_, q.bb.d, fmad_(cb_, q.cc.d, y * q.dd.d)));
-      if (acc != 0.0) Mb[r.rec] += (float)acc;
-    }
+  for (int s = 0; s < 32; ++s) {  // 17 active loads
+    r.cnt = 0;
+    r.seed = s;
+    r.rec = -1;
+    const WCoef q = w_coef(r, c0 + k * sk, sk, asq, dts, alph, beta);
+    if (s >= r.cnt) break;
+    const double acc = fmad_(ab_, q.aa.d, fmad_(bb_, q.bb.d, fmad_(cb_, q.cc.d, y * q.dd.d)));
+    if (acc != 0.0) adj_add(Mb + r.rec, (float)acc);
   }
-  for (int k = 0; k <= nz; ++k) {
-    S1[k * sk] = 0.0f;
-    S2[k * sk] = 0.0f;
+}
+
+// Clears the adjoint words of cA and cB (the forward's dead scratch).
+TE_HD void w_solve_clear(float* Mb, const AdjArg& a, const ADims& g, int i, int j) {
+  if (i >= g.nx || j >= g.ny) return;
+  const int64_t sk = g.sk(), c0 = g.idx(i, j, 0);
+  for (int k = 0; k <= g.nz; ++k) {
+    Mb[a.p[10] + c0 + k * sk] = 0.0f;
+    Mb[a.p[11] + c0 + k * sk] = 0.0f;
   }
 }
 

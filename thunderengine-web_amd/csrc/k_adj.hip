@@ -26,12 +26,29 @@ __global__ __launch_bounds__(256) void k_adj(const float* M, float* Mb, AdjArg a
   adj_point<B>(M, Mb, a, (int)x, (int)y, (int)z);
 }
 
-__global__ __launch_bounds__(64) void k_adj_w_solve(const float* M, float* Mb, AdjArg a) {
+// acoustic_w_solve: column sweeps, then the levels' coefficient adjoints,
+// then the scratch words cleared
+__global__ __launch_bounds__(64) void k_adj_w_sweep(const float* M, float* Mb, AdjArg a) {
   const uint64_t t = (uint64_t)blockIdx.x * 64u + threadIdx.x;
   const uint64_t n0 = a.n[0];
   if (t >= n0 * a.n[1]) return;
   const uint64_t y = t / n0, x = t - y * n0;
-  w_solve_column(M, Mb, a, (int)x, (int)y);
+  w_solve_sweep(M, Mb, a, (int)x, (int)y);
+}
+__global__ __launch_bounds__(256) void k_adj_w_level(const float* M, float* Mb, AdjArg a, uint32_t nz) {
+  const uint64_t t = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+  const uint64_t n0 = a.n[0], n01 = n0 * a.n[1];
+  if (t >= n01 * nz) return;
+  const uint64_t z = t / n01, rem = t - z * n01, y = rem / n0, x = rem - y * n0;
+  w_solve_level(M, Mb, a, (int)x, (int)y, (int)z);
+}
+__global__ __launch_bounds__(256) void k_adj_w_clear(const float* M, float* Mb, AdjArg a) {
+  const uint64_t t = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+  const uint64_t n0 = a.n[0];
+  if (t >= n0 * a.n[1]) return;
+  const uint64_t y = t / n0, x = t - y * n0;
+  Acc r(M, Mb, &a);
+  w_solve_clear(Mb, a, r.dims(), (int)x, (int)y);
 }
 
 struct Entry {
@@ -45,7 +62,7 @@ struct Entry {
 #define AK(n, B, nw, ...) {n, reinterpret_cast<const void*>(&k_adj<B>), nw, __VA_ARGS__, false},
 static const Entry kAdj[] = {
     TE_ADJ_POINT_KERNELS(AK)
-    {"acoustic_w_solve", reinterpret_cast<const void*>(&k_adj_w_solve), 3, {9, 10, 11}, true},
+    {"acoustic_w_solve", reinterpret_cast<const void*>(&k_adj_w_sweep), 3, {9, 10, 11}, true},
 };
 #undef AK
 
@@ -60,16 +77,25 @@ int writes(int id, int out[8]) {
   return kAdj[id].nw;
 }
 
-hipError_t launch(int id, const uint32_t* A, const float* M, float* Mb, hipStream_t stream) {
+hipError_t launch(int id, const uint32_t* A, const float* M, float* Mb, int nz, hipStream_t stream) {
   const Entry& e = kAdj[id];
   AdjArg a;
   a.n[0] = A[0]; a.n[1] = A[1]; a.n[2] = A[2];
   for (int q = 0; q < 32; ++q) a.p[q] = A[4 + q];
   void* args[] = {(void*)&M, (void*)&Mb, (void*)&a};
-  if (e.column) {
+  if (e.column) {  // acoustic_w_solve: sweep, levels, clear
     const uint64_t cols = (uint64_t)a.n[0] * a.n[1];
     if (cols == 0) return hipSuccess;
-    return hipLaunchKernel(e.fn, dim3((unsigned)((cols + 63) / 64)), dim3(64), args, 0, stream);
+    hipError_t r = hipLaunchKernel(e.fn, dim3((unsigned)((cols + 63) / 64)), dim3(64), args, 0, stream);
+    if (r != hipSuccess) return r;
+    uint32_t nzu = (uint32_t)nz;
+    void* largs[] = {(void*)&M, (void*)&Mb, (void*)&a, (void*)&nzu};
+    const uint64_t pts = cols * nzu;
+    r = hipLaunchKernel(reinterpret_cast<const void*>(&k_adj_w_level), dim3((unsigned)((pts + 255) / 256)), dim3(256),
+                        largs, 0, stream);
+    if (r != hipSuccess) return r;
+    return hipLaunchKernel(reinterpret_cast<const void*>(&k_adj_w_clear), dim3((unsigned)((cols + 255) / 256)),
+                           dim3(256), args, 0, stream);
   }
   const uint64_t n = (uint64_t)a.n[0] * a.n[1] * a.n[2];
   if (n == 0) return hipSuccess;

@@ -2,9 +2,11 @@
 //
 // Memory (one MI355X holds all of it for the grids of this path):
 //   * checkpoints: the state words [0, arena end) before every `every`-th step;
-//   * step states: the states at the starts of one segment's steps;
-//   * the write log of one step: every region each dispatch writes, saved
-//     before the dispatch (a whole FieldArena slot per written field);
+//   * the write log: every region each dispatch writes, saved before the
+//     dispatch (a whole FieldArena slot per written field), for a whole
+//     checkpoint segment when that fits (each step is then recomputed once),
+//     else for one step, with the segment's step-start states recomputed
+//     first;
 //   * the adjoint memory: one float per word of M, with M's guards.
 // The reverse of one step restores, dispatch by dispatch from the last, the
 // regions that dispatch wrote, so the memory holds exactly the dispatch's
@@ -12,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -31,7 +34,8 @@ struct te_adjoint {
   int nsteps = 0;
   uint32_t* dSteps = nullptr;  // `every` step-start states
   uint32_t* dLog = nullptr;
-  uint64_t log_words = 0;
+  uint64_t log_words = 0;     // one step's write log
+  bool segment_log = false;   // the log holds a whole checkpoint segment
   std::vector<int> adj_id;                                        // per dispatch
   std::vector<std::vector<std::pair<uint64_t, uint64_t>>> wreg;  // (offset, words) per dispatch
   hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr, e3 = nullptr;
@@ -55,30 +59,36 @@ static int run_rec_ops(te_adjoint* a, int first, int last) {
   return TE_OK;
 }
 
-// One logged forward step on rec's memory, then its reverse.
-static int reverse_step(te_adjoint* a) {
+// Logged forward of `steps` steps on rec's memory (the write log of each
+// dispatch appended at log position `pos`), then their reverse.
+static int reverse_steps(te_adjoint* a, int steps) {
   te_ctx* r = a->rec;
   const int n = (int)r->ops.size();
-  std::vector<uint64_t> at(n + 1, 0);
+  std::vector<uint64_t> at((size_t)n * steps, 0);
   uint64_t pos = 0;
-  for (int i = 0; i < n; ++i) {
-    at[i] = pos;
-    for (auto& w : a->wreg[i]) {
-      TA_HIP(hipMemcpyAsync(a->dLog + pos, r->dM + w.first, w.second * 4, hipMemcpyDeviceToDevice, r->stream), "log");
-      pos += w.second;
+  for (int s = 0; s < steps; ++s)
+    for (int i = 0; i < n; ++i) {
+      at[(size_t)s * n + i] = pos;
+      for (auto& w : a->wreg[i]) {
+        TA_HIP(hipMemcpyAsync(a->dLog + pos, r->dM + w.first, w.second * 4, hipMemcpyDeviceToDevice, r->stream),
+               "log");
+        pos += w.second;
+      }
+      int rc = run_rec_ops(a, i, i);
+      if (rc != TE_OK) return rc;
     }
-    int rc = run_rec_ops(a, i, i);
-    if (rc != TE_OK) return rc;
-  }
-  for (int i = n - 1; i >= 0; --i) {
-    uint64_t p = at[i];
-    for (auto& w : a->wreg[i]) {
-      TA_HIP(hipMemcpyAsync(r->dM + w.first, a->dLog + p, w.second * 4, hipMemcpyDeviceToDevice, r->stream), "restore");
-      p += w.second;
+  for (int s = steps - 1; s >= 0; --s)
+    for (int i = n - 1; i >= 0; --i) {
+      uint64_t p = at[(size_t)s * n + i];
+      for (auto& w : a->wreg[i]) {
+        TA_HIP(hipMemcpyAsync(r->dM + w.first, a->dLog + p, w.second * 4, hipMemcpyDeviceToDevice, r->stream),
+               "restore");
+        p += w.second;
+      }
+      TA_HIP(te::adj::launch(a->adj_id[i], r->ops[i].A, reinterpret_cast<const float*>(r->dM), a->dMb, r->layout.nz,
+                             r->stream),
+             std::string("adjoint of ") + r->op_names[i]);
     }
-    TA_HIP(te::adj::launch(a->adj_id[i], r->ops[i].A, reinterpret_cast<const float*>(r->dM), a->dMb, r->stream),
-           std::string("adjoint of ") + r->op_names[i]);
-  }
   return TE_OK;
 }
 
@@ -138,8 +148,19 @@ int te_adjoint_create(te_ctx* ctx, int32_t every, te_adjoint** out) {
   TA_HIP(hipMalloc((void**)&a->dMbBase, mbw * 4), "hipMalloc(adjoint memory)");
   TA_HIP(hipMemsetAsync(a->dMbBase, 0, mbw * 4, s), "clear adjoint memory");
   a->dMb = a->dMbBase + r->front;
-  TA_HIP(hipMalloc((void**)&a->dLog, std::max<uint64_t>(a->log_words, 1) * 4), "hipMalloc(write log)");
-  TA_HIP(hipMalloc((void**)&a->dSteps, (uint64_t)every * a->state_words * 4), "hipMalloc(step states)");
+  // a whole segment's write log when it fits the budget (one recompute per
+  // step); otherwise one step's log and the segment's step-start states
+  // (budget: TE_ADJOINT_LOG_GIB, default 96 GiB of the 288 GiB HBM)
+  uint64_t budget = 96ull << 30;
+  if (const char* env = std::getenv("TE_ADJOINT_LOG_GIB")) budget = (uint64_t)std::strtoull(env, nullptr, 10) << 30;
+  if (every > 1 && (uint64_t)every * a->log_words * 4 <= budget &&
+      hipMalloc((void**)&a->dLog, (uint64_t)every * a->log_words * 4) == hipSuccess) {
+    a->segment_log = true;
+  } else {
+    (void)hipGetLastError();
+    TA_HIP(hipMalloc((void**)&a->dLog, std::max<uint64_t>(a->log_words, 1) * 4), "hipMalloc(write log)");
+    TA_HIP(hipMalloc((void**)&a->dSteps, (uint64_t)every * a->state_words * 4), "hipMalloc(step states)");
+  }
   TA_HIP(hipEventCreate(&a->e0), "event");
   TA_HIP(hipEventCreate(&a->e1), "event");
   TA_HIP(hipEventCreate(&a->e2), "event");
@@ -184,6 +205,11 @@ int te_bwd(te_adjoint* a) {
   for (int sg = nseg - 1; sg >= 0; --sg) {
     const int s0 = sg * a->every, s1 = std::min(a->nsteps, s0 + a->every);
     TA_HIP(hipMemcpyAsync(r->dM, a->ckpt[sg], sw * 4, hipMemcpyDeviceToDevice, r->stream), "load checkpoint");
+    if (a->segment_log) {
+      int rc = reverse_steps(a, s1 - s0);
+      if (rc != TE_OK) return rc;
+      continue;
+    }
     for (int s = s0; s < s1; ++s) {
       TA_HIP(hipMemcpyAsync(a->dSteps + (uint64_t)(s - s0) * sw, r->dM, sw * 4, hipMemcpyDeviceToDevice, r->stream),
              "step state");
@@ -195,7 +221,7 @@ int te_bwd(te_adjoint* a) {
     for (int s = s1 - 1; s >= s0; --s) {
       TA_HIP(hipMemcpyAsync(r->dM, a->dSteps + (uint64_t)(s - s0) * sw, sw * 4, hipMemcpyDeviceToDevice, r->stream),
              "load step state");
-      int rc = reverse_step(a);
+      int rc = reverse_steps(a, 1);
       if (rc != TE_OK) return rc;
     }
   }
@@ -211,7 +237,8 @@ int te_adj_ops(te_adjoint* a, int32_t first, int32_t last) {
   const int n = (int)r->ops.size();
   if (first < 0 || last >= n || first > last) return afail(a, TE_ERR_ARG, "bad dispatch range");
   for (int i = last; i >= first; --i)
-    TA_HIP(te::adj::launch(a->adj_id[i], r->ops[i].A, reinterpret_cast<const float*>(r->dM), a->dMb, r->stream),
+    TA_HIP(te::adj::launch(a->adj_id[i], r->ops[i].A, reinterpret_cast<const float*>(r->dM), a->dMb, r->layout.nz,
+                           r->stream),
            std::string("adjoint of ") + r->op_names[i]);
   TA_HIP(hipStreamSynchronize(r->stream), "sync");
   return TE_OK;

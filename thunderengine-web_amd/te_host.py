@@ -405,3 +405,59 @@ class _Borrowed(_Ctx):
 
     def close(self):
         self.h = ctypes.c_void_p()
+
+
+class Box:
+    """Word indices of a field (default w) over an (i, j, k) box of interior points:
+    i, j, k are (start, stop) ranges."""
+
+    def __init__(self, e, i, j, k, field="w"):
+        o = e.field_offset(field)
+        ii, jj, kk = np.meshgrid(np.arange(*i), np.arange(*j), np.arange(*k), indexing="ij")
+        self.words = (o + (ii + 3) + e.sJ * ((jj + 3) + e.sK * kk)).ravel()
+
+
+class Controls:
+    """Initial-condition controls: an interior value of each named field at
+    levels 0..nz-1.  A perturbation of one moves that point's word in the field
+    and in its _a copy, and every periodic halo image of them (the t=0 state
+    holds the field in both, halos filled periodically)."""
+
+    def __init__(self, e, names):
+        self.e = e
+        nx, ny, nz, sJ, sK = e.nx, e.ny, e.nz, e.sJ, e.sK
+        ii = np.arange(-3, nx + 4)
+        jj = np.arange(-3, ny + 4)
+        I, J = np.meshgrid(ii, jj, indexing="ij")
+        src = ((I % nx) + nx * (J % ny)).ravel()          # interior point each image copies
+        rel = ((I + 3) + sJ * (J + 3)).ravel()            # word of the image within a level
+        self.maps = []
+        for f in names:
+            offs = [e.field_offset(f)] + ([e.field_offset(f + "_a")] if f + "_a" in e.fields else [])
+            self.maps.append(offs)
+        self.src, self.rel = src, rel
+        self.per_field = nx * ny * nz
+        self.size = self.per_field * len(names)
+
+    def _iter(self):
+        e = self.e
+        for q, offs in enumerate(self.maps):
+            for k in range(e.nz):
+                base = k * e.sJ * e.sK
+                cols = q * self.per_field + k * e.nx * e.ny + self.src
+                for o in offs:
+                    yield o + base + self.rel, cols
+
+    def pull(self, Mb):
+        """Gradient with respect to the controls from an adjoint memory."""
+        g = np.zeros(self.size)
+        for words, cols in self._iter():
+            np.add.at(g, cols, Mb[words].astype(np.float64))
+        return g
+
+    def push(self, d, n):
+        """Word perturbation of length n for a control direction d."""
+        x = np.zeros(n)
+        for words, cols in self._iter():
+            x[words] += d[cols]
+        return x
