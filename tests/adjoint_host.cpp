@@ -6,6 +6,7 @@
 // suite can check every adjoint against the oracle's FP64 tangent-linear
 // build without a GPU.  The GPU tests check the device launches the same way.
 #include <cstring>
+#include <vector>
 #include <xmmintrin.h>
 
 #include "../thunderengine-web_amd/csrc/adj_bodies.hpp"
@@ -22,10 +23,15 @@ void grid(const float* M, float* Mb, const AdjArg& a) {
 }
 
 void columns(const float* M, float* Mb, const AdjArg& a) {
-  for (uint32_t y = 0; y < a.n[1]; ++y)
-    for (uint32_t x = 0; x < a.n[0]; ++x) w_solve_sweep(M, Mb, a, (int)x, (int)y);
   Acc r(M, Mb, &a);
   const ADims g = r.dims();
+  const uint64_t fw = (uint64_t)g.sJ * g.sK * (g.nz + 1);
+  std::vector<float> scr(4 * fw, 0.0f);
+  for (int z = 0; z < g.nz; ++z)
+    for (uint32_t y = 0; y < a.n[1]; ++y)
+      for (uint32_t x = 0; x < a.n[0]; ++x) w_solve_coef(M, a, scr.data(), fw, (int)x, (int)y, z);
+  for (uint32_t y = 0; y < a.n[1]; ++y)
+    for (uint32_t x = 0; x < a.n[0]; ++x) w_solve_sweep(M, Mb, a, scr.data(), fw, (int)x, (int)y);
   for (int z = 0; z < g.nz; ++z)
     for (uint32_t y = 0; y < a.n[1]; ++y)
       for (uint32_t x = 0; x < a.n[0]; ++x) w_solve_level(M, Mb, a, (int)x, (int)y, z);

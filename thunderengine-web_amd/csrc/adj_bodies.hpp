@@ -1063,48 +1063,69 @@ TE_HD WCoef w_coef(Acc& r, int64_t c, int64_t sk, float asq, float dts, float al
   return q;
 }
 
-// Column part: forward factors, the new w, U^T z = wbar and L^T dbar = z.
-// Leaves dbar_k in the adjoint words of cA and the new w in those of cB
-// (levels 1..nz-1) and clears wbar for the inputs' adjoints.
-TE_HD void w_solve_sweep(const float* M, float* Mb, const AdjArg& a, int i, int j) {
+// Coefficients of row k (k = 1..nz-1) at one (i, j), into four planes of the
+// caller's scratch (a, b, c, d; plane stride fw words, field indexing).
+TE_HD void w_solve_coef(const float* M, const AdjArg& a, float* scr, uint64_t fw, int i, int j, int k) {
+  Acc r(M, nullptr, &a);
+  const ADims g = r.dims();
+  if (i >= g.nx || j >= g.ny || k < 1 || k > g.nz - 1) return;
+  const float dts = r.F(1, 0), alph = r.F(1, 1), beta = r.F(1, 2);
+  const int64_t sk = g.sk(), c = g.idx(i, j, k);
+  const WCoef q = w_coef(r, c, sk, alph * alph, dts, alph, beta);
+  scr[c] = (float)q.aa.v;
+  scr[fw + c] = (float)q.bb.v;
+  scr[2 * fw + c] = (float)q.cc.v;
+  scr[3 * fw + c] = (float)q.dd.v;
+}
+
+// Column part: forward factors, the new w, U^T z = wbar and L^T dbar = z,
+// from the coefficient planes.  Leaves dbar_k in the adjoint words of cA and
+// the new w in those of cB (levels 1..nz-1) and clears wbar for the inputs'
+// adjoints.
+TE_HD void w_solve_sweep(const float* M, float* Mb, const AdjArg& a, const float* scr, uint64_t fw, int i, int j) {
   Acc r(M, Mb, &a);
   const ADims g = r.dims();
   if (i >= g.nx || j >= g.ny) return;
-  const float dts = r.F(1, 0), alph = r.F(1, 1), beta = r.F(1, 2);
-  const float asq = alph * alph;
   const int nz = g.nz;
   const int64_t sk = g.sk(), c0 = g.idx(i, j, 0);
+  const float *AA = scr + c0, *BB = scr + fw + c0, *CC = scr + 2 * fw + c0, *DD = scr + 3 * fw + c0;
   float* wb = Mb + a.p[9] + c0;   // adjoint of w (column)
   float* S1 = Mb + a.p[10] + c0;  // adjoint words of cA: cA_k, then dbar_k
   float* S2 = Mb + a.p[11] + c0;  // adjoint words of cB: cB_k, then the new w
   // forward factors and the new w (the forward's recurrences, in binary64)
+  double cAp = 0.0, cBp = 0.0;
   for (int k = 1; k <= nz - 1; ++k) {
-    const WCoef q = w_coef(r, c0 + k * sk, sk, asq, dts, alph, beta);
-    if (k == 1) {
-      const double rr = 1.0 / q.bb.v;
-      S1[k * sk] = (float)((-q.cc.v) * rr);
-      S2[k * sk] = (float)(q.dd.v * rr);
-    } else {
-      const double rr = 1.0 / fmad_(q.aa.v, S1[(k - 1) * sk], q.bb.v);
-      if (k < nz - 1) S1[k * sk] = (float)((-q.cc.v) * rr);
-      S2[k * sk] = (float)(fmad_(-q.aa.v, S2[(k - 1) * sk], q.dd.v) * rr);  // cB_k, or w[nz-1]
-    }
+    const double aa = AA[k * sk], bb = BB[k * sk], cc = CC[k * sk], dd = DD[k * sk];
+    const double rr = k == 1 ? 1.0 / bb : 1.0 / fmad_(aa, cAp, bb);
+    const double cB = (k == 1 ? dd : fmad_(-aa, cBp, dd)) * rr;
+    cAp = (-cc) * rr;
+    cBp = cB;
+    S1[k * sk] = (float)cAp;
+    S2[k * sk] = (float)cB;  // cB_k, or w[nz-1]
   }
-  for (int k = nz - 2; k >= 1; --k) S2[k * sk] = (float)fmad_(S1[k * sk], S2[(k + 1) * sk], S2[k * sk]);
+  double wn = S2[(nz - 1) * sk];
+  for (int k = nz - 2; k >= 1; --k) {
+    wn = fmad_(S1[k * sk], wn, S2[k * sk]);
+    S2[k * sk] = (float)wn;
+  }
   // U^T z = wbar (z kept in wb), levels 0 and nz are constant outputs
   wb[0] = 0.0f;
   wb[nz * sk] = 0.0f;
-  for (int k = 2; k <= nz - 1; ++k) wb[k * sk] = (float)fmad_(S1[(k - 1) * sk], wb[(k - 1) * sk], wb[k * sk]);
+  double z = wb[sk];
+  for (int k = 2; k <= nz - 1; ++k) {
+    z = fmad_(S1[(k - 1) * sk], z, wb[k * sk]);
+    wb[k * sk] = (float)z;
+  }
   // L^T dbar = z, from the top (dbar kept in S1, wb cleared for the inputs' adjoints)
   double aNext = 0.0, yNext = 0.0;
   for (int k = nz - 1; k >= 1; --k) {
-    const WCoef q = w_coef(r, c0 + k * sk, sk, asq, dts, alph, beta);
-    const double rr = k == 1 ? 1.0 / q.bb.v : 1.0 / fmad_(q.aa.v, S1[(k - 1) * sk], q.bb.v);
-    const double z = wb[k * sk];
-    const double y = (k == nz - 1 ? z : fmad_(-aNext, yNext, z)) * rr;
+    const double aa = AA[k * sk], bb = BB[k * sk];
+    const double rr = k == 1 ? 1.0 / bb : 1.0 / fmad_(aa, S1[(k - 1) * sk], bb);
+    const double zk = wb[k * sk];
+    const double y = (k == nz - 1 ? zk : fmad_(-aNext, yNext, zk)) * rr;
     S1[k * sk] = (float)y;
     wb[k * sk] = 0.0f;
-    aNext = q.aa.v;
+    aNext = aa;
     yNext = y;
   }
 }

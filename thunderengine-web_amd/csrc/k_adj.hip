@@ -9,6 +9,7 @@
 // before the forward dispatch and restores them before the adjoint one.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
 
 #include "adj_bodies.hpp"
@@ -26,14 +27,21 @@ __global__ __launch_bounds__(256) void k_adj(const float* M, float* Mb, AdjArg a
   adj_point<B>(M, Mb, a, (int)x, (int)y, (int)z);
 }
 
-// acoustic_w_solve: column sweeps, then the levels' coefficient adjoints,
-// then the scratch words cleared
-__global__ __launch_bounds__(64) void k_adj_w_sweep(const float* M, float* Mb, AdjArg a) {
+// acoustic_w_solve: the rows' coefficients, column sweeps, the levels'
+// coefficient adjoints, then the scratch words cleared
+__global__ __launch_bounds__(256) void k_adj_w_coef(const float* M, AdjArg a, float* scr, uint64_t fw, uint32_t nz) {
+  const uint64_t t = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+  const uint64_t n0 = a.n[0], n01 = n0 * a.n[1];
+  if (t >= n01 * nz) return;
+  const uint64_t z = t / n01, rem = t - z * n01, y = rem / n0, x = rem - y * n0;
+  w_solve_coef(M, a, scr, fw, (int)x, (int)y, (int)z);
+}
+__global__ __launch_bounds__(64) void k_adj_w_sweep(const float* M, float* Mb, AdjArg a, const float* scr, uint64_t fw) {
   const uint64_t t = (uint64_t)blockIdx.x * 64u + threadIdx.x;
   const uint64_t n0 = a.n[0];
   if (t >= n0 * a.n[1]) return;
   const uint64_t y = t / n0, x = t - y * n0;
-  w_solve_sweep(M, Mb, a, (int)x, (int)y);
+  w_solve_sweep(M, Mb, a, scr, fw, (int)x, (int)y);
 }
 __global__ __launch_bounds__(256) void k_adj_w_level(const float* M, float* Mb, AdjArg a, uint32_t nz) {
   const uint64_t t = (uint64_t)blockIdx.x * 256u + threadIdx.x;
@@ -66,6 +74,36 @@ static const Entry kAdj[] = {
 };
 #undef AK
 
+// Up to 8 regions of n words each (blockIdx.y = region): the reverse sweep's
+// write log and restores, one launch per dispatch; 16-byte accesses when
+// every region is 16-byte aligned and n is a multiple of 4.
+template <bool VEC>
+__global__ __launch_bounds__(256) void k_adj_copy(CopyArg c) {
+  const uint32_t q = blockIdx.y;
+  const uint64_t stride = (uint64_t)gridDim.x * 256u;
+  if (VEC) {
+    const uint4* __restrict__ s = reinterpret_cast<const uint4*>(c.src[q]);
+    uint4* __restrict__ d = reinterpret_cast<uint4*>(c.dst[q]);
+    for (uint64_t w = (uint64_t)blockIdx.x * 256u + threadIdx.x; w < c.n / 4; w += stride) d[w] = s[w];
+  } else {
+    const uint32_t* __restrict__ s = c.src[q];
+    uint32_t* __restrict__ d = c.dst[q];
+    for (uint64_t w = (uint64_t)blockIdx.x * 256u + threadIdx.x; w < c.n; w += stride) d[w] = s[w];
+  }
+}
+
+hipError_t copy_regions(const CopyArg& c, int nreg, hipStream_t stream) {
+  if (nreg <= 0 || c.n == 0) return hipSuccess;
+  bool vec = c.n % 4 == 0;
+  for (int q = 0; q < nreg; ++q)
+    vec = vec && ((reinterpret_cast<uintptr_t>(c.src[q]) | reinterpret_cast<uintptr_t>(c.dst[q])) & 15u) == 0;
+  const uint64_t items = vec ? c.n / 4 : c.n;
+  const uint64_t blocks = std::min<uint64_t>((items + 255) / 256, 2048);
+  void* args[] = {(void*)&c};
+  const void* fn = vec ? reinterpret_cast<const void*>(&k_adj_copy<true>) : reinterpret_cast<const void*>(&k_adj_copy<false>);
+  return hipLaunchKernel(fn, dim3((unsigned)blocks, (unsigned)nreg), dim3(256), args, 0, stream);
+}
+
 int find(const char* name) {
   for (int i = 0; i < (int)(sizeof(kAdj) / sizeof(kAdj[0])); ++i)
     if (std::strcmp(kAdj[i].name, name) == 0) return i;
@@ -77,20 +115,28 @@ int writes(int id, int out[8]) {
   return kAdj[id].nw;
 }
 
-hipError_t launch(int id, const uint32_t* A, const float* M, float* Mb, int nz, hipStream_t stream) {
+hipError_t launch(int id, const uint32_t* A, const float* M, float* Mb, int nz, float* scr, uint64_t fw,
+                  hipStream_t stream) {
   const Entry& e = kAdj[id];
   AdjArg a;
   a.n[0] = A[0]; a.n[1] = A[1]; a.n[2] = A[2];
   for (int q = 0; q < 32; ++q) a.p[q] = A[4 + q];
   void* args[] = {(void*)&M, (void*)&Mb, (void*)&a};
-  if (e.column) {  // acoustic_w_solve: sweep, levels, clear
+  if (e.column) {  // acoustic_w_solve: coefficients, sweeps, levels, clear
     const uint64_t cols = (uint64_t)a.n[0] * a.n[1];
     if (cols == 0) return hipSuccess;
-    hipError_t r = hipLaunchKernel(e.fn, dim3((unsigned)((cols + 63) / 64)), dim3(64), args, 0, stream);
-    if (r != hipSuccess) return r;
+    if (!scr) return hipErrorInvalidValue;
     uint32_t nzu = (uint32_t)nz;
-    void* largs[] = {(void*)&M, (void*)&Mb, (void*)&a, (void*)&nzu};
     const uint64_t pts = cols * nzu;
+    void* cargs[] = {(void*)&M, (void*)&a, (void*)&scr, (void*)&fw, (void*)&nzu};
+    hipError_t r = hipLaunchKernel(reinterpret_cast<const void*>(&k_adj_w_coef), dim3((unsigned)((pts + 255) / 256)),
+                                   dim3(256), cargs, 0, stream);
+    if (r != hipSuccess) return r;
+    const float* cscr = scr;
+    void* sargs[] = {(void*)&M, (void*)&Mb, (void*)&a, (void*)&cscr, (void*)&fw};
+    r = hipLaunchKernel(e.fn, dim3((unsigned)((cols + 63) / 64)), dim3(64), sargs, 0, stream);
+    if (r != hipSuccess) return r;
+    void* largs[] = {(void*)&M, (void*)&Mb, (void*)&a, (void*)&nzu};
     r = hipLaunchKernel(reinterpret_cast<const void*>(&k_adj_w_level), dim3((unsigned)((pts + 255) / 256)), dim3(256),
                         largs, 0, stream);
     if (r != hipSuccess) return r;

@@ -30,6 +30,7 @@ struct te_adjoint {
   uint64_t state_words = 0;  // [0, arena end)
   float* dMbBase = nullptr;
   float* dMb = nullptr;
+  float* dScr = nullptr;  // 4 planes of one field: the column solve's coefficients
   std::vector<uint32_t*> ckpt;
   int nsteps = 0;
   uint32_t* dSteps = nullptr;  // `every` step-start states
@@ -38,6 +39,7 @@ struct te_adjoint {
   bool segment_log = false;   // the log holds a whole checkpoint segment
   std::vector<int> adj_id;                                        // per dispatch
   std::vector<std::vector<std::pair<uint64_t, uint64_t>>> wreg;  // (offset, words) per dispatch
+  std::vector<uint64_t> stride;  // log words per region of each dispatch (its length rounded up to 64)
   hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr, e3 = nullptr;
   float fwd_ms = 0.0f, bwd_ms = 0.0f;
   std::string err;
@@ -65,28 +67,33 @@ static int reverse_steps(te_adjoint* a, int steps) {
   te_ctx* r = a->rec;
   const int n = (int)r->ops.size();
   std::vector<uint64_t> at((size_t)n * steps, 0);
+  // save (restore) the regions of dispatch i at log position p: one launch
+  auto copy = [&](int i, uint64_t p, bool save) -> hipError_t {
+    const auto& w = a->wreg[i];
+    te::adj::CopyArg c{};
+    c.n = w.empty() ? 0 : w[0].second;
+    for (size_t q = 0; q < w.size(); ++q) {
+      uint32_t* m = r->dM + w[q].first;
+      uint32_t* l = a->dLog + p + q * a->stride[i];
+      c.src[q] = save ? m : l;
+      c.dst[q] = save ? l : m;
+    }
+    return te::adj::copy_regions(c, (int)w.size(), r->stream);
+  };
   uint64_t pos = 0;
   for (int s = 0; s < steps; ++s)
     for (int i = 0; i < n; ++i) {
       at[(size_t)s * n + i] = pos;
-      for (auto& w : a->wreg[i]) {
-        TA_HIP(hipMemcpyAsync(a->dLog + pos, r->dM + w.first, w.second * 4, hipMemcpyDeviceToDevice, r->stream),
-               "log");
-        pos += w.second;
-      }
+      TA_HIP(copy(i, pos, true), "log");
+      pos += a->wreg[i].size() * a->stride[i];
       int rc = run_rec_ops(a, i, i);
       if (rc != TE_OK) return rc;
     }
   for (int s = steps - 1; s >= 0; --s)
     for (int i = n - 1; i >= 0; --i) {
-      uint64_t p = at[(size_t)s * n + i];
-      for (auto& w : a->wreg[i]) {
-        TA_HIP(hipMemcpyAsync(r->dM + w.first, a->dLog + p, w.second * 4, hipMemcpyDeviceToDevice, r->stream),
-               "restore");
-        p += w.second;
-      }
+      TA_HIP(copy(i, at[(size_t)s * n + i], false), "restore");
       TA_HIP(te::adj::launch(a->adj_id[i], r->ops[i].A, reinterpret_cast<const float*>(r->dM), a->dMb, r->layout.nz,
-                             r->stream),
+                             a->dScr, r->layout.field_words, r->stream),
              std::string("adjoint of ") + r->op_names[i]);
     }
   return TE_OK;
@@ -141,13 +148,17 @@ int te_adjoint_create(te_ctx* ctx, int32_t every, te_adjoint** out) {
       if (!dup) a->wreg[i].push_back({off, L.field_words});
     }
   }
-  for (auto& v : a->wreg)
-    for (auto& w : v) a->log_words += w.second;
+  a->stride.resize(n);
+  for (int i = 0; i < n; ++i) {
+    a->stride[i] = a->wreg[i].empty() ? 0 : (a->wreg[i][0].second + 63) / 64 * 64;
+    a->log_words += a->stride[i] * a->wreg[i].size();
+  }
   hipStream_t s = r->stream;
   const uint64_t mbw = r->front + r->words;
   TA_HIP(hipMalloc((void**)&a->dMbBase, mbw * 4), "hipMalloc(adjoint memory)");
   TA_HIP(hipMemsetAsync(a->dMbBase, 0, mbw * 4, s), "clear adjoint memory");
   a->dMb = a->dMbBase + r->front;
+  TA_HIP(hipMalloc((void**)&a->dScr, 4 * L.field_words * 4), "hipMalloc(column scratch)");
   // a whole segment's write log when it fits the budget (one recompute per
   // step); otherwise one step's log and the segment's step-start states
   // (budget: TE_ADJOINT_LOG_GIB, default 96 GiB of the 288 GiB HBM)
@@ -238,7 +249,7 @@ int te_adj_ops(te_adjoint* a, int32_t first, int32_t last) {
   if (first < 0 || last >= n || first > last) return afail(a, TE_ERR_ARG, "bad dispatch range");
   for (int i = last; i >= first; --i)
     TA_HIP(te::adj::launch(a->adj_id[i], r->ops[i].A, reinterpret_cast<const float*>(r->dM), a->dMb, r->layout.nz,
-                           r->stream),
+                           a->dScr, r->layout.field_words, r->stream),
            std::string("adjoint of ") + r->op_names[i]);
   TA_HIP(hipStreamSynchronize(r->stream), "sync");
   return TE_OK;
@@ -280,6 +291,7 @@ void te_adjoint_destroy(te_adjoint* a) {
   if (a->rec && a->rec->stream) (void)hipStreamSynchronize(a->rec->stream);
   for (uint32_t* p : a->ckpt) (void)hipFree(p);
   if (a->dMbBase) (void)hipFree(a->dMbBase);
+  if (a->dScr) (void)hipFree(a->dScr);
   if (a->dLog) (void)hipFree(a->dLog);
   if (a->dSteps) (void)hipFree(a->dSteps);
   for (hipEvent_t e : {a->e0, a->e1, a->e2, a->e3})

@@ -12,8 +12,17 @@ int find(const char* name);
 int writes(int id, int out[8]);
 // Adjoint launch of one dispatch (64-word argument slot A) on value memory M
 // and adjoint memory Mb (both word 0 of the program memory); nz is the
-// grid's level count (the dims block's third word).
-hipError_t launch(int id, const uint32_t* A, const float* M, float* Mb, int nz, hipStream_t stream);
+// grid's level count (the dims block's third word); scr is 4 * fw floats of
+// scratch (fw >= one field's words) for the column solve's coefficients.
+hipError_t launch(int id, const uint32_t* A, const float* M, float* Mb, int nz, float* scr, uint64_t fw,
+                  hipStream_t stream);
+// Copies of up to 8 regions of n words each, in one launch.
+struct CopyArg {
+  const uint32_t* src[8];
+  uint32_t* dst[8];
+  uint64_t n;
+};
+hipError_t copy_regions(const CopyArg& c, int nreg, hipStream_t stream);
 
 }  // namespace adj
 }  // namespace te

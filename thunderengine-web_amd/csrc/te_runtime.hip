@@ -231,6 +231,7 @@ static const uint64_t kFrontGuard = 1u << 20;
 
 static int alloc_memory(te_ctx* c, uint64_t words, uint64_t front = kFrontGuard) {
   front = std::max(front, std::max<uint64_t>(c->front, kFrontGuard));
+  front = (front + 63) / 64 * 64;  // M starts 256-byte aligned
   uint32_t* base = nullptr;
   TE_HIP(hipMalloc((void**)&base, (front + words) * 4), "hipMalloc(M)");
   TE_HIP(hipMemsetAsync(base, 0, (front + words) * 4, c->stream), "hipMemset(M)");
