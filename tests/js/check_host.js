@@ -4,6 +4,8 @@
 //   node check_host.js encode '<json>' -> program summary of a config (CPU)
 //   node check_host.js replay <out>    -> supercell tape replayed on the GPU with hashes
 //   node check_host.js run '<json>' n  -> field FNV-1a after n steps on the GPU
+//   node check_host.js adjoint '<json>' n <out> -> gradient of sum of w^2 over a
+//        box after n checkpointed steps w.r.t. the initial th, u, v (GPU)
 const path = require('path');
 const fs = require('fs');
 const zlib = require('zlib');
@@ -59,6 +61,29 @@ if (mode === 'plan') {
   for (const name of r.arena.names) out[name] = r.fnv1a(r.arena.offsetWords(name), words);
   console.log(JSON.stringify(out));
   r.destroy();
+} else if (mode === 'adjoint') {
+  const conf = JSON.parse(process.argv[3]);
+  const n = Number(process.argv[4]);
+  const r = te.buildRun(conf);
+  r.fill();
+  const fn = new te.LargeStepFunction(r, { every: 2 });
+  fn.forward(n);
+  const L = r.layout;
+  const w = r.arena.field('w');
+  const seed = new Float32Array(w.length);
+  const i0 = L.nx >> 1, j0 = L.ny >> 1;
+  for (let k = 1; k < 6; ++k)
+    for (let j = j0 - 3; j < j0 + 3; ++j)
+      for (let i = i0 - 3; i < i0 + 3; ++i) { const c = r.arena.index(i, j, k); seed[c] = 2 * w[c]; }
+  const g = fn.backward({ w: seed });
+  const parts = ['th', 'u', 'v'].map(f => g.initial(f));
+  const out = new Float64Array(parts.reduce((a, p) => a + p.length, 0));
+  let o = 0;
+  for (const p of parts) { out.set(p, o); o += p.length; }
+  fs.writeFileSync(process.argv[5], Buffer.from(out.buffer));
+  fn.destroy();
+  r.destroy();
+  console.log('done');
 } else {
   console.error('unknown mode ' + mode);
   process.exit(2);

@@ -104,3 +104,29 @@ def test_js_run_bit_exact_vs_oracle(gpu):
     bad = [f for f in e.fields if f not in runs.FUSED_SCRATCH
            and got[f] != fnv(ref.M[e.field_offset(f): e.field_offset(f) + n])]
     assert not bad, bad
+
+
+@pytest.mark.gpu
+def test_js_adjoint_gradient_matches_the_c_abi(gpu, tmp_path):
+    """LargeStepFunction (forward / backward) gives the same gradient of
+    sum w^2 over a box w.r.t. the initial th, u, v as te_host.Adjoint."""
+    conf = dict(nx=20, ny=16, nz=10, dz=2000.0)
+    out = tmp_path / "grad.bin"
+    node("adjoint", json.dumps(conf), 2, out, timeout=600)
+    js = np.fromfile(out, dtype=np.float64)
+    run = te_host.Run(te_host.config(**conf))
+    run.init_state()
+    adj = te_host.Adjoint(run, every=2)
+    adj.forward(2)
+    end = run.field_offset(run.fields[0]) + run.field_words * len(run.fields)
+    state = run.read(0, end).view(np.float32)
+    box = te_host.Box(run, i=(7, 13), j=(5, 11), k=(1, 6))
+    seed = np.zeros(end, np.float32)
+    seed[box.words] = 2.0 * state[box.words]
+    adj.upload(0, seed)
+    adj.backward()
+    py = te_host.Controls(run, ("th", "u", "v")).pull(adj.read(0, end))
+    adj.close()
+    run.close()
+    assert js.shape == py.shape
+    assert np.linalg.norm(js - py) <= 1e-5 * np.linalg.norm(py)

@@ -211,4 +211,60 @@ function buildRun(conf, options) {
   return run;
 }
 
-module.exports = { addon, planTape, Replayer, presetToConfig, buildRun, FieldArena };
+// ---------------------------------------------------------------- adjoint
+// Autograd-style function over large steps (te_adjoint_*): forward(n) runs n
+// checkpointed steps of a run; backward(gradOutputs) takes dJ/d(state after
+// the steps) as {field: Float32Array of the field's words} and returns the
+// gradient with respect to the state before them.  No counterpart in the
+// reference (its step is forward only).
+class LargeStepFunction {
+  constructor(run, options) {
+    options = options || {};
+    this.run = run;
+    this.layout = run.layout;
+    this.adj = addon().adjointCreate(run.ctx, options.every || 10);
+  }
+  forward(nsteps) {
+    addon().fwdCkpt(this.adj, nsteps);
+    return this.run;
+  }
+  backward(gradOutputs) {
+    addon().adjZero(this.adj);
+    for (const name of Object.keys(gradOutputs)) addon().adjUpload(this.adj, this.run.arena.offsetWords(name), gradOutputs[name]);
+    addon().bwd(this.adj);
+    return new StateGradient(this);
+  }
+  lastMs() { const t = addon().adjLastMs(this.adj); return { forward: t[0], backward: t[1] }; }
+  destroy() { if (this.adj) addon().adjDestroy(this.adj); this.adj = null; }
+}
+
+// Gradient with respect to the state at the first recorded step.
+class StateGradient {
+  constructor(fn) { this.fn = fn; this.layout = fn.layout; }
+  // adjoint of every word of a field (strideJ*strideK*(nz+1) floats)
+  field(name) {
+    const L = this.layout;
+    return addon().adjDownload(this.fn.adj, this.fn.run.arena.offsetWords(name), L.strideJ * L.strideK * (L.nz + 1));
+  }
+  // gradient with respect to an initial interior field (levels 0..nz-1,
+  // i fastest): the t=0 state holds the field and its _a copy, halos filled
+  // periodically, so each interior value collects the adjoints of all the
+  // words that copy it
+  initial(name) {
+    const L = this.layout, nx = L.nx, ny = L.ny, nz = L.nz;
+    const out = new Float64Array(nx * ny * nz);
+    const names = [name].concat(L.fields[name + '_a'] !== undefined ? [name + '_a'] : []);
+    for (const f of names) {
+      const g = this.field(f);
+      for (let k = 0; k < nz; ++k)
+        for (let j = -3; j < ny + 4; ++j)
+          for (let i = -3; i < nx + 4; ++i) {
+            const si = ((i % nx) + nx) % nx, sj = ((j % ny) + ny) % ny;
+            out[si + nx * (sj + ny * k)] += g[(i + 3) + L.strideJ * ((j + 3) + L.strideK * k)];
+          }
+    }
+    return out;
+  }
+}
+
+module.exports = { addon, planTape, Replayer, presetToConfig, buildRun, FieldArena, LargeStepFunction, StateGradient };

@@ -448,6 +448,134 @@ napi_value Destroy(napi_env env, napi_callback_info info) {
   return nullptr;
 }
 
+// ---- adjoint (te_adjoint_*): externals owning a te_adjoint
+struct AdjHolder {
+  te_adjoint* adj = nullptr;
+};
+
+void finalize_adj(napi_env, void* data, void*) {
+  AdjHolder* h = static_cast<AdjHolder*>(data);
+  if (h->adj) te_adjoint_destroy(h->adj);
+  delete h;
+}
+
+te_adjoint* adj_of(napi_env env, napi_value v) {
+  void* p = nullptr;
+  if (napi_get_value_external(env, v, &p) != napi_ok || !p || !static_cast<AdjHolder*>(p)->adj) {
+    napi_throw_type_error(env, nullptr, "expected a te adjoint");
+    return nullptr;
+  }
+  return static_cast<AdjHolder*>(p)->adj;
+}
+
+napi_value throw_adj(napi_env env, const te_adjoint* a, int rc, const char* what) {
+  std::string msg = std::string(what) + " failed (" + std::to_string(rc) + "): " + te_adjoint_last_error(a);
+  napi_throw_error(env, nullptr, msg.c_str());
+  return nullptr;
+}
+
+napi_value AdjointCreate(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!get_args(env, info, 2, argv)) return nullptr;
+  te_ctx* c = ctx_of(env, argv[0]);
+  if (!c) return nullptr;
+  AdjHolder* h = new AdjHolder();
+  int rc = te_adjoint_create(c, (int32_t)num(env, argv[1]), &h->adj);
+  if (rc != TE_OK) {
+    napi_value r = throw_adj(env, h->adj, rc, "te_adjoint_create");
+    if (h->adj) te_adjoint_destroy(h->adj);
+    delete h;
+    return r;
+  }
+  napi_value r;
+  NAPI_OK(napi_create_external(env, h, finalize_adj, nullptr, &r));
+  return r;
+}
+
+napi_value FwdCkpt(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!get_args(env, info, 2, argv)) return nullptr;
+  te_adjoint* a = adj_of(env, argv[0]);
+  if (!a) return nullptr;
+  int rc = te_fwd_ckpt(a, (int32_t)num(env, argv[1]));
+  if (rc != TE_OK) return throw_adj(env, a, rc, "te_fwd_ckpt");
+  return nullptr;
+}
+
+napi_value Bwd(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!get_args(env, info, 1, argv)) return nullptr;
+  te_adjoint* a = adj_of(env, argv[0]);
+  if (!a) return nullptr;
+  int rc = te_bwd(a);
+  if (rc != TE_OK) return throw_adj(env, a, rc, "te_bwd");
+  return nullptr;
+}
+
+napi_value AdjZero(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!get_args(env, info, 1, argv)) return nullptr;
+  te_adjoint* a = adj_of(env, argv[0]);
+  if (!a) return nullptr;
+  int rc = te_adj_zero(a);
+  if (rc != TE_OK) return throw_adj(env, a, rc, "te_adj_zero");
+  return nullptr;
+}
+
+napi_value AdjUpload(napi_env env, napi_callback_info info) {
+  napi_value argv[3];
+  if (!get_args(env, info, 3, argv)) return nullptr;
+  te_adjoint* a = adj_of(env, argv[0]);
+  if (!a) return nullptr;
+  void* data = nullptr;
+  size_t bytes = 0;
+  if (!typed(env, argv[2], &data, &bytes)) return nullptr;
+  int rc = te_adj_upload(a, (uint64_t)num(env, argv[1]), static_cast<const float*>(data), bytes / 4);
+  if (rc != TE_OK) return throw_adj(env, a, rc, "te_adj_upload");
+  return nullptr;
+}
+
+napi_value AdjDownload(napi_env env, napi_callback_info info) {
+  napi_value argv[3];
+  if (!get_args(env, info, 3, argv)) return nullptr;
+  te_adjoint* a = adj_of(env, argv[0]);
+  if (!a) return nullptr;
+  uint64_t off = (uint64_t)num(env, argv[1]), n = (uint64_t)num(env, argv[2]);
+  void* data = nullptr;
+  napi_value ab, arr;
+  NAPI_OK(napi_create_arraybuffer(env, n * 4, &data, &ab));
+  int rc = te_adj_download(a, off, static_cast<float*>(data), n);
+  if (rc != TE_OK) return throw_adj(env, a, rc, "te_adj_download");
+  NAPI_OK(napi_create_typedarray(env, napi_float32_array, n, ab, 0, &arr));
+  return arr;
+}
+
+napi_value AdjLastMs(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!get_args(env, info, 1, argv)) return nullptr;
+  te_adjoint* a = adj_of(env, argv[0]);
+  if (!a) return nullptr;
+  float f = 0.0f, b = 0.0f;
+  te_adj_last_ms(a, &f, &b);
+  napi_value out;
+  NAPI_OK(napi_create_array_with_length(env, 2, &out));
+  NAPI_OK(napi_set_element(env, out, 0, make_num(env, f)));
+  NAPI_OK(napi_set_element(env, out, 1, make_num(env, b)));
+  return out;
+}
+
+napi_value AdjDestroy(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!get_args(env, info, 1, argv)) return nullptr;
+  void* p = nullptr;
+  if (napi_get_value_external(env, argv[0], &p) == napi_ok && p) {
+    AdjHolder* h = static_cast<AdjHolder*>(p);
+    if (h->adj) te_adjoint_destroy(h->adj);
+    h->adj = nullptr;
+  }
+  return nullptr;
+}
+
 napi_value Init(napi_env env, napi_value exports) {
   struct { const char* name; napi_callback fn; } fns[] = {
       {"configDefaults", ConfigDefaults}, {"create", Create}, {"programCreate", ProgramCreate},
@@ -455,6 +583,8 @@ napi_value Init(napi_env env, napi_value exports) {
       {"initState", InitState}, {"lastRunMs", LastRunMs}, {"regionHash", RegionHash}, {"fnv1a", Fnv1a},
       {"memoryWords", MemoryWords}, {"program", Program}, {"layout", Layout}, {"kernelNames", KernelNames},
       {"profileOps", ProfileOps}, {"destroy", Destroy},
+      {"adjointCreate", AdjointCreate}, {"fwdCkpt", FwdCkpt}, {"bwd", Bwd}, {"adjZero", AdjZero},
+      {"adjUpload", AdjUpload}, {"adjDownload", AdjDownload}, {"adjLastMs", AdjLastMs}, {"adjDestroy", AdjDestroy},
   };
   for (auto& f : fns) {
     napi_value v;
