@@ -29,7 +29,8 @@ def lib():
             subprocess.check_call(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-mfma", "-ffp-contract=off",
                                    "-fno-fast-math", "-o", OUT, SRC])
         L = ctypes.CDLL(OUT)
-        L.adjhost_op.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_uint32), ctypes.c_void_p, ctypes.c_void_p]
+        L.adjhost_op.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_uint32), ctypes.c_void_p, ctypes.c_void_p,
+                                 ctypes.c_int]
         L.adjhost_writes.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_int)]
         L.adjhost_name.restype = ctypes.c_char_p
         _lib = L
@@ -57,12 +58,13 @@ def writes(name, A, field_words):
     return regs
 
 
-def host_adjoint(name, A, Mw, Mb):
+def host_adjoint(name, A, Mw, Mb, scatter_only=False):
     """Adds dispatch (name, A)'s J^T to adjoint memory Mb (float32, same indexing
-    as the uint32 word memory Mw)."""
+    as the uint32 word memory Mw); scatter_only: the per-point body's scatter
+    form even where the library runs a gather-form adjoint."""
     A = np.ascontiguousarray(A, dtype=np.uint32)
     rc = lib().adjhost_op(name.encode(), A.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
-                          Mw.ctypes.data, Mb.ctypes.data)
+                          Mw.ctypes.data, Mb.ctypes.data, 1 if scatter_only else 0)
     if rc != 0:
         raise KeyError("no adjoint for " + name)
 

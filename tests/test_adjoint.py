@@ -76,6 +76,34 @@ def test_dispatch_adjoints_transpose_the_tangent_linear_build(name):
     assert len(worst) >= 30
 
 
+@pytest.mark.parametrize("name", sorted(CONFIGS))
+def test_gather_adjoints_equal_the_scatter_form(name):
+    """adv_assemble, acoustic_uv and acoustic_fpk run gather-form adjoints in
+    the library; word for word they equal the per-point bodies' scatter form
+    (up to binary32 against binary64 rounding of the Jacobian entries)."""
+    R = runs.OracleRun(te_host.config(fuse=0, **CONFIGS[name]))
+    e = R.enc
+    a0, aend = _arena(e)
+    rng = np.random.default_rng(13)
+    seen = set()
+    for i, nm in enumerate(R.names):
+        if nm in ("adv_assemble", "acoustic_uv", "acoustic_fpk"):
+            Mw = R._buf.copy()[runs.FRONT:]
+            ybar = np.zeros(aend, np.float32)
+            ybar[a0:] = rng.standard_normal(aend - a0)
+            out = []
+            for scatter in (False, True):
+                Mb = np.zeros(R._buf.size, np.float32)[runs.FRONT:]
+                Mb[:aend] = ybar
+                al.host_adjoint(nm, R.args[i], Mw, Mb, scatter_only=scatter)
+                out.append(Mb[:aend].astype(np.float64))
+            err = np.abs(out[0] - out[1]).max() / np.abs(out[1]).max()
+            assert err < 1e-5, (nm, i, err)
+            seen.add(nm)
+        R.prog.run(R.M, i, i)
+    assert seen == {"adv_assemble", "acoustic_uv", "acoustic_fpk"}
+
+
 def test_reverse_sweep_transposes_two_steps():
     """The tangent-linear build runs along the same binary32 trajectory as the
     sweep (its values reset to the oracle's float state before each

@@ -68,10 +68,10 @@ struct B_update_diagnostics_moist {
     const ADims g = r.dims();
     const int i = x - 3, j = y - 3, kk = z;
     if (i > g.nx + 3 || j > g.ny + 3 || kk >= g.nz) return;
-    const float *thP = r.f(1), *ppP = r.f(2), *th0 = r.f(3), *pi0 = r.f(4), *qv0 = r.f(5), *qc0 = r.f(6),
-                *qi0 = r.f(7), *rth0 = r.f(8), *qv = r.f(9), *qc = r.f(10), *qr = r.f(11), *qi = r.f(12),
-                *qs = r.f(13), *qg = r.f(14), *thv = r.f(15), *ppterm = r.f(16), *piadv = r.f(17), *thterm = r.f(18),
-                *buoy = r.f(19);
+    const P thP = r.f(1), ppP = r.f(2), th0 = r.f(3), pi0 = r.f(4), qv0 = r.f(5), qc0 = r.f(6),
+                qi0 = r.f(7), rth0 = r.f(8), qv = r.f(9), qc = r.f(10), qr = r.f(11), qi = r.f(12),
+                qs = r.f(13), qg = r.f(14), thv = r.f(15), ppterm = r.f(16), piadv = r.f(17), thterm = r.f(18),
+                buoy = r.f(19);
     const int64_t c = g.idx(i, j, kk);
     const D1 rd = RD, rv = RV, cp = CP, cv = CV, cpv = CPV, cvv = CVV, rovcp = ROVCP;
     const D1 reps = rv / rd;
@@ -100,8 +100,8 @@ struct B_update_diagnostics {
     const ADims g = r.dims();
     const int i = x - 3, j = y - 3, kk = z;
     if (i > g.nx + 3 || j > g.ny + 3 || kk >= g.nz) return;
-    const float *thP = r.f(1), *ppP = r.f(2), *th0 = r.f(3), *pi0 = r.f(4), *thv = r.f(5), *ppterm = r.f(6),
-                *piadv = r.f(7), *rth0 = r.f(8), *buoy = r.f(9);
+    const P thP = r.f(1), ppP = r.f(2), th0 = r.f(3), pi0 = r.f(4), thv = r.f(5), ppterm = r.f(6),
+                piadv = r.f(7), rth0 = r.f(8), buoy = r.f(9);
     const D1 rddcv = D1(RD) / (D1(CP) - D1(RD));
     const int64_t c = g.idx(i, j, kk);
     const D1 tP = r.ld(thP, c);
@@ -147,7 +147,7 @@ struct B_massflux_from_state {
     const ADims g = r.dims();
     const int i = x - 3, j = y - 3, kk = z;
     if (i > g.nx + 3 || j > g.ny + 3 || kk > g.nz) return;
-    const float *u = r.f(1), *v = r.f(2), *w = r.f(3), *rho0 = r.f(4), *rf0 = r.f(5);
+    const P u = r.f(1), v = r.f(2), w = r.f(3), rho0 = r.f(4), rf0 = r.f(5);
     const int64_t c = g.idx(i, j, kk);
     if (kk < g.nz) {
       r.out<0>(r.f(6), c, D1(rho0[kk]) * r.ld(u, c));
@@ -163,7 +163,7 @@ struct B_velocity_to_massflux {
     const ADims g = r.dims();
     const int i = x - 3, j = y - 3, kk = z;
     if (i > g.nx + 3 || j > g.ny + 3 || kk > g.nz) return;
-    const float *rho0 = r.f(1), *rf0 = r.f(2), *rru = r.f(3), *rrv = r.f(4), *rrw = r.f(5);
+    const P rho0 = r.f(1), rf0 = r.f(2), rru = r.f(3), rrv = r.f(4), rrw = r.f(5);
     const int64_t c = g.idx(i, j, kk);
     if (kk < g.nz) {
       r.out<0>(rru, c, r.ld(rru, c) * D1(rho0[kk]));
@@ -179,7 +179,7 @@ struct B_div_massflux {
     const ADims g = r.dims();
     const int i = x - 3, j = y - 3, kk = z;
     if (i > g.nx + 3 - 1 || j > g.ny + 3 - 1 || kk >= g.nz) return;
-    const float *rru = r.f(1), *rrv = r.f(2), *rrw = r.f(3), *mh = r.f(4), *hm = r.f(30);
+    const P rru = r.f(1), rrv = r.f(2), rrw = r.f(3), mh = r.f(4), hm = r.f(30);
     const int64_t sj = g.sJ, sk = g.sk(), c = g.idx(i, j, kk);
     const D1 dz = (r.ld(rrw, c + sk) - r.ld(rrw, c)) * D1(g.rdz);
     const D1 dx = (r.ld(rru, c + 1) - r.ld(rru, c)) * D1(g.rdx);
@@ -194,7 +194,7 @@ struct B_divergence3d {
     const ADims g = r.dims();
     const int i = x, j = y, kk = z;
     if (i >= g.nx || j >= g.ny || kk >= g.nz) return;
-    const float *u = r.f(1), *v = r.f(2), *w = r.f(3), *mh = r.f(4), *hm = r.f(30);
+    const P u = r.f(1), v = r.f(2), w = r.f(3), mh = r.f(4), hm = r.f(30);
     const int64_t sj = g.sJ, sk = g.sk(), c = g.idx(i, j, kk);
     const D1 dz = (r.ld(w, c + sk) - r.ld(w, c)) * D1(g.rdz);
     const D1 dx = (r.ld(u, c + 1) - r.ld(u, c)) * D1(g.rdx);
@@ -210,7 +210,7 @@ struct B_buoyancy_apply {
     const ADims g = r.dims();
     const int i = x, j = y, kk = z + 1;
     if (i >= g.nx || j >= g.ny || kk > g.nz - 1) return;
-    const float *buoy = r.f(1), *c1 = r.f(2), *c2 = r.f(3), *wten = r.f(4);
+    const P buoy = r.f(1), c1 = r.f(2), c2 = r.f(3), wten = r.f(4);
     const int64_t sk = g.sk(), c = g.idx(i, j, kk);
     const D1 b = fma_(D1(c1[kk]), r.ld(buoy, c - sk), D1(c2[kk]) * r.ld(buoy, c));
     r.out<0>(wten, c, r.ld(wten, c) + b);
@@ -247,7 +247,7 @@ struct B_apply_thterm {
     const ADims g = r.dims();
     if (x >= g.nx || y >= g.ny || z >= g.nz) return;
     const int64_t c = g.idx(x, y, z);
-    const float* tend = r.f(3);
+    const P tend = r.f(3);
     r.out<0>(tend, c, fma_(-r.ld(r.f(1), c), r.ld(r.f(2), c), r.ld(tend, c)));
   }
 };
@@ -259,7 +259,7 @@ struct B_turb_add_tendency {
     const int i = x - 3, j = y - 3, kk = z;
     if (i > g.nx + 3 || j > g.ny + 3 || kk > g.nz) return;
     const int64_t c = g.idx(i, j, kk);
-    const float* dst = r.f(2);
+    const P dst = r.f(2);
     r.out<0>(dst, c, fma_(D1(r.F(3)), r.ld(r.f(1), c), r.ld(dst, c)));
   }
 };
@@ -279,7 +279,7 @@ struct B_ppx_finish {
     const ADims g = r.dims();
     if (x >= g.nx || y >= g.ny || z >= g.nz) return;
     const int64_t c = g.idx(x, y, z);
-    const float* ppx = r.f(2);
+    const P ppx = r.f(2);
     r.out<0>(ppx, c, r.ld(r.f(1), c) + r.ld(ppx, c));
   }
 };
@@ -291,8 +291,8 @@ struct B_diabatic_unapply {
     if (x >= g.nx || y >= g.ny || z >= g.nz) return;
     const int64_t c = g.idx(x, y, z);
     const D1 mdt = D1(-r.F(1));
-    const float *pp3d = r.f(2), *th = r.f(3), *qv = r.f(4), *qc = r.f(5), *qpten = r.f(6), *qtten = r.f(7),
-                *qvten = r.f(8), *qcten = r.f(9);
+    const P pp3d = r.f(2), th = r.f(3), qv = r.f(4), qc = r.f(5), qpten = r.f(6), qtten = r.f(7),
+                qvten = r.f(8), qcten = r.f(9);
     const D1 p = fma_(mdt, r.ld(qpten, c), r.ld(pp3d, c));
     const D1 t = fma_(mdt, r.ld(qtten, c), r.ld(th, c));
     const D1 v = fma_(mdt, r.ld(qvten, c), r.ld(qv, c));
@@ -311,8 +311,8 @@ struct B_diabatic_rates {
     if (x >= g.nx || y >= g.ny || z >= g.nz) return;
     const int64_t c = g.idx(x, y, z);
     const D1 rdt = D1(1.0f) / D1(r.F(1));
-    const float *pp3d = r.f(2), *th = r.f(3), *qv = r.f(4), *qc = r.f(5), *qpten = r.f(6), *qtten = r.f(7),
-                *qvten = r.f(8), *qcten = r.f(9);
+    const P pp3d = r.f(2), th = r.f(3), qv = r.f(4), qc = r.f(5), qpten = r.f(6), qtten = r.f(7),
+                qvten = r.f(8), qcten = r.f(9);
     r.out<0>(qpten, c, (r.ld(pp3d, c) - r.ld(qpten, c)) * rdt);
     r.out<1>(qtten, c, (r.ld(th, c) - r.ld(qtten, c)) * rdt);
     r.out<2>(qvten, c, (r.ld(qv, c) - r.ld(qvten, c)) * rdt);
@@ -326,8 +326,8 @@ struct B_pressure_tendency {
     const ADims g = r.dims();
     if (x >= g.nx || y >= g.ny || z >= g.nz) return;
     const int64_t c = g.idx(x, y, z);
-    const float *th = r.f(1), *qv = r.f(2), *rho = r.f(3), *ppi = r.f(4), *thten = r.f(5), *qpten = r.f(6),
-                *th0 = r.f(7), *pi0 = r.f(8), *ppten = r.f(11), *qvt = r.f(13);
+    const P th = r.f(1), qv = r.f(2), rho = r.f(3), ppi = r.f(4), thten = r.f(5), qpten = r.f(6),
+                th0 = r.f(7), pi0 = r.f(8), ppten = r.f(11), qvt = r.f(13);
     const D1 dt = D1(r.F(9));
     const int hasDiabatic = r.I(10), hasStepTend = r.I(12), moist = r.I(14);
     const D1 rd = RD, rv = RV, cp = CP, p00 = P00;
@@ -365,7 +365,7 @@ struct B_update_rho_prs {
     if (x >= g.nx || y >= g.ny || z >= g.nz) return;
     const int64_t c = g.idx(x, y, z);
     const int moist = r.I(1);
-    const float *thP = r.f(2), *ppP = r.f(3), *th0 = r.f(4), *pi0 = r.f(5), *qv = r.f(6);
+    const P thP = r.f(2), ppP = r.f(3), th0 = r.f(4), pi0 = r.f(5), qv = r.f(6);
     const float cpdrd = CP / RD;
     const D1 pi = D1(pi0[z]) + r.ld(ppP, c);
     const D1 th = D1(th0[z]) + r.ld(thP, c);
@@ -384,7 +384,7 @@ struct B_adv_flux_h {
   static TE_HD void run(Acc& r, int i, int j, int kk) {
     const ADims g = r.dims();
     if (DIR == 0 ? (i > g.nx || j >= g.ny || kk >= g.nz) : (i >= g.nx || j > g.ny || kk >= g.nz)) return;
-    const float *s = r.f(1), *rr = r.f(2);
+    const P s = r.f(1), rr = r.f(2);
     const int64_t st = DIR == 0 ? 1 : (int64_t)g.sJ, c = g.idx(i, j, kk);
     const D1 u = r.ld(rr, c);
     D1 f;
@@ -399,7 +399,7 @@ struct B_adv_flux_z {
   static TE_HD void run(Acc& r, int i, int j, int kk) {
     const ADims g = r.dims();
     if (i >= g.nx || j >= g.ny || kk > g.nz) return;
-    const float *s = r.f(1), *rrw = r.f(2), *c1 = r.f(3), *c2 = r.f(4), *fz = r.f(5);
+    const P s = r.f(1), rrw = r.f(2), c1 = r.f(3), c2 = r.f(4), fz = r.f(5);
     const int64_t sk = g.sk(), c = g.idx(i, j, kk);
     if (kk == 0 || kk == g.nz) { r.out<0>(fz, c, D1(0.0f)); return; }
     const D1 w = r.ld(rrw, c);
@@ -421,7 +421,7 @@ struct B_adv_flux_h_weno {
   static TE_HD void run(Acc& r, int i, int j, int kk) {
     const ADims g = r.dims();
     if (i > g.nx || j > g.ny || kk >= g.nz) return;
-    const float *a = r.f(1), *rr = r.f(2), *fl = r.f(5);
+    const P a = r.f(1), rr = r.f(2), fl = r.f(5);
     const float weps = r.F(3);
     const bool pdef = r.I(4) == 1;
     const int64_t st = DIR == 0 ? 1 : (int64_t)g.sJ, c = g.idx(i, j, kk);
@@ -443,7 +443,7 @@ struct B_adv_flux_z_weno {
     const int kf = z + 1;
     if (i >= g.nx || j >= g.ny || kf > g.nz + 1) return;
     const int kk = kf - 1;
-    const float *a = r.f(1), *rrw = r.f(2), *c1 = r.f(3), *c2 = r.f(4), *fz = r.f(8);
+    const P a = r.f(1), rrw = r.f(2), c1 = r.f(3), c2 = r.f(4), fz = r.f(8);
     const float weps = r.F(5);
     const int pdefweno = r.I(7);
     const int64_t sk = g.sk(), c = g.idx(i, j, kk);
@@ -482,7 +482,7 @@ struct B_adv_flux_z_weno {
 };
 
 // f_flx5_upwind (adv_flux_x_stag.wgsl:106-128)
-TE_HD D1 flx5_upwind(Acc& r, const float* a, int64_t c, int64_t s, bool up, bool shifted) {
+TE_HD D1 flx5_upwind(Acc& r, P a, int64_t c, int64_t s, bool up, bool shifted) {
   if (shifted)
     return up ? flx5(r.ld(a, c - 2 * s), r.ld(a, c - s), r.ld(a, c), r.ld(a, c + s), r.ld(a, c + 2 * s))
               : flx5(r.ld(a, c + 3 * s), r.ld(a, c + 2 * s), r.ld(a, c + s), r.ld(a, c), r.ld(a, c - s));
@@ -490,7 +490,7 @@ TE_HD D1 flx5_upwind(Acc& r, const float* a, int64_t c, int64_t s, bool up, bool
             : flx5(r.ld(a, c + 2 * s), r.ld(a, c + s), r.ld(a, c), r.ld(a, c - s), r.ld(a, c - 2 * s));
 }
 // f_weno5_upwind (adv_flux_x_weno_uv.wgsl:106-129)
-TE_HD D1 weno5_upwind(Acc& r, const float* a, int64_t c, int64_t s, bool up, bool shifted, float weps) {
+TE_HD D1 weno5_upwind(Acc& r, P a, int64_t c, int64_t s, bool up, bool shifted, float weps) {
   if (shifted)
     return up ? weno5(r.ld(a, c - 2 * s), r.ld(a, c - s), r.ld(a, c), r.ld(a, c + s), r.ld(a, c + 2 * s), weps)
               : weno5(r.ld(a, c + 3 * s), r.ld(a, c + 2 * s), r.ld(a, c + s), r.ld(a, c), r.ld(a, c - s), weps);
@@ -506,7 +506,7 @@ struct B_adv_flux_h_stag {
     const int i = x - 1, j = y - 1;
     if (i > g.nx || j > g.ny || kk >= g.nz) return;
     const int stag = r.I(1);
-    const float *a = r.f(2), *rr = DIR == 0 ? r.f(3) : r.f(4), *c2 = r.f(7), *fl = r.f(8);
+    const P a = r.f(2), rr = DIR == 0 ? r.f(3) : r.f(4), c2 = r.f(7), fl = r.f(8);
     const int64_t sj = g.sJ, sk = g.sk(), st = DIR == 0 ? 1 : sj, c = g.idx(i, j, kk);
     if (stag == 4 && kk == 0) { r.out<0>(fl, c, D1(0.0f)); return; }
     D1 m;
@@ -524,7 +524,7 @@ struct B_adv_flux_z_stag {
     const ADims g = r.dims();
     if (i > g.nx || j > g.ny || kk > g.nz) return;
     const int stag = r.I(1);
-    const float *a = r.f(2), *rrw = r.f(5), *c1 = r.f(6), *c2 = r.f(7), *fz = r.f(8);
+    const P a = r.f(2), rrw = r.f(5), c1 = r.f(6), c2 = r.f(7), fz = r.f(8);
     const int64_t sj = g.sJ, sk = g.sk(), c = g.idx(i, j, kk);
     // the forward kernel forms the face mass flux before its range checks
     D1 m;
@@ -560,7 +560,7 @@ struct B_adv_flux_h_weno_w {
     const ADims g = r.dims();
     const int i = x - 1, j = y - 1;
     if (i < 0 || j < 0 || i > g.nx || j > g.ny || kk >= g.nz) return;
-    const float *a = r.f(1), *rr = r.f(2), *c2 = r.f(3), *fl = r.f(5);
+    const P a = r.f(1), rr = r.f(2), c2 = r.f(3), fl = r.f(5);
     const float weps = r.F(4);
     const int64_t sk = g.sk(), st = DIR == 0 ? 1 : (int64_t)g.sJ, c = g.idx(i, j, kk);
     if (kk == 0) { r.out<0>(fl, c, D1(0.0f)); return; }
@@ -575,7 +575,7 @@ struct B_adv_flux_z_weno_w {
   static TE_HD void run(Acc& r, int i, int j, int kk) {
     const ADims g = r.dims();
     if (i > g.nx || j > g.ny || kk > g.nz) return;
-    const float *a = r.f(1), *rrw = r.f(2), *fz = r.f(4);
+    const P a = r.f(1), rrw = r.f(2), fz = r.f(4);
     const float weps = r.F(3);
     const int64_t sk = g.sk(), c = g.idx(i, j, kk);
     if (kk > g.nz - 1) { r.out<0>(fz, c, D1(0.0f)); return; }
@@ -600,7 +600,7 @@ struct B_adv_flux_h_weno_uv {
     const int i = x - 1, j = y - 1;
     if (i > g.nx || j > g.ny || kk >= g.nz) return;
     const int stag = r.I(1);
-    const float *a = r.f(2), *rr = r.f(3), *fl = r.f(5);
+    const P a = r.f(2), rr = r.f(3), fl = r.f(5);
     const float weps = r.F(4);
     const int64_t sj = g.sJ, st = DIR == 0 ? 1 : sj, c = g.idx(i, j, kk);
     D1 m;
@@ -616,7 +616,7 @@ struct B_adv_flux_z_weno_uv {
     const ADims g = r.dims();
     if (i > g.nx || j > g.ny || kk > g.nz) return;
     const int stag = r.I(1);
-    const float *a = r.f(2), *rrw = r.f(3), *c1 = r.f(4), *c2 = r.f(5), *fz = r.f(7);
+    const P a = r.f(2), rrw = r.f(3), c1 = r.f(4), c2 = r.f(5), fz = r.f(7);
     const float weps = r.F(6);
     const int64_t sj = g.sJ, sk = g.sk(), c = g.idx(i, j, kk);
     if (kk == 0 || kk == g.nz) { r.out<0>(fz, c, D1(0.0f)); return; }
@@ -646,9 +646,9 @@ struct B_adv_assemble {
     const int openx = r.I(17, 0), openy = r.I(17, 1);
     if (openx != 0 && st == 2 && (i == 0 || i == g.nx)) return;
     if (openy != 0 && st == 3 && (j == 0 || j == g.ny)) return;
-    const float *s = r.f(2), *fx = r.f(3), *fy = r.f(4), *fz = r.f(5), *divx = r.f(6), *rho0 = r.f(7), *rf0 = r.f(8),
-                *mh = r.f(9), *mf = r.f(10), *tend = r.f(11), *rru = r.f(13), *rrv = r.f(14), *c1 = r.f(15),
-                *c2 = r.f(16), *hm = r.f(30);
+    const P s = r.f(2), fx = r.f(3), fy = r.f(4), fz = r.f(5), divx = r.f(6), rho0 = r.f(7), rf0 = r.f(8),
+                mh = r.f(9), mf = r.f(10), tend = r.f(11), rru = r.f(13), rrv = r.f(14), c1 = r.f(15),
+                c2 = r.f(16), hm = r.f(30);
     const int64_t sj = g.sJ, sk = g.sk(), c = g.idx(i, j, kk);
     D1 dFx = st == 2 ? r.ld(fx, c) - r.ld(fx, c - 1) : r.ld(fx, c + 1) - r.ld(fx, c);
     D1 dFy = st == 3 ? r.ld(fy, c) - r.ld(fy, c - sj) : r.ld(fy, c + sj) - r.ld(fy, c);
@@ -720,8 +720,8 @@ struct B_acoustic_prepare {
     const ADims g = r.dims();
     if (i >= g.nx || j >= g.ny || kk >= g.nz) return;
     const float dts = r.F(1, 0);
-    const float *ppterm = r.f(2), *piadv = r.f(3), *thv = r.f(4), *rr0 = r.f(5), *rf0 = r.f(6), *mh = r.f(7),
-                *mf = r.f(8), *c1 = r.f(9), *c2 = r.f(10);
+    const P ppterm = r.f(2), piadv = r.f(3), thv = r.f(4), rr0 = r.f(5), rf0 = r.f(6), mh = r.f(7),
+                mf = r.f(8), c1 = r.f(9), c2 = r.f(10);
     const int64_t sk = g.sk(), c = g.idx(i, j, kk);
     const D1 cp = CP;
     const D1 tem = (D1(dts) * D1(g.rdz)) * D1(mh[kk]);
@@ -767,7 +767,7 @@ struct B_acoustic_uv {
     const ADims g = r.dims();
     if (i > g.nx || j > g.ny || kk >= g.nz) return;
     const D1 dts = D1(r.F(1, 0));
-    const float *ppd = r.f(2), *thv = r.f(3), *uten = r.f(4), *vten = r.f(5), *u = r.f(6), *v = r.f(7), *hm = r.f(30);
+    const P ppd = r.f(2), thv = r.f(3), uten = r.f(4), vten = r.f(5), u = r.f(6), v = r.f(7), hm = r.f(30);
     const int openx = r.I(8, 0), openy = r.I(8, 1);
     const int64_t sj = g.sJ, c = g.idx(i, j, kk);
     const D1 cp = CP;
@@ -791,8 +791,8 @@ struct B_acoustic_fpk {
     if (i >= g.nx || j >= g.ny || kk >= g.nz) return;
     const D1 dts = D1(r.F(1, 0)), beta = D1(r.F(1, 2));
     const float smeps = r.F(1, 4);
-    const float *u = r.f(2), *v = r.f(3), *w = r.f(4), *piadv = r.f(5), *ppten = r.f(6), *ppterm = r.f(7),
-                *pk1 = r.f(8), *pk2 = r.f(9), *qk = r.f(10), *hm = r.f(30);
+    const P u = r.f(2), v = r.f(3), w = r.f(4), piadv = r.f(5), ppten = r.f(6), ppterm = r.f(7),
+                pk1 = r.f(8), pk2 = r.f(9), qk = r.f(10), hm = r.f(30);
     const int64_t sj = g.sJ, sk = g.sk(), c = g.idx(i, j, kk);
     const D1 temx = (dts * D1(0.5f)) * D1(g.rdx);
     const D1 temy = (dts * D1(0.5f)) * D1(g.rdy);
@@ -816,8 +816,8 @@ struct B_acoustic_pp {
     if (i >= g.nx || j >= g.ny || kk >= g.nz) return;
     const D1 alph = D1(r.F(1, 1)), kdiv = D1(r.F(1, 3));
     const float smeps = r.F(1, 4);
-    const float *w = r.f(2), *pk1 = r.f(3), *pk2 = r.f(4), *qk = r.f(5), *fpk = r.f(6), *pp3d = r.f(7), *ppd = r.f(8),
-                *ppx = r.f(9);
+    const P w = r.f(2), pk1 = r.f(3), pk2 = r.f(4), qk = r.f(5), fpk = r.f(6), pp3d = r.f(7), ppd = r.f(8),
+                ppx = r.f(9);
     const int saveX = r.I(10);
     const int64_t sk = g.sk(), c = g.idx(i, j, kk);
     const D1 prev = r.ld(pp3d, c);
@@ -838,7 +838,7 @@ struct B_acoustic_accumulate {
     const int i = x - 3, j = y - 3;
     if (i > g.nx + 3 || j > g.ny + 3 || kk > g.nz) return;
     const D1 alph = D1(r.F(1, 1));
-    const float *u = r.f(2), *v = r.f(3), *w = r.f(4), *rru = r.f(5), *rrv = r.f(6), *rrw = r.f(7);
+    const P u = r.f(2), v = r.f(3), w = r.f(4), rru = r.f(5), rrv = r.f(6), rrw = r.f(7);
     const float tavg = r.F(8);
     const int64_t c = g.idx(i, j, kk);
     if (tavg > 0.0f) {
@@ -874,8 +874,8 @@ struct B_pdef_dum {
     const ADims g = r.dims();
     if (i >= g.nx || j >= g.ny || kk >= g.nz) return;
     const int dir = r.I(1);
-    const float *s = r.f(2), *s0 = r.f(3), *f = r.f(4), *rr = r.f(5), *rho0 = r.f(6), *mh = r.f(7), *dum = r.f(10),
-                *hm = r.f(30);
+    const P s = r.f(2), s0 = r.f(3), f = r.f(4), rr = r.f(5), rho0 = r.f(6), mh = r.f(7), dum = r.f(10),
+                hm = r.f(30);
     const D1 dt = D1(r.F(8));
     const int openx = r.I(9, 0), openy = r.I(9, 1);
     const int64_t c = g.idx(i, j, kk);
@@ -923,9 +923,9 @@ struct B_pdef_apply {
   static TE_HD void run(Acc& r, int i, int j, int kk) {
     const ADims g = r.dims();
     if (i >= g.nx || j >= g.ny || kk >= g.nz) return;
-    const float *s = r.f(1), *s0 = r.f(2), *fx = r.f(3), *fy = r.f(4), *fz = r.f(5), *rru = r.f(6), *rrv = r.f(7),
-                *rrw = r.f(8), *dx = r.f(9), *dy = r.f(10), *dz = r.f(11), *rho0 = r.f(12), *mh = r.f(13),
-                *tend = r.f(16), *hm = r.f(30);
+    const P s = r.f(1), s0 = r.f(2), fx = r.f(3), fy = r.f(4), fz = r.f(5), rru = r.f(6), rrv = r.f(7),
+                rrw = r.f(8), dx = r.f(9), dy = r.f(10), dz = r.f(11), rho0 = r.f(12), mh = r.f(13),
+                tend = r.f(16), hm = r.f(30);
     const float dt = r.F(14);
     const int openx = r.I(15, 0), openy = r.I(15, 1);
     const int64_t sj = g.sJ, sk = g.sk(), c = g.idx(i, j, kk);
@@ -984,7 +984,7 @@ struct B_bc_periodic {
     const ADims g = r.dims();
     const int q = y - 3, n = DIR == 0 ? g.nx : g.ny;
     if (h > 3 || q > (DIR == 0 ? g.ny : g.nx) + 3 || kk > g.nz) return;
-    const float* f = r.f(1);
+    const P f = r.f(1);
     const int lo = (((-1 - h) % n) + n) % n;
     const int hi = (((n + h) % n) + n) % n;
     auto at = [&](int p) { return DIR == 0 ? g.idx(p, q, kk) : g.idx(q, p, kk); };
@@ -1006,8 +1006,8 @@ struct B_rayleigh_damping {
   static TE_HD void run(Acc& r, int i, int j, int kk) {
     const ADims g = r.dims();
     if (i > g.nx || j > g.ny || kk > g.nz) return;
-    const float *u = r.f(1), *v = r.f(2), *w = r.f(3), *th = r.f(4), *u0 = r.f(5), *v0 = r.f(6), *tauh = r.f(7),
-                *tauf = r.f(8), *hdx = r.f(14), *hdy = r.f(15), *avg = r.f(18), *th0 = r.f(19);
+    const P u = r.f(1), v = r.f(2), w = r.f(3), th = r.f(4), u0 = r.f(5), v0 = r.f(6), tauh = r.f(7),
+                tauf = r.f(8), hdx = r.f(14), hdy = r.f(15), avg = r.f(18), th0 = r.f(19);
     const D1 rdalpha = D1(r.F(9));
     const int lateral = r.I(16), toAverage = r.I(17);
     const int64_t c = g.idx(i, j, kk);
@@ -1050,8 +1050,8 @@ struct WCoef {
 };
 // lam_1 (acoustic_w_solve.wgsl:124-134) on dual numbers
 TE_HD WCoef w_coef(Acc& r, int64_t c, int64_t sk, float asq, float dts, float alph, float beta) {
-  const float *pk1 = r.f(2), *pk2 = r.f(3), *qk = r.f(4), *mm = r.f(5), *fpk = r.f(6), *wten = r.f(7), *pp3d = r.f(8),
-              *w = r.f(9);
+  const P pk1 = r.f(2), pk2 = r.f(3), qk = r.f(4), mm = r.f(5), fpk = r.f(6), wten = r.f(7), pp3d = r.f(8),
+              w = r.f(9);
   const int64_t cm = c - sk;
   WCoef q;
   const D1 m = r.ld(mm, c);
@@ -1079,55 +1079,55 @@ TE_HD void w_solve_coef(const float* M, const AdjArg& a, float* scr, uint64_t fw
 }
 
 // Column part: forward factors, the new w, U^T z = wbar and L^T dbar = z,
-// from the coefficient planes.  Leaves dbar_k in the adjoint words of cA and
-// the new w in those of cB (levels 1..nz-1) and clears wbar for the inputs'
-// adjoints.
-TE_HD void w_solve_sweep(const float* M, float* Mb, const AdjArg& a, const float* scr, uint64_t fw, int i, int j) {
+// from the coefficient planes (binary32, the forward's recurrences).  tA / tB
+// (stride ts) hold cA and cB, then z, for the column (LDS on the device);
+// the reciprocal pivots replace c in its plane.  Leaves dbar_k in the adjoint
+// words of cA and the new w in those of cB (levels 1..nz-1) and clears wbar
+// for the inputs' adjoints.
+TE_HD void w_solve_sweep(const float* M, float* Mb, const AdjArg& a, float* scr, uint64_t fw, float* tA, float* tB,
+                         int64_t ts, int i, int j) {
   Acc r(M, Mb, &a);
   const ADims g = r.dims();
   if (i >= g.nx || j >= g.ny) return;
   const int nz = g.nz;
   const int64_t sk = g.sk(), c0 = g.idx(i, j, 0);
-  const float *AA = scr + c0, *BB = scr + fw + c0, *CC = scr + 2 * fw + c0, *DD = scr + 3 * fw + c0;
+  const float *AA = scr + c0, *BB = scr + fw + c0, *DD = scr + 3 * fw + c0;
+  float* RR = scr + 2 * fw + c0;  // c, then 1 / pivot
   float* wb = Mb + a.p[9] + c0;   // adjoint of w (column)
-  float* S1 = Mb + a.p[10] + c0;  // adjoint words of cA: cA_k, then dbar_k
-  float* S2 = Mb + a.p[11] + c0;  // adjoint words of cB: cB_k, then the new w
-  // forward factors and the new w (the forward's recurrences, in binary64)
-  double cAp = 0.0, cBp = 0.0;
+  float* S1 = Mb + a.p[10] + c0;  // adjoint words of cA: dbar_k
+  float* S2 = Mb + a.p[11] + c0;  // adjoint words of cB: the new w
+  float cAp = 0.0f, cBp = 0.0f;
   for (int k = 1; k <= nz - 1; ++k) {
-    const double aa = AA[k * sk], bb = BB[k * sk], cc = CC[k * sk], dd = DD[k * sk];
-    const double rr = k == 1 ? 1.0 / bb : 1.0 / fmad_(aa, cAp, bb);
-    const double cB = (k == 1 ? dd : fmad_(-aa, cBp, dd)) * rr;
+    const float aa = AA[k * sk], bb = BB[k * sk], cc = RR[k * sk], dd = DD[k * sk];
+    const float rr = 1.0f / (k == 1 ? bb : fmaf_(aa, cAp, bb));
     cAp = (-cc) * rr;
-    cBp = cB;
-    S1[k * sk] = (float)cAp;
-    S2[k * sk] = (float)cB;  // cB_k, or w[nz-1]
+    cBp = (k == 1 ? dd : fmaf_(-aa, cBp, dd)) * rr;
+    RR[k * sk] = rr;
+    tA[k * ts] = cAp;
+    tB[k * ts] = cBp;
   }
-  double wn = S2[(nz - 1) * sk];
+  float wn = cBp;  // w[nz-1]
+  S2[(nz - 1) * sk] = wn;
   for (int k = nz - 2; k >= 1; --k) {
-    wn = fmad_(S1[k * sk], wn, S2[k * sk]);
-    S2[k * sk] = (float)wn;
+    wn = fmaf_(tA[k * ts], wn, tB[k * ts]);
+    S2[k * sk] = wn;
   }
-  // U^T z = wbar (z kept in wb), levels 0 and nz are constant outputs
-  wb[0] = 0.0f;
-  wb[nz * sk] = 0.0f;
-  double z = wb[sk];
+  // U^T z = wbar (z in tB), levels 0 and nz are constant outputs
+  float z = wb[sk];
+  tB[ts] = z;
   for (int k = 2; k <= nz - 1; ++k) {
-    z = fmad_(S1[(k - 1) * sk], z, wb[k * sk]);
-    wb[k * sk] = (float)z;
+    z = fmaf_(tA[(k - 1) * ts], z, wb[k * sk]);
+    tB[k * ts] = z;
   }
-  // L^T dbar = z, from the top (dbar kept in S1, wb cleared for the inputs' adjoints)
-  double aNext = 0.0, yNext = 0.0;
+  // L^T dbar = z, from the top
+  float aNext = 0.0f, yNext = 0.0f;
   for (int k = nz - 1; k >= 1; --k) {
-    const double aa = AA[k * sk], bb = BB[k * sk];
-    const double rr = k == 1 ? 1.0 / bb : 1.0 / fmad_(aa, S1[(k - 1) * sk], bb);
-    const double zk = wb[k * sk];
-    const double y = (k == nz - 1 ? zk : fmad_(-aNext, yNext, zk)) * rr;
-    S1[k * sk] = (float)y;
-    wb[k * sk] = 0.0f;
-    aNext = aa;
+    const float y = (k == nz - 1 ? tB[k * ts] : fmaf_(-aNext, yNext, tB[k * ts])) * RR[k * sk];
+    S1[k * sk] = y;
+    aNext = AA[k * sk];
     yNext = y;
   }
+  for (int k = 0; k <= nz; ++k) wb[k * sk] = 0.0f;
 }
 
 // Level part, one (i, j, k), k = 1..nz-1, after every column's sweep:
@@ -1168,6 +1168,228 @@ TE_HD void w_solve_clear(float* Mb, const AdjArg& a, const ADims& g, int i, int 
     Mb[a.p[11] + c0 + k * sk] = 0.0f;
   }
 }
+
+// ---- gather-form adjoints of linear stencils.  One thread per word q of the
+// field grid (all (nx+7)(ny+7)(nz+1) words) adds to each input field's adjoint
+// at q the contributions of every cell whose stencil reads q: no atomics,
+// single pass, binary32 (the local Jacobian entries are the forward's own
+// values).  Outputs the dispatch overwrites are first copied to a scratch
+// plane and cleared (prep, over the forward grid), since neighbouring threads
+// read them.  Periodic boundaries only (programs with open boundaries have no
+// adjoint).
+
+// adv_assemble.wgsl:139-279: tend = (s dv - divF) / rho (+ tend when accumulating)
+struct G_adv_assemble {
+  static TE_HD bool cell(const AdjArg& a, const ADims& g, int st, int i, int j, int k) {
+    const int iMax = st == 2 ? g.nx : g.nx - 1, jMax = st == 3 ? g.ny : g.ny - 1, kLo = st == 4 ? 1 : 0;
+    return i >= 0 && j >= 0 && i <= iMax && j <= jMax && k >= kLo && k <= g.nz - 1 && i < (int)a.n[0] &&
+           j < (int)a.n[1] && k < (int)a.n[2];
+  }
+  static TE_HD void prep(const float* M, float* Mb, const AdjArg& a, float* scr, int i, int j, int k) {
+    Acc r(M, Mb, &a);
+    const ADims g = r.dims();
+    const int stag = r.I(1), st = stag < 0 ? -stag : stag;
+    if (!cell(a, g, st, i, j, k)) return;
+    const int64_t c = g.idx(i, j, k), w = (int64_t)a.p[11] + c;
+    scr[c] = Mb[w];
+    if (r.I(12) == 0) Mb[w] = 0.0f;
+  }
+  static TE_HD void gather(const float* M, float* Mb, const AdjArg& a, const float* scr, int x, int y, int k) {
+    Acc r(M, Mb, &a);
+    const ADims g = r.dims();
+    const int i = x - 3, j = y - 3;
+    const int stag = r.I(1), st = stag < 0 ? -stag : stag;
+    const P s = r.f(2), divx = r.f(6), rho0 = r.f(7), rf0 = r.f(8), mh = r.f(9), mf = r.f(10),
+                c1 = r.f(15), c2 = r.f(16), hm = r.f(30);
+    const int64_t sj = g.sJ, sk = g.sk(), q = g.idx(i, j, k);
+    auto ok = [&](int ii, int jj, int kk) { return cell(a, g, st, ii, jj, kk); };
+    auto rho = [&](int kk) { return st == 4 ? rf0[kk] : rho0[kk]; };
+    // -tbar / rho times the flux difference's factor, per direction (0 off the cells)
+    auto gx = [&](int ii) { return ok(ii, j, k) ? ((-scr[g.idx(ii, j, k)] / rho(k)) * g.rdx) * (st == 2 ? hm_uf(hm, g, ii) : hm_uh(hm, g, ii)) : 0.0f; };
+    auto gy = [&](int jj) { return ok(i, jj, k) ? ((-scr[g.idx(i, jj, k)] / rho(k)) * g.rdy) * (st == 3 ? hm_vf(hm, g, jj) : hm_vh(hm, g, jj)) : 0.0f; };
+    auto gz = [&](int kk) {
+      if (!ok(i, j, kk)) return 0.0f;
+      const float mz = stag < 0 ? 1.0f : (stag == 4 ? mf[kk] : mh[kk]);
+      return ((-scr[g.idx(i, j, kk)] / rho(kk)) * g.rdz) * mz;
+    };
+    const float fxb = st == 2 ? gx(i) - gx(i + 1) : gx(i - 1) - gx(i);
+    const float fyb = st == 3 ? gy(j) - gy(j + 1) : gy(j - 1) - gy(j);
+    const float fzb = st == 4 ? gz(k) - gz(k + 1) : gz(k - 1) - gz(k);
+    // tbar s / rho at a cell: the adjoint of its dv
+    auto dvb = [&](int ii, int jj, int kk) { return ok(ii, jj, kk) ? (scr[g.idx(ii, jj, kk)] * s[g.idx(ii, jj, kk)]) / rho(kk) : 0.0f; };
+    float divb;
+    if (st == 2) divb = 0.5f * dvb(i, j, k) + 0.5f * dvb(i + 1, j, k);
+    else if (st == 3) divb = 0.5f * dvb(i, j, k) + 0.5f * dvb(i, j + 1, k);
+    else if (st == 4) divb = (ok(i, j, k) ? c2[k] * dvb(i, j, k) : 0.0f) + (ok(i, j, k + 1) ? c1[k + 1] * dvb(i, j, k + 1) : 0.0f);
+    else divb = dvb(i, j, k);
+    float sb = 0.0f;
+    if (ok(i, j, k)) {
+      float dv;
+      if (st == 2) dv = 0.5f * (divx[q] + divx[q - 1]);
+      else if (st == 3) dv = 0.5f * (divx[q] + divx[q - sj]);
+      else if (st == 4) dv = fmaf_(c2[k], divx[q], c1[k] * divx[q - sk]);
+      else dv = divx[q];
+      sb = (scr[q] * dv) / rho(k);
+    }
+    if (fxb != 0.0f) Mb[a.p[3] + q] += fxb;
+    if (fyb != 0.0f) Mb[a.p[4] + q] += fyb;
+    if (fzb != 0.0f) Mb[a.p[5] + q] += fzb;
+    if (divb != 0.0f) Mb[a.p[6] + q] += divb;
+    if (sb != 0.0f) Mb[a.p[2] + q] += sb;
+  }
+  static constexpr bool kPrep = true;
+};
+
+// acoustic_uv.wgsl:144-181: u += dts (uten - P (thv + thv[-1])), P = tem1 (ppd - ppd[-1]) uf
+// (likewise v); u and v are updated in place, so their adjoints pass through.
+struct G_acoustic_uv {
+  static TE_HD void prep(const float*, float*, const AdjArg&, float*, int, int, int) {}
+  static TE_HD void gather(const float* M, float* Mb, const AdjArg& a, const float*, int x, int y, int k) {
+    Acc r(M, Mb, &a);
+    const ADims g = r.dims();
+    const int i = x - 3, j = y - 3;
+    if (k >= g.nz) return;
+    const float dts = r.F(1, 0);
+    const P ppd = r.f(2), thv = r.f(3), hm = r.f(30);
+    const float *ub = Mb + a.p[6], *vb = Mb + a.p[7];
+    const int64_t sj = g.sJ, q = g.idx(i, j, k);
+    const float tem1 = (g.rdx * CP) * 0.5f, tem2 = (g.rdy * CP) * 0.5f;
+    auto ok = [&](int ii, int jj) { return ii >= 0 && jj >= 0 && ii <= g.nx && jj <= g.ny && ii < (int)a.n[0] && jj < (int)a.n[1] && k < (int)a.n[2]; };
+    // cell c: -dts ubar (P at c) and -dts ubar (thv + thv[-1]) tem1 uf
+    float ppdb = 0.0f, thvb = 0.0f, utb = 0.0f, vtb = 0.0f;
+    for (int d = 0; d < 2; ++d) {  // u of cells q (d = 0) and q + 1 (d = 1)
+      const int ii = i + d;
+      if (!ok(ii, j)) continue;
+      const int64_t c = q + d;
+      const float u_b = ub[c];
+      if (u_b == 0.0f) continue;
+      const float P = (tem1 * (ppd[c] - ppd[c - 1])) * hm_uf(hm, g, ii);
+      const float T = thv[c] + thv[c - 1];
+      const float gP = -dts * u_b;
+      ppdb += (d == 0 ? 1.0f : -1.0f) * ((gP * T) * tem1) * hm_uf(hm, g, ii);
+      thvb += gP * P;
+      if (d == 0) utb = dts * u_b;
+    }
+    for (int d = 0; d < 2; ++d) {  // v of cells q (d = 0) and q + sj (d = 1)
+      const int jj = j + d;
+      if (!ok(i, jj)) continue;
+      const int64_t c = q + d * sj;
+      const float v_b = vb[c];
+      if (v_b == 0.0f) continue;
+      const float P = (tem2 * (ppd[c] - ppd[c - sj])) * hm_vf(hm, g, jj);
+      const float T = thv[c] + thv[c - sj];
+      const float gP = -dts * v_b;
+      ppdb += (d == 0 ? 1.0f : -1.0f) * ((gP * T) * tem2) * hm_vf(hm, g, jj);
+      thvb += gP * P;
+      if (d == 0) vtb = dts * v_b;
+    }
+    if (ppdb != 0.0f) Mb[a.p[2] + q] += ppdb;
+    if (thvb != 0.0f) Mb[a.p[3] + q] += thvb;
+    if (utb != 0.0f) Mb[a.p[4] + q] += utb;
+    if (vtb != 0.0f) Mb[a.p[5] + q] += vtb;
+  }
+  static constexpr bool kPrep = false;
+};
+
+// acoustic_fpk.wgsl:167-207: fpk = beta A + (dts B + advPi) with A = w pk1+qk-part,
+// B = ppten - ppterm div, advPi from u, v and piadv differences.
+struct G_acoustic_fpk {
+  static TE_HD bool cell(const AdjArg& a, const ADims& g, int i, int j, int k) {
+    return i >= 0 && j >= 0 && k >= 0 && i < g.nx && j < g.ny && k < g.nz && i < (int)a.n[0] && j < (int)a.n[1] &&
+           k < (int)a.n[2];
+  }
+  static TE_HD void prep(const float* M, float* Mb, const AdjArg& a, float* scr, int i, int j, int k) {
+    Acc r(M, Mb, &a);
+    const ADims g = r.dims();
+    if (!cell(a, g, i, j, k)) return;
+    const int64_t c = g.idx(i, j, k), w = (int64_t)a.p[11] + c;
+    scr[c] = Mb[w];
+    Mb[w] = 0.0f;
+  }
+  static TE_HD void gather(const float* M, float* Mb, const AdjArg& a, const float* scr, int x, int y, int k) {
+    Acc r(M, Mb, &a);
+    const ADims g = r.dims();
+    const int i = x - 3, j = y - 3;
+    const float dts = r.F(1, 0), beta = r.F(1, 2), smeps = r.F(1, 4);
+    const P u = r.f(2), v = r.f(3), w = r.f(4), pi = r.f(5), ppterm = r.f(7), pk1 = r.f(8), pk2 = r.f(9),
+                qk = r.f(10), hm = r.f(30);
+    const int64_t sj = g.sJ, sk = g.sk(), q = g.idx(i, j, k);
+    const float temx = (dts * 0.5f) * g.rdx, temy = (dts * 0.5f) * g.rdy;
+    auto fb = [&](int ii, int jj, int kk) { return cell(a, g, ii, jj, kk) ? scr[g.idx(ii, jj, kk)] : 0.0f; };
+    // adjoint of div at a cell (0 where the forward zeroed div)
+    auto divb = [&](int ii, int jj, int kk, int64_t c) {
+      const float f = fb(ii, jj, kk);
+      if (f == 0.0f) return 0.0f;
+      const float div = fmaf_((u[c + 1] - u[c]) * g.rdx, hm_uh(hm, g, ii), ((v[c + sj] - v[c]) * g.rdy) * hm_vh(hm, g, jj));
+      if (ab(D1(div)).v < smeps) return 0.0f;
+      return -(dts * f) * ppterm[c];
+    };
+    auto aub = [&](int ii, int jj, int kk) { return -(fb(ii, jj, kk) * temx) * hm_uh(hm, g, ii); };
+    auto avb = [&](int ii, int jj, int kk) { return -(fb(ii, jj, kk) * temy) * hm_vh(hm, g, jj); };
+    float ubs = 0.0f, vbs = 0.0f, wbs = 0.0f, pib = 0.0f, pk1b = 0.0f, pk2b = 0.0f, qkb = 0.0f, pptenb = 0.0f,
+          pptermb = 0.0f;
+    // cell q: u0, v0, w0, pc, pk1, pk2, qk, ppten, ppterm
+    if (cell(a, g, i, j, k)) {
+      const float f = scr[q];
+      if (f != 0.0f) {
+        const float A = beta * f, B = dts * f;
+        const float w0 = w[q], w1 = w[q + sk];
+        wbs += A * (pk1[q] + qk[q]);
+        pk1b += A * w0;
+        pk2b += A * w1;
+        qkb += A * (w0 - w1);
+        pptenb += B;
+        const float div = fmaf_((u[q + 1] - u[q]) * g.rdx, hm_uh(hm, g, i), ((v[q + sj] - v[q]) * g.rdy) * hm_vh(hm, g, j));
+        const bool zeroed = ab(D1(div)).v < smeps;
+        if (!zeroed) pptermb += -B * div;
+        const float db = zeroed ? 0.0f : -B * ppterm[q];
+        ubs += -(db * g.rdx) * hm_uh(hm, g, i);
+        vbs += -(db * g.rdy) * hm_vh(hm, g, j);
+        const float Au = aub(i, j, k), Av = avb(i, j, k);
+        ubs += Au * (pi[q] - pi[q - 1]);
+        vbs += Av * (pi[q] - pi[q - sj]);
+        pib += Au * (u[q] - u[q + 1]) + Av * (v[q] - v[q + sj]);
+      }
+    }
+    // cell q - 1: u1 = u[q], pip = pi[q]
+    if (fb(i - 1, j, k) != 0.0f) {
+      const int64_t c = q - 1;
+      const float db = divb(i - 1, j, k, c);
+      ubs += (db * g.rdx) * hm_uh(hm, g, i - 1);
+      const float Au = aub(i - 1, j, k);
+      ubs += Au * (pi[c + 1] - pi[c]);
+      pib += Au * u[c + 1];
+    }
+    // cell q + 1: pim = pi[q]
+    if (fb(i + 1, j, k) != 0.0f) pib -= aub(i + 1, j, k) * u[q + 1];
+    // cell q - sj: v1 = v[q], pjp = pi[q]
+    if (fb(i, j - 1, k) != 0.0f) {
+      const int64_t c = q - sj;
+      const float db = divb(i, j - 1, k, c);
+      vbs += (db * g.rdy) * hm_vh(hm, g, j - 1);
+      const float Av = avb(i, j - 1, k);
+      vbs += Av * (pi[c + sj] - pi[c]);
+      pib += Av * v[c + sj];
+    }
+    // cell q + sj: pjm = pi[q]
+    if (fb(i, j + 1, k) != 0.0f) pib -= avb(i, j + 1, k) * v[q + sj];
+    // cell q - sk: w1 = w[q]
+    if (fb(i, j, k - 1) != 0.0f) {
+      const int64_t c = q - sk;
+      wbs += (beta * scr[c]) * (pk2[c] - qk[c]);
+    }
+    if (ubs != 0.0f) Mb[a.p[2] + q] += ubs;
+    if (vbs != 0.0f) Mb[a.p[3] + q] += vbs;
+    if (wbs != 0.0f) Mb[a.p[4] + q] += wbs;
+    if (pib != 0.0f) Mb[a.p[5] + q] += pib;
+    if (pptenb != 0.0f) Mb[a.p[6] + q] += pptenb;
+    if (pptermb != 0.0f) Mb[a.p[7] + q] += pptermb;
+    if (pk1b != 0.0f) Mb[a.p[8] + q] += pk1b;
+    if (pk2b != 0.0f) Mb[a.p[9] + q] += pk2b;
+    if (qkb != 0.0f) Mb[a.p[10] + q] += qkb;
+  }
+  static constexpr bool kPrep = true;
+};
 
 }  // namespace adj
 }  // namespace te
@@ -1223,3 +1445,9 @@ TE_HD void w_solve_clear(float* Mb, const AdjArg& a, const ADims& g, int i, int 
   X("_copy", B_copy, 1, {1}) \
   X("rayleigh_damping", B_rayleigh_damping, 4, {10, 11, 12, 13})
 
+// Kernels whose adjoint runs in gather form (replacing the per-point body's
+// scatter form): X(name, gather struct).
+#define TE_ADJ_GATHER_KERNELS(X) \
+  X("adv_assemble", G_adv_assemble) \
+  X("acoustic_uv", G_acoustic_uv) \
+  X("acoustic_fpk", G_acoustic_fpk)

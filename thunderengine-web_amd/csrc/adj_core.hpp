@@ -84,11 +84,23 @@ TE_HD D1 powc(D1 x, double p) {
   return D1(y, x.d * ((p * y) / x.v));
 }
 
-// ---- dispatch argument: thread grid and word offsets of parameters 0..31
-// (the reference's 64-word argument slot: A[0..2] grid, A[4 + n] parameter n)
+// ---- dispatch argument: thread grid, word offsets of parameters 0..31 (the
+// reference's 64-word argument slot: A[0..2] grid, A[4 + n] parameter n) and
+// where each parameter's values are read: the program memory, or the reverse
+// sweep's write log when a later dispatch of the step has overwritten them.
+// Adjoints always go to word p[n] + c of the adjoint memory.
 struct AdjArg {
   uint32_t n[3];
   uint32_t p[32];
+  const float* v[32];
+};
+
+// A parameter as seen by a body: its values and its adjoint offset.
+struct P {
+  const float* v;
+  uint32_t off;
+  TE_HD operator const float*() const { return v; }
+  TE_HD float operator[](int64_t i) const { return v[i]; }
 };
 
 // GridDims bytes (timestep.js:88-90): nx ny nz strideJ strideK 1/dx 1/dy 1/dz.
@@ -106,7 +118,7 @@ constexpr int MAXO = 8;  // outputs per point
 // Accessor handed to the bodies: memory, adjoint memory and the pass state.
 struct Acc {
   const float* M;  // values (word 0 of the program memory)
-  float* Mb;       // adjoints (same word indexing)
+  float* Mb;       // adjoints (word 0: word p[n] + c is the adjoint of parameter n's value c)
   const AdjArg* a;
   int mode;  // 0: discovery pass, 1: seeded pass
   int cnt, seed;
@@ -120,14 +132,14 @@ struct Acc {
       : M(M_), Mb(Mb_), a(a_), mode(0), cnt(0), seed(-1), rec(-1), acc(0.0) {
     for (int o = 0; o < MAXO; ++o) { yb[o] = 0.0; oa[o] = 0; has[o] = false; }
   }
-  // parameter n as a value pointer / its words
-  TE_HD const float* f(int n) const { return M + a->p[n]; }
+  // parameter n: its values and adjoint offset / its words
+  TE_HD P f(int n) const { return P{a->v[n], a->p[n]}; }
   TE_HD uint32_t word(int n, int k = 0) const {
-    union { float f; uint32_t u; } q{M[a->p[n] + k]};
+    union { float f; uint32_t u; } q{a->v[n][k]};
     return q.u;
   }
   TE_HD int32_t I(int n, int k = 0) const { return (int32_t)word(n, k); }
-  TE_HD float F(int n, int k = 0) const { return M[a->p[n] + k]; }
+  TE_HD float F(int n, int k = 0) const { return a->v[n][k]; }
   TE_HD ADims dims() const {
     ADims g;
     g.nx = I(0, 0); g.ny = I(0, 1); g.nz = I(0, 2); g.sJ = I(0, 3); g.sK = I(0, 4);
@@ -135,21 +147,21 @@ struct Acc {
     return g;
   }
   // active load of p[c]
-  TE_HD D1 ld(const float* p, int64_t c) {
-    D1 r((double)p[c]);
+  TE_HD D1 ld(P p, int64_t c) {
+    D1 r((double)p.v[c]);
     if (mode != 0 && cnt == seed) {
       r.d = 1.0;
-      rec = (int64_t)(p - M) + c;
+      rec = (int64_t)p.off + c;
     }
     ++cnt;
     return r;
   }
   // output y to p[c]
   template <int O>
-  TE_HD void out(const float* p, int64_t c, D1 y) {
+  TE_HD void out(P p, int64_t c, D1 y) {
     static_assert(O < MAXO, "too many outputs");
     if (mode == 0) {
-      const int64_t w = (int64_t)(p - M) + c;
+      const int64_t w = (int64_t)p.off + c;
       oa[O] = w;
       yb[O] = (double)Mb[w];
       has[O] = true;

@@ -36,12 +36,14 @@ __global__ __launch_bounds__(256) void k_adj_w_coef(const float* M, AdjArg a, fl
   const uint64_t z = t / n01, rem = t - z * n01, y = rem / n0, x = rem - y * n0;
   w_solve_coef(M, a, scr, fw, (int)x, (int)y, (int)z);
 }
-__global__ __launch_bounds__(64) void k_adj_w_sweep(const float* M, float* Mb, AdjArg a, const float* scr, uint64_t fw) {
+__global__ __launch_bounds__(64) void k_adj_w_sweep(const float* M, float* Mb, AdjArg a, float* scr, uint64_t fw,
+                                                     uint32_t nz1) {
+  extern __shared__ float te_col[];  // cA and cB / z: 2 (nz + 1) floats per lane, lane-interleaved
   const uint64_t t = (uint64_t)blockIdx.x * 64u + threadIdx.x;
   const uint64_t n0 = a.n[0];
   if (t >= n0 * a.n[1]) return;
   const uint64_t y = t / n0, x = t - y * n0;
-  w_solve_sweep(M, Mb, a, scr, fw, (int)x, (int)y);
+  w_solve_sweep(M, Mb, a, scr, fw, te_col + threadIdx.x, te_col + 64 * nz1 + threadIdx.x, 64, (int)x, (int)y);
 }
 __global__ __launch_bounds__(256) void k_adj_w_level(const float* M, float* Mb, AdjArg a, uint32_t nz) {
   const uint64_t t = (uint64_t)blockIdx.x * 256u + threadIdx.x;
@@ -58,6 +60,36 @@ __global__ __launch_bounds__(256) void k_adj_w_clear(const float* M, float* Mb, 
   Acc r(M, Mb, &a);
   w_solve_clear(Mb, a, r.dims(), (int)x, (int)y);
 }
+
+// gather-form adjoints: prep over the forward grid, gather over every word of
+// the field grid
+template <class G>
+__global__ __launch_bounds__(256) void k_adj_prep(const float* M, float* Mb, AdjArg a, float* scr) {
+  const uint64_t t = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+  const uint64_t n0 = a.n[0], n01 = n0 * a.n[1];
+  if (t >= n01 * a.n[2]) return;
+  const uint64_t z = t / n01, rem = t - z * n01, y = rem / n0, x = rem - y * n0;
+  G::prep(M, Mb, a, scr, (int)x, (int)y, (int)z);
+}
+template <class G>
+__global__ __launch_bounds__(256) void k_adj_gather(const float* M, float* Mb, AdjArg a, const float* scr, uint32_t sJ,
+                                                    uint32_t sK, uint32_t nz1) {
+  const uint64_t t = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+  const uint64_t n01 = (uint64_t)sJ * sK;
+  if (t >= n01 * nz1) return;
+  const uint64_t z = t / n01, rem = t - z * n01, y = rem / sJ, x = rem - y * sJ;
+  G::gather(M, Mb, a, scr, (int)x, (int)y, (int)z);
+}
+
+struct GatherEntry {
+  const char* name;
+  const void* prep;
+  const void* gather;
+};
+#define GK(n, G) {n, G::kPrep ? reinterpret_cast<const void*>(&k_adj_prep<G>) : nullptr, \
+                  reinterpret_cast<const void*>(&k_adj_gather<G>)},
+static const GatherEntry kGather[] = {TE_ADJ_GATHER_KERNELS(GK)};
+#undef GK
 
 struct Entry {
   const char* name;
@@ -115,13 +147,33 @@ int writes(int id, int out[8]) {
   return kAdj[id].nw;
 }
 
-hipError_t launch(int id, const uint32_t* A, const float* M, float* Mb, int nz, float* scr, uint64_t fw,
-                  hipStream_t stream) {
+hipError_t launch(int id, const uint32_t* A, const float* const* vals, const float* M, float* Mb, int nz, int sJ, int sK,
+                  float* scr, uint64_t fw, hipStream_t stream) {
   const Entry& e = kAdj[id];
   AdjArg a;
   a.n[0] = A[0]; a.n[1] = A[1]; a.n[2] = A[2];
-  for (int q = 0; q < 32; ++q) a.p[q] = A[4 + q];
+  for (int q = 0; q < 32; ++q) {
+    a.p[q] = A[4 + q];
+    a.v[q] = vals ? vals[q] : M + a.p[q];
+  }
   void* args[] = {(void*)&M, (void*)&Mb, (void*)&a};
+  for (const GatherEntry& ge : kGather) {
+    if (std::strcmp(ge.name, e.name) != 0) continue;
+    if (!scr || fw < (uint64_t)sJ * sK * (nz + 1)) return hipErrorInvalidValue;
+    if (ge.prep) {
+      const uint64_t n = (uint64_t)a.n[0] * a.n[1] * a.n[2];
+      void* pargs[] = {(void*)&M, (void*)&Mb, (void*)&a, (void*)&scr};
+      if (n > 0) {
+        hipError_t r = hipLaunchKernel(ge.prep, dim3((unsigned)((n + 255) / 256)), dim3(256), pargs, 0, stream);
+        if (r != hipSuccess) return r;
+      }
+    }
+    const float* cscr = scr;
+    uint32_t uj = (uint32_t)sJ, uk = (uint32_t)sK, un = (uint32_t)(nz + 1);
+    void* gargs[] = {(void*)&M, (void*)&Mb, (void*)&a, (void*)&cscr, (void*)&uj, (void*)&uk, (void*)&un};
+    const uint64_t words = (uint64_t)uj * uk * un;
+    return hipLaunchKernel(ge.gather, dim3((unsigned)((words + 255) / 256)), dim3(256), gargs, 0, stream);
+  }
   if (e.column) {  // acoustic_w_solve: coefficients, sweeps, levels, clear
     const uint64_t cols = (uint64_t)a.n[0] * a.n[1];
     if (cols == 0) return hipSuccess;
@@ -132,9 +184,11 @@ hipError_t launch(int id, const uint32_t* A, const float* M, float* Mb, int nz, 
     hipError_t r = hipLaunchKernel(reinterpret_cast<const void*>(&k_adj_w_coef), dim3((unsigned)((pts + 255) / 256)),
                                    dim3(256), cargs, 0, stream);
     if (r != hipSuccess) return r;
-    const float* cscr = scr;
-    void* sargs[] = {(void*)&M, (void*)&Mb, (void*)&a, (void*)&cscr, (void*)&fw};
-    r = hipLaunchKernel(e.fn, dim3((unsigned)((cols + 63) / 64)), dim3(64), sargs, 0, stream);
+    uint32_t nz1 = nzu + 1;
+    const size_t lds = (size_t)2 * 64 * nz1 * sizeof(float);
+    if (lds > 65536) return hipErrorInvalidConfiguration;  // nz <= 127
+    void* sargs[] = {(void*)&M, (void*)&Mb, (void*)&a, (void*)&scr, (void*)&fw, (void*)&nz1};
+    r = hipLaunchKernel(e.fn, dim3((unsigned)((cols + 63) / 64)), dim3(64), sargs, lds, stream);
     if (r != hipSuccess) return r;
     void* largs[] = {(void*)&M, (void*)&Mb, (void*)&a, (void*)&nzu};
     r = hipLaunchKernel(reinterpret_cast<const void*>(&k_adj_w_level), dim3((unsigned)((pts + 255) / 256)), dim3(256),

@@ -8,14 +8,15 @@
 //     else for one step, with the segment's step-start states recomputed
 //     first;
 //   * the adjoint memory: one float per word of M, with M's guards.
-// The reverse of one step restores, dispatch by dispatch from the last, the
-// regions that dispatch wrote, so the memory holds exactly the dispatch's
-// inputs when its adjoint runs.
+// The reverse of a step runs the dispatches' adjoints from the last, each
+// reading its inputs where they are at that point: in the write log slot of
+// the next dispatch that overwrote them, or in M.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -89,11 +90,25 @@ static int reverse_steps(te_adjoint* a, int steps) {
       int rc = run_rec_ops(a, i, i);
       if (rc != TE_OK) return rc;
     }
+  // reverse: a dispatch's inputs are the values before it.  Walking back,
+  // each field's value before the dispatch is in the log slot of the first
+  // write at or after it, or still in M when the segment never wrote it after
+  // the dispatch; the adjoint reads its parameters there (no restores).
+  std::map<uint64_t, const float*> cur;  // field offset -> its values before the dispatch being reversed
+  const float* M = reinterpret_cast<const float*>(r->dM);
+  const float* vals[32];
   for (int s = steps - 1; s >= 0; --s)
     for (int i = n - 1; i >= 0; --i) {
-      TA_HIP(copy(i, at[(size_t)s * n + i], false), "restore");
-      TA_HIP(te::adj::launch(a->adj_id[i], r->ops[i].A, reinterpret_cast<const float*>(r->dM), a->dMb, r->layout.nz,
-                             a->dScr, r->layout.field_words, r->stream),
+      const uint64_t p = at[(size_t)s * n + i];
+      for (size_t q = 0; q < a->wreg[i].size(); ++q)
+        cur[a->wreg[i][q].first] = reinterpret_cast<const float*>(a->dLog + p + q * a->stride[i]);
+      const uint32_t* A = r->ops[i].A;
+      for (int q = 0; q < 32; ++q) {
+        auto it = cur.find(A[4 + q]);
+        vals[q] = it != cur.end() ? it->second : M + A[4 + q];
+      }
+      TA_HIP(te::adj::launch(a->adj_id[i], A, vals, M, a->dMb, r->layout.nz, r->layout.sJ, r->layout.sK, a->dScr,
+                             r->layout.field_words, r->stream),
              std::string("adjoint of ") + r->op_names[i]);
     }
   return TE_OK;
@@ -248,7 +263,8 @@ int te_adj_ops(te_adjoint* a, int32_t first, int32_t last) {
   const int n = (int)r->ops.size();
   if (first < 0 || last >= n || first > last) return afail(a, TE_ERR_ARG, "bad dispatch range");
   for (int i = last; i >= first; --i)
-    TA_HIP(te::adj::launch(a->adj_id[i], r->ops[i].A, reinterpret_cast<const float*>(r->dM), a->dMb, r->layout.nz,
+    TA_HIP(te::adj::launch(a->adj_id[i], r->ops[i].A, nullptr, reinterpret_cast<const float*>(r->dM), a->dMb,
+                           r->layout.nz, r->layout.sJ, r->layout.sK,
                            a->dScr, r->layout.field_words, r->stream),
            std::string("adjoint of ") + r->op_names[i]);
   TA_HIP(hipStreamSynchronize(r->stream), "sync");

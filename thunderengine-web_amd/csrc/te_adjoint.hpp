@@ -12,10 +12,12 @@ int find(const char* name);
 int writes(int id, int out[8]);
 // Adjoint launch of one dispatch (64-word argument slot A) on value memory M
 // and adjoint memory Mb (both word 0 of the program memory); nz is the
-// grid's level count (the dims block's third word); scr is 4 * fw floats of
-// scratch (fw >= one field's words) for the column solve's coefficients.
-hipError_t launch(int id, const uint32_t* A, const float* M, float* Mb, int nz, float* scr, uint64_t fw,
-                  hipStream_t stream);
+// grid's level count and sJ, sK its strides (the dims block's words 2..4);
+// scr is 4 * fw floats of scratch (fw >= one field's words) for the column
+// solve's coefficients and the gather-form adjoints' output copies.
+// vals: where each of the 32 parameters' values are read (null: M + offset).
+hipError_t launch(int id, const uint32_t* A, const float* const* vals, const float* M, float* Mb, int nz, int sJ, int sK,
+                  float* scr, uint64_t fw, hipStream_t stream);
 // Copies of up to 8 regions of n words each, in one launch.
 struct CopyArg {
   const uint32_t* src[8];
