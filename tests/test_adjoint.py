@@ -24,6 +24,7 @@ CONFIGS = {
     "dry_weno_momentum": dict(SMALL, moist=0, weno_momentum=1),
 }
 DEAD = ("cA", "cB")  # elimination scratch of the column solve: never read after it
+GATHERED = ("adv_assemble", "acoustic_uv", "acoustic_fpk", "acoustic_pp")
 
 
 def _arena(e):
@@ -78,8 +79,8 @@ def test_dispatch_adjoints_transpose_the_tangent_linear_build(name):
 
 @pytest.mark.parametrize("name", sorted(CONFIGS))
 def test_gather_adjoints_equal_the_scatter_form(name):
-    """adv_assemble, acoustic_uv and acoustic_fpk run gather-form adjoints in
-    the library; word for word they equal the per-point bodies' scatter form
+    """adv_assemble, acoustic_uv, acoustic_fpk and acoustic_pp run
+    gather-form adjoints in the library; word for word they equal the per-point bodies' scatter form
     (up to binary32 against binary64 rounding of the Jacobian entries)."""
     R = runs.OracleRun(te_host.config(fuse=0, **CONFIGS[name]))
     e = R.enc
@@ -87,7 +88,7 @@ def test_gather_adjoints_equal_the_scatter_form(name):
     rng = np.random.default_rng(13)
     seen = set()
     for i, nm in enumerate(R.names):
-        if nm in ("adv_assemble", "acoustic_uv", "acoustic_fpk"):
+        if nm in GATHERED:
             Mw = R._buf.copy()[runs.FRONT:]
             ybar = np.zeros(aend, np.float32)
             ybar[a0:] = rng.standard_normal(aend - a0)
@@ -101,7 +102,7 @@ def test_gather_adjoints_equal_the_scatter_form(name):
             assert err < 1e-5, (nm, i, err)
             seen.add(nm)
         R.prog.run(R.M, i, i)
-    assert seen == {"adv_assemble", "acoustic_uv", "acoustic_fpk"}
+    assert seen == set(GATHERED)
 
 
 def test_reverse_sweep_transposes_two_steps():

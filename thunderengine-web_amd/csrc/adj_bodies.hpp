@@ -1130,33 +1130,74 @@ TE_HD void w_solve_sweep(const float* M, float* Mb, const AdjArg& a, float* scr,
   for (int k = 0; k <= nz; ++k) wb[k * sk] = 0.0f;
 }
 
-// Level part, one (i, j, k), k = 1..nz-1, after every column's sweep:
-// abar, bbar, cbar, dbar of row k pushed back to its loads at levels k and
-// k-1 (rows k and k+1 both touch level k: adj_add).
+// Level part, one word (i, j, k), k = 0..nz-1, after every column's sweep:
+// the adjoints of row k's and row k+1's coefficients pushed back to the words
+// of level k, in closed form (lam_1, acoustic_w_solve.wgsl:124-134, with
+// m = mm[k], cm the level below a row):
+//   a = -asq m (pk1[cm] + qk[cm]),  c = -asq (-m) (pk2 - qk),
+//   b = 1 - asq m (-(pk1 + qk) + (pk2[cm] - qk[cm])),
+//   d = w + dts wten - beta m (pp - pp[cm]) - alph m ((pp + fpk) - (pp[cm] + fpk[cm])),
+// abar = -dbar w[k-1], bbar = -dbar w[k], cbar = -dbar w[k+1] (the new w).
 TE_HD void w_solve_level(const float* M, float* Mb, const AdjArg& a, int i, int j, int k) {
   Acc r(M, Mb, &a);
   const ADims g = r.dims();
   const int nz = g.nz;
-  if (i >= g.nx || j >= g.ny || k < 1 || k > nz - 1) return;
+  if (i >= g.nx || j >= g.ny || k < 0 || k > nz - 1) return;
   const float dts = r.F(1, 0), alph = r.F(1, 1), beta = r.F(1, 2);
-  const float asq = alph * alph;
-  const int64_t sk = g.sk(), c0 = g.idx(i, j, 0);
-  const float* S1 = Mb + a.p[10] + c0;
-  const float* S2 = Mb + a.p[11] + c0;
-  const double y = S1[k * sk];
-  if (y == 0.0) return;
-  const double wm = k == 1 ? 0.0 : S2[(k - 1) * sk], w0 = S2[k * sk], wp = k == nz - 1 ? 0.0 : S2[(k + 1) * sk];
-  const double ab_ = k == 1 ? 0.0 : -y * wm, bb_ = -y * w0, cb_ = k == nz - 1 ? 0.0 : -y * wp;
-  r.mode = 1;
-  for (int s = 0; s < 32; ++s) {  // 17 active loads
-    r.cnt = 0;
-    r.seed = s;
-    r.rec = -1;
-    const WCoef q = w_coef(r, c0 + k * sk, sk, asq, dts, alph, beta);
-    if (s >= r.cnt) break;
-    const double acc = fmad_(ab_, q.aa.d, fmad_(bb_, q.bb.d, fmad_(cb_, q.cc.d, y * q.dd.d)));
-    if (acc != 0.0) adj_add(Mb + r.rec, (float)acc);
+  const float asq = alph * alph, ab_ = alph + beta;
+  const P pk1 = r.f(2), pk2 = r.f(3), qk = r.f(4), mm = r.f(5), fpk = r.f(6), pp = r.f(8);
+  const int64_t sk = g.sk(), c0 = g.idx(i, j, 0), c = c0 + k * sk;
+  const float* S1 = Mb + a.p[10] + c0;  // dbar
+  const float* S2 = Mb + a.p[11] + c0;  // new w
+  // coefficient adjoints of row kk (0 outside rows 1..nz-1)
+  auto row = [&](int kk, float& ab, float& bb, float& cb, float& db) {
+    ab = bb = cb = db = 0.0f;
+    if (kk < 1 || kk > nz - 1) return;
+    const float y = S1[kk * sk];
+    db = y;
+    bb = -y * S2[kk * sk];
+    if (kk > 1) ab = -y * S2[(kk - 1) * sk];
+    if (kk < nz - 1) cb = -y * S2[(kk + 1) * sk];
+  };
+  float a0, b0, c0b, d0, a1, b1, c1b, d1;
+  row(k, a0, b0, c0b, d0);
+  row(k + 1, a1, b1, c1b, d1);
+  if (a0 == 0.0f && b0 == 0.0f && c0b == 0.0f && d0 == 0.0f && a1 == 0.0f && b1 == 0.0f && c1b == 0.0f && d1 == 0.0f)
+    return;
+  float pk1b = 0.0f, pk2b = 0.0f, qkb = 0.0f, ppb = 0.0f, fpkb = 0.0f, mmb = 0.0f, wtb = 0.0f, wb = 0.0f;
+  if (k >= 1) {  // row k: this level is c
+    const int64_t cm = c - sk;
+    const float m = mm[c];
+    const float am = asq * m;
+    pk1b += b0 * am;
+    qkb += am * (b0 - c0b);
+    pk2b += c0b * am;
+    wtb += d0 * dts;
+    ppb += -d0 * m * ab_;
+    fpkb += -d0 * alph * m;
+    wb += d0;
+    const float dpf = (pp[c] + fpk[c]) - (pp[cm] + fpk[cm]);
+    mmb += a0 * (-asq) * (pk1[cm] + qk[cm]) + c0b * asq * (pk2[c] - qk[c]) +
+           b0 * (-asq) * ((-(pk1[c] + qk[c])) + (pk2[cm] - qk[cm])) + d0 * (-alph * dpf - beta * (pp[c] - pp[cm]));
   }
+  if (k + 1 <= nz - 1) {  // row k+1: this level is its cm
+    const float m = mm[c + sk];
+    const float am = asq * m;
+    pk1b += -a1 * am;
+    qkb += -am * (a1 - b1);
+    pk2b += -b1 * am;
+    ppb += d1 * m * ab_;
+    fpkb += d1 * alph * m;
+  }
+  float* B = Mb;
+  if (pk1b != 0.0f) B[a.p[2] + c] += pk1b;
+  if (pk2b != 0.0f) B[a.p[3] + c] += pk2b;
+  if (qkb != 0.0f) B[a.p[4] + c] += qkb;
+  if (mmb != 0.0f) B[a.p[5] + c] += mmb;
+  if (fpkb != 0.0f) B[a.p[6] + c] += fpkb;
+  if (wtb != 0.0f) B[a.p[7] + c] += wtb;
+  if (ppb != 0.0f) B[a.p[8] + c] += ppb;
+  if (wb != 0.0f) B[a.p[9] + c] += wb;
 }
 
 // Clears the adjoint words of cA and cB (the forward's dead scratch).
@@ -1391,6 +1432,53 @@ struct G_acoustic_fpk {
   static constexpr bool kPrep = true;
 };
 
+// acoustic_pp.wgsl:131-167: p = alph (w pk1+qk-part) + (prev + fpk), zeroed
+// below smeps; pp3d = p (in place), ppd = p + kdiv (p - prev), ppx = kdiv (p - prev).
+// prep: everything local to the cell, and alph pbar into the scratch plane;
+// gather: w[k] as the w[k+1] of the cell below.
+struct G_acoustic_pp {
+  static TE_HD void prep(const float* M, float* Mb, const AdjArg& a, float* scr, int i, int j, int k) {
+    Acc r(M, Mb, &a);
+    const ADims g = r.dims();
+    if (i >= g.nx || j >= g.ny || k >= g.nz) return;
+    const float alph = r.F(1, 1), kdiv = r.F(1, 3), smeps = r.F(1, 4);
+    const P w = r.f(2), pk1 = r.f(3), pk2 = r.f(4), qk = r.f(5), fpk = r.f(6), pp = r.f(7);
+    const int saveX = r.I(10);
+    const int64_t sk = g.sk(), c = g.idx(i, j, k);
+    float* ppb = Mb + a.p[7] + c;
+    float* ppdb = Mb + a.p[8] + c;
+    const float xb = saveX != 0 ? Mb[a.p[9] + c] : 0.0f;
+    const float pbar_out = *ppb + (1.0f + kdiv) * *ppdb + kdiv * xb;
+    const float w1 = w[c], w2 = w[c + sk], q = qk[c];
+    const float prev = pp[c];
+    const float pval = fmaf_(alph, fmaf_(w1, pk1[c] + q, w2 * (pk2[c] - q)), prev + fpk[c]);
+    const bool zeroed = ab(D1(pval)).v < smeps;
+    const float pb = zeroed ? 0.0f : pbar_out;
+    *ppb = pb - kdiv * (*ppdb + xb);  // adjoint of the old pp3d
+    *ppdb = 0.0f;
+    if (saveX != 0) Mb[a.p[9] + c] = 0.0f;
+    const float A = alph * pb;
+    scr[c] = A;
+    if (pb == 0.0f) return;
+    Mb[a.p[6] + c] += pb;
+    Mb[a.p[3] + c] += A * w1;
+    Mb[a.p[4] + c] += A * w2;
+    Mb[a.p[5] + c] += A * (w1 - w2);
+    Mb[a.p[2] + c] += A * (pk1[c] + q);
+  }
+  static TE_HD void gather(const float* M, float* Mb, const AdjArg& a, const float* scr, int x, int y, int k) {
+    Acc r(M, Mb, &a);
+    const ADims g = r.dims();
+    const int i = x - 3, j = y - 3;
+    if (i < 0 || j < 0 || i >= g.nx || j >= g.ny || k < 1 || k > g.nz) return;
+    const P pk2 = r.f(4), qk = r.f(5);
+    const int64_t cb = g.idx(i, j, k - 1);  // the cell below
+    const float A = scr[cb];
+    if (A != 0.0f) Mb[a.p[2] + cb + g.sk()] += A * (pk2[cb] - qk[cb]);
+  }
+  static constexpr bool kPrep = true;
+};
+
 }  // namespace adj
 }  // namespace te
 
@@ -1450,4 +1538,5 @@ struct G_acoustic_fpk {
 #define TE_ADJ_GATHER_KERNELS(X) \
   X("adv_assemble", G_adv_assemble) \
   X("acoustic_uv", G_acoustic_uv) \
-  X("acoustic_fpk", G_acoustic_fpk)
+  X("acoustic_fpk", G_acoustic_fpk) \
+  X("acoustic_pp", G_acoustic_pp)

@@ -41,6 +41,7 @@ struct te_adjoint {
   std::vector<int> adj_id;                                        // per dispatch
   std::vector<std::vector<std::pair<uint64_t, uint64_t>>> wreg;  // (offset, words) per dispatch
   std::vector<uint64_t> stride;  // log words per region of each dispatch (its length rounded up to 64)
+  std::vector<char> halo_pair;    // dispatch i is bc_periodic_y right after bc_periodic_x of the same field
   hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr, e3 = nullptr;
   float fwd_ms = 0.0f, bwd_ms = 0.0f;
   std::string err;
@@ -103,6 +104,13 @@ static int reverse_steps(te_adjoint* a, int steps) {
       for (size_t q = 0; q < a->wreg[i].size(); ++q)
         cur[a->wreg[i][q].first] = reinterpret_cast<const float*>(a->dLog + p + q * a->stride[i]);
       const uint32_t* A = r->ops[i].A;
+      if (a->halo_pair[i]) {  // with the bc_periodic_x before it, as one map
+        TA_HIP(te::adj::halo_xy(a->dMb, A[4 + 1], r->layout.nx, r->layout.ny, r->layout.nz, r->layout.sJ,
+                                r->layout.sK, r->stream),
+               "adjoint of the periodic halos");
+        continue;
+      }
+      if (i + 1 < n && a->halo_pair[i + 1]) continue;
       for (int q = 0; q < 32; ++q) {
         auto it = cur.find(A[4 + q]);
         vals[q] = it != cur.end() ? it->second : M + A[4 + q];
@@ -163,6 +171,11 @@ int te_adjoint_create(te_ctx* ctx, int32_t every, te_adjoint** out) {
       if (!dup) a->wreg[i].push_back({off, L.field_words});
     }
   }
+  a->halo_pair.assign(n, 0);
+  for (int i = 1; i < n; ++i)
+    if (r->op_names[i] == "bc_periodic_y" && r->op_names[i - 1] == "bc_periodic_x" &&
+        r->ops[i].A[4 + 1] == r->ops[i - 1].A[4 + 1] && r->ops[i].A[4] == r->ops[i - 1].A[4])
+      a->halo_pair[i] = 1;
   a->stride.resize(n);
   for (int i = 0; i < n; ++i) {
     a->stride[i] = a->wreg[i].empty() ? 0 : (a->wreg[i][0].second + 63) / 64 * 64;
