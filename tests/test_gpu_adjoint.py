@@ -37,7 +37,7 @@ def _zero_fields(e, vec, names):
 
 @pytest.mark.parametrize("kw", [dict(GRID), dict(GRID, moist=0, weno_momentum=1)], ids=["moist", "dry_weno_uv"])
 def test_gpu_dispatch_adjoints_transpose_the_tangent_linear_build(gpu, kw):
-    cfg = te_host.config(**kw)
+    cfg = te_host.config(fuse=0, **kw)  # the adjoint's program is then the reference's dispatch list
     say("dispatch adjoints", kw)
     R = runs.OracleRun(te_host.config(fuse=0, **kw))
     e = R.enc
@@ -75,15 +75,17 @@ def test_gpu_dispatch_adjoints_transpose_the_tangent_linear_build(gpu, kw):
     assert not bad, bad
 
 
-@pytest.mark.parametrize("every,segment_log", [(1, True), (2, True), (2, False)])
-def test_gpu_reverse_sweep_matches_the_host_sweep(gpu, every, segment_log, monkeypatch):
-    """segment_log False: a zero log budget forces one step's log at a time
-    with the segment's step-start states recomputed first."""
+@pytest.mark.parametrize("every,segment_log,fuse", [(1, True, 1), (2, True, 1), (2, False, 1), (2, True, 0)])
+def test_gpu_reverse_sweep_matches_the_host_sweep(gpu, every, segment_log, fuse, monkeypatch):
+    """fuse 1: the adjoint recomputes the fused program and runs each fused
+    launch's adjoint as the reference dispatches it stands for; fuse 0: the
+    reference's list.  segment_log False: a zero log budget forces one step's
+    log at a time with the segment's step-start states recomputed first."""
     steps = 3
-    say("reverse sweep, steps", steps, "checkpoint every", every, "segment log", segment_log)
+    say("reverse sweep, steps", steps, "checkpoint every", every, "segment log", segment_log, "fuse", fuse)
     if not segment_log:
         monkeypatch.setenv("TE_ADJOINT_LOG_GIB", "0")
-    run = te_host.Run(te_host.config(**GRID))
+    run = te_host.Run(te_host.config(fuse=fuse, **GRID))
     run.init_state()
     adj = te_host.Adjoint(run, every=every)
     R = runs.OracleRun(te_host.config(fuse=0, **GRID))
@@ -95,7 +97,9 @@ def test_gpu_reverse_sweep_matches_the_host_sweep(gpu, every, segment_log, monke
     rng = np.random.default_rng(23)
     ybar = np.zeros(aend, np.float32)
     ybar[a0:] = rng.standard_normal(aend - a0)
-    _zero_fields(e, ybar, DEAD)
+    # the scratch fields' final values are not part of the fused step's
+    # results (its launches keep fluxes and coefficients in registers)
+    _zero_fields(e, ybar, runs.FUSED_SCRATCH if fuse else DEAD)
     adj.upload(0, ybar)
     adj.backward()
     dev = adj.read(0, aend).astype(np.float64)

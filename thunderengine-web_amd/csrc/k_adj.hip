@@ -84,29 +84,37 @@ __global__ __launch_bounds__(256) void k_adj_gather(const float* M, float* Mb, A
 // bc_periodic_x then bc_periodic_y on one field leave every halo word
 // (i in -3..nx+3, j in -3..ny+3, levels 0..nz) equal to its interior image
 // (i mod nx, j mod ny): the pair's adjoint moves each halo word's adjoint onto
-// its image.  One thread per interior word gathers (and clears) its images.
+// its image.  One thread per halo word (7 halo rows of strideJ words and
+// 7 halo columns of ny words per level); several halo words share an image,
+// so the adds are atomic.
 __global__ __launch_bounds__(256) void k_adj_halo_xy(float* Mb, uint64_t off, int nx, int ny, int nz1, int sJ, int sK) {
   const uint64_t t = (uint64_t)blockIdx.x * 256u + threadIdx.x;
-  const uint64_t nxy = (uint64_t)nx * ny;
-  if (t >= nxy * nz1) return;
-  const int k = (int)(t / nxy), rem = (int)(t - (uint64_t)k * nxy), j = rem / nx, i = rem - j * nx;
+  const uint64_t ring = (uint64_t)7 * sJ + (uint64_t)7 * ny;
+  if (t >= ring * nz1) return;
+  const int k = (int)(t / ring);
+  const int u = (int)(t - (uint64_t)k * ring);
+  int i, j;
+  if (u < 7 * sJ) {
+    const int rr = u / sJ;
+    j = rr < 3 ? rr - 3 : ny + (rr - 3);
+    i = u - rr * sJ - 3;
+  } else {
+    const int v = u - 7 * sJ, rr = v / ny;
+    i = rr < 3 ? rr - 3 : nx + (rr - 3);
+    j = v - rr * ny;
+  }
   float* f = Mb + off + (uint64_t)sJ * sK * k;
-  float acc = 0.0f;
-  for (int dj = -1; dj <= 1; ++dj)
-    for (int di = -1; di <= 1; ++di) {
-      if (di == 0 && dj == 0) continue;
-      const int ii = i + di * nx, jj = j + dj * ny;
-      if (ii < -3 || ii > nx + 3 || jj < -3 || jj > ny + 3) continue;
-      float* h = f + (ii + 3) + (int64_t)sJ * (jj + 3);
-      acc += *h;
-      *h = 0.0f;
-    }
-  if (acc != 0.0f) f[(i + 3) + (int64_t)sJ * (j + 3)] += acc;
+  float* h = f + (i + 3) + (int64_t)sJ * (j + 3);
+  const float hb = *h;
+  if (hb == 0.0f) return;
+  *h = 0.0f;
+  const int ii = ((i % nx) + nx) % nx, jj = ((j % ny) + ny) % ny;
+  adj_add(f + (ii + 3) + (int64_t)sJ * (jj + 3), hb);
 }
 
 hipError_t halo_xy(float* Mb, uint64_t off, int nx, int ny, int nz, int sJ, int sK, hipStream_t stream) {
   int nz1 = nz + 1;
-  const uint64_t n = (uint64_t)nx * ny * nz1;
+  const uint64_t n = ((uint64_t)7 * sJ + (uint64_t)7 * ny) * nz1;
   void* args[] = {(void*)&Mb, (void*)&off, (void*)&nx, (void*)&ny, (void*)&nz1, (void*)&sJ, (void*)&sK};
   return hipLaunchKernel(reinterpret_cast<const void*>(&k_adj_halo_xy), dim3((unsigned)((n + 255) / 256)), dim3(256),
                          args, 0, stream);

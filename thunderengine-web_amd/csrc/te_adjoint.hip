@@ -24,9 +24,19 @@
 #include "te_adjoint.hpp"
 #include "te_internal.hpp"
 
+// One reference dispatch of a program's dispatch: the adjoint of a fused
+// launch is the adjoints of the dispatches it stands for, in reverse order,
+// all reading the values from before the launch (none of them reads a value
+// another of them writes, other than linearly).
+struct Unit {
+  int adj = -1;         // adjoint registry index, or -1: periodic halos of `field` (x then y)
+  uint32_t A[64] = {};  // the reference dispatch's argument slot
+  uint64_t field = 0;
+};
+
 struct te_adjoint {
   te_ctx* fwd = nullptr;  // the caller's context
-  te_ctx* rec = nullptr;  // unfused dispatch list on memory of its own (owned)
+  te_ctx* rec = nullptr;  // the same configuration's program on memory of its own (owned)
   int every = 1;
   uint64_t state_words = 0;  // [0, arena end)
   float* dMbBase = nullptr;
@@ -38,7 +48,7 @@ struct te_adjoint {
   uint32_t* dLog = nullptr;
   uint64_t log_words = 0;     // one step's write log
   bool segment_log = false;   // the log holds a whole checkpoint segment
-  std::vector<int> adj_id;                                        // per dispatch
+  std::vector<std::vector<Unit>> units;                          // per dispatch, in forward order
   std::vector<std::vector<std::pair<uint64_t, uint64_t>>> wreg;  // (offset, words) per dispatch
   std::vector<uint64_t> stride;  // log words per region of each dispatch (its length rounded up to 64)
   std::vector<char> halo_pair;    // dispatch i is bc_periodic_y right after bc_periodic_x of the same field
@@ -58,7 +68,7 @@ static int afail(te_adjoint* a, int code, const std::string& msg) {
   } while (0)
 
 static int run_rec_ops(te_adjoint* a, int first, int last) {
-  int rc = te_run(a->rec, 1, first, last);
+  int rc = te::run_ops_plain(a->rec, first, last);
   if (rc != TE_OK) return afail(a, rc, std::string("forward dispatches: ") + te_last_error(a->rec));
   return TE_OK;
 }
@@ -111,14 +121,198 @@ static int reverse_steps(te_adjoint* a, int steps) {
         continue;
       }
       if (i + 1 < n && a->halo_pair[i + 1]) continue;
-      for (int q = 0; q < 32; ++q) {
-        auto it = cur.find(A[4 + q]);
-        vals[q] = it != cur.end() ? it->second : M + A[4 + q];
+      const std::vector<Unit>& us = a->units[i];
+      for (int u = (int)us.size() - 1; u >= 0; --u) {
+        if (us[u].adj < 0) {
+          TA_HIP(te::adj::halo_xy(a->dMb, us[u].field, r->layout.nx, r->layout.ny, r->layout.nz, r->layout.sJ,
+                                  r->layout.sK, r->stream),
+                 "adjoint of the periodic halos");
+          continue;
+        }
+        for (int q = 0; q < 32; ++q) {
+          auto it = cur.find(us[u].A[4 + q]);
+          vals[q] = it != cur.end() ? it->second : M + us[u].A[4 + q];
+        }
+        TA_HIP(te::adj::launch(us[u].adj, us[u].A, vals, M, a->dMb, r->layout.nz, r->layout.sJ, r->layout.sK, a->dScr,
+                               r->layout.field_words, r->stream),
+               std::string("adjoint of ") + r->op_names[i]);
       }
-      TA_HIP(te::adj::launch(a->adj_id[i], A, vals, M, a->dMb, r->layout.nz, r->layout.sJ, r->layout.sK, a->dScr,
-                             r->layout.field_words, r->stream),
-             std::string("adjoint of ") + r->op_names[i]);
     }
+  return TE_OK;
+}
+
+// The reference dispatches a program dispatch stands for, with the
+// parameters it writes.  Fused launches follow their definitions in
+// k_fused.hip / k_adv.hip (the oracle restates the same expansion).
+static int decompose(te_adjoint* a, int i, std::vector<Unit>& units, std::vector<uint64_t>& writes) {
+  te_ctx* r = a->rec;
+  const std::string& nm = r->op_names[i];
+  const uint32_t* A = r->ops[i].A;
+  const std::vector<uint32_t>& W = r->image;
+  auto unit = [&](const char* kname, const uint32_t* slot) -> int {
+    Unit u;
+    u.adj = te::adj::find(kname);
+    if (u.adj < 0) return afail(a, TE_ERR_UNSUPPORTED, std::string("no adjoint for kernel '") + kname + "'");
+    std::memcpy(u.A, slot, sizeof(u.A));
+    units.push_back(u);
+    return TE_OK;
+  };
+  auto slot = [&](uint32_t g0, uint32_t g1, uint32_t g2, std::initializer_list<std::pair<int, uint32_t>> ps,
+                  uint32_t* X) {
+    std::memset(X, 0, 64 * 4);
+    X[0] = g0; X[1] = g1; X[2] = g2;
+    X[4] = A[4];     // dims
+    X[34] = A[34];   // metric
+    for (auto& pr : ps) X[4 + pr.first] = pr.second;
+  };
+  const uint32_t nx = W[A[4]], ny = W[A[4] + 1], nz = W[A[4] + 2];
+  uint32_t X[64];
+  if (nm == "bc_periodic_xy") {
+    const uint32_t nf = std::min<uint32_t>(W[A[4 + 9]], 8);
+    for (uint32_t q = 0; q < nf; ++q) {
+      Unit u;
+      u.field = A[5 + q];
+      units.push_back(u);
+      writes.push_back(A[5 + q]);
+    }
+    return TE_OK;
+  }
+  if (nm == "adv_fused") {
+    const uint32_t kind = W[A[4 + 3]], chain = W[A[4 + 32]];
+    const uint32_t S = A[4 + 2], RU = A[4 + 13], RV = A[4 + 14], RW = A[4 + 4], C1 = A[4 + 15], C2 = A[4 + 16],
+                   ST = A[4 + 1], WE = A[4 + 5], PD = A[4 + 20], PW = A[4 + 21], FX = A[4 + 22], FY = A[4 + 23],
+                   FZ = A[4 + 24], TEND = A[4 + 11];
+    if (chain & 16u) {
+      slot(nx + 7, ny + 7, nz + 1, {{1, TEND}}, X);
+      if (unit("turb_zero", X) != TE_OK) return TE_ERR_UNSUPPORTED;
+    }
+    if (kind == 0) {
+      slot(nx + 1, ny, nz, {{1, S}, {2, RU}, {3, FX}}, X); if (unit("adv_flux_x", X) != TE_OK) return TE_ERR_UNSUPPORTED;
+      slot(nx, ny + 1, nz, {{1, S}, {2, RV}, {3, FY}}, X); if (unit("adv_flux_y", X) != TE_OK) return TE_ERR_UNSUPPORTED;
+      slot(nx, ny, nz + 1, {{1, S}, {2, RW}, {3, C1}, {4, C2}, {5, FZ}}, X);
+      if (unit("adv_flux_z", X) != TE_OK) return TE_ERR_UNSUPPORTED;
+    } else if (kind == 1) {
+      slot(nx + 1, ny, nz, {{1, S}, {2, RU}, {3, WE}, {4, PD}, {5, FX}}, X);
+      if (unit("adv_flux_x_weno", X) != TE_OK) return TE_ERR_UNSUPPORTED;
+      slot(nx, ny + 1, nz, {{1, S}, {2, RV}, {3, WE}, {4, PD}, {5, FY}}, X);
+      if (unit("adv_flux_y_weno", X) != TE_OK) return TE_ERR_UNSUPPORTED;
+      slot(nx, ny, nz + 1, {{1, S}, {2, RW}, {3, C1}, {4, C2}, {5, WE}, {6, PD}, {7, PW}, {8, FZ}}, X);
+      if (unit("adv_flux_z_weno", X) != TE_OK) return TE_ERR_UNSUPPORTED;
+    } else if (kind == 2) {
+      slot(nx + 2, ny + 2, nz, {{1, ST}, {2, S}, {3, RU}, {4, RV}, {5, RW}, {6, C1}, {7, C2}, {8, FX}}, X);
+      if (unit("adv_flux_x_stag", X) != TE_OK) return TE_ERR_UNSUPPORTED;
+      slot(nx + 2, ny + 2, nz, {{1, ST}, {2, S}, {3, RU}, {4, RV}, {5, RW}, {6, C1}, {7, C2}, {8, FY}}, X);
+      if (unit("adv_flux_y_stag", X) != TE_OK) return TE_ERR_UNSUPPORTED;
+      slot(nx + 1, ny + 1, nz + 1, {{1, ST}, {2, S}, {3, RU}, {4, RV}, {5, RW}, {6, C1}, {7, C2}, {8, FZ}}, X);
+      if (unit("adv_flux_z_stag", X) != TE_OK) return TE_ERR_UNSUPPORTED;
+    } else {
+      slot(nx + 2, ny + 2, nz, {{1, S}, {2, RU}, {3, C2}, {4, WE}, {5, FX}}, X);
+      if (unit("adv_flux_x_weno_w", X) != TE_OK) return TE_ERR_UNSUPPORTED;
+      slot(nx + 2, ny + 2, nz, {{1, S}, {2, RV}, {3, C2}, {4, WE}, {5, FY}}, X);
+      if (unit("adv_flux_y_weno_w", X) != TE_OK) return TE_ERR_UNSUPPORTED;
+      slot(nx + 1, ny + 1, nz + 1, {{1, S}, {2, RW}, {3, WE}, {4, FZ}}, X);
+      if (unit("adv_flux_z_weno_w", X) != TE_OK) return TE_ERR_UNSUPPORTED;
+    }
+    std::memcpy(X, A, sizeof(X));
+    X[4 + 3] = FX; X[4 + 4] = FY; X[4 + 5] = FZ;
+    for (int q = 20; q < 32; ++q) X[4 + q] = A[4 + q] * 0u;  // only parameters 0..19 and 30 belong to adv_assemble
+    X[4 + 30] = A[4 + 30];
+    if (unit("adv_assemble", X) != TE_OK) return TE_ERR_UNSUPPORTED;
+    const uint32_t full[3] = {nx, ny, nz}, whole[3] = {nx + 7, ny + 7, nz + 1};
+    if (chain & 1u) {
+      slot(full[0], full[1], full[2], {{1, A[4 + 33]}, {2, A[4 + 34]}, {3, TEND}}, X);
+      if (unit("apply_thterm", X) != TE_OK) return TE_ERR_UNSUPPORTED;
+    }
+    if (chain & 2u) {
+      slot(whole[0], whole[1], whole[2], {{1, A[4 + 35]}, {2, TEND}, {3, A[4 + 36]}}, X);
+      if (unit("turb_add_tendency", X) != TE_OK) return TE_ERR_UNSUPPORTED;
+    }
+    if (chain & 4u) {
+      slot(whole[0], whole[1], whole[2], {{1, A[4 + 37]}, {2, TEND}, {3, A[4 + 38]}}, X);
+      if (unit("turb_add_tendency", X) != TE_OK) return TE_ERR_UNSUPPORTED;
+    }
+    writes.push_back(TEND);
+    if (chain & 8u) {
+      slot(A[0], A[1], A[2], {{1, A[4 + 1]}, {2, A[4 + 39]}, {3, TEND}, {4, A[4 + 40]}, {5, A[4 + 41]}}, X);
+      if (unit("rk_update", X) != TE_OK) return TE_ERR_UNSUPPORTED;
+      writes.push_back(A[4 + 40]);
+    }
+    return TE_OK;
+  }
+  if (nm == "ac_pp_accumulate") {
+    std::memset(X, 0, sizeof(X));
+    X[0] = nx; X[1] = ny; X[2] = nz;
+    X[4] = A[4]; X[5] = A[5]; X[34] = A[34];
+    for (int q = 2; q <= 10; ++q) X[4 + q] = A[4 + q];  // w pk1 pk2 qk fpk pp3d ppd ppx saveX
+    if (unit("acoustic_pp", X) != TE_OK) return TE_ERR_UNSUPPORTED;
+    std::memset(X, 0, sizeof(X));
+    X[0] = A[0]; X[1] = A[1]; X[2] = A[2];
+    X[4] = A[4]; X[5] = A[5]; X[34] = A[34];
+    const int as[7] = {11, 12, 2, 13, 14, 15, 16};  // u v w rru rrv rrw tavg
+    for (int q = 0; q < 7; ++q) X[4 + 2 + q] = A[4 + as[q]];
+    if (unit("acoustic_accumulate", X) != TE_OK) return TE_ERR_UNSUPPORTED;
+    writes.insert(writes.end(), {A[4 + 7], A[4 + 8], A[4 + 13], A[4 + 14], A[4 + 15]});
+    if (W[A[4 + 10]] != 0u) writes.push_back(A[4 + 9]);
+    return TE_OK;
+  }
+  if (nm == "tendency_reset_add") {
+    std::memset(X, 0, sizeof(X));
+    X[0] = A[0]; X[1] = A[1]; X[2] = A[2];
+    X[4] = A[4]; X[5] = A[5]; X[6] = A[6]; X[7] = A[7]; X[34] = A[34];
+    if (unit("stage_tendency_reset", X) != TE_OK) return TE_ERR_UNSUPPORTED;
+    for (int q = 0; q < 3; ++q) {
+      slot(A[0], A[1], A[2], {{1, A[4 + 4 + q]}, {2, A[4 + 1 + q]}, {3, A[4 + 7]}}, X);
+      if (unit("turb_add_tendency", X) != TE_OK) return TE_ERR_UNSUPPORTED;
+    }
+    writes.insert(writes.end(), {A[5], A[6], A[7]});
+    return TE_OK;
+  }
+  if (nm == "add_fields") {
+    const uint32_t nf = std::min<uint32_t>(W[A[4 + 13]], 4);
+    for (uint32_t q = 0; q < nf; ++q) {
+      slot(A[0], A[1], A[2], {{1, A[5 + q]}, {2, A[9 + q]}, {3, A[13 + q]}}, X);
+      if (unit("turb_add_tendency", X) != TE_OK) return TE_ERR_UNSUPPORTED;
+      writes.push_back(A[9 + q]);
+    }
+    return TE_OK;
+  }
+  if (nm == "zero_fields") {
+    const uint32_t nf = std::min<uint32_t>(W[A[4 + 9]], 8);
+    for (uint32_t q = 0; q < nf; ++q) {
+      slot(A[0], A[1], A[2], {{1, A[5 + q]}}, X);
+      if (unit("turb_zero", X) != TE_OK) return TE_ERR_UNSUPPORTED;
+      writes.push_back(A[5 + q]);
+    }
+    return TE_OK;
+  }
+  if (nm == "copy_fields") {
+    const uint32_t nf = std::min<uint32_t>(W[A[4 + 9]], 4);
+    for (uint32_t q = 0; q < nf; ++q) {
+      std::memset(X, 0, sizeof(X));
+      X[0] = A[0]; X[1] = 1; X[2] = 1;
+      X[4] = A[5 + q]; X[5] = A[9 + q];
+      if (unit("_copy", X) != TE_OK) return TE_ERR_UNSUPPORTED;
+      writes.push_back(A[9 + q]);
+    }
+    return TE_OK;
+  }
+  std::string ref = nm;
+  const bool lds = ref.size() > 4 && ref.compare(ref.size() - 4, 4, "_lds") == 0;
+  if (lds) ref.resize(ref.size() - 4);  // the same sweep with its elimination coefficients in LDS
+  if (unit(ref.c_str(), A) != TE_OK) return TE_ERR_UNSUPPORTED;
+  if (ref == "rayleigh_damping" && (W[A[4 + 17]] != 0u || W[A[4 + 16]] != 0u))
+    return afail(a, TE_ERR_UNSUPPORTED, "no adjoint for Rayleigh damping toward horizontal means or at lateral edges");
+  if (ref == "_copy") {
+    writes.push_back(A[5]);
+    return TE_OK;
+  }
+  int wp[8];
+  const int nw = te::adj::writes(units.back().adj, wp);
+  for (int q = 0; q < nw; ++q) {
+    if (ref == "acoustic_pp" && wp[q] == 9 && W[A[4 + 10]] == 0u) continue;  // ppx unused
+    if (lds && ref == "acoustic_w_solve" && wp[q] != 9) continue;            // cA, cB stay in LDS
+    writes.push_back(A[4 + wp[q]]);
+  }
   return TE_OK;
 }
 
@@ -133,42 +327,30 @@ int te_adjoint_create(te_ctx* ctx, int32_t every, te_adjoint** out) {
   if (every < 1) return afail(a, TE_ERR_ARG, "checkpoint interval must be at least 1");
   a->fwd = ctx;
   a->every = every;
-  te_config cfg = ctx->cfg;
-  cfg.fuse = 0;
+  te_config cfg = ctx->cfg;  // the same program (fused or the reference's list) on memory of its own
   int rc = te_create(&cfg, &a->rec);
-  if (rc != TE_OK) return afail(a, rc, std::string("unfused program: ") + (a->rec ? te_last_error(a->rec) : ""));
+  if (rc != TE_OK) return afail(a, rc, std::string("program: ") + (a->rec ? te_last_error(a->rec) : ""));
   te_ctx* r = a->rec;
   const te::Layout &L = r->layout, &F = ctx->layout;
   if (L.arena != F.arena || L.field_words != F.field_words || L.fields != F.fields)
-    return afail(a, TE_ERR_STATE, "fused and unfused layouts differ");
+    return afail(a, TE_ERR_STATE, "layouts differ");
   a->state_words = L.arena + L.field_words * L.fields.size();
   const uint64_t arena_end = a->state_words;
-  // per dispatch: its adjoint and the regions it writes
+  // per dispatch: the reference dispatches it stands for and the regions it writes
   const int n = (int)r->ops.size();
-  a->adj_id.resize(n);
+  a->units.resize(n);
   a->wreg.resize(n);
   for (int i = 0; i < n; ++i) {
+    std::vector<uint64_t> writes;
+    rc = decompose(a, i, a->units[i], writes);
+    if (rc != TE_OK) return rc;
     const std::string& nm = r->op_names[i];
-    const int id = te::adj::find(nm.c_str());
-    if (id < 0) return afail(a, TE_ERR_UNSUPPORTED, "no adjoint for kernel '" + nm + "'");
-    a->adj_id[i] = id;
-    const uint32_t* A = r->ops[i].A;
-    if (nm == "rayleigh_damping" && (r->image[A[4 + 17]] != 0u || r->image[A[4 + 16]] != 0u))
-      return afail(a, TE_ERR_UNSUPPORTED, "no adjoint for Rayleigh damping toward horizontal means or at lateral edges");
-    if (nm == "_copy") {
-      a->wreg[i].push_back({A[5], A[0]});
-      continue;
-    }
-    int wp[8];
-    const int nw = te::adj::writes(id, wp);
-    for (int q = 0; q < nw; ++q) {
-      const uint64_t off = A[4 + wp[q]];
-      if (nm == "acoustic_pp" && wp[q] == 9 && r->image[A[4 + 10]] == 0u) continue;  // ppx unused
+    for (uint64_t off : writes) {
       if (off < L.arena || off >= arena_end || (off - L.arena) % L.field_words != 0)
         return afail(a, TE_ERR_UNSUPPORTED, "dispatch " + std::to_string(i) + " (" + nm + ") writes outside the FieldArena");
       bool dup = false;
       for (auto& w : a->wreg[i]) dup = dup || w.first == off;
-      if (!dup) a->wreg[i].push_back({off, L.field_words});
+      if (!dup) a->wreg[i].push_back({off, (nm == "_copy" || nm == "copy_fields") ? (uint64_t)r->ops[i].A[0] : L.field_words});
     }
   }
   a->halo_pair.assign(n, 0);
@@ -276,10 +458,17 @@ int te_adj_ops(te_adjoint* a, int32_t first, int32_t last) {
   const int n = (int)r->ops.size();
   if (first < 0 || last >= n || first > last) return afail(a, TE_ERR_ARG, "bad dispatch range");
   for (int i = last; i >= first; --i)
-    TA_HIP(te::adj::launch(a->adj_id[i], r->ops[i].A, nullptr, reinterpret_cast<const float*>(r->dM), a->dMb,
-                           r->layout.nz, r->layout.sJ, r->layout.sK,
-                           a->dScr, r->layout.field_words, r->stream),
-           std::string("adjoint of ") + r->op_names[i]);
+    for (int u = (int)a->units[i].size() - 1; u >= 0; --u) {
+      const Unit& un = a->units[i][u];
+      if (un.adj < 0)
+        TA_HIP(te::adj::halo_xy(a->dMb, un.field, r->layout.nx, r->layout.ny, r->layout.nz, r->layout.sJ, r->layout.sK,
+                                r->stream),
+               "adjoint of the periodic halos");
+      else
+        TA_HIP(te::adj::launch(un.adj, un.A, nullptr, reinterpret_cast<const float*>(r->dM), a->dMb, r->layout.nz,
+                               r->layout.sJ, r->layout.sK, a->dScr, r->layout.field_words, r->stream),
+               std::string("adjoint of ") + r->op_names[i]);
+    }
   TA_HIP(hipStreamSynchronize(r->stream), "sync");
   return TE_OK;
 }

@@ -50,6 +50,14 @@ void fill_initial_state(const te_config& cfg, const Layout& L, std::vector<uint3
 
 }  // namespace te
 
+struct te_ctx;
+namespace te {
+// Launches ops [first, last] of a context in order on its stream (no events,
+// no graph); TE_OK or a status with the context's error set.
+int run_ops_plain(te_ctx* c, int first, int last);
+
+}  // namespace te
+
 struct te_ctx {
   uint32_t* dBase = nullptr;  // allocation (front guard + M + back guard)
   uint64_t front = 0;         // words of the front guard (dM = dBase + front)

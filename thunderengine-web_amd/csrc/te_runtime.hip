@@ -372,6 +372,14 @@ static int ensure_graph(te_ctx* c) {
   return TE_OK;
 }
 
+int te::run_ops_plain(te_ctx* c, int first, int last) {
+  if (!c || !c->dM) return fail(c, TE_ERR_STATE, "context has no memory");
+  if (first < 0 || last >= (int)c->ops.size() || first > last) return fail(c, TE_ERR_ARG, "bad op range");
+  int rc = ensure_guard(c);
+  if (rc != TE_OK) return rc;
+  return run_ops(c, first, last);
+}
+
 extern "C" {
 
 int te_program_create(uint64_t mem_words, int32_t nops, const char* const* names, const uint32_t* args, te_ctx** out) {
