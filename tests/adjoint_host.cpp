@@ -50,9 +50,8 @@ void gathered(const float* M, float* Mb, const AdjArg& a) {
     for (uint32_t z = 0; z < a.n[2]; ++z)
       for (uint32_t y = 0; y < a.n[1]; ++y)
         for (uint32_t x = 0; x < a.n[0]; ++x) G::prep(M, Mb, a, scr.data(), (int)x, (int)y, (int)z);
-  for (int z = 0; z <= g.nz; ++z)
-    for (int y = 0; y < g.sK; ++y)
-      for (int x = 0; x < g.sJ; ++x) G::gather(M, Mb, a, scr.data(), x, y, z);
+  const int64_t lead = G::lead(g), words = (int64_t)fw;
+  for (int64_t q = -lead; q < words; ++q) G::gather(M, Mb, a, scr.data(), q);
 }
 
 struct HostEntry {

@@ -24,7 +24,8 @@ CONFIGS = {
     "dry_weno_momentum": dict(SMALL, moist=0, weno_momentum=1),
 }
 DEAD = ("cA", "cB")  # elimination scratch of the column solve: never read after it
-GATHERED = ("adv_assemble", "acoustic_uv", "acoustic_fpk", "acoustic_pp")
+GATHERED = ("adv_assemble", "acoustic_uv", "acoustic_fpk", "acoustic_pp", "adv_flux_x", "adv_flux_y", "adv_flux_x_stag",
+            "adv_flux_y_stag")
 
 
 def _arena(e):
@@ -79,8 +80,7 @@ def test_dispatch_adjoints_transpose_the_tangent_linear_build(name):
 
 @pytest.mark.parametrize("name", sorted(CONFIGS))
 def test_gather_adjoints_equal_the_scatter_form(name):
-    """adv_assemble, acoustic_uv, acoustic_fpk and acoustic_pp run
-    gather-form adjoints in the library; word for word they equal the per-point bodies' scatter form
+    """The kernels in GATHERED run gather-form adjoints in the library; word for word they equal the per-point bodies' scatter form
     (up to binary32 against binary64 rounding of the Jacobian entries)."""
     R = runs.OracleRun(te_host.config(fuse=0, **CONFIGS[name]))
     e = R.enc

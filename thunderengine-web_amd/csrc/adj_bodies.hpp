@@ -1211,7 +1211,8 @@ TE_HD void w_solve_clear(float* Mb, const AdjArg& a, const ADims& g, int i, int 
 }
 
 // ---- gather-form adjoints of linear stencils.  One thread per word q of the
-// field grid (all (nx+7)(ny+7)(nz+1) words) adds to each input field's adjoint
+// field grid (all (nx+7)(ny+7)(nz+1) words, and lead(g) words before the
+// field's first one that a window reaches by address) adds to each input field's adjoint
 // at q the contributions of every cell whose stencil reads q: no atomics,
 // single pass, binary32 (the local Jacobian entries are the forward's own
 // values).  Outputs the dispatch overwrites are first copied to a scratch
@@ -1235,9 +1236,16 @@ struct G_adv_assemble {
     scr[c] = Mb[w];
     if (r.I(12) == 0) Mb[w] = 0.0f;
   }
-  static TE_HD void gather(const float* M, float* Mb, const AdjArg& a, const float* scr, int x, int y, int k) {
+  static TE_HD void gather(const float* M, float* Mb, const AdjArg& a, const float* scr, int64_t qw) {
     Acc r(M, Mb, &a);
     const ADims g = r.dims();
+    int x, y, k;  // the word's coordinates, offset by the halo
+    {
+      int i_, j_;
+      g.coords(qw, i_, j_, k);
+      x = i_ + 3;
+      y = j_ + 3;
+    }
     const int i = x - 3, j = y - 3;
     const int stag = r.I(1), st = stag < 0 ? -stag : stag;
     const P s = r.f(2), divx = r.f(6), rho0 = r.f(7), rf0 = r.f(8), mh = r.f(9), mf = r.f(10),
@@ -1279,15 +1287,23 @@ struct G_adv_assemble {
     if (sb != 0.0f) Mb[a.p[2] + q] += sb;
   }
   static constexpr bool kPrep = true;
+  static TE_HD int64_t lead(const ADims&) { return 0; }
 };
 
 // acoustic_uv.wgsl:144-181: u += dts (uten - P (thv + thv[-1])), P = tem1 (ppd - ppd[-1]) uf
 // (likewise v); u and v are updated in place, so their adjoints pass through.
 struct G_acoustic_uv {
   static TE_HD void prep(const float*, float*, const AdjArg&, float*, int, int, int) {}
-  static TE_HD void gather(const float* M, float* Mb, const AdjArg& a, const float*, int x, int y, int k) {
+  static TE_HD void gather(const float* M, float* Mb, const AdjArg& a, const float*, int64_t qw) {
     Acc r(M, Mb, &a);
     const ADims g = r.dims();
+    int x, y, k;  // the word's coordinates, offset by the halo
+    {
+      int i_, j_;
+      g.coords(qw, i_, j_, k);
+      x = i_ + 3;
+      y = j_ + 3;
+    }
     const int i = x - 3, j = y - 3;
     if (k >= g.nz) return;
     const float dts = r.F(1, 0);
@@ -1330,6 +1346,7 @@ struct G_acoustic_uv {
     if (vtb != 0.0f) Mb[a.p[5] + q] += vtb;
   }
   static constexpr bool kPrep = false;
+  static TE_HD int64_t lead(const ADims&) { return 0; }
 };
 
 // acoustic_fpk.wgsl:167-207: fpk = beta A + (dts B + advPi) with A = w pk1+qk-part,
@@ -1347,9 +1364,16 @@ struct G_acoustic_fpk {
     scr[c] = Mb[w];
     Mb[w] = 0.0f;
   }
-  static TE_HD void gather(const float* M, float* Mb, const AdjArg& a, const float* scr, int x, int y, int k) {
+  static TE_HD void gather(const float* M, float* Mb, const AdjArg& a, const float* scr, int64_t qw) {
     Acc r(M, Mb, &a);
     const ADims g = r.dims();
+    int x, y, k;  // the word's coordinates, offset by the halo
+    {
+      int i_, j_;
+      g.coords(qw, i_, j_, k);
+      x = i_ + 3;
+      y = j_ + 3;
+    }
     const int i = x - 3, j = y - 3;
     const float dts = r.F(1, 0), beta = r.F(1, 2), smeps = r.F(1, 4);
     const P u = r.f(2), v = r.f(3), w = r.f(4), pi = r.f(5), ppterm = r.f(7), pk1 = r.f(8), pk2 = r.f(9),
@@ -1430,6 +1454,7 @@ struct G_acoustic_fpk {
     if (qkb != 0.0f) Mb[a.p[10] + q] += qkb;
   }
   static constexpr bool kPrep = true;
+  static TE_HD int64_t lead(const ADims&) { return 0; }
 };
 
 // acoustic_pp.wgsl:131-167: p = alph (w pk1+qk-part) + (prev + fpk), zeroed
@@ -1466,9 +1491,16 @@ struct G_acoustic_pp {
     Mb[a.p[5] + c] += A * (w1 - w2);
     Mb[a.p[2] + c] += A * (pk1[c] + q);
   }
-  static TE_HD void gather(const float* M, float* Mb, const AdjArg& a, const float* scr, int x, int y, int k) {
+  static TE_HD void gather(const float* M, float* Mb, const AdjArg& a, const float* scr, int64_t qw) {
     Acc r(M, Mb, &a);
     const ADims g = r.dims();
+    int x, y, k;  // the word's coordinates, offset by the halo
+    {
+      int i_, j_;
+      g.coords(qw, i_, j_, k);
+      x = i_ + 3;
+      y = j_ + 3;
+    }
     const int i = x - 3, j = y - 3;
     if (i < 0 || j < 0 || i >= g.nx || j >= g.ny || k < 1 || k > g.nz) return;
     const P pk2 = r.f(4), qk = r.f(5);
@@ -1477,6 +1509,123 @@ struct G_acoustic_pp {
     if (A != 0.0f) Mb[a.p[2] + cb + g.sk()] += A * (pk2[cb] - qk[cb]);
   }
   static constexpr bool kPrep = true;
+  static TE_HD int64_t lead(const ADims&) { return 0; }
+};
+
+// Horizontal flx5 fluxes, scalar (adv_flux_x.wgsl:120-143, adv_flux_y.wgsl:120-144)
+// and staggered momentum (adv_flux_x_stag.wgsl:144-186, _y_): fl = m f, f the
+// upwind (2, -13, 47, 27, -3)/60 face value of a 5-point window along the
+// direction, m the face mass flux.  A word of the advected field collects
+// fbar m w_p over the (at most 7) cells whose window holds it at position p;
+// a mass-flux word collects fbar f (times the interpolation weight) over the
+// cells whose m reads it.
+template <int DIR, bool STAG>
+struct G_flux_h {
+  static constexpr bool kPrep = true;
+  // a y window at row j = -1 of level 0 reaches one row before the field
+  static TE_HD int64_t lead(const ADims& g) { return (STAG && DIR == 1) ? g.sJ : 0; }
+  static TE_HD float wt(int p) {
+    return p == 0 ? 2.0f / 6e+01f : p == 1 ? -13.0f / 6e+01f : p == 2 ? 47.0f / 6e+01f : p == 3 ? 27.0f / 6e+01f : -3.0f / 6e+01f;
+  }
+  // cells the forward writes / whose value depends on its inputs
+  static TE_HD bool cell_out(const AdjArg& a, const ADims& g, int i, int j, int k) {
+    if (STAG) {
+      if (i < -1 || j < -1 || i > g.nx || j > g.ny || k < 0 || k >= g.nz) return false;
+      return i + 1 < (int)a.n[0] && j + 1 < (int)a.n[1] && k < (int)a.n[2];
+    }
+    if (i < 0 || j < 0 || k < 0 || k >= g.nz) return false;
+    if (DIR == 0 ? (i > g.nx || j >= g.ny) : (i >= g.nx || j > g.ny)) return false;
+    return i < (int)a.n[0] && j < (int)a.n[1] && k < (int)a.n[2];
+  }
+  static TE_HD bool cell_dep(const AdjArg& a, const ADims& g, int stag, int i, int j, int k) {
+    return cell_out(a, g, i, j, k) && !(STAG && stag == 4 && k == 0);
+  }
+  static TE_HD int pa() { return STAG ? 2 : 1; }                  // advected field
+  static TE_HD int prr() { return STAG ? (DIR == 0 ? 3 : 4) : 2; }  // mass flux
+  static TE_HD int pout() { return STAG ? 8 : 3; }
+  static TE_HD float mass(const float* rr, const float* c2, const ADims& g, int stag, int64_t c, int k) {
+    if (!STAG) return rr[c];
+    const int64_t sj = g.sJ, sk = g.sk();
+    if (stag == 2) return 0.5f * (rr[c] + rr[c + (DIR == 0 ? 1 : -1)]);
+    if (stag == 3) return 0.5f * (rr[c] + rr[c + (DIR == 0 ? -sj : sj)]);
+    if (stag == 4) return fmaf_(c2[k], rr[c], (1.0f - c2[k]) * rr[c - sk]);
+    return rr[c];
+  }
+  // window offset of position p (0..4) for an upwind / shifted stencil
+  static TE_HD int offset(bool up, bool shifted, int p) { return up ? (shifted ? p - 2 : p - 3) : (shifted ? 3 - p : 2 - p); }
+  static TE_HD void prep(const float* M, float* Mb, const AdjArg& a, float* scr, int x, int y, int k) {
+    Acc r(M, Mb, &a);
+    const ADims g = r.dims();
+    const int i = STAG ? x - 1 : x, j = STAG ? y - 1 : y;
+    if (!cell_out(a, g, i, j, k)) return;
+    const int stag = STAG ? r.I(1) : 1;
+    const int64_t c = g.idx(i, j, k), w = (int64_t)a.p[pout()] + c;
+    scr[c] = cell_dep(a, g, stag, i, j, k) ? Mb[w] : 0.0f;
+    Mb[w] = 0.0f;
+  }
+  static TE_HD void gather(const float* M, float* Mb, const AdjArg& a, const float* scr, int64_t qw) {
+    Acc r(M, Mb, &a);
+    const ADims g = r.dims();
+    int x, y, k;  // the word's coordinates, offset by the halo
+    {
+      int i_, j_;
+      g.coords(qw, i_, j_, k);
+      x = i_ + 3;
+      y = j_ + 3;
+    }
+    const int i = x - 3, j = y - 3;
+    if (k >= g.nz) return;
+    const int stag = STAG ? r.I(1) : 1;
+    const bool shifted = STAG && stag == (DIR == 0 ? 2 : 3);
+    const P av = r.f(pa()), rr = r.f(prr());
+    const float* c2 = STAG ? (const float*)r.f(7) : nullptr;
+    const int64_t sj = g.sJ, sk = g.sk(), st = DIR == 0 ? 1 : sj, q = g.idx(i, j, k);
+    auto face = [&](int64_t c, bool up) {
+      float v[5];
+      for (int p = 0; p < 5; ++p) v[p] = av[c + offset(up, shifted, p) * st];
+      return fmaf_(-3.0f / 6e+01f, v[4],
+                   fmaf_(27.0f / 6e+01f, v[3], fmaf_(47.0f / 6e+01f, v[2], fmaf_(2.0f / 6e+01f, v[0], (-13.0f / 6e+01f) * v[1]))));
+    };
+    // the advected field.  Candidate cells by linear index, decoded: a window
+    // at the grid's first row / column reaches into the previous one by
+    // address, exactly as the forward reads it.
+    float ab_ = 0.0f;
+    for (int o = -3; o <= 3; ++o) {
+      const int64_t c = q - o * st;
+      int ci = i, cj = j, ck = k;
+      g.step(DIR, -o, ci, cj, ck);
+      if (!cell_dep(a, g, stag, ci, cj, ck)) continue;
+      const float fb = scr[c];
+      if (fb == 0.0f) continue;
+      const float m = mass(rr, c2, g, stag, c, ck);
+      const bool up = m >= 0.0f;
+      for (int p = 0; p < 5; ++p)
+        if (offset(up, shifted, p) == o) ab_ += (fb * m) * wt(p);
+    }
+    // the mass flux: the cells whose m reads word q, with their weights
+    float rb = 0.0f;
+    auto add_m = [&](int ci, int cj, int ck, float lam) {
+      if (ck < 0 || !cell_dep(a, g, stag, ci, cj, ck)) return;
+      const int64_t c = g.idx(ci, cj, ck);
+      const float fb = scr[c];
+      if (fb == 0.0f) return;
+      rb += lam * (fb * face(c, mass(rr, c2, g, stag, c, ck) >= 0.0f));
+    };
+    if (!STAG || (stag != 2 && stag != 3 && stag != 4)) {
+      add_m(i, j, k, 1.0f);
+    } else if (stag == 2) {  // m at c uses rr[c] and rr[c + 1] (x) / rr[c - 1] (y)
+      add_m(i, j, k, 0.5f);
+      if (DIR == 0) add_m(i - 1, j, k, 0.5f); else add_m(i + 1, j, k, 0.5f);
+    } else if (stag == 3) {  // rr[c] and rr[c - sj] (x) / rr[c + sj] (y)
+      add_m(i, j, k, 0.5f);
+      if (DIR == 0) add_m(i, j + 1, k, 0.5f); else add_m(i, j - 1, k, 0.5f);
+    } else {  // stag 4: c2[k] rr[c] + (1 - c2[k]) rr[c - sk]
+      add_m(i, j, k, c2[k]);
+      if (k + 1 < g.nz) add_m(i, j, k + 1, 1.0f - c2[k + 1]);
+    }
+    if (ab_ != 0.0f) Mb[a.p[pa()] + q] += ab_;
+    if (rb != 0.0f) Mb[a.p[prr()] + q] += rb;
+  }
 };
 
 }  // namespace adj
@@ -1534,9 +1683,14 @@ struct G_acoustic_pp {
   X("rayleigh_damping", B_rayleigh_damping, 4, {10, 11, 12, 13})
 
 // Kernels whose adjoint runs in gather form (replacing the per-point body's
-// scatter form): X(name, gather struct).
+// scatter form): X(name, gather struct); COMMA separates template arguments.
+#define COMMA ,
 #define TE_ADJ_GATHER_KERNELS(X) \
   X("adv_assemble", G_adv_assemble) \
   X("acoustic_uv", G_acoustic_uv) \
   X("acoustic_fpk", G_acoustic_fpk) \
-  X("acoustic_pp", G_acoustic_pp)
+  X("acoustic_pp", G_acoustic_pp) \
+  X("adv_flux_x", G_flux_h<0 COMMA false>) \
+  X("adv_flux_y", G_flux_h<1 COMMA false>) \
+  X("adv_flux_x_stag", G_flux_h<0 COMMA true>) \
+  X("adv_flux_y_stag", G_flux_h<1 COMMA true>)

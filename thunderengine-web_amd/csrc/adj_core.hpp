@@ -111,6 +111,39 @@ struct ADims {
   TE_HD int64_t idx(int i, int j, int k) const {
     return (int64_t)(i + 3) + (int64_t)sJ * ((j + 3) + (int64_t)sK * k);
   }
+  // inverse of idx (floor division: words before the field's first word
+  // decode to k = -1)
+  TE_HD void coords(int64_t q, int& i, int& j, int& k) const {
+    const uint32_t plane = (uint32_t)(sJ * sK);
+    uint32_t u;  // 32-bit divisions: a field has fewer than 2^31 words
+    if (q >= 0) {
+      u = (uint32_t)q;
+      k = (int)(u / plane);
+    } else {
+      const int64_t up = q + (int64_t)plane;  // words before the field: level -1
+      u = (uint32_t)up;
+      k = -1;
+    }
+    const uint32_t rem = q >= 0 ? u - (uint32_t)k * plane : u;
+    const uint32_t jj = rem / (uint32_t)sJ;
+    j = (int)jj - 3;
+    i = (int)(rem - jj * (uint32_t)sJ) - 3;
+  }
+  // (i, j, k) moved by d along i (axis 0) or j (axis 1), wrapping into the
+  // neighbouring row / level as the linear index does (|d| < sJ, sK)
+  TE_HD void step(int axis, int d, int& i, int& j, int& k) const {
+    if (axis == 0) {
+      i += d;
+      if (i < -3) { i += sJ; --j; }
+      else if (i > sJ - 4) { i -= sJ; ++j; }
+      if (j < -3) { j += sK; --k; }
+      else if (j > sK - 4) { j -= sK; ++k; }
+    } else {
+      j += d;
+      if (j < -3) { j += sK; --k; }
+      else if (j > sK - 4) { j -= sK; ++k; }
+    }
+  }
 };
 
 constexpr int MAXO = 8;  // outputs per point

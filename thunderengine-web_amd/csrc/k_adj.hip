@@ -72,13 +72,18 @@ __global__ __launch_bounds__(256) void k_adj_prep(const float* M, float* Mb, Adj
   G::prep(M, Mb, a, scr, (int)x, (int)y, (int)z);
 }
 template <class G>
-__global__ __launch_bounds__(256) void k_adj_gather(const float* M, float* Mb, AdjArg a, const float* scr, uint32_t sJ,
-                                                    uint32_t sK, uint32_t nz1) {
+__global__ __launch_bounds__(256) void k_adj_gather(const float* M, float* Mb, AdjArg a, const float* scr, uint64_t lead,
+                                                    uint64_t words) {
   const uint64_t t = (uint64_t)blockIdx.x * 256u + threadIdx.x;
-  const uint64_t n01 = (uint64_t)sJ * sK;
-  if (t >= n01 * nz1) return;
-  const uint64_t z = t / n01, rem = t - z * n01, y = rem / sJ, x = rem - y * sJ;
-  G::gather(M, Mb, a, scr, (int)x, (int)y, (int)z);
+  if (t >= lead + words) return;
+  G::gather(M, Mb, a, scr, (int64_t)t - (int64_t)lead);
+}
+template <class G>
+uint64_t gather_lead(int nx, int ny, int nz, int sJ, int sK) {
+  ADims g;
+  g.nx = nx; g.ny = ny; g.nz = nz; g.sJ = sJ; g.sK = sK;
+  g.rdx = g.rdy = g.rdz = 0.0f;
+  return (uint64_t)G::lead(g);
 }
 
 // bc_periodic_x then bc_periodic_y on one field leave every halo word
@@ -124,9 +129,10 @@ struct GatherEntry {
   const char* name;
   const void* prep;
   const void* gather;
+  uint64_t (*lead)(int, int, int, int, int);
 };
 #define GK(n, G) {n, G::kPrep ? reinterpret_cast<const void*>(&k_adj_prep<G>) : nullptr, \
-                  reinterpret_cast<const void*>(&k_adj_gather<G>)},
+                  reinterpret_cast<const void*>(&k_adj_gather<G>), &gather_lead<G>},
 static const GatherEntry kGather[] = {TE_ADJ_GATHER_KERNELS(GK)};
 #undef GK
 
@@ -208,10 +214,9 @@ hipError_t launch(int id, const uint32_t* A, const float* const* vals, const flo
       }
     }
     const float* cscr = scr;
-    uint32_t uj = (uint32_t)sJ, uk = (uint32_t)sK, un = (uint32_t)(nz + 1);
-    void* gargs[] = {(void*)&M, (void*)&Mb, (void*)&a, (void*)&cscr, (void*)&uj, (void*)&uk, (void*)&un};
-    const uint64_t words = (uint64_t)uj * uk * un;
-    return hipLaunchKernel(ge.gather, dim3((unsigned)((words + 255) / 256)), dim3(256), gargs, 0, stream);
+    uint64_t words = (uint64_t)sJ * sK * (nz + 1), lead = ge.lead(0, 0, nz, sJ, sK);
+    void* gargs[] = {(void*)&M, (void*)&Mb, (void*)&a, (void*)&cscr, (void*)&lead, (void*)&words};
+    return hipLaunchKernel(ge.gather, dim3((unsigned)((lead + words + 255) / 256)), dim3(256), gargs, 0, stream);
   }
   if (e.column) {  // acoustic_w_solve: coefficients, sweeps, levels, clear
     const uint64_t cols = (uint64_t)a.n[0] * a.n[1];
