@@ -25,7 +25,7 @@ CONFIGS = {
 }
 DEAD = ("cA", "cB")  # elimination scratch of the column solve: never read after it
 GATHERED = ("adv_assemble", "acoustic_uv", "acoustic_fpk", "acoustic_pp", "adv_flux_x", "adv_flux_y", "adv_flux_x_stag",
-            "adv_flux_y_stag")
+            "adv_flux_y_stag", "acoustic_accumulate")
 
 
 def _arena(e):

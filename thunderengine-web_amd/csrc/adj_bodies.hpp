@@ -1091,11 +1091,13 @@ TE_HD void w_solve_sweep(const float* M, float* Mb, const AdjArg& a, float* scr,
   if (i >= g.nx || j >= g.ny) return;
   const int nz = g.nz;
   const int64_t sk = g.sk(), c0 = g.idx(i, j, 0);
-  const float *AA = scr + c0, *BB = scr + fw + c0, *DD = scr + 3 * fw + c0;
-  float* RR = scr + 2 * fw + c0;  // c, then 1 / pivot
-  float* wb = Mb + a.p[9] + c0;   // adjoint of w (column)
-  float* S1 = Mb + a.p[10] + c0;  // adjoint words of cA: dbar_k
-  float* S2 = Mb + a.p[11] + c0;  // adjoint words of cB: the new w
+  const float* __restrict__ AA = scr + c0;
+  const float* __restrict__ BB = scr + fw + c0;
+  const float* __restrict__ DD = scr + 3 * fw + c0;
+  float* __restrict__ RR = scr + 2 * fw + c0;  // c, then 1 / pivot
+  float* __restrict__ wb = Mb + a.p[9] + c0;   // adjoint of w (column)
+  float* __restrict__ S1 = Mb + a.p[10] + c0;  // adjoint words of cA: dbar_k
+  float* __restrict__ S2 = Mb + a.p[11] + c0;  // adjoint words of cB: the new w
   float cAp = 0.0f, cBp = 0.0f;
   for (int k = 1; k <= nz - 1; ++k) {
     const float aa = AA[k * sk], bb = BB[k * sk], cc = RR[k * sk], dd = DD[k * sk];
@@ -1628,6 +1630,39 @@ struct G_flux_h {
   }
 };
 
+// acoustic_accumulate.wgsl:129-158: rr += u (times tavg on the last
+// substep, rrw with alph w), all in place and pointwise: per word, no
+// neighbours, no prep.
+struct G_acoustic_accumulate {
+  static constexpr bool kPrep = false;
+  static TE_HD int64_t lead(const ADims&) { return 0; }
+  static TE_HD void prep(const float*, float*, const AdjArg&, float*, int, int, int) {}
+  static TE_HD void gather(const float* M, float* Mb, const AdjArg& a, const float*, int64_t qw) {
+    Acc r(M, Mb, &a);
+    const ADims g = r.dims();
+    int i, j, k;
+    g.coords(qw, i, j, k);
+    if (i < -3 || j < -3 || k < 0 || i > g.nx + 3 || j > g.ny + 3 || k > g.nz) return;
+    if (i + 3 >= (int)a.n[0] || j + 3 >= (int)a.n[1] || k >= (int)a.n[2]) return;
+    const float alph = r.F(1, 1), tavg = r.F(8);
+    const int64_t c = qw;
+    float* rub = Mb + a.p[5] + c;
+    float* rvb = Mb + a.p[6] + c;
+    float* rwb = Mb + a.p[7] + c;
+    if (tavg > 0.0f) {
+      const float u = *rub * tavg, v = *rvb * tavg, w = *rwb * tavg;
+      *rub = u; *rvb = v; *rwb = w;
+      if (u != 0.0f) Mb[a.p[2] + c] += u;
+      if (v != 0.0f) Mb[a.p[3] + c] += v;
+      if (w != 0.0f) Mb[a.p[4] + c] += w * alph;
+    } else {
+      if (*rub != 0.0f) Mb[a.p[2] + c] += *rub;
+      if (*rvb != 0.0f) Mb[a.p[3] + c] += *rvb;
+      if (*rwb != 0.0f) Mb[a.p[4] + c] += *rwb;
+    }
+  }
+};
+
 }  // namespace adj
 }  // namespace te
 
@@ -1693,4 +1728,5 @@ struct G_flux_h {
   X("adv_flux_x", G_flux_h<0 COMMA false>) \
   X("adv_flux_y", G_flux_h<1 COMMA false>) \
   X("adv_flux_x_stag", G_flux_h<0 COMMA true>) \
-  X("adv_flux_y_stag", G_flux_h<1 COMMA true>)
+  X("adv_flux_y_stag", G_flux_h<1 COMMA true>) \
+  X("acoustic_accumulate", G_acoustic_accumulate)
