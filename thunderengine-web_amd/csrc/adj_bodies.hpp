@@ -1085,26 +1085,36 @@ TE_HD void w_solve_coef(const float* M, const AdjArg& a, float* scr, uint64_t fw
 // words of cA and the new w in those of cB (levels 1..nz-1) and clears wbar
 // for the inputs' adjoints.
 TE_HD void w_solve_sweep(const float* M, float* Mb, const AdjArg& a, float* scr, uint64_t fw, float* tA, float* tB,
-                         int64_t ts, int i, int j) {
+                         int64_t ts, int i, int j, float* pre = nullptr, int64_t ps = 0) {
   Acc r(M, Mb, &a);
   const ADims g = r.dims();
   if (i >= g.nx || j >= g.ny) return;
   const int nz = g.nz;
   const int64_t sk = g.sk(), c0 = g.idx(i, j, 0);
-  const float* __restrict__ AA = scr + c0;
-  const float* __restrict__ BB = scr + fw + c0;
-  const float* __restrict__ DD = scr + 3 * fw + c0;
-  float* __restrict__ RR = scr + 2 * fw + c0;  // c, then 1 / pivot
   float* __restrict__ wb = Mb + a.p[9] + c0;   // adjoint of w (column)
   float* __restrict__ S1 = Mb + a.p[10] + c0;  // adjoint words of cA: dbar_k
   float* __restrict__ S2 = Mb + a.p[11] + c0;  // adjoint words of cB: the new w
+  // the column's a, b, c (then 1 / pivot), d and wbar: the scratch planes and
+  // wb by default, or a copy of them (5 arrays of stride ps, level-major
+  // groups of nz + 1) the caller preloaded, e.g. into LDS
+  const float *AA, *BB, *DD, *WB;
+  float* RR;
+  int64_t cs;
+  if (pre) {
+    AA = pre; BB = pre + (nz + 1) * ps; RR = pre + 2 * (nz + 1) * ps; DD = pre + 3 * (nz + 1) * ps;
+    WB = pre + 4 * (nz + 1) * ps;
+    cs = ps;
+  } else {
+    AA = scr + c0; BB = scr + fw + c0; RR = scr + 2 * fw + c0; DD = scr + 3 * fw + c0; WB = wb;
+    cs = sk;
+  }
   float cAp = 0.0f, cBp = 0.0f;
   for (int k = 1; k <= nz - 1; ++k) {
-    const float aa = AA[k * sk], bb = BB[k * sk], cc = RR[k * sk], dd = DD[k * sk];
+    const float aa = AA[k * cs], bb = BB[k * cs], cc = RR[k * cs], dd = DD[k * cs];
     const float rr = 1.0f / (k == 1 ? bb : fmaf_(aa, cAp, bb));
     cAp = (-cc) * rr;
     cBp = (k == 1 ? dd : fmaf_(-aa, cBp, dd)) * rr;
-    RR[k * sk] = rr;
+    RR[k * cs] = rr;
     tA[k * ts] = cAp;
     tB[k * ts] = cBp;
   }
@@ -1115,18 +1125,18 @@ TE_HD void w_solve_sweep(const float* M, float* Mb, const AdjArg& a, float* scr,
     S2[k * sk] = wn;
   }
   // U^T z = wbar (z in tB), levels 0 and nz are constant outputs
-  float z = wb[sk];
+  float z = WB[cs];
   tB[ts] = z;
   for (int k = 2; k <= nz - 1; ++k) {
-    z = fmaf_(tA[(k - 1) * ts], z, wb[k * sk]);
+    z = fmaf_(tA[(k - 1) * ts], z, WB[k * cs]);
     tB[k * ts] = z;
   }
   // L^T dbar = z, from the top
   float aNext = 0.0f, yNext = 0.0f;
   for (int k = nz - 1; k >= 1; --k) {
-    const float y = (k == nz - 1 ? tB[k * ts] : fmaf_(-aNext, yNext, tB[k * ts])) * RR[k * sk];
+    const float y = (k == nz - 1 ? tB[k * ts] : fmaf_(-aNext, yNext, tB[k * ts])) * RR[k * cs];
     S1[k * sk] = y;
-    aNext = AA[k * sk];
+    aNext = AA[k * cs];
     yNext = y;
   }
   for (int k = 0; k <= nz; ++k) wb[k * sk] = 0.0f;

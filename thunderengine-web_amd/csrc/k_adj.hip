@@ -45,6 +45,35 @@ __global__ __launch_bounds__(64) void k_adj_w_sweep(const float* M, float* Mb, A
   const uint64_t y = t / n0, x = t - y * n0;
   w_solve_sweep(M, Mb, a, scr, fw, te_col + threadIdx.x, te_col + 64 * nz1 + threadIdx.x, 64, (int)x, (int)y);
 }
+// The same with the column's coefficients and incoming adjoint preloaded into
+// LDS by independent loads (32 columns per block: 7 arrays of nz + 1 floats
+// per lane), so the serial phases wait on LDS, not on memory.
+__global__ __launch_bounds__(32) void k_adj_w_sweep_pre(const float* M, float* Mb, AdjArg a, float* scr, uint64_t fw,
+                                                        uint32_t nz1) {
+  extern __shared__ float te_col[];
+  const uint64_t t = (uint64_t)blockIdx.x * 32u + threadIdx.x;
+  const uint64_t n0 = a.n[0];
+  if (t >= n0 * a.n[1]) return;
+  const uint64_t y = t / n0, x = t - y * n0;
+  float* tA = te_col + threadIdx.x;
+  float* tB = te_col + 32 * nz1 + threadIdx.x;
+  float* pre = te_col + 2 * 32 * nz1 + threadIdx.x;
+  Acc r(M, Mb, &a);
+  const ADims g = r.dims();
+  if ((int)x < g.nx && (int)y < g.ny) {
+    const int64_t sk = g.sk(), c0 = g.idx((int)x, (int)y, 0);
+    const float* wb = Mb + a.p[9] + c0;
+#pragma unroll 4
+    for (uint32_t k = 0; k < nz1; ++k) {
+      pre[(0 * nz1 + k) * 32] = scr[c0 + k * sk];
+      pre[(1 * nz1 + k) * 32] = scr[fw + c0 + k * sk];
+      pre[(2 * nz1 + k) * 32] = scr[2 * fw + c0 + k * sk];
+      pre[(3 * nz1 + k) * 32] = scr[3 * fw + c0 + k * sk];
+      pre[(4 * nz1 + k) * 32] = wb[k * sk];
+    }
+  }
+  w_solve_sweep(M, Mb, a, scr, fw, tA, tB, 32, (int)x, (int)y, pre, 32);
+}
 __global__ __launch_bounds__(256) void k_adj_w_level(const float* M, float* Mb, AdjArg a, uint32_t nz) {
   const uint64_t t = (uint64_t)blockIdx.x * 256u + threadIdx.x;
   const uint64_t n0 = a.n[0], n01 = n0 * a.n[1];
@@ -229,10 +258,15 @@ hipError_t launch(int id, const uint32_t* A, const float* const* vals, const flo
                                    dim3(256), cargs, 0, stream);
     if (r != hipSuccess) return r;
     uint32_t nz1 = nzu + 1;
-    const size_t lds = (size_t)2 * 64 * nz1 * sizeof(float);
-    if (lds > 65536) return hipErrorInvalidConfiguration;  // nz <= 127
     void* sargs[] = {(void*)&M, (void*)&Mb, (void*)&a, (void*)&scr, (void*)&fw, (void*)&nz1};
-    r = hipLaunchKernel(e.fn, dim3((unsigned)((cols + 63) / 64)), dim3(64), sargs, lds, stream);
+    const size_t lds_pre = (size_t)7 * 32 * nz1 * sizeof(float), lds = (size_t)2 * 64 * nz1 * sizeof(float);
+    if (lds_pre <= 65536) {
+      r = hipLaunchKernel(reinterpret_cast<const void*>(&k_adj_w_sweep_pre), dim3((unsigned)((cols + 31) / 32)), dim3(32),
+                          sargs, lds_pre, stream);
+    } else {
+      if (lds > 65536) return hipErrorInvalidConfiguration;  // nz <= 127
+      r = hipLaunchKernel(e.fn, dim3((unsigned)((cols + 63) / 64)), dim3(64), sargs, lds, stream);
+    }
     if (r != hipSuccess) return r;
     void* largs[] = {(void*)&M, (void*)&Mb, (void*)&a, (void*)&nzu};
     r = hipLaunchKernel(reinterpret_cast<const void*>(&k_adj_w_level), dim3((unsigned)((pts + 255) / 256)), dim3(256),
