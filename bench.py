@@ -262,6 +262,8 @@ def main():
                     help="skip the 32-member uneven ensemble (members stepped together)")
     ap.add_argument("--no-adjoint", action="store_true",
                     help="skip the adjoint cost (128x128x60, 60 steps, checkpoint every 10)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default launch shapes (no te_autotune before the timed steps)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -273,6 +275,12 @@ def main():
     run = te_host.Run(cfg)
     run.init_state()
     pts = cfg.nx * cfg.ny * cfg.nz
+    tuned, tune_s, tune_ms = 0, 0.0, (0.0, 0.0)
+    if not args.no_autotune:  # untimed: shapes chosen on the t=0 state, memory restored
+        t0 = time.perf_counter()
+        tuned = run.autotune(reps=5)
+        tune_s = time.perf_counter() - t0
+        tune_ms = run.autotune_ms()
     if args.warmup > 0:
         run.step(args.warmup)
     run.sync()
@@ -318,7 +326,8 @@ def main():
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": "supercell hour 192x192x80 dz=250m dt=6s nsound=6 TKE (synthetic WK82 + bubble)",
                    "nx": cfg.nx, "ny": cfg.ny, "nz": cfg.nz, "substeps_per_step": 11, "ops_per_step": len(names),
-                   "parallelism": "single GPU"},
+                   "parallelism": "single GPU", "autotuned_ops": tuned, "autotune_s": round(tune_s, 2),
+                   "autotune_eager_ms": [round(tune_ms[0], 4), round(tune_ms[1], 4)]},
         "step_hbm": {"alg_bytes_per_pt_step": B_ALG_STEP, "achieved_GBs": round(step_gbs, 1),
                      "frac": round(step_gbs / PEAK_HBM_GBS, 4),
                      "launch_alg_bytes_per_pt_step": round(kernels_alg_pt, 1)},

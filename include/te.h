@@ -83,6 +83,21 @@ const char* te_kernel_name(int32_t i);
 /* Device duration of the last te_run in milliseconds (HIP events on the
  * context's stream). */
 int te_last_run_ms(te_ctx* ctx, float* ms);
+/* Launch-shape autotuner.  No counterpart in the reference (WebGPU fixes the
+ * workgroup size per pipeline).  For every dispatch on the generic 3-D path
+ * (not the column sweeps, fused advection or flat copies) it times candidate
+ * block shapes and the (i, j)-plane numbering inside `reps` whole eager steps
+ * (an event after each launch, every trial from the same memory), keeps each
+ * dispatch's fastest, and then keeps that set only if `reps` whole steps run
+ * at least 0.5% faster with it than with the default shapes (three
+ * interleaved comparisons), or always when `force` is nonzero.  The memory is restored and a captured step
+ * graph captured again.  A shape only changes which thread computes which
+ * point, never the arithmetic.  te_tuned_ops: dispatches left on a tuned
+ * shape (0 when the comparison rejected the set); te_autotune_ms: mean eager
+ * step time of the comparison, default and tuned shapes. */
+int te_autotune(te_ctx* ctx, int32_t reps, int32_t force);
+int32_t te_tuned_ops(const te_ctx* ctx);
+int te_autotune_ms(const te_ctx* ctx, float* untuned, float* tuned);
 
 const char* te_last_error(const te_ctx* ctx);
 void te_destroy(te_ctx* ctx);

@@ -83,6 +83,35 @@ def test_config_bit_exact(gpu, name, steps, fuse):
     run.close()
 
 
+@pytest.mark.parametrize("name", ["species7_ragged", "tke_rayleigh"])
+@pytest.mark.parametrize("fuse", [1, 0])
+def test_autotuned_launch_shapes_bit_exact(gpu, name, fuse):
+    """te_autotune leaves the memory as it found it, and the program with the
+    shapes it chose still matches the oracle word for word."""
+    cfg = te_host.config(fuse=fuse, **CONFIGS[name])
+    say("autotune", name, "fuse", fuse)
+    run = te_host.Run(cfg)
+    run.init_state()
+    run.step(1)  # a graph exists: the tuner must capture it again
+    before = run.read(0, run.mem_words)
+    tuned = run.autotune(reps=2, force=True)
+    say("tuned dispatches", tuned, "eager step ms default / tuned", run.autotune_ms())
+    assert tuned > 0
+    assert np.array_equal(run.read(0, run.mem_words), before)
+    run.step(2)
+    run.sync()
+    ref = runs.OracleRun(cfg)
+    ref.step(3)
+    end = runs.arena_end(run) if fuse else run.mem_words
+    got, want = run.read(0, end), ref.M[:end].copy()
+    if fuse:
+        keep = runs.state_mask(run)
+        got[~keep] = 0
+        want[~keep] = 0
+    assert np.array_equal(got, want), runs.first_difference(run, got, want)
+    run.close()
+
+
 def test_ensemble_members_match_single_runs(gpu):
     """Members of uneven grids and perturbed soundings stepped together on
     their own streams equal the same configurations stepped alone, and one

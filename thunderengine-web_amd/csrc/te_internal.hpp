@@ -20,6 +20,9 @@ struct Op {
   int32_t nz = 0;  // levels of the op's grid (set before the first launch)
   bool flat = false;  // launched flat over its words (set with nz)
   bool plane = false;  // launched over its (i, j) plane's points (set with nz)
+  // launch shape chosen by te_autotune (generic 3-D kernels only)
+  bool tuned = false, tplane = false;
+  uint16_t tb[3] = {0, 0, 0};
 };
 
 int kernel_id(const char* name);
@@ -73,6 +76,8 @@ struct te_ctx {
   std::vector<std::string> op_names;
   hipGraphExec_t graph = nullptr;
   bool graph_failed = false;
+  int32_t tuned_ops = 0;  // ops whose launch shape te_autotune chose
+  float tune_base_ms = 0.0f, tune_best_ms = 0.0f;  // its eager step times, untuned / tuned
   std::string err;
   // config contexts
   bool has_config = false;

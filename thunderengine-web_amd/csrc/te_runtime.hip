@@ -142,7 +142,8 @@ static hipError_t launch(te_ctx* c, const Op& op) {
       return hipLaunchKernel(kKernels[op.kid].fn, g, b, args, lds, c->stream);
     }
   }
-  if (op.plane) {  // (i, j) points numbered consecutively, level on grid y (te_tid3)
+  if (op.tuned && !op.tplane) b = dim3(op.tb[0], op.tb[1], op.tb[2]);
+  if (op.tuned ? op.tplane : op.plane) {  // (i, j) points numbered consecutively, level on grid y (te_tid3)
     const uint64_t pts = (uint64_t)a.n[0] * a.n[1];
     b = dim3(256, 1, 1);
     const dim3 g((uint32_t)((pts + 255) / 256), a.n[2], 1);
@@ -264,6 +265,16 @@ static int ctx_alloc(te_ctx* c, uint64_t words) {
 // memory.  Before the first run the grid of every op is read back and the
 // allocation grown so that footprint past the end of the program's memory is
 // still allocated: no launch of a validated program can leave M.
+// kernels that map threads themselves (columns, fused advection, copies)
+static const std::set<std::string>& own_mapping() {
+  static const std::set<std::string> s = {
+      "acoustic_w_solve", "acoustic_w_solve_lds", "adv_fused", "_copy", "copy_fields", "turb_sfcstress",
+      "turb_u_tendency", "turb_u_tendency_lds", "turb_v_tendency", "turb_v_tendency_lds", "turb_w_tendency",
+      "turb_w_tendency_lds", "turb_scalar_vert_implicit", "turb_scalar_vert_implicit_lds", "tke_transport",
+      "tke_transport_lds"};
+  return s;
+}
+
 static int ensure_guard(te_ctx* c) {
   if (c->guard_ok) return TE_OK;
   uint64_t need = 0;
@@ -311,12 +322,7 @@ static int ensure_guard(te_ctx* c) {
       const bool words1d = std::strcmp(kn, "copy_fields") == 0 && op.A[1] == 1 && op.A[2] == 1;
       op.flat = (wholeGrid || words1d) && aligned(op, fl->second);
     }
-    // kernels that map threads themselves (columns, fused advection, copies)
-    static const std::set<std::string> ownMapping = {
-        "acoustic_w_solve", "acoustic_w_solve_lds", "adv_fused", "_copy", "copy_fields", "turb_sfcstress",
-        "turb_u_tendency", "turb_u_tendency_lds", "turb_v_tendency", "turb_v_tendency_lds", "turb_w_tendency",
-        "turb_w_tendency_lds", "turb_scalar_vert_implicit", "turb_scalar_vert_implicit_lds", "tke_transport",
-        "tke_transport_lds"};
+    const std::set<std::string>& ownMapping = own_mapping();
     // (te_div is exact for numerators below 2^24, divisors up to 2^16)
     op.plane = !op.flat && !ownMapping.count(kn) && op.A[0] % 64 != 0 && op.A[1] > 1 && op.A[0] <= (1u << 16) &&
                (uint64_t)op.A[0] * op.A[1] + 256 <= (1u << 24);
@@ -494,6 +500,149 @@ int te_last_run_ms(te_ctx* c, float* ms) {
   *ms = c->last_ms;
   return TE_OK;
 }
+
+int te_autotune(te_ctx* c, int32_t reps, int32_t force) {
+  if (!c || !c->dM) return fail(c, TE_ERR_STATE, "context has no memory");
+  if (reps < 1) return fail(c, TE_ERR_ARG, "reps must be at least 1");
+  const int n = (int)c->ops.size();
+  for (int i = 0; i < n; ++i)
+    if (c->ops[i].kid < 0) return fail(c, TE_ERR_UNSUPPORTED, "kernel '" + c->op_names[i] + "' is not on this path");
+  int rc = ensure_guard(c);
+  if (rc != TE_OK) return rc;
+  // the trial steps change the memory: every trial starts from a copy of it,
+  // and the copy is put back at the end
+  struct Hold {
+    uint32_t* p = nullptr;
+    std::vector<hipEvent_t> ev;
+    ~Hold() {
+      if (p) (void)hipFree(p);
+      for (hipEvent_t e : ev)
+        if (e) (void)hipEventDestroy(e);
+    }
+  } hold;
+  const uint64_t bytes = c->prog_words * 4;
+  TE_HIP(hipMalloc((void**)&hold.p, bytes), "hipMalloc(autotune copy)");
+  TE_HIP(hipMemcpyAsync(hold.p, c->dM, bytes, hipMemcpyDeviceToDevice, c->stream), "autotune copy");
+  hold.ev.assign(n + 1, nullptr);
+  for (hipEvent_t& e : hold.ev) TE_HIP(hipEventCreate(&e), "hipEventCreate");
+  auto restore = [&]() { return hipMemcpyAsync(c->dM, hold.p, bytes, hipMemcpyDeviceToDevice, c->stream); };
+
+  // dispatches on the generic 3-D path: not the column sweeps, the fused
+  // advection or the flat copies, which fix their own thread mapping
+  std::vector<char> eligible(n, 0), planeOk(n, 0);
+  for (int i = 0; i < n; ++i) {
+    const te::Op& op = c->ops[i];
+    const char* kn = te::kernel_name(op.kid);
+    if (op.flat || own_mapping().count(kn) || (op.A[2] == 1 && te::is_column_kernel(op.kid))) continue;
+    if (op.A[0] == 0 || op.A[1] == 0 || op.A[2] == 0) continue;
+    eligible[i] = 1;
+    planeOk[i] = op.A[1] > 1 && op.A[0] <= (1u << 16) && (uint64_t)op.A[0] * op.A[1] + 256 <= (1u << 24);
+  }
+  struct Shape { bool plane; uint16_t x, y, z; };
+  static const Shape kShapes[] = {{false, 64, 4, 1}, {false, 64, 2, 2}, {false, 64, 1, 4}, {false, 64, 2, 1},
+                                  {false, 64, 1, 2}, {false, 64, 1, 1}, {false, 128, 2, 1}, {false, 128, 1, 2},
+                                  {false, 128, 1, 1}, {false, 256, 1, 1}, {false, 32, 8, 1}, {false, 32, 4, 2},
+                                  {true, 256, 1, 1}};
+  constexpr int kCand = (int)(sizeof(kShapes) / sizeof(kShapes[0]));
+  auto p2 = [](uint32_t v) { uint32_t r = 1; while (r < v) r <<= 1; return r; };
+  auto fits = [&](int i, const Shape& sh) {
+    const te::Op& op = c->ops[i];
+    if (sh.plane) return planeOk[i] != 0;
+    return sh.x <= std::max<uint32_t>(64, p2(op.A[0])) && sh.y <= p2(op.A[1]) && sh.z <= p2(op.A[2]);
+  };
+  const std::vector<te::Op> base = c->ops;  // shapes as the context had them
+  std::vector<te::Op> best = base;
+  std::vector<float> bestMs(n, 1e30f), ms(n);
+
+  // each candidate in context: `reps` eager steps from the same memory with
+  // an event after every launch, so each dispatch is timed with the caches
+  // as the step leaves them (candidate -1: the untuned shapes)
+  for (int cand = -1; cand < kCand; ++cand) {
+    for (int i = 0; i < n; ++i) {
+      if (!eligible[i]) continue;
+      te::Op& op = c->ops[i];
+      op = base[i];
+      op.tuned = false;
+      if (cand >= 0 && fits(i, kShapes[cand])) {
+        op.tuned = true;
+        op.tplane = kShapes[cand].plane;
+        op.tb[0] = kShapes[cand].x; op.tb[1] = kShapes[cand].y; op.tb[2] = kShapes[cand].z;
+      }
+    }
+    TE_HIP(restore(), "autotune restore");
+    std::fill(ms.begin(), ms.end(), 0.0f);
+    for (int s = 0; s < reps; ++s) {
+      TE_HIP(hipEventRecord(hold.ev[0], c->stream), "event");
+      for (int i = 0; i < n; ++i) {
+        TE_HIP(te::launch(c, c->ops[i]), "autotune launch");
+        TE_HIP(hipEventRecord(hold.ev[i + 1], c->stream), "event");
+      }
+      TE_HIP(hipStreamSynchronize(c->stream), "autotune sync");
+      for (int i = 0; i < n; ++i) {
+        float t = 0.0f;
+        TE_HIP(hipEventElapsedTime(&t, hold.ev[i], hold.ev[i + 1]), "elapsed");
+        ms[i] += t;
+      }
+    }
+    for (int i = 0; i < n; ++i)
+      if (eligible[i] && (cand < 0 || c->ops[i].tuned) && ms[i] < bestMs[i]) {
+        bestMs[i] = ms[i];
+        best[i] = c->ops[i];
+      }
+  }
+
+  // keep the per-dispatch choices only if whole steps are faster with them
+  auto steps_ms = [&](const std::vector<te::Op>& ops, float* out) -> int {
+    c->ops = ops;
+    TE_HIP(restore(), "autotune restore");
+    TE_HIP(hipEventRecord(hold.ev[0], c->stream), "event");
+    for (int s = 0; s < reps; ++s) {
+      int r = run_ops(c, 0, n - 1);
+      if (r != TE_OK) return r;
+    }
+    TE_HIP(hipEventRecord(hold.ev[1], c->stream), "event");
+    TE_HIP(hipEventSynchronize(hold.ev[1]), "event sync");
+    TE_HIP(hipEventElapsedTime(out, hold.ev[0], hold.ev[1]), "elapsed");
+    return TE_OK;
+  };
+  float tBase = 0.0f, tBest = 0.0f;
+  for (int rep = 0; rep < 3 && rc == TE_OK; ++rep) {  // interleaved A/B
+    float t = 0.0f;
+    rc = steps_ms(base, &t);
+    tBase += t;
+    if (rc == TE_OK) rc = steps_ms(best, &t);
+    tBest += t;
+  }
+  if (rc != TE_OK) {
+    c->ops = base;
+    return rc;
+  }
+  const bool keepTuned = force != 0 || tBest < 0.995f * tBase;
+  c->ops = keepTuned ? best : base;
+  int tuned = 0;
+  for (int i = 0; i < n; ++i) tuned += c->ops[i].tuned ? 1 : 0;
+  TE_HIP(restore(), "autotune restore");
+  TE_HIP(hipStreamSynchronize(c->stream), "autotune sync");
+  c->tune_base_ms = tBase / (3.0f * reps);
+  c->tune_best_ms = tBest / (3.0f * reps);
+  const bool hadGraph = c->graph != nullptr;
+  if (hadGraph) {  // the step graph holds the old shapes: capture it again
+    (void)hipGraphExecDestroy(c->graph);
+    c->graph = nullptr;
+  }
+  c->graph_failed = false;
+  c->tuned_ops = tuned;
+  return hadGraph ? ensure_graph(c) : TE_OK;
+}
+
+int te_autotune_ms(const te_ctx* c, float* untuned, float* tuned) {
+  if (!c || !untuned || !tuned) return TE_ERR_ARG;
+  *untuned = c->tune_base_ms;
+  *tuned = c->tune_best_ms;
+  return TE_OK;
+}
+
+int32_t te_tuned_ops(const te_ctx* c) { return c ? c->tuned_ops : 0; }
 
 int te_region_hash(te_ctx* c, uint64_t off, uint64_t n, uint32_t* out) {
   if (!c || !out || off + n > c->words) return fail(c, TE_ERR_ARG, "hash outside memory");

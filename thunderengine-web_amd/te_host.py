@@ -81,6 +81,9 @@ def lib():
         "te_op_get": ([ctx, ctypes.c_int32, P(ctypes.c_char_p), u32p], ctypes.c_int),
         "te_last_run_ms": ([ctx, P(ctypes.c_float)], ctypes.c_int),
         "te_last_error": ([ctx], ctypes.c_char_p),
+        "te_autotune": ([ctx, ctypes.c_int32, ctypes.c_int32], ctypes.c_int),
+        "te_tuned_ops": ([ctx], ctypes.c_int32),
+        "te_autotune_ms": ([ctx, P(ctypes.c_float), P(ctypes.c_float)], ctypes.c_int),
         "te_kernel_count": ([], ctypes.c_int32),
         "te_kernel_name": ([ctypes.c_int32], ctypes.c_char_p),
         "te_destroy": ([ctx], None),
@@ -216,6 +219,20 @@ class _Ctx:
         self._check(lib().te_last_run_ms(self.h, ctypes.byref(ms)), "te_last_run_ms")
         return ms.value
 
+    def autotune(self, reps=5, force=False):
+        """Chooses each generic dispatch's launch shape by timing candidates
+        inside whole steps (te_autotune); the memory is left as it was.
+        Returns the number of dispatches left on a tuned shape (force: keep
+        the choices even when whole steps are not faster with them)."""
+        self._check(lib().te_autotune(self.h, reps, int(force)), "te_autotune")
+        return int(lib().te_tuned_ops(self.h))
+
+    def autotune_ms(self):
+        """(default, tuned) mean eager step ms of the tuner's final comparison."""
+        a, b = ctypes.c_float(), ctypes.c_float()
+        self._check(lib().te_autotune_ms(self.h, ctypes.byref(a), ctypes.byref(b)), "te_autotune_ms")
+        return a.value, b.value
+
     def program(self):
         """(names, (nops, 64) uint32 argument slots) of the context's program."""
         n = int(lib().te_op_count(self.h))
@@ -306,6 +323,16 @@ class Ensemble:
     def sync(self):
         if lib().te_ensemble_sync(self.h) != TE_OK:
             raise TeError("te_ensemble_sync failed")
+
+    def autotune(self, reps=5):
+        """te_autotune on every member (each keeps its own shapes)."""
+        n = 0
+        for m in range(len(self)):
+            c = lib().te_ensemble_member(self.h, m)
+            if lib().te_autotune(c, reps, 0) != TE_OK:
+                raise TeError("te_autotune failed on member %d: %s" % (m, lib().te_last_error(c).decode()))
+            n += int(lib().te_tuned_ops(c))
+        return n
 
     @property
     def points(self):
