@@ -164,16 +164,17 @@ def ensemble_configs(n=32, seed=20260102):
     return cfgs
 
 
-def ensemble_bench(steps, warmup):
+def ensemble_bench(steps, warmup, tune=True):
     cfgs = ensemble_configs()
     ens = te_host.Ensemble(cfgs)
+    tuned = ens.autotune() if tune else 0  # per member, untimed
     ens.step(warmup)
     ens.sync()
     t0 = time.perf_counter()
     ens.step(steps)
     ens.sync()
     wall = time.perf_counter() - t0
-    out = {"members": len(cfgs), "points": ens.points, "steps": steps,
+    out = {"members": len(cfgs), "points": ens.points, "steps": steps, "autotuned_ops": tuned,
            "value": round(ens.points * steps / wall / 1e6, 2), "unit": "Mpt-steps/s",
            "grids": sorted({"%dx%dx%d" % (c.nx, c.ny, c.nz) for c in cfgs})}
     ens.close()
@@ -338,7 +339,7 @@ def main():
     }
     run.close()
     if not args.no_ensemble:
-        ens = ensemble_bench(max(5, args.steps // 5), 2)
+        ens = ensemble_bench(max(5, args.steps // 5), 2, tune=not args.no_autotune)
         ens["per_point_vs_plain"] = round(ens["value"] / value, 3)
         out["ensemble"] = ens
     if not args.no_adjoint:

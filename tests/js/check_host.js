@@ -3,7 +3,8 @@
 //   node check_host.js plan            -> tape plan summary (CPU)
 //   node check_host.js encode '<json>' -> program summary of a config (CPU)
 //   node check_host.js replay <out>    -> supercell tape replayed on the GPU with hashes
-//   node check_host.js run '<json>' n  -> field FNV-1a after n steps on the GPU
+//   node check_host.js run '<json>' n [tune] -> field FNV-1a after n steps on the GPU
+//        (tune: launch shapes chosen by run.autotune first, kept regardless)
 //   node check_host.js adjoint '<json>' n <out> -> gradient of sum of w^2 over a
 //        box after n checkpointed steps w.r.t. the initial th, u, v (GPU)
 const path = require('path');
@@ -54,6 +55,7 @@ if (mode === 'plan') {
   const n = Number(process.argv[4]);
   const r = te.buildRun(conf);
   r.fill();
+  if (process.argv[5] === 'tune' && !(r.autotune(2, true) > 0)) throw new Error('autotune tuned no dispatch');
   r.step(n);
   const out = {};
   const L = r.layout;

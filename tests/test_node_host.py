@@ -94,9 +94,11 @@ def test_replay_supercell_tape_on_gpu(gpu, tmp_path):
 
 
 @pytest.mark.gpu
-def test_js_run_bit_exact_vs_oracle(gpu):
+@pytest.mark.parametrize("tune", ["", "tune"])
+def test_js_run_bit_exact_vs_oracle(gpu, tune):
+    """tune: run.autotune's launch shapes (forced) leave the fields as the oracle's."""
     conf = dict(nx=32, ny=24, nz=16, tke=1, initial_tke=3, rayleigh=1)
-    got = json.loads(node("run", json.dumps(conf), 2))
+    got = json.loads(node("run", json.dumps(conf), 2, tune))
     ref = runs.OracleRun(te_host.config(**conf))
     ref.step(2)
     e = ref.enc

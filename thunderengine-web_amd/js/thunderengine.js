@@ -203,6 +203,8 @@ function buildRun(conf, options) {
   run.encodeStep = () => { addon().step(ctx, 1); return run; };
   run.step = n => { addon().step(ctx, n == null ? 1 : n); addon().sync(ctx); return run; };
   run.lastStepMs = () => addon().lastRunMs(ctx);
+  // launch shapes chosen by measurement (te_autotune); results are unchanged
+  run.autotune = (reps, force) => addon().autotune(ctx, reps == null ? 5 : reps, force ? 1 : 0);
   run.read = (words, offWords) => addon().download(ctx, offWords, words);
   run.fnv1a = (offWords, words) => addon().fnv1a(ctx, offWords, words);
   run.regionHash = (offWords, words) => addon().regionHash(ctx, offWords, words);

@@ -345,6 +345,17 @@ napi_value LastRunMs(napi_env env, napi_callback_info info) {
   return make_num(env, ms);
 }
 
+// autotune(ctx, reps, force) -> number of dispatches left on a tuned shape
+napi_value Autotune(napi_env env, napi_callback_info info) {
+  napi_value argv[3];
+  if (!get_args(env, info, 3, argv)) return nullptr;
+  te_ctx* c = ctx_of(env, argv[0]);
+  if (!c) return nullptr;
+  int rc = te_autotune(c, (int32_t)num(env, argv[1]), (int32_t)num(env, argv[2]));
+  if (rc != TE_OK) return throw_status(env, c, rc, "te_autotune");
+  return make_num(env, te_tuned_ops(c));
+}
+
 napi_value HashCall(napi_env env, napi_callback_info info, bool fnv) {
   napi_value argv[3];
   if (!get_args(env, info, 3, argv)) return nullptr;
@@ -580,7 +591,7 @@ napi_value Init(napi_env env, napi_value exports) {
   struct { const char* name; napi_callback fn; } fns[] = {
       {"configDefaults", ConfigDefaults}, {"create", Create}, {"programCreate", ProgramCreate},
       {"upload", Upload}, {"download", Download}, {"run", Run}, {"step", Step}, {"sync", Sync},
-      {"initState", InitState}, {"lastRunMs", LastRunMs}, {"regionHash", RegionHash}, {"fnv1a", Fnv1a},
+      {"initState", InitState}, {"lastRunMs", LastRunMs}, {"autotune", Autotune}, {"regionHash", RegionHash}, {"fnv1a", Fnv1a},
       {"memoryWords", MemoryWords}, {"program", Program}, {"layout", Layout}, {"kernelNames", KernelNames},
       {"profileOps", ProfileOps}, {"destroy", Destroy},
       {"adjointCreate", AdjointCreate}, {"fwdCkpt", FwdCkpt}, {"bwd", Bwd}, {"adjZero", AdjZero},
