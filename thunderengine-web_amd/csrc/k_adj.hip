@@ -121,7 +121,8 @@ uint64_t gather_lead(int nx, int ny, int nz, int sJ, int sK) {
 // its image.  One thread per halo word (7 halo rows of strideJ words and
 // 7 halo columns of ny words per level); several halo words share an image,
 // so the adds are atomic.
-__global__ __launch_bounds__(256) void k_adj_halo_xy(float* Mb, uint64_t off, int nx, int ny, int nz1, int sJ, int sK) {
+__global__ __launch_bounds__(256) void k_adj_halo_xy(float* Mb, HaloArg fields, int nx, int ny, int nz1, int sJ, int sK) {
+  const uint64_t off = fields.off[blockIdx.y];  // one field per grid row
   const uint64_t t = (uint64_t)blockIdx.x * 256u + threadIdx.x;
   const uint64_t ring = (uint64_t)7 * sJ + (uint64_t)7 * ny;
   if (t >= ring * nz1) return;
@@ -146,12 +147,16 @@ __global__ __launch_bounds__(256) void k_adj_halo_xy(float* Mb, uint64_t off, in
   adj_add(f + (ii + 3) + (int64_t)sJ * (jj + 3), hb);
 }
 
-hipError_t halo_xy(float* Mb, uint64_t off, int nx, int ny, int nz, int sJ, int sK, hipStream_t stream) {
+hipError_t halo_xy(float* Mb, const HaloArg& fields, int nf, int nx, int ny, int nz, int sJ, int sK,
+                   hipStream_t stream) {
+  if (nf <= 0) return hipSuccess;
+  if (nf > kHaloFields) return hipErrorInvalidValue;
   int nz1 = nz + 1;
   const uint64_t n = ((uint64_t)7 * sJ + (uint64_t)7 * ny) * nz1;
-  void* args[] = {(void*)&Mb, (void*)&off, (void*)&nx, (void*)&ny, (void*)&nz1, (void*)&sJ, (void*)&sK};
-  return hipLaunchKernel(reinterpret_cast<const void*>(&k_adj_halo_xy), dim3((unsigned)((n + 255) / 256)), dim3(256),
-                         args, 0, stream);
+  HaloArg h = fields;
+  void* args[] = {(void*)&Mb, (void*)&h, (void*)&nx, (void*)&ny, (void*)&nz1, (void*)&sJ, (void*)&sK};
+  return hipLaunchKernel(reinterpret_cast<const void*>(&k_adj_halo_xy), dim3((unsigned)((n + 255) / 256), (unsigned)nf),
+                         dim3(256), args, 0, stream);
 }
 
 struct GatherEntry {

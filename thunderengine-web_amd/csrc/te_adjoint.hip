@@ -73,33 +73,61 @@ static int run_rec_ops(te_adjoint* a, int first, int last) {
   return TE_OK;
 }
 
+// Adjoint of the periodic halos (x then y) of the fields of a run of halo
+// units, up to kHaloFields fields per launch (a field listed twice is done
+// once: the second forward fill left the halos as the first did).
+static int halo_units(te_adjoint* a, const Unit* us, int nu) {
+  te_ctx* r = a->rec;
+  te::adj::HaloArg h{};
+  int nf = 0;
+  for (int u = 0; u <= nu; ++u) {
+    if (u == nu || nf == te::adj::kHaloFields) {
+      TA_HIP(te::adj::halo_xy(a->dMb, h, nf, r->layout.nx, r->layout.ny, r->layout.nz, r->layout.sJ, r->layout.sK,
+                              r->stream),
+             "adjoint of the periodic halos");
+      nf = 0;
+      if (u == nu) break;
+    }
+    bool dup = false;
+    for (int t = 0; t < nf; ++t) dup = dup || h.off[t] == us[u].field;
+    if (!dup) h.off[nf++] = us[u].field;
+  }
+  return TE_OK;
+}
+
 // Logged forward of `steps` steps on rec's memory (the write log of each
 // dispatch appended at log position `pos`), then their reverse.
 static int reverse_steps(te_adjoint* a, int steps) {
   te_ctx* r = a->rec;
   const int n = (int)r->ops.size();
   std::vector<uint64_t> at((size_t)n * steps, 0);
-  // save (restore) the regions of dispatch i at log position p: one launch
-  auto copy = [&](int i, uint64_t p, bool save) -> hipError_t {
-    const auto& w = a->wreg[i];
-    te::adj::CopyArg c{};
-    c.n = w.empty() ? 0 : w[0].second;
-    for (size_t q = 0; q < w.size(); ++q) {
-      uint32_t* m = r->dM + w[q].first;
-      uint32_t* l = a->dLog + p + q * a->stride[i];
-      c.src[q] = save ? m : l;
-      c.dst[q] = save ? l : m;
-    }
-    return te::adj::copy_regions(c, (int)w.size(), r->stream);
-  };
+  // Logged forward.  The regions a dispatch writes are saved to its log slot
+  // before it runs; a run of dispatches none of which writes a region another
+  // of them writes saves all their regions in one launch before the run (a
+  // region's values before the run are its values before its own writer).
   uint64_t pos = 0;
   for (int s = 0; s < steps; ++s)
-    for (int i = 0; i < n; ++i) {
-      at[(size_t)s * n + i] = pos;
-      TA_HIP(copy(i, pos, true), "log");
-      pos += a->wreg[i].size() * a->stride[i];
-      int rc = run_rec_ops(a, i, i);
+    for (int i = 0; i < n;) {
+      te::adj::CopyArg c{};
+      int nreg = 0, e = i;
+      for (; e < n; ++e) {
+        const auto& w = a->wreg[e];
+        bool fits = nreg + (int)w.size() <= te::adj::kCopyRegions && (w.empty() || nreg == 0 || w[0].second == c.n);
+        for (size_t q = 0; q < w.size() && fits; ++q)
+          for (int t = 0; t < nreg && fits; ++t) fits = c.src[t] != r->dM + w[q].first;
+        if (!fits) break;
+        at[(size_t)s * n + e] = pos;
+        for (size_t q = 0; q < w.size(); ++q, ++nreg) {
+          c.src[nreg] = r->dM + w[q].first;
+          c.dst[nreg] = a->dLog + pos + q * a->stride[e];
+          c.n = w[q].second;
+        }
+        pos += w.size() * a->stride[e];
+      }
+      TA_HIP(te::adj::copy_regions(c, nreg, r->stream), "log");
+      int rc = run_rec_ops(a, i, e - 1);
       if (rc != TE_OK) return rc;
+      i = e;
     }
   // reverse: a dispatch's inputs are the values before it.  Walking back,
   // each field's value before the dispatch is in the log slot of the first
@@ -115,20 +143,20 @@ static int reverse_steps(te_adjoint* a, int steps) {
         cur[a->wreg[i][q].first] = reinterpret_cast<const float*>(a->dLog + p + q * a->stride[i]);
       const uint32_t* A = r->ops[i].A;
       if (a->halo_pair[i]) {  // with the bc_periodic_x before it, as one map
-        TA_HIP(te::adj::halo_xy(a->dMb, A[4 + 1], r->layout.nx, r->layout.ny, r->layout.nz, r->layout.sJ,
-                                r->layout.sK, r->stream),
-               "adjoint of the periodic halos");
+        Unit h;
+        h.field = A[4 + 1];
+        int rc = halo_units(a, &h, 1);
+        if (rc != TE_OK) return rc;
         continue;
       }
       if (i + 1 < n && a->halo_pair[i + 1]) continue;
       const std::vector<Unit>& us = a->units[i];
+      if (!us.empty() && us[0].adj < 0) {  // bc_periodic_xy: halo units only, fields independent
+        int rc = halo_units(a, us.data(), (int)us.size());
+        if (rc != TE_OK) return rc;
+        continue;
+      }
       for (int u = (int)us.size() - 1; u >= 0; --u) {
-        if (us[u].adj < 0) {
-          TA_HIP(te::adj::halo_xy(a->dMb, us[u].field, r->layout.nx, r->layout.ny, r->layout.nz, r->layout.sJ,
-                                  r->layout.sK, r->stream),
-                 "adjoint of the periodic halos");
-          continue;
-        }
         for (int q = 0; q < 32; ++q) {
           auto it = cur.find(us[u].A[4 + q]);
           vals[q] = it != cur.end() ? it->second : M + us[u].A[4 + q];
@@ -352,6 +380,8 @@ int te_adjoint_create(te_ctx* ctx, int32_t every, te_adjoint** out) {
       for (auto& w : a->wreg[i]) dup = dup || w.first == off;
       if (!dup) a->wreg[i].push_back({off, (nm == "_copy" || nm == "copy_fields") ? (uint64_t)r->ops[i].A[0] : L.field_words});
     }
+    if (a->wreg[i].size() > (size_t)te::adj::kCopyRegions)
+      return afail(a, TE_ERR_UNSUPPORTED, "dispatch " + std::to_string(i) + " (" + nm + ") writes too many fields");
   }
   a->halo_pair.assign(n, 0);
   for (int i = 1; i < n; ++i)
@@ -460,11 +490,10 @@ int te_adj_ops(te_adjoint* a, int32_t first, int32_t last) {
   for (int i = last; i >= first; --i)
     for (int u = (int)a->units[i].size() - 1; u >= 0; --u) {
       const Unit& un = a->units[i][u];
-      if (un.adj < 0)
-        TA_HIP(te::adj::halo_xy(a->dMb, un.field, r->layout.nx, r->layout.ny, r->layout.nz, r->layout.sJ, r->layout.sK,
-                                r->stream),
-               "adjoint of the periodic halos");
-      else
+      if (un.adj < 0) {
+        int rc = halo_units(a, &un, 1);
+        if (rc != TE_OK) return rc;
+      } else
         TA_HIP(te::adj::launch(un.adj, un.A, nullptr, reinterpret_cast<const float*>(r->dM), a->dMb, r->layout.nz,
                                r->layout.sJ, r->layout.sK, a->dScr, r->layout.field_words, r->stream),
                std::string("adjoint of ") + r->op_names[i]);

@@ -18,13 +18,20 @@ int writes(int id, int out[8]);
 // vals: where each of the 32 parameters' values are read (null: M + offset).
 hipError_t launch(int id, const uint32_t* A, const float* const* vals, const float* M, float* Mb, int nz, int sJ, int sK,
                   float* scr, uint64_t fw, hipStream_t stream);
-// Adjoint of bc_periodic_x followed by bc_periodic_y on the field at word
-// `off` of the adjoint memory.
-hipError_t halo_xy(float* Mb, uint64_t off, int nx, int ny, int nz, int sJ, int sK, hipStream_t stream);
-// Copies of up to 8 regions of n words each, in one launch.
+// Adjoint of bc_periodic_x followed by bc_periodic_y on each of the nf
+// (at most kHaloFields) fields at words fields.off[0..nf) of the adjoint
+// memory, in one launch.
+constexpr int kHaloFields = 8;
+struct HaloArg {
+  uint64_t off[kHaloFields];
+};
+hipError_t halo_xy(float* Mb, const HaloArg& fields, int nf, int nx, int ny, int nz, int sJ, int sK,
+                   hipStream_t stream);
+// Copies of up to kCopyRegions regions of n words each, in one launch.
+constexpr int kCopyRegions = 16;
 struct CopyArg {
-  const uint32_t* src[8];
-  uint32_t* dst[8];
+  const uint32_t* src[kCopyRegions];
+  uint32_t* dst[kCopyRegions];
   uint64_t n;
 };
 hipError_t copy_regions(const CopyArg& c, int nreg, hipStream_t stream);
