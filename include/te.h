@@ -85,8 +85,9 @@ const char* te_kernel_name(int32_t i);
 int te_last_run_ms(te_ctx* ctx, float* ms);
 /* Launch-shape autotuner.  No counterpart in the reference (WebGPU fixes the
  * workgroup size per pipeline).  For every dispatch on the generic 3-D path
- * (not the column sweeps, fused advection or flat copies) it times candidate
- * block shapes and the (i, j)-plane numbering inside `reps` whole eager steps
+ * (not the column sweeps or flat copies; fused advection among the shapes
+ * that keep one row of cells per wave) it times candidate block shapes and
+ * the (i, j)-plane numbering inside `reps` whole eager steps
  * (an event after each launch, every trial from the same memory), keeps each
  * dispatch's fastest, and then keeps that set only if `reps` whole steps run
  * at least 0.5% faster with it than with the default shapes (three

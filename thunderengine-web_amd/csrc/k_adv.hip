@@ -457,11 +457,13 @@ struct FusedFlux {
 
 template <int KIND, int STC>
 __device__ __forceinline__ void adv_fused_body(uint32_t* M, const KArg& a) {
-  // launched as 64 x 4 x 1 blocks (runtime launch()): a wave is one row of
-  // cells, so j and k are wave-uniform and branches on them are scalar
+  // launched as 64 x Y x Z blocks (Y * Z <= 4; runtime launch(), te_autotune):
+  // a wave is one row of cells, so j and k are wave-uniform and branches on
+  // them are scalar
   const Blk b_ = te_block(a);
-  const int i = b_.x * 64 + (int)threadIdx.x, j = b_.y * 4 + (int)__builtin_amdgcn_readfirstlane(threadIdx.y),
-            k = b_.z;
+  const int i = b_.x * 64 + (int)threadIdx.x,
+            j = b_.y * TE_BY + (int)__builtin_amdgcn_readfirstlane(threadIdx.y),
+            k = b_.z * TE_BZ + (int)__builtin_amdgcn_readfirstlane(threadIdx.z);
   if (j >= (int)a.n[1] || k >= (int)a.n[2] || i >= (int)a.n[0]) return;
   Dims g = dims(a.q[0], 0);
   const int stag = STC != 0 ? STC : ldi(a.q[1], 0), st = stag < 0 ? -stag : stag;

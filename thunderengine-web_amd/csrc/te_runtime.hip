@@ -529,11 +529,15 @@ int te_autotune(te_ctx* c, int32_t reps, int32_t force) {
 
   // dispatches on the generic 3-D path: not the column sweeps, the fused
   // advection or the flat copies, which fix their own thread mapping
+  // (adv_fused too, among the shapes that keep one row of cells per wave)
+  static const int kAdvFused = te::kernel_id("adv_fused");
   std::vector<char> eligible(n, 0), planeOk(n, 0);
   for (int i = 0; i < n; ++i) {
     const te::Op& op = c->ops[i];
     const char* kn = te::kernel_name(op.kid);
-    if (op.flat || own_mapping().count(kn) || (op.A[2] == 1 && te::is_column_kernel(op.kid))) continue;
+    if (op.flat || (own_mapping().count(kn) && op.kid != kAdvFused) ||
+        (op.A[2] == 1 && te::is_column_kernel(op.kid)))
+      continue;
     if (op.A[0] == 0 || op.A[1] == 0 || op.A[2] == 0) continue;
     eligible[i] = 1;
     planeOk[i] = op.A[1] > 1 && op.A[0] <= (1u << 16) && (uint64_t)op.A[0] * op.A[1] + 256 <= (1u << 24);
@@ -547,6 +551,7 @@ int te_autotune(te_ctx* c, int32_t reps, int32_t force) {
   auto p2 = [](uint32_t v) { uint32_t r = 1; while (r < v) r <<= 1; return r; };
   auto fits = [&](int i, const Shape& sh) {
     const te::Op& op = c->ops[i];
+    if (op.kid == kAdvFused) return !sh.plane && sh.x == 64 && sh.y * sh.z <= 4;
     if (sh.plane) return planeOk[i] != 0;
     return sh.x <= std::max<uint32_t>(64, p2(op.A[0])) && sh.y <= p2(op.A[1]) && sh.z <= p2(op.A[2]);
   };
