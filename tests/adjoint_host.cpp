@@ -19,7 +19,7 @@ template <class B>
 void grid(const float* M, float* Mb, const AdjArg& a) {
   for (uint32_t z = 0; z < a.n[2]; ++z)
     for (uint32_t y = 0; y < a.n[1]; ++y)
-      for (uint32_t x = 0; x < a.n[0]; ++x) adj_point<B>(M, Mb, a, (int)x, (int)y, (int)z);
+      for (uint32_t x = 0; x < a.n[0]; ++x) PointAdj<B>::run(M, Mb, a, (int)x, (int)y, (int)z);
 }
 
 void columns(const float* M, float* Mb, const AdjArg& a) {

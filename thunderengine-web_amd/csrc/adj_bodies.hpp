@@ -918,62 +918,61 @@ TE_HD D1 pdef_mass(D1 dm2, D1 dm1, D1 d0, D1 dp1, D1 dp2, bool& flag) {
   return mass;
 }
 
-// pdef_apply.wgsl:207-282
-struct B_pdef_apply {
+// pdef_apply.wgsl:207-282.  The kernel adds the corrections of the three
+// directions to the tendency (tend + (dx + dy + dz) / rho0), so its adjoint
+// is the identity on tend plus the adjoint of each direction's share: three
+// bodies with about a third of the active loads each.
+template <int D>
+struct B_pdef_part {
   static TE_HD void run(Acc& r, int i, int j, int kk) {
     const ADims g = r.dims();
     if (i >= g.nx || j >= g.ny || kk >= g.nz) return;
     const P s = r.f(1), s0 = r.f(2), fx = r.f(3), fy = r.f(4), fz = r.f(5), rru = r.f(6), rrv = r.f(7),
-                rrw = r.f(8), dx = r.f(9), dy = r.f(10), dz = r.f(11), rho0 = r.f(12), mh = r.f(13),
-                tend = r.f(16), hm = r.f(30);
+            rrw = r.f(8), dx = r.f(9), dy = r.f(10), dz = r.f(11), rho0 = r.f(12), mh = r.f(13), tend = r.f(16),
+            hm = r.f(30);
     const float dt = r.F(14);
     const int openx = r.I(15, 0), openy = r.I(15, 1);
     const int64_t sj = g.sJ, sk = g.sk(), c = g.idx(i, j, kk);
     const D1 rdt = D1(1.0f) / D1(dt);
-    const D1 base = D1(rho0[kk]) * r.ld(s0, c);
-    D1 delta = D1(0.0f);
     bool flag;
-    {
+    D1 delta;
+    if (D == 0) {
       const D1 d0 = r.ld(dx, c);
       const D1 mxs = pdef_mass(r.ld(dx, c - 2), r.ld(dx, c - 1), d0, r.ld(dx, c + 1), r.ld(dx, c + 2), flag);
-      if (flag) {
-        const D1 sdiv = ((r.ld(s, c) * (r.ld(rru, c + 1) - r.ld(rru, c))) * D1(g.rdx)) * D1(hm_uh(hm, g, i));
-        const D1 advOld = pdef_inflow(g, 0, i, j, c, 1, rru, openx, openy)
-                              ? -sdiv
-                              : ((-(r.ld(fx, c + 1) - r.ld(fx, c))) * D1(g.rdx)) * D1(hm_uh(hm, g, i));
-        const D1 advNew = fma_((d0 + mxs) - base, rdt, -sdiv);
-        delta = delta + (advNew - advOld);
-      }
-    }
-    {
+      if (!flag) return;
+      const D1 base = D1(rho0[kk]) * r.ld(s0, c);
+      const D1 sdiv = ((r.ld(s, c) * (r.ld(rru, c + 1) - r.ld(rru, c))) * D1(g.rdx)) * D1(hm_uh(hm, g, i));
+      const D1 advOld = pdef_inflow(g, 0, i, j, c, 1, rru, openx, openy)
+                            ? -sdiv
+                            : ((-(r.ld(fx, c + 1) - r.ld(fx, c))) * D1(g.rdx)) * D1(hm_uh(hm, g, i));
+      delta = fma_((d0 + mxs) - base, rdt, -sdiv) - advOld;
+    } else if (D == 1) {
       const D1 d0 = r.ld(dy, c);
       const D1 mys = pdef_mass(r.ld(dy, c - 2 * sj), r.ld(dy, c - sj), d0, r.ld(dy, c + sj), r.ld(dy, c + 2 * sj), flag);
-      if (flag) {
-        const D1 sdiv = ((r.ld(s, c) * (r.ld(rrv, c + sj) - r.ld(rrv, c))) * D1(g.rdy)) * D1(hm_vh(hm, g, j));
-        const D1 advOld = pdef_inflow(g, 1, i, j, c, sj, rrv, openx, openy)
-                              ? -sdiv
-                              : ((-(r.ld(fy, c + sj) - r.ld(fy, c))) * D1(g.rdy)) * D1(hm_vh(hm, g, j));
-        const D1 advNew = fma_((d0 + mys) - base, rdt, -sdiv);
-        delta = delta + (advNew - advOld);
-      }
-    }
-    {
+      if (!flag) return;
+      const D1 base = D1(rho0[kk]) * r.ld(s0, c);
+      const D1 sdiv = ((r.ld(s, c) * (r.ld(rrv, c + sj) - r.ld(rrv, c))) * D1(g.rdy)) * D1(hm_vh(hm, g, j));
+      const D1 advOld = pdef_inflow(g, 1, i, j, c, sj, rrv, openx, openy)
+                            ? -sdiv
+                            : ((-(r.ld(fy, c + sj) - r.ld(fy, c))) * D1(g.rdy)) * D1(hm_vh(hm, g, j));
+      delta = fma_((d0 + mys) - base, rdt, -sdiv) - advOld;
+    } else {
       const D1 zm2 = kk >= 2 ? r.ld(dz, c - 2 * sk) : D1(0.0f);
       const D1 zm1 = kk >= 1 ? r.ld(dz, c - sk) : D1(0.0f);
       const D1 d0 = r.ld(dz, c);
       const D1 zp1 = kk + 1 < g.nz ? r.ld(dz, c + sk) : D1(0.0f);
       const D1 zp2 = kk + 2 < g.nz ? r.ld(dz, c + 2 * sk) : D1(0.0f);
       const D1 mzs = pdef_mass(zm2, zm1, d0, zp1, zp2, flag);
-      if (flag) {
-        const D1 sdiv = ((r.ld(s, c) * (r.ld(rrw, c + sk) - r.ld(rrw, c))) * D1(g.rdz)) * D1(mh[kk]);
-        const D1 advOld = ((-(r.ld(fz, c + sk) - r.ld(fz, c))) * D1(g.rdz)) * D1(mh[kk]);
-        const D1 advNew = fma_((d0 + mzs) - base, rdt, -sdiv);
-        delta = delta + (advNew - advOld);
-      }
+      if (!flag) return;
+      const D1 base = D1(rho0[kk]) * r.ld(s0, c);
+      const D1 sdiv = ((r.ld(s, c) * (r.ld(rrw, c + sk) - r.ld(rrw, c))) * D1(g.rdz)) * D1(mh[kk]);
+      const D1 advOld = ((-(r.ld(fz, c + sk) - r.ld(fz, c))) * D1(g.rdz)) * D1(mh[kk]);
+      delta = fma_((d0 + mzs) - base, rdt, -sdiv) - advOld;
     }
-    r.out<0>(tend, c, r.ld(tend, c) + delta / D1(rho0[kk]));
+    r.keep<0>(tend, c, delta / D1(rho0[kk]));
   }
 };
+using B_pdef_apply = Parts<B_pdef_part<0>, B_pdef_part<1>, B_pdef_part<2>>;
 
 // ---- halos, copies, damping
 

@@ -160,10 +160,11 @@ struct Acc {
   double yb[MAXO];
   int64_t oa[MAXO];
   bool has[MAXO];
+  bool clr[MAXO];  // the output overwrites its word (out), or adds to it (keep)
 
   TE_HD Acc(const float* M_, float* Mb_, const AdjArg* a_)
       : M(M_), Mb(Mb_), a(a_), mode(0), cnt(0), seed(-1), rec(-1), acc(0.0) {
-    for (int o = 0; o < MAXO; ++o) { yb[o] = 0.0; oa[o] = 0; has[o] = false; }
+    for (int o = 0; o < MAXO; ++o) { yb[o] = 0.0; oa[o] = 0; has[o] = false; clr[o] = false; }
   }
   // parameter n: its values and adjoint offset / its words
   TE_HD P f(int n) const { return P{a->v[n], a->p[n]}; }
@@ -198,9 +199,17 @@ struct Acc {
       oa[O] = w;
       yb[O] = (double)Mb[w];
       has[O] = true;
+      clr[O] = true;
     } else {
       acc = fmad_(yb[O], y.d, acc);
     }
+  }
+  // y added to p[c] (p[c] += y, the old value not among the body's loads):
+  // the word's adjoint stays as it is, the loads of y receive ybar * dy/dx
+  template <int O>
+  TE_HD void keep(P p, int64_t c, D1 y) {
+    out<O>(p, c, y);
+    if (mode == 0) clr[O] = false;
   }
 };
 
@@ -222,7 +231,7 @@ TE_HD void adj_point(const float* M, float* Mb, const AdjArg& a, int x, int y, i
   if (!any) return;
   const int n = r.cnt;
   for (int o = 0; o < MAXO; ++o)
-    if (r.has[o]) Mb[r.oa[o]] = 0.0f;
+    if (r.has[o] && r.clr[o]) Mb[r.oa[o]] = 0.0f;
   r.mode = 1;
   for (int s = 0; s < n; ++s) {
     r.cnt = 0;
@@ -233,6 +242,24 @@ TE_HD void adj_point(const float* M, float* Mb, const AdjArg& a, int x, int y, i
     if (r.acc != 0.0 && r.rec >= 0) adj_add(Mb + r.rec, (float)r.acc);
   }
 }
+
+// A kernel whose per-point update is a sum of independent parts, each a body
+// that adds its part with keep<> (the seeded passes of one body grow with the
+// square of its active loads; parts split them).
+template <class... Bs>
+struct Parts {};
+template <class B>
+struct PointAdj {
+  static TE_HD void run(const float* M, float* Mb, const AdjArg& a, int x, int y, int z) {
+    adj_point<B>(M, Mb, a, x, y, z);
+  }
+};
+template <class... Bs>
+struct PointAdj<Parts<Bs...>> {
+  static TE_HD void run(const float* M, float* Mb, const AdjArg& a, int x, int y, int z) {
+    (adj_point<Bs>(M, Mb, a, x, y, z), ...);
+  }
+};
 
 // Metric buffer (grid.js:200-214): uh | uf | vh | vf ...
 TE_HD float hm_uh(const float* hm, const ADims& g, int i) { return hm[i + 3]; }

@@ -24,7 +24,7 @@ __global__ __launch_bounds__(256) void k_adj(const float* M, float* Mb, AdjArg a
   const uint64_t n0 = a.n[0], n01 = n0 * a.n[1];
   if (t >= n01 * a.n[2]) return;
   const uint64_t z = t / n01, rem = t - z * n01, y = rem / n0, x = rem - y * n0;
-  adj_point<B>(M, Mb, a, (int)x, (int)y, (int)z);
+  PointAdj<B>::run(M, Mb, a, (int)x, (int)y, (int)z);
 }
 
 // acoustic_w_solve: the rows' coefficients, column sweeps, the levels'
