@@ -52,6 +52,7 @@ struct te_adjoint {
   std::vector<std::vector<std::pair<uint64_t, uint64_t>>> wreg;  // (offset, words) per dispatch
   std::vector<uint64_t> stride;  // log words per region of each dispatch (its length rounded up to 64)
   std::vector<char> halo_pair;    // dispatch i is bc_periodic_y right after bc_periodic_x of the same field
+  std::vector<std::vector<char>> live;  // per dispatch and region: some adjoint reads its saved values
   hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr, e3 = nullptr;
   float fwd_ms = 0.0f, bwd_ms = 0.0f;
   std::string err;
@@ -110,17 +111,23 @@ static int reverse_steps(te_adjoint* a, int steps) {
     for (int i = 0; i < n;) {
       te::adj::CopyArg c{};
       int nreg = 0, e = i;
+      std::vector<uint64_t> wr;  // every region the run writes (saved or not)
       for (; e < n; ++e) {
         const auto& w = a->wreg[e];
-        bool fits = nreg + (int)w.size() <= te::adj::kCopyRegions && (w.empty() || nreg == 0 || w[0].second == c.n);
+        int nlive = 0;
+        for (size_t q = 0; q < w.size(); ++q) nlive += a->live[e][q];
+        bool fits = nreg + nlive <= te::adj::kCopyRegions && (nlive == 0 || nreg == 0 || w[0].second == c.n);
         for (size_t q = 0; q < w.size() && fits; ++q)
-          for (int t = 0; t < nreg && fits; ++t) fits = c.src[t] != r->dM + w[q].first;
+          for (uint64_t o : wr) fits = fits && o != w[q].first;
         if (!fits) break;
         at[(size_t)s * n + e] = pos;
-        for (size_t q = 0; q < w.size(); ++q, ++nreg) {
+        for (size_t q = 0; q < w.size(); ++q) {
+          wr.push_back(w[q].first);
+          if (!a->live[e][q]) continue;  // no adjoint reads these values: not saved
           c.src[nreg] = r->dM + w[q].first;
           c.dst[nreg] = a->dLog + pos + q * a->stride[e];
           c.n = w[q].second;
+          ++nreg;
         }
         pos += w.size() * a->stride[e];
       }
@@ -140,7 +147,7 @@ static int reverse_steps(te_adjoint* a, int steps) {
     for (int i = n - 1; i >= 0; --i) {
       const uint64_t p = at[(size_t)s * n + i];
       for (size_t q = 0; q < a->wreg[i].size(); ++q)
-        cur[a->wreg[i][q].first] = reinterpret_cast<const float*>(a->dLog + p + q * a->stride[i]);
+        if (a->live[i][q]) cur[a->wreg[i][q].first] = reinterpret_cast<const float*>(a->dLog + p + q * a->stride[i]);
       const uint32_t* A = r->ops[i].A;
       if (a->halo_pair[i]) {  // with the bc_periodic_x before it, as one map
         Unit h;
@@ -382,6 +389,41 @@ int te_adjoint_create(te_ctx* ctx, int32_t every, te_adjoint** out) {
     }
     if (a->wreg[i].size() > (size_t)te::adj::kCopyRegions)
       return afail(a, TE_ERR_UNSUPPORTED, "dispatch " + std::to_string(i) + " (" + nm + ") writes too many fields");
+  }
+  // A region's saved values (written before dispatch i overwrites field f)
+  // are what the adjoints of dispatch i and of the dispatches since f's
+  // previous writer read as f (walking back, cur[f] points at them until
+  // that writer).  When none of those has f among its parameters the
+  // region is not saved.  The walk wraps around the step (the previous
+  // writer may be in the step before); at a segment's first step it covers
+  // dispatches before the segment too, which only keeps more regions.
+  {
+    std::vector<std::vector<uint64_t>> reads(n);
+    for (int j = 0; j < n; ++j)
+      for (const Unit& u : a->units[j])
+        if (u.adj >= 0)
+          for (int q = 0; q < 32; ++q) reads[j].push_back(u.A[4 + q]);
+    auto has = [](const std::vector<uint64_t>& v, uint64_t x) { return std::find(v.begin(), v.end(), x) != v.end(); };
+    auto writes = [&](int j, uint64_t f) {
+      for (auto& w : a->wreg[j])
+        if (w.first == f) return true;
+      return false;
+    };
+    a->live.resize(n);
+    for (int i = 0; i < n; ++i) {
+      a->live[i].assign(a->wreg[i].size(), 0);
+      for (size_t q = 0; q < a->wreg[i].size(); ++q) {
+        const uint64_t f = a->wreg[i][q].first;
+        for (int t = 0; t < n; ++t) {
+          const int j = (i - t + n) % n;
+          if (t > 0 && writes(j, f)) break;
+          if (has(reads[j], f)) {
+            a->live[i][q] = 1;
+            break;
+          }
+        }
+      }
+    }
   }
   a->halo_pair.assign(n, 0);
   for (int i = 1; i < n; ++i)
