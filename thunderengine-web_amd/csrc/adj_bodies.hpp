@@ -714,38 +714,49 @@ struct B_adv_assemble {
 
 // ---- acoustic substeps
 
-// acoustic_prepare.wgsl:130-184 (pk1/pk2 operand order switched by vec_v)
-struct B_acoustic_prepare {
+// acoustic_prepare.wgsl:130-184 (pk1/pk2 operand order switched by vec_v).
+// Its four outputs read disjoint loads (ppterm; piadv at k, k+1; piadv at
+// k, k-1; thv), so each output is a body of its own: a seeded pass then
+// evaluates one output, not all four.
+template <int O>
+struct B_acoustic_prepare_part {
   static TE_HD void run(Acc& r, int i, int j, int kk) {
     const ADims g = r.dims();
     if (i >= g.nx || j >= g.ny || kk >= g.nz) return;
     const float dts = r.F(1, 0);
     const P ppterm = r.f(2), piadv = r.f(3), thv = r.f(4), rr0 = r.f(5), rf0 = r.f(6), mh = r.f(7),
-                mf = r.f(8), c1 = r.f(9), c2 = r.f(10);
+            mf = r.f(8), c1 = r.f(9), c2 = r.f(10);
     const int64_t sk = g.sk(), c = g.idx(i, j, kk);
-    const D1 cp = CP;
     const D1 tem = (D1(dts) * D1(g.rdz)) * D1(mh[kk]);
-    const D1 r2 = (tem * D1(rr0[kk])) * D1(rf0[kk + 1]);
-    const D1 r1 = (tem * D1(rr0[kk])) * D1(rf0[kk]);
-    const D1 tem1 = ((D1(dts) * D1(g.rdz)) * cp) * D1(mf[kk]);
     const D1 c1a = D1(c1[kk]), c1b = D1(c1[kk + 1]), c2a = D1(c2[kk]), c2b = D1(c2[kk + 1]);
     const bool vec_v = g.nx > 3 && i < (g.nx / 4) * 4;
-    r.out<0>(r.f(11), c, tem * r.ld(ppterm, c));
-    D1 p2 = D1(0.0f), p1 = D1(0.0f);
-    if (kk != g.nz - 1) {
-      p2 = vec_v ? fma_(-c2b, r.ld(piadv, c + sk), (D1(1.0f) - c1b) * r.ld(piadv, c))
-                 : fma_(D1(1.0f) - c1b, r.ld(piadv, c), -(c2b * r.ld(piadv, c + sk)));
+    if (O == 0) {
+      r.out<0>(r.f(11), c, tem * r.ld(ppterm, c));
+    } else if (O == 1) {
+      const D1 r2 = (tem * D1(rr0[kk])) * D1(rf0[kk + 1]);
+      D1 p2 = D1(0.0f);
+      if (kk != g.nz - 1) {
+        p2 = vec_v ? fma_(-c2b, r.ld(piadv, c + sk), (D1(1.0f) - c1b) * r.ld(piadv, c))
+                   : fma_(D1(1.0f) - c1b, r.ld(piadv, c), -(c2b * r.ld(piadv, c + sk)));
+      }
+      r.out<0>(r.f(13), c, kk == g.nz - 1 ? D1(0.0f) : r2 * p2);
+    } else if (O == 2) {
+      const D1 r1 = (tem * D1(rr0[kk])) * D1(rf0[kk]);
+      D1 p1 = D1(0.0f);
+      if (kk != 0) {
+        p1 = vec_v ? fma_(c2a - D1(1.0f), r.ld(piadv, c), c1a * r.ld(piadv, c - sk))
+                   : fma_(c1a, r.ld(piadv, c - sk), (c2a - D1(1.0f)) * r.ld(piadv, c));
+      }
+      r.out<0>(r.f(12), c, kk == 0 ? D1(0.0f) : r1 * p1);
+    } else {
+      const D1 tem1 = ((D1(dts) * D1(g.rdz)) * D1(CP)) * D1(mf[kk]);
+      const D1 tc = r.ld(thv, c);
+      r.out<0>(r.f(14), c, tem1 * fma_(c2a, tc, c1a * (kk == 0 ? tc : r.ld(thv, c - sk))));
     }
-    if (kk != 0) {
-      p1 = vec_v ? fma_(c2a - D1(1.0f), r.ld(piadv, c), c1a * r.ld(piadv, c - sk))
-                 : fma_(c1a, r.ld(piadv, c - sk), (c2a - D1(1.0f)) * r.ld(piadv, c));
-    }
-    r.out<1>(r.f(13), c, kk == g.nz - 1 ? D1(0.0f) : r2 * p2);
-    r.out<2>(r.f(12), c, kk == 0 ? D1(0.0f) : r1 * p1);
-    const D1 tc = r.ld(thv, c);
-    r.out<3>(r.f(14), c, tem1 * fma_(c2a, tc, c1a * (kk == 0 ? tc : r.ld(thv, c - sk))));
   }
 };
+using B_acoustic_prepare = Parts<B_acoustic_prepare_part<0>, B_acoustic_prepare_part<1>, B_acoustic_prepare_part<2>,
+                                 B_acoustic_prepare_part<3>>;
 
 // acoustic_begin.wgsl:128-149
 struct B_acoustic_begin {
