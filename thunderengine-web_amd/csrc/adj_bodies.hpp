@@ -127,66 +127,77 @@ struct B_stage_tendency_reset {
 };
 
 // stage_state_reset.wgsl:110-138
-struct B_stage_state_reset {
+// five independent copies: one body per output
+template <int O>
+struct B_stage_state_reset_part {
   static TE_HD void run(Acc& r, int x, int y, int z) {
     const ADims g = r.dims();
     const int i = x - 3, j = y - 3, kk = z;
     if (i > g.nx + 3 || j > g.ny + 3 || kk > g.nz) return;
     const int64_t c = g.idx(i, j, kk);
-    r.out<0>(r.f(5), c, r.ld(r.f(1), c));
-    r.out<1>(r.f(6), c, r.ld(r.f(2), c));
-    r.out<2>(r.f(7), c, r.ld(r.f(3), c));
-    r.out<3>(r.f(8), c, r.ld(r.f(4), c));
-    r.out<4>(r.f(9), c, r.ld(r.f(10), c));
+    r.out<0>(r.f(5 + O), c, r.ld(r.f(O == 4 ? 10 : 1 + O), c));
   }
 };
+using B_stage_state_reset = Parts<B_stage_state_reset_part<0>, B_stage_state_reset_part<1>,
+                                  B_stage_state_reset_part<2>, B_stage_state_reset_part<3>,
+                                  B_stage_state_reset_part<4>>;
 
 // massflux_from_state.wgsl:111-137
-struct B_massflux_from_state {
+// three independent products: one body per output
+template <int O>
+struct B_massflux_from_state_part {
   static TE_HD void run(Acc& r, int x, int y, int z) {
     const ADims g = r.dims();
     const int i = x - 3, j = y - 3, kk = z;
     if (i > g.nx + 3 || j > g.ny + 3 || kk > g.nz) return;
     const P u = r.f(1), v = r.f(2), w = r.f(3), rho0 = r.f(4), rf0 = r.f(5);
     const int64_t c = g.idx(i, j, kk);
-    if (kk < g.nz) {
-      r.out<0>(r.f(6), c, D1(rho0[kk]) * r.ld(u, c));
-      r.out<1>(r.f(7), c, D1(rho0[kk]) * r.ld(v, c));
-    }
-    r.out<2>(r.f(8), c, (kk == 0 || kk == g.nz) ? D1(0.0f) : D1(rf0[kk]) * r.ld(w, c));
+    if (O == 0 && kk < g.nz) r.out<0>(r.f(6), c, D1(rho0[kk]) * r.ld(u, c));
+    if (O == 1 && kk < g.nz) r.out<0>(r.f(7), c, D1(rho0[kk]) * r.ld(v, c));
+    if (O == 2) r.out<0>(r.f(8), c, (kk == 0 || kk == g.nz) ? D1(0.0f) : D1(rf0[kk]) * r.ld(w, c));
   }
 };
+using B_massflux_from_state =
+    Parts<B_massflux_from_state_part<0>, B_massflux_from_state_part<1>, B_massflux_from_state_part<2>>;
 
 // velocity_to_massflux.wgsl:111-134
-struct B_velocity_to_massflux {
+// three independent in-place products: one body per output
+template <int O>
+struct B_velocity_to_massflux_part {
   static TE_HD void run(Acc& r, int x, int y, int z) {
     const ADims g = r.dims();
     const int i = x - 3, j = y - 3, kk = z;
     if (i > g.nx + 3 || j > g.ny + 3 || kk > g.nz) return;
     const P rho0 = r.f(1), rf0 = r.f(2), rru = r.f(3), rrv = r.f(4), rrw = r.f(5);
     const int64_t c = g.idx(i, j, kk);
-    if (kk < g.nz) {
-      r.out<0>(rru, c, r.ld(rru, c) * D1(rho0[kk]));
-      r.out<1>(rrv, c, r.ld(rrv, c) * D1(rho0[kk]));
-    }
-    r.out<2>(rrw, c, (kk == 0 || kk == g.nz) ? D1(0.0f) : r.ld(rrw, c) * D1(rf0[kk]));
+    if (O == 0 && kk < g.nz) r.out<0>(rru, c, r.ld(rru, c) * D1(rho0[kk]));
+    if (O == 1 && kk < g.nz) r.out<0>(rrv, c, r.ld(rrv, c) * D1(rho0[kk]));
+    if (O == 2) r.out<0>(rrw, c, (kk == 0 || kk == g.nz) ? D1(0.0f) : r.ld(rrw, c) * D1(rf0[kk]));
   }
 };
+using B_velocity_to_massflux =
+    Parts<B_velocity_to_massflux_part<0>, B_velocity_to_massflux_part<1>, B_velocity_to_massflux_part<2>>;
 
 // div_massflux.wgsl:125-147
-struct B_div_massflux {
+// The divergence is the sum of three one-direction differences: one body
+// per direction adds its share (keep), then a body with no loads takes the
+// output's adjoint away (the kernel overwrites it).
+template <int D>
+struct B_div_massflux_part {
   static TE_HD void run(Acc& r, int x, int y, int z) {
     const ADims g = r.dims();
     const int i = x - 3, j = y - 3, kk = z;
     if (i > g.nx + 3 - 1 || j > g.ny + 3 - 1 || kk >= g.nz) return;
     const P rru = r.f(1), rrv = r.f(2), rrw = r.f(3), mh = r.f(4), hm = r.f(30);
     const int64_t sj = g.sJ, sk = g.sk(), c = g.idx(i, j, kk);
-    const D1 dz = (r.ld(rrw, c + sk) - r.ld(rrw, c)) * D1(g.rdz);
-    const D1 dx = (r.ld(rru, c + 1) - r.ld(rru, c)) * D1(g.rdx);
-    const D1 dy = (r.ld(rrv, c + sj) - r.ld(rrv, c)) * D1(g.rdy);
-    r.out<0>(r.f(5), c, fma_(dz, D1(mh[kk]), fma_(dx, D1(hm_uh(hm, g, i)), dy * D1(hm_vh(hm, g, j)))));
+    if (D == 0) r.keep<0>(r.f(5), c, ((r.ld(rru, c + 1) - r.ld(rru, c)) * D1(g.rdx)) * D1(hm_uh(hm, g, i)));
+    else if (D == 1) r.keep<0>(r.f(5), c, ((r.ld(rrv, c + sj) - r.ld(rrv, c)) * D1(g.rdy)) * D1(hm_vh(hm, g, j)));
+    else if (D == 2) r.keep<0>(r.f(5), c, ((r.ld(rrw, c + sk) - r.ld(rrw, c)) * D1(g.rdz)) * D1(mh[kk]));
+    else r.out<0>(r.f(5), c, D1(0.0f));
   }
 };
+using B_div_massflux = Parts<B_div_massflux_part<0>, B_div_massflux_part<1>, B_div_massflux_part<2>,
+                             B_div_massflux_part<3>>;
 
 // divergence3d.wgsl:125-147
 struct B_divergence3d {
@@ -247,8 +258,8 @@ struct B_apply_thterm {
     const ADims g = r.dims();
     if (x >= g.nx || y >= g.ny || z >= g.nz) return;
     const int64_t c = g.idx(x, y, z);
-    const P tend = r.f(3);
-    r.out<0>(tend, c, fma_(-r.ld(r.f(1), c), r.ld(r.f(2), c), r.ld(tend, c)));
+    // tend += -div3d * thterm: the identity on tend's adjoint (keep)
+    r.keep<0>(r.f(3), c, -r.ld(r.f(1), c) * r.ld(r.f(2), c));
   }
 };
 
@@ -259,8 +270,8 @@ struct B_turb_add_tendency {
     const int i = x - 3, j = y - 3, kk = z;
     if (i > g.nx + 3 || j > g.ny + 3 || kk > g.nz) return;
     const int64_t c = g.idx(i, j, kk);
-    const P dst = r.f(2);
-    r.out<0>(dst, c, fma_(D1(r.F(3)), r.ld(r.f(1), c), r.ld(dst, c)));
+    // dst += scale * src: the identity on dst's adjoint (keep; src may be dst)
+    r.keep<0>(r.f(2), c, D1(r.F(3)) * r.ld(r.f(1), c));
   }
 };
 
