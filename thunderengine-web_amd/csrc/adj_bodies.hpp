@@ -333,6 +333,7 @@ struct B_diabatic_rates {
 
 // pressure_tendency.wgsl:234-291
 struct B_pressure_tendency {
+  static constexpr bool kReload = true;
   static TE_HD void run(Acc& r, int x, int y, int z) {
     const ADims g = r.dims();
     if (x >= g.nx || y >= g.ny || z >= g.nz) return;
@@ -449,6 +450,7 @@ struct B_adv_flux_h_weno {
 
 // adv_flux_z_weno.wgsl:174-279
 struct B_adv_flux_z_weno {
+  static constexpr bool kReload = true;
   static TE_HD void run(Acc& r, int i, int j, int z) {
     const ADims g = r.dims();
     const int kf = z + 1;
@@ -946,6 +948,7 @@ TE_HD D1 pdef_mass(D1 dm2, D1 dm1, D1 d0, D1 dp1, D1 dp2, bool& flag) {
 // bodies with about a third of the active loads each.
 template <int D>
 struct B_pdef_part {
+  static constexpr bool kReload = true;
   static TE_HD void run(Acc& r, int i, int j, int kk) {
     const ADims g = r.dims();
     if (i >= g.nx || j >= g.ny || kk >= g.nz) return;
@@ -1024,6 +1027,7 @@ struct B_copy {
 
 // rayleigh_damping.wgsl:111-164 (relaxation to the u0 / v0 profiles)
 struct B_rayleigh_damping {
+  static constexpr bool kReload = true;
   static TE_HD void run(Acc& r, int i, int j, int kk) {
     const ADims g = r.dims();
     if (i > g.nx || j > g.ny || kk > g.nz) return;

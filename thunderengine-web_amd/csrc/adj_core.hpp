@@ -146,7 +146,8 @@ struct ADims {
   }
 };
 
-constexpr int MAXO = 8;  // outputs per point
+constexpr int MAXO = 8;   // outputs per point
+constexpr int MAXL = 32;  // active loads whose values the discovery pass keeps for the seeded passes
 
 // Accessor handed to the bodies: memory, adjoint memory and the pass state.
 struct Acc {
@@ -154,6 +155,7 @@ struct Acc {
   float* Mb;       // adjoints (word 0: word p[n] + c is the adjoint of parameter n's value c)
   const AdjArg* a;
   int mode;  // 0: discovery pass, 1: seeded pass
+  bool cache = true;  // seeded passes read the kept load values (lv)
   int cnt, seed;
   int64_t rec;
   double acc;
@@ -161,6 +163,7 @@ struct Acc {
   int64_t oa[MAXO];
   bool has[MAXO];
   bool clr[MAXO];  // the output overwrites its word (out), or adds to it (keep)
+  float lv[MAXL];  // values of the first MAXL active loads, in load order
 
   TE_HD Acc(const float* M_, float* Mb_, const AdjArg* a_)
       : M(M_), Mb(Mb_), a(a_), mode(0), cnt(0), seed(-1), rec(-1), acc(0.0) {
@@ -180,9 +183,18 @@ struct Acc {
     g.rdx = F(0, 5); g.rdy = F(0, 6); g.rdz = F(0, 7);
     return g;
   }
-  // active load of p[c]
+  // active load of p[c].  The seeded passes take the same loads in the same
+  // order (their branches are decided on the same values), so they read the
+  // values the discovery pass kept instead of the memory.
   TE_HD D1 ld(P p, int64_t c) {
-    D1 r((double)p.v[c]);
+    float x;
+    if (mode == 0 || !cache) {
+      x = p.v[c];
+      if (cache && cnt < MAXL) lv[cnt] = x;
+    } else {
+      x = cnt < MAXL ? lv[cnt] : p.v[c];
+    }
+    D1 r((double)x);
     if (mode != 0 && cnt == seed) {
       r.d = 1.0;
       rec = (int64_t)p.off + c;
@@ -221,10 +233,22 @@ TE_HD void adj_add(float* p, float v) {
 #endif
 }
 
+// Bodies declaring kReload re-read their loads in every seeded pass: for
+// them the kept values cost more registers (or scratch) than the reloads.
+template <class B, class = void>
+struct KeepLoads {
+  static constexpr bool value = true;
+};
+template <class B>
+struct KeepLoads<B, decltype((void)B::kReload)> {
+  static constexpr bool value = false;
+};
+
 // Adjoint of one point of a kernel whose per-point body is B::run(Acc&, x, y, z).
 template <class B>
 TE_HD void adj_point(const float* M, float* Mb, const AdjArg& a, int x, int y, int z) {
   Acc r(M, Mb, &a);
+  r.cache = KeepLoads<B>::value;
   B::run(r, x, y, z);
   bool any = false;
   for (int o = 0; o < MAXO; ++o) any = any || r.has[o];
