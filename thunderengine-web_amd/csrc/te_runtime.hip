@@ -549,10 +549,18 @@ int te_autotune(te_ctx* c, int32_t reps, int32_t force) {
                                   {true, 256, 1, 1}};
   constexpr int kCand = (int)(sizeof(kShapes) / sizeof(kShapes[0]));
   auto p2 = [](uint32_t v) { uint32_t r = 1; while (r < v) r <<= 1; return r; };
+  // the grid a shape gives must stay within launch()'s limits (te_block's exact range)
+  auto grid_ok = [](uint64_t gx, uint64_t gy, uint64_t gz) {
+    return gx * gy * gz < (1u << 24) && gx <= (1u << 16) && gy <= (1u << 16);
+  };
   auto fits = [&](int i, const Shape& sh) {
     const te::Op& op = c->ops[i];
-    if (op.kid == kAdvFused) return !sh.plane && sh.x == 64 && sh.y * sh.z <= 4;
-    if (sh.plane) return planeOk[i] != 0;
+    if (sh.plane) {
+      const uint64_t pts = (uint64_t)op.A[0] * op.A[1];
+      return op.kid != kAdvFused && planeOk[i] != 0 && grid_ok((pts + 255) / 256, op.A[2], 1);
+    }
+    if (!grid_ok((op.A[0] + sh.x - 1) / sh.x, (op.A[1] + sh.y - 1) / sh.y, (op.A[2] + sh.z - 1) / sh.z)) return false;
+    if (op.kid == kAdvFused) return sh.x == 64 && sh.y * sh.z <= 4;
     return sh.x <= std::max<uint32_t>(64, p2(op.A[0])) && sh.y <= p2(op.A[1]) && sh.z <= p2(op.A[2]);
   };
   const std::vector<te::Op> base = c->ops;  // shapes as the context had them
