@@ -72,3 +72,12 @@ def test_workgroup_renumbering_division_is_exact():
         m = magic(d)
         l = np.arange(1 << 24, dtype=np.uint64)
         assert np.array_equal((l * np.uint64(m)) >> np.uint64(40), l // np.uint64(d))
+
+
+def test_autotune_rejects_missing_contexts():
+    """te_autotune and its queries on no context: an error code (no GPU call)."""
+    L = te_host.lib()
+    assert L.te_autotune(None, 5, 0) != te_host.TE_OK
+    assert L.te_tuned_ops(None) == 0
+    a, b = ctypes.c_float(), ctypes.c_float()
+    assert L.te_autotune_ms(None, ctypes.byref(a), ctypes.byref(b)) != te_host.TE_OK
