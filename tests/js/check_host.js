@@ -2,6 +2,7 @@
 // Node-side checks of the JavaScript host, driven by tests/test_node_host.py.
 //   node check_host.js plan            -> tape plan summary (CPU)
 //   node check_host.js encode '<json>' -> program summary of a config (CPU)
+//   node check_host.js ranges          -> bad offsets / counts are refused (CPU)
 //   node check_host.js replay <out>    -> supercell tape replayed on the GPU with hashes
 //   node check_host.js run '<json>' n [tune] -> field FNV-1a after n steps on the GPU
 //        (tune: launch shapes chosen by run.autotune first, kept regardless)
@@ -32,6 +33,20 @@ if (mode === 'plan') {
   const prog = r.program();
   console.log(JSON.stringify({ ops: prog.names.length, names: prog.names, argsFnv: fnvWords(prog.args),
     fields: r.arena.names.length, memWords: r.memoryWords() }));
+} else if (mode === 'ranges') {
+  // offsets and counts that are negative, fractional or beyond 2^53 are
+  // refused by the addon (RangeError) before they reach the C ABI
+  const r = te.buildRun({ nx: 8, ny: 8, nz: 4 }, { encodeOnly: true });
+  const bad = [[-1, 4], [2 ** 64 - 1, 4], [0.5, 4], [NaN, 4], [0, -1], [0, 2 ** 64 - 1], [r.memoryWords(), 4]];
+  const refused = [];
+  for (const [off, n] of bad) {
+    for (const f of [() => r.read(n, off), () => r.fnv1a(off, n), () => r.regionHash(off, n)]) {
+      try { f(); refused.push(false); } catch (e) { refused.push(e instanceof RangeError || /outside memory/.test(e.message)); }
+    }
+  }
+  const ok = r.read(16, 0).length === 16;
+  console.log(JSON.stringify({ refused, ok }));
+  r.destroy();
 } else if (mode === 'replay') {
   const tape = loadTape();
   const rp = new te.Replayer(tape, loadParams());

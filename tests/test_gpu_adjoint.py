@@ -167,3 +167,33 @@ def test_gpu_gradient_matches_fp64_central_differences(gpu):
         say("direction", t, "adjoint", ad, "fd", fd, "tl", tl)
         assert abs(ad - fd) <= 3e-4 * abs(fd), (ad, fd)
         assert abs(ad - tl) <= 3e-4 * abs(tl), (ad, tl)
+
+
+def test_gpu_backward_twice_and_closed_run(gpu):
+    """Two backward passes on one Adjoint (zero, seed, te_bwd each time: a
+    training loop) give the same gradient to the float-atomics' reordering,
+    and an Adjoint whose run was closed first raises instead of using it."""
+    run = te_host.Run(te_host.config(**GRID))
+    run.init_state()
+    adj = te_host.Adjoint(run, every=2)
+    aend = runs.arena_end(run)
+    a0 = run.field_offset(run.fields[0])
+    adj.forward(2)
+    rng = np.random.default_rng(31)
+    seed = np.zeros(aend, np.float32)
+    seed[a0:] = rng.standard_normal(aend - a0)
+    _zero_fields(run, seed, runs.FUSED_SCRATCH)
+    grads = []
+    for rep in range(2):
+        say("backward pass", rep)
+        adj.zero()
+        adj.upload(0, seed)
+        adj.backward()
+        grads.append(adj.read(0, aend).astype(np.float64))
+    err = np.linalg.norm(grads[0] - grads[1]) / np.linalg.norm(grads[0])
+    say("relative difference between the passes", err)
+    assert err < 1e-6
+    run.close()
+    with pytest.raises(te_host.TeError, match="destroyed"):
+        adj.forward(1)
+    adj.close()

@@ -81,3 +81,23 @@ def test_autotune_rejects_missing_contexts():
     assert L.te_tuned_ops(None) == 0
     a, b = ctypes.c_float(), ctypes.c_float()
     assert L.te_autotune_ms(None, ctypes.byref(a), ctypes.byref(b)) != te_host.TE_OK
+
+
+def test_word_ranges_that_wrap_are_rejected():
+    """Offsets near 2^64 (a negative offset from a caller) would wrap
+    `off + n`: the C ABI rejects them with TE_ERR_ARG before any device
+    call (an encoded-only context, no GPU needed)."""
+    run = te_host.Run(te_host.config(nx=8, ny=8, nz=4), encode_only=True)
+    L = te_host.lib()
+    buf = np.zeros(16, np.uint32)
+    ptr = buf.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))
+    for off in (2**64 - 1, 2**64 - 8, run.mem_words):
+        assert L.te_upload_words(run.h, ctypes.c_uint64(off), ptr, ctypes.c_uint64(16)) == te_host.TE_ERR_ARG
+        assert L.te_download_words(run.h, ctypes.c_uint64(off), ptr, ctypes.c_uint64(16)) == te_host.TE_ERR_ARG
+    assert L.te_download_words(run.h, ctypes.c_uint64(0), ptr, ctypes.c_uint64(2**64 - 1)) == te_host.TE_ERR_ARG
+    h = ctypes.c_uint32()
+    assert L.te_region_hash(run.h, ctypes.c_uint64(2**64 - 1), ctypes.c_uint64(2), ctypes.byref(h)) == te_host.TE_ERR_ARG
+    assert L.te_fnv1a(run.h, ctypes.c_uint64(2**64 - 1), ctypes.c_uint64(2), ctypes.byref(h)) == te_host.TE_ERR_ARG
+    # an in-range read of the encoded image still works
+    assert L.te_download_words(run.h, ctypes.c_uint64(0), ptr, ctypes.c_uint64(16)) == te_host.TE_OK
+    run.close()

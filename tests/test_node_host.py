@@ -51,6 +51,11 @@ def test_tape_plan_matches_reference_plan():
     assert got["slotFnv"] == ol.fnv1a(np.array([s[2] for s in P.slots], np.uint32))
 
 
+def test_addon_refuses_bad_offsets():
+    got = json.loads(node("ranges"))
+    assert got["ok"] and all(got["refused"]), got
+
+
 def test_build_run_encodes_like_the_c_abi():
     conf = dict(nx=24, ny=20, nz=12, tke=1, rayleigh=1)
     got = json.loads(node("encode", json.dumps(conf)))

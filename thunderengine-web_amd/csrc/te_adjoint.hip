@@ -41,6 +41,7 @@ struct te_adjoint {
   uint64_t state_words = 0;  // [0, arena end)
   float* dMbBase = nullptr;
   float* dMb = nullptr;
+  uint64_t mb_words = 0;  // words of the adjoint allocation (rec's front guard + memory + back guard)
   float* dScr = nullptr;  // 4 planes of one field: the column solve's coefficients
   std::vector<uint32_t*> ckpt;
   int nsteps = 0;
@@ -351,6 +352,10 @@ static int decompose(te_adjoint* a, int i, std::vector<Unit>& units, std::vector
   return TE_OK;
 }
 
+void te::adjoint_detach(te_adjoint* a) {
+  if (a) a->fwd = nullptr;
+}
+
 extern "C" {
 
 int te_adjoint_create(te_ctx* ctx, int32_t every, te_adjoint** out) {
@@ -436,10 +441,15 @@ int te_adjoint_create(te_ctx* ctx, int32_t every, te_adjoint** out) {
     a->log_words += a->stride[i] * a->wreg[i].size();
   }
   hipStream_t s = r->stream;
+  // the adjoint launches index the adjoint memory as the forward's index M:
+  // give it the same front and back guards, grown to their final size first
+  rc = te::ensure_guard(r);
+  if (rc != TE_OK) return afail(a, rc, std::string("program: ") + te_last_error(r));
   const uint64_t mbw = r->front + r->words;
   TA_HIP(hipMalloc((void**)&a->dMbBase, mbw * 4), "hipMalloc(adjoint memory)");
   TA_HIP(hipMemsetAsync(a->dMbBase, 0, mbw * 4, s), "clear adjoint memory");
   a->dMb = a->dMbBase + r->front;
+  a->mb_words = mbw;
   TA_HIP(hipMalloc((void**)&a->dScr, 4 * L.field_words * 4), "hipMalloc(column scratch)");
   // a whole segment's write log when it fits the budget (one recompute per
   // step); otherwise one step's log and the segment's step-start states
@@ -459,6 +469,7 @@ int te_adjoint_create(te_ctx* ctx, int32_t every, te_adjoint** out) {
   TA_HIP(hipEventCreate(&a->e2), "event");
   TA_HIP(hipEventCreate(&a->e3), "event");
   TA_HIP(hipStreamSynchronize(s), "sync");
+  ctx->adjoints.push_back(a);
   return TE_OK;
 }
 
@@ -466,6 +477,7 @@ te_ctx* te_adjoint_program(te_adjoint* a) { return a ? a->rec : nullptr; }
 
 int te_fwd_ckpt(te_adjoint* a, int32_t nsteps) {
   if (!a || !a->rec || nsteps < 0) return afail(a, TE_ERR_ARG, "bad arguments");
+  if (!a->fwd) return afail(a, TE_ERR_STATE, "the forward context was destroyed");
   te_ctx* f = a->fwd;
   const int nseg = (nsteps + a->every - 1) / a->every;
   while ((int)a->ckpt.size() < nseg) {
@@ -491,7 +503,9 @@ int te_bwd(te_adjoint* a) {
   if (!a || !a->rec) return afail(a, TE_ERR_ARG, "bad arguments");
   if (a->nsteps <= 0) return afail(a, TE_ERR_STATE, "no recorded steps (te_fwd_ckpt)");
   te_ctx* r = a->rec;
-  TA_HIP(hipStreamSynchronize(a->fwd->stream), "sync forward");
+  // the checkpoints were written on the forward context's stream; a
+  // destroyed context synchronised its stream before it went
+  if (a->fwd) TA_HIP(hipStreamSynchronize(a->fwd->stream), "sync forward");
   TA_HIP(hipEventRecord(a->e2, r->stream), "event");
   const int nseg = (a->nsteps + a->every - 1) / a->every;
   const uint64_t sw = a->state_words;
@@ -545,14 +559,16 @@ int te_adj_ops(te_adjoint* a, int32_t first, int32_t last) {
 }
 
 int te_adj_upload(te_adjoint* a, uint64_t off, const float* src, uint64_t n) {
-  if (!a || !a->rec || !src || off + n > a->rec->prog_words) return afail(a, TE_ERR_ARG, "upload outside memory");
+  if (!a || !a->rec || !src || n > a->rec->prog_words || off > a->rec->prog_words - n)
+    return afail(a, TE_ERR_ARG, "upload outside memory");
   TA_HIP(hipMemcpyAsync(a->dMb + off, src, n * 4, hipMemcpyHostToDevice, a->rec->stream), "upload");
   TA_HIP(hipStreamSynchronize(a->rec->stream), "sync");
   return TE_OK;
 }
 
 int te_adj_download(te_adjoint* a, uint64_t off, float* dst, uint64_t n) {
-  if (!a || !a->rec || !dst || off + n > a->rec->prog_words) return afail(a, TE_ERR_ARG, "download outside memory");
+  if (!a || !a->rec || !dst || n > a->rec->prog_words || off > a->rec->prog_words - n)
+    return afail(a, TE_ERR_ARG, "download outside memory");
   TA_HIP(hipMemcpyAsync(dst, a->dMb + off, n * 4, hipMemcpyDeviceToHost, a->rec->stream), "download");
   TA_HIP(hipStreamSynchronize(a->rec->stream), "sync");
   return TE_OK;
@@ -560,8 +576,7 @@ int te_adj_download(te_adjoint* a, uint64_t off, float* dst, uint64_t n) {
 
 int te_adj_zero(te_adjoint* a) {
   if (!a || !a->rec) return afail(a, TE_ERR_ARG, "bad arguments");
-  const uint64_t mbw = a->rec->front + a->rec->words;
-  TA_HIP(hipMemsetAsync(a->dMbBase, 0, mbw * 4, a->rec->stream), "clear");
+  TA_HIP(hipMemsetAsync(a->dMbBase, 0, a->mb_words * 4, a->rec->stream), "clear");
   TA_HIP(hipStreamSynchronize(a->rec->stream), "sync");
   return TE_OK;
 }
@@ -577,6 +592,10 @@ const char* te_adjoint_last_error(const te_adjoint* a) { return a ? a->err.c_str
 
 void te_adjoint_destroy(te_adjoint* a) {
   if (!a) return;
+  if (a->fwd) {
+    auto& v = a->fwd->adjoints;
+    v.erase(std::remove(v.begin(), v.end(), a), v.end());
+  }
   if (a->rec && a->rec->stream) (void)hipStreamSynchronize(a->rec->stream);
   for (uint32_t* p : a->ckpt) (void)hipFree(p);
   if (a->dMbBase) (void)hipFree(a->dMbBase);

@@ -54,10 +54,18 @@ void fill_initial_state(const te_config& cfg, const Layout& L, std::vector<uint3
 }  // namespace te
 
 struct te_ctx;
+struct te_adjoint;
 namespace te {
 // Launches ops [first, last] of a context in order on its stream (no events,
 // no graph); TE_OK or a status with the context's error set.
 int run_ops_plain(te_ctx* c, int first, int last);
+// Reads every op's grid and grows the context's guard regions so that no
+// launch of its program leaves the allocation; idempotent.  Afterwards
+// c->front and c->words are final.
+int ensure_guard(te_ctx* c);
+// Called by te_destroy for each adjoint still built on the context: the
+// adjoint forgets the context, and its calls that need it fail.
+void adjoint_detach(te_adjoint* a);
 
 }  // namespace te
 
@@ -85,4 +93,7 @@ struct te_ctx {
   std::vector<float> zf;  // owned copy of cfg.zf
   te::Layout layout;
   std::vector<uint32_t> image;
+  // adjoints built on this context (te_adjoint_create); te_destroy detaches
+  // them so that their later calls fail instead of using freed state
+  std::vector<te_adjoint*> adjoints;
 };

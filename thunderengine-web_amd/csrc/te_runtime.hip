@@ -275,7 +275,7 @@ static const std::set<std::string>& own_mapping() {
   return s;
 }
 
-static int ensure_guard(te_ctx* c) {
+static int grow_guards(te_ctx* c) {
   if (c->guard_ok) return TE_OK;
   uint64_t need = 0;
   std::map<uint32_t, std::array<uint32_t, 5>> seen;  // grid parameter blocks already read
@@ -345,6 +345,8 @@ static int ensure_guard(te_ctx* c) {
   return TE_OK;
 }
 
+int te::ensure_guard(te_ctx* c) { return grow_guards(c); }
+
 static int run_ops(te_ctx* c, int first, int last) {
   for (int i = first; i <= last; ++i) {
     hipError_t e = te::launch(c, c->ops[i]);
@@ -381,7 +383,7 @@ static int ensure_graph(te_ctx* c) {
 int te::run_ops_plain(te_ctx* c, int first, int last) {
   if (!c || !c->dM) return fail(c, TE_ERR_STATE, "context has no memory");
   if (first < 0 || last >= (int)c->ops.size() || first > last) return fail(c, TE_ERR_ARG, "bad op range");
-  int rc = ensure_guard(c);
+  int rc = grow_guards(c);
   if (rc != TE_OK) return rc;
   return run_ops(c, first, last);
 }
@@ -415,14 +417,14 @@ int te_program_create(uint64_t mem_words, int32_t nops, const char* const* names
 }
 
 int te_upload_words(te_ctx* c, uint64_t off, const uint32_t* src, uint64_t n) {
-  if (!c || !src || off + n > c->prog_words) return fail(c, TE_ERR_ARG, "upload outside memory");
+  if (!c || !src || n > c->prog_words || off > c->prog_words - n) return fail(c, TE_ERR_ARG, "upload outside memory");
   TE_HIP(hipMemcpyAsync(c->dM + off, src, n * 4, hipMemcpyHostToDevice, c->stream), "upload");
   TE_HIP(hipStreamSynchronize(c->stream), "upload sync");
   return TE_OK;
 }
 
 int te_download_words(te_ctx* c, uint64_t off, uint32_t* dst, uint64_t n) {
-  if (!c || !dst || off + n > c->prog_words) return fail(c, TE_ERR_ARG, "download outside memory");
+  if (!c || !dst || n > c->prog_words || off > c->prog_words - n) return fail(c, TE_ERR_ARG, "download outside memory");
   if (!c->dM) {  // encoded program without device memory: read the host image
     if (off + n > c->image.size()) return fail(c, TE_ERR_STATE, "no memory image");
     std::memcpy(dst, c->image.data() + off, n * 4);
@@ -438,7 +440,7 @@ int te_run(te_ctx* c, int32_t nsteps, int32_t first, int32_t last) {
   const int n = (int)c->ops.size();
   if (last < 0 || last >= n) last = n - 1;
   if (first < 0 || first > last || nsteps < 0) return fail(c, TE_ERR_ARG, "bad op range");
-  int rc0 = ensure_guard(c);
+  int rc0 = grow_guards(c);
   if (rc0 != TE_OK) return rc0;
   const bool full = first == 0 && last == n - 1;
   if (full && nsteps > 1) ensure_graph(c);
@@ -460,7 +462,7 @@ int te_profile_ops(te_ctx* c, int32_t nsteps, float* op_ms) {
   const int n = (int)c->ops.size();
   for (int i = 0; i < n; ++i)
     if (c->ops[i].kid < 0) return fail(c, TE_ERR_UNSUPPORTED, "kernel '" + c->op_names[i] + "' is not on this path");
-  int rc0 = ensure_guard(c);
+  int rc0 = grow_guards(c);
   if (rc0 != TE_OK) return rc0;
   std::vector<hipEvent_t> ev(n + 1, nullptr);
   int rc = TE_OK;
@@ -507,7 +509,7 @@ int te_autotune(te_ctx* c, int32_t reps, int32_t force) {
   const int n = (int)c->ops.size();
   for (int i = 0; i < n; ++i)
     if (c->ops[i].kid < 0) return fail(c, TE_ERR_UNSUPPORTED, "kernel '" + c->op_names[i] + "' is not on this path");
-  int rc = ensure_guard(c);
+  int rc = grow_guards(c);
   if (rc != TE_OK) return rc;
   // the trial steps change the memory: every trial starts from a copy of it,
   // and the copy is put back at the end
@@ -564,6 +566,19 @@ int te_autotune(te_ctx* c, int32_t reps, int32_t force) {
     return sh.x <= std::max<uint32_t>(64, p2(op.A[0])) && sh.y <= p2(op.A[1]) && sh.z <= p2(op.A[2]);
   };
   const std::vector<te::Op> base = c->ops;  // shapes as the context had them
+  // every return before the end (a failed launch, event or copy) leaves the
+  // context as it was: its own shapes, and the memory from before the trials
+  bool done = false;
+  auto undo = [&]() {
+    if (done) return;
+    c->ops = base;
+    (void)restore();
+    (void)hipStreamSynchronize(c->stream);
+  };
+  struct OnExit {
+    decltype(undo)& f;
+    ~OnExit() { f(); }
+  } onExit{undo};
   std::vector<te::Op> best = base;
   std::vector<float> bestMs(n, 1e30f), ms(n);
 
@@ -636,6 +651,7 @@ int te_autotune(te_ctx* c, int32_t reps, int32_t force) {
   for (int i = 0; i < n; ++i) tuned += c->ops[i].tuned ? 1 : 0;
   TE_HIP(restore(), "autotune restore");
   TE_HIP(hipStreamSynchronize(c->stream), "autotune sync");
+  done = true;
   c->tune_base_ms = tBase / (3.0f * reps);
   c->tune_best_ms = tBest / (3.0f * reps);
   const bool hadGraph = c->graph != nullptr;
@@ -658,7 +674,7 @@ int te_autotune_ms(const te_ctx* c, float* untuned, float* tuned) {
 int32_t te_tuned_ops(const te_ctx* c) { return c ? c->tuned_ops : 0; }
 
 int te_region_hash(te_ctx* c, uint64_t off, uint64_t n, uint32_t* out) {
-  if (!c || !out || off + n > c->words) return fail(c, TE_ERR_ARG, "hash outside memory");
+  if (!c || !out || n > c->words || off > c->words - n) return fail(c, TE_ERR_ARG, "hash outside memory");
   hipLaunchKernelGGL(te::k_zero_word, dim3(1), dim3(1), 0, c->stream, c->dHash);
   hipLaunchKernelGGL(te::k_region_hash, dim3(256), dim3(256), 0, c->stream, (const uint32_t*)c->dM, off, n, c->dHash);
   TE_HIP(hipGetLastError(), "hash launch");
@@ -668,7 +684,7 @@ int te_region_hash(te_ctx* c, uint64_t off, uint64_t n, uint32_t* out) {
 }
 
 int te_fnv1a(te_ctx* c, uint64_t off, uint64_t n, uint32_t* out) {
-  if (!c || !out || off + n > c->words) return fail(c, TE_ERR_ARG, "fnv outside memory");
+  if (!c || !out || n > c->prog_words || off > c->prog_words - n) return fail(c, TE_ERR_ARG, "fnv outside memory");
   std::vector<uint32_t> h(n);
   int rc = te_download_words(c, off, h.data(), n);
   if (rc != TE_OK) return rc;
@@ -694,6 +710,7 @@ const char* te_last_error(const te_ctx* c) { return c ? c->err.c_str() : "null c
 
 void te_destroy(te_ctx* c) {
   if (!c) return;
+  for (te_adjoint* a : c->adjoints) te::adjoint_detach(a);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   if (c->graph) (void)hipGraphExecDestroy(c->graph);
   if (c->dBase) (void)hipFree(c->dBase);
@@ -797,7 +814,7 @@ int te_ensemble_create(const te_config* cfgs, int32_t n, te_ensemble** out) {
     te_ctx* c = nullptr;
     int rc = te_create(&cfgs[m], &c);
     if (rc == TE_OK) rc = te_init_state(c);
-    if (rc == TE_OK) rc = ensure_guard(c);
+    if (rc == TE_OK) rc = grow_guards(c);
     if (rc == TE_OK) rc = ensure_graph(c);
     if (rc != TE_OK) {
       e->err = "member " + std::to_string(m) + ": " + (c ? c->err : std::string("create failed"));

@@ -63,6 +63,19 @@ double num(napi_env env, napi_value v) {
   return d;
 }
 
+// A word offset or count from JS: a finite integer in [0, 2^53); anything
+// else throws a RangeError (false returned).
+bool index_arg(napi_env env, napi_value v, uint64_t* out) {
+  double d = 0;
+  if (napi_get_value_double(env, v, &d) != napi_ok || !(d >= 0.0) || !(d < 9007199254740992.0) ||
+      d != (double)(uint64_t)d) {
+    napi_throw_range_error(env, nullptr, "offsets and counts must be integers in [0, 2^53)");
+    return false;
+  }
+  *out = (uint64_t)d;
+  return true;
+}
+
 napi_value make_num(napi_env env, double d) {
   napi_value r;
   napi_create_double(env, d, &r);
@@ -274,7 +287,9 @@ napi_value Upload(napi_env env, napi_callback_info info) {
   void* data = nullptr;
   size_t bytes = 0;
   if (!typed(env, argv[2], &data, &bytes)) return nullptr;
-  int rc = te_upload_words(c, (uint64_t)num(env, argv[1]), static_cast<const uint32_t*>(data), bytes / 4);
+  uint64_t off = 0;
+  if (!index_arg(env, argv[1], &off)) return nullptr;
+  int rc = te_upload_words(c, off, static_cast<const uint32_t*>(data), bytes / 4);
   if (rc != TE_OK) return throw_status(env, c, rc, "te_upload_words");
   return nullptr;
 }
@@ -284,7 +299,12 @@ napi_value Download(napi_env env, napi_callback_info info) {
   if (!get_args(env, info, 3, argv)) return nullptr;
   te_ctx* c = ctx_of(env, argv[0]);
   if (!c) return nullptr;
-  uint64_t off = (uint64_t)num(env, argv[1]), n = (uint64_t)num(env, argv[2]);
+  uint64_t off = 0, n = 0;
+  if (!index_arg(env, argv[1], &off) || !index_arg(env, argv[2], &n)) return nullptr;
+  if (n > te_memory_words(c) || off > te_memory_words(c) - n) {
+    napi_throw_range_error(env, nullptr, "download outside memory");
+    return nullptr;
+  }
   void* data = nullptr;
   napi_value ab, arr;
   NAPI_OK(napi_create_arraybuffer(env, n * 4, &data, &ab));
@@ -362,8 +382,9 @@ napi_value HashCall(napi_env env, napi_callback_info info, bool fnv) {
   te_ctx* c = ctx_of(env, argv[0]);
   if (!c) return nullptr;
   uint32_t h = 0;
-  int rc = fnv ? te_fnv1a(c, (uint64_t)num(env, argv[1]), (uint64_t)num(env, argv[2]), &h)
-               : te_region_hash(c, (uint64_t)num(env, argv[1]), (uint64_t)num(env, argv[2]), &h);
+  uint64_t off = 0, n = 0;
+  if (!index_arg(env, argv[1], &off) || !index_arg(env, argv[2], &n)) return nullptr;
+  int rc = fnv ? te_fnv1a(c, off, n, &h) : te_region_hash(c, off, n, &h);
   if (rc != TE_OK) return throw_status(env, c, rc, fnv ? "te_fnv1a" : "te_region_hash");
   return make_num(env, h);
 }
@@ -541,7 +562,9 @@ napi_value AdjUpload(napi_env env, napi_callback_info info) {
   void* data = nullptr;
   size_t bytes = 0;
   if (!typed(env, argv[2], &data, &bytes)) return nullptr;
-  int rc = te_adj_upload(a, (uint64_t)num(env, argv[1]), static_cast<const float*>(data), bytes / 4);
+  uint64_t off = 0;
+  if (!index_arg(env, argv[1], &off)) return nullptr;
+  int rc = te_adj_upload(a, off, static_cast<const float*>(data), bytes / 4);
   if (rc != TE_OK) return throw_adj(env, a, rc, "te_adj_upload");
   return nullptr;
 }
@@ -551,7 +574,13 @@ napi_value AdjDownload(napi_env env, napi_callback_info info) {
   if (!get_args(env, info, 3, argv)) return nullptr;
   te_adjoint* a = adj_of(env, argv[0]);
   if (!a) return nullptr;
-  uint64_t off = (uint64_t)num(env, argv[1]), n = (uint64_t)num(env, argv[2]);
+  uint64_t off = 0, n = 0;
+  if (!index_arg(env, argv[1], &off) || !index_arg(env, argv[2], &n)) return nullptr;
+  const uint64_t words = te_memory_words(te_adjoint_program(a));
+  if (n > words || off > words - n) {
+    napi_throw_range_error(env, nullptr, "download outside memory");
+    return nullptr;
+  }
   void* data = nullptr;
   napi_value ab, arr;
   NAPI_OK(napi_create_arraybuffer(env, n * 4, &data, &ab));

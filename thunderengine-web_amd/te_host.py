@@ -20,6 +20,10 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "lib", "libte.so")
 
 TE_OK = 0
+TE_ERR_ARG = -1
+TE_ERR_HIP = -2
+TE_ERR_UNSUPPORTED = -3
+TE_ERR_STATE = -4
 
 
 class TeError(RuntimeError):
