@@ -157,6 +157,18 @@ def config(**kw):
     return c
 
 
+def tanh_faces(nz=120, dz_bottom=20.0, dz_top=300.0, scale=5000.0):
+    """Face heights (m) of a column whose spacing grows from dz_bottom at the
+    surface towards dz_top as dz(z) = dz_bottom + (dz_top - dz_bottom) tanh(z / scale):
+    with the defaults 120 levels, 20 m at the surface, 297 m at 16 km, top
+    24.7 km (the tornado-resolving grid's column).  For te_config.zf
+    (VerticalGrid(faceHeights), web/core/grid.js:256)."""
+    z = [0.0]
+    for _ in range(nz):
+        z.append(z[-1] + dz_bottom + (dz_top - dz_bottom) * float(np.tanh(z[-1] / scale)))
+    return z
+
+
 class _Ctx:
     def __init__(self):
         self.h = ctypes.c_void_p()
