@@ -99,6 +99,14 @@ def alg_bytes_per_launch(run, names, A, op):
         pp = count([(3, 1, 0), (4, 1, 0), (5, 1, 0), (6, 1, 0), (7, 1, 1), (8, 0, 1)] + ([(9, 0, 1)] if save_x else []))
         avg = count([(2, 1, 0), (11, 1, 0), (12, 1, 0), (13, 1, 1), (14, 1, 1), (15, 1, 1)])
         return 4.0 * (pp * inner + avg * pts)
+    if name == "ac_column":  # one acoustic substep after acoustic_uv, over the interior points
+        inner = run.nx * run.ny * run.nz
+        first = _flag(run, A, op, 33) != 0
+        io = [(2, 1, 0), (3, 1, 0), (4, 1, 1), (5, 1, 0), (6, 1, 0), (7, 1, 0), (8, 1, 0), (9, 1, 0), (10, 1, 1),
+              (11, 0, 1), (13, not first, 1), (14, not first, 1), (15, 1, 1)]
+        if _flag(run, A, op, 16) != 0:
+            io.append((12, 0, 1))
+        return 4.0 * count(io) * inner
     if name == "bc_periodic_xy":  # each halo cell reads its interior image and writes itself
         n = _flag(run, A, op, 9)
         io = [(1 + q, True, True) for q in range(n)]

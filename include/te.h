@@ -135,8 +135,10 @@ typedef struct te_config {
   int32_t initial_tke;     /* init3d testcase of the t=0 TKE profile: 0 none, 3, 4, 5, 9, 11
                               (scenario.js:347-369) */
   int32_t fuse;            /* 1 (default): merged / fused launches where the step allows them
-                              (same state bits; scratch fields may differ); 0: the reference's
-                              dispatch list one for one */
+                              (same state bits; scratch fields may differ), on periodic domains one
+                              launch per acoustic substep after acoustic_uv; 2: the same with the
+                              acoustic substep as separate launches (the program the adjoint
+                              recomputes); 0: the reference's dispatch list one for one */
   uint64_t sounding_seed;  /* perturbed soundings (ensemble members): seed of the draws below */
   float th_pert_sigma;     /* K: theta(z) += sigma * N(0,1) noise smoothed over ~1 km in z */
   float qv_pert_sigma;     /* qv(z) *= 1 + sigma * N(0,1), one draw per sounding */

@@ -7,6 +7,7 @@
 #ifdef _OPENMP
 #include <omp.h>
 #endif
+#include <array>
 #include <cstdio>
 #include <cstring>
 #include <stdexcept>
@@ -93,6 +94,47 @@ static int run_program(uint32_t* w, T* v, const char* const* names, const uint32
         Ax[34] = Ay[34] = A[34];
         K<T> kx{w, v, Ax}, ky{w, v, Ay};
         call(ex, kx); call(ey, ky);
+      }
+      continue;
+    }
+    if (nm == "ac_column") {
+      // the dispatches one acoustic substep launch stands for: on the stage's
+      // first substep acoustic_prepare and acoustic_begin, then acoustic_fpk,
+      // acoustic_w_solve, acoustic_pp, acoustic_accumulate and the periodic
+      // halo fill of ppd, u, v, w (bc_periodic_x then bc_periodic_y each)
+      const uint32_t* dm = w + A[4];
+      const uint32_t nx = dm[0], ny = dm[1], nz = dm[2];
+      auto run1 = [&](const char* kn, std::array<uint32_t, 3> g3, std::vector<std::pair<int, uint32_t>> ps) {
+        uint32_t X[64] = {0};
+        X[0] = g3[0]; X[1] = g3[1]; X[2] = g3[2];
+        X[4] = A[4];      // dims
+        X[5] = A[5];      // acoustic parameters
+        X[34] = A[34];    // metric
+        for (auto& pr : ps) X[4 + pr.first] = pr.second;
+        const KernelEntry* e1 = find_kernel(kn);
+        K<T> k1{w, v, X};
+        call(e1, k1);
+      };
+      const std::array<uint32_t, 3> full = {nx, ny, nz}, whole = {nx + 7, ny + 7, nz + 1}, cols = {nx, ny, 1};
+      auto P = [&](int q) { return A[4 + q]; };
+      const uint32_t U = P(2), V = P(3), W = P(4), PIADV = P(5), PPTEN = P(6), PPTERM = P(7), THV = P(8), WTEN = P(9),
+                     PP3D = P(10), PPD = P(11), PPX = P(12), RRU = P(13), RRV = P(14), RRW = P(15), QK = P(24),
+                     PK1 = P(25), PK2 = P(26), MM = P(27), FPK = P(28), CA = P(29), CB = P(31);
+      if (w[P(33)] != 0u) {
+        run1("acoustic_prepare", full, {{2, PPTERM}, {3, PIADV}, {4, THV}, {5, P(18)}, {6, P(19)}, {7, P(20)}, {8, P(21)},
+                                        {9, P(22)}, {10, P(23)}, {11, QK}, {12, PK1}, {13, PK2}, {14, MM}});
+        run1("acoustic_begin", whole, {{2, W}, {3, RRU}, {4, RRV}, {5, RRW}});
+      }
+      run1("acoustic_fpk", full, {{2, U}, {3, V}, {4, W}, {5, PIADV}, {6, PPTEN}, {7, PPTERM}, {8, PK1}, {9, PK2},
+                                  {10, QK}, {11, FPK}, {15, P(32)}, {16, P(18)}, {17, P(19)}});
+      run1("acoustic_w_solve", cols, {{2, PK1}, {3, PK2}, {4, QK}, {5, MM}, {6, FPK}, {7, WTEN}, {8, PP3D}, {9, W},
+                                      {10, CA}, {11, CB}});
+      run1("acoustic_pp", full, {{2, W}, {3, PK1}, {4, PK2}, {5, QK}, {6, FPK}, {7, PP3D}, {8, PPD}, {9, PPX},
+                                 {10, P(16)}});
+      run1("acoustic_accumulate", whole, {{2, U}, {3, V}, {4, W}, {5, RRU}, {6, RRV}, {7, RRW}, {8, P(17)}});
+      for (uint32_t f : {PPD, U, V, W}) {
+        run1("bc_periodic_x", {4, ny + 7, nz + 1}, {{1, f}});
+        run1("bc_periodic_y", {4, nx + 7, nz + 1}, {{1, f}});
       }
       continue;
     }

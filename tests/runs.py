@@ -57,9 +57,11 @@ class OracleRun:
 
 # Scratch fields the fused launches may leave with other values than the
 # reference's dispatch list (fluxes, elimination coefficients, fpk, tend,
-# sadv).  tests/test_oracle_invariants.py checks that each is dead at the
-# start of a step: its contents then never reach another field.
-FUSED_SCRATCH = ("fx", "fy", "fz", "cA", "cB", "fpk", "tend", "sadv", "tketen_s")
+# sadv, and the acoustic substep's per-stage terms qk, mm, pk1, pk2, which
+# the one-launch substep forms per level instead of storing).
+# tests/test_oracle_invariants.py checks that each is dead at the start of a
+# step: its contents then never reach another field.
+FUSED_SCRATCH = ("fx", "fy", "fz", "cA", "cB", "fpk", "tend", "sadv", "tketen_s", "qk", "mm", "pk1", "pk2")
 
 
 def state_mask(e):

@@ -115,9 +115,13 @@ def test_schedule_closed_form():
         names, _ = r.program()
         assert names.count("acoustic_uv") == total, (ns, names.count("acoustic_uv"))
         assert names.count("acoustic_w_solve") == total
-        r = te_host.Run(te_host.config(nx=16, ny=16, nz=12, nsound=ns, fuse=1), encode_only=True)
+        r = te_host.Run(te_host.config(nx=16, ny=16, nz=12, nsound=ns, fuse=2), encode_only=True)
         names, _ = r.program()
         assert names.count("acoustic_uv") == names.count("acoustic_w_solve_lds") == names.count("ac_pp_accumulate") == total
+        r = te_host.Run(te_host.config(nx=16, ny=16, nz=12, nsound=ns, fuse=1), encode_only=True)
+        names, _ = r.program()
+        assert names.count("acoustic_uv") == names.count("ac_column") == total
+        assert "acoustic_prepare" not in names and "acoustic_begin" not in names
 
 
 def test_encoder_rejects_bad_config():
@@ -165,6 +169,15 @@ def test_fused_launches_write_no_field_their_neighbours_read(kw):
             written = {off(q) for q in (7, 8, 9, 13, 14, 15)}
             assert off(2) not in written, (op, name)
             seen.add(name)
+        elif name == "ac_column":
+            # one column per thread: w, pp3d are read and written in the own
+            # column only; piadv is read at neighbouring columns and never
+            # written; u, v are read at i+1 / j+1 and written only at halo
+            # cells, by the one thread that reads them there
+            written = {off(q) for q in (4, 10, 11, 12, 13, 14, 15)}
+            assert off(5) not in written and not {off(q) for q in (6, 7, 8, 9)} & written, (op, name)
+            assert off(2) not in written and off(3) not in written, (op, name)
+            seen.add(name)
         elif name == "add_fields":  # independent triples
             n = word(op, 13)
             srcs, dsts = {off(1 + q) for q in range(n)}, [off(5 + q) for q in range(n)]
@@ -175,4 +188,4 @@ def test_fused_launches_write_no_field_their_neighbours_read(kw):
             srcs, dsts = {off(1 + q) for q in range(n)}, [off(5 + q) for q in range(n)]
             assert len(set(dsts)) == n and not srcs & set(dsts), (op, name)
             seen.add(name)
-    assert "adv_fused" in seen and "ac_pp_accumulate" in seen
+    assert "adv_fused" in seen and ("ac_pp_accumulate" in seen or "ac_column" in seen)

@@ -367,7 +367,12 @@ int te_adjoint_create(te_ctx* ctx, int32_t every, te_adjoint** out) {
   if (every < 1) return afail(a, TE_ERR_ARG, "checkpoint interval must be at least 1");
   a->fwd = ctx;
   a->every = every;
-  te_config cfg = ctx->cfg;  // the same program (fused or the reference's list) on memory of its own
+  // the same program (fused or the reference's list) on memory of its own,
+  // with the acoustic substep as its separate launches (fuse = 2): their
+  // adjoints read the per-stage terms and fpk the one-launch substep keeps
+  // in registers
+  te_config cfg = ctx->cfg;
+  if (cfg.fuse == 1) cfg.fuse = 2;
   int rc = te_create(&cfg, &a->rec);
   if (rc != TE_OK) return afail(a, rc, std::string("program: ") + (a->rec ? te_last_error(a->rec) : ""));
   te_ctx* r = a->rec;

@@ -764,11 +764,17 @@ void Encoder::stages() {
        {fb(1, "u_a"), fb(2, "v_a"), fb(3, "w_a"), fb(4, "ppi"), fb(5, "u"), fb(6, "v"), fb(7, "w"), fb(8, "pp3d"),
         fb(9, "ppd"), fb(10, "ppx")});
     acoustic = acousticBytes((double)(float)sch.dts[nrk]);
-    op("acoustic_prepare", full,
-       {fb(2, "ppterm"), fb(3, "piadv"), fb(4, "thv"), rb(5, "rr0"), rb(6, "rf0"), rb(7, "mh"), rb(8, "mf"),
-        rb(9, "c1"), rb(10, "c2"), fb(11, "qk"), fb(12, "pk1"), fb(13, "pk2"), fb(14, "mm")},
-       true);
-    op("acoustic_begin", whole, {fb(2, "w"), fb(3, "rru"), fb(4, "rrv"), fb(5, "rrw")}, true);
+    // fuse = 1 on a periodic domain: after acoustic_uv, one ac_column launch
+    // per substep (fpk, column solve, pp, accumulate, halo fill; prepare's
+    // terms formed in it, begin folded into the stage's first substep)
+    const bool column = cfg.fuse == 1 && !openX && !openY && nx >= 4 && ny >= 4 && nz >= 4 && (nz + 7) / 8 <= 16;
+    if (!column) {
+      op("acoustic_prepare", full,
+         {fb(2, "ppterm"), fb(3, "piadv"), fb(4, "thv"), rb(5, "rr0"), rb(6, "rf0"), rb(7, "mh"), rb(8, "mf"),
+          rb(9, "c1"), rb(10, "c2"), fb(11, "qk"), fb(12, "pk1"), fb(13, "pk2"), fb(14, "mm")},
+         true);
+      op("acoustic_begin", whole, {fb(2, "w"), fb(3, "rru"), fb(4, "rrv"), fb(5, "rrw")}, true);
+    }
     const int nloop = sch.count[nrk];
     for (int n = 0; n < nloop; ++n) {
       if (openX || openY)
@@ -778,6 +784,17 @@ void Encoder::stages() {
            true);
       op("acoustic_uv", stag,
          {fb(2, "ppd"), fb(3, "thv"), fb(4, "uten"), fb(5, "vten"), fb(6, "u"), fb(7, "v"), bb(8, open)}, true);
+      if (column) {
+        const float tavg = n == nloop - 1 ? (float)(1.0 / nloop) : 0.0f;
+        op("ac_column", {nx, ny, 1},
+           {fb(2, "u"), fb(3, "v"), fb(4, "w"), fb(5, "piadv"), fb(6, "ppten"), fb(7, "ppterm"), fb(8, "thv"),
+            fb(9, "wten"), fb(10, "pp3d"), fb(11, "ppd"), fb(12, "ppx"), fb(13, "rru"), fb(14, "rrv"),
+            fb(15, "rrw"), bb(16, I1(nrk == 2 ? 1 : 0)), bb(17, F1(tavg)), rb(18, "rr0"), rb(19, "rf0"),
+            rb(20, "mh"), rb(21, "mf"), rb(22, "c1"), rb(23, "c2"), fb(24, "qk"), fb(25, "pk1"), fb(26, "pk2"),
+            fb(27, "mm"), fb(28, "fpk"), fb(29, "cA"), fb(31, "cB"), rb(32, "rho0"), bb(33, I1(n == 0 ? 1 : 0))},
+           true);
+        continue;
+      }
       if (cfg.fuse != 0) {  // pp + accumulate in one pass; accumulate reads no ppd, so ppd's
                             // halo joins u, v, w's
         op("acoustic_fpk", full,

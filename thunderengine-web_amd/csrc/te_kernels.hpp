@@ -27,8 +27,18 @@
   X(adv_fused) X(add_fields) X(tendency_reset_add) X(acoustic_w_solve_lds) X(turb_scalar_vert_implicit_lds) X(turb_u_tendency_lds) \
   X(turb_v_tendency_lds) X(turb_w_tendency_lds) X(tke_transport_lds)
 
+// Fused launches of 64 x AC_WAVES threads per workgroup (one acoustic
+// substep per launch, k_fused.hip).
+#define AC_WAVES 8
+#define TE_WIDE_LIST(X) X(ac_column)
+
 namespace te {
 #define TE_DECL(n) __global__ __launch_bounds__(256) void k_##n(uint32_t* M, KArg a);
 TE_KERNEL_LIST(TE_DECL)
 #undef TE_DECL
+#define TE_DECL_WIDE(n) __global__ __launch_bounds__(64 * AC_WAVES) void k_##n(uint32_t* M, KArg a);
+TE_WIDE_LIST(TE_DECL_WIDE)
+#undef TE_DECL_WIDE
+// k_ac_column instantiated for 1..16 levels per wave (nullptr outside)
+const void* ac_column_kernel(int levels_per_wave);
 }  // namespace te

@@ -24,7 +24,7 @@ struct KernelInfo {
   const void* fn;
 };
 #define TE_ROW(n) {#n, (const void*)&k_##n},
-static const KernelInfo kKernels[] = {TE_KERNEL_LIST(TE_ROW)};
+static const KernelInfo kKernels[] = {TE_KERNEL_LIST(TE_ROW) TE_WIDE_LIST(TE_ROW)};
 #undef TE_ROW
 static const int kNumKernels = (int)(sizeof(kKernels) / sizeof(kKernels[0]));
 
@@ -112,6 +112,22 @@ static hipError_t launch(te_ctx* c, const Op& op) {
   dim3 b = block_for(a.n);
   static const int kAdvFused = kernel_id("adv_fused");
   if (op.kid == kAdvFused) b = dim3(64, 4, 1);  // one row of cells per wave (k_adv_fused)
+  static const int kAcColumn = kernel_id("ac_column");
+  if (op.kid == kAcColumn) {  // 64 columns of a row per workgroup, AC_WAVES waves over the levels
+    const uint32_t nz = op.nz, lw = (nz + AC_WAVES - 1) / AC_WAVES;
+    if (nz < 4 || lw > 16) return hipErrorInvalidConfiguration;
+    b = dim3(64, AC_WAVES, 1);
+    const dim3 g((a.n[0] + 63) / 64, a.n[1], 1);
+    if (!launch_shape(a, b, g)) return hipErrorInvalidConfiguration;
+    const void* fn = ac_column_kernel((int)lw);
+    const size_t dyn = (size_t)2 * nz * 64 * sizeof(float);  // per level and lane: mm / cA / w and cB
+    if (dyn > 65536) {
+      const hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn);
+      if (e != hipSuccess) return e;
+    }
+    void* args[] = {(void*)&c->dM, (void*)&a};
+    return hipLaunchKernel(fn, g, b, args, dyn, c->stream);
+  }
   size_t lds = 0;
   if (op.flat) {  // runs of 4 words per thread over the op's n0*n1*n2 words (te_flat)
     const uint64_t words = (uint64_t)a.n[0] * a.n[1] * a.n[2], runs = (words + 3) / 4;
@@ -181,6 +197,7 @@ __global__ void k_region_hash(const uint32_t* M, uint64_t off, uint64_t n, uint3
 __global__ void k_zero_word(uint32_t* p) { *p = 0u; }
 
 }  // namespace te
+
 
 // ------------------------------------------------------------------ helpers
 static int fail(te_ctx* c, int code, const std::string& msg) {
@@ -271,7 +288,7 @@ static const std::set<std::string>& own_mapping() {
       "acoustic_w_solve", "acoustic_w_solve_lds", "adv_fused", "_copy", "copy_fields", "turb_sfcstress",
       "turb_u_tendency", "turb_u_tendency_lds", "turb_v_tendency", "turb_v_tendency_lds", "turb_w_tendency",
       "turb_w_tendency_lds", "turb_scalar_vert_implicit", "turb_scalar_vert_implicit_lds", "tke_transport",
-      "tke_transport_lds"};
+      "tke_transport_lds", "ac_column"};
   return s;
 }
 
