@@ -70,6 +70,23 @@ int te_region_hash(te_ctx* ctx, uint64_t word_off, uint64_t nwords, uint32_t* ou
 /* word-wise FNV-1a (web/view.js:1064-1068) over nwords of M. */
 int te_fnv1a(te_ctx* ctx, uint64_t word_off, uint64_t nwords, uint32_t* out);
 
+/* region_hash (as te_region_hash) of n regions of a caller's device memory,
+ * allocating nothing.  No counterpart in the reference: for callers that
+ * cannot allocate device memory (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc) and have no context.  d_mem is a device
+ * pointer on the current device; region r is counts[r] words from word
+ * offs[r] (host arrays), its hash goes to out[r] (host).  The per-region
+ * accumulators live in a device array of the library of
+ * TE_NOALLOC_MAX_REGIONS words per device, cleared by a kernel on every call;
+ * n above that limit returns TE_ERR_ARG, as do missing pointers and ranges
+ * whose byte addresses overflow (the caller keeps every region inside its
+ * buffer: the device memory's size is not known here).  Runs on the device's
+ * null stream and waits for it.  Calls that overlap on one device share the
+ * accumulators and race. */
+#define TE_NOALLOC_MAX_REGIONS 256
+int te_region_hashes_noalloc(const uint32_t* d_mem, const uint64_t* offs, const uint64_t* counts, int32_t n,
+                             uint32_t* out);
+
 /* Total words of M, number of ops, and op i as (name, 64 argument words). */
 uint64_t te_memory_words(const te_ctx* ctx);
 int32_t te_op_count(const te_ctx* ctx);

@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <set>
 
 #include "te_internal.hpp"
@@ -195,9 +196,17 @@ __global__ void k_region_hash(const uint32_t* M, uint64_t off, uint64_t n, uint3
   if (threadIdx.x == 0) atomicAdd(out, part[0] + part[1] + part[2] + part[3]);
 }
 __global__ void k_zero_word(uint32_t* p) { *p = 0u; }
+__global__ void k_zero_words(uint32_t* p, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0u;
+}
 
 }  // namespace te
 
+// Hash accumulators of te_region_hashes_noalloc: one word per region, in a
+// device array of the library (each device holds its own copy until the
+// process ends), so that entry point allocates nothing.
+static __device__ __align__(256) uint32_t scratch[TE_NOALLOC_MAX_REGIONS];
 
 // ------------------------------------------------------------------ helpers
 static int fail(te_ctx* c, int code, const std::string& msg) {
@@ -697,6 +706,41 @@ int te_region_hash(te_ctx* c, uint64_t off, uint64_t n, uint32_t* out) {
   TE_HIP(hipGetLastError(), "hash launch");
   TE_HIP(hipMemcpyAsync(out, c->dHash, 4, hipMemcpyDeviceToHost, c->stream), "hash copy");
   TE_HIP(hipStreamSynchronize(c->stream), "hash sync");
+  return TE_OK;
+}
+
+// Device address of `scratch` per device, looked up once per device with that
+// device current (the array's host-side name is not a device address).
+static uint32_t* noalloc_scratch(int dev) {
+  static std::mutex mu;
+  static std::map<int, uint32_t*> ptrs;
+  std::lock_guard<std::mutex> lock(mu);
+  auto it = ptrs.find(dev);
+  if (it != ptrs.end()) return it->second;
+  uint32_t* p = nullptr;
+  if (hipGetSymbolAddress((void**)&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return nullptr;
+  ptrs[dev] = p;
+  return p;
+}
+
+int te_region_hashes_noalloc(const uint32_t* d_mem, const uint64_t* offs, const uint64_t* counts, int32_t n,
+                             uint32_t* out) {
+  if (n < 0 || n > TE_NOALLOC_MAX_REGIONS || (n > 0 && (!d_mem || !offs || !counts || !out))) return TE_ERR_ARG;
+  for (int32_t r = 0; r < n; ++r)  // byte addresses 4 * (off + count) stay representable
+    if (counts[r] > UINT64_MAX / 8 || offs[r] > UINT64_MAX / 8 - counts[r]) return TE_ERR_ARG;
+  if (n == 0) return TE_OK;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return TE_ERR_HIP;
+  uint32_t* acc = noalloc_scratch(dev);
+  if (!acc) return TE_ERR_HIP;
+  hipStream_t s = 0;  // the device's null stream: no stream is created
+  hipLaunchKernelGGL(te::k_zero_words, dim3((n + 255) / 256), dim3(256), 0, s, acc, (int)n);
+  for (int32_t r = 0; r < n; ++r)
+    if (counts[r] > 0)
+      hipLaunchKernelGGL(te::k_region_hash, dim3(256), dim3(256), 0, s, d_mem, offs[r], counts[r], acc + r);
+  if (hipGetLastError() != hipSuccess) return TE_ERR_HIP;
+  if (hipMemcpyAsync(out, acc, 4 * (size_t)n, hipMemcpyDeviceToHost, s) != hipSuccess) return TE_ERR_HIP;
+  if (hipStreamSynchronize(s) != hipSuccess) return TE_ERR_HIP;
   return TE_OK;
 }
 

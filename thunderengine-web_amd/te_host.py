@@ -80,6 +80,7 @@ def lib():
         "te_profile_ops": ([ctx, ctypes.c_int32, P(ctypes.c_float)], ctypes.c_int),
         "te_region_hash": ([ctx, u64, u64, u32p], ctypes.c_int),
         "te_fnv1a": ([ctx, u64, u64, u32p], ctypes.c_int),
+        "te_region_hashes_noalloc": ([ctypes.c_void_p, P(u64), P(u64), ctypes.c_int32, u32p], ctypes.c_int),
         "te_memory_words": ([ctx], u64),
         "te_op_count": ([ctx], ctypes.c_int32),
         "te_op_get": ([ctx, ctypes.c_int32, P(ctypes.c_char_p), u32p], ctypes.c_int),
