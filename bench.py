@@ -27,8 +27,16 @@ import te_host  # noqa: E402
 
 PEAK_HBM_GBS = 8000.0
 # Algorithmic bytes of one large step per interior point for this workload:
-# 4 B x [18 + 3 (12 + 3*3 + 3 + 4) + 23*11 + 8] (fused design, TKE, 3 scalars)
-B_ALG_STEP = 1452.0
+# 4 B x [P_pre + 3 (12 + 3 N_s + 3 + T_s) + 23 S + 8] with TKE (P_pre = 18,
+# T_s = 4), S = 11 acoustic substeps and N_s = 4 advected scalars (th, qv, qc,
+# tke): 4 x [18 + 3 (12 + 12 + 3 + 4) + 253 + 8] = 1488 (fused design)
+B_ALG_STEP = 1488.0
+
+
+def b_alg(n_scalars, substeps, tke):
+    """Algorithmic bytes per interior point and large step (the formula above)."""
+    pre, ts = (18, 4) if tke else (0, 0)
+    return 4.0 * (pre + 3 * (12 + 3 * n_scalars + 3 + ts) + 23 * substeps + 8)
 
 # kernels launched with one lane per (i, j) column that sweep k serially
 COLUMN_KERNELS = {"acoustic_w_solve", "turb_scalar_vert_implicit", "turb_u_tendency", "turb_v_tendency",
@@ -240,6 +248,39 @@ def adjoint_bench(steps=60, every=10):
             "device_ms": {"fwd_ckpt": round(fwd_ms, 2), "bwd": round(bwd_ms, 2)}}
 
 
+# The tornado-resolving grid's forward step: 512 x 512 x 120 at dx = dy = 100 m,
+# tanh-stretched dz from 20 m (297 m at 16 km), dt = 0.5 s, nsound = 8 (15
+# acoustic substeps), TKE, +3 K bubble, quarter-circle hodograph of radius
+# 13.8 m/s over 0-1 km (0-1 km storm-relative helicity ~ 300 m^2/s^2).
+TORNADO = dict(nx=512, ny=512, nz=120, zf=te_host.tanh_faces(), dz=0.0, dx=100.0, dy=100.0, dt=0.5, nsound=8,
+               tke=1, initial_tke=3, rayleigh=1, rayleigh_zd=18000.0, bubble_amp=3.0, bubble_rh=2000.0,
+               bubble_rv=1000.0, bubble_zc=1000.0, hodo_radius=13.8, hodo_depth=1000.0)
+
+
+def tornado_bench(steps, warmup, tune=True):
+    """Forward Mpt-steps/s of the tornado-resolving grid (graph replays, state resident)."""
+    cfg = te_host.config(**TORNADO)
+    run = te_host.Run(cfg)
+    run.init_state()
+    if tune:
+        run.autotune(reps=2)
+    run.step(warmup)
+    run.sync()
+    t0 = time.perf_counter()
+    run.step(steps)
+    run.sync()
+    wall = time.perf_counter() - t0
+    pts = cfg.nx * cfg.ny * cfg.nz
+    value = pts * steps / wall / 1e6
+    balg = b_alg(4, 15, True)
+    out = {"workload": "512x512x120 tanh dz 20-300 m, dx 100 m, dt 0.5 s, nsound 8 (15 substeps), TKE, 3 K bubble",
+           "steps": steps, "value": round(value, 2), "unit": "Mpt-steps/s", "ms_per_step": round(wall * 1e3 / steps, 3),
+           "alg_bytes_per_pt_step": balg, "hbm_frac": round(value * 1e6 * balg / 8.0e12, 4),
+           "state_GB": round(run.mem_words * 4 / 1e9, 2)}
+    run.close()
+    return out
+
+
 def cpu_baseline(cfg, seconds=15.0):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib
@@ -256,7 +297,11 @@ def cpu_baseline(cfg, seconds=15.0):
     pts = cfg.nx * cfg.ny * cfg.nz
     return {"value": round(pts * n / dt / 1e6, 3), "unit": "Mpt-steps/s", "cores": int(oracle_lib.lib().oracle_threads()),
             "kind": "port",
-            "sample": "%d large steps of the same 192x192x80 workload after 1 warm-up step, oracle (C++ -O2, OpenMP)" % n}
+            "sample": "%d large steps of the same 192x192x80 workload after 1 warm-up step, oracle (C++ -O2, OpenMP)" % n,
+            "reference_cpu_paths": {"WGSL step on a CPU WebGPU adapter": "unavailable (no WebGPU implementation on the "
+                                    "host, and the reference's ES2020 modules do not load in its Node 12)",
+                                    "CM1 r20.2 Fortran FP32": "unavailable (no Fortran compiler, CM1 source not "
+                                    "in the reference)"}}
 
 
 def main():
@@ -271,6 +316,9 @@ def main():
                     help="skip the 32-member uneven ensemble (members stepped together)")
     ap.add_argument("--no-adjoint", action="store_true",
                     help="skip the adjoint cost (128x128x60, 60 steps, checkpoint every 10)")
+    ap.add_argument("--no-tornado", action="store_true",
+                    help="skip the 512x512x120 tornado-resolving grid's forward step")
+    ap.add_argument("--ensemble-steps", type=int, default=120, help="large steps of the ensemble timing")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default launch shapes (no te_autotune before the timed steps)")
     args = ap.parse_args()
@@ -347,11 +395,13 @@ def main():
     }
     run.close()
     if not args.no_ensemble:
-        ens = ensemble_bench(max(5, args.steps // 5), 2, tune=not args.no_autotune)
+        ens = ensemble_bench(args.ensemble_steps, 2, tune=not args.no_autotune)
         ens["per_point_vs_plain"] = round(ens["value"] / value, 3)
         out["ensemble"] = ens
     if not args.no_adjoint:
         out["adjoint"] = adjoint_bench()
+    if not args.no_tornado:
+        out["tornado_grid"] = tornado_bench(10, 2, tune=not args.no_autotune)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(cfg)
     print(json.dumps(out), flush=True)
