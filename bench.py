@@ -304,6 +304,10 @@ def cpu_baseline(cfg, seconds=15.0):
                                     "in the reference)"}}
 
 
+def log(*a):
+    print("bench:", *a, file=sys.stderr, flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -348,6 +352,7 @@ def main():
     dev_ms = run.last_run_ms()
     ms_per_step = wall * 1e3 / args.steps
     value = pts * args.steps / wall / 1e6
+    log("%d timed steps: %.3f ms/step, %.1f Mpt-steps/s" % (args.steps, ms_per_step, value))
 
     # dominant kernel: largest device time per step, per-op HIP events
     names, A = run.program()
@@ -394,16 +399,21 @@ def main():
         "top_kernels_hbm_frac": top_frac,
     }
     run.close()
+    log("per-launch profile done, dominant kernel", dom)
     if not args.no_ensemble:
         ens = ensemble_bench(args.ensemble_steps, 2, tune=not args.no_autotune)
         ens["per_point_vs_plain"] = round(ens["value"] / value, 3)
         out["ensemble"] = ens
+        log("ensemble", ens["value"], "Mpt-steps/s")
     if not args.no_adjoint:
         out["adjoint"] = adjoint_bench()
+        log("adjoint done")
     if not args.no_tornado:
         out["tornado_grid"] = tornado_bench(10, 2, tune=not args.no_autotune)
+        log("tornado grid", out["tornado_grid"]["value"], "Mpt-steps/s")
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(cfg)
+        log("cpu baseline", out["cpu_baseline"]["value"], "Mpt-steps/s")
     print(json.dumps(out), flush=True)
 
 

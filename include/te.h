@@ -166,6 +166,16 @@ typedef struct te_config {
   int32_t cache_moisture_turbulence;  /* moisture SGS fluxes computed once per step */
   int32_t n_arena_fields;             /* > 0: explicit FieldArena field list (sorted by the caller) */
   const char* const* arena_fields;
+  /* horizontal stretching (namelist stretch_x, dx_inner, dx_outer, nos_x_len,
+     tot_x_len and the y set; HorizontalGrid.stretched, web/core/grid.js:102-143):
+     0 uniform dx; 1 spacing dx_inner over the central nos_x_len (m) growing to
+     dx_outer on both sides up to tot_x_len (m); 2 the same with the growth on
+     the east side only.  nx must equal the cell count those lengths imply
+     (web/core/grid.js:117-120); dx stays the nominal spacing of the map
+     factors dx / (local spacing). */
+  int32_t stretch_x, stretch_y;
+  double dx_inner, dx_outer, nos_x_len, tot_x_len;
+  double dy_inner, dy_outer, nos_y_len, tot_y_len;
 } te_config;
 
 /* Fills a config with the synthetic 64x64x40 case defaults (dx=dy=1 km,

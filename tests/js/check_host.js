@@ -32,7 +32,7 @@ if (mode === 'plan') {
   const r = te.buildRun(JSON.parse(process.argv[3]), { encodeOnly: true });
   const prog = r.program();
   console.log(JSON.stringify({ ops: prog.names.length, names: prog.names, argsFnv: fnvWords(prog.args),
-    fields: r.arena.names.length, memWords: r.memoryWords() }));
+    fields: r.arena.names.length, memWords: r.memoryWords(), memFnv: fnvWords(r.read(r.memoryWords(), 0)) }));
 } else if (mode === 'ranges') {
   // offsets and counts that are negative, fractional or beyond 2^53 are
   // refused by the addon (RangeError) before they reach the C ABI

@@ -1,12 +1,21 @@
-"""The C++ encoder (te_encode) against the reference's recorded tape (CPU).
+"""The C++ encoder (te_encode) against the reference's recorded tapes (CPU).
 
-The supercell recording holds the reference encoder's output for step 1
-(web/core/timestep.js TimeStep.encode).  te_encode with the same options must
-produce the same dispatch sequence, launch grids and parameter bindings
-(arena field + word offset, profile/parameter buffer + offset, and the
-content of every setBytes payload) for every dispatch of the dynamics path,
-i.e. up to the microphysics tail, whose kernels are not on this path.
+The supercell recordings hold the reference encoder's output for step 1
+(web/core/timestep.js TimeStep.encode) of the 120x120x40 supercell and of the
+same storm on the 144x144x40 stretched grid (500 m over the central 40 km
+growing to 2 km at +-60 km, data/presets/supercell_stretched.json).
+te_encode with the same options must produce the same dispatch sequence
+(the stretched grid's acoustic substep count follows from its 500 m finest
+spacing), launch grids and parameter bindings (arena field + word offset,
+profile/parameter buffer + offset, and the content of every setBytes
+payload) for every dispatch of the dynamics path, i.e. up to the
+microphysics tail, whose kernels are not on this path; and the same t=0
+content of every profile and parameter buffer the recording holds
+(hmetric: the stretched grid's map factors and face positions).
 """
+import gzip
+import os
+
 import numpy as np
 import pytest
 
@@ -20,6 +29,15 @@ REGIONS = ["c1", "c2", "mh", "mf", "zf", "rho0", "rr0", "rf0", "th0", "pi0", "rt
 MASS_LABEL = {"mass_rows": "buf30", "mass_levels": "buf31", "mass_ref": "buf32", "mass_out": "buf33"}
 
 
+HERE = os.path.dirname(os.path.abspath(__file__))
+# data/presets/supercell_stretched.json: stretch_x = stretch_y = 1, dx_inner 500,
+# dx_outer 2000, nos_x_len 40 km, tot_x_len 120 km (and the same in y)
+STRETCHED = dict(nx=144, ny=144, dx=1250., dy=1250., dt=3., stretch_x=1, stretch_y=1, dx_inner=500.,
+                 dx_outer=2000., nos_x_len=40000., tot_x_len=120000., dy_inner=500., dy_outer=2000.,
+                 nos_y_len=40000., tot_y_len=120000.)
+PRESETS = {"supercell": {}, "supercell_stretched": STRETCHED}
+
+
 def supercell_config(fields, **kw):
     base = dict(nx=120, ny=120, nz=40, dx=1000., dy=1000., dz=500., dt=6., nsound=0, moist=1, species=11,
                 open_x=1, open_y=1, weno_scalars=1, weno_momentum=0, tke=1, explicit_vdiff=0,
@@ -29,13 +47,34 @@ def supercell_config(fields, **kw):
     return te_host.config(**base)
 
 
-@pytest.fixture(scope="module")
-def encoded():
-    P = golden_check.plan()
+@pytest.fixture(scope="module", params=sorted(PRESETS))
+def encoded(request):
+    P = golden_check.plan(request.param)
     fields = [n for n, _ in sorted(P.tape["fields"].items(), key=lambda x: x[1])]
-    r = te_host.Run(supercell_config(fields), encode_only=True)
+    r = te_host.Run(supercell_config(fields, **PRESETS[request.param]), encode_only=True)
     names, A = r.program()
     return P, fields, r, names, A
+
+
+@pytest.mark.parametrize("preset", sorted(PRESETS))
+def test_profile_buffers_match_recording(preset):
+    """Every recorded t=0 profile / parameter buffer of the dynamics path
+    (state/<id>.bin.gz) equals te_encode's region word for word; hmetric
+    carries the horizontal grid (map factors uh uf vh vf, faces, centres)."""
+    P = golden_check.plan(preset)
+    fields = [n for n, _ in sorted(P.tape["fields"].items(), key=lambda x: x[1])]
+    r = te_host.Run(supercell_config(fields, **PRESETS[preset]), encode_only=True)
+    compared = []
+    for b in P.tape["buffers"]:
+        name = b["label"]
+        if b["arena"] or name not in ("hmetric", "c1", "c2", "mh", "mf", "rf0", "rr0", "rho0", "tauh", "tauf"):
+            continue
+        path = os.path.join(HERE, "golden", preset, "state", "%d.bin.gz" % b["id"])
+        want = np.frombuffer(gzip.open(path).read(), dtype=np.uint32)
+        got = r.read(r.region_offset(name), want.size)
+        assert np.array_equal(got, want), (preset, name, np.flatnonzero(got != want)[:8])
+        compared.append(name)
+    assert "hmetric" in compared, compared
 
 
 def test_dispatch_sequence(encoded):

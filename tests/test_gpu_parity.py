@@ -51,6 +51,19 @@ CONFIGS = {
     "species7_ragged": dict(nx=37, ny=29, nz=17, species=7, tke=1, initial_tke=4, nsound=4, dt=4.0),
     "open_mass_conservation": dict(nx=30, ny=26, nz=16, open_x=1, open_y=1, tke=1, initial_tke=3, umove=5.0,
                                    mass_conservation=1, rayleigh=1),
+    # stretched horizontal grids: x 14 + 20 + 14 cells from 500 m to 2 km on both
+    # sides, y 24 + 16 cells stretched on the north side only; substeps from the
+    # 500 m finest spacing (nsound 8 -> 15 substeps)
+    "stretched_tke": dict(nx=48, ny=40, nz=20, nsound=0, tke=1, initial_tke=3, rayleigh=1, stretch_x=1,
+                          dx_inner=500.0, dx_outer=2000.0, nos_x_len=10000.0, tot_x_len=45000.0, stretch_y=2,
+                          dy_inner=500.0, dy_outer=2000.0, nos_y_len=12000.0, tot_y_len=32000.0),
+    # the supercell_stretched preset's options on a 36 x 36 grid (12 + 12 + 12
+    # cells per axis, 500 m centre, 2 km edges), open sides
+    "stretched_open_supercell": dict(nx=36, ny=36, nz=20, dx=1250.0, dy=1250.0, dt=3.0, nsound=0, open_x=1,
+                                     open_y=1, tke=1, initial_tke=3, umove=12.5, vmove=3.0, dissipative_heating=1,
+                                     rayleigh=1, mass_conservation=1, stretch_x=1, dx_inner=500.0, dx_outer=2000.0,
+                                     nos_x_len=6000.0, tot_x_len=36000.0, stretch_y=1, dy_inner=500.0,
+                                     dy_outer=2000.0, nos_y_len=6000.0, tot_y_len=36000.0),
 }
 
 

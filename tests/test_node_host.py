@@ -78,6 +78,28 @@ def test_preset_mapping():
     assert got["argsFnv"] == ol.fnv1a(A.reshape(-1))
 
 
+def test_stretched_preset_mapping():
+    """stretch_x / stretch_y and their lengths reach te_config from a namelist
+    preset; the encoded memory (hmetric: map factors, faces) is the C ABI's."""
+    conf = {"param0": {"nx": 48, "ny": 40, "nz": 12}, "param1": {"dx": 1000.0, "dy": 800.0, "dz": 500.0, "dtl": 6.0},
+            "param2": {"wbc": 2, "ebc": 2, "sbc": 2, "nbc": 2},
+            "param4": {"stretch_x": 1, "dx_inner": 500.0, "dx_outer": 2000.0, "nos_x_len": 10000.0,
+                       "tot_x_len": 45000.0},
+            "param5": {"stretch_y": 2, "dy_inner": 500.0, "dy_outer": 2000.0, "nos_y_len": 12000.0,
+                       "tot_y_len": 32000.0}}
+    got = json.loads(node("encode", json.dumps(conf)))
+    r = te_host.Run(te_host.config(nx=48, ny=40, nz=12, dx=1000.0, dy=800.0, open_x=1, open_y=1, stretch_x=1,
+                                   dx_inner=500.0, dx_outer=2000.0, nos_x_len=10000.0, tot_x_len=45000.0,
+                                   stretch_y=2, dy_inner=500.0, dy_outer=2000.0, nos_y_len=12000.0,
+                                   tot_y_len=32000.0), encode_only=True)
+    names, A = r.program()
+    assert got["names"] == names
+    assert got["argsFnv"] == ol.fnv1a(A.reshape(-1))
+    assert got["memFnv"] == ol.fnv1a(r.read(0, r.mem_words))
+    flat = te_host.config(nx=48, ny=40, nz=12, dx=1000.0, dy=800.0, open_x=1, open_y=1)
+    assert got["memFnv"] != ol.fnv1a(te_host.Run(flat, encode_only=True).read(0, r.mem_words))
+
+
 @pytest.mark.gpu
 def test_replay_supercell_tape_on_gpu(gpu, tmp_path):
     out = tmp_path / "hashes.bin"

@@ -80,6 +80,7 @@ class Encoder {
   Encoded E;
   std::map<Bytes, uint64_t> bytesAt;
   uint64_t dimsOff = 0, hmOff = 0;
+  double minSpacing = 0;
   float dtf = 0;
 
   uint64_t region(const std::string& n) const {
@@ -316,7 +317,12 @@ void Encoder::setup() {
   put("pi0", pr.pi0, pz); put("rth0", pr.rth0, pz); put("qv0", pr.qv0, pz); put("qc0", pr.qc0, pz);
   put("qi0", pr.qi0, pz); put("tauh", pr.tauh, pz); put("tauf", pr.tauf, pz); put("u0", pr.u0, pz);
   put("v0", pr.v0, pz);
-  put("hmetric", make_metric(nx, ny, cfg.dx, cfg.dy), 0);
+  {
+    const HAxis hx = axis_x(cfg), hy = axis_y(cfg);
+    // AcousticSchedule's minimum horizontal spacing (timestep.js:403)
+    minSpacing = std::min(hx.min_spacing, hy.min_spacing);
+    put("hmetric", make_metric(hx, hy), 0);
+  }
   hmOff = L.region["hmetric"];
   put("radbcw", {}, std::max(ny, 1) + 8); put("radbce", {}, std::max(ny, 1) + 8);
   put("radbcs", {}, std::max(nx, 1) + 8); put("radbcn", {}, std::max(nx, 1) + 8);
@@ -709,7 +715,7 @@ void Encoder::stages() {
   const int nx = cfg.nx, ny = cfg.ny, nz = cfg.nz;
   const std::array<int, 3> full{nx, ny, nz}, whole{nx + 7, ny + 7, nz + 1}, stag{nx + 1, ny + 1, nz}, cols{nx, ny, 1};
   const Bytes open = I2(openX ? 1 : 0, openY ? 1 : 0);
-  const int ns = cfg.nsound > 0 ? cfg.nsound : schedule_nsound(dtf, std::min(cfg.dx, cfg.dy));
+  const int ns = cfg.nsound > 0 ? cfg.nsound : schedule_nsound(dtf, minSpacing);
   const Schedule sch = make_schedule(ns, dtf);
   const char* ks[] = {"qv", "qc", "qr", "qi", "qs", "qg"};
   for (int nrk = 0; nrk < 3; ++nrk) {

@@ -9,6 +9,7 @@
 #include <cmath>
 #include <cstring>
 #include <stdexcept>
+#include <string>
 
 #include "te_scenario.hpp"
 
@@ -218,17 +219,127 @@ Profiles make_profiles(const te_config& cfg, const Vertical& v, const Base& b) {
   return p;
 }
 
-// Metric buffer of a uniform, centred horizontal grid (grid.js:77-85,200-214).
-std::vector<float> make_metric(int nx, int ny, float dx, float dy) {
-  const int sJ = nx + 7, sK = ny + 7;
+// One horizontal axis with its origin at the domain centre (iorigin = 2).
+// Uniform: faces i dx - n dx / 2 (grid.js:77-85).  Stretched (grid.js:102-143,
+// CM1 param.F stretch_x = 1 / 2): a central zone of ni2 cells of `inner`
+// flanked by ni1 (west) and ni3 (east) cells whose spacing grows
+// quadratically from `inner` to `outer` with mean `nominal`; halo faces
+// continue at `outer` (at `inner` on the unstretched west side of
+// stretch 2).  The map factors are dx over the local spacing: uh at centres,
+// uf at faces, 1 on the outermost faces (and copied inwards on open sides).
+// The face positions are the reference's double expressions, operation for
+// operation, because their binary32 roundings are kernel inputs.
+HAxis make_axis(int n, double dx, int stretch, double inner, double outer, double nos, double tot, bool open_lo,
+                bool open_hi, const char* name) {
+  const int H = 3, W = n + 2 * H + 1;  // halo index -3 .. n+3
+  HAxis a;
+  if (stretch == 0) {
+    const double half = 0.5 * (n * dx);
+    for (int i = -H; i <= n + H; ++i) {
+      a.xf.push_back(i * dx - half);
+      a.xh.push_back((i + 0.5) * dx - half);
+    }
+    a.uh.assign(W, 1.0);
+    a.uf.assign(W, 1.0);
+    a.min_spacing = dx;
+    return a;
+  }
+  const std::string what = std::string(name);
+  if (stretch != 1 && stretch != 2) throw std::runtime_error(what + ": stretch must be 0, 1 or 2");
+  if (!(inner > 0 && outer > 0 && nos >= 0 && tot > nos))
+    throw std::runtime_error(what + ": stretching needs inner, outer > 0 and total length > unstretched length");
+  const double nominal = 0.5 * (inner + outer);
+  auto rha = [](double x) { return x < 0 ? -std::round(-x) : std::round(x); };
+  int n1, n2, n3;
+  if (stretch == 1) {
+    n1 = (int)rha((tot - nos) * 0.5 / nominal);
+    n2 = (int)rha(nos / inner);
+    n3 = n1;
+  } else {
+    n1 = 0;
+    n2 = (int)rha(nos / inner);
+    n3 = (int)rha((tot - nos) / nominal);
+  }
+  if (!(n == n1 + n2 + n3 && n1 >= 0 && n2 >= 0 && n3 >= 0))
+    throw std::runtime_error(what + ": " + std::to_string(n) + " points given, the stretching lengths need " +
+                             std::to_string(n1) + " + " + std::to_string(n2) + " + " + std::to_string(n3) +
+                             " (CM1 stops here, param.F:6805)");
+  if (n3 < 2) throw std::runtime_error(what + ": the stretched zone needs at least 2 cells");
+  const double c2 = (nominal - inner) / (nominal * nominal * (n3 - 1));
+  const double c1 = (inner / nominal) - c2 * nominal;
+  const double shift = 0.5 * tot;
+  // F[i] = face with CM1 index i (1-based interior faces 1..n+1), i = 1-H .. n+1+H
+  std::vector<double> Fv(n + 2 * H + 2, 0.0);
+  auto F = [&](int i) -> double& { return Fv[(size_t)(i + H - 1)]; };
+  for (int i = n1 + 1; i <= n1 + n2 + 1; ++i) F(i) = n1 * nominal + (i - n1 - 1) * inner - shift;
+  const int east_end = n1 + n2 + n3 + (stretch == 1 ? H + 1 : H);
+  for (int i = n1 + n2 + 2; i <= east_end; ++i) {
+    const double m = (i - 1 - n1 - n2) * nominal;
+    F(i) = n1 * nominal + n2 * inner + (c1 + c2 * m) * m - shift;
+  }
+  for (int i = 1 - H; i <= n1; ++i) {
+    const double m = (n1 + 1 - i) * nominal;
+    F(i) = n1 * nominal - (c1 + c2 * m) * m - shift;
+  }
+  const double west = stretch == 1 ? outer : inner;
+  for (int i = 1; i <= H; ++i) {
+    F(1 - i) = F(1) - i * west;
+    F(n + 1 + i) = F(n + 1) + i * outer;
+  }
+  // HorizontalGrid.derive (grid.js:145-164)
+  for (int i = -H; i <= n + H; ++i) a.xf.push_back(F(i + 1));
+  for (int i = -H; i <= n - 1 + H; ++i) {
+    const double lo = F(i + 1), hi = F(i + 2);
+    a.xh.push_back(0.5 * (hi + lo));
+    a.uh.push_back(dx / (hi - lo));
+  }
+  a.xh.push_back(a.xh.back() + (F(n + 1 + H) - F(n + H)));
+  a.uh.push_back(a.uh.back());
+  a.uf.push_back(1.0);
+  for (int i = -H + 1; i <= n + H - 1; ++i) {
+    const double c = 0.5 * (F(i + 1) + F(i + 2)), cm = 0.5 * (F(i) + F(i + 1));
+    a.uf.push_back(dx / (c - cm));
+  }
+  a.uf.push_back(1.0);
+  if (open_lo)
+    for (int i = 0; i < H; ++i) a.uf[i] = a.uf[H];
+  if (open_hi)
+    for (int i = n + 1 + H; i < W; ++i) a.uf[i] = a.uf[n + H];
+  a.uniform = false;
+  a.min_spacing = INFINITY;
+  for (int i = 0; i < n; ++i) a.min_spacing = std::min(a.min_spacing, a.xf[i + 1 + H] - a.xf[i + H]);
+  return a;
+}
+
+HAxis axis_x(const te_config& c) {
+  return make_axis(c.nx, c.dx, c.stretch_x, c.dx_inner, c.dx_outer, c.nos_x_len, c.tot_x_len, c.open_x != 0,
+                   c.open_x != 0, "stretch_x");
+}
+HAxis axis_y(const te_config& c) {
+  return make_axis(c.ny, c.dy, c.stretch_y, c.dy_inner, c.dy_outer, c.nos_y_len, c.tot_y_len, c.open_y != 0,
+                   c.open_y != 0, "stretch_y");
+}
+
+// Metric buffer (Grid.metricValues, grid.js:200-214): uh | uf (strideJ) |
+// vh | vf (strideK) | xf (strideJ) | yf (strideK) | axisymmetric terms
+// arh1 arh2 arf1 arf2 (1), rxh (0) and xh (strideJ each).
+std::vector<float> make_metric(const HAxis& x, const HAxis& y) {
+  const int sJ = (int)x.xf.size(), sK = (int)y.xf.size();
   std::vector<float> m((size_t)9 * sJ + 3 * sK, 1.0f);
-  auto xf = [](int n, double d, int i) { return i * d - 0.5 * n * d; };
-  for (int i = 0; i < sJ; ++i) m[2 * sJ + 2 * sK + i] = f32(xf(nx, dx, i - 3));
-  for (int j = 0; j < sK; ++j) m[3 * sJ + 2 * sK + j] = f32(xf(ny, dy, j - 3));
+  for (int i = 0; i < sJ; ++i) {
+    m[i] = f32(x.uh[i]);
+    m[sJ + i] = f32(x.uf[i]);
+    m[2 * sJ + 2 * sK + i] = f32(x.xf[i]);
+  }
+  for (int j = 0; j < sK; ++j) {
+    m[2 * sJ + j] = f32(y.uh[j]);
+    m[2 * sJ + sK + j] = f32(y.uf[j]);
+    m[3 * sJ + 2 * sK + j] = f32(y.xf[j]);
+  }
   const int base = 3 * sJ + 3 * sK;
   for (int i = 0; i < sJ; ++i) {
-    m[base + 4 * sJ + i] = 0.0f;                                      // rxh (axisymmetric only)
-    m[base + 5 * sJ + i] = f32((i - 3 + 0.5) * dx - 0.5 * nx * dx);   // xh
+    m[base + 4 * sJ + i] = 0.0f;  // rxh (axisymmetric only)
+    m[base + 5 * sJ + i] = f32(x.xh[i]);
   }
   return m;
 }
@@ -259,12 +370,12 @@ void fill_state(const te_config& cfg, const Vertical& v, const Base& b, const La
   float *th = field("th"), *tha = field("th_a"), *u = field("u"), *ua = field("u_a"), *vv = field("v"), *va = field("v_a"),
         *prs = field("prs"), *rho = field("rho"), *qv = field("qv"), *qva = field("qv_a"), *qc = field("qc"),
         *qca = field("qc_a");
-  const double lx = nx * (double)cfg.dx, ly = ny * (double)cfg.dy;
+  const HAxis ax = axis_x(cfg), ay = axis_y(cfg);
   std::vector<float> thp((size_t)sJ * sK * (nz + 1), 0.0f);
   for (int k = 0; k < nz; ++k)
     for (int j = 0; j < ny; ++j)
       for (int i = 0; i < nx; ++i) {
-        const double x = (i + 0.5) * cfg.dx - 0.5 * lx, y = (j + 0.5) * cfg.dy - 0.5 * ly, z = v.zh[k];
+        const double x = ax.xh[i + 3], y = ay.xh[j + 3], z = v.zh[k];
         const double bx = x / cfg.bubble_rh, by = y / cfg.bubble_rh, bz = (z - cfg.bubble_zc) / cfg.bubble_rv;
         const double beta = std::sqrt(bx * bx + by * by + bz * bz);
         double t = 0.0;

@@ -24,7 +24,19 @@ struct Profiles {
 Vertical make_vertical(const std::vector<double>& faces, double nominal_dz);
 Base make_base(const te_config& cfg, const Vertical& v);
 Profiles make_profiles(const te_config& cfg, const Vertical& v, const Base& b);
-std::vector<float> make_metric(int nx, int ny, float dx, float dy);
+// One horizontal axis: n + 7 values from halo index -3 of the faces xf,
+// centres xh and map factors uh (centres) and uf (faces); uniform or CM1's
+// stretched layout (HorizontalGrid, web/core/grid.js:52-172).
+struct HAxis {
+  std::vector<double> xf, xh, uh, uf;
+  bool uniform = true;
+  double min_spacing = 0;  // smallest interior face spacing (m)
+};
+HAxis make_axis(int n, double dx, int stretch, double inner, double outer, double nos, double tot, bool open_lo,
+                bool open_hi, const char* name);
+HAxis axis_x(const te_config& cfg);
+HAxis axis_y(const te_config& cfg);
+std::vector<float> make_metric(const HAxis& x, const HAxis& y);
 void fill_state(const te_config& cfg, const Vertical& v, const Base& b, const Layout& L, uint32_t* M);
 
 }  // namespace te

@@ -143,7 +143,8 @@ class Replayer {
 // grid (param0/param1), dt (dtl), boundary codes (wbc/ebc: 1 periodic,
 // 2 open), WENO (advwenos/advwenov = 2), TKE (sgsmodel 1), implicit vertical
 // diffusion (doimpl), Rayleigh damping (irdamp, zd, rdalpha), idiss, imove
-// (umove, vmove), kdiv, alph, imoist and the explicit nsound (psolver 3).
+// (umove, vmove), kdiv, alph, imoist, the explicit nsound (psolver 3) and
+// horizontal stretching (stretch_x/y, dx/dy_inner, _outer, nos_/tot_x/y_len).
 function presetToConfig(preset, overrides) {
   const g = k => { for (const grp of Object.keys(preset || {})) { const v = preset[grp]; if (v && typeof v === 'object' && k in v) return v[k]; } return undefined; };
   const c = {};
@@ -167,6 +168,12 @@ function presetToConfig(preset, overrides) {
   if (g('ptype') !== undefined && g('ptype') !== 0) c.microphysics = g('ptype');
   if (g('apmasscon') !== undefined) c.mass_conservation = g('apmasscon');
   set('nsound', g('nsound'));
+  for (const a of ['x', 'y']) {              // stretch_x / stretch_y (config.js:562-573)
+    if (g('stretch_' + a) === undefined || g('stretch_' + a) === 0) continue;
+    c['stretch_' + a] = g('stretch_' + a);
+    set('d' + a + '_inner', g('d' + a + '_inner')); set('d' + a + '_outer', g('d' + a + '_outer'));
+    set('nos_' + a + '_len', g('nos_' + a + '_len')); set('tot_' + a + '_len', g('tot_' + a + '_len'));
+  }
   return Object.assign(c, overrides || {});
 }
 

@@ -115,7 +115,7 @@ napi_value wrap(napi_env env, Holder* h) {
 }
 
 // ---- te_config <-> JS object
-enum Kind { I32, F32, U64 };
+enum Kind { I32, F32, U64, F64 };
 struct FieldDesc { const char* name; Kind kind; size_t off; };
 #define FD(n, k) {#n, k, offsetof(te_config, n)}
 const FieldDesc kFields[] = {
@@ -127,6 +127,8 @@ const FieldDesc kFields[] = {
     FD(hodo_depth, F32), FD(shear_umax, F32), FD(shear_depth, F32), FD(bubble_amp, F32), FD(bubble_rh, F32),
     FD(bubble_rv, F32), FD(bubble_zc, F32), FD(noise_sigma, F32), FD(seed, U64), FD(initial_tke, I32), FD(fuse, I32), FD(sounding_seed, U64), FD(th_pert_sigma, F32), FD(qv_pert_sigma, F32),
     FD(species, I32), FD(microphysics, I32), FD(mass_conservation, I32), FD(cache_moisture_turbulence, I32),
+    FD(stretch_x, I32), FD(stretch_y, I32), FD(dx_inner, F64), FD(dx_outer, F64), FD(nos_x_len, F64),
+    FD(tot_x_len, F64), FD(dy_inner, F64), FD(dy_outer, F64), FD(nos_y_len, F64), FD(tot_y_len, F64),
 };
 #undef FD
 
@@ -135,8 +137,9 @@ napi_value config_to_js(napi_env env, const te_config& c) {
   NAPI_OK(napi_create_object(env, &o));
   for (const FieldDesc& f : kFields) {
     const char* p = reinterpret_cast<const char*>(&c) + f.off;
-    double d = f.kind == I32 ? *reinterpret_cast<const int32_t*>(p)
+    double d = f.kind == I32   ? *reinterpret_cast<const int32_t*>(p)
                : f.kind == F32 ? *reinterpret_cast<const float*>(p)
+               : f.kind == F64 ? *reinterpret_cast<const double*>(p)
                                : (double)*reinterpret_cast<const uint64_t*>(p);
     NAPI_OK(napi_set_named_property(env, o, f.name, make_num(env, d)));
   }
@@ -172,6 +175,7 @@ bool config_from_js(napi_env env, napi_value o, te_config* c, std::vector<float>
     char* p = reinterpret_cast<char*>(c) + f.off;
     if (f.kind == I32) *reinterpret_cast<int32_t*>(p) = (int32_t)d;
     else if (f.kind == F32) *reinterpret_cast<float*>(p) = (float)d;
+    else if (f.kind == F64) *reinterpret_cast<double*>(p) = d;
     else *reinterpret_cast<uint64_t*>(p) = (uint64_t)d;
   }
   bool has = false;

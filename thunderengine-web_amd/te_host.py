@@ -54,6 +54,11 @@ class Config(ctypes.Structure):
         ("species", ctypes.c_int32), ("microphysics", ctypes.c_int32), ("mass_conservation", ctypes.c_int32),
         ("cache_moisture_turbulence", ctypes.c_int32), ("n_arena_fields", ctypes.c_int32),
         ("arena_fields", ctypes.POINTER(ctypes.c_char_p)),
+        ("stretch_x", ctypes.c_int32), ("stretch_y", ctypes.c_int32),
+        ("dx_inner", ctypes.c_double), ("dx_outer", ctypes.c_double), ("nos_x_len", ctypes.c_double),
+        ("tot_x_len", ctypes.c_double),
+        ("dy_inner", ctypes.c_double), ("dy_outer", ctypes.c_double), ("nos_y_len", ctypes.c_double),
+        ("tot_y_len", ctypes.c_double),
     ]
 
 
