@@ -176,6 +176,14 @@ typedef struct te_config {
   int32_t stretch_x, stretch_y;
   double dx_inner, dx_outer, nos_x_len, tot_x_len;
   double dy_inner, dy_outer, nos_y_len, tot_y_len;
+  /* adapt_dt = 1 (web/core/adaptive.js, web/core/driver.js:36-50): the step
+     ends with the stability maxima (stability_max) and te_adaptive_start /
+     te_adaptive_step choose each large step's dt and nsound from them; dt
+     above is the namelist dtl; tapfrq, statfrq, rstfrq (s) are the output
+     clocks whose times the steps land on (0: none).  Needs tke = 1
+     (web/core/config.js:725-729). */
+  int32_t adaptive_dt;
+  float tapfrq, statfrq, rstfrq;
 } te_config;
 
 /* Fills a config with the synthetic 64x64x40 case defaults (dx=dy=1 km,
@@ -215,6 +223,49 @@ int te_region_offset(const te_ctx* ctx, const char* name, uint64_t* word_off);
 
 /* Grid of a config context: nx ny nz strideJ strideK. */
 int te_grid(const te_ctx* ctx, int32_t out5[5]);
+
+/* ---- variable large step (adapt_dt = 1) */
+
+/* TimeStep.setTimeStep (timestep.js:711-716): the step encoded anew for
+ * large step dt (s) and acoustic parameter nsound, the state kept.  When the
+ * dispatch list is unchanged (same nsound) only the parameter words change
+ * and the step graph is kept; otherwise the new list replaces the old (and
+ * launch shapes chosen by te_autotune are dropped). */
+int te_set_time_step(te_ctx* ctx, float dt, int32_t nsound);
+
+/* AdaptiveTimeStep (web/core/adaptive.js:15-109): CM1's getnewdt and
+ * dtsmall (misclibs.F:543-847) and output clocks, in the reference's
+ * mixed binary32 / double arithmetic.  freq[6]: tape, restart, stat,
+ * domainDiag, azimAvg, hifrq frequencies (s; NaN = clock absent);
+ * min_dz > 0 only with explicit vertical acoustics (psolver = 2). */
+typedef struct te_clock te_clock;
+typedef struct te_clock_state {
+  double mtime;      /* model time (s) */
+  double dbldt;      /* the next step (s, double) */
+  float dt, dtlast;  /* the next and the previous step (s) */
+  float cflmax, ksmax; /* last maxima, scaled to the new dt */
+  int32_t nsound;    /* acoustic parameter of the next step */
+  int32_t steps;     /* steps begun */
+  uint32_t fired;    /* bit n: clock kind n fired this step (0 tape .. 5 hifrq) */
+} te_clock_state;
+te_clock* te_clock_create(double dtl, double min_dx, double min_dz, int32_t two_dimensional, int32_t ks_check,
+                          const float freq[6]);
+void te_clock_start(te_clock* c, float cflmax, float ksmax);  /* start(): t = 0 maxima */
+void te_clock_begin_step(te_clock* c);                         /* beginStep() */
+void te_clock_end_step(te_clock* c, float cflmax, float ksmax); /* endStep(): the step's maxima */
+void te_clock_get(const te_clock* c, te_clock_state* out);
+void te_clock_destroy(te_clock* c);
+
+/* The driver's adaptive run (driver.js:43-49, view.js:513-550) on a context
+ * created with adaptive_dt = 1.  te_adaptive_start: the stability maxima of
+ * the current state (stability_max alone, encodeInitial) start the clock and
+ * set the first step.  te_adaptive_step: n times beginStep, one large step,
+ * read and clear the two stability words (stabilityFromWords: square roots
+ * of the maxima), endStep, te_set_time_step with the clock's dt and nsound.
+ * te_adaptive_clock: the clock's state. */
+int te_adaptive_start(te_ctx* ctx);
+int te_adaptive_step(te_ctx* ctx, int32_t nsteps);
+int te_adaptive_clock(const te_ctx* ctx, te_clock_state* out);
 
 /* ---- ensembles: members of uneven grids and perturbed soundings stepped
  * together.  No counterpart in the reference (one run per page there).  Each

@@ -7,6 +7,7 @@
 #ifdef _OPENMP
 #include <omp.h>
 #endif
+#include <algorithm>
 #include <array>
 #include <cstdio>
 #include <cstring>
@@ -41,7 +42,7 @@ static const KernelEntry kTable[] = {
     KE(turb_dissipative_heating), KE(turb_gettau), KE(turb_sfcstress), KE(tke_epst), KE(tke_sources),
     KE(turb_u_tendency), KE(turb_v_tendency), KE(turb_w_tendency), KE(turb_scalar_flux_h),
     KE(turb_scalar_vert_implicit), KE(tke_flux_h), KE(tke_transport), KE(mass_row_sums), KE(mass_level_sums),
-    KE(mass_reference_init), KE(mass_correction_apply),
+    KE(mass_reference_init), KE(mass_correction_apply), KE(stability_max),
 };
 #undef KE
 #undef KT

@@ -36,7 +36,8 @@ def test_preset_step1_region_hashes(preset):
 
 def test_kernels_of_the_path_are_all_in_the_tape():
     """Every kernel the oracle implements is exercised by the golden tape,
-    except the options the supercell run does not use."""
+    except the options the supercell run does not use (stability_max runs
+    only with adapt_dt = 1; its pins are in test_adaptive.py)."""
     P = golden_check.plan()
     used = {o["name"] for o in P.ops}
     have = set(ol.kernel_names())
@@ -46,5 +47,5 @@ def test_kernels_of_the_path_are_all_in_the_tape():
                "adv_flux_z_stag", "adv_flux_z_weno_w", "adv_flux_x_weno_uv", "adv_flux_y_weno_uv",
                "tke_epst", "diabatic_unapply", "pdef_dum", "pdef_cell", "pdef_mass", "pdef_apply",
                "velocity_to_massflux", "stage_state_reset", "turb_sfcstress", "update_diagnostics",
-               "adv_flux_z_weno_uv"}
+               "adv_flux_z_weno_uv", "stability_max"}
     assert unused <= allowed, sorted(unused - allowed)

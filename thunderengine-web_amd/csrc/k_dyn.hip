@@ -413,4 +413,60 @@ __global__ __launch_bounds__(256) void k_update_rho_prs(uint32_t* M, KArg a) {
   rho[c] = p / ((th * pi) * rmix);
 }
 
+// stability_max.wgsl:120-178 (adapt_dt = 1): max over the grid of the
+// squared advective Courant number c2 = x^2 + y^2 + z^2 (x = (u_i + u_i+1)
+// dt / (2 dx) uh, and alike) and of the squared diffusion number of kmh,
+// khh, kmv at levels k >= 1, into two words: the bits of non-negative floats
+// order like the floats, so an unsigned max of the bits is the float max
+// and the result does not depend on the order of the workgroups.  Each
+// workgroup reduces in LDS and adds one global atomic per word.
+__global__ __launch_bounds__(256) void k_stability_max(uint32_t* M, KArg a) {
+  __shared__ uint32_t red[2];
+  const uint32_t t = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
+  if (t == 0) {
+    red[0] = 0u;
+    red[1] = 0u;
+  }
+  __syncthreads();
+  int gx, gy, gz;
+  te_tid3(a, gx, gy, gz);
+  const Dims g = dims(a.q[0], 0);
+  float c2 = 0.0f, m = 0.0f;
+  if (gx < (int)a.n[0] && gy < (int)a.n[1] && gz < (int)a.n[2] && gx < g.nx && gy < g.ny && gz < g.nz) {
+    const uint32_t* P = a.q[1];
+    const float dtdx = ldf(P, 0), dtdy = ldf(P, 1), dtdz = ldf(P, 2), ksdx = ldf(P, 3), ksdy = ldf(P, 4);
+    const int twoD = ldi(P, 5), withKs = ldi(P, 6);
+    const float *u = fq(a.q[2]), *v = fq(a.q[3]), *w = fq(a.q[4]), *mh = fq(a.q[5]), *hm = fq(a.q[30]);
+    const int64_t c = g.idx(gx, gy, gz);
+    const float uh = hm_uh(hm, g, gx), vh = hm_vh(hm, g, gy);
+    const float x = ((u[c] + u[c + 1]) * dtdx) * uh;
+    const float z = ((w[c] + w[c + g.sk()]) * dtdz) * mh[gz];
+    if (twoD != 0) {
+      c2 = fma_(x, x, z * z);
+    } else {
+      const float y = ((v[c] + v[c + g.sJ]) * dtdy) * vh;
+      c2 = fma_(z, z, fma_(x, x, y * y));
+    }
+    if (withKs != 0 && gz >= 1) {
+      const float K[3] = {fq(a.q[6])[c], fq(a.q[8])[c], fq(a.q[7])[c]};  // kmh, khh, kmv
+      for (int n = 0; n < 3; ++n) {
+        const float aa = ((K[n] * ksdx) * uh) * uh;
+        float tt = aa;
+        if (twoD == 0) {
+          const float bb = ((K[n] * ksdy) * vh) * vh;
+          tt = fma_(aa, aa, bb * bb);
+        }
+        m = mx(m, tt);
+      }
+    }
+  }
+  atomicMax(&red[0], __float_as_uint(c2));
+  atomicMax(&red[1], __float_as_uint(m));
+  __syncthreads();
+  if (t == 0) {
+    atomicMax(a.q[9], red[0]);
+    atomicMax(a.q[9] + 1, red[1]);
+  }
+}
+
 }  // namespace te

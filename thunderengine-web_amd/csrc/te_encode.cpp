@@ -64,6 +64,7 @@ struct Bind {
 class Encoder {
  public:
   explicit Encoder(const te_config& c) : cfg(c) {}
+  bool withArena = true;  // false: the image leaves the arena out (Encoded::skipped)
   Encoded run();
 
  private:
@@ -103,7 +104,7 @@ class Encoder {
   uint64_t bytes(const Bytes& w) {
     auto it = bytesAt.find(w);
     if (it != bytesAt.end()) return it->second;
-    uint64_t o = E.image.size();
+    uint64_t o = E.image.size() + E.skipped;
     E.image.insert(E.image.end(), w.begin(), w.end());
     while (E.image.size() % 4) E.image.push_back(0u);
     E.image.push_back(0u); E.image.push_back(0u); E.image.push_back(0u); E.image.push_back(0u);
@@ -257,6 +258,8 @@ class Encoder {
 void Encoder::setup() {
   const int nx = cfg.nx, ny = cfg.ny, nz = cfg.nz;
   if (nx < 4 || ny < 4 || nz < 4) throw std::runtime_error("grid must be at least 4 x 4 x 4");
+  if (cfg.adaptive_dt && !cfg.tke)
+    throw std::runtime_error("adaptive_dt needs tke = 1 (ksmax comes from the TKE scheme's K, config.js:725-729)");
   if (!(cfg.dx > 0 && cfg.dy > 0 && cfg.dt > 0)) throw std::runtime_error("dx, dy and dt must be positive");
   moist = cfg.moist != 0;
   if (moist) {
@@ -321,11 +324,13 @@ void Encoder::setup() {
     const HAxis hx = axis_x(cfg), hy = axis_y(cfg);
     // AcousticSchedule's minimum horizontal spacing (timestep.js:403)
     minSpacing = std::min(hx.min_spacing, hy.min_spacing);
+    E.L.min_spacing = minSpacing;
     put("hmetric", make_metric(hx, hy), 0);
   }
   hmOff = L.region["hmetric"];
   put("radbcw", {}, std::max(ny, 1) + 8); put("radbce", {}, std::max(ny, 1) + 8);
   put("radbcs", {}, std::max(nx, 1) + 8); put("radbcn", {}, std::max(nx, 1) + 8);
+  if (cfg.adaptive_dt) put("stability", {}, 2);  // TimeStep.stabilityb (timestep.js:706-709)
   put("sfc", {}, (size_t)11 * L.sJ * L.sK);
   if (massCons) {
     put("mass_rows", {}, (size_t)2 * L.sK * nz * 2 * 2);
@@ -368,7 +373,12 @@ void Encoder::setup() {
   }
   L.fields = fl;
   L.arena = (E.image.size() + 63) / 64 * 64;
-  E.image.resize(L.arena + L.field_words * fl.size(), 0u);
+  if (withArena) {
+    E.image.resize(L.arena + L.field_words * fl.size(), 0u);
+  } else {  // the image holds the words below the arena and the payloads above it
+    E.image.resize(L.arena, 0u);
+    E.skipped = L.field_words * fl.size();
+  }
 }
 
 void Encoder::openHalo(const std::string& n) {
@@ -899,6 +909,14 @@ void Encoder::end() {
   flushAdd();
   flushHalo();
   flushZero();
+  if (cfg.adaptive_dt) {  // encodeStability (timestep.js:742-764): the step's stability maxima
+    const float d = dtf, rdx = 1.0f / cfg.dx, rdy = 1.0f / cfg.dy, rdz = (float)(1.0 / v.dz_nominal);
+    op("stability_max", full,
+       {bb(1, {fbits((0.5f * d) * rdx), fbits((0.5f * d) * rdy), fbits((0.5f * d) * rdz), fbits((d * rdx) * rdx),
+               fbits((d * rdy) * rdy), 0u, tke ? 1u : 0u}),
+        fb(2, "u"), fb(3, "v"), fb(4, "w"), rb(5, "mh"), fb(6, tke ? "kmh" : "u"), fb(7, tke ? "kmv" : "u"),
+        fb(8, tke ? "khh" : "u"), rb(9, "stability")});
+  }
   // state swap (timestep.js:766-774)
   const uint64_t count = (uint64_t)E.L.sJ * E.L.sK * (nz + 1);
   std::vector<std::pair<std::string, std::string>> pairs;
@@ -955,15 +973,16 @@ Encoded Encoder::run() {
   }
   stages();
   end();
-  E.L.words = (E.image.size() + 63) / 64 * 64;
-  E.image.resize(E.L.words, 0u);
+  E.L.words = (E.image.size() + E.skipped + 63) / 64 * 64;
+  E.image.resize(E.L.words - E.skipped, 0u);
   return std::move(E);
 }
 
 }  // namespace
 
-Encoded encode_step(const te_config& cfg) {
+Encoded encode_step(const te_config& cfg, bool with_arena) {
   Encoder e(cfg);
+  e.withArena = with_arena;
   return e.run();
 }
 

@@ -35,6 +35,7 @@ struct Layout {
   std::vector<std::string> fields;          // arena field names, sorted
   std::map<std::string, uint64_t> region;   // profile/parameter regions by name
   int nx = 0, ny = 0, nz = 0, sJ = 0, sK = 0;
+  double min_spacing = 0;  // finest horizontal face spacing (m; the acoustic schedule's)
   uint64_t field(const std::string& n) const;
 };
 
@@ -43,11 +44,14 @@ struct Layout {
 struct Encoded {
   Layout L;
   std::vector<uint32_t> image;
+  // words of the arena left out of `image` (encode_step without the arena):
+  // word w >= L.arena + skipped of M is image[w - skipped]
+  uint64_t skipped = 0;
   std::vector<Op> ops;
   std::vector<std::string> names;  // kernel name of each op (also for kernels outside this path)
 };
 
-Encoded encode_step(const te_config& cfg);
+Encoded encode_step(const te_config& cfg, bool with_arena = true);
 // Fills the arena part of `image` with the t=0 state of the scenario.
 void fill_initial_state(const te_config& cfg, const Layout& L, std::vector<uint32_t>& image);
 
@@ -96,4 +100,9 @@ struct te_ctx {
   // adjoints built on this context (te_adjoint_create); te_destroy detaches
   // them so that their later calls fail instead of using freed state
   std::vector<te_adjoint*> adjoints;
+  // adapt_dt = 1: the namelist dtl, the clock (te_adaptive_start) and the
+  // stability words' offset
+  float dtl = 0.0f;
+  te_clock* clock = nullptr;
+  uint64_t stab_off = 0;
 };

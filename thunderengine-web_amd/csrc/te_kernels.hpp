@@ -18,6 +18,7 @@
   X(turb_dissipative_heating) X(turb_gettau) X(turb_sfcstress) X(tke_epst) X(tke_sources) X(turb_u_tendency)   \
   X(turb_v_tendency) X(turb_w_tendency) X(turb_scalar_flux_h) X(turb_scalar_vert_implicit) X(tke_flux_h)       \
   X(tke_transport) X(mass_row_sums) X(mass_level_sums) X(mass_reference_init) X(mass_correction_apply) X(_copy)  \
+  X(stability_max) \
   TE_FUSED_LIST(X)
 
 // Launches of this build with no counterpart in the reference's kernel list:

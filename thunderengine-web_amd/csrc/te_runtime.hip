@@ -4,10 +4,12 @@
 // word addressed, one hipMalloc) and launches the program's kernels in order on
 // one HIP stream.  A full step is captured once into a HIP graph and replayed.
 #include <algorithm>
+#include <cmath>
 #include <cstdlib>
 #include <array>
 #include <cstdio>
 #include <cstring>
+#include <limits>
 #include <map>
 #include <mutex>
 #include <set>
@@ -772,6 +774,7 @@ const char* te_last_error(const te_ctx* c) { return c ? c->err.c_str() : "null c
 void te_destroy(te_ctx* c) {
   if (!c) return;
   for (te_adjoint* a : c->adjoints) te::adjoint_detach(a);
+  te_clock_destroy(c->clock);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   if (c->graph) (void)hipGraphExecDestroy(c->graph);
   if (c->dBase) (void)hipFree(c->dBase);
@@ -795,8 +798,11 @@ int te_encode(const te_config* cfg, te_ctx** out) {
     c->cfg.zf = c->zf.data();
   }
   c->cfg.arena_fields = nullptr;  // the caller's list is consumed by the encoder only
+  te_config first = *cfg;
+  if (cfg->adaptive_dt) first.nsound = 4;  // setTimeStep(dtl, 4) before the clock starts (driver.js:41)
+  c->cfg.nsound = first.nsound;
   try {
-    te::Encoded e = te::encode_step(*cfg);
+    te::Encoded e = te::encode_step(first);
     c->layout = std::move(e.L);
     c->image = std::move(e.image);
     c->ops = std::move(e.ops);
@@ -805,6 +811,23 @@ int te_encode(const te_config* cfg, te_ctx** out) {
     return fail(c, TE_ERR_UNSUPPORTED, ex.what());
   }
   c->words = c->prog_words = c->layout.words;
+  if (cfg->adaptive_dt) {
+    // parameter words for any later (dt, nsound): the most substeps, and slack
+    // for payloads that coincide for one dt and not for another
+    c->dtl = cfg->dt;
+    c->stab_off = c->layout.region.at("stability");
+    te_config most = first;
+    most.nsound = 192;
+    uint64_t words = c->layout.words;
+    try {
+      words = std::max<uint64_t>(words, te::encode_step(most, false).L.words);
+    } catch (const std::exception& ex) {
+      return fail(c, TE_ERR_UNSUPPORTED, ex.what());
+    }
+    words += 4096;
+    c->image.resize(words, 0u);
+    c->words = c->prog_words = words;
+  }
   return TE_OK;
 }
 
@@ -817,9 +840,139 @@ int te_create(const te_config* cfg, te_ctx** out) {
       return fail(c, TE_ERR_UNSUPPORTED, "kernel '" + c->op_names[i] + "' is not on this path (microphysics)");
   rc = validate_ops(c);
   if (rc != TE_OK) return rc;
-  rc = ctx_alloc(c, c->layout.words);
+  rc = ctx_alloc(c, c->prog_words);
   if (rc != TE_OK) return rc;
   return te_upload_words(c, 0, c->image.data(), c->image.size());
+}
+
+// TimeStep.setTimeStep and the next encode (timestep.js:711-716): the step
+// encoded for (dt, nsound) without its arena; the parameter words above the
+// arena replace the old ones in the image and on the device, and the
+// dispatch list replaces the old one when it differs (then the graph, the
+// guard sizes and the tuned shapes are rebuilt).
+static int retime(te_ctx* c, float dt, int32_t nsound) {
+  if (!c || !c->has_config) return fail(c, TE_ERR_STATE, "not a config context");
+  if (!c->adjoints.empty()) return fail(c, TE_ERR_STATE, "the time step of a context with adjoints is fixed");
+  if (!(dt > 0.0f) || nsound < 4 || nsound > 192 || nsound % 2 != 0)
+    return fail(c, TE_ERR_ARG, "dt must be positive and nsound even in [4, 192]");
+  te_config cfg = c->cfg;
+  cfg.dt = dt;
+  cfg.nsound = nsound;
+  te::Encoded e;
+  try {
+    e = te::encode_step(cfg, false);
+  } catch (const std::exception& ex) {
+    return fail(c, TE_ERR_ARG, ex.what());
+  }
+  const te::Layout& L = c->layout;
+  if (e.L.arena != L.arena || e.L.field_words != L.field_words || e.L.fields != L.fields)
+    return fail(c, TE_ERR_STATE, "the re-encoded step has another memory layout");
+  if (e.L.words > c->prog_words || c->image.size() < c->prog_words)
+    return fail(c, TE_ERR_STATE, "the re-encoded step's parameter words exceed the context's memory");
+  for (const te::Op& op : e.ops)
+    if (op.kid < 0) return fail(c, TE_ERR_UNSUPPORTED, "the re-encoded step has kernels off this path");
+  const uint64_t pay0 = L.arena + e.skipped;
+  std::copy(e.image.begin() + L.arena, e.image.end(), c->image.begin() + pay0);
+  std::fill(c->image.begin() + e.L.words, c->image.begin() + c->prog_words, 0u);
+  bool same = e.ops.size() == c->ops.size();
+  for (size_t i = 0; same && i < e.ops.size(); ++i)
+    same = e.ops[i].kid == c->ops[i].kid && std::memcmp(e.ops[i].A, c->ops[i].A, sizeof(e.ops[i].A)) == 0;
+  if (!same) {
+    std::vector<te::Op> keep = std::move(c->ops);
+    c->ops = std::move(e.ops);
+    int rc = validate_ops(c);
+    if (rc != TE_OK) {
+      c->ops = std::move(keep);
+      return rc;
+    }
+    c->op_names = std::move(e.names);
+    if (c->graph) (void)hipGraphExecDestroy(c->graph);
+    c->graph = nullptr;
+    c->graph_failed = false;
+    c->guard_ok = false;
+    c->tuned_ops = 0;
+  }
+  c->cfg.dt = dt;
+  c->cfg.nsound = nsound;
+  if (!c->dM) return TE_OK;  // encoded program: the host image is the memory
+  TE_HIP(hipMemcpyAsync(c->dM + pay0, c->image.data() + pay0, (c->prog_words - pay0) * 4, hipMemcpyHostToDevice,
+                        c->stream),
+         "upload step parameters");
+  TE_HIP(hipStreamSynchronize(c->stream), "upload sync");
+  return same ? TE_OK : grow_guards(c);
+}
+
+int te_set_time_step(te_ctx* c, float dt, int32_t nsound) { return retime(c, dt, nsound); }
+
+// stabilityFromWords (adaptive.js:111-116): square roots of the two maxima,
+// read and cleared for the next step (view.js:546-549).
+static int read_stability(te_ctx* c, float* cfl, float* ks) {
+  uint32_t w[2] = {0u, 0u};
+  TE_HIP(hipMemcpyAsync(w, c->dM + c->stab_off, 8, hipMemcpyDeviceToHost, c->stream), "read stability");
+  TE_HIP(hipMemsetAsync(c->dM + c->stab_off, 0, 8, c->stream), "clear stability");
+  TE_HIP(hipStreamSynchronize(c->stream), "stability sync");
+  float v[2];
+  std::memcpy(v, w, 8);
+  *cfl = (float)std::sqrt((double)v[0]);
+  *ks = (float)std::sqrt((double)v[1]);
+  return TE_OK;
+}
+
+static bool adaptive_ctx(const te_ctx* c) { return c && c->dM && c->has_config && c->cfg.adaptive_dt; }
+
+int te_adaptive_start(te_ctx* c) {
+  if (!adaptive_ctx(c)) return fail(c, TE_ERR_STATE, "not a context created with adaptive_dt = 1");
+  int stab = -1;
+  for (int i = (int)c->ops.size() - 1; i >= 0 && stab < 0; --i)
+    if (std::strcmp(te::kernel_name(c->ops[i].kid), "stability_max") == 0) stab = i;
+  if (stab < 0) return fail(c, TE_ERR_STATE, "the step has no stability_max dispatch");
+  TE_HIP(hipMemsetAsync(c->dM + c->stab_off, 0, 8, c->stream), "clear stability");
+  int rc = te_run(c, 1, stab, stab);  // encodeInitial (driver.js:46): the t = 0 maxima only
+  if (rc != TE_OK) return rc;
+  float cfl = 0.0f, ks = 0.0f;
+  rc = read_stability(c, &cfl, &ks);
+  if (rc != TE_OK) return rc;
+  const float nan = std::numeric_limits<float>::quiet_NaN();
+  const float freq[6] = {c->cfg.tapfrq, c->cfg.rstfrq, c->cfg.statfrq, nan, nan, nan};
+  te_clock_destroy(c->clock);
+  c->clock = te_clock_create(c->dtl, c->layout.min_spacing, 0.0, 0, 1, freq);
+  if (!c->clock) return fail(c, TE_ERR_STATE, "clock allocation failed");
+  te_clock_start(c->clock, cfl, ks);
+  te_clock_state st;
+  te_clock_get(c->clock, &st);
+  return retime(c, st.dt, st.nsound);
+}
+
+int te_adaptive_step(te_ctx* c, int32_t nsteps) {
+  if (!adaptive_ctx(c) || !c->clock) return fail(c, TE_ERR_STATE, "te_adaptive_start has not run on this context");
+  if (nsteps < 0) return fail(c, TE_ERR_ARG, "nsteps must be non-negative");
+  for (int s = 0; s < nsteps; ++s) {
+    te_clock_begin_step(c->clock);
+    int rc = grow_guards(c);
+    if (rc != TE_OK) return rc;
+    ensure_graph(c);
+    if (c->graph) {
+      TE_HIP(hipGraphLaunch(c->graph, c->stream), "hipGraphLaunch");
+    } else {
+      rc = run_ops(c, 0, (int)c->ops.size() - 1);
+      if (rc != TE_OK) return rc;
+    }
+    float cfl = 0.0f, ks = 0.0f;
+    rc = read_stability(c, &cfl, &ks);
+    if (rc != TE_OK) return rc;
+    te_clock_end_step(c->clock, cfl, ks);
+    te_clock_state st;
+    te_clock_get(c->clock, &st);
+    rc = retime(c, st.dt, st.nsound);
+    if (rc != TE_OK) return rc;
+  }
+  return TE_OK;
+}
+
+int te_adaptive_clock(const te_ctx* c, te_clock_state* out) {
+  if (!c || !out || !c->clock) return TE_ERR_STATE;
+  te_clock_get(c->clock, out);
+  return TE_OK;
 }
 
 int te_init_state(te_ctx* c) {

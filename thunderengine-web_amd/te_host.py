@@ -59,7 +59,19 @@ class Config(ctypes.Structure):
         ("tot_x_len", ctypes.c_double),
         ("dy_inner", ctypes.c_double), ("dy_outer", ctypes.c_double), ("nos_y_len", ctypes.c_double),
         ("tot_y_len", ctypes.c_double),
+        ("adaptive_dt", ctypes.c_int32), ("tapfrq", ctypes.c_float), ("statfrq", ctypes.c_float),
+        ("rstfrq", ctypes.c_float),
     ]
+
+
+class ClockState(ctypes.Structure):
+    """te_clock_state: the adaptive clock after its last call."""
+    _fields_ = [("mtime", ctypes.c_double), ("dbldt", ctypes.c_double), ("dt", ctypes.c_float),
+                ("dtlast", ctypes.c_float), ("cflmax", ctypes.c_float), ("ksmax", ctypes.c_float),
+                ("nsound", ctypes.c_int32), ("steps", ctypes.c_int32), ("fired", ctypes.c_uint32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 _lib = None
@@ -100,6 +112,17 @@ def lib():
         "te_config_defaults": ([P(Config)], None),
         "te_encode": ([P(Config), P(ctx)], ctypes.c_int),
         "te_create": ([P(Config), P(ctx)], ctypes.c_int),
+        "te_set_time_step": ([ctx, ctypes.c_float, ctypes.c_int32], ctypes.c_int),
+        "te_adaptive_start": ([ctx], ctypes.c_int),
+        "te_adaptive_step": ([ctx, ctypes.c_int32], ctypes.c_int),
+        "te_adaptive_clock": ([ctx, P(ClockState)], ctypes.c_int),
+        "te_clock_create": ([ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int32, ctypes.c_int32,
+                             P(ctypes.c_float)], ctypes.c_void_p),
+        "te_clock_start": ([ctypes.c_void_p, ctypes.c_float, ctypes.c_float], None),
+        "te_clock_begin_step": ([ctypes.c_void_p], None),
+        "te_clock_end_step": ([ctypes.c_void_p, ctypes.c_float, ctypes.c_float], None),
+        "te_clock_get": ([ctypes.c_void_p, P(ClockState)], None),
+        "te_clock_destroy": ([ctypes.c_void_p], None),
         "te_init_state": ([ctx], ctypes.c_int),
         "te_step": ([ctx, ctypes.c_int32], ctypes.c_int),
         "te_field_offset": ([ctx, ctypes.c_char_p, P(u64)], ctypes.c_int),
@@ -302,6 +325,23 @@ class Run(_Ctx):
     def step(self, n=1):
         self._check(lib().te_step(self.h, n), "te_step")
 
+    def set_time_step(self, dt, nsound):
+        """TimeStep.setTimeStep (web/core/timestep.js:711-716): the step for a new dt and nsound."""
+        self._check(lib().te_set_time_step(self.h, dt, nsound), "te_set_time_step")
+
+    def adaptive_start(self):
+        """adapt_dt = 1: the t = 0 stability maxima start the clock (driver.js:46-47)."""
+        self._check(lib().te_adaptive_start(self.h), "te_adaptive_start")
+
+    def adaptive_step(self, n=1):
+        """n large steps, each with the dt and nsound the clock chose from the last one."""
+        self._check(lib().te_adaptive_step(self.h, n), "te_adaptive_step")
+
+    def clock(self):
+        st = ClockState()
+        self._check(lib().te_adaptive_clock(self.h, ctypes.byref(st)), "te_adaptive_clock")
+        return st.as_dict()
+
     def field_offset(self, name):
         o = ctypes.c_uint64()
         self._check(lib().te_field_offset(self.h, name.encode(), ctypes.byref(o)), "te_field_offset(%s)" % name)
@@ -321,6 +361,49 @@ class Run(_Ctx):
     def interior(self, name, stagger=(0, 0, 0)):
         f = self.field(name)
         return f[: self.nz + stagger[2], 3: 3 + self.ny + stagger[1], 3: 3 + self.nx + stagger[0]]
+
+
+class Clock:
+    """AdaptiveTimeStep (web/core/adaptive.js:15-109) through te_clock_*:
+    freq maps 'tape' 'restart' 'stat' 'domainDiag' 'azimAvg' 'hifrq' to
+    frequencies (s); absent kinds have no clock."""
+    KINDS = ("tape", "restart", "stat", "domainDiag", "azimAvg", "hifrq")
+
+    def __init__(self, dtl, min_dx, min_dz=0.0, two_dimensional=False, ks_check=True, freq=None):
+        fr = (ctypes.c_float * 6)(*[float((freq or {}).get(k, float("nan"))) for k in self.KINDS])
+        self.h = lib().te_clock_create(dtl, min_dx, min_dz, int(two_dimensional), int(ks_check), fr)
+        if not self.h:
+            raise TeError("te_clock_create failed")
+
+    def start(self, cflmax, ksmax):
+        lib().te_clock_start(self.h, cflmax, ksmax)
+
+    def begin_step(self):
+        lib().te_clock_begin_step(self.h)
+
+    def end_step(self, cflmax, ksmax):
+        lib().te_clock_end_step(self.h, cflmax, ksmax)
+
+    def state(self):
+        st = ClockState()
+        lib().te_clock_get(self.h, ctypes.byref(st))
+        d = st.as_dict()
+        d["fired_kinds"] = [k for i, k in enumerate(self.KINDS) if d["fired"] >> i & 1]
+        return d
+
+    def fired(self, kind):
+        return kind in self.state()["fired_kinds"]
+
+    def close(self):
+        if self.h:
+            lib().te_clock_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Ensemble:
