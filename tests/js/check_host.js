@@ -3,6 +3,7 @@
 //   node check_host.js plan            -> tape plan summary (CPU)
 //   node check_host.js encode '<json>' -> program summary of a config (CPU)
 //   node check_host.js ranges          -> bad offsets / counts are refused (CPU)
+//   node check_host.js clock '<json>'  -> AdaptiveTimeStep sequence for given maxima (CPU)
 //   node check_host.js replay <out>    -> supercell tape replayed on the GPU with hashes
 //   node check_host.js run '<json>' n [tune] -> field FNV-1a after n steps on the GPU
 //        (tune: launch shapes chosen by run.autotune first, kept regardless)
@@ -33,6 +34,19 @@ if (mode === 'plan') {
   const prog = r.program();
   console.log(JSON.stringify({ ops: prog.names.length, names: prog.names, argsFnv: fnvWords(prog.args),
     fields: r.arena.names.length, memWords: r.memoryWords(), memFnv: fnvWords(r.read(r.memoryWords(), 0)) }));
+} else if (mode === 'clock') {
+  // AdaptiveTimeStep through the addon: settings JSON, then (cflmax, ksmax) per step
+  const a = JSON.parse(process.argv[3]);
+  const c = new te.AdaptiveTimeStep({ dtl: a.dtl, frequency: a.frequency }, { minHorizontalSpacing: a.dmin, twoDimensional: false });
+  c.start(a.cfl[0], a.ks[0]);
+  const seq = [[c.dt, c.nsound, c.mtime, c.firedKinds.join(',')]];
+  for (let i = 1; i < a.cfl.length; i++) {
+    c.beginStep();
+    const fired = c.firedKinds.join(',');
+    c.endStep(a.cfl[i], a.ks[i]);
+    seq.push([c.dt, c.nsound, c.mtime, fired]);
+  }
+  console.log(JSON.stringify(seq));
 } else if (mode === 'ranges') {
   // offsets and counts that are negative, fractional or beyond 2^53 are
   // refused by the addon (RangeError) before they reach the C ABI

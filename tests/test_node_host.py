@@ -100,6 +100,26 @@ def test_stretched_preset_mapping():
     assert got["memFnv"] != ol.fnv1a(te_host.Run(flat, encode_only=True).read(0, r.mem_words))
 
 
+def test_adaptive_clock_binding():
+    """The JavaScript AdaptiveTimeStep (N-API) takes the C ABI clock's steps."""
+    rng = np.random.default_rng(5)
+    cfl = [float(np.float32(x)) for x in rng.uniform(0.1, 1.6, 30)]
+    ks = [float(np.float32(x)) for x in rng.uniform(0.0, 0.12, 30)]
+    arg = dict(dtl=6.0, dmin=750.0, frequency={"tape": 90.0, "restart": 0.0, "stat": 30.0}, cfl=cfl, ks=ks)
+    got = json.loads(node("clock", json.dumps(arg)))
+    c = te_host.Clock(6.0, 750.0, freq=arg["frequency"])
+    c.start(cfl[0], ks[0])
+    st = c.state()
+    want = [[st["dt"], st["nsound"], st["mtime"], ",".join(st["fired_kinds"])]]
+    for i in range(1, len(cfl)):
+        c.begin_step()
+        fired = ",".join(c.state()["fired_kinds"])
+        c.end_step(cfl[i], ks[i])
+        st = c.state()
+        want.append([st["dt"], st["nsound"], st["mtime"], fired])
+    assert got == want
+
+
 @pytest.mark.gpu
 def test_replay_supercell_tape_on_gpu(gpu, tmp_path):
     out = tmp_path / "hashes.bin"

@@ -167,6 +167,10 @@ function presetToConfig(preset, overrides) {
   if (g('imoist') !== undefined) c.moist = g('imoist') === 1 ? 1 : 0;
   if (g('ptype') !== undefined && g('ptype') !== 0) c.microphysics = g('ptype');
   if (g('apmasscon') !== undefined) c.mass_conservation = g('apmasscon');
+  if (g('adapt_dt') === 1) {          // config.js:725-733
+    c.adaptive_dt = 1;
+    set('tapfrq', g('tapfrq')); set('statfrq', g('statfrq')); set('rstfrq', g('rstfrq'));
+  }
   set('nsound', g('nsound'));
   for (const a of ['x', 'y']) {              // stretch_x / stretch_y (config.js:562-573)
     if (g('stretch_' + a) === undefined || g('stretch_' + a) === 0) continue;
@@ -217,7 +221,42 @@ function buildRun(conf, options) {
   run.regionHash = (offWords, words) => addon().regionHash(ctx, offWords, words);
   run.memoryWords = () => addon().memoryWords(ctx);
   run.destroy = () => addon().destroy(ctx);
+  // TimeStep.setTimeStep (timestep.js:711-716): the step for a new dt, nsound
+  run.setTimeStep = (dt, nsound) => { addon().setTimeStep(ctx, dt, nsound); return run; };
+  // adapt_dt = 1 (driver.js:36-50): start() reads the t = 0 stability maxima,
+  // step(n) takes n steps of the clock's dt; clock() is its state
+  if (flat.adaptive_dt) {
+    run.adaptive = {
+      start() { addon().adaptiveStart(ctx); return this.clock(); },
+      step(n) { addon().adaptiveStep(ctx, n == null ? 1 : n); return this.clock(); },
+      clock() { return addon().adaptiveClock(ctx); },
+    };
+  }
   return run;
+}
+
+// ---------------------------------------------------------------- clock
+// AdaptiveTimeStep (web/core/adaptive.js:15-109): settings { dtl, frequency:
+// { tape, restart, stat, domainDiag, azimAvg, hifrq }, ksCheck } and the
+// grid's spacings; start / beginStep / endStep as in the reference, the
+// state (dt, dbldt, nsound, mtime, cflmax, ksmax) after each call.
+const CLOCK_KINDS = ['tape', 'restart', 'stat', 'domainDiag', 'azimAvg', 'hifrq'];
+class AdaptiveTimeStep {
+  constructor(settings, { minHorizontalSpacing, twoDimensional, minVerticalSpacing = null }) {
+    const fr = CLOCK_KINDS.map(k => (settings.frequency && settings.frequency[k] !== undefined ? settings.frequency[k] : NaN));
+    this.h = addon().clockCreate(settings.dtl, minHorizontalSpacing, minVerticalSpacing == null ? 0 : minVerticalSpacing,
+      twoDimensional ? 1 : 0, settings.ksCheck === false ? 0 : 1, fr);
+    this._set(addon().clockCall(this.h, 'get'));
+  }
+  _set(st) {
+    const { fired, ...rest } = st;
+    Object.assign(this, rest);
+    this.firedKinds = CLOCK_KINDS.filter((k, i) => (fired >> i) & 1);
+  }
+  fired(kind) { return this.firedKinds.includes(kind); }
+  start(cflmax, ksmax) { this._set(addon().clockCall(this.h, 'start', cflmax, ksmax)); }
+  beginStep() { this._set(addon().clockCall(this.h, 'begin')); }
+  endStep(cflmax, ksmax) { this._set(addon().clockCall(this.h, 'end', cflmax, ksmax)); }
 }
 
 // ---------------------------------------------------------------- adjoint
@@ -276,4 +315,5 @@ class StateGradient {
   }
 }
 
-module.exports = { addon, planTape, Replayer, presetToConfig, buildRun, FieldArena, LargeStepFunction, StateGradient };
+module.exports = { addon, planTape, Replayer, presetToConfig, buildRun, FieldArena, LargeStepFunction, StateGradient,
+  AdaptiveTimeStep };

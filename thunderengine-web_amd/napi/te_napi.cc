@@ -8,6 +8,7 @@
 #include <cstddef>
 #include <cstring>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/te.h"
@@ -129,6 +130,7 @@ const FieldDesc kFields[] = {
     FD(species, I32), FD(microphysics, I32), FD(mass_conservation, I32), FD(cache_moisture_turbulence, I32),
     FD(stretch_x, I32), FD(stretch_y, I32), FD(dx_inner, F64), FD(dx_outer, F64), FD(nos_x_len, F64),
     FD(tot_x_len, F64), FD(dy_inner, F64), FD(dy_outer, F64), FD(nos_y_len, F64), FD(tot_y_len, F64),
+    FD(adaptive_dt, I32), FD(tapfrq, F32), FD(statfrq, F32), FD(rstfrq, F32),
 };
 #undef FD
 
@@ -380,6 +382,122 @@ napi_value Autotune(napi_env env, napi_callback_info info) {
   return make_num(env, te_tuned_ops(c));
 }
 
+// ---- adapt_dt = 1
+napi_value clock_to_js(napi_env env, const te_clock_state& st) {
+  napi_value o;
+  NAPI_OK(napi_create_object(env, &o));
+  const std::pair<const char*, double> kv[] = {
+      {"mtime", st.mtime}, {"dbldt", st.dbldt}, {"dt", st.dt}, {"dtlast", st.dtlast}, {"cflmax", st.cflmax},
+      {"ksmax", st.ksmax}, {"nsound", (double)st.nsound}, {"steps", (double)st.steps}, {"fired", (double)st.fired}};
+  for (const auto& p : kv) NAPI_OK(napi_set_named_property(env, o, p.first, make_num(env, p.second)));
+  return o;
+}
+
+napi_value SetTimeStep(napi_env env, napi_callback_info info) {
+  napi_value argv[3];
+  if (!get_args(env, info, 3, argv)) return nullptr;
+  te_ctx* c = ctx_of(env, argv[0]);
+  if (!c) return nullptr;
+  int rc = te_set_time_step(c, (float)num(env, argv[1]), (int32_t)num(env, argv[2]));
+  if (rc != TE_OK) return throw_status(env, c, rc, "te_set_time_step");
+  return nullptr;
+}
+
+napi_value AdaptiveStart(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!get_args(env, info, 1, argv)) return nullptr;
+  te_ctx* c = ctx_of(env, argv[0]);
+  if (!c) return nullptr;
+  int rc = te_adaptive_start(c);
+  if (rc != TE_OK) return throw_status(env, c, rc, "te_adaptive_start");
+  return nullptr;
+}
+
+napi_value AdaptiveStep(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!get_args(env, info, 2, argv)) return nullptr;
+  te_ctx* c = ctx_of(env, argv[0]);
+  if (!c) return nullptr;
+  int rc = te_adaptive_step(c, (int32_t)num(env, argv[1]));
+  if (rc != TE_OK) return throw_status(env, c, rc, "te_adaptive_step");
+  return nullptr;
+}
+
+napi_value AdaptiveClock(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!get_args(env, info, 1, argv)) return nullptr;
+  te_ctx* c = ctx_of(env, argv[0]);
+  if (!c) return nullptr;
+  te_clock_state st;
+  int rc = te_adaptive_clock(c, &st);
+  if (rc != TE_OK) return throw_status(env, c, rc, "te_adaptive_clock");
+  return clock_to_js(env, st);
+}
+
+// te_clock as an external: clockCreate(dtl, minDx, minDz, twoD, ksCheck, [6 freqs, NaN = absent])
+void clock_finalize(napi_env, void* p, void*) { te_clock_destroy(static_cast<te_clock*>(p)); }
+te_clock* clock_of(napi_env env, napi_value v) {
+  void* p = nullptr;
+  if (napi_get_value_external(env, v, &p) != napi_ok || !p) {
+    napi_throw_type_error(env, nullptr, "expected a clock");
+    return nullptr;
+  }
+  return static_cast<te_clock*>(p);
+}
+napi_value ClockCreate(napi_env env, napi_callback_info info) {
+  napi_value argv[6];
+  if (!get_args(env, info, 6, argv)) return nullptr;
+  float fr[6];
+  for (uint32_t i = 0; i < 6; ++i) {
+    napi_value e;
+    NAPI_OK(napi_get_element(env, argv[5], i, &e));
+    fr[i] = (float)num(env, e);
+  }
+  te_clock* c = te_clock_create(num(env, argv[0]), num(env, argv[1]), num(env, argv[2]), (int32_t)num(env, argv[3]),
+                                (int32_t)num(env, argv[4]), fr);
+  if (!c) {
+    napi_throw_error(env, nullptr, "te_clock_create failed");
+    return nullptr;
+  }
+  napi_value r;
+  if (napi_create_external(env, c, clock_finalize, nullptr, &r) != napi_ok) {
+    te_clock_destroy(c);
+    napi_throw_error(env, nullptr, "napi_create_external");
+    return nullptr;
+  }
+  return r;
+}
+napi_value ClockCall(napi_env env, napi_callback_info info) {
+  // clockCall(clock, 'start' | 'begin' | 'end' | 'get', cflmax, ksmax)
+  napi_value argv[4];
+  size_t argc = 4;
+  NAPI_OK(napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  if (argc < 2) {
+    napi_throw_type_error(env, nullptr, "clockCall(clock, what, ...)");
+    return nullptr;
+  }
+  te_clock* c = clock_of(env, argv[0]);
+  if (!c) return nullptr;
+  char what[8] = {0};
+  size_t len = 0;
+  napi_get_value_string_utf8(env, argv[1], what, sizeof(what), &len);
+  const std::string w(what);
+  if ((w == "start" || w == "end") && argc < 4) {
+    napi_throw_type_error(env, nullptr, "clockCall: cflmax and ksmax expected");
+    return nullptr;
+  }
+  if (w == "start") te_clock_start(c, (float)num(env, argv[2]), (float)num(env, argv[3]));
+  else if (w == "begin") te_clock_begin_step(c);
+  else if (w == "end") te_clock_end_step(c, (float)num(env, argv[2]), (float)num(env, argv[3]));
+  else if (w != "get") {
+    napi_throw_type_error(env, nullptr, "clockCall: unknown call");
+    return nullptr;
+  }
+  te_clock_state st;
+  te_clock_get(c, &st);
+  return clock_to_js(env, st);
+}
+
 napi_value HashCall(napi_env env, napi_callback_info info, bool fnv) {
   napi_value argv[3];
   if (!get_args(env, info, 3, argv)) return nullptr;
@@ -629,6 +747,8 @@ napi_value Init(napi_env env, napi_value exports) {
       {"profileOps", ProfileOps}, {"destroy", Destroy},
       {"adjointCreate", AdjointCreate}, {"fwdCkpt", FwdCkpt}, {"bwd", Bwd}, {"adjZero", AdjZero},
       {"adjUpload", AdjUpload}, {"adjDownload", AdjDownload}, {"adjLastMs", AdjLastMs}, {"adjDestroy", AdjDestroy},
+      {"setTimeStep", SetTimeStep}, {"adaptiveStart", AdaptiveStart}, {"adaptiveStep", AdaptiveStep},
+      {"adaptiveClock", AdaptiveClock}, {"clockCreate", ClockCreate}, {"clockCall", ClockCall},
   };
   for (auto& f : fns) {
     napi_value v;
