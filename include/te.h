@@ -184,6 +184,15 @@ typedef struct te_config {
      (web/core/config.js:725-729). */
   int32_t adaptive_dt;
   float tapfrq, statfrq, rstfrq;
+  /* the caller's sounding (buildRun(configuration, sounding), driver.js:23;
+     isnd = 7 / 17, parseSounding config.js:899-901): the text of a CM1
+     input_sounding file (header p_sfc mb, theta_sfc K, qv_sfc g/kg; rows z m,
+     theta K, qv g/kg and with sounding_winds = 1 also u v m/s), interpolated
+     to the grid (Sounding.interpolated, base.js:716-757); NULL: the analytic
+     Weisman-Klemp sounding and hodograph above (the perturbed-sounding
+     options apply to that one only).  te_encode / te_create keep a copy. */
+  const char* input_sounding;
+  int32_t sounding_winds;
 } te_config;
 
 /* Fills a config with the synthetic 64x64x40 case defaults (dx=dy=1 km,
@@ -220,6 +229,12 @@ const char* te_field_name(const te_ctx* ctx, int32_t i);
  * (c1 c2 mh mf zf rho0 rr0 rf0 th0 pi0 rth0 qv0 qc0 qi0 tauh tauf u0 v0
  * hmetric radbcw radbce radbcs radbcn sfc dims). */
 int te_region_offset(const te_ctx* ctx, const char* name, uint64_t* word_off);
+
+/* The named regions of a config context in memory order (names for
+ * te_region_offset): the profile and parameter buffers a tape export lists
+ * (buildRun's tape(), driver.js:63-73, one buffer per region). */
+int32_t te_region_count(const te_ctx* ctx);
+const char* te_region_name(const te_ctx* ctx, int32_t i);
 
 /* Grid of a config context: nx ny nz strideJ strideK. */
 int te_grid(const te_ctx* ctx, int32_t out5[5]);

@@ -1,11 +1,12 @@
 'use strict';
 // Node-side checks of the JavaScript host, driven by tests/test_node_host.py.
 //   node check_host.js plan            -> tape plan summary (CPU)
-//   node check_host.js encode '<json>' -> program summary of a config (CPU)
+//   node check_host.js encode '<json>' [sounding file] -> program summary of a config (CPU)
 //   node check_host.js ranges          -> bad offsets / counts are refused (CPU)
 //   node check_host.js clock '<json>'  -> AdaptiveTimeStep sequence for given maxima (CPU)
+//   node check_host.js tape '<json>' <out> [sounding] -> run.tape(2) and run.tapeState() written to <out> (CPU)
 //   node check_host.js replay <out>    -> supercell tape replayed on the GPU with hashes
-//   node check_host.js run '<json>' n [tune] -> field FNV-1a after n steps on the GPU
+//   node check_host.js run '<json>' n [tune] [sounding] -> field FNV-1a after n steps on the GPU
 //        (tune: launch shapes chosen by run.autotune first, kept regardless)
 //   node check_host.js adjoint '<json>' n <out> -> gradient of sum of w^2 over a
 //        box after n checkpointed steps w.r.t. the initial th, u, v (GPU)
@@ -30,7 +31,7 @@ if (mode === 'plan') {
   console.log(JSON.stringify({ memWords: p.memWords, ops: p.names.length, slots: p.slots.length,
     argsFnv: fnvWords(p.args), slotFnv: fnvWords(Uint32Array.from(p.slots.map(s => s.off))) }));
 } else if (mode === 'encode') {
-  const r = te.buildRun(JSON.parse(process.argv[3]), { encodeOnly: true });
+  const r = te.buildRun(JSON.parse(process.argv[3]), process.argv[4] ? fs.readFileSync(process.argv[4], 'utf8') : null, { encodeOnly: true });
   const prog = r.program();
   console.log(JSON.stringify({ ops: prog.names.length, names: prog.names, argsFnv: fnvWords(prog.args),
     fields: r.arena.names.length, memWords: r.memoryWords(), memFnv: fnvWords(r.read(r.memoryWords(), 0)) }));
@@ -47,10 +48,25 @@ if (mode === 'plan') {
     seq.push([c.dt, c.nsound, c.mtime, fired]);
   }
   console.log(JSON.stringify(seq));
+} else if (mode === 'tape') {
+  // run.tape(n) / run.tapeState() of a config (+ sounding file) into <out>/tape.json, <out>/state/<id>.bin.gz
+  const out = process.argv[4];
+  const r = te.buildRun(JSON.parse(process.argv[3]), process.argv[5] ? fs.readFileSync(process.argv[5], 'utf8') : null,
+    { encodeOnly: true });
+  const tape = r.tape(2);
+  const state = r.tapeState();
+  fs.mkdirSync(path.join(out, 'state'), { recursive: true });
+  fs.writeFileSync(path.join(out, 'tape.json'), JSON.stringify(tape));
+  for (const id of Object.keys(state)) {
+    const w = state[id];
+    fs.writeFileSync(path.join(out, 'state', id + '.bin.gz'), zlib.gzipSync(Buffer.from(w.buffer, w.byteOffset, w.length * 4)));
+  }
+  console.log(JSON.stringify({ buffers: tape.buffers.length, commands: tape.steps[0].length, steps: tape.steps.length }));
+  r.destroy();
 } else if (mode === 'ranges') {
   // offsets and counts that are negative, fractional or beyond 2^53 are
   // refused by the addon (RangeError) before they reach the C ABI
-  const r = te.buildRun({ nx: 8, ny: 8, nz: 4 }, { encodeOnly: true });
+  const r = te.buildRun({ nx: 8, ny: 8, nz: 4 }, null, { encodeOnly: true });
   const bad = [[-1, 4], [2 ** 64 - 1, 4], [0.5, 4], [NaN, 4], [0, -1], [0, 2 ** 64 - 1], [r.memoryWords(), 4]];
   const refused = [];
   for (const [off, n] of bad) {
@@ -82,7 +98,7 @@ if (mode === 'plan') {
 } else if (mode === 'run') {
   const conf = JSON.parse(process.argv[3]);
   const n = Number(process.argv[4]);
-  const r = te.buildRun(conf);
+  const r = te.buildRun(conf, process.argv[6] ? fs.readFileSync(process.argv[6], 'utf8') : null);
   r.fill();
   if (process.argv[5] === 'tune' && !(r.autotune(2, true) > 0)) throw new Error('autotune tuned no dispatch');
   r.step(n);

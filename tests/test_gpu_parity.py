@@ -8,6 +8,7 @@ recorded hashes, through the C ABI (libte.so).
   for one and several large steps on the GPU (several steps replay a captured
   HIP graph) and compared word for word with the oracle on the same memory.
 """
+import os
 import sys
 
 import numpy as np
@@ -41,8 +42,14 @@ def test_preset_tape_region_hashes(gpu, preset):
     assert ok >= {"supercell": 825, "squall_line": 815, "supercell_stretched": 52}[preset]
 
 
+HURRICANE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "soundings",
+                         "hurricane_3d.input_sounding")
+
 CONFIGS = {
     "A_64x64x40": dict(),
+    # the caller's sounding: the reference's hurricane_3d input_sounding (isnd = 7)
+    "sounding_hurricane": dict(nx=40, ny=32, nz=24, dz=600.0, tke=1, initial_tke=3, rayleigh=1, umove=2.0,
+                               input_sounding=open(HURRICANE).read(), sounding_winds=1),
     "tke_rayleigh": dict(nx=48, ny=40, nz=24, tke=1, rayleigh=1, initial_tke=3),
     "open_tke_heating": dict(nx=40, ny=48, nz=20, open_x=1, open_y=1, tke=1, initial_tke=3, umove=12.5, vmove=3.0,
                              dissipative_heating=1, rayleigh=1),

@@ -20,6 +20,7 @@ import tapeplan
 import te_host
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HERE = os.path.join(ROOT, "tests")
 SCRIPT = os.path.join(ROOT, "tests", "js", "check_host.js")
 ADDON = os.path.join(ROOT, "thunderengine-web_amd", "lib", "te_napi.node")
 
@@ -120,6 +121,43 @@ def test_adaptive_clock_binding():
     assert got == want
 
 
+TAPE_CONFS = {
+    "tke_open": (dict(nx=20, ny=16, nz=10, tke=1, initial_tke=3, open_x=1, open_y=1, umove=5.0, rayleigh=1), None),
+    "sounding": (dict(nx=16, ny=12, nz=20, dz=400.0, tke=1, initial_tke=3, rayleigh=1),
+                 os.path.join(HERE, "golden", "soundings", "hurricane_3d.input_sounding")),
+}
+
+
+@pytest.mark.parametrize("name", sorted(TAPE_CONFS))
+def test_tape_export_replays_like_the_program(name, tmp_path):
+    """run.tape(2) + run.tapeState(): the replay plan of the exported tape
+    (replay.js:46-121, the plan recorded tapes go through) stepped by the
+    oracle gives the arena of the oracle stepping the program directly."""
+    conf, snd = TAPE_CONFS[name]
+    args = ["tape", json.dumps(conf), str(tmp_path)] + ([snd] if snd else [])
+    info = json.loads(node(*args))
+    tape = json.load(open(tmp_path / "tape.json"))
+    assert info["steps"] == 2 and len(tape["steps"]) == 2
+    io = json.load(open(os.path.join(ROOT, "thunderengine-web_amd", "kernel_io.json")))
+    params = {k: [[i, "ptr", int(w)] for i, r, w in v] for k, v in io.items()}
+    P = tapeplan.Plan(tape, params)
+    names, A = P.args()
+    M = np.zeros(runs.FRONT + P.mem_words + 8 * tape["fieldStride"] // 4, dtype=np.uint32)
+    view = M[runs.FRONT:]
+    view[: P.mem_words] = P.memory(str(tmp_path / "state"))
+    prog = ol.Program(names, A)
+    kw = dict(conf, input_sounding=open(snd).read(), sounding_winds=1) if snd else dict(conf)
+    ref = runs.OracleRun(te_host.config(fuse=0, **kw))
+    arena = P.base[len(tape["buffers"]) - 1]
+    words = len(tape["fields"]) * tape["fieldStride"] // 4
+    assert np.array_equal(view[arena: arena + words], ref.M[runs.arena_end(ref.enc) - words: runs.arena_end(ref.enc)])
+    for _ in range(2):
+        prog.run(view)
+        ref.step(1)
+        assert np.array_equal(view[arena: arena + words],
+                              ref.M[runs.arena_end(ref.enc) - words: runs.arena_end(ref.enc)])
+
+
 @pytest.mark.gpu
 def test_replay_supercell_tape_on_gpu(gpu, tmp_path):
     out = tmp_path / "hashes.bin"
@@ -141,12 +179,20 @@ def test_replay_supercell_tape_on_gpu(gpu, tmp_path):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("tune", ["", "tune"])
-def test_js_run_bit_exact_vs_oracle(gpu, tune):
-    """tune: run.autotune's launch shapes (forced) leave the fields as the oracle's."""
+@pytest.mark.parametrize("tune,sounding", [("", ""), ("tune", ""), ("", "hurricane")])
+def test_js_run_bit_exact_vs_oracle(gpu, tune, sounding):
+    """tune: run.autotune's launch shapes (forced) leave the fields as the
+    oracle's; hurricane: buildRun(conf, sounding) with the reference's
+    hurricane_3d input_sounding steps as the oracle does."""
     conf = dict(nx=32, ny=24, nz=16, tke=1, initial_tke=3, rayleigh=1)
-    got = json.loads(node("run", json.dumps(conf), 2, tune))
-    ref = runs.OracleRun(te_host.config(**conf))
+    kw = dict(conf)
+    args = ["run", json.dumps(conf), 2, tune]
+    if sounding:
+        path = os.path.join(HERE, "golden", "soundings", "hurricane_3d.input_sounding")
+        args.append(path)
+        kw.update(input_sounding=open(path).read(), sounding_winds=1)
+    got = json.loads(node(*args))
+    ref = runs.OracleRun(te_host.config(**kw))
     ref.step(2)
     e = ref.enc
     n = e.sJ * e.sK * (e.nz + 1)

@@ -95,6 +95,7 @@ struct te_ctx {
   bool has_config = false;
   te_config cfg{};
   std::vector<float> zf;  // owned copy of cfg.zf
+  std::string sounding;   // owned copy of cfg.input_sounding
   te::Layout layout;
   std::vector<uint32_t> image;
   // adjoints built on this context (te_adjoint_create); te_destroy detaches

@@ -798,6 +798,10 @@ int te_encode(const te_config* cfg, te_ctx** out) {
     c->cfg.zf = c->zf.data();
   }
   c->cfg.arena_fields = nullptr;  // the caller's list is consumed by the encoder only
+  if (cfg->input_sounding) {
+    c->sounding = cfg->input_sounding;
+    c->cfg.input_sounding = c->sounding.c_str();
+  }
   te_config first = *cfg;
   if (cfg->adaptive_dt) first.nsound = 4;  // setTimeStep(dtl, 4) before the clock starts (driver.js:41)
   c->cfg.nsound = first.nsound;
@@ -1001,6 +1005,15 @@ int te_region_offset(const te_ctx* c, const char* name, uint64_t* off) {
   if (it == c->layout.region.end()) return TE_ERR_ARG;
   *off = it->second;
   return TE_OK;
+}
+int32_t te_region_count(const te_ctx* c) { return c && c->has_config ? (int32_t)c->layout.region.size() : 0; }
+const char* te_region_name(const te_ctx* c, int32_t i) {
+  if (!c || !c->has_config || i < 0 || i >= (int32_t)c->layout.region.size()) return nullptr;
+  std::vector<std::pair<uint64_t, const std::string*>> order;
+  for (const auto& kv : c->layout.region) order.push_back({kv.second, &kv.first});
+  std::sort(order.begin(), order.end(),
+            [](const auto& a, const auto& b) { return a.first != b.first ? a.first < b.first : *a.second < *b.second; });
+  return order[(size_t)i].second->c_str();
 }
 uint64_t te_field_stride_words(const te_ctx* c) { return c ? c->layout.field_words : 0; }
 int32_t te_field_count(const te_ctx* c) { return c ? (int32_t)c->layout.fields.size() : 0; }

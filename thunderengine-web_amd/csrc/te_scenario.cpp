@@ -7,6 +7,8 @@
 // same analytic profiles: an initial state, not part of the per-step parity.
 #include <algorithm>
 #include <cmath>
+#include <cctype>
+#include <cstdlib>
 #include <cstring>
 #include <stdexcept>
 #include <string>
@@ -121,7 +123,175 @@ static std::vector<double> theta_perturbation(const te_config& cfg, const Vertic
   return out;
 }
 
+// ---- caller's sounding (isnd = 7 / 17, CM1's input_sounding file)
+static float powf_(float x, float y) { return f32(std::pow((double)x, (double)y)); }
+
+// One token of the file as a binary32 (parseFloat32, base.js:15-52: the
+// decimal string rounded once to the nearest float; strtof rounds once);
+// false for what is not a decimal number.
+static bool token_f32(const std::string& t, float* out) {
+  size_t i = 0, n = t.size();
+  if (i < n && (t[i] == '+' || t[i] == '-')) ++i;
+  size_t d0 = i;
+  while (i < n && std::isdigit((unsigned char)t[i])) ++i;
+  size_t dig = i - d0;
+  if (i < n && t[i] == '.') {
+    ++i;
+    size_t f0 = i;
+    while (i < n && std::isdigit((unsigned char)t[i])) ++i;
+    dig += i - f0;
+  }
+  if (dig == 0) return false;
+  if (i < n && (t[i] == 'e' || t[i] == 'E')) {
+    ++i;
+    if (i < n && (t[i] == '+' || t[i] == '-')) ++i;
+    size_t e0 = i;
+    while (i < n && std::isdigit((unsigned char)t[i])) ++i;
+    if (i == e0) return false;
+  }
+  if (i != n) return false;
+  *out = std::strtof(t.c_str(), nullptr);
+  return true;
+}
+
+// Sounding (base.js:659-714): header p_sfc (mb), theta_sfc (K), qv_sfc (g/kg);
+// rows z (m), theta (K), qv (g/kg) and, with winds, u v (m/s).  A surface
+// level with winds extrapolated from the first two rows is put below them;
+// pi is integrated hydrostatically with the virtual temperature, and levels
+// drier than 1e-12 kg/kg get 0.1 % of saturation.
+struct SoundingLevels {
+  std::vector<float> z, th, qv, u, v, pi, prs, t, rh;
+  float pS = 0, thS = 0, qvS = 0;
+};
+static SoundingLevels parse_sounding(const char* text, bool winds, bool moist) {
+  const float rd = f32(RDd), cp = f32(CPd), rv = f32(RVd), p00 = f32(P00d), g = f32(G);
+  const float reps = rv / rd, eps = rd / rv;
+  std::vector<std::vector<float>> rows;
+  std::string all(text), line;
+  for (char& ch : all)
+    if (ch == '\r') ch = '\n';
+  size_t pos = 0;
+  while (pos <= all.size()) {
+    size_t e = all.find('\n', pos);
+    if (e == std::string::npos) e = all.size();
+    line = all.substr(pos, e - pos);
+    pos = e + 1;
+    std::vector<float> nums;
+    size_t q = 0;
+    bool any = false;
+    while (q < line.size()) {
+      while (q < line.size() && (line[q] == ' ' || line[q] == '\t')) ++q;
+      size_t b = q;
+      while (q < line.size() && line[q] != ' ' && line[q] != '\t') ++q;
+      if (q > b) {
+        any = true;
+        float x;
+        if (token_f32(line.substr(b, q - b), &x)) nums.push_back(x);
+      }
+    }
+    if (any) rows.push_back(nums);
+  }
+  if (rows.empty()) throw std::runtime_error("input_sounding is empty");
+  const std::vector<float>& h = rows[0];
+  if (h.size() < 3) throw std::runtime_error("input_sounding header: expected p_sfc (mb), theta_sfc (K), qv_sfc (g/kg)");
+  SoundingLevels S;
+  S.pS = f32((double)h[0] * 100.0);
+  S.thS = h[1];
+  S.qvS = moist ? f32((double)h[2] / 1000.0) : 0.0f;
+  const size_t need = winds ? 5 : 3;
+  std::vector<float> z, th, qv, u, vv;
+  for (size_t r = 1; r < rows.size(); ++r) {
+    if (rows[r].size() < need)
+      throw std::runtime_error("input_sounding line " + std::to_string(r + 1) + ": expected z, theta, qv" +
+                               (winds ? ", u, v" : ""));
+    z.push_back(rows[r][0]);
+    th.push_back(rows[r][1]);
+    qv.push_back(moist ? f32((double)rows[r][2] / 1000.0) : 0.0f);
+    u.push_back(winds ? rows[r][3] : 0.0f);
+    vv.push_back(winds ? rows[r][4] : 0.0f);
+  }
+  if (z.size() < 2) throw std::runtime_error("input_sounding needs at least 2 levels (base.F:627-634)");
+  for (size_t i = 1; i < z.size(); ++i)
+    if (!(z[i] > z[i - 1])) throw std::runtime_error("input_sounding heights must increase");
+  if (!(z[0] > 0.0f)) throw std::runtime_error("input_sounding heights must start above the surface");
+  const float dz = z[1] - z[0];
+  const float uS = dz != 0.0f ? u[0] - ((z[0] - 0.0f) * (u[1] - u[0])) / dz : u[0];
+  const float vS = dz != 0.0f ? vv[0] - ((z[0] - 0.0f) * (vv[1] - vv[0])) / dz : vv[0];
+  S.z.push_back(0.0f); S.th.push_back(S.thS); S.qv.push_back(S.qvS); S.u.push_back(uS); S.v.push_back(vS);
+  S.z.insert(S.z.end(), z.begin(), z.end());
+  S.th.insert(S.th.end(), th.begin(), th.end());
+  S.qv.insert(S.qv.end(), qv.begin(), qv.end());
+  S.u.insert(S.u.end(), u.begin(), u.end());
+  S.v.insert(S.v.end(), vv.begin(), vv.end());
+  const size_t n = S.z.size();
+  std::vector<float> thv(n);
+  for (size_t k = 0; k < n; ++k) thv[k] = (S.th[k] * std::fma(reps, S.qv[k], 1.0f)) / (1.0f + S.qv[k]);
+  S.pi.assign(n, 0.0f);
+  S.pi[0] = powf_(S.pS / p00, rd / cp);
+  for (size_t k = 1; k < n; ++k)
+    S.pi[k] = S.pi[k - 1] - (g * (S.z[k] - S.z[k - 1])) / ((cp * 0.5f) * (thv[k] + thv[k - 1]));
+  const float cpOverRd = cp / rd;
+  S.prs.resize(n); S.t.resize(n); S.rh.assign(n, 0.0f);
+  for (size_t k = 0; k < n; ++k) {
+    S.prs[k] = p00 * powf_(S.pi[k], cpOverRd);
+    S.t[k] = S.th[k] * S.pi[k];
+  }
+  if (moist)
+    for (size_t k = 0; k < n; ++k) {
+      if (S.qv[k] < 1.0e-12f) S.qv[k] = 0.001f * rslf(S.prs[k], S.th[k] * S.pi[k], eps);
+      S.rh[k] = S.qv[k] / rslf(S.prs[k], S.t[k], eps);
+    }
+  return S;
+}
+
+// Sounding.interpolated (base.js:716-757) and BaseState.initProfiles / tail
+// (base.js:148-178): theta, u, v, p, T and relative humidity interpolated
+// linearly to the scalar levels, qv from the humidity at the interpolated
+// p and T, pi integrated up from the sounding's surface, winds relative to
+// the domain's motion.
+static Base sounding_base(const te_config& cfg, const Vertical& v) {
+  const float rd = f32(RDd), cp = f32(CPd), rv = f32(RVd), p00 = f32(P00d), g = f32(G);
+  const float reps = rv / rd, eps = rd / rv, cpdrd = cp / rd;
+  const bool moist = cfg.moist != 0;
+  const SoundingLevels S = parse_sounding(cfg.input_sounding, cfg.sounding_winds != 0, moist);
+  const int nk = v.nz;
+  if (!(S.z.back() >= v.zh[nk - 1]))
+    throw std::runtime_error("input_sounding reaches " + std::to_string(S.z.back()) + " m, below the top scalar level " +
+                             std::to_string(v.zh[nk - 1]) + " m (base.F:734-739)");
+  Base b;
+  b.th0.assign(nk, 0); b.qv0.assign(nk, 0); b.qc0.assign(nk, 0); b.pi0.assign(nk, 0); b.prs0.assign(nk, 0);
+  b.rho0.assign(nk, 0); b.u0.assign(nk, 0); b.v0.assign(nk, 0);
+  std::vector<float> pI(nk), tI(nk), rhI(nk), thv(nk);
+  for (int k = 0; k < nk; ++k) {
+    const float z = v.zh[k];
+    size_t kk = 1;
+    while (S.z[kk] < z) ++kk;
+    const size_t dn = kk - 1, up = kk;
+    const float fr = (z - S.z[dn]) / (S.z[up] - S.z[dn]);
+    auto lerp = [&](const std::vector<float>& a) { return std::fma(a[up] - a[dn], fr, a[dn]); };
+    b.th0[k] = lerp(S.th); b.u0[k] = lerp(S.u); b.v0[k] = lerp(S.v);
+    pI[k] = lerp(S.prs); tI[k] = lerp(S.t); rhI[k] = lerp(S.rh);
+  }
+  for (int k = 0; k < nk; ++k) {
+    b.qv0[k] = moist ? rhI[k] * rslf(pI[k], tI[k], eps) : 0.0f;
+    thv[k] = (b.th0[k] * std::fma(reps, b.qv0[k], 1.0f)) / (1.0f + b.qv0[k]);
+  }
+  const float piSfc = powf_(S.pS / p00, rd / cp);
+  const float thvSfc = (S.thS * std::fma(S.qvS, reps, 1.0f)) / (1.0f + S.qvS);
+  b.pi0[0] = piSfc - (g * v.zh[0]) / ((cp * 0.5f) * (thvSfc + thv[0]));
+  for (int k = 1; k < nk; ++k)
+    b.pi0[k] = b.pi0[k - 1] - (g * (v.zh[k] - v.zh[k - 1])) / ((cp * 0.5f) * (thv[k] + thv[k - 1]));
+  for (int k = 0; k < nk; ++k) {
+    b.prs0[k] = p00 * powf_(b.pi0[k], cpdrd);
+    b.rho0[k] = b.prs0[k] / (((rd * b.th0[k]) * b.pi0[k]) * std::fma(b.qv0[k], reps, 1.0f));
+    b.u0[k] = b.u0[k] - cfg.umove;
+    b.v0[k] = b.v0[k] - cfg.vmove;
+  }
+  return b;
+}
+
 Base make_base(const te_config& cfg, const Vertical& v) {
+  if (cfg.input_sounding) return sounding_base(cfg, v);
   const int nk = v.nz;
   const float rd = f32(RDd), cp = f32(CPd), rv = f32(RVd), p00 = f32(P00d), g = f32(G);
   const float reps = rv / rd, eps = rd / rv, cpdrd = cp / rd;

@@ -7,7 +7,8 @@
 //                     into one word-addressed memory M plus one 64-word
 //                     argument slot per dispatch, uploads buffers, runs steps
 //                     (optionally hashing every written region), reads back.
-//   buildRun(conf)    web/core/driver.js buildRun: a preset / namelist object
+//   buildRun(conf, sounding) web/core/driver.js buildRun: a preset / namelist object
+//                     and the caller's input_sounding (isnd 7 / 17)
 //                     (or flat options) -> step program, FieldArena, t=0 state.
 //   presetToConfig    the namelist keys of this path (web/core/config.js).
 // Nothing here computes on the CPU: every call goes to the HIP library and
@@ -145,8 +146,12 @@ class Replayer {
 // diffusion (doimpl), Rayleigh damping (irdamp, zd, rdalpha), idiss, imove
 // (umove, vmove), kdiv, alph, imoist, the explicit nsound (psolver 3) and
 // horizontal stretching (stretch_x/y, dx/dy_inner, _outer, nos_/tot_x/y_len).
+function presetValue(preset, k) {
+  for (const grp of Object.keys(preset || {})) { const v = preset[grp]; if (v && typeof v === 'object' && k in v) return v[k]; }
+  return undefined;
+}
 function presetToConfig(preset, overrides) {
-  const g = k => { for (const grp of Object.keys(preset || {})) { const v = preset[grp]; if (v && typeof v === 'object' && k in v) return v[k]; } return undefined; };
+  const g = k => presetValue(preset, k);
   const c = {};
   const set = (key, v) => { if (v !== undefined && v !== null) c[key] = v; };
   set('nx', g('nx')); set('ny', g('ny')); set('nz', g('nz'));
@@ -203,9 +208,20 @@ class FieldArena {
   index(i, j, k) { const L = this.layout; return (i + 3) + L.strideJ * ((j + 3) + L.strideK * k); }
 }
 
-function buildRun(conf, options) {
+// buildRun(configuration, sounding = null, options) as driver.js:23: the
+// sounding is the text of a CM1 input_sounding file or an object with its
+// `text` (the reference's Sounding keeps it) and `windsFromFile` (isnd 7:
+// true, 17: false); options: { encodeOnly, overrides }.
+function buildRun(conf, sounding, options) {
   options = options || {};
   const flat = conf && (conf.param0 || conf.param1 || conf.param2) ? presetToConfig(conf, options.overrides) : Object.assign({}, conf || {}, options.overrides || {});
+  if (sounding != null) {
+    flat.input_sounding = typeof sounding === 'string' ? sounding : sounding.text;
+    if (typeof flat.input_sounding !== 'string') throw new TypeError('sounding: input_sounding text expected');
+    const isnd = conf && conf.param0 ? presetValue(conf, 'isnd') : undefined;
+    const winds = typeof sounding === 'object' && sounding.windsFromFile !== undefined ? sounding.windsFromFile : isnd !== 17;
+    flat.sounding_winds = winds ? 1 : 0;
+  }
   const ctx = addon().create(flat, !!options.encodeOnly);
   const run = { ctx, config: flat, layout: addon().layout(ctx) };
   run.arena = new FieldArena(run);
@@ -221,6 +237,13 @@ function buildRun(conf, options) {
   run.regionHash = (offWords, words) => addon().regionHash(ctx, offWords, words);
   run.memoryWords = () => addon().memoryWords(ctx);
   run.destroy = () => addon().destroy(ctx);
+  // run.tape(n) (driver.js:63-73): the header of a tape of n large steps of
+  // this configuration as the reference's dispatch list (fuse = 0), with one
+  // buffer per profile / parameter region plus the arena and the parameter
+  // words as setBytes commands; run.tapeState() gives each buffer's t = 0
+  // words (the recordings keep them as state/<id>.bin.gz).
+  run.tape = nsteps => exportTape(flat, nsteps == null ? 1 : nsteps).tape;
+  run.tapeState = () => exportTape(flat, 0).state;
   // TimeStep.setTimeStep (timestep.js:711-716): the step for a new dt, nsound
   run.setTimeStep = (dt, nsound) => { addon().setTimeStep(ctx, dt, nsound); return run; };
   // adapt_dt = 1 (driver.js:36-50): start() reads the t = 0 stability maxima,
@@ -233,6 +256,77 @@ function buildRun(conf, options) {
     };
   }
   return run;
+}
+
+// ---------------------------------------------------------------- tape export
+function exportTape(flat, nsteps) {
+  const enc = addon().create(Object.assign({}, flat, { fuse: 0 }), true);
+  try {
+    addon().initState(enc);
+    const L = addon().layout(enc);
+    const regions = addon().regions(enc);
+    const prog = addon().program(enc);
+    const memWords = addon().memoryWords(enc);
+    const mem = addon().download(enc, 0, memWords);
+    const fieldNames = Object.keys(L.fields);
+    const arena = Math.min(...fieldNames.map(n => L.fields[n]));
+    const arenaEnd = arena + fieldNames.length * L.fieldStrideWords;
+    const buffers = [], starts = [], state = {};
+    regions.forEach((r, i) => {
+      const end = i + 1 < regions.length ? regions[i + 1].off : arena;
+      buffers.push({ id: buffers.length, length: (end - r.off) * 4, label: r.name, arena: false });
+      starts.push(r.off);
+      state[buffers.length - 1] = mem.slice(r.off, end);
+    });
+    const arenaId = buffers.length;
+    buffers.push({ id: arenaId, length: (arenaEnd - arena) * 4, label: 'arena', arena: true });
+    state[arenaId] = mem.slice(arena, arenaEnd);
+    const fields = {};
+    for (const n of fieldNames) fields[n] = (L.fields[n] - arena) * 4;
+    const n = prog.names.length;
+    const A = prog.args;
+    const payloads = new Set();
+    for (let i = 0; i < n; i++) for (let q = 0; q < 48; q++) if (A[i * 64 + 4 + q] >= arenaEnd) payloads.add(A[i * 64 + 4 + q]);
+    const sorted = [...payloads].sort((a, b) => a - b);
+    const nextPayload = new Map(sorted.map((o, k) => [o, k + 1 < sorted.length ? sorted[k + 1] : memWords]));
+    const regionOf = w => {
+      for (let b = starts.length - 1; b >= 0; b--) if (w >= starts[b] && buffers[b].length > 0) return b;
+      return -1;
+    };
+    const b64 = (w0, w1) => Buffer.from(mem.buffer, mem.byteOffset + w0 * 4, (w1 - w0) * 4).toString('base64');
+    const cmds = [];
+    for (let i = 0; i < n; i++) {
+      const a = A.subarray(i * 64, i * 64 + 64);
+      if (prog.names[i] === '_copy') {
+        cmds.push({ op: 'copy', src: arenaId, srcOffset: (a[4] - arena) * 4, dst: arenaId, dstOffset: (a[5] - arena) * 4,
+          size: a[0] * 4 });
+        continue;
+      }
+      cmds.push({ op: 'pipe', name: prog.names[i] });
+      for (let q = 0; q < 48; q++) {
+        const w = a[4 + q];
+        if (w === 0 && q !== 0) continue;
+        if (w >= arenaEnd) cmds.push({ op: 'bytes', index: q, data: b64(w, nextPayload.get(w)) });
+        else if (w >= arena) cmds.push({ op: 'buf', index: q, buf: arenaId, offset: (w - arena) * 4 });
+        else {
+          const b = regionOf(w);
+          cmds.push({ op: 'buf', index: q, buf: b, offset: (w - starts[b]) * 4 });
+        }
+      }
+      cmds.push({ op: 'dispatch', grid: [a[0], a[1], a[2]], kind: 'threads', tg: [1, 1, 1] });
+      cmds.push({ op: 'end' });
+    }
+    const steps = [];
+    for (let s = 0; s < nsteps; s++) steps.push(cmds);
+    const tape = {
+      nx: L.nx, ny: L.ny, nz: L.nz, dt: flat.dt != null ? flat.dt : addon().configDefaults().dt, halo: 3, warmSteps: 0,
+      constants: { terrain: false, axisymmetric: false, warmOnly: false, thermoDefault: true, testcase: 0 },
+      buffers, fields, fieldStride: L.fieldStrideWords * 4, steps, diag: [],
+    };
+    return { tape, state };
+  } finally {
+    addon().destroy(enc);
+  }
 }
 
 // ---------------------------------------------------------------- clock

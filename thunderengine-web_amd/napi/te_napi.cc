@@ -130,7 +130,7 @@ const FieldDesc kFields[] = {
     FD(species, I32), FD(microphysics, I32), FD(mass_conservation, I32), FD(cache_moisture_turbulence, I32),
     FD(stretch_x, I32), FD(stretch_y, I32), FD(dx_inner, F64), FD(dx_outer, F64), FD(nos_x_len, F64),
     FD(tot_x_len, F64), FD(dy_inner, F64), FD(dy_outer, F64), FD(nos_y_len, F64), FD(tot_y_len, F64),
-    FD(adaptive_dt, I32), FD(tapfrq, F32), FD(statfrq, F32), FD(rstfrq, F32),
+    FD(adaptive_dt, I32), FD(tapfrq, F32), FD(statfrq, F32), FD(rstfrq, F32), FD(sounding_winds, I32),
 };
 #undef FD
 
@@ -151,7 +151,7 @@ napi_value config_to_js(napi_env env, const te_config& c) {
 // Reads every known key present on `o` into c; zf (array of nz+1 face
 // heights) and arena_fields (array of names) into the holders' storage.
 bool config_from_js(napi_env env, napi_value o, te_config* c, std::vector<float>* zf, std::vector<std::string>* names,
-                    std::vector<const char*>* cnames) {
+                    std::vector<const char*>* cnames, std::string* sounding) {
   te_config_defaults(c);
   for (const FieldDesc& f : kFields) {
     bool has = false;
@@ -196,6 +196,23 @@ bool config_from_js(napi_env env, napi_value o, te_config* c, std::vector<float>
       c->zf = zf->data();
     }
   }
+  napi_has_named_property(env, o, "input_sounding", &has);
+  if (has) {
+    napi_value t;
+    napi_get_named_property(env, o, "input_sounding", &t);
+    napi_valuetype ty;
+    napi_typeof(env, t, &ty);
+    if (ty == napi_string) {
+      size_t len = 0;
+      napi_get_value_string_utf8(env, t, nullptr, 0, &len);
+      sounding->assign(len, '\0');
+      napi_get_value_string_utf8(env, t, &(*sounding)[0], len + 1, &len);
+      c->input_sounding = sounding->c_str();
+    } else if (ty != napi_undefined && ty != napi_null) {
+      napi_throw_type_error(env, nullptr, "config.input_sounding must be a string");
+      return false;
+    }
+  }
   napi_has_named_property(env, o, "arena_fields", &has);
   if (has) {
     napi_value a;
@@ -234,8 +251,9 @@ napi_value Create(napi_env env, napi_callback_info info) {
   std::vector<float> zf;
   std::vector<std::string> names;
   std::vector<const char*> cnames;
+  std::string sounding;
   if (argc >= 1) {
-    if (!config_from_js(env, argv[0], &c, &zf, &names, &cnames)) return nullptr;
+    if (!config_from_js(env, argv[0], &c, &zf, &names, &cnames, &sounding)) return nullptr;
   } else {
     te_config_defaults(&c);
   }
@@ -567,6 +585,28 @@ napi_value Layout(napi_env env, napi_callback_info info) {
   return out;
 }
 
+// [{name, off}] of the context's regions in memory order
+napi_value Regions(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!get_args(env, info, 1, argv)) return nullptr;
+  te_ctx* c = ctx_of(env, argv[0]);
+  if (!c) return nullptr;
+  const int32_t n = te_region_count(c);
+  napi_value arr;
+  NAPI_OK(napi_create_array_with_length(env, n, &arr));
+  for (int32_t i = 0; i < n; ++i) {
+    const char* nm = te_region_name(c, i);
+    uint64_t off = 0;
+    te_region_offset(c, nm, &off);
+    napi_value o;
+    NAPI_OK(napi_create_object(env, &o));
+    NAPI_OK(napi_set_named_property(env, o, "name", make_str(env, nm)));
+    NAPI_OK(napi_set_named_property(env, o, "off", make_num(env, (double)off)));
+    NAPI_OK(napi_set_element(env, arr, i, o));
+  }
+  return arr;
+}
+
 napi_value KernelNames(napi_env env, napi_callback_info) {
   napi_value arr;
   int32_t n = te_kernel_count();
@@ -749,6 +789,7 @@ napi_value Init(napi_env env, napi_value exports) {
       {"adjUpload", AdjUpload}, {"adjDownload", AdjDownload}, {"adjLastMs", AdjLastMs}, {"adjDestroy", AdjDestroy},
       {"setTimeStep", SetTimeStep}, {"adaptiveStart", AdaptiveStart}, {"adaptiveStep", AdaptiveStep},
       {"adaptiveClock", AdaptiveClock}, {"clockCreate", ClockCreate}, {"clockCall", ClockCall},
+      {"regions", Regions},
   };
   for (auto& f : fns) {
     napi_value v;

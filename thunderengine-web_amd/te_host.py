@@ -60,7 +60,7 @@ class Config(ctypes.Structure):
         ("dy_inner", ctypes.c_double), ("dy_outer", ctypes.c_double), ("nos_y_len", ctypes.c_double),
         ("tot_y_len", ctypes.c_double),
         ("adaptive_dt", ctypes.c_int32), ("tapfrq", ctypes.c_float), ("statfrq", ctypes.c_float),
-        ("rstfrq", ctypes.c_float),
+        ("rstfrq", ctypes.c_float), ("input_sounding", ctypes.c_char_p), ("sounding_winds", ctypes.c_int32),
     ]
 
 
@@ -175,6 +175,10 @@ def config(**kw):
             arr = (ctypes.c_float * len(v))(*v)
             keep.append(arr)
             c.zf = ctypes.cast(arr, ctypes.POINTER(ctypes.c_float))
+        elif k == "input_sounding":
+            b = v.encode() if isinstance(v, str) else v
+            keep.append(b)
+            c.input_sounding = b
         elif k == "arena_fields":
             arr = (ctypes.c_char_p * len(v))(*[s.encode() for s in v])
             keep.append(arr)
