@@ -228,6 +228,7 @@ __device__ __forceinline__ void ac_column(uint32_t* M, const KArg& a) {
   __shared__ float xw[AC_WAVES][64];     // new w at each wave's first level
   __shared__ float lv[6][KMAX];          // acoustic_prepare's per-level factors: tem, r1, r2, tem1, c1, c2
   __shared__ int hcnt;
+  __shared__ int fwd_done, back_next;  // the sweep's hand-offs between waves
   __shared__ short hcell[AC_HALO_CELLS][3];
   // dynamic: per level and lane mm (then cA, then the new w) and the w
   // row's right-hand side part (then cB): 2 * nz * 64 floats
@@ -246,7 +247,11 @@ __device__ __forceinline__ void ac_column(uint32_t* M, const KArg& a) {
   auto F = [&](int q) { return reinterpret_cast<float*>(M) + a.p[q]; };
   const bool saveX = ldi(a.q[16], 0) != 0, first = ldi(a.q[33], 0) != 0;
   const float tavg = ldf(a.q[17], 0);
-  if (threadIdx.x == 0 && threadIdx.y == 0) hcnt = 0;
+  if (threadIdx.x == 0 && threadIdx.y == 0) {
+    hcnt = 0;
+    fwd_done = 0;
+    back_next = AC_WAVES - 1;
+  }
   {
     const float *rr0 = F(18), *rf0 = F(19), *mh = F(20), *mf = F(21), *c1 = F(22), *c2 = F(23);
     for (int k = (int)(threadIdx.y * 64 + threadIdx.x); k <= nz; k += 64 * AC_WAVES) {
@@ -346,13 +351,31 @@ __device__ __forceinline__ void ac_column(uint32_t* M, const KArg& a) {
         xa[wave][2][lane] = PF[t];
       }
   }
-  __syncthreads();
+  // The sweep passes from wave to wave through LDS counters instead of
+  // workgroup barriers, so a wave's share of the serial recurrence overlaps
+  // the loads of the waves above it (forward) and the stores of the waves
+  // that finished (backward).  Every wave passes every hand-off, in wave
+  // order: wave w waits for fwd_done == w (set by wave w-1 after its
+  // elimination) and for back_next == w (set by wave w+1 after its back
+  // substitution; the top wave is the last of the forward chain), so each
+  // wait ends once the waves below / above it have run.  The waits are
+  // bounded (2^22 polls) all the same.
+  auto wait_for = [&](int* flag, int want) {
+    for (int s = 0; s < (1 << 22); ++s) {
+      const int v = __builtin_amdgcn_readfirstlane(
+          __hip_atomic_load(flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP));
+      if (v == want) break;
+      __builtin_amdgcn_s_sleep(1);
+    }
+  };
+  auto pass = [&](int* flag, int v) { __hip_atomic_store(flag, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP); };
   // ---- forward elimination, wave after wave (cA over mm, cB over the
   // right-hand side part; the top row leaves w at nz-1 there).  Rows k = 1
   // and k = nz-1 are the reference's special cases, taken by selects.  Lanes
   // past nx compute on unset values and touch no global memory here.
-  for (int step = 0; step < AC_WAVES; ++step) {
-    if (wave == step && Lq > 0) {
+  wait_for(&fwd_done, wave);
+  {
+    if (Lq > 0) {
       float pa = 0.0f, pb = 0.0f, pq_m = 0.0f, p2q_m = 0.0f, ppf_m = 0.0f;
       if (wave > 0) {
         pa = xb[0][lane];
@@ -386,11 +409,12 @@ __device__ __forceinline__ void ac_column(uint32_t* M, const KArg& a) {
       xb[0][lane] = pa;
       xb[1][lane] = pb;
     }
-    __syncthreads();
+    pass(&fwd_done, wave + 1);
   }
   // ---- back substitution, wave after wave downwards (new w over cA)
-  for (int step = AC_WAVES - 1; step >= 0; --step) {
-    if (wave == step && Lq > 0) {
+  wait_for(&back_next, wave);
+  {
+    if (Lq > 0) {
       float wn = top ? 0.0f : xw[wave + 1][lane];
 #pragma unroll
       for (int t = LMAX - 1; t >= 0; --t) {
@@ -403,7 +427,7 @@ __device__ __forceinline__ void ac_column(uint32_t* M, const KArg& a) {
       }
       xw[wave][lane] = me0[k0 * 64 + lane];
     }
-    __syncthreads();
+    pass(&back_next, wave - 1);
   }
   // ---- acoustic_pp, the rru / rrv / rrw accumulation (acoustic_begin's
   // values first on the stage's first substep), new w; the loads of 4 levels
