@@ -373,6 +373,15 @@ __device__ __forceinline__ void ac_column(uint32_t* M, const KArg& a) {
   // right-hand side part; the top row leaves w at nz-1 there).  Rows k = 1
   // and k = nz-1 are the reference's special cases, taken by selects.  Lanes
   // past nx compute on unset values and touch no global memory here.
+  // The rows' operands from LDS into registers before the wait (the
+  // recurrence then waits on nothing but itself).
+  float ME[LMAX], EE[LMAX];
+#pragma unroll
+  for (int t = 0; t < LMAX; ++t)
+    if (t < Lq) {
+      ME[t] = me0[(k0 + t) * 64 + lane];
+      EE[t] = me1[(k0 + t) * 64 + lane];
+    }
   wait_for(&fwd_done, wave);
   {
     if (Lq > 0) {
@@ -389,7 +398,7 @@ __device__ __forceinline__ void ac_column(uint32_t* M, const KArg& a) {
         const int k = k0 + t;
         if (t < Lq) {
           if (k >= 1) {
-            const float m = me0[k * 64 + lane], e = me1[k * 64 + lane];
+            const float m = ME[t], e = EE[t];
             const float aa = (-asq) * (m * pq_m);
             const float cc = (-asq) * ((-m) * P2Q[t]);
             const float bb = fma_(-asq, m * ((-PQ[t]) + p2q_m), 1.0f);
@@ -398,8 +407,8 @@ __device__ __forceinline__ void ac_column(uint32_t* M, const KArg& a) {
             const float r = 1.0f / (first_row ? bb : fma_(aa, pa, bb));
             pa = (-cc) * r;
             pb = (first_row ? dd : fma_(-aa, pb, dd)) * r;
-            me0[k * 64 + lane] = pa;  // not read for the top row (w there is pb)
-            me1[k * 64 + lane] = pb;
+            ME[t] = pa;  // not read for the top row (w there is pb)
+            EE[t] = pb;
           }
           pq_m = PQ[t];
           p2q_m = P2Q[t];
@@ -411,7 +420,8 @@ __device__ __forceinline__ void ac_column(uint32_t* M, const KArg& a) {
     }
     pass(&fwd_done, wave + 1);
   }
-  // ---- back substitution, wave after wave downwards (new w over cA)
+  // ---- back substitution, wave after wave downwards (new w over cA); the
+  // elimination coefficients of the wave's levels stayed in registers (ME, EE)
   wait_for(&back_next, wave);
   {
     if (Lq > 0) {
@@ -420,12 +430,11 @@ __device__ __forceinline__ void ac_column(uint32_t* M, const KArg& a) {
       for (int t = LMAX - 1; t >= 0; --t) {
         const int k = k0 + t;
         if (t < Lq) {
-          const float ca = me0[k * 64 + lane], cb = me1[k * 64 + lane];
-          wn = k == nz - 1 ? cb : (k >= 1 ? fma_(ca, wn, cb) : 0.0f);
+          wn = k == nz - 1 ? EE[t] : (k >= 1 ? fma_(ME[t], wn, EE[t]) : 0.0f);
           me0[k * 64 + lane] = wn;
         }
       }
-      xw[wave][lane] = me0[k0 * 64 + lane];
+      xw[wave][lane] = wn;
     }
     pass(&back_next, wave - 1);
   }
