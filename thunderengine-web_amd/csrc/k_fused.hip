@@ -2,6 +2,8 @@
 // `fuse`).  Each produces the same bits as the run it replaces on every
 // field the step carries.
 #include "te_device.hpp"
+#include <type_traits>
+
 #include "te_kernels.hpp"
 
 namespace te {
@@ -202,15 +204,17 @@ namespace te {
 // sequential Thomas recurrence, level by level.
 //
 // Halo cells (every halo cell of ppd, u, v, w is the image of interior cell
-// (i mod nx, j mod ny), as the halo fill leaves it):
-//  * w and ppd: the thread of the interior column writes its images; it also
-//    accumulates rrw at the image with the image's old w (the accumulate
-//    runs before the fill).
-//  * u and v: the thread of the nearest interior column accumulates rru, rrv
-//    at the halo cell with the value there (acoustic_uv's at i = nx or
-//    j = ny, else the old image) and then copies the image in.  It is also the
-//    only thread that reads u at i = nx (acoustic_fpk of column nx-1) or v at
-//    j = ny, so that read comes before the copy.
+// (i mod nx, j mod ny), as the halo fill leaves it), each updated by the
+// workgroup of its interior cell, at each level by the wave that owns the
+// level, after that wave's own stores:
+//  * w and ppd: rrw accumulates the image's old w (the accumulate runs
+//    before the fill), then the new values are copied in.
+//  * u and v: rru, rrv accumulate the value there (acoustic_uv's at i = nx or
+//    j = ny, else the old image), then the image is copied in.  u at
+//    (nx, j) and v at (i, ny) are updated by the workgroup of column nx-1 /
+//    row ny-1, the only one that reads them (acoustic_fpk), after its reads.
+// The row's own images (hx = i) go column by column, lane = column; the
+// x-images and corners through a short per-workgroup list.
 // p: 0 dims, 1 acoustic, 2 u, 3 v, 4 w, 5 piadv, 6 ppten, 7 ppterm, 8 thv,
 //    9 wten, 10 pp3d, 11 ppd, 12 ppx, 13 rru, 14 rrv, 15 rrw, 16 -> save ppx,
 //    17 -> tavg, 18 rr0, 19 rf0, 20 mh, 21 mf, 22 c1, 23 c2, 30 metric,
@@ -219,7 +223,7 @@ namespace te {
 //    25 pk1, 26 pk2, 27 mm, 28 fpk, 29 cA, 31 cB, and 32 rho0.
 // Grid: 64 x AC_WAVES threads per workgroup, ceil(nx / 64) x ny workgroups.
 // Needs nx, ny >= 4 and 4 <= nz <= AC_WAVES * LMAX.
-#define AC_HALO_CELLS 160  // per workgroup: at most 64 y-images, 64 (i, ny) cells, 7 x-images, 4 (nx, j), 21 corners
+#define AC_HALO_CELLS 40  // per workgroup: at most 7 x-images, 1 (nx, j) cell, 14 corners
 template <int LMAX>
 __device__ __forceinline__ void ac_column(uint32_t* M, const KArg& a) {
   constexpr int KMAX = AC_WAVES * 16 + 1;
@@ -252,6 +256,35 @@ __device__ __forceinline__ void ac_column(uint32_t* M, const KArg& a) {
     fwd_done = 0;
     back_next = AC_WAVES - 1;
   }
+  // Halo cells this workgroup updates, (hx, hy, groups):
+  // group 1 = u, rru; 2 = v, rrv; 4 = w, rrw, ppd.  A halo cell's groups are
+  // handled by the workgroup of its image (i mod nx, j mod ny), except u at
+  // (nx, j) and v at (i, ny): acoustic_fpk of column nx-1 / row ny-1 reads
+  // them, so that column's / row's workgroup updates them after its reads.
+  // The images of the workgroup's own row (hx = i: hy = j -+ ny, and v at
+  // (i, ny) for row ny-1) are handled column by column (row_images); the list
+  // holds the x-images, the corners and u at (nx, j).
+  const int ys[2] = {j >= ny - 3 ? j - ny : j, j <= 3 ? j + ny : j};
+  if (active && wave == 0) {
+    const int xs[2] = {i >= nx - 3 ? i - nx : i, i <= 3 ? i + nx : i};
+    const int nxi = (i >= nx - 3) + (i <= 3), nyi = (j >= ny - 3) + (j <= 3);
+    const int n = nxi * (1 + nyi) + (i == nx - 1);
+    if (n > 0) {
+      int q = atomicAdd(&hcnt, n);
+      auto put = [&](int hx, int hy, int m) {
+        hcell[q][0] = (short)hx; hcell[q][1] = (short)hy; hcell[q][2] = (short)m;
+        ++q;
+      };
+      for (int by = 0; by < 3; ++by)
+        for (int bx = 1; bx < 3; ++bx) {
+          const int hx = xs[bx - 1], hy = by == 0 ? j : ys[by - 1];
+          if (hx == i || (by > 0 && hy == j)) continue;
+          put(hx, hy, 7 & ~(hx == nx && hy == j ? 1 : 0));
+        }
+      if (i == nx - 1) put(nx, j, 1);
+    }
+  }
+
   {
     const float *rr0 = F(18), *rf0 = F(19), *mh = F(20), *mf = F(21), *c1 = F(22), *c2 = F(23);
     for (int k = (int)(threadIdx.y * 64 + threadIdx.x); k <= nz; k += 64 * AC_WAVES) {
@@ -273,33 +306,115 @@ __device__ __forceinline__ void ac_column(uint32_t* M, const KArg& a) {
   const bool vec_v = nx > 3 && i < (nx / 4) * 4;
   auto acc = [&](float base, float x) { return tavg > 0.0f ? (base + x) * tavg : base + x; };
   auto accw = [&](float base, float x) { return tavg > 0.0f ? fma_(x, ac.alph, base) * tavg : base + x; };
-  // Halo cells this workgroup updates after the sweep, (hx, hy, groups):
-  // group 1 = u, rru; 2 = v, rrv; 4 = w, rrw, ppd.  A halo cell's groups are
-  // handled by the workgroup of its image (i mod nx, j mod ny), except u at
-  // (nx, j) and v at (i, ny): acoustic_fpk of column nx-1 / row ny-1 reads
-  // them, so that column's / row's workgroup updates them after its reads.
-  if (active && wave == 0) {
-    const int xs[2] = {i >= nx - 3 ? i - nx : i, i <= 3 ? i + nx : i};
-    const int ys[2] = {j >= ny - 3 ? j - ny : j, j <= 3 ? j + ny : j};
-    const int nxi = (i >= nx - 3) + (i <= 3), nyi = (j >= ny - 3) + (j <= 3);
-    const int n = (1 + nxi) * (1 + nyi) - 1 + (i == nx - 1) + (j == ny - 1);
-    if (n > 0) {
-      int q = atomicAdd(&hcnt, n);
-      auto put = [&](int hx, int hy, int m) {
-        hcell[q][0] = (short)hx; hcell[q][1] = (short)hy; hcell[q][2] = (short)m;
-        ++q;
-      };
-      for (int by = 0; by < 3; ++by)
-        for (int bx = 0; bx < 3; ++bx) {
-          const int hx = bx == 0 ? i : xs[bx - 1], hy = by == 0 ? j : ys[by - 1];
-          if ((bx > 0 && hx == i) || (by > 0 && hy == j) || (bx == 0 && by == 0)) continue;
-          put(hx, hy, 7 & ~(hx == nx && hy == j ? 1 : 0) & ~(hy == ny && hx == i ? 2 : 0));
-        }
-      if (i == nx - 1) put(nx, j, 1);
-      if (j == ny - 1) put(i, ny, 2);
+  // Halo cells, each wave at its own levels (and level nz for the top wave)
+  // once its stores above are done.  u, v (groups 1, 2): rru, rrv accumulate
+  // the value there (acoustic_uv's at i = nx or j = ny, else the old image),
+  // then the image is copied in; the interior values are acoustic_uv's, and
+  // the wave's own acoustic_fpk reads of u at i = nx and v at j = ny came
+  // first.  w, ppd (group 4): rrw accumulates the old w there, then the new w
+  // and ppd of the interior cell are copied in; the interior cell is in this
+  // workgroup's row at the same level, written by this wave.
+  // halo(): the cells of the list, 2 (cell, level) items per lane per round
+  // with their loads before their stores.
+  // The row's own images, lane = column, 2 levels of loads before their
+  // stores: r = 0, 1 the images at hy = j -+ ny (v excluded at hy = ny: row
+  // ny-1's workgroup handles v there), r = 2 v at (i, ny) for row ny-1.
+  auto row_images = [&](auto G) {
+    constexpr int groups = decltype(G)::value;
+    if (!active) return;
+    float *u = F(2), *v = F(3), *w = F(4), *ppd = F(11), *rru = F(13), *rrv = F(14), *rrw = F(15);
+    const int nl = Lq + (top ? 1 : 0);
+    for (int r = 0; r < 3; ++r) {
+      int hy, m;
+      if (r < 2) {
+        hy = ys[r];
+        if (hy == j || (r == 1 && ys[0] == ys[1])) continue;
+        m = 7 & ~(hy == ny ? 2 : 0);
+      } else {
+        if (j != ny - 1) continue;
+        hy = ny;
+        m = 2;
+      }
+      m &= groups;
+      if (m == 0) continue;
+      const int sy = hy < 0 ? hy + ny : (hy >= ny ? hy - ny : hy);
+      for (int t0 = 0; t0 < nl; t0 += 2) {
+        float x[2][10];
+#pragma unroll
+        for (int d = 0; d < 2; ++d)
+          if (t0 + d < nl) {
+            const int t = t0 + d, k = t < Lq ? k0 + t : nz;
+            const int64_t ch = g.idx(i, hy, k), cs = g.idx(i, sy, k);
+            if (m & 1) { x[d][0] = u[ch]; x[d][1] = u[cs]; x[d][2] = rru[ch]; }
+            if (m & 2) { x[d][3] = v[ch]; x[d][4] = v[cs]; x[d][5] = rrv[ch]; }
+            if (m & 4) { x[d][6] = w[ch]; x[d][7] = rrw[ch]; x[d][8] = w[cs]; x[d][9] = ppd[cs]; }
+          }
+#pragma unroll
+        for (int d = 0; d < 2; ++d)
+          if (t0 + d < nl) {
+            const int t = t0 + d, k = t < Lq ? k0 + t : nz;
+            const int64_t ch = g.idx(i, hy, k);
+            if (m & 1) {
+              rru[ch] = acc(first ? 0.0f : x[d][2], x[d][0]);
+              u[ch] = x[d][1];
+            }
+            if (m & 2) {
+              rrv[ch] = acc(first ? 0.0f : x[d][5], x[d][3]);
+              v[ch] = x[d][4];
+            }
+            if (m & 4) {
+              rrw[ch] = accw(first ? x[d][6] * ac.beta : x[d][7], x[d][6]);
+              w[ch] = x[d][8];
+              ppd[ch] = x[d][9];
+            }
+          }
+      }
     }
-  }
-
+  };
+  auto halo = [&](auto G) {
+    constexpr int groups = decltype(G)::value;
+    float *u = F(2), *v = F(3), *w = F(4), *ppd = F(11), *rru = F(13), *rrv = F(14), *rrw = F(15);
+    const int nc = hcnt, nl = Lq + (top ? 1 : 0), items = nc * nl;
+    constexpr int B = 2;
+    for (int it0 = lane; it0 < items; it0 += B * 64) {
+      float x[B][10];
+      int64_t hc[B];
+      int mask[B];
+#pragma unroll
+      for (int d = 0; d < B; ++d) {
+        const int it = it0 + d * 64;
+        mask[d] = 0;
+        if (it < items) {
+          const int kk = it / nc, q = it - kk * nc, k = kk < Lq ? k0 + kk : nz;
+          const int hx = hcell[q][0], hy = hcell[q][1];
+          const int sx = hx < 0 ? hx + nx : (hx >= nx ? hx - nx : hx), sy = hy < 0 ? hy + ny : (hy >= ny ? hy - ny : hy);
+          const int64_t ch = g.idx(hx, hy, k), cs = g.idx(sx, sy, k);
+          hc[d] = ch;
+          mask[d] = hcell[q][2] & groups;
+          if (mask[d] & 1) { x[d][0] = u[ch]; x[d][1] = u[cs]; x[d][2] = rru[ch]; }
+          if (mask[d] & 2) { x[d][3] = v[ch]; x[d][4] = v[cs]; x[d][5] = rrv[ch]; }
+          if (mask[d] & 4) { x[d][6] = w[ch]; x[d][7] = rrw[ch]; x[d][8] = w[cs]; x[d][9] = ppd[cs]; }
+        }
+      }
+#pragma unroll
+      for (int d = 0; d < B; ++d) {
+        const int64_t ch = hc[d];
+        if (mask[d] & 1) {
+          rru[ch] = acc(first ? 0.0f : x[d][2], x[d][0]);
+          u[ch] = x[d][1];
+        }
+        if (mask[d] & 2) {
+          rrv[ch] = acc(first ? 0.0f : x[d][5], x[d][3]);
+          v[ch] = x[d][4];
+        }
+        if (mask[d] & 4) {
+          rrw[ch] = accw(first ? x[d][6] * ac.beta : x[d][7], x[d][6]);
+          w[ch] = x[d][8];
+          ppd[ch] = x[d][9];
+        }
+      }
+    }
+  };
   float PQ[LMAX], P2Q[LMAX], PF[LMAX];
   // ---- pointwise: acoustic_prepare's terms, acoustic_fpk, the w row's
   // right-hand side (levels -1 and nz, read at the column's ends and not
@@ -490,56 +605,11 @@ __device__ __forceinline__ void ac_column(uint32_t* M, const KArg& a) {
       rrw[c] = accw(rw, 0.0f);
     }
   }
-  __syncthreads();
-  // ---- halo cells, (cell, level) items over the whole workgroup, 2 items
-  // per thread per round with their loads before their stores.  u, v: rru,
-  // rrv accumulate the value there (acoustic_uv's at i = nx or j = ny, else
-  // the old image), then the image is copied in.  w, ppd: rrw accumulates
-  // the old w there, then the new w and ppd of the interior cell (written
-  // above by this workgroup) are copied in.
-  {
-    float *u = F(2), *v = F(3), *w = F(4), *ppd = F(11), *rru = F(13), *rrv = F(14), *rrw = F(15);
-    const int nc = hcnt, items = nc * (nz + 1);
-    constexpr int B = 2;
-    for (int it0 = (int)(threadIdx.y * 64 + threadIdx.x); it0 < items; it0 += B * 64 * AC_WAVES) {
-      float x[B][10];
-      int64_t hc[B];
-      int mask[B];
-#pragma unroll
-      for (int d = 0; d < B; ++d) {
-        const int it = it0 + d * 64 * AC_WAVES;
-        mask[d] = 0;
-        if (it < items) {
-          const int k = it / nc, q = it - k * nc;
-          const int hx = hcell[q][0], hy = hcell[q][1];
-          const int sx = hx < 0 ? hx + nx : (hx >= nx ? hx - nx : hx), sy = hy < 0 ? hy + ny : (hy >= ny ? hy - ny : hy);
-          const int64_t ch = g.idx(hx, hy, k), cs = g.idx(sx, sy, k);
-          hc[d] = ch;
-          mask[d] = hcell[q][2];
-          if (mask[d] & 1) { x[d][0] = u[ch]; x[d][1] = u[cs]; x[d][2] = rru[ch]; }
-          if (mask[d] & 2) { x[d][3] = v[ch]; x[d][4] = v[cs]; x[d][5] = rrv[ch]; }
-          if (mask[d] & 4) { x[d][6] = w[ch]; x[d][7] = rrw[ch]; x[d][8] = w[cs]; x[d][9] = ppd[cs]; }
-        }
-      }
-#pragma unroll
-      for (int d = 0; d < B; ++d) {
-        const int64_t ch = hc[d];
-        if (mask[d] & 1) {
-          rru[ch] = acc(first ? 0.0f : x[d][2], x[d][0]);
-          u[ch] = x[d][1];
-        }
-        if (mask[d] & 2) {
-          rrv[ch] = acc(first ? 0.0f : x[d][5], x[d][3]);
-          v[ch] = x[d][4];
-        }
-        if (mask[d] & 4) {
-          rrw[ch] = accw(first ? x[d][6] * ac.beta : x[d][7], x[d][6]);
-          w[ch] = x[d][8];
-          ppd[ch] = x[d][9];
-        }
-      }
-    }
-  }
+  // ---- halo cells of the wave's levels, after its own stores above
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+  row_images(std::integral_constant<int, 7>());
+  halo(std::integral_constant<int, 7>());
 }
 // instantiations by levels per wave: the runtime picks the one with
 // ceil(nz / AC_WAVES) levels (k_ac_column: 16)
