@@ -375,7 +375,7 @@ __global__ __launch_bounds__(256) void k_pressure_tendency(uint32_t* M, KArg a) 
     float p = 0.0f;
     if (ab(dt * tht) > tem) {
       float thnew = fma_(dt, tht, th[c]);
-      float pinew = rr_pow(((rho[c] * (th0[k] + thnew)) * RD) * rp00, rddcv) - pi0[k];
+      float pinew = fast_pow(((rho[c] * (th0[k] + thnew)) * RD) * rp00, rddcv) - pi0[k];
       p = (pinew - ppi[c]) / dt;
     }
     ppten[c] = p;
@@ -386,7 +386,7 @@ __global__ __launch_bounds__(256) void k_pressure_tendency(uint32_t* M, KArg a) 
   if (ab(dt * tht) > tem || ab(dt * qvten) > 1e-14f) {
     float thnew = fma_(dt, tht, th[c]);
     float qvnew = fma_(dt, qvten, qv[c]);
-    float pinew = rr_pow(((rho[c] * (th0[k] + thnew)) * fma_(RV, qvnew, RD)) * rp00, rddcv) - pi0[k];
+    float pinew = fast_pow(((rho[c] * (th0[k] + thnew)) * fma_(RV, qvnew, RD)) * rp00, rddcv) - pi0[k];
     p = (pinew - ppi[c]) / dt;
   }
   if (hasDiabatic != 0) p = p + qpten[c];
@@ -407,7 +407,7 @@ __global__ __launch_bounds__(256) void k_update_rho_prs(uint32_t* M, KArg a) {
   const float cpdrd = CP / RD;
   float pi = pi0[k] + ppP[c];
   float th = th0[k] + thP[c];
-  float p = P00 * rr_pow(pi, cpdrd);
+  float p = P00 * fast_pow(pi, cpdrd);
   prs[c] = p;
   float rmix = moist != 0 ? fma_(mx(0.0f, qv[c]), RV, RD) : RD;
   rho[c] = p / ((th * pi) * rmix);

@@ -83,7 +83,7 @@ __global__ __launch_bounds__(256) void k_mass_reference_init(uint32_t* M, KArg a
 // f_powm1 (mass_correction_apply.wgsl:121-128).  The |r| > 0.05 branch calls
 // the WGSL pow builtin; the correctly rounded pow stands in for it.
 TE_DEV float powm1(float r, float a) {
-  if (ab(r) > 0.05f) return rr_pow(1.0f + r, a) - 1.0f;
+  if (ab(r) > 0.05f) return fast_pow(1.0f + r, a) - 1.0f;
   return r * fma_(r, fma_(a * (a - 1.0f), 0.5f, (((r * a) * (a - 1.0f)) * (a - 2.0f)) / 6.0f), a);
 }
 
@@ -115,7 +115,7 @@ __global__ __launch_bounds__(256) void k_mass_correction_apply(uint32_t* M, KArg
   const float cpdrd = CP / RD;
   const float pi = pi0[k] + pp;
   const float th = th0[k] + thP[c];
-  const float p = P00 * rr_pow(pi, cpdrd);
+  const float p = P00 * fast_pow(pi, cpdrd);
   prs[c] = p;
   const float rmix = moist != 0 ? fma_(mx(0.0f, qv[c]), RV, RD) : RD;
   rho[c] = p / ((th * pi) * rmix);

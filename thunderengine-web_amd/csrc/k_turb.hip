@@ -126,12 +126,12 @@ __global__ __launch_bounds__(256) void k_turb_def_assemble(uint32_t* M, KArg a) 
 
 // saturation mixing ratios over liquid / ice (turb_nm.wgsl:204-230)
 TE_DEV float mp_rslf(float p, float t) {
-  float esl = 611.2f * rr_exp((17.67f * (t - 273.15f)) / (t - 29.65f));
+  float esl = 611.2f * fast_exp((17.67f * (t - 273.15f)) / (t - 29.65f));
   esl = mn(esl, p * 0.5f);
   return ((RD / RV) * esl) / (p - esl);
 }
 TE_DEV float mp_rsif(float p, float t) {
-  float esi = 611.2f * rr_exp((21.874557f * (t - 273.15f)) / (t - 7.66f));
+  float esi = 611.2f * fast_exp((21.874557f * (t - 273.15f)) / (t - 7.66f));
   esi = mn(esi, p * 0.5f);
   return ((RD / RV) * esi) / (p - esi);
 }
@@ -154,7 +154,7 @@ __global__ __launch_bounds__(256) void k_turb_nm(uint32_t* M, KArg a) {
   float thA = th0[k - 1] + thP[c - sk];
   float thB = th0[k] + thP[c];
   if (moist == 0) {
-    nm[c] = ((rr_log(thB / thA) * gAcc) * g.rdz) * mf[k];
+    nm[c] = ((fast_log(thB / thA) * gAcc) * g.rdz) * mf[k];
     return;
   }
   float piA = pi0[k - 1] + ppP[c - sk], piB = pi0[k] + ppP[c];
@@ -166,9 +166,9 @@ __global__ __launch_bounds__(256) void k_turb_nm(uint32_t* M, KArg a) {
   float thvA = (thA * fma_(reps, qv[c - sk], 1.0f)) / (1.0f + qtA);
   float thvB = (thB * fma_(reps, qv[c], 1.0f)) / (1.0f + qtB);
   float rmf = g.rdz * mf[k];
-  float dlnT = rr_log(tB / tA) * rmf;
-  float dlnQt = rr_log((1.0f + qtB) / (1.0f + qtA)) * rmf;
-  float out = (gAcc * rr_log(thvB / thvA)) * rmf;
+  float dlnT = fast_log(tB / tA) * rmf;
+  float dlnQt = fast_log((1.0f + qtB) / (1.0f + qtA)) * rmf;
+  float out = (gAcc * fast_log(thvB / thvA)) * rmf;
   float A = c1[k], B = c2[k];
   float pavg = fma_(A, prs[c - sk], B * prs[c]);
   float tavg = fma_(A, tA, B * tB);
@@ -238,7 +238,7 @@ __global__ __launch_bounds__(256) void k_turb_km_kh(uint32_t* M, KArg a) {
   float ce1z = ((cmz * c_l) * c_l) * (1.0f / ri_c - 1.0f);
   float ce2z = mx(0.0f, fma_(cmz * 3.1415927f, 3.1415927f, -ce1z));
   float dzk = (1.0f / g.rdz) / mf[k];
-  float grdscl = rr_pow((((1.0f / g.rdx) / hm_uh(hm, g, i)) * ((1.0f / g.rdy) / hm_vh(hm, g, j))) * dzk, 0.33333334f);
+  float grdscl = fast_pow((((1.0f / g.rdx) / hm_uh(hm, g, i)) * ((1.0f / g.rdy) / hm_vh(hm, g, j))) * dzk, 0.33333334f);
   float rgrdscl = 1.0f / grdscl;
   float tk, len;
   if (tkea[c] <= tke_min) {
@@ -283,7 +283,7 @@ __global__ __launch_bounds__(256) void k_turb_dissipative_heating(uint32_t* M, K
   if (k == 0 && bbc3 != 0) {
     float ust = sfc[3 * sk + g.idx(i, j, 0)];
     float z0 = zntField != 0 ? sfc[4 * sk + g.idx(i, j, 0)] : znt;
-    epsd = (((ust * ust) * ust) * rr_log((dz1 + z0) / z0)) / (0.4f * dz1);
+    epsd = (((ust * ust) * ust) * fast_log((dz1 + z0) / z0)) / (0.4f * dz1);
   } else {
     epsd = 0.5f * (dissten[c] + dissten[c + sk]);
   }

@@ -219,6 +219,40 @@ TE_DEV float rr_exp(float x) {
   return ff_exp(FF{x, 0.0f});
 }
 
+// The same results faster: the routines above round a double-float
+// approximation of log / exp / pow to binary32.  A binary64 evaluation
+// (device libm, within a few ulp of binary64) rounds to the same float
+// whenever the value is not within 2^-40 (pow: 2^-32, its double-float
+// argument carries the error of log x times p) of a rounding boundary between
+// two floats (both approximations are closer than that); near a boundary
+// the double-float routine decides.  The library's self-test compares each
+// fast_* with its rr_* on every binary32 input (pow: at the exponents the
+// kernels use), bit for bit.
+TE_DEV bool f32_clear(double d, float& f, double margin = 0x1p-40) {
+  f = (float)d;
+  const double df = (double)f;
+  const uint32_t b = __float_as_uint(f);
+  // the neighbour of f on d's side: one step up in magnitude when d lies
+  // further from zero than f (signed zeros included)
+  const float nb = __uint_as_float(b + (((d > df) != ((b >> 31) != 0u)) ? 1u : 0xFFFFFFFFu));
+  return __builtin_fabs(d - 0.5 * (df + (double)nb)) > margin * __builtin_fabs(d);
+}
+TE_DEV float fast_log(float x) {
+  if (!(x > 0.0f) || (__float_as_uint(x) & 0x7fffffffu) == 0x7f800000u) return logf(x);
+  float f;
+  return f32_clear(log((double)x), f) ? f : rr_log(x);
+}
+TE_DEV float fast_exp(float x) {
+  if (!(ab(x) < 87.0f)) return expf(x);
+  float f;
+  return f32_clear(exp((double)x), f) ? f : rr_exp(x);
+}
+TE_DEV float fast_pow(float x, float p) {
+  if (!(x > 0.0f) || (__float_as_uint(x) & 0x7fffffffu) == 0x7f800000u) return powf(x, p);
+  float f;
+  return f32_clear(exp((double)p * log((double)x)), f, 0x1p-32) ? f : rr_pow(x, p);
+}
+
 // global thread ids of a 3-D launch
 // Elimination coefficients of a column sweep: level k of this lane's column,
 // either in the field at p (cA / cB of the reference) or, for the fused

@@ -210,6 +210,41 @@ __global__ void k_zero_words(uint32_t* p, int n) {
 // process ends), so that entry point allocates nothing.
 static __device__ __align__(256) uint32_t scratch[TE_NOALLOC_MAX_REGIONS];
 
+// Every binary32 bit pattern through fast_* and rr_* (te_selftest_math):
+// out[0..4] mismatch counts, [5..9] deferrals (64-bit), first[0..4].
+__global__ __launch_bounds__(256) void k_selftest_math(unsigned long long* out, uint32_t* first) {
+  const float cpdrd = te::CP / te::RD, rddcv = te::RD / (te::CP - te::RD), third = 0.33333334f;
+  uint32_t bad[5] = {0, 0, 0, 0, 0}, slow[5] = {0, 0, 0, 0, 0};
+  for (uint64_t n = blockIdx.x * 256ull + threadIdx.x; n < (1ull << 32); n += (uint64_t)gridDim.x * 256ull) {
+    const uint32_t bits = (uint32_t)n;
+    const float x = __uint_as_float(bits);
+    float r[5], q[5];
+    r[0] = te::fast_log(x); q[0] = te::rr_log(x);
+    r[1] = te::fast_exp(x); q[1] = te::rr_exp(x);
+    r[2] = te::fast_pow(x, cpdrd); q[2] = te::rr_pow(x, cpdrd);
+    r[3] = te::fast_pow(x, rddcv); q[3] = te::rr_pow(x, rddcv);
+    r[4] = te::fast_pow(x, third); q[4] = te::rr_pow(x, third);
+    double d[5] = {log((double)x), exp((double)x), exp((double)cpdrd * log((double)x)),
+                   exp((double)rddcv * log((double)x)), exp((double)third * log((double)x))};
+    const bool special[5] = {!(x > 0.0f) || (bits & 0x7fffffffu) == 0x7f800000u, !(te::ab(x) < 87.0f),
+                             !(x > 0.0f) || (bits & 0x7fffffffu) == 0x7f800000u,
+                             !(x > 0.0f) || (bits & 0x7fffffffu) == 0x7f800000u,
+                             !(x > 0.0f) || (bits & 0x7fffffffu) == 0x7f800000u};
+    for (int f = 0; f < 5; ++f) {
+      if (__float_as_uint(r[f]) != __float_as_uint(q[f])) {
+        ++bad[f];
+        atomicMin(&first[f], bits);
+      }
+      float t;
+      if (!special[f] && !te::f32_clear(d[f], t, f < 2 ? 0x1p-40 : 0x1p-32)) ++slow[f];
+    }
+  }
+  for (int f = 0; f < 5; ++f) {
+    if (bad[f]) atomicAdd(&out[f], (unsigned long long)bad[f]);
+    if (slow[f]) atomicAdd(&out[5 + f], (unsigned long long)slow[f]);
+  }
+}
+
 // ------------------------------------------------------------------ helpers
 static int fail(te_ctx* c, int code, const std::string& msg) {
   if (c) c->err = msg;
@@ -763,6 +798,33 @@ int te_op_get(const te_ctx* c, int32_t i, const char** name, uint32_t* args64) {
   if (!c || i < 0 || i >= (int32_t)c->ops.size()) return TE_ERR_ARG;
   if (name) *name = i < (int32_t)c->op_names.size() ? c->op_names[i].c_str() : te::kernel_name(c->ops[i].kid);
   if (args64) std::memcpy(args64, c->ops[i].A, 64 * 4);
+  return TE_OK;
+}
+
+int te_selftest_math(uint64_t mismatch[5], uint32_t first[5], uint64_t slow[5]) {
+  if (!mismatch || !first || !slow) return TE_ERR_ARG;
+  unsigned long long* out = nullptr;
+  uint32_t* fb = nullptr;
+  const uint32_t ones[5] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu};
+  unsigned long long h[10];
+  hipError_t e = hipMalloc((void**)&out, sizeof(h));
+  if (e == hipSuccess) e = hipMalloc((void**)&fb, sizeof(ones));
+  if (e == hipSuccess) e = hipMemset(out, 0, sizeof(h));
+  if (e == hipSuccess) e = hipMemcpy(fb, ones, sizeof(ones), hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_selftest_math, dim3(1 << 16), dim3(256), 0, 0, out, fb);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e == hipSuccess) e = hipMemcpy(h, out, sizeof(h), hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(first, fb, sizeof(ones), hipMemcpyDeviceToHost);
+  if (out) (void)hipFree(out);
+  if (fb) (void)hipFree(fb);
+  if (e != hipSuccess) return TE_ERR_HIP;
+  for (int f = 0; f < 5; ++f) {
+    mismatch[f] = h[f];
+    slow[f] = h[5 + f];
+  }
   return TE_OK;
 }
 

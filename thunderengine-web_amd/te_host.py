@@ -113,6 +113,7 @@ def lib():
         "te_encode": ([P(Config), P(ctx)], ctypes.c_int),
         "te_create": ([P(Config), P(ctx)], ctypes.c_int),
         "te_set_time_step": ([ctx, ctypes.c_float, ctypes.c_int32], ctypes.c_int),
+        "te_selftest_math": ([P(ctypes.c_uint64), P(ctypes.c_uint32), P(ctypes.c_uint64)], ctypes.c_int),
         "te_adaptive_start": ([ctx], ctypes.c_int),
         "te_adaptive_step": ([ctx, ctypes.c_int32], ctypes.c_int),
         "te_adaptive_clock": ([ctx, P(ClockState)], ctypes.c_int),
@@ -163,6 +164,18 @@ def kernel_names():
     """Names of the kernels libte.so implements."""
     L = lib()
     return [L.te_kernel_name(i).decode() for i in range(L.te_kernel_count())]
+
+
+def selftest_math():
+    """te_selftest_math: per function (log, exp, pow(cp/rd), pow(rd/cv),
+    pow(1/3)) the binary32 inputs where the kernels' binary64 shortcut
+    differs from the double-float routine, the first one, and the inputs
+    where the shortcut defers to the routine."""
+    mis, first, slow = (ctypes.c_uint64 * 5)(), (ctypes.c_uint32 * 5)(), (ctypes.c_uint64 * 5)()
+    if lib().te_selftest_math(mis, first, slow) != TE_OK:
+        raise TeError("te_selftest_math failed")
+    names = ("log", "exp", "pow_cpdrd", "pow_rddcv", "pow_third")
+    return {n: {"mismatch": int(mis[q]), "first": int(first[q]), "deferred": int(slow[q])} for q, n in enumerate(names)}
 
 
 def config(**kw):
