@@ -1145,6 +1145,190 @@ __global__ __launch_bounds__(256) void k_turb_scalar_vert_implicit_lds(uint32_t*
   scalar_vert_implicit<true>(M, a);
 }
 
+// turb_scalar_vert_implicit as a launch of 64 columns of a row x AC_WAVES
+// waves per workgroup (the fused program's form): each wave takes a
+// contiguous range of LMAX levels, forms its rows' coefficients and
+// right-hand sides and the explicit parts of the tendency in registers, the
+// Thomas recurrence passes from wave to wave through LDS (forward upwards,
+// back substitution downwards; wave w waits for fwd_done == w and for
+// back_next == w, set by the wave below / above once its share is done, with
+// the waits bounded all the same), and each wave updates its levels' tendency
+// as soon as its back substitution is done.  Values are the sequential
+// sweep's, level by level: the first row's missing sub-diagonal and the last
+// row's missing super-diagonal are zeros, which give the reference's special
+// rows' bits.  cA / cB are not written (dead scratch of the fused program).
+template <int LMAX>
+__device__ __forceinline__ void svi_waves(uint32_t* M, const KArg& a) {
+  __shared__ float xb[2][64];   // (pa, pb) of the wave below's last level
+  __shared__ float xw[64];      // new value at the wave above's first level
+  __shared__ int fwd_done, back_next;
+  const Blk blk = te_block(a);
+  const int lane = (int)threadIdx.x;
+  const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.y);
+  const Dims g = dims(a.q[0], 0);
+  const int i = blk.x * 64 + lane, j = blk.y, nz = g.nz;
+  const bool active = i < g.nx && j < g.ny && i < (int)a.n[0] && j < (int)a.n[1];
+  const int Lw = (nz + AC_WAVES - 1) / AC_WAVES, k0 = wave * Lw, k1 = min(nz, k0 + Lw), Lq = max(0, k1 - k0);
+  if (threadIdx.x == 0 && threadIdx.y == 0) {
+    fwd_done = 0;
+    back_next = AC_WAVES - 1;
+  }
+  __syncthreads();
+  auto F = [&](int q) { return reinterpret_cast<float*>(M) + a.p[q]; };
+  const float *s = F(1), *rho = F(2), *rr = F(3), *rf = F(4), *khv = F(5), *mh = F(6), *mf = F(7), *fx = F(8),
+              *fy = F(9), *sflux = F(14), *hm = F(30);
+  float* sten = F(13);
+  const float dt = ldf(a.q[10], 0);
+  const int useFlux = ldi(a.q[15], 0), explicitZ = ldi(a.q[16], 0), bcturbs = ldi(a.q[18], 0);
+  const uint32_t* ow = a.q[17];
+  const float wall0 = ldf(ow, 0), wall1 = ldf(ow, 1), wall2 = ldf(ow, 2), wall3 = ldf(ow, 3);
+  const int64_t sj = g.sJ, sk = g.sk(), base = g.idx(active ? i : 0, active ? j : 0, 0);
+  const float uh = hm_uh(hm, g, active ? i : 0), vh = hm_vh(hm, g, active ? j : 0);
+  auto hdiv = [&](int64_t c) {
+    return ((-(fx[c + 1] - fx[c])) * g.rdx) * uh + ((-(fy[c + sj] - fy[c])) * g.rdy) * vh;
+  };
+  auto L2 = [&](int k) {
+    const int64_t cc = base + k * sk;
+    return ((((-khv[cc]) * (s[cc] - s[cc - sk])) * g.rdz) * mf[k]) * rf[cc];
+  };
+  auto wait_for = [&](int* flag, int want) {
+    for (int n = 0; n < (1 << 22); ++n) {
+      const int v = __builtin_amdgcn_readfirstlane(__hip_atomic_load(flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP));
+      if (v == want) break;
+      __builtin_amdgcn_s_sleep(1);
+    }
+  };
+  auto pass = [&](int* flag, int v) { __hip_atomic_store(flag, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP); };
+  if (explicitZ != 0) {  // explicit vertical diffusion: every level on its own
+    if (active) {
+      auto L1 = [&](int kf) -> float {
+        const int64_t c = base + kf * sk;
+        if (kf == 0) {
+          if (wall3 == 0.0f) return useFlux != 0 ? sflux[base] * rf[c] : (bcturbs == 2 ? L2(1) : 0.0f);
+          if (wall3 == 2.0f) return (wall0 * wall1) * rf[c];
+          return (((((-wall0) * 2.0f) * (s[c] - wall1)) * g.rdz) * mf[0]) * rf[c];
+        }
+        if (kf == nz) {
+          if (wall3 == 0.0f) return bcturbs == 2 ? L2(nz - 1) : 0.0f;
+          const int64_t cb = base + (nz - 1) * sk;
+          if (wall3 == 2.0f) return (wall0 * wall2) * rf[c];
+          return (((((-wall0) * 2.0f) * (wall2 - s[cb])) * g.rdz) * mf[nz]) * rf[c];
+        }
+        return L2(kf);
+      };
+      for (int k = k0; k < k1; ++k) {
+        const int64_t c = base + k * sk;
+        const float tz = ((-(L1(k + 1) - L1(k))) * g.rdz) * mh[k];
+        sten[c] = fma_(hdiv(c) + tz, rr[c], sten[c]);
+      }
+    }
+    return;
+  }
+  const float tema = ((-dt) * g.rdz) * g.rdz, rdt = 1.0f / dt, temc = dt * g.rdz;
+  // rows of the wave's levels: sub-diagonal CA, super-diagonal CC, right-hand
+  // side D, and s, rr for the tendency
+  float CA[LMAX], CC[LMAX], D[LMAX], S[LMAX], RR[LMAX];
+  if (active) {
+    auto X = [&](int k) { const int64_t c = base + k * sk; return (khv[c] * mf[k]) * rf[c]; };
+    float Xk = X(k0 == 0 ? 1 : k0);  // X(k) (k0 = 0: not used)
+#pragma unroll
+    for (int t = 0; t < LMAX; ++t) {
+      if (t < Lq) {
+        const int k = k0 + t;
+        const int64_t c = base + k * sk;
+        const float r = rr[c], m = mh[k];
+        const float Xn = k + 1 < nz ? X(k + 1) : 0.0f;
+        CA[t] = k == 0 ? 0.0f : tema * ((Xk * m) * r);
+        CC[t] = k == nz - 1 ? 0.0f : tema * (((k == 0 ? Xk : Xn) * m) * r);
+        S[t] = s[c];
+        RR[t] = r;
+        float d = S[t];
+        if (k == 0 && (useFlux != 0 || bcturbs == 2)) {
+          float dumz;
+          if (useFlux != 0) dumz = sflux[base] * rf[base];
+          else {
+            const int64_t c1 = base + sk;
+            dumz = ((((-khv[c1]) * (s[c1] - s[base])) * g.rdz) * mf[1]) * rf[c1];
+          }
+          d = fma_((temc * dumz) * mh[0], rr[base], s[base]);
+        }
+        if (k == nz - 1 && bcturbs == 2) {
+          const float dumz = ((((-khv[c]) * (s[c] - s[c - sk])) * g.rdz) * mf[k]) * rf[c];
+          d = fma_(-((temc * dumz) * mh[k]), rr[c], s[c]);
+        }
+        D[t] = d;
+        Xk = k == 0 ? Xk : Xn;
+      }
+    }
+  }
+  // forward elimination: (pa, pb) per level over CA (sub-diagonal) then kept
+  // in CA, CC (the back substitution's coefficients)
+  wait_for(&fwd_done, wave);
+  if (Lq > 0) {
+    float pa = 0.0f, pb = 0.0f;
+    if (wave > 0) {
+      pa = xb[0][lane];
+      pb = xb[1][lane];
+    }
+#pragma unroll
+    for (int t = 0; t < LMAX; ++t) {
+      if (t < Lq) {
+        const float cfa = CA[t], cfc = CC[t];
+        const float cfb = (1.0f - cfa) - cfc;
+        const float tt = 1.0f / fma_(cfa, pa, cfb);
+        pa = (-cfc) * tt;
+        pb = fma_(-cfa, pb, D[t]) * tt;
+        CA[t] = pa;
+        CC[t] = pb;
+      }
+    }
+    xb[0][lane] = pa;
+    xb[1][lane] = pb;
+  }
+  pass(&fwd_done, wave + 1);
+  // back substitution downwards; the new value of each level into D
+  const bool top = k1 == nz && Lq > 0;
+  wait_for(&back_next, wave);
+  if (Lq > 0) {
+    float nb = top ? 0.0f : xw[lane];
+#pragma unroll
+    for (int t = LMAX - 1; t >= 0; --t) {
+      if (t < Lq) {
+        nb = k0 + t == nz - 1 ? CC[t] : fma_(CA[t], nb, CC[t]);
+        D[t] = nb;
+      }
+    }
+    xw[lane] = nb;
+  }
+  pass(&back_next, wave - 1);
+  // the tendency of the wave's levels
+  if (active) {
+#pragma unroll
+    for (int t = 0; t < LMAX; ++t) {
+      if (t < Lq) {
+        const int64_t c = base + (k0 + t) * sk;
+        sten[c] = fma_(hdiv(c) + (rho[c] * (D[t] - S[t])) * rdt, RR[t], sten[c]);
+      }
+    }
+  }
+}
+#define SVI_INST(L) \
+  __global__ __launch_bounds__(64 * AC_WAVES) void k_svi_waves##L(uint32_t* M, KArg a) { svi_waves<L>(M, a); }
+SVI_INST(1) SVI_INST(2) SVI_INST(3) SVI_INST(4) SVI_INST(5) SVI_INST(6) SVI_INST(7) SVI_INST(8) SVI_INST(9)
+SVI_INST(10) SVI_INST(11) SVI_INST(12) SVI_INST(13) SVI_INST(14) SVI_INST(15) SVI_INST(16)
+#undef SVI_INST
+const void* svi_waves_kernel(int levels_per_wave) {
+  static const void* const fns[] = {nullptr, (const void*)&k_svi_waves1, (const void*)&k_svi_waves2,
+                                    (const void*)&k_svi_waves3, (const void*)&k_svi_waves4,
+                                    (const void*)&k_svi_waves5, (const void*)&k_svi_waves6,
+                                    (const void*)&k_svi_waves7, (const void*)&k_svi_waves8,
+                                    (const void*)&k_svi_waves9, (const void*)&k_svi_waves10,
+                                    (const void*)&k_svi_waves11, (const void*)&k_svi_waves12,
+                                    (const void*)&k_svi_waves13, (const void*)&k_svi_waves14,
+                                    (const void*)&k_svi_waves15, (const void*)&k_svi_waves16};
+  return levels_per_wave >= 1 && levels_per_wave <= 16 ? fns[levels_per_wave] : nullptr;
+}
+
 // tke_flux_h.wgsl:125-146
 __global__ __launch_bounds__(256) void k_tke_flux_h(uint32_t* M, KArg a) {
   TE_TID3

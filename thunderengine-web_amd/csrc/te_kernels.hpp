@@ -42,4 +42,7 @@ TE_WIDE_LIST(TE_DECL_WIDE)
 #undef TE_DECL_WIDE
 // k_ac_column instantiated for 1..16 levels per wave (nullptr outside)
 const void* ac_column_kernel(int levels_per_wave);
+// turb_scalar_vert_implicit_lds in 64 x AC_WAVES workgroups (k_turb.hip),
+// instantiated for 1..16 levels per wave (nullptr outside)
+const void* svi_waves_kernel(int levels_per_wave);
 }  // namespace te

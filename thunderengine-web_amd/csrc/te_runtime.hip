@@ -131,6 +131,16 @@ static hipError_t launch(te_ctx* c, const Op& op) {
     void* args[] = {(void*)&c->dM, (void*)&a};
     return hipLaunchKernel(fn, g, b, args, dyn, c->stream);
   }
+  static const int kSviLds = kernel_id("turb_scalar_vert_implicit_lds");
+  if (op.kid == kSviLds) {  // 64 columns of a row per workgroup, AC_WAVES waves over the levels
+    const uint32_t lw = (op.nz + AC_WAVES - 1) / AC_WAVES;
+    if (op.nz < 4 || lw > 16) return hipErrorInvalidConfiguration;
+    b = dim3(64, AC_WAVES, 1);
+    const dim3 g((a.n[0] + 63) / 64, a.n[1], 1);
+    if (!launch_shape(a, b, g)) return hipErrorInvalidConfiguration;
+    void* args[] = {(void*)&c->dM, (void*)&a};
+    return hipLaunchKernel(svi_waves_kernel((int)lw), g, b, args, 0, c->stream);
+  }
   size_t lds = 0;
   if (op.flat) {  // runs of 4 words per thread over the op's n0*n1*n2 words (te_flat)
     const uint64_t words = (uint64_t)a.n[0] * a.n[1] * a.n[2], runs = (words + 3) / 4;
@@ -145,9 +155,8 @@ static hipError_t launch(te_ctx* c, const Op& op) {
     b = dim3(column_lanes(), 1, 1);
     if (uses_column_lds(op.kid)) lds = (size_t)2 * op.nz * b.x * sizeof(float);
     // turbulence sweeps: 4 waves per 64 columns, one row per block (the
-    // kernels' 4-wave forms, 3 nz floats of LDS per column)
-    // (turb_scalar_vert_implicit measured slower in that form and keeps one
-    // wave per 64 columns)
+    // kernels' 4-wave forms, 3 nz floats of LDS per column;
+    // turb_scalar_vert_implicit_lds runs the AC_WAVES-wave form above)
     static const int kU = kernel_id("turb_u_tendency_lds"), kV = kernel_id("turb_v_tendency_lds"),
                      kW = kernel_id("turb_w_tendency_lds"), kT = kernel_id("tke_transport_lds");
     const size_t mwLds = (size_t)3 * op.nz * 64 * sizeof(float);
@@ -389,8 +398,10 @@ static int grow_guards(te_ctx* c) {
     // (te_div is exact for numerators below 2^24, divisors up to 2^16)
     op.plane = !op.flat && !ownMapping.count(kn) && op.A[0] % 64 != 0 && op.A[1] > 1 && op.A[0] <= (1u << 16) &&
                (uint64_t)op.A[0] * op.A[1] + 256 <= (1u << 24);
-    if (te::uses_column_lds(op.kid) && (size_t)2 * nz * te::column_lanes() * sizeof(float) > 65536)
-      return fail(c, TE_ERR_UNSUPPORTED, std::string(te::kernel_name(op.kid)) + ": nz too large for LDS coefficients");
+    static const int kSviLds = te::kernel_id("turb_scalar_vert_implicit_lds");
+    if (op.kid == kSviLds ? nz > AC_WAVES * 16
+                          : te::uses_column_lds(op.kid) && (size_t)2 * nz * te::column_lanes() * sizeof(float) > 65536)
+      return fail(c, TE_ERR_UNSUPPORTED, std::string(te::kernel_name(op.kid)) + ": nz too large for the column sweep");
     need = std::max<uint64_t>(need, (uint64_t)sJ * sK * (uint64_t)(nz + 3));
     if (std::strcmp(te::kernel_name(op.kid), "turb_sfcstress") == 0) {
       int32_t mode = 0;
