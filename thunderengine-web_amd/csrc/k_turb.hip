@@ -558,172 +558,243 @@ __device__ void uv_tendency(uint32_t* M, const KArg& a, int i, int j) {
     }
   }
 }
-// Serial phase of the 4-wave column sweeps (wave 0): the tridiagonal
-// recurrence over levels kb..ke from the coefficients in LDS (cfa, cfc, rhs at
-// LA / LC / LR + 64 k), leaving the solution in LR.  With cfa = 0 at kb and
-// cfc = 0 at ke it is every sweep's first / last level form as well.
-TE_DEV void mw_recurrence(float* LA, const float* LC, float* LR, int kb, int ke) {
-  float pa = 0.0f, pb = 0.0f;
-  for (int k = kb; k <= ke; ++k) {
-    const float cfa = LA[k * 64], cfc = LC[k * 64];
-    const float cfb = (1.0f - cfa) - cfc;
-    const float tt = 1.0f / fma_(cfa, pa, cfb);
-    pa = (-cfc) * tt;
-    pb = fma_(-cfa, pb, LR[k * 64]) * tt;
-    LA[k * 64] = pa;
-    LR[k * 64] = pb;
-  }
-  for (int k = ke - 1; k >= kb; --k) {
-    pb = fma_(LA[k * 64], pb, LR[k * 64]);
-    LR[k * 64] = pb;
-  }
-}
-
-// The same sweeps with 4 waves per 64 columns (the _lds launches when the
-// runtime gives them 64 x 4 blocks): every level's dwd and tridiagonal
-// coefficients are formed by all waves at once, the recurrence -- the only
-// serial part -- runs in wave 0 out of LDS, and all waves then form the
-// tendencies.  Level by level these are uv_tendency's operations: its first
-// and last levels' special forms are the general recurrence with cfa = 0 at
-// the bottom and cfc = 0 at the top (fma(0, 0, b) = b, fma(-0, 0, d) = d,
-// (1 - 0) - c = 1 - c, x - 0 = x).  LDS: 3 nz floats per column (cfa, cfc,
-// rhs; then pa, -, pb; then the solution in the rhs slots).
-template <int DIR>
-__device__ void uv_tendency_mw(uint32_t* M, const KArg& a, int i, int j) {
-  Dims g = dims(a.q[0], 0);
-  const int wv = (int)threadIdx.y, lane = (int)threadIdx.x;
-  // lanes past the row's columns touch no memory; they only meet the barriers
-  const bool live = i < (int)a.n[0] && (DIR == 0 ? (i <= g.nx && j < g.ny) : (i < g.nx && j <= g.ny));
-  const float *q = fq(a.q[1]), *w = fq(a.q[2]), *rho = fq(a.q[3]), *rf = fq(a.q[4]), *kmv = fq(a.q[5]),
-              *tA = fq(a.q[6]), *tB = fq(a.q[7]), *tC = fq(a.q[8]), *mh = fq(a.q[9]), *mf = fq(a.q[10]),
-              *hm = fq(a.q[30]);
-  const float dt = ldf(a.q[11], 0);
-  float *dwd = fq(a.q[14]), *ten = fq(a.q[15]);
-  const int openx = ldi(a.q[16], 0), openy = ldi(a.q[16], 1), explicitZ = ldi(a.q[17], 0);
-  const int nz = g.nz;
-  const int64_t sj = g.sJ, sk = g.sk(), st = DIR == 0 ? 1 : sj, base = g.idx(i, j, 0);
-  const bool zeroX = DIR == 0 ? (openx != 0 && (i == 0 || i == g.nx)) : (openx != 0 && (i == 0 || i == g.nx - 1));
-  const bool zeroY = DIR == 0 ? (openy != 0 && (j == 0 || j == g.ny - 1)) : (openy != 0 && (j == 0 || j == g.ny));
-  const float tema = ((-0.0625f * dt) * g.rdz) * g.rdz;
-  const float temc = (0.5f * dt) * g.rdz;
-  const float rdt = 0.5f / dt;
-  const float rdh = DIR == 0 ? g.rdx : g.rdy;
-  float* LA = te_lds + lane;                  // cfa, then pa
-  float* LC = te_lds + (size_t)nz * 64 + lane;      // cfc
-  float* LR = te_lds + (size_t)2 * nz * 64 + lane;  // rhs, then pb, then the solution
-  auto Y = [&](int lev) {
-    const int64_t cl = base + lev * sk;
-    return ((kmv[cl - st] + kmv[cl]) * (mf[lev] + mf[lev])) * (rf[cl - st] + rf[cl]);
-  };
-  auto turbxy = [&](int64_t c, float& tx, float& ty) {
-    if (DIR == 0) {
-      tx = zeroX ? 0.0f : ((tA[c] - tA[c - 1]) * g.rdx) * hm_uf(hm, g, i);
-      ty = zeroY ? 0.0f : ((tB[c + sj] - tB[c]) * g.rdy) * hm_vh(hm, g, j);
-    } else {
-      tx = zeroX ? 0.0f : ((tA[c + 1] - tA[c]) * g.rdx) * hm_uh(hm, g, i);
-      ty = zeroY ? 0.0f : ((tB[c] - tB[c - sj]) * g.rdy) * hm_vf(hm, g, j);
-    }
-  };
-  if (explicitZ != 0) {  // uniform: no barrier is reached
-    if (live)
-      for (int k = wv; k < nz; k += 4) {
-        const int64_t c = base + k * sk;
-        float tx, ty;
-        turbxy(c, tx, ty);
-        float tz = (((tC[c + sk] - tC[c]) * g.rdz) * 0.5f) * (mh[k] + mh[k]);
-        ten[c] = ten[c] + ((tx + ty) + tz) / (0.5f * (rho[c - st] + rho[c]));
-      }
-    return;
-  }
-  // levels k = wv + 4 t; MW_CH of them per pass, all loads of a pass issued
-  // before its arithmetic (level nz + 1 and beyond are never touched)
-  constexpr int MW_CH = 4;
-  if (live) {
-    const float hmf = DIR == 0 ? hm_uf(hm, g, i) : hm_vf(hm, g, j);
-    const float tCb = tC[base], tCt = tC[base + nz * sk];
-    for (int k0 = wv; k0 <= nz; k0 += 4 * MW_CH) {
-      float w0[MW_CH], w1[MW_CH], ka[MW_CH], kb[MW_CH], ra[MW_CH], rb[MW_CH], mfk[MW_CH], ro0[MW_CH], ro1[MW_CH],
-          mhk[MW_CH], qk[MW_CH], kna[MW_CH], knb[MW_CH], rna[MW_CH], rnb[MW_CH], mfn[MW_CH];
-#pragma unroll
-      for (int t = 0; t < MW_CH; ++t) {
-        const int k = k0 + 4 * t;
-        if (k <= nz) {
-          const int64_t c = base + k * sk;
-          w0[t] = w[c]; w1[t] = w[c - st]; ka[t] = kmv[c - st]; kb[t] = kmv[c]; ra[t] = rf[c - st]; rb[t] = rf[c];
-          mfk[t] = mf[k];
-          if (k < nz) {
-            ro0[t] = rho[c - st]; ro1[t] = rho[c]; mhk[t] = mh[k]; qk[t] = q[c];
-            if (k < nz - 1) {
-              kna[t] = kmv[c + sk - st]; knb[t] = kmv[c + sk]; rna[t] = rf[c + sk - st]; rnb[t] = rf[c + sk];
-              mfn[t] = mf[k + 1];
-            }
-          }
-        }
-      }
-#pragma unroll
-      for (int t = 0; t < MW_CH; ++t) {
-        const int k = k0 + 4 * t;
-        if (k <= nz) {
-          const int64_t c = base + k * sk;
-          dwd[c] = (((((w0[t] - w1[t]) * rdh) * hmf) * 0.25f) * (ka[t] + kb[t])) * (ra[t] + rb[t]);
-          if (k < nz) {
-            const float rinvk = 1.0f / (0.5f * (ro0[t] + ro1[t]));
-            const float tem = (mhk[t] + mhk[t]) * rinvk;
-            const float Yk = ((ka[t] + kb[t]) * (mfk[t] + mfk[t])) * (ra[t] + rb[t]);
-            const float cfa = k == 0 ? 0.0f : tema * (Yk * tem);
-            float cfc = 0.0f;
-            if (k < nz - 1) cfc = tema * ((((kna[t] + knb[t]) * (mfn[t] + mfn[t])) * (rna[t] + rnb[t])) * tem);
-            float rhs;
-            if (k == 0) rhs = fma_(-((temc * tCb) * (mhk[t] + mhk[t])), rinvk, qk[t]);
-            else if (k == nz - 1) rhs = fma_((temc * tCt) * (mhk[t] + mhk[t]), rinvk, qk[t]);
-            else rhs = qk[t];
-            LA[k * 64] = cfa;
-            LC[k * 64] = cfc;
-            LR[k * 64] = rhs;
-          }
-        }
-      }
-    }
-  }
-  __syncthreads();
-  if (live && wv == 0) mw_recurrence(LA, LC, LR, 0, nz - 1);
-  __syncthreads();
-  if (!live) return;
-  for (int k0 = wv; k0 < nz; k0 += 4 * MW_CH) {
-    float tx[MW_CH], ty[MW_CH], ro0[MW_CH], ro1[MW_CH], qk[MW_CH], d0[MW_CH], d1[MW_CH], mhk[MW_CH], tv[MW_CH];
-#pragma unroll
-    for (int t = 0; t < MW_CH; ++t) {
-      const int k = k0 + 4 * t;
-      if (k < nz) {
-        const int64_t c = base + k * sk;
-        turbxy(c, tx[t], ty[t]);
-        ro0[t] = rho[c - st]; ro1[t] = rho[c]; qk[t] = q[c]; d0[t] = dwd[c]; d1[t] = dwd[c + sk]; mhk[t] = mh[k];
-        tv[t] = ten[c];
-      }
-    }
-#pragma unroll
-    for (int t = 0; t < MW_CH; ++t) {
-      const int k = k0 + 4 * t;
-      if (k < nz) {
-        const int64_t c = base + k * sk;
-        float tz = ((ro0[t] + ro1[t]) * (LR[k * 64] - qk[t])) * rdt;
-        tz = fma_(((d1[t] - d0[t]) * g.rdz) * 0.5f, mhk[t] + mhk[t], tz);
-        ten[c] = fma_((tx[t] + ty[t]) + tz, 1.0f / (0.5f * (ro0[t] + ro1[t])), tv[t]);
-      }
-    }
-  }
-}
-// 64 x 4 blocks (x: columns, y: waves; grid y: rows) when the runtime chose
-// them, else one wave per block as uv_tendency<DIR, true>.
+// one lane per column (columns of fewer than 4 levels; deeper ones run
+// uv_waves)
 template <int DIR>
 __device__ void uv_tendency_lds(uint32_t* M, const KArg& a) {
   const Blk b_ = te_block(a);
-  if (TE_BY == 4) {  // grid (columns / 64, rows)
-    uv_tendency_mw<DIR>(M, a, b_.x * 64 + (int)threadIdx.x, b_.y);
-    return;
-  }
   const int i = b_.x * TE_BX + (int)threadIdx.x, j = b_.y * TE_BY + (int)threadIdx.y;
   if (i < (int)a.n[0] && j < (int)a.n[1]) uv_tendency<DIR, true>(M, a, i, j);
+}
+
+// ---------------------------------------------------------------------------
+// Column sweeps as 64 columns of a row x AC_WAVES waves: each wave holds a
+// contiguous range of levels, its rows' coefficients in registers, and the
+// Thomas recurrence passes between the waves through LDS -- the forward
+// elimination upwards ((pa, pb) of the wave's last level), the back
+// substitution downwards (the new value of the wave's first level).  Wave w
+// waits until fwd_done == w (back_next == w), which the wave below (above)
+// sets once its share is done; the waits poll at most 2^22 times.  Level by
+// level the operations are the sequential sweep's, the first row's missing
+// sub-diagonal and the last row's missing super-diagonal being zeros:
+// fma(0, 0, b) = b, fma(-0, p, d) = d, (1 - 0) - c = 1 - c, x - 0 = x.
+struct WaveRelay {
+  float xb[2][64];  // (pa, pb) of the wave below's last level
+  float xw[64];     // the new value at the wave above's first level
+  int fwd_done, back_next;
+};
+TE_DEV void relay_init(WaveRelay& r) {
+  if (threadIdx.x == 0 && threadIdx.y == 0) {
+    r.fwd_done = 0;
+    r.back_next = AC_WAVES - 1;
+  }
+  __syncthreads();
+}
+TE_DEV void relay_wait(int* flag, int want) {
+  for (int n = 0; n < (1 << 22); ++n) {
+    const int v = __builtin_amdgcn_readfirstlane(__hip_atomic_load(flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP));
+    if (v == want) break;
+    __builtin_amdgcn_s_sleep(1);
+  }
+}
+TE_DEV void relay_pass(int* flag, int v) { __hip_atomic_store(flag, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP); }
+// forward elimination of rows (A, C, R) over the wave's Lq levels: A becomes
+// pa, R becomes pb
+template <int LMAX>
+TE_DEV void relay_forward(WaveRelay& r, int wave, int lane, int Lq, float (&A)[LMAX], const float (&C)[LMAX],
+                          float (&R)[LMAX]) {
+  relay_wait(&r.fwd_done, wave);
+  if (Lq > 0) {
+    float pa = 0.0f, pb = 0.0f;
+    if (wave > 0) {
+      pa = r.xb[0][lane];
+      pb = r.xb[1][lane];
+    }
+#pragma unroll
+    for (int t = 0; t < LMAX; ++t) {
+      if (t < Lq) {
+        const float cfa = A[t], cfc = C[t];
+        const float cfb = (1.0f - cfa) - cfc;
+        const float tt = 1.0f / fma_(cfa, pa, cfb);
+        pa = (-cfc) * tt;
+        pb = fma_(-cfa, pb, R[t]) * tt;
+        A[t] = pa;
+        R[t] = pb;
+      }
+    }
+    r.xb[0][lane] = pa;
+    r.xb[1][lane] = pb;
+  }
+  relay_pass(&r.fwd_done, wave + 1);
+}
+// back substitution: R becomes the solution (top: the wave holds the last row)
+template <int LMAX>
+TE_DEV void relay_back(WaveRelay& r, int wave, int lane, int Lq, bool top, const float (&A)[LMAX], float (&R)[LMAX]) {
+  relay_wait(&r.back_next, wave);
+  if (Lq > 0) {
+    float nb = top ? R[0] : r.xw[lane];
+#pragma unroll
+    for (int t = LMAX - 1; t >= 0; --t) {
+      if (t < Lq) {
+        nb = top && t == Lq - 1 ? R[t] : fma_(A[t], nb, R[t]);
+        R[t] = nb;
+      }
+    }
+    r.xw[lane] = nb;
+  }
+  relay_pass(&r.back_next, wave - 1);
+}
+
+// uv_tendency in that form (turb_u_tendency_lds / turb_v_tendency_lds): each
+// wave forms dwd over its levels and one past (level nz for the top wave),
+// its rows, the solution, and its levels' tendencies; all loads of a phase
+// are issued before its stores.
+template <int DIR, int LMAX>
+__device__ __forceinline__ void uv_waves(uint32_t* M, const KArg& a) {
+  __shared__ WaveRelay rl;
+  const Blk blk = te_block(a);
+  const int lane = (int)threadIdx.x;
+  const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.y);
+  const Dims g = dims(a.q[0], 0);
+  const int i = blk.x * 64 + lane, j = blk.y, nz = g.nz;
+  const int Lw = (nz + AC_WAVES - 1) / AC_WAVES, k0 = wave * Lw, k1 = min(nz, k0 + Lw), Lq = max(0, k1 - k0);
+  // a wave past the top (Lq = 0, when AC_WAVES does not divide nz) touches no level
+  const bool live = i < (int)a.n[0] && j < (int)a.n[1] && (DIR == 0 ? (i <= g.nx && j < g.ny) : (i < g.nx && j <= g.ny)) &&
+                    Lq > 0;
+  relay_init(rl);
+  auto F = [&](int n) { return FieldB(reinterpret_cast<float*>(M) + a.p[n]); };
+  const FieldB q = F(1), w = F(2), rho = F(3), rf = F(4), kmv = F(5), tA = F(6), tB = F(7), tC = F(8), dwd = F(14),
+               ten = F(15);
+  const float *mh = reinterpret_cast<float*>(M) + a.p[9], *mf = reinterpret_cast<float*>(M) + a.p[10],
+              *hm = reinterpret_cast<float*>(M) + a.p[30];
+  const float dt = ldf(a.q[11], 0);
+  const int openx = ldi(a.q[16], 0), openy = ldi(a.q[16], 1), explicitZ = ldi(a.q[17], 0);
+  // Addresses: the level as a wave-uniform displacement of the field's base
+  // (L(k), scalar registers), the column as a 32-bit byte offset (o, with
+  // the staggered neighbour at o - ost and the next / previous row at
+  // o +- osj) shared by every level and field.
+  const int ii = live ? i : 0, jj = live ? j : 0;
+  const uint32_t o = 4u * (uint32_t)g.idx(ii, jj, 0), osj = 4u * (uint32_t)g.sJ, ost = DIR == 0 ? 4u : osj;
+  const uint64_t osk = 4u * (uint64_t)g.sk();
+  auto L = [&](int k) { return (uint64_t)k * osk; };
+  const bool zeroX = DIR == 0 ? (openx != 0 && (i == 0 || i == g.nx)) : (openx != 0 && (i == 0 || i == g.nx - 1));
+  const bool zeroY = DIR == 0 ? (openy != 0 && (j == 0 || j == g.ny - 1)) : (openy != 0 && (j == 0 || j == g.ny));
+  const float hx = DIR == 0 ? hm_uf(hm, g, ii) : hm_uh(hm, g, ii), hy = DIR == 0 ? hm_vh(hm, g, jj) : hm_vf(hm, g, jj);
+  auto txy = [&](uint64_t d) {
+    const FieldB A = tA + d, B = tB + d;
+    float tx, ty;
+    if (DIR == 0) {
+      tx = zeroX ? 0.0f : ((A(o) - A(o - 4u)) * g.rdx) * hx;
+      ty = zeroY ? 0.0f : ((B(o + osj) - B(o)) * g.rdy) * hy;
+    } else {
+      tx = zeroX ? 0.0f : ((A(o + 4u) - A(o)) * g.rdx) * hx;
+      ty = zeroY ? 0.0f : ((B(o) - B(o - osj)) * g.rdy) * hy;
+    }
+    return tx + ty;
+  };
+  if (explicitZ != 0) {
+    if (live)
+      for (int k = k0; k < k1; ++k) {
+        const uint64_t d = L(k);
+        const float tz = ((((tC + L(k + 1))(o) - (tC + d)(o)) * g.rdz) * 0.5f) * (mh[k] + mh[k]);
+        const FieldB rk = rho + d;
+        (ten + d).put(o, (ten + d)(o) + (txy(d) + tz) / (0.5f * (rk(o - ost) + rk(o))));
+      }
+    return;
+  }
+  const float tema = ((-0.0625f * dt) * g.rdz) * g.rdz, temc = (0.5f * dt) * g.rdz, rdt = 0.5f / dt;
+  const float rdh = DIR == 0 ? g.rdx : g.rdy;
+  const bool top = k1 == nz && Lq > 0;
+  float CA[LMAX], CC[LMAX], R[LMAX], DW[LMAX + 1];
+  if (live) {
+    const float hmf = DIR == 0 ? hx : hy;
+    // Level t's dwd and Y = ((kmv + kmv') (mf + mf)) (rf + rf'), then row
+    // t - 1 (Y of its level and the next; level nz: dwd only).  Every level's
+    // loads are unconditional (slots past the wave's range re-read its last
+    // level), so that they go out ahead of the arithmetic, four levels at a
+    // time (sched_barrier) to bound the registers in flight.
+    const float tCb = tC(o), tCt = (tC + L(nz))(o);
+    float Yp = 0.0f;
+#pragma unroll
+    for (int t = 0; t <= LMAX; ++t) {
+      const int k = min(k0 + t, nz);
+      const uint64_t d = L(k);
+      const FieldB km = kmv + d, rk = rf + d, wk = w + d;
+      const float ka = km(o - ost), kb = km(o), ra = rk(o - ost), rb = rk(o);
+      DW[t] = (((((wk(o) - wk(o - ost)) * rdh) * hmf) * 0.25f) * (ka + kb)) * (ra + rb);
+      te_pin(DW[t]);
+      float Yt = ((ka + kb) * (mf[k] + mf[k])) * (ra + rb);
+      te_pin(Yt);
+      if (t > 0) {
+        const int r = t - 1, kr = min(k0 + r, nz - 1);
+        const FieldB ro = rho + L(kr);
+        const float rinvk = 1.0f / (0.5f * (ro(o - ost) + ro(o)));
+        const float mhk = mh[kr], qk = (q + L(kr))(o);
+        const float tem = (mhk + mhk) * rinvk;
+        CA[r] = kr == 0 ? 0.0f : tema * (Yp * tem);
+        CC[r] = kr < nz - 1 ? tema * (Yt * tem) : 0.0f;
+        const float tb = kr == 0 ? -((temc * tCb) * (mhk + mhk)) : (temc * tCt) * (mhk + mhk);
+        R[r] = kr == 0 || kr == nz - 1 ? fma_(tb, rinvk, qk) : qk;
+        te_pin(CA[r]);
+        te_pin(CC[r]);
+        te_pin(R[r]);
+      }
+      Yp = Yt;
+      if ((t & 3) == 3) __builtin_amdgcn_sched_barrier(0);
+    }
+#pragma unroll
+    for (int t = 0; t <= LMAX; ++t)
+      if (t < Lq || (t == Lq && top)) (dwd + L(k0 + t)).put(o, DW[t]);
+  }
+  relay_forward<LMAX>(rl, wave, lane, Lq, CA, CC, R);
+  // the horizontal divergences and old tendencies, loaded while the back
+  // substitution comes down (into CC, free after the elimination); the
+  // vertical difference of dwd in place of DW
+  float T0[LMAX];
+  if (live) {
+#pragma unroll
+    for (int t = 0; t < LMAX; ++t) {
+      const uint64_t d = L(min(k0 + t, nz - 1));
+      T0[t] = txy(d);
+      te_pin(T0[t]);
+      CC[t] = (ten + d)(o);
+      DW[t] = ((DW[t + 1] - DW[t]) * g.rdz) * 0.5f;
+      if ((t & 3) == 3) __builtin_amdgcn_sched_barrier(0);
+    }
+  }
+  relay_back<LMAX>(rl, wave, lane, Lq, top, CA, R);
+  if (live) {
+    // every level's new tendency into R, then the stores (no load waits
+    // behind a store)
+#pragma unroll
+    for (int t = 0; t < LMAX; ++t) {
+      const int k = min(k0 + t, nz - 1);
+      const uint64_t d = L(k);
+      const FieldB rk = rho + d;
+      const float ro = rk(o - ost) + rk(o), mhk = mh[k];
+      float tz = (ro * (R[t] - (q + d)(o))) * rdt;
+      tz = fma_(DW[t], mhk + mhk, tz);
+      R[t] = fma_(T0[t] + tz, 1.0f / (0.5f * ro), CC[t]);
+      te_pin(R[t]);
+      if ((t & 3) == 3) __builtin_amdgcn_sched_barrier(0);
+    }
+#pragma unroll
+    for (int t = 0; t < LMAX; ++t)
+      if (t < Lq) (ten + L(k0 + t)).put(o, R[t]);
+  }
+}
+#define UV_INST(L)                                                                                           \
+  __global__ __launch_bounds__(64 * AC_WAVES, (L) <= 10 ? 6 : 1) void k_u_waves##L(uint32_t* M, KArg a) { uv_waves<0, L>(M, a); } \
+  __global__ __launch_bounds__(64 * AC_WAVES, (L) <= 10 ? 6 : 1) void k_v_waves##L(uint32_t* M, KArg a) { uv_waves<1, L>(M, a); }
+UV_INST(1) UV_INST(2) UV_INST(3) UV_INST(4) UV_INST(5) UV_INST(6) UV_INST(7) UV_INST(8) UV_INST(9) UV_INST(10)
+UV_INST(11) UV_INST(12) UV_INST(13) UV_INST(14) UV_INST(15) UV_INST(16)
+#undef UV_INST
+const void* uv_waves_kernel(int dir, int levels_per_wave) {
+#define E(L) {(const void*)&k_u_waves##L, (const void*)&k_v_waves##L}
+  static const void* const fns[][2] = {{nullptr, nullptr}, E(1), E(2), E(3), E(4), E(5), E(6), E(7), E(8),
+                                       E(9), E(10), E(11), E(12), E(13), E(14), E(15), E(16)};
+#undef E
+  return levels_per_wave >= 1 && levels_per_wave <= 16 && (dir == 0 || dir == 1) ? fns[levels_per_wave][dir] : nullptr;
 }
 
 __global__ __launch_bounds__(256) void k_turb_u_tendency(uint32_t* M, KArg a) {
@@ -866,79 +937,101 @@ struct FaceTendency {
   __device__ void at(int t, int64_t c, float xn, float xo, float rfc) { apply(c, xn, xo, rfc, txv[t], tyv[t], tv[t]); }
 };
 
-// face_implicit with 4 waves per 64 columns (grid: columns / 64, rows):
-// every level's coefficients by all waves into LDS, the recurrence in wave 0
-// (mw_recurrence; cfa = 0 at level 1, cfc = 0 at level nz-1 give the first /
-// last forms), then the tendencies by all waves.  Lanes with live = false
-// touch no memory and only meet the barriers.
-constexpr int MW_CH = 4;
-template <class OUT>
-__device__ void face_implicit_mw(bool live, int wv, int lane, const float* kmh, const float* mh, const float* rho,
-                                 const float* mf, const float* rf, const float* x, int64_t base, int64_t sk, int nz,
-                                 float tema, float cfd1, float cfdT, OUT& out) {
-  float* LA = te_lds + lane;
-  float* LC = te_lds + (size_t)nz * 64 + lane;
-  float* LR = te_lds + (size_t)2 * nz * 64 + lane;
+// face_implicit in the AC_WAVES-wave form (relay_forward / relay_back):
+// levels 1 .. nz-1 split into contiguous ranges, one per wave; each level's
+// row is face_implicit's (cfa = 0 at level 1, cfc = 0 at level nz-1 give its
+// first and last forms), then the tendency of each level from the solution.
+template <int LMAX>
+__device__ __forceinline__ void face_waves(WaveRelay& rl, bool live, int wave, int lane, const float* kmh,
+                                           const float* mh, const float* rho, const float* mf, const float* rf,
+                                           const float* x, int64_t base, int64_t sk, int nz, float tema, float cfd1,
+                                           float cfdT, const FaceTendency& out) {
+  const int Lw = (nz - 1 + AC_WAVES - 1) / AC_WAVES, k0 = 1 + wave * Lw, k1 = min(nz, k0 + Lw),
+            Lq = max(0, k1 - k0);
+  live = live && Lq > 0;
+  const bool top = k1 == nz && Lq > 0;
+  // addresses: the level as a wave-uniform displacement of each field's base,
+  // the column as a 32-bit byte offset (o; o + 4 / o + osj the next column /
+  // row) shared by every level and field
+  const uint32_t o = 4u * (uint32_t)base, osj = 4u * (uint32_t)out.g.sJ;
+  const uint64_t osk = 4u * (uint64_t)sk;
+  auto L = [&](int k) { return (uint64_t)k * osk; };
+  const FieldB KMH(kmh), RHO(rho), RF_(rf), X(x), HX(out.hx), HY(out.hy), TEN(out.ten);
+  const float hxm = out.zeroX ? 0.0f : hm_uh(out.hm, out.g, live ? out.i : 0),
+              hym = out.zeroY ? 0.0f : hm_vh(out.hm, out.g, live ? out.j : 0);
+  float CA[LMAX], CC[LMAX], R[LMAX], XO[LMAX], RF[LMAX];
   if (live) {
-    for (int k0 = 1 + wv; k0 <= nz - 1; k0 += 4 * MW_CH) {
-      float ka[MW_CH], kb[MW_CH], kc[MW_CH], mhp[MW_CH], mhk[MW_CH], rop[MW_CH], rok[MW_CH], mfk[MW_CH], rfk[MW_CH],
-          xv[MW_CH];
+    // Z(l) = ((kmh_l + kmh_l+1) mh_l) rho_l at l = k0-1 .. k1-1, then the
+    // rows.  Every level's loads are unconditional (slots past the wave's
+    // range re-read its last level), four levels at a time (sched_barrier),
+    // each value materialised where it is formed (te_pin).
+    float Zp = 0.0f;
 #pragma unroll
-      for (int t = 0; t < MW_CH; ++t) {
-        const int k = k0 + 4 * t;
-        if (k <= nz - 1) {
-          const int64_t c = base + k * sk;
-          ka[t] = kmh[c - sk]; kb[t] = kmh[c]; kc[t] = kmh[c + sk]; mhp[t] = mh[k - 1]; mhk[t] = mh[k];
-          rop[t] = rho[c - sk]; rok[t] = rho[c]; mfk[t] = mf[k]; rfk[t] = rf[c]; xv[t] = x[c];
-        }
+    for (int t = 0; t <= LMAX; ++t) {
+      const int l = min(k0 - 1 + t, nz - 1);
+      const uint64_t d = L(l);
+      float Zt = (((KMH + d)(o) + (KMH + d + osk)(o)) * mh[l]) * (RHO + d)(o);
+      te_pin(Zt);
+      if (t > 0) {
+        const int r = t - 1, k = min(k0 + r, nz - 1);
+        const uint64_t dk = L(k);
+        XO[r] = (X + dk)(o);
+        RF[r] = (RF_ + dk)(o);
+        const float mfrf = mf[k] / RF[r];
+        CA[r] = k == 1 ? 0.0f : tema * (Zp * mfrf);
+        CC[r] = k == nz - 1 ? 0.0f : tema * (Zt * mfrf);
+        R[r] = k == 1 ? cfd1 : (k == nz - 1 ? cfdT : XO[r]);
+        te_pin(CA[r]);
+        te_pin(CC[r]);
       }
-#pragma unroll
-      for (int t = 0; t < MW_CH; ++t) {
-        const int k = k0 + 4 * t;
-        if (k <= nz - 1) {
-          const float mfrf = mfk[t] / rfk[t];
-          const float Zp = ((ka[t] + kb[t]) * mhp[t]) * rop[t];  // Z(k-1)
-          const float Zk = ((kb[t] + kc[t]) * mhk[t]) * rok[t];  // Z(k)
-          LA[k * 64] = k == 1 ? 0.0f : tema * (Zp * mfrf);
-          LC[k * 64] = k == nz - 1 ? 0.0f : tema * (Zk * mfrf);
-          LR[k * 64] = k == 1 ? cfd1 : (k == nz - 1 ? cfdT : xv[t]);
-        }
-      }
+      Zp = Zt;
+      if ((t & 3) == 3) __builtin_amdgcn_sched_barrier(0);
     }
   }
-  __syncthreads();
-  if (live && wv == 0) mw_recurrence(LA, LC, LR, 1, nz - 1);
-  __syncthreads();
-  if (!live) return;
-  for (int k0 = 1 + wv; k0 <= nz - 1; k0 += 4 * MW_CH) {
-    float xo[MW_CH], rfk[MW_CH];
+  relay_forward<LMAX>(rl, wave, lane, Lq, CA, CC, R);
+  // the horizontal divergences (tx + ty, into CC: free after the
+  // elimination) and the old tendencies, loaded while the back substitution
+  // comes down (FaceTendency::fluxes)
+  float TV[LMAX];
+  if (live) {
 #pragma unroll
-    for (int t = 0; t < MW_CH; ++t) {
-      const int k = k0 + 4 * t;
-      if (k <= nz - 1) {
-        const int64_t c = base + k * sk;
-        xo[t] = x[c]; rfk[t] = rf[c];
-        out.prefetch(t, c);
-      }
+    for (int t = 0; t < LMAX; ++t) {
+      const uint64_t d = L(min(k0 + t, nz - 1));
+      const FieldB A = HX + d, B = HY + d;
+      const float dx = A(o + 4u) - A(o), dy = B(o + osj) - B(o);
+      const float tx = out.zeroX ? 0.0f : ((out.negate ? -dx : dx) * out.g.rdx) * hxm;
+      const float ty = out.zeroY ? 0.0f : ((out.negate ? -dy : dy) * out.g.rdy) * hym;
+      CC[t] = tx + ty;
+      te_pin(CC[t]);
+      TV[t] = (TEN + d)(o);
+      if ((t & 3) == 3) __builtin_amdgcn_sched_barrier(0);
+    }
+  }
+  relay_back<LMAX>(rl, wave, lane, Lq, top, CA, R);
+  if (live) {
+#pragma unroll
+    for (int t = 0; t < LMAX; ++t) {  // FaceTendency::apply
+      const float tz = (RF[t] * (R[t] - XO[t])) * out.rdt;
+      R[t] = TV[t] + (CC[t] + tz) / RF[t];
     }
 #pragma unroll
-    for (int t = 0; t < MW_CH; ++t) {
-      const int k = k0 + 4 * t;
-      if (k <= nz - 1) out.at(t, base + k * sk, LR[k * 64], xo[t], rfk[t]);
-    }
+    for (int t = 0; t < LMAX; ++t)
+      if (t < Lq) (TEN + L(k0 + t)).put(o, R[t]);
   }
 }
 
 // turb_w_tendency.wgsl:152-287
-template <bool LDS>
+// LMAX > 0: 64 x AC_WAVES workgroups, grid (columns / 64, rows): face_waves;
+// else one lane per column
+template <bool LDS, int LMAX = 0>
 __device__ void w_tendency(uint32_t* M, const KArg& a) {
   const Blk b_ = te_block(a);
-  const bool mw = LDS && TE_BY == 4;  // 64 x 4 blocks, grid (columns / 64, rows): face_implicit_mw
-  const int i = mw ? b_.x * 64 + (int)threadIdx.x : b_.x * TE_BX + (int)threadIdx.x;
-  const int j = mw ? b_.y : b_.y * TE_BY + (int)threadIdx.y;
+  const bool wv = LMAX > 0;
+  const int i = wv ? b_.x * 64 + (int)threadIdx.x : b_.x * TE_BX + (int)threadIdx.x;
+  const int j = wv ? b_.y : b_.y * TE_BY + (int)threadIdx.y;
   Dims g = dims(a.q[0], 0);
   const bool live = i < (int)a.n[0] && j < (int)a.n[1] && i < g.nx && j < g.ny;
-  if (!live && !mw) return;
+  if (!live && !wv) return;
   const float *w = fq(a.q[1]), *rho = fq(a.q[2]), *rf = fq(a.q[3]), *kmh = fq(a.q[4]), *t13 = fq(a.q[5]),
               *t23 = fq(a.q[6]), *t33 = fq(a.q[7]), *mh = fq(a.q[8]), *mf = fq(a.q[9]), *hm = fq(a.q[30]);
   float dt = ldf(a.q[10], 0);
@@ -951,8 +1044,8 @@ __device__ void w_tendency(uint32_t* M, const KArg& a) {
     tx = zeroX ? 0.0f : ((t13[c + 1] - t13[c]) * g.rdx) * hm_uh(hm, g, i);
     ty = zeroY ? 0.0f : ((t23[c + sj] - t23[c]) * g.rdy) * hm_vh(hm, g, j);
   };
-  if (explicitZ != 0) {  // uniform: no barrier is reached; in 4-wave blocks wave 0 does the columns
-    if (!live || (mw && threadIdx.y != 0)) return;
+  if (explicitZ != 0) {  // uniform: no wait is reached; in AC_WAVES-wave blocks wave 0 does the columns
+    if (!live || (wv && threadIdx.y != 0)) return;
     for (int k = 1; k < g.nz; ++k) {
       const int64_t c = base + k * sk;
       float tx, ty;
@@ -970,9 +1063,11 @@ __device__ void w_tendency(uint32_t* M, const KArg& a) {
   const float cfd1 = live ? fma_(-(temc * t33[c1 - sk]), mf[1] / rf[c1], w[c1]) : 0.0f;
   const float cfdT = live ? fma_(temc * t33[ct], mf[nz - 1] / rf[ct], w[ct]) : 0.0f;
   FaceTendency out{t13, t23, hm, wten, g, i, j, zeroX, zeroY, false, rdt, {}, {}, {}};
-  if (mw) {
-    face_implicit_mw(live, (int)threadIdx.y, (int)threadIdx.x, kmh, mh, rho, mf, rf, w, base, sk, nz, tema, cfd1, cfdT,
-                     out);
+  if constexpr (LMAX > 0) {
+    __shared__ WaveRelay rl;
+    relay_init(rl);
+    face_waves<LMAX>(rl, live, __builtin_amdgcn_readfirstlane((int)threadIdx.y), (int)threadIdx.x, kmh, mh, rho, mf,
+                     rf, w, base, sk, nz, tema, cfd1, cfdT, out);
     return;
   }
   const ColStore<LDS> CA{cA, base, sk, TE_LANE, TE_NLANES, 0}, CB{cB, base, sk, TE_LANE, TE_NLANES, nz};
@@ -1159,21 +1254,16 @@ __global__ __launch_bounds__(256) void k_turb_scalar_vert_implicit_lds(uint32_t*
 // rows' bits.  cA / cB are not written (dead scratch of the fused program).
 template <int LMAX>
 __device__ __forceinline__ void svi_waves(uint32_t* M, const KArg& a) {
-  __shared__ float xb[2][64];   // (pa, pb) of the wave below's last level
-  __shared__ float xw[64];      // new value at the wave above's first level
-  __shared__ int fwd_done, back_next;
+  __shared__ WaveRelay rl;
   const Blk blk = te_block(a);
   const int lane = (int)threadIdx.x;
   const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.y);
   const Dims g = dims(a.q[0], 0);
   const int i = blk.x * 64 + lane, j = blk.y, nz = g.nz;
-  const bool active = i < g.nx && j < g.ny && i < (int)a.n[0] && j < (int)a.n[1];
   const int Lw = (nz + AC_WAVES - 1) / AC_WAVES, k0 = wave * Lw, k1 = min(nz, k0 + Lw), Lq = max(0, k1 - k0);
-  if (threadIdx.x == 0 && threadIdx.y == 0) {
-    fwd_done = 0;
-    back_next = AC_WAVES - 1;
-  }
-  __syncthreads();
+  // a wave past the top (Lq = 0, when AC_WAVES does not divide nz) touches no level
+  const bool active = i < g.nx && j < g.ny && i < (int)a.n[0] && j < (int)a.n[1] && Lq > 0;
+  relay_init(rl);
   auto F = [&](int q) { return reinterpret_cast<float*>(M) + a.p[q]; };
   const float *s = F(1), *rho = F(2), *rr = F(3), *rf = F(4), *khv = F(5), *mh = F(6), *mf = F(7), *fx = F(8),
               *fy = F(9), *sflux = F(14), *hm = F(30);
@@ -1191,14 +1281,6 @@ __device__ __forceinline__ void svi_waves(uint32_t* M, const KArg& a) {
     const int64_t cc = base + k * sk;
     return ((((-khv[cc]) * (s[cc] - s[cc - sk])) * g.rdz) * mf[k]) * rf[cc];
   };
-  auto wait_for = [&](int* flag, int want) {
-    for (int n = 0; n < (1 << 22); ++n) {
-      const int v = __builtin_amdgcn_readfirstlane(__hip_atomic_load(flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP));
-      if (v == want) break;
-      __builtin_amdgcn_s_sleep(1);
-    }
-  };
-  auto pass = [&](int* flag, int v) { __hip_atomic_store(flag, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP); };
   if (explicitZ != 0) {  // explicit vertical diffusion: every level on its own
     if (active) {
       auto L1 = [&](int kf) -> float {
@@ -1225,95 +1307,91 @@ __device__ __forceinline__ void svi_waves(uint32_t* M, const KArg& a) {
     return;
   }
   const float tema = ((-dt) * g.rdz) * g.rdz, rdt = 1.0f / dt, temc = dt * g.rdz;
+  // addresses: the level as a wave-uniform displacement of each field's base,
+  // the column as a 32-bit byte offset (o; o + 4 / o + osj the next column /
+  // row) shared by every level and field
+  const uint32_t o = 4u * (uint32_t)base, osj = 4u * (uint32_t)sj;
+  const uint64_t osk = 4u * (uint64_t)sk;
+  auto L = [&](int k) { return (uint64_t)k * osk; };
+  const FieldB S_(s), RHO(rho), RR_(rr), RF(rf), KHV(khv), FX(fx), FY(fy), STEN(sten);
   // rows of the wave's levels: sub-diagonal CA, super-diagonal CC, right-hand
-  // side D, and s, rr for the tendency
-  float CA[LMAX], CC[LMAX], D[LMAX], S[LMAX], RR[LMAX];
+  // side D.  Every level's loads are unconditional (slots past the wave's
+  // range re-read its last level), four levels at a time (sched_barrier),
+  // each value materialised where it is formed (te_pin); only the stores are
+  // guarded.
+  const bool top = k1 == nz && Lq > 0;
+  float CA[LMAX], CC[LMAX], D[LMAX];
   if (active) {
-    auto X = [&](int k) { const int64_t c = base + k * sk; return (khv[c] * mf[k]) * rf[c]; };
+    auto X = [&](int k) { const uint64_t d = L(k); return ((KHV + d)(o) * mf[k]) * (RF + d)(o); };
+    // the first / last rows' right-hand sides where a boundary flux enters
+    const bool flux0 = k0 == 0 && (useFlux != 0 || bcturbs == 2), fluxT = top && bcturbs == 2;
+    float d0 = 0.0f, dT = 0.0f;
+    if (flux0) {
+      float dumz;
+      if (useFlux != 0) dumz = sflux[base] * rf[base];
+      else {
+        const int64_t c1 = base + sk;
+        dumz = ((((-khv[c1]) * (s[c1] - s[base])) * g.rdz) * mf[1]) * rf[c1];
+      }
+      d0 = fma_((temc * dumz) * mh[0], rr[base], s[base]);
+    }
+    if (fluxT) {
+      const int64_t c = base + (nz - 1) * sk;
+      const float dumz = ((((-khv[c]) * (s[c] - s[c - sk])) * g.rdz) * mf[nz - 1]) * rf[c];
+      dT = fma_(-((temc * dumz) * mh[nz - 1]), rr[c], s[c]);
+    }
     float Xk = X(k0 == 0 ? 1 : k0);  // X(k) (k0 = 0: not used)
 #pragma unroll
     for (int t = 0; t < LMAX; ++t) {
-      if (t < Lq) {
-        const int k = k0 + t;
-        const int64_t c = base + k * sk;
-        const float r = rr[c], m = mh[k];
-        const float Xn = k + 1 < nz ? X(k + 1) : 0.0f;
-        CA[t] = k == 0 ? 0.0f : tema * ((Xk * m) * r);
-        CC[t] = k == nz - 1 ? 0.0f : tema * (((k == 0 ? Xk : Xn) * m) * r);
-        S[t] = s[c];
-        RR[t] = r;
-        float d = S[t];
-        if (k == 0 && (useFlux != 0 || bcturbs == 2)) {
-          float dumz;
-          if (useFlux != 0) dumz = sflux[base] * rf[base];
-          else {
-            const int64_t c1 = base + sk;
-            dumz = ((((-khv[c1]) * (s[c1] - s[base])) * g.rdz) * mf[1]) * rf[c1];
-          }
-          d = fma_((temc * dumz) * mh[0], rr[base], s[base]);
-        }
-        if (k == nz - 1 && bcturbs == 2) {
-          const float dumz = ((((-khv[c]) * (s[c] - s[c - sk])) * g.rdz) * mf[k]) * rf[c];
-          d = fma_(-((temc * dumz) * mh[k]), rr[c], s[c]);
-        }
-        D[t] = d;
-        Xk = k == 0 ? Xk : Xn;
-      }
+      const int k = min(k0 + t, nz - 1);
+      const uint64_t d = L(k);
+      const float r = (RR_ + d)(o), m = mh[k];
+      float Xn = X(min(k + 1, nz - 1));
+      Xn = k + 1 < nz ? Xn : 0.0f;
+      te_pin(Xn);
+      CA[t] = k == 0 ? 0.0f : tema * ((Xk * m) * r);
+      CC[t] = k == nz - 1 ? 0.0f : tema * (((k == 0 ? Xk : Xn) * m) * r);
+      const float sv = (S_ + d)(o);
+      D[t] = k == 0 && flux0 ? d0 : (k == nz - 1 && fluxT ? dT : sv);
+      te_pin(CA[t]);
+      te_pin(CC[t]);
+      te_pin(D[t]);
+      Xk = k == 0 ? Xk : Xn;
+      if ((t & 3) == 3) __builtin_amdgcn_sched_barrier(0);
     }
   }
-  // forward elimination: (pa, pb) per level over CA (sub-diagonal) then kept
-  // in CA, CC (the back substitution's coefficients)
-  wait_for(&fwd_done, wave);
-  if (Lq > 0) {
-    float pa = 0.0f, pb = 0.0f;
-    if (wave > 0) {
-      pa = xb[0][lane];
-      pb = xb[1][lane];
-    }
-#pragma unroll
-    for (int t = 0; t < LMAX; ++t) {
-      if (t < Lq) {
-        const float cfa = CA[t], cfc = CC[t];
-        const float cfb = (1.0f - cfa) - cfc;
-        const float tt = 1.0f / fma_(cfa, pa, cfb);
-        pa = (-cfc) * tt;
-        pb = fma_(-cfa, pb, D[t]) * tt;
-        CA[t] = pa;
-        CC[t] = pb;
-      }
-    }
-    xb[0][lane] = pa;
-    xb[1][lane] = pb;
-  }
-  pass(&fwd_done, wave + 1);
-  // back substitution downwards; the new value of each level into D
-  const bool top = k1 == nz && Lq > 0;
-  wait_for(&back_next, wave);
-  if (Lq > 0) {
-    float nb = top ? 0.0f : xw[lane];
-#pragma unroll
-    for (int t = LMAX - 1; t >= 0; --t) {
-      if (t < Lq) {
-        nb = k0 + t == nz - 1 ? CC[t] : fma_(CA[t], nb, CC[t]);
-        D[t] = nb;
-      }
-    }
-    xw[lane] = nb;
-  }
-  pass(&back_next, wave - 1);
-  // the tendency of the wave's levels
+  relay_forward<LMAX>(rl, wave, lane, Lq, CA, CC, D);
+  // the horizontal divergence (into CC, free after the elimination) and the
+  // old tendency, loaded while the back substitution comes down
+  float TV[LMAX];
   if (active) {
 #pragma unroll
     for (int t = 0; t < LMAX; ++t) {
-      if (t < Lq) {
-        const int64_t c = base + (k0 + t) * sk;
-        sten[c] = fma_(hdiv(c) + (rho[c] * (D[t] - S[t])) * rdt, RR[t], sten[c]);
-      }
+      const uint64_t d = L(min(k0 + t, nz - 1));
+      const FieldB A = FX + d, B = FY + d;
+      CC[t] = ((-(A(o + 4u) - A(o))) * g.rdx) * uh + ((-(B(o + osj) - B(o))) * g.rdy) * vh;
+      te_pin(CC[t]);
+      TV[t] = (STEN + d)(o);
+      if ((t & 3) == 3) __builtin_amdgcn_sched_barrier(0);
     }
+  }
+  relay_back<LMAX>(rl, wave, lane, Lq, top, CA, D);
+  if (active) {
+    // every level's new tendency into D, then the stores
+#pragma unroll
+    for (int t = 0; t < LMAX; ++t) {
+      const uint64_t d = L(min(k0 + t, nz - 1));
+      D[t] = fma_(CC[t] + ((RHO + d)(o) * (D[t] - (S_ + d)(o))) * rdt, (RR_ + d)(o), TV[t]);
+      te_pin(D[t]);
+      if ((t & 3) == 3) __builtin_amdgcn_sched_barrier(0);
+    }
+#pragma unroll
+    for (int t = 0; t < LMAX; ++t)
+      if (t < Lq) (STEN + L(k0 + t)).put(o, D[t]);
   }
 }
 #define SVI_INST(L) \
-  __global__ __launch_bounds__(64 * AC_WAVES) void k_svi_waves##L(uint32_t* M, KArg a) { svi_waves<L>(M, a); }
+  __global__ __launch_bounds__(64 * AC_WAVES, (L) <= 10 ? 6 : 1) void k_svi_waves##L(uint32_t* M, KArg a) { svi_waves<L>(M, a); }
 SVI_INST(1) SVI_INST(2) SVI_INST(3) SVI_INST(4) SVI_INST(5) SVI_INST(6) SVI_INST(7) SVI_INST(8) SVI_INST(9)
 SVI_INST(10) SVI_INST(11) SVI_INST(12) SVI_INST(13) SVI_INST(14) SVI_INST(15) SVI_INST(16)
 #undef SVI_INST
@@ -1344,15 +1422,17 @@ __global__ __launch_bounds__(256) void k_tke_flux_h(uint32_t* M, KArg a) {
 }
 
 // tke_transport.wgsl:158-301
-template <bool LDS>
+// LMAX > 0: 64 x AC_WAVES workgroups, grid (columns / 64, rows): face_waves;
+// else one lane per column
+template <bool LDS, int LMAX = 0>
 __device__ void tke_transport(uint32_t* M, const KArg& a) {
   const Blk b_ = te_block(a);
-  const bool mw = LDS && TE_BY == 4;  // 64 x 4 blocks, grid (columns / 64, rows): face_implicit_mw
-  const int i = mw ? b_.x * 64 + (int)threadIdx.x : b_.x * TE_BX + (int)threadIdx.x;
-  const int j = mw ? b_.y : b_.y * TE_BY + (int)threadIdx.y;
+  const bool wv = LMAX > 0;
+  const int i = wv ? b_.x * 64 + (int)threadIdx.x : b_.x * TE_BX + (int)threadIdx.x;
+  const int j = wv ? b_.y : b_.y * TE_BY + (int)threadIdx.y;
   Dims g = dims(a.q[0], 0);
   const bool live = i < (int)a.n[0] && j < (int)a.n[1] && i < g.nx && j < g.ny;
-  if (!live && !mw) return;
+  if (!live && !wv) return;
   const float *t = fq(a.q[1]), *rho = fq(a.q[2]), *rf = fq(a.q[3]), *kmh = fq(a.q[4]), *kmv = fq(a.q[5]),
               *mh = fq(a.q[6]), *mf = fq(a.q[7]), *fx = fq(a.q[11]), *fy = fq(a.q[12]), *hm = fq(a.q[30]);
   float dt = ldf(a.q[8], 0);
@@ -1366,8 +1446,8 @@ __device__ void tke_transport(uint32_t* M, const KArg& a) {
     ty = zeroY ? 0.0f : ((-(fy[c + sj] - fy[c])) * g.rdy) * hm_vh(hm, g, j);
   };
   auto mfrf = [&](int k) { return mf[k] / rf[base + k * sk]; };
-  if (explicitZ != 0) {  // uniform: no barrier is reached; in 4-wave blocks wave 0 does the columns
-    if (!live || (mw && threadIdx.y != 0)) return;
+  if (explicitZ != 0) {  // uniform: no wait is reached; in AC_WAVES-wave blocks wave 0 does the columns
+    if (!live || (wv && threadIdx.y != 0)) return;
     auto L1 = [&](int k) {
       const int64_t c = base + k * sk;
       return ((((-(kmv[c] + kmv[c + sk])) * (t[c + sk] - t[c])) * g.rdz) * mh[k]) * rho[c];
@@ -1397,9 +1477,11 @@ __device__ void tke_transport(uint32_t* M, const KArg& a) {
     cfdT = fma_(-(temc * temT), mf[nz - 1] / rf[ct], t[ct]);
   }
   FaceTendency out{fx, fy, hm, tketen, g, i, j, zeroX, zeroY, true, rdt, {}, {}, {}};
-  if (mw) {
-    face_implicit_mw(live, (int)threadIdx.y, (int)threadIdx.x, kmh, mh, rho, mf, rf, t, base, sk, nz, tema, cfd1, cfdT,
-                     out);
+  if constexpr (LMAX > 0) {
+    __shared__ WaveRelay rl;
+    relay_init(rl);
+    face_waves<LMAX>(rl, live, __builtin_amdgcn_readfirstlane((int)threadIdx.y), (int)threadIdx.x, kmh, mh, rho, mf,
+                     rf, t, base, sk, nz, tema, cfd1, cfdT, out);
     return;
   }
   const ColStore<LDS> CA{cA, base, sk, TE_LANE, TE_NLANES, 0}, CB{cB, base, sk, TE_LANE, TE_NLANES, nz};
@@ -1407,5 +1489,19 @@ __device__ void tke_transport(uint32_t* M, const KArg& a) {
 }
 __global__ __launch_bounds__(256) void k_tke_transport(uint32_t* M, KArg a) { tke_transport<false>(M, a); }
 __global__ __launch_bounds__(256) void k_tke_transport_lds(uint32_t* M, KArg a) { tke_transport<true>(M, a); }
+#define FACE_INST(L)                                                                                      \
+  __global__ __launch_bounds__(64 * AC_WAVES, (L) <= 10 ? 6 : 1) void k_w_waves##L(uint32_t* M, KArg a) { w_tendency<true, L>(M, a); } \
+  __global__ __launch_bounds__(64 * AC_WAVES, (L) <= 10 ? 6 : 1) void k_tke_waves##L(uint32_t* M, KArg a) { tke_transport<true, L>(M, a); }
+FACE_INST(1) FACE_INST(2) FACE_INST(3) FACE_INST(4) FACE_INST(5) FACE_INST(6) FACE_INST(7) FACE_INST(8)
+FACE_INST(9) FACE_INST(10) FACE_INST(11) FACE_INST(12) FACE_INST(13) FACE_INST(14) FACE_INST(15) FACE_INST(16)
+#undef FACE_INST
+const void* face_waves_kernel(int which, int levels_per_wave) {
+#define E(L) {(const void*)&k_w_waves##L, (const void*)&k_tke_waves##L}
+  static const void* const fns[][2] = {{nullptr, nullptr}, E(1), E(2), E(3), E(4), E(5), E(6), E(7), E(8),
+                                       E(9), E(10), E(11), E(12), E(13), E(14), E(15), E(16)};
+#undef E
+  return levels_per_wave >= 1 && levels_per_wave <= 16 && (which == 0 || which == 1) ? fns[levels_per_wave][which]
+                                                                                      : nullptr;
+}
 
 }  // namespace te

@@ -39,6 +39,23 @@ TE_DEV int32_t ldi(const uint32_t* M, uint32_t o) { return (int32_t)M[o]; }
 TE_DEV float* fp(uint32_t* M, uint32_t o) { return reinterpret_cast<float*>(M + o); }
 TE_DEV float* fq(uint32_t* p) { return reinterpret_cast<float*>(p); }
 
+// A field addressed by 32-bit byte offsets from a wave-uniform base: loads and
+// stores take the scalar-base form (one VGPR per offset, shared by every field
+// of the same layout, instead of a 64-bit address per field).  The field must
+// span less than 2^32 bytes from its base.
+struct FieldB {
+  char* b;
+  TE_DEV explicit FieldB(const float* p) : b(reinterpret_cast<char*>(const_cast<float*>(p))) {}
+  TE_DEV float operator()(uint32_t o) const { return *reinterpret_cast<const float*>(b + o); }
+  TE_DEV void put(uint32_t o, float v) const { *reinterpret_cast<float*>(b + o) = v; }
+  // the same field from a wave-uniform byte displacement (a level)
+  TE_DEV FieldB operator+(uint64_t d) const { return FieldB(reinterpret_cast<const float*>(b + d)); }
+};
+
+// Materialise x here: the value is computed at this point of the program
+// (its operands die) rather than sunk towards a later use.
+TE_DEV void te_pin(float& x) { asm volatile("" : "+v"(x)); }
+
 struct Dims {
   int nx, ny, nz, sJ, sK;
   float rdx, rdy, rdz;

@@ -99,6 +99,16 @@ static bool uses_column_lds(int kid) {
   const size_t l = std::strlen(nm);
   return l > 4 && std::strcmp(nm + l - 4, "_lds") == 0;
 }
+// The turbulence sweeps launched as 64 columns x AC_WAVES waves (k_turb.hip:
+// svi_waves_kernel, uv_waves_kernel, face_waves_kernel).
+static bool wave_sweep(int kid) {
+  static const int ids[] = {kernel_id("turb_scalar_vert_implicit_lds"), kernel_id("turb_u_tendency_lds"),
+                            kernel_id("turb_v_tendency_lds"), kernel_id("turb_w_tendency_lds"),
+                            kernel_id("tke_transport_lds")};
+  for (int id : ids)
+    if (kid == id) return true;
+  return false;
+}
 static uint32_t column_lanes() {
   static uint32_t lanes = [] {
     const char* e = std::getenv("TE_COLUMN_LANES");
@@ -131,15 +141,23 @@ static hipError_t launch(te_ctx* c, const Op& op) {
     void* args[] = {(void*)&c->dM, (void*)&a};
     return hipLaunchKernel(fn, g, b, args, dyn, c->stream);
   }
-  static const int kSviLds = kernel_id("turb_scalar_vert_implicit_lds");
-  if (op.kid == kSviLds) {  // 64 columns of a row per workgroup, AC_WAVES waves over the levels
-    const uint32_t lw = (op.nz + AC_WAVES - 1) / AC_WAVES;
-    if (op.nz < 4 || lw > 16) return hipErrorInvalidConfiguration;
+  if (a.n[2] == 1 && op.nz >= 4 && wave_sweep(op.kid)) {
+    // 64 columns of a row per workgroup, AC_WAVES waves over the levels
+    // (levels 0 .. nz-1, or 1 .. nz-1 for the face-level sweeps); shallower
+    // columns take the one-lane-per-column form below
+    static const int kSviLds = kernel_id("turb_scalar_vert_implicit_lds"), kULds = kernel_id("turb_u_tendency_lds"),
+                     kVLds = kernel_id("turb_v_tendency_lds"), kWLds = kernel_id("turb_w_tendency_lds");
+    const bool face = op.kid != kSviLds && op.kid != kULds && op.kid != kVLds;
+    const uint32_t lw = (op.nz - (face ? 1 : 0) + AC_WAVES - 1) / AC_WAVES;
+    if (lw > 16) return hipErrorInvalidConfiguration;
     b = dim3(64, AC_WAVES, 1);
     const dim3 g((a.n[0] + 63) / 64, a.n[1], 1);
     if (!launch_shape(a, b, g)) return hipErrorInvalidConfiguration;
     void* args[] = {(void*)&c->dM, (void*)&a};
-    return hipLaunchKernel(svi_waves_kernel((int)lw), g, b, args, 0, c->stream);
+    const void* fn = op.kid == kSviLds ? svi_waves_kernel((int)lw)
+                     : !face       ? uv_waves_kernel(op.kid == kULds ? 0 : 1, (int)lw)
+                                   : face_waves_kernel(op.kid == kWLds ? 0 : 1, (int)lw);
+    return hipLaunchKernel(fn, g, b, args, 0, c->stream);
   }
   size_t lds = 0;
   if (op.flat) {  // runs of 4 words per thread over the op's n0*n1*n2 words (te_flat)
@@ -154,21 +172,6 @@ static hipError_t launch(te_ctx* c, const Op& op) {
   if (a.n[2] == 1 && is_column_kernel(op.kid)) {
     b = dim3(column_lanes(), 1, 1);
     if (uses_column_lds(op.kid)) lds = (size_t)2 * op.nz * b.x * sizeof(float);
-    // turbulence sweeps: 4 waves per 64 columns, one row per block (the
-    // kernels' 4-wave forms, 3 nz floats of LDS per column;
-    // turb_scalar_vert_implicit_lds runs the AC_WAVES-wave form above)
-    static const int kU = kernel_id("turb_u_tendency_lds"), kV = kernel_id("turb_v_tendency_lds"),
-                     kW = kernel_id("turb_w_tendency_lds"), kT = kernel_id("tke_transport_lds");
-    const size_t mwLds = (size_t)3 * op.nz * 64 * sizeof(float);
-    const bool mwKernel = op.kid == kU || op.kid == kV || op.kid == kW || op.kid == kT;
-    if (mwKernel && op.nz >= 3 && mwLds <= 65536) {
-      b = dim3(64, 4, 1);
-      lds = mwLds;
-      const dim3 g((a.n[0] + 63) / 64, a.n[1], 1);
-      if (!launch_shape(a, b, g)) return hipErrorInvalidConfiguration;
-      void* args[] = {(void*)&c->dM, (void*)&a};
-      return hipLaunchKernel(kKernels[op.kid].fn, g, b, args, lds, c->stream);
-    }
   }
   if (op.tuned && !op.tplane) b = dim3(op.tb[0], op.tb[1], op.tb[2]);
   if (op.tuned ? op.tplane : op.plane) {  // (i, j) points numbered consecutively, level on grid y (te_tid3)
@@ -398,10 +401,12 @@ static int grow_guards(te_ctx* c) {
     // (te_div is exact for numerators below 2^24, divisors up to 2^16)
     op.plane = !op.flat && !ownMapping.count(kn) && op.A[0] % 64 != 0 && op.A[1] > 1 && op.A[0] <= (1u << 16) &&
                (uint64_t)op.A[0] * op.A[1] + 256 <= (1u << 24);
-    static const int kSviLds = te::kernel_id("turb_scalar_vert_implicit_lds");
-    if (op.kid == kSviLds ? nz > AC_WAVES * 16
+    if (te::wave_sweep(op.kid) ? nz > AC_WAVES * 16
                           : te::uses_column_lds(op.kid) && (size_t)2 * nz * te::column_lanes() * sizeof(float) > 65536)
       return fail(c, TE_ERR_UNSUPPORTED, std::string(te::kernel_name(op.kid)) + ": nz too large for the column sweep");
+    // the wave sweeps address a column within a level by a 32-bit byte offset
+    if (te::wave_sweep(op.kid) && (uint64_t)sJ * sK >= (1ull << 30))
+      return fail(c, TE_ERR_UNSUPPORTED, std::string(te::kernel_name(op.kid)) + ": a level spans 2^32 bytes or more");
     need = std::max<uint64_t>(need, (uint64_t)sJ * sK * (uint64_t)(nz + 3));
     if (std::strcmp(te::kernel_name(op.kid), "turb_sfcstress") == 0) {
       int32_t mode = 0;
