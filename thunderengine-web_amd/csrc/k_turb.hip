@@ -598,6 +598,19 @@ TE_DEV void relay_wait(int* flag, int want) {
   }
 }
 TE_DEV void relay_pass(int* flag, int v) { __hip_atomic_store(flag, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP); }
+// The column of a lane: a launch numbers the op's n0 x n1 columns row by
+// row, 64 per workgroup (the workgroup's number l = y * grid width + x), so
+// that no workgroup is left with a few columns of a row end.
+struct WaveColumn {
+  int i, j;
+  bool in;  // within the op's n0 x n1
+};
+TE_DEV WaveColumn wave_column(const KArg& a, int lane) {
+  const Blk b = te_block(a);
+  const uint32_t l = (uint32_t)b.y * a.lg[3] + (uint32_t)b.x, n0 = a.n[0];
+  const uint32_t c = l * 64u + (uint32_t)lane, j = c / n0;
+  return {(int)(c - j * n0), (int)j, c < n0 * a.n[1]};
+}
 // forward elimination of rows (A, C, R) over the wave's Lq levels: A becomes
 // pa, R becomes pb
 template <int LMAX>
@@ -652,15 +665,14 @@ TE_DEV void relay_back(WaveRelay& r, int wave, int lane, int Lq, bool top, const
 template <int DIR, int LMAX>
 __device__ __forceinline__ void uv_waves(uint32_t* M, const KArg& a) {
   __shared__ WaveRelay rl;
-  const Blk blk = te_block(a);
   const int lane = (int)threadIdx.x;
   const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.y);
   const Dims g = dims(a.q[0], 0);
-  const int i = blk.x * 64 + lane, j = blk.y, nz = g.nz;
+  const WaveColumn col = wave_column(a, lane);
+  const int i = col.i, j = col.j, nz = g.nz;
   const int Lw = (nz + AC_WAVES - 1) / AC_WAVES, k0 = wave * Lw, k1 = min(nz, k0 + Lw), Lq = max(0, k1 - k0);
   // a wave past the top (Lq = 0, when AC_WAVES does not divide nz) touches no level
-  const bool live = i < (int)a.n[0] && j < (int)a.n[1] && (DIR == 0 ? (i <= g.nx && j < g.ny) : (i < g.nx && j <= g.ny)) &&
-                    Lq > 0;
+  const bool live = col.in && (DIR == 0 ? (i <= g.nx && j < g.ny) : (i < g.nx && j <= g.ny)) && Lq > 0;
   relay_init(rl);
   auto F = [&](int n) { return FieldB(reinterpret_cast<float*>(M) + a.p[n]); };
   const FieldB q = F(1), w = F(2), rho = F(3), rf = F(4), kmv = F(5), tA = F(6), tB = F(7), tC = F(8), dwd = F(14),
@@ -1027,10 +1039,11 @@ template <bool LDS, int LMAX = 0>
 __device__ void w_tendency(uint32_t* M, const KArg& a) {
   const Blk b_ = te_block(a);
   const bool wv = LMAX > 0;
-  const int i = wv ? b_.x * 64 + (int)threadIdx.x : b_.x * TE_BX + (int)threadIdx.x;
-  const int j = wv ? b_.y : b_.y * TE_BY + (int)threadIdx.y;
+  const WaveColumn col = wv ? wave_column(a, (int)threadIdx.x) : WaveColumn{0, 0, true};
+  const int i = wv ? col.i : b_.x * TE_BX + (int)threadIdx.x;
+  const int j = wv ? col.j : b_.y * TE_BY + (int)threadIdx.y;
   Dims g = dims(a.q[0], 0);
-  const bool live = i < (int)a.n[0] && j < (int)a.n[1] && i < g.nx && j < g.ny;
+  const bool live = col.in && i < (int)a.n[0] && j < (int)a.n[1] && i < g.nx && j < g.ny;
   if (!live && !wv) return;
   const float *w = fq(a.q[1]), *rho = fq(a.q[2]), *rf = fq(a.q[3]), *kmh = fq(a.q[4]), *t13 = fq(a.q[5]),
               *t23 = fq(a.q[6]), *t33 = fq(a.q[7]), *mh = fq(a.q[8]), *mf = fq(a.q[9]), *hm = fq(a.q[30]);
@@ -1255,14 +1268,14 @@ __global__ __launch_bounds__(256) void k_turb_scalar_vert_implicit_lds(uint32_t*
 template <int LMAX>
 __device__ __forceinline__ void svi_waves(uint32_t* M, const KArg& a) {
   __shared__ WaveRelay rl;
-  const Blk blk = te_block(a);
   const int lane = (int)threadIdx.x;
   const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.y);
   const Dims g = dims(a.q[0], 0);
-  const int i = blk.x * 64 + lane, j = blk.y, nz = g.nz;
+  const WaveColumn col = wave_column(a, lane);
+  const int i = col.i, j = col.j, nz = g.nz;
   const int Lw = (nz + AC_WAVES - 1) / AC_WAVES, k0 = wave * Lw, k1 = min(nz, k0 + Lw), Lq = max(0, k1 - k0);
   // a wave past the top (Lq = 0, when AC_WAVES does not divide nz) touches no level
-  const bool active = i < g.nx && j < g.ny && i < (int)a.n[0] && j < (int)a.n[1] && Lq > 0;
+  const bool active = col.in && i < g.nx && j < g.ny && Lq > 0;
   relay_init(rl);
   auto F = [&](int q) { return reinterpret_cast<float*>(M) + a.p[q]; };
   const float *s = F(1), *rho = F(2), *rr = F(3), *rf = F(4), *khv = F(5), *mh = F(6), *mf = F(7), *fx = F(8),
@@ -1428,10 +1441,11 @@ template <bool LDS, int LMAX = 0>
 __device__ void tke_transport(uint32_t* M, const KArg& a) {
   const Blk b_ = te_block(a);
   const bool wv = LMAX > 0;
-  const int i = wv ? b_.x * 64 + (int)threadIdx.x : b_.x * TE_BX + (int)threadIdx.x;
-  const int j = wv ? b_.y : b_.y * TE_BY + (int)threadIdx.y;
+  const WaveColumn col = wv ? wave_column(a, (int)threadIdx.x) : WaveColumn{0, 0, true};
+  const int i = wv ? col.i : b_.x * TE_BX + (int)threadIdx.x;
+  const int j = wv ? col.j : b_.y * TE_BY + (int)threadIdx.y;
   Dims g = dims(a.q[0], 0);
-  const bool live = i < (int)a.n[0] && j < (int)a.n[1] && i < g.nx && j < g.ny;
+  const bool live = col.in && i < (int)a.n[0] && j < (int)a.n[1] && i < g.nx && j < g.ny;
   if (!live && !wv) return;
   const float *t = fq(a.q[1]), *rho = fq(a.q[2]), *rf = fq(a.q[3]), *kmh = fq(a.q[4]), *kmv = fq(a.q[5]),
               *mh = fq(a.q[6]), *mf = fq(a.q[7]), *fx = fq(a.q[11]), *fy = fq(a.q[12]), *hm = fq(a.q[30]);

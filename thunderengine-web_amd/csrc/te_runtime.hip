@@ -151,7 +151,9 @@ static hipError_t launch(te_ctx* c, const Op& op) {
     const uint32_t lw = (op.nz - (face ? 1 : 0) + AC_WAVES - 1) / AC_WAVES;
     if (lw > 16) return hipErrorInvalidConfiguration;
     b = dim3(64, AC_WAVES, 1);
-    const dim3 g((a.n[0] + 63) / 64, a.n[1], 1);
+    // the n0 x n1 columns numbered row by row, 64 per workgroup (wave_column)
+    const uint64_t nb = ((uint64_t)a.n[0] * a.n[1] + 63) / 64;
+    const dim3 g = nb <= 65535 ? dim3((uint32_t)nb, 1, 1) : dim3(4096, (uint32_t)((nb + 4095) / 4096), 1);
     if (!launch_shape(a, b, g)) return hipErrorInvalidConfiguration;
     void* args[] = {(void*)&c->dM, (void*)&a};
     const void* fn = op.kid == kSviLds ? svi_waves_kernel((int)lw)
