@@ -97,14 +97,14 @@ int te_op_get(const te_ctx* ctx, int32_t i, const char** name, uint32_t* args64)
 int32_t te_kernel_count(void);
 const char* te_kernel_name(int32_t i);
 
-/* Self-test of the kernels' binary64 shortcuts for log, exp and pow (no
- * counterpart in the reference): over every binary32 input x, on device 0,
- * counts the x where the shortcut differs in any bit from the double-float
- * routine the kernels restate (mismatch[f]), the first such x (first[f],
- * 0xFFFFFFFF if none), and the x where the shortcut defers to that routine
+/* Self-test of the kernels' shortcuts (no counterpart in the reference):
+ * over every binary32 input x, on device 0, counts the x where the shortcut
+ * differs in any bit from what it stands for (mismatch[f]), the first such x
+ * (first[f], 0xFFFFFFFF if none), and the x where the shortcut defers to it
  * (slow[f]); f = 0 log, 1 exp, 2 pow(x, cp/rd), 3 pow(x, rd/cv),
- * 4 pow(x, 1/3). */
-int te_selftest_math(uint64_t mismatch[5], uint32_t first[5], uint64_t slow[5]);
+ * 4 pow(x, 1/3) (binary64 evaluations against the double-float routines the
+ * kernels restate), 5 the recurrences' reciprocal against 1.0f / x. */
+int te_selftest_math(uint64_t mismatch[6], uint32_t first[6], uint64_t slow[6]);
 
 /* Device duration of the last te_run in milliseconds (HIP events on the
  * context's stream). */

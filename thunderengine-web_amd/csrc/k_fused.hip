@@ -519,7 +519,7 @@ __device__ __forceinline__ void ac_column(uint32_t* M, const KArg& a) {
             const float bb = fma_(-asq, m * ((-PQ[t]) + p2q_m), 1.0f);
             const float dd = fma_(ac.alph, (-m) * (PF[t] - ppf_m), e);
             const bool first_row = k == 1;
-            const float r = 1.0f / (first_row ? bb : fma_(aa, pa, bb));
+            const float r = recip(first_row ? bb : fma_(aa, pa, bb));
             pa = (-cc) * r;
             pb = (first_row ? dd : fma_(-aa, pb, dd)) * r;
             ME[t] = pa;  // not read for the top row (w there is pb)

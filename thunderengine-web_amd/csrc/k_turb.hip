@@ -628,7 +628,7 @@ TE_DEV void relay_forward(WaveRelay& r, int wave, int lane, int Lq, float (&A)[L
       if (t < Lq) {
         const float cfa = A[t], cfc = C[t];
         const float cfb = (1.0f - cfa) - cfc;
-        const float tt = 1.0f / fma_(cfa, pa, cfb);
+        const float tt = recip(fma_(cfa, pa, cfb));
         pa = (-cfc) * tt;
         pb = fma_(-cfa, pb, R[t]) * tt;
         A[t] = pa;
@@ -721,38 +721,65 @@ __device__ __forceinline__ void uv_waves(uint32_t* M, const KArg& a) {
   if (live) {
     const float hmf = DIR == 0 ? hx : hy;
     // Level t's dwd and Y = ((kmv + kmv') (mf + mf)) (rf + rf'), then row
-    // t - 1 (Y of its level and the next; level nz: dwd only).  Every level's
-    // loads are unconditional (slots past the wave's range re-read its last
-    // level), so that they go out ahead of the arithmetic, four levels at a
-    // time (sched_barrier) to bound the registers in flight.
+    // t - 1 (Y of its level and the next; level nz: dwd only), in chunks of
+    // UV_CH levels: the chunk's loads (unconditional: slots past the wave's
+    // range re-read its last level) all issued, then its arithmetic, each
+    // value materialised where it is formed (te_pin) so that the loaded
+    // operands die there.
+    constexpr int UV_CH = 4;
     const float tCb = tC(o), tCt = (tC + L(nz))(o);
     float Yp = 0.0f;
 #pragma unroll
-    for (int t = 0; t <= LMAX; ++t) {
-      const int k = min(k0 + t, nz);
-      const uint64_t d = L(k);
-      const FieldB km = kmv + d, rk = rf + d, wk = w + d;
-      const float ka = km(o - ost), kb = km(o), ra = rk(o - ost), rb = rk(o);
-      DW[t] = (((((wk(o) - wk(o - ost)) * rdh) * hmf) * 0.25f) * (ka + kb)) * (ra + rb);
-      te_pin(DW[t]);
-      float Yt = ((ka + kb) * (mf[k] + mf[k])) * (ra + rb);
-      te_pin(Yt);
-      if (t > 0) {
-        const int r = t - 1, kr = min(k0 + r, nz - 1);
-        const FieldB ro = rho + L(kr);
-        const float rinvk = 1.0f / (0.5f * (ro(o - ost) + ro(o)));
-        const float mhk = mh[kr], qk = (q + L(kr))(o);
-        const float tem = (mhk + mhk) * rinvk;
-        CA[r] = kr == 0 ? 0.0f : tema * (Yp * tem);
-        CC[r] = kr < nz - 1 ? tema * (Yt * tem) : 0.0f;
-        const float tb = kr == 0 ? -((temc * tCb) * (mhk + mhk)) : (temc * tCt) * (mhk + mhk);
-        R[r] = kr == 0 || kr == nz - 1 ? fma_(tb, rinvk, qk) : qk;
-        te_pin(CA[r]);
-        te_pin(CC[r]);
-        te_pin(R[r]);
+    for (int t0 = 0; t0 <= LMAX; t0 += UV_CH) {
+      float KA[UV_CH], KB[UV_CH], RA[UV_CH], RB[UV_CH], W0[UV_CH], W1[UV_CH], MF[UV_CH], P0[UV_CH], P1[UV_CH],
+          MH[UV_CH], QK[UV_CH];
+#pragma unroll
+      for (int e = 0; e < UV_CH; ++e) {
+        const int t = t0 + e;
+        if (t > LMAX) break;
+        const int k = min(k0 + t, nz);
+        const uint64_t d = L(k);
+        const FieldB km = kmv + d, rk = rf + d, wk = w + d;
+        KA[e] = km(o - ost);
+        KB[e] = km(o);
+        RA[e] = rk(o - ost);
+        RB[e] = rk(o);
+        W0[e] = wk(o);
+        W1[e] = wk(o - ost);
+        MF[e] = mf[k];
+        if (t > 0) {
+          const int kr = min(k0 + t - 1, nz - 1);
+          const uint64_t dr = L(kr);
+          const FieldB ro = rho + dr;
+          P0[e] = ro(o - ost);
+          P1[e] = ro(o);
+          MH[e] = mh[kr];
+          QK[e] = (q + dr)(o);
+        }
       }
-      Yp = Yt;
-      if ((t & 3) == 3) __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int e = 0; e < UV_CH; ++e) {
+        const int t = t0 + e;
+        if (t > LMAX) break;
+        DW[t] = (((((W0[e] - W1[e]) * rdh) * hmf) * 0.25f) * (KA[e] + KB[e])) * (RA[e] + RB[e]);
+        te_pin(DW[t]);
+        float Yt = ((KA[e] + KB[e]) * (MF[e] + MF[e])) * (RA[e] + RB[e]);
+        te_pin(Yt);
+        if (t > 0) {
+          const int r = t - 1, kr = min(k0 + r, nz - 1);
+          const float rinvk = 1.0f / (0.5f * (P0[e] + P1[e]));
+          const float mhk = MH[e], qk = QK[e];
+          const float tem = (mhk + mhk) * rinvk;
+          CA[r] = kr == 0 ? 0.0f : tema * (Yp * tem);
+          CC[r] = kr < nz - 1 ? tema * (Yt * tem) : 0.0f;
+          const float tb = kr == 0 ? -((temc * tCb) * (mhk + mhk)) : (temc * tCt) * (mhk + mhk);
+          R[r] = kr == 0 || kr == nz - 1 ? fma_(tb, rinvk, qk) : qk;
+          te_pin(CA[r]);
+          te_pin(CC[r]);
+          te_pin(R[r]);
+        }
+        Yp = Yt;
+      }
     }
 #pragma unroll
     for (int t = 0; t <= LMAX; ++t)
@@ -762,33 +789,69 @@ __device__ __forceinline__ void uv_waves(uint32_t* M, const KArg& a) {
   // the horizontal divergences and old tendencies, loaded while the back
   // substitution comes down (into CC, free after the elimination); the
   // vertical difference of dwd in place of DW
+  // (the top wave, first in the back substitution, loads them after it)
   float T0[LMAX];
-  if (live) {
+  auto prefetch = [&]() {
+    constexpr int PF_CH = 5;
 #pragma unroll
-    for (int t = 0; t < LMAX; ++t) {
-      const uint64_t d = L(min(k0 + t, nz - 1));
-      T0[t] = txy(d);
-      te_pin(T0[t]);
-      CC[t] = (ten + d)(o);
-      DW[t] = ((DW[t + 1] - DW[t]) * g.rdz) * 0.5f;
-      if ((t & 3) == 3) __builtin_amdgcn_sched_barrier(0);
+    for (int t0 = 0; t0 < LMAX; t0 += PF_CH) {
+      float A0[PF_CH], A1[PF_CH], B0[PF_CH], B1[PF_CH];
+#pragma unroll
+      for (int e = 0; e < PF_CH; ++e) {
+        const int t = t0 + e;
+        if (t >= LMAX) break;
+        const uint64_t d = L(min(k0 + t, nz - 1));
+        const FieldB A = tA + d, B = tB + d;
+        A0[e] = DIR == 0 ? A(o) : A(o + 4u);
+        A1[e] = DIR == 0 ? A(o - 4u) : A(o);
+        B0[e] = DIR == 0 ? B(o + osj) : B(o);
+        B1[e] = DIR == 0 ? B(o) : B(o - osj);
+        CC[t] = (ten + d)(o);
+      }
+#pragma unroll
+      for (int e = 0; e < PF_CH; ++e) {
+        const int t = t0 + e;
+        if (t >= LMAX) break;
+        const float tx = zeroX ? 0.0f : ((A0[e] - A1[e]) * g.rdx) * hx;
+        const float ty = zeroY ? 0.0f : ((B0[e] - B1[e]) * g.rdy) * hy;
+        T0[t] = tx + ty;
+        te_pin(T0[t]);
+        DW[t] = ((DW[t + 1] - DW[t]) * g.rdz) * 0.5f;
+      }
     }
-  }
+  };
+  if (live && !top) prefetch();
   relay_back<LMAX>(rl, wave, lane, Lq, top, CA, R);
+  if (live && top) prefetch();
   if (live) {
     // every level's new tendency into R, then the stores (no load waits
-    // behind a store)
+    // behind a store); loads in chunks as above
+    constexpr int FN_CH = 5;
 #pragma unroll
-    for (int t = 0; t < LMAX; ++t) {
-      const int k = min(k0 + t, nz - 1);
-      const uint64_t d = L(k);
-      const FieldB rk = rho + d;
-      const float ro = rk(o - ost) + rk(o), mhk = mh[k];
-      float tz = (ro * (R[t] - (q + d)(o))) * rdt;
-      tz = fma_(DW[t], mhk + mhk, tz);
-      R[t] = fma_(T0[t] + tz, 1.0f / (0.5f * ro), CC[t]);
-      te_pin(R[t]);
-      if ((t & 3) == 3) __builtin_amdgcn_sched_barrier(0);
+    for (int t0 = 0; t0 < LMAX; t0 += FN_CH) {
+      float P0[FN_CH], P1[FN_CH], QK[FN_CH], MH[FN_CH];
+#pragma unroll
+      for (int e = 0; e < FN_CH; ++e) {
+        const int t = t0 + e;
+        if (t >= LMAX) break;
+        const int k = min(k0 + t, nz - 1);
+        const uint64_t d = L(k);
+        const FieldB rk = rho + d;
+        P0[e] = rk(o - ost);
+        P1[e] = rk(o);
+        QK[e] = (q + d)(o);
+        MH[e] = mh[k];
+      }
+#pragma unroll
+      for (int e = 0; e < FN_CH; ++e) {
+        const int t = t0 + e;
+        if (t >= LMAX) break;
+        const float ro = P0[e] + P1[e];
+        float tz = (ro * (R[t] - QK[e])) * rdt;
+        tz = fma_(DW[t], MH[e] + MH[e], tz);
+        R[t] = fma_(T0[t] + tz, 1.0f / (0.5f * ro), CC[t]);
+        te_pin(R[t]);
+      }
     }
 #pragma unroll
     for (int t = 0; t < LMAX; ++t)
@@ -971,60 +1034,114 @@ __device__ __forceinline__ void face_waves(WaveRelay& rl, bool live, int wave, i
   const FieldB KMH(kmh), RHO(rho), RF_(rf), X(x), HX(out.hx), HY(out.hy), TEN(out.ten);
   const float hxm = out.zeroX ? 0.0f : hm_uh(out.hm, out.g, live ? out.i : 0),
               hym = out.zeroY ? 0.0f : hm_vh(out.hm, out.g, live ? out.j : 0);
-  float CA[LMAX], CC[LMAX], R[LMAX], XO[LMAX], RF[LMAX];
+  float CA[LMAX], CC[LMAX], R[LMAX];
   if (live) {
     // Z(l) = ((kmh_l + kmh_l+1) mh_l) rho_l at l = k0-1 .. k1-1, then the
-    // rows.  Every level's loads are unconditional (slots past the wave's
-    // range re-read its last level), four levels at a time (sched_barrier),
-    // each value materialised where it is formed (te_pin).
+    // rows, in chunks of FW_CH levels: the chunk's loads (unconditional: slots
+    // past the wave's range re-read its last level) all issued, then its
+    // arithmetic, each value materialised where it is formed (te_pin).
+    constexpr int FW_CH = 4;
     float Zp = 0.0f;
 #pragma unroll
-    for (int t = 0; t <= LMAX; ++t) {
-      const int l = min(k0 - 1 + t, nz - 1);
-      const uint64_t d = L(l);
-      float Zt = (((KMH + d)(o) + (KMH + d + osk)(o)) * mh[l]) * (RHO + d)(o);
-      te_pin(Zt);
-      if (t > 0) {
-        const int r = t - 1, k = min(k0 + r, nz - 1);
-        const uint64_t dk = L(k);
-        XO[r] = (X + dk)(o);
-        RF[r] = (RF_ + dk)(o);
-        const float mfrf = mf[k] / RF[r];
-        CA[r] = k == 1 ? 0.0f : tema * (Zp * mfrf);
-        CC[r] = k == nz - 1 ? 0.0f : tema * (Zt * mfrf);
-        R[r] = k == 1 ? cfd1 : (k == nz - 1 ? cfdT : XO[r]);
-        te_pin(CA[r]);
-        te_pin(CC[r]);
+    for (int t0 = 0; t0 <= LMAX; t0 += FW_CH) {
+      float K0[FW_CH], K1[FW_CH], MHl[FW_CH], RH[FW_CH], XV[FW_CH], RV[FW_CH], MF[FW_CH];
+#pragma unroll
+      for (int e = 0; e < FW_CH; ++e) {
+        const int t = t0 + e;
+        if (t > LMAX) break;
+        const int l = min(k0 - 1 + t, nz - 1);
+        const uint64_t d = L(l);
+        K0[e] = (KMH + d)(o);
+        K1[e] = (KMH + d + osk)(o);
+        MHl[e] = mh[l];
+        RH[e] = (RHO + d)(o);
+        if (t > 0) {
+          const int k = min(k0 + t - 1, nz - 1);
+          const uint64_t dk = L(k);
+          XV[e] = (X + dk)(o);
+          RV[e] = (RF_ + dk)(o);
+          MF[e] = mf[k];
+        }
       }
-      Zp = Zt;
-      if ((t & 3) == 3) __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int e = 0; e < FW_CH; ++e) {
+        const int t = t0 + e;
+        if (t > LMAX) break;
+        float Zt = ((K0[e] + K1[e]) * MHl[e]) * RH[e];
+        te_pin(Zt);
+        if (t > 0) {
+          const int r = t - 1, k = min(k0 + r, nz - 1);
+          const float mfrf = MF[e] / RV[e];
+          CA[r] = k == 1 ? 0.0f : tema * (Zp * mfrf);
+          CC[r] = k == nz - 1 ? 0.0f : tema * (Zt * mfrf);
+          R[r] = k == 1 ? cfd1 : (k == nz - 1 ? cfdT : XV[e]);
+          te_pin(CA[r]);
+          te_pin(CC[r]);
+        }
+        Zp = Zt;
+      }
     }
   }
   relay_forward<LMAX>(rl, wave, lane, Lq, CA, CC, R);
   // the horizontal divergences (tx + ty, into CC: free after the
   // elimination) and the old tendencies, loaded while the back substitution
-  // comes down (FaceTendency::fluxes)
+  // comes down (FaceTendency::fluxes; the top wave, first in the back
+  // substitution, loads them after it)
   float TV[LMAX];
-  if (live) {
+  auto prefetch = [&]() {
+    constexpr int PF_CH = 5;
 #pragma unroll
-    for (int t = 0; t < LMAX; ++t) {
-      const uint64_t d = L(min(k0 + t, nz - 1));
-      const FieldB A = HX + d, B = HY + d;
-      const float dx = A(o + 4u) - A(o), dy = B(o + osj) - B(o);
-      const float tx = out.zeroX ? 0.0f : ((out.negate ? -dx : dx) * out.g.rdx) * hxm;
-      const float ty = out.zeroY ? 0.0f : ((out.negate ? -dy : dy) * out.g.rdy) * hym;
-      CC[t] = tx + ty;
-      te_pin(CC[t]);
-      TV[t] = (TEN + d)(o);
-      if ((t & 3) == 3) __builtin_amdgcn_sched_barrier(0);
+    for (int t0 = 0; t0 < LMAX; t0 += PF_CH) {
+      float A0[PF_CH], A1[PF_CH], B0[PF_CH], B1[PF_CH];
+#pragma unroll
+      for (int e = 0; e < PF_CH; ++e) {
+        const int t = t0 + e;
+        if (t >= LMAX) break;
+        const uint64_t d = L(min(k0 + t, nz - 1));
+        const FieldB A = HX + d, B = HY + d;
+        A0[e] = A(o + 4u);
+        A1[e] = A(o);
+        B0[e] = B(o + osj);
+        B1[e] = B(o);
+        TV[t] = (TEN + d)(o);
+      }
+#pragma unroll
+      for (int e = 0; e < PF_CH; ++e) {
+        const int t = t0 + e;
+        if (t >= LMAX) break;
+        const float dx = A0[e] - A1[e], dy = B0[e] - B1[e];
+        const float tx = out.zeroX ? 0.0f : ((out.negate ? -dx : dx) * out.g.rdx) * hxm;
+        const float ty = out.zeroY ? 0.0f : ((out.negate ? -dy : dy) * out.g.rdy) * hym;
+        CC[t] = tx + ty;
+        te_pin(CC[t]);
+      }
     }
-  }
+  };
+  if (live && !top) prefetch();
   relay_back<LMAX>(rl, wave, lane, Lq, top, CA, R);
+  if (live && top) prefetch();
   if (live) {
+    // FaceTendency::apply, x and rf loaded again (chunks as above)
+    constexpr int FN_CH = 5;
 #pragma unroll
-    for (int t = 0; t < LMAX; ++t) {  // FaceTendency::apply
-      const float tz = (RF[t] * (R[t] - XO[t])) * out.rdt;
-      R[t] = TV[t] + (CC[t] + tz) / RF[t];
+    for (int t0 = 0; t0 < LMAX; t0 += FN_CH) {
+      float XO[FN_CH], RF[FN_CH];
+#pragma unroll
+      for (int e = 0; e < FN_CH; ++e) {
+        const int t = t0 + e;
+        if (t >= LMAX) break;
+        const uint64_t d = L(min(k0 + t, nz - 1));
+        XO[e] = (X + d)(o);
+        RF[e] = (RF_ + d)(o);
+      }
+#pragma unroll
+      for (int e = 0; e < FN_CH; ++e) {
+        const int t = t0 + e;
+        if (t >= LMAX) break;
+        const float tz = (RF[e] * (R[t] - XO[e])) * out.rdt;
+        R[t] = TV[t] + (CC[t] + tz) / RF[e];
+        te_pin(R[t]);
+      }
     }
 #pragma unroll
     for (int t = 0; t < LMAX; ++t)
@@ -1354,49 +1471,98 @@ __device__ __forceinline__ void svi_waves(uint32_t* M, const KArg& a) {
       dT = fma_(-((temc * dumz) * mh[nz - 1]), rr[c], s[c]);
     }
     float Xk = X(k0 == 0 ? 1 : k0);  // X(k) (k0 = 0: not used)
+    constexpr int SV_CH = 4;
 #pragma unroll
-    for (int t = 0; t < LMAX; ++t) {
-      const int k = min(k0 + t, nz - 1);
-      const uint64_t d = L(k);
-      const float r = (RR_ + d)(o), m = mh[k];
-      float Xn = X(min(k + 1, nz - 1));
-      Xn = k + 1 < nz ? Xn : 0.0f;
-      te_pin(Xn);
-      CA[t] = k == 0 ? 0.0f : tema * ((Xk * m) * r);
-      CC[t] = k == nz - 1 ? 0.0f : tema * (((k == 0 ? Xk : Xn) * m) * r);
-      const float sv = (S_ + d)(o);
-      D[t] = k == 0 && flux0 ? d0 : (k == nz - 1 && fluxT ? dT : sv);
-      te_pin(CA[t]);
-      te_pin(CC[t]);
-      te_pin(D[t]);
-      Xk = k == 0 ? Xk : Xn;
-      if ((t & 3) == 3) __builtin_amdgcn_sched_barrier(0);
+    for (int t0 = 0; t0 < LMAX; t0 += SV_CH) {
+      float RV[SV_CH], MH[SV_CH], KN[SV_CH], MFN[SV_CH], RFN[SV_CH], SV[SV_CH];
+#pragma unroll
+      for (int e = 0; e < SV_CH; ++e) {
+        const int t = t0 + e;
+        if (t >= LMAX) break;
+        const int k = min(k0 + t, nz - 1), kn = min(k + 1, nz - 1);
+        const uint64_t d = L(k), dn = L(kn);
+        RV[e] = (RR_ + d)(o);
+        MH[e] = mh[k];
+        KN[e] = (KHV + dn)(o);
+        MFN[e] = mf[kn];
+        RFN[e] = (RF + dn)(o);
+        SV[e] = (S_ + d)(o);
+      }
+#pragma unroll
+      for (int e = 0; e < SV_CH; ++e) {
+        const int t = t0 + e;
+        if (t >= LMAX) break;
+        const int k = min(k0 + t, nz - 1);
+        const float r = RV[e], m = MH[e];
+        float Xn = (KN[e] * MFN[e]) * RFN[e];
+        Xn = k + 1 < nz ? Xn : 0.0f;
+        te_pin(Xn);
+        CA[t] = k == 0 ? 0.0f : tema * ((Xk * m) * r);
+        CC[t] = k == nz - 1 ? 0.0f : tema * (((k == 0 ? Xk : Xn) * m) * r);
+        D[t] = k == 0 && flux0 ? d0 : (k == nz - 1 && fluxT ? dT : SV[e]);
+        te_pin(CA[t]);
+        te_pin(CC[t]);
+        te_pin(D[t]);
+        Xk = k == 0 ? Xk : Xn;
+      }
     }
   }
   relay_forward<LMAX>(rl, wave, lane, Lq, CA, CC, D);
   // the horizontal divergence (into CC, free after the elimination) and the
-  // old tendency, loaded while the back substitution comes down
+  // old tendency, loaded while the back substitution comes down (the top
+  // wave, first in the back substitution, loads them after it)
   float TV[LMAX];
-  if (active) {
+  auto prefetch = [&]() {
+    constexpr int PF_CH = 5;
 #pragma unroll
-    for (int t = 0; t < LMAX; ++t) {
-      const uint64_t d = L(min(k0 + t, nz - 1));
-      const FieldB A = FX + d, B = FY + d;
-      CC[t] = ((-(A(o + 4u) - A(o))) * g.rdx) * uh + ((-(B(o + osj) - B(o))) * g.rdy) * vh;
-      te_pin(CC[t]);
-      TV[t] = (STEN + d)(o);
-      if ((t & 3) == 3) __builtin_amdgcn_sched_barrier(0);
+    for (int t0 = 0; t0 < LMAX; t0 += PF_CH) {
+      float A0[PF_CH], A1[PF_CH], B0[PF_CH], B1[PF_CH];
+#pragma unroll
+      for (int e = 0; e < PF_CH; ++e) {
+        const int t = t0 + e;
+        if (t >= LMAX) break;
+        const uint64_t d = L(min(k0 + t, nz - 1));
+        const FieldB A = FX + d, B = FY + d;
+        A0[e] = A(o + 4u);
+        A1[e] = A(o);
+        B0[e] = B(o + osj);
+        B1[e] = B(o);
+        TV[t] = (STEN + d)(o);
+      }
+#pragma unroll
+      for (int e = 0; e < PF_CH; ++e) {
+        const int t = t0 + e;
+        if (t >= LMAX) break;
+        CC[t] = ((-(A0[e] - A1[e])) * g.rdx) * uh + ((-(B0[e] - B1[e])) * g.rdy) * vh;
+        te_pin(CC[t]);
+      }
     }
-  }
+  };
+  if (active && !top) prefetch();
   relay_back<LMAX>(rl, wave, lane, Lq, top, CA, D);
+  if (active && top) prefetch();
   if (active) {
     // every level's new tendency into D, then the stores
+    constexpr int FN_CH = 5;
 #pragma unroll
-    for (int t = 0; t < LMAX; ++t) {
-      const uint64_t d = L(min(k0 + t, nz - 1));
-      D[t] = fma_(CC[t] + ((RHO + d)(o) * (D[t] - (S_ + d)(o))) * rdt, (RR_ + d)(o), TV[t]);
-      te_pin(D[t]);
-      if ((t & 3) == 3) __builtin_amdgcn_sched_barrier(0);
+    for (int t0 = 0; t0 < LMAX; t0 += FN_CH) {
+      float RH[FN_CH], SO[FN_CH], RV[FN_CH];
+#pragma unroll
+      for (int e = 0; e < FN_CH; ++e) {
+        const int t = t0 + e;
+        if (t >= LMAX) break;
+        const uint64_t d = L(min(k0 + t, nz - 1));
+        RH[e] = (RHO + d)(o);
+        SO[e] = (S_ + d)(o);
+        RV[e] = (RR_ + d)(o);
+      }
+#pragma unroll
+      for (int e = 0; e < FN_CH; ++e) {
+        const int t = t0 + e;
+        if (t >= LMAX) break;
+        D[t] = fma_(CC[t] + (RH[e] * (D[t] - SO[e])) * rdt, RV[e], TV[t]);
+        te_pin(D[t]);
+      }
     }
 #pragma unroll
     for (int t = 0; t < LMAX; ++t)

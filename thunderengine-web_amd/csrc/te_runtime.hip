@@ -224,11 +224,12 @@ __global__ void k_zero_words(uint32_t* p, int n) {
 // process ends), so that entry point allocates nothing.
 static __device__ __align__(256) uint32_t scratch[TE_NOALLOC_MAX_REGIONS];
 
-// Every binary32 bit pattern through fast_* and rr_* (te_selftest_math):
-// out[0..4] mismatch counts, [5..9] deferrals (64-bit), first[0..4].
+// Every binary32 bit pattern through fast_* and rr_*, and through recip and
+// the IEEE division (te_selftest_math): out[0..5] mismatch counts, [6..11]
+// deferrals (64-bit), first[0..5].
 __global__ __launch_bounds__(256) void k_selftest_math(unsigned long long* out, uint32_t* first) {
   const float cpdrd = te::CP / te::RD, rddcv = te::RD / (te::CP - te::RD), third = 0.33333334f;
-  uint32_t bad[5] = {0, 0, 0, 0, 0}, slow[5] = {0, 0, 0, 0, 0};
+  uint32_t bad[6] = {0, 0, 0, 0, 0, 0}, slow[6] = {0, 0, 0, 0, 0, 0};
   for (uint64_t n = blockIdx.x * 256ull + threadIdx.x; n < (1ull << 32); n += (uint64_t)gridDim.x * 256ull) {
     const uint32_t bits = (uint32_t)n;
     const float x = __uint_as_float(bits);
@@ -252,10 +253,18 @@ __global__ __launch_bounds__(256) void k_selftest_math(unsigned long long* out, 
       float t;
       if (!special[f] && !te::f32_clear(d[f], t, f < 2 ? 0x1p-40 : 0x1p-32)) ++slow[f];
     }
+    // recip against the IEEE division (deferrals: x outside [2^-64, 2^64])
+    const float rc = te::recip(x), qc = 1.0f / x;
+    if (__float_as_uint(rc) != __float_as_uint(qc) && !(rc != rc && qc != qc)) {
+      ++bad[5];
+      atomicMin(&first[5], bits);
+    }
+    const float ax = te::ab(x);
+    if (!(ax >= 0x1p-64f && ax <= 0x1p64f)) ++slow[5];
   }
-  for (int f = 0; f < 5; ++f) {
+  for (int f = 0; f < 6; ++f) {
     if (bad[f]) atomicAdd(&out[f], (unsigned long long)bad[f]);
-    if (slow[f]) atomicAdd(&out[5 + f], (unsigned long long)slow[f]);
+    if (slow[f]) atomicAdd(&out[6 + f], (unsigned long long)slow[f]);
   }
 }
 
@@ -819,12 +828,12 @@ int te_op_get(const te_ctx* c, int32_t i, const char** name, uint32_t* args64) {
   return TE_OK;
 }
 
-int te_selftest_math(uint64_t mismatch[5], uint32_t first[5], uint64_t slow[5]) {
+int te_selftest_math(uint64_t mismatch[6], uint32_t first[6], uint64_t slow[6]) {
   if (!mismatch || !first || !slow) return TE_ERR_ARG;
   unsigned long long* out = nullptr;
   uint32_t* fb = nullptr;
-  const uint32_t ones[5] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu};
-  unsigned long long h[10];
+  const uint32_t ones[6] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu};
+  unsigned long long h[12];
   hipError_t e = hipMalloc((void**)&out, sizeof(h));
   if (e == hipSuccess) e = hipMalloc((void**)&fb, sizeof(ones));
   if (e == hipSuccess) e = hipMemset(out, 0, sizeof(h));
@@ -839,9 +848,9 @@ int te_selftest_math(uint64_t mismatch[5], uint32_t first[5], uint64_t slow[5]) 
   if (out) (void)hipFree(out);
   if (fb) (void)hipFree(fb);
   if (e != hipSuccess) return TE_ERR_HIP;
-  for (int f = 0; f < 5; ++f) {
+  for (int f = 0; f < 6; ++f) {
     mismatch[f] = h[f];
-    slow[f] = h[5 + f];
+    slow[f] = h[6 + f];
   }
   return TE_OK;
 }

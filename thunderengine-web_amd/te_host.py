@@ -168,13 +168,14 @@ def kernel_names():
 
 def selftest_math():
     """te_selftest_math: per function (log, exp, pow(cp/rd), pow(rd/cv),
-    pow(1/3)) the binary32 inputs where the kernels' binary64 shortcut
-    differs from the double-float routine, the first one, and the inputs
-    where the shortcut defers to the routine."""
-    mis, first, slow = (ctypes.c_uint64 * 5)(), (ctypes.c_uint32 * 5)(), (ctypes.c_uint64 * 5)()
+    pow(1/3): the kernels' binary64 shortcuts against the double-float
+    routines; recip: the recurrences' reciprocal against 1.0f / x) the
+    binary32 inputs where the two differ, the first one, and the inputs
+    where the shortcut defers."""
+    mis, first, slow = (ctypes.c_uint64 * 6)(), (ctypes.c_uint32 * 6)(), (ctypes.c_uint64 * 6)()
     if lib().te_selftest_math(mis, first, slow) != TE_OK:
         raise TeError("te_selftest_math failed")
-    names = ("log", "exp", "pow_cpdrd", "pow_rddcv", "pow_third")
+    names = ("log", "exp", "pow_cpdrd", "pow_rddcv", "pow_third", "recip")
     return {n: {"mismatch": int(mis[q]), "first": int(first[q]), "deferred": int(slow[q])} for q, n in enumerate(names)}
 
 
