@@ -507,6 +507,22 @@ __device__ __forceinline__ void adv_fused_body(uint32_t* M, const KArg& a) {
       });
 }
 
+// One kernel per specialised body (adv_fused_kernel), so that each carries
+// only its own registers; k_adv_fused selects the body at run time.
+template <int KIND, int STC>
+__global__ __launch_bounds__(256) void k_adv_fused_v(uint32_t* M, KArg a) {
+  adv_fused_body<KIND, STC>(M, a);
+}
+const void* adv_fused_kernel(int kind, int stag) {
+  if (kind == 0 && stag == 1) return (const void*)&k_adv_fused_v<0, 1>;
+  if (kind == 1 && stag == 1) return (const void*)&k_adv_fused_v<1, 1>;
+  if (kind == 2 && stag == 2) return (const void*)&k_adv_fused_v<2, 2>;
+  if (kind == 2 && stag == 3) return (const void*)&k_adv_fused_v<2, 3>;
+  if (kind == 2 && stag == 4) return (const void*)&k_adv_fused_v<2, 4>;
+  if (kind == 3 && stag == 4) return (const void*)&k_adv_fused_v<3, 4>;
+  return nullptr;
+}
+
 // The three flux dispatches of one advected field and adv_assemble in one
 // launch (see above); the flux kind selects one specialised body.
 __global__ __launch_bounds__(256) void k_adv_fused(uint32_t* M, KArg a) {

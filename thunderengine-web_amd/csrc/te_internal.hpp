@@ -23,6 +23,9 @@ struct Op {
   // launch shape chosen by te_autotune (generic 3-D kernels only)
   bool tuned = false, tplane = false;
   uint16_t tb[3] = {0, 0, 0};
+  // the kernel launched in place of the op's table entry: a body specialised
+  // on parameters the op fixes (set with nz; null: the table's kernel)
+  const void* fn = nullptr;
 };
 
 int kernel_id(const char* name);

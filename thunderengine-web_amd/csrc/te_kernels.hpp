@@ -45,6 +45,9 @@ const void* ac_column_kernel(int levels_per_wave);
 // turb_scalar_vert_implicit_lds in 64 x AC_WAVES workgroups (k_turb.hip),
 // instantiated for 1..16 levels per wave (nullptr outside)
 const void* svi_waves_kernel(int levels_per_wave);
+// adv_fused specialised on (flux kind, stagger) (k_adv.hip); null for the
+// pairs only the run-time dispatch of k_adv_fused serves
+const void* adv_fused_kernel(int kind, int stag);
 // turb_u_tendency_lds (dir 0) / turb_v_tendency_lds (dir 1) in the same form
 const void* uv_waves_kernel(int dir, int levels_per_wave);
 // turb_w_tendency_lds (which 0) / tke_transport_lds (which 1): levels 1 .. nz-1

@@ -191,7 +191,7 @@ static hipError_t launch(te_ctx* c, const Op& op) {
   dim3 g((a.n[0] + b.x - 1) / b.x, (a.n[1] + b.y - 1) / b.y, (a.n[2] + b.z - 1) / b.z);
   if (!launch_shape(a, b, g)) return hipErrorInvalidConfiguration;  // beyond te_block()'s exact range
   void* args[] = {(void*)&c->dM, (void*)&a};
-  return hipLaunchKernel(kKernels[op.kid].fn, g, b, args, lds, c->stream);
+  return hipLaunchKernel(op.fn ? op.fn : kKernels[op.kid].fn, g, b, args, lds, c->stream);
 }
 
 // region_hash (replay.js:408-418): per-lane sums of a 32-bit mix, reduced
@@ -419,6 +419,12 @@ static int grow_guards(te_ctx* c) {
     if (te::wave_sweep(op.kid) && (uint64_t)sJ * sK >= (1ull << 30))
       return fail(c, TE_ERR_UNSUPPORTED, std::string(te::kernel_name(op.kid)) + ": a level spans 2^32 bytes or more");
     need = std::max<uint64_t>(need, (uint64_t)sJ * sK * (uint64_t)(nz + 3));
+    if (std::strcmp(kn, "adv_fused") == 0) {
+      int32_t stag = 0, kind = 0;
+      TE_HIP(hipMemcpy(&stag, c->dM + op.A[4 + 1], 4, hipMemcpyDeviceToHost), "read stagger");
+      TE_HIP(hipMemcpy(&kind, c->dM + op.A[4 + 3], 4, hipMemcpyDeviceToHost), "read flux kind");
+      op.fn = te::adv_fused_kernel(kind, stag);
+    }
     if (std::strcmp(te::kernel_name(op.kid), "turb_sfcstress") == 0) {
       int32_t mode = 0;
       TE_HIP(hipMemcpy(&mode, c->dM + op.A[4 + 5], 4, hipMemcpyDeviceToHost), "read mode");
