@@ -42,15 +42,32 @@ TE_DEV Up5 up5(const Win7& W, bool up) {
 TE_DEV float flx5(const Up5& o) { return flx5(o.s1, o.s2, o.s3, o.s4, o.s5); }
 TE_DEV float weno5(const Up5& o, float weps) { return weno5(o.s1, o.s2, o.s3, o.s4, o.s5, weps); }
 
+// A face value in two parts: fp_* issues every load the face may need (its
+// window and velocity or mass-flux operands; per-level profiles at the level
+// clamped into [0, nz]) and fe_* decides and computes from them.  fv_* is
+// fe_*(fp_*(...)); the fused kernel issues all six faces' loads before any
+// of their branches, so a wave waits for memory once.
+struct FacePre {
+  Win7 W;
+  float a, b, c;
+};
+TE_DEV int lev(const Dims& g, int k) { return k < 0 ? 0 : (k > g.nz ? g.nz : k); }
+
 // adv_flux_x.wgsl:120-143 / adv_flux_y.wgsl:120-144
 template <int DIR>
-TE_DEV bool fv_h5(const Dims& g, const float* s, const float* rr, int i, int j, int k, int64_t c, float& out) {
+TE_DEV FacePre fp_h5(const Dims& g, const float* s, const float* rr, int64_t c) {
+  return {hwin(s, c, DIR == 0 ? 1 : (int64_t)g.sJ), rr[c], 0.0f, 0.0f};
+}
+template <int DIR>
+TE_DEV bool fe_h5(const Dims& g, const FacePre& p, int i, int j, int k, float& out) {
   if (DIR == 0 ? (i > g.nx || j >= g.ny || k >= g.nz) : (i >= g.nx || j > g.ny || k >= g.nz)) return false;
-  const int64_t st = DIR == 0 ? 1 : (int64_t)g.sJ;
-  const float u = rr[c];
-  const Win7 W = hwin(s, c, st);
-  out = u * flx5(up5<0>(W, u >= 0.0f));
+  const float u = p.a;
+  out = u * flx5(up5<0>(p.W, u >= 0.0f));
   return true;
+}
+template <int DIR>
+TE_DEV bool fv_h5(const Dims& g, const float* s, const float* rr, int i, int j, int k, int64_t c, float& out) {
+  return fe_h5<DIR>(g, fp_h5<DIR>(g, s, rr, c), i, j, k, out);
 }
 template <int DIR>
 __device__ void flux_h5(uint32_t* M, const KArg& a, int i, int j, int k) {
@@ -62,13 +79,15 @@ __global__ __launch_bounds__(256) void k_adv_flux_x(uint32_t* M, KArg a) { TE_TI
 __global__ __launch_bounds__(256) void k_adv_flux_y(uint32_t* M, KArg a) { TE_TID3 flux_h5<1>(M, a, gx, gy, gz); }
 
 // adv_flux_z.wgsl:127-200
-TE_DEV bool fv_z5(const Dims& g, const float* s, const float* rrw, const float* c1, const float* c2, int i, int j, int k, int64_t c,
-                  float& out) {
+TE_DEV FacePre fp_z5(const Dims& g, const float* s, const float* rrw, const float* c1, const float* c2, int k, int64_t c) {
+  const int l = lev(g, k);
+  return {vwin(s, c, g.sk()), rrw[c], c1[l], c2[l]};
+}
+TE_DEV bool fe_z5(const Dims& g, const FacePre& p, int i, int j, int k, float& out) {
   if (i >= g.nx || j >= g.ny || k > g.nz) return false;
-  const int64_t sk = g.sk();
   if (k == 0 || k == g.nz) { out = 0.0f; return true; }
-  const float w = rrw[c], cc1 = c1[k], cc2 = c2[k];
-  const Win7 W = vwin(s, c, sk);
+  const float w = p.a, cc1 = p.b, cc2 = p.c;
+  const Win7& W = p.W;
   const bool up = w >= 0.0f;
   float f;
   if (k == 1) f = up ? fma_(cc1, W(-1), cc2 * W(0)) : flx3(W(1), W(0), W(-1));
@@ -78,6 +97,10 @@ TE_DEV bool fv_z5(const Dims& g, const float* s, const float* rrw, const float* 
   else f = flx5(up5<0>(W, up));
   out = w * f;
   return true;
+}
+TE_DEV bool fv_z5(const Dims& g, const float* s, const float* rrw, const float* c1, const float* c2, int i, int j, int k, int64_t c,
+                  float& out) {
+  return fe_z5(g, fp_z5(g, s, rrw, c1, c2, k, c), i, j, k, out);
 }
 __global__ __launch_bounds__(256) void k_adv_flux_z(uint32_t* M, KArg a) {
   TE_TID3
@@ -89,15 +112,19 @@ __global__ __launch_bounds__(256) void k_adv_flux_z(uint32_t* M, KArg a) {
 
 // adv_flux_x_weno.wgsl:158-200 / adv_flux_y_weno.wgsl:158-201
 template <int DIR>
-TE_DEV bool fv_h_weno(const Dims& g, const float* s, const float* rr, float weps, bool pdef, int i, int j, int k, int64_t c,
-                      float& out) {
+TE_DEV FacePre fp_h_weno(const Dims& g, const float* s, const float* rr, int64_t c) { return fp_h5<DIR>(g, s, rr, c); }
+template <int DIR>
+TE_DEV bool fe_h_weno(const Dims& g, const FacePre& p, float weps, bool pdef, int i, int j, int k, float& out) {
   if (i > g.nx || j > g.ny || k >= g.nz) return false;
-  const int64_t st = DIR == 0 ? 1 : (int64_t)g.sJ;
-  const float u = rr[c];
-  const Win7 W = hwin(s, c, st);
-  const Up5 o = up5<0>(W, u >= 0.0f);
+  const float u = p.a;
+  const Up5 o = up5<0>(p.W, u >= 0.0f);
   out = weno_pdef_ok(o.s1, o.s2, o.s3, o.s4, o.s5, weps, pdef) ? u * weno5(o, weps) : 0.0f;
   return true;
+}
+template <int DIR>
+TE_DEV bool fv_h_weno(const Dims& g, const float* s, const float* rr, float weps, bool pdef, int i, int j, int k, int64_t c,
+                      float& out) {
+  return fe_h_weno<DIR>(g, fp_h_weno<DIR>(g, s, rr, c), weps, pdef, i, j, k, out);
 }
 template <int DIR>
 __device__ void flux_h_weno(uint32_t* M, const KArg& a, int i, int j, int k) {
@@ -110,22 +137,24 @@ __global__ __launch_bounds__(256) void k_adv_flux_x_weno(uint32_t* M, KArg a) { 
 __global__ __launch_bounds__(256) void k_adv_flux_y_weno(uint32_t* M, KArg a) { TE_TID3 flux_h_weno<1>(M, a, gx, gy, gz); }
 
 // adv_flux_z_weno.wgsl:174-279 (face level k = kf - 1)
-TE_DEV bool fv_z_weno(const Dims& g, const float* s, const float* rrw, const float* c1, const float* c2, float weps,
-                      int pdefweno, int i, int j, int k, int64_t c, float& out) {
+TE_DEV FacePre fp_z_weno(const Dims& g, const float* s, const float* rrw, const float* c1, const float* c2, int k,
+                         int64_t c) {
+  return fp_z5(g, s, rrw, c1, c2, k, c);
+}
+TE_DEV bool fe_z_weno(const Dims& g, const FacePre& p, float weps, int pdefweno, int i, int j, int k, float& out) {
   const int kf = k + 1;
   if (i >= g.nx || j >= g.ny || k < 0 || kf > g.nz + 1) return false;
-  const int64_t sk = g.sk();
   if (kf == 1 || kf == g.nz + 1) { out = 0.0f; return true; }
-  const Win7 W = vwin(s, c, sk);
+  const Win7& W = p.W;
 #define L(n) W(n)
-  const float w = rrw[c];
+  const float w = p.a, ck1 = p.b, ck2 = p.c;
   const int nk = g.nz;
   if (kf == 2) {
-    out = (w >= 0.0f) ? w * fma_(c1[k], L(-1), c2[k] * L(0)) : w * upstrpd(L(1), L(0), L(-1), weps);
+    out = (w >= 0.0f) ? w * fma_(ck1, L(-1), ck2 * L(0)) : w * upstrpd(L(1), L(0), L(-1), weps);
     return true;
   }
   if (kf == nk) {
-    out = (w > 0.0f) ? w * upstrpd(L(-2), L(-1), L(0), weps) : w * fma_(c1[k], L(-1), c2[k] * L(0));
+    out = (w > 0.0f) ? w * upstrpd(L(-2), L(-1), L(0), weps) : w * fma_(ck1, L(-1), ck2 * L(0));
     return true;
   }
   const Up5 o = up5<0>(W, w >= 0.0f);
@@ -149,6 +178,10 @@ TE_DEV bool fv_z_weno(const Dims& g, const float* s, const float* rrw, const flo
   return true;
 #undef L
 }
+TE_DEV bool fv_z_weno(const Dims& g, const float* s, const float* rrw, const float* c1, const float* c2, float weps,
+                      int pdefweno, int i, int j, int k, int64_t c, float& out) {
+  return fe_z_weno(g, fp_z_weno(g, s, rrw, c1, c2, k, c), weps, pdefweno, i, j, k, out);
+}
 __global__ __launch_bounds__(256) void k_adv_flux_z_weno(uint32_t* M, KArg a) {
   TE_TID3
   Dims g = dims(a.q[0], 0);
@@ -168,19 +201,30 @@ TE_DEV float weno5_upwind(const float* s, int64_t c, int64_t st, bool up, bool s
 }
 
 // adv_flux_x_stag.wgsl:144-186 / adv_flux_y_stag.wgsl (rr = rru for x, rrv for y)
+// p.a, p.b: the mass flux at c and at the other point of the stagger's
+// average (stagger 4: c - sk, with p.c = c2[k]); stagger 1: p.a only
+template <int DIR>
+TE_DEV FacePre fp_h_stag(const Dims& g, int stag, const float* s, const float* rr, const float* c2, int k, int64_t c) {
+  const int64_t sj = g.sJ, sk = g.sk(), st = DIR == 0 ? 1 : sj;
+  const int64_t o = stag == 2 ? (DIR == 0 ? 1 : -1) : stag == 3 ? (DIR == 0 ? -sj : sj) : stag == 4 ? -sk : 0;
+  return {hwin(s, c, st), rr[c], rr[c + o], stag == 4 ? c2[lev(g, k)] : 0.0f};
+}
+template <int DIR>
+TE_DEV bool fe_h_stag(const Dims& g, int stag, const FacePre& p, int i, int j, int k, float& out) {
+  if (i > g.nx || j > g.ny || k >= g.nz) return false;
+  if (stag == 4 && k == 0) { out = 0.0f; return true; }
+  float m;
+  if (stag == 2 || stag == 3) m = 0.5f * (p.a + p.b);
+  else if (stag == 4) { const float cc2 = p.c; m = fma_(cc2, p.a, (1.0f - cc2) * p.b); }
+  else m = p.a;
+  const bool up = m >= 0.0f;
+  out = m * (stag == (DIR == 0 ? 2 : 3) ? flx5(up5<1>(p.W, up)) : flx5(up5<0>(p.W, up)));
+  return true;
+}
 template <int DIR>
 TE_DEV bool fv_h_stag(const Dims& g, int stag, const float* s, const float* rr, const float* c2, int i, int j, int k, int64_t c,
                       float& out) {
-  if (i > g.nx || j > g.ny || k >= g.nz) return false;
-  const int64_t sj = g.sJ, sk = g.sk(), st = DIR == 0 ? 1 : sj;
-  if (stag == 4 && k == 0) { out = 0.0f; return true; }
-  float m;
-  if (stag == 2) m = 0.5f * (rr[c] + rr[c + (DIR == 0 ? 1 : -1)]);
-  else if (stag == 3) m = 0.5f * (rr[c] + rr[c + (DIR == 0 ? -sj : sj)]);
-  else if (stag == 4) { float cc2 = c2[k]; m = fma_(cc2, rr[c], (1.0f - cc2) * rr[c - sk]); }
-  else m = rr[c];
-  out = m * flx5_upwind(s, c, st, m >= 0.0f, stag == (DIR == 0 ? 2 : 3));
-  return true;
+  return fe_h_stag<DIR>(g, stag, fp_h_stag<DIR>(g, stag, s, rr, c2, k, c), i, j, k, out);
 }
 template <int DIR>
 __device__ void flux_h_stag(uint32_t* M, const KArg& a, int i, int j, int k) {
@@ -193,17 +237,20 @@ __global__ __launch_bounds__(256) void k_adv_flux_x_stag(uint32_t* M, KArg a) { 
 __global__ __launch_bounds__(256) void k_adv_flux_y_stag(uint32_t* M, KArg a) { TE_TID3 flux_h_stag<1>(M, a, gx - 1, gy - 1, gz); }
 
 // adv_flux_z_stag.wgsl:127-268
-TE_DEV bool fv_z_stag(const Dims& g, int stag, const float* s, const float* rrw, const float* c1, const float* c2, int i,
-                      int j, int k, int64_t c, float& out) {
-  if (i > g.nx || j > g.ny || k > g.nz) return false;
+// p.a = m (the mass flux average), p.b, p.c = c1[k], c2[k]
+TE_DEV FacePre fp_z_stag(const Dims& g, int stag, const float* s, const float* rrw, const float* c1, const float* c2, int k,
+                         int64_t c) {
   const int64_t sj = g.sJ, sk = g.sk();
-  float m;
-  if (stag == 2) m = 0.5f * (rrw[c] + rrw[c - 1]);
-  else if (stag == 3) m = 0.5f * (rrw[c] + rrw[c - sj]);
-  else m = 0.5f * (rrw[c] + rrw[c + sk]);
+  const int64_t o = stag == 2 ? -1 : stag == 3 ? -sj : sk;
+  const int l = lev(g, k);
+  return {vwin(s, c, sk), 0.5f * (rrw[c] + rrw[c + o]), c1[l], c2[l]};
+}
+TE_DEV bool fe_z_stag(const Dims& g, int stag, const FacePre& p, int i, int j, int k, float& out) {
+  if (i > g.nx || j > g.ny || k > g.nz) return false;
+  const float m = p.a;
   const bool up = m >= 0.0f;
-  const float cc1 = c1[k], cc2 = c2[k];
-  const Win7 W = vwin(s, c, sk);
+  const float cc1 = p.b, cc2 = p.c;
+  const Win7& W = p.W;
 #define A_(n) W(n)
   float f;
   if (stag == 4) {
@@ -225,6 +272,10 @@ TE_DEV bool fv_z_stag(const Dims& g, int stag, const float* s, const float* rrw,
   out = m * f;
   return true;
 }
+TE_DEV bool fv_z_stag(const Dims& g, int stag, const float* s, const float* rrw, const float* c1, const float* c2, int i,
+                      int j, int k, int64_t c, float& out) {
+  return fe_z_stag(g, stag, fp_z_stag(g, stag, s, rrw, c1, c2, k, c), i, j, k, out);
+}
 __global__ __launch_bounds__(256) void k_adv_flux_z_stag(uint32_t* M, KArg a) {
   TE_TID3
   Dims g = dims(a.q[0], 0);
@@ -234,17 +285,24 @@ __global__ __launch_bounds__(256) void k_adv_flux_z_stag(uint32_t* M, KArg a) {
 }
 
 // adv_flux_x_weno_w.wgsl:143-175 / adv_flux_y_weno_w.wgsl
+// p.a, p.b = rr at c and c - sk, p.c = c2[k]
+template <int DIR>
+TE_DEV FacePre fp_h_weno_w(const Dims& g, const float* s, const float* rr, const float* c2, int k, int64_t c) {
+  return {hwin(s, c, DIR == 0 ? 1 : (int64_t)g.sJ), rr[c], rr[c - g.sk()], c2[lev(g, k)]};
+}
+template <int DIR>
+TE_DEV bool fe_h_weno_w(const Dims& g, const FacePre& p, float weps, int i, int j, int k, float& out) {
+  if (i < 0 || j < 0 || i > g.nx || j > g.ny || k >= g.nz) return false;
+  if (k == 0) { out = 0.0f; return true; }
+  float cc2 = p.c, cc1 = 1.0f - cc2;
+  float ub = fma_(cc2, p.a, cc1 * p.b);
+  out = ub * weno5(up5<0>(p.W, ub >= 0.0f), weps);
+  return true;
+}
 template <int DIR>
 TE_DEV bool fv_h_weno_w(const Dims& g, const float* s, const float* rr, const float* c2, float weps, int i, int j, int k, int64_t c,
                         float& out) {
-  if (i < 0 || j < 0 || i > g.nx || j > g.ny || k >= g.nz) return false;
-  const int64_t sk = g.sk(), st = DIR == 0 ? 1 : (int64_t)g.sJ;
-  if (k == 0) { out = 0.0f; return true; }
-  float cc2 = c2[k], cc1 = 1.0f - cc2;
-  float ub = fma_(cc2, rr[c], cc1 * rr[c - sk]);
-  const Win7 W = hwin(s, c, st);
-  out = ub * weno5(up5<0>(W, ub >= 0.0f), weps);
-  return true;
+  return fe_h_weno_w<DIR>(g, fp_h_weno_w<DIR>(g, s, rr, c2, k, c), weps, i, j, k, out);
 }
 template <int DIR>
 __device__ void flux_h_weno_w(uint32_t* M, const KArg& a, int i, int j, int k) {
@@ -257,13 +315,16 @@ __global__ __launch_bounds__(256) void k_adv_flux_x_weno_w(uint32_t* M, KArg a) 
 __global__ __launch_bounds__(256) void k_adv_flux_y_weno_w(uint32_t* M, KArg a) { TE_TID3 flux_h_weno_w<1>(M, a, gx - 1, gy - 1, gz); }
 
 // adv_flux_z_weno_w.wgsl:159-231
-TE_DEV bool fv_z_weno_w(const Dims& g, const float* s, const float* rrw, float weps, int i, int j, int k, int64_t c, float& out) {
-  if (i > g.nx || j > g.ny || k > g.nz) return false;
+TE_DEV FacePre fp_z_weno_w(const Dims& g, const float* s, const float* rrw, int64_t c) {
   const int64_t sk = g.sk();
+  return {vwin(s, c, sk), 0.5f * (rrw[c] + rrw[c + sk]), 0.0f, 0.0f};
+}
+TE_DEV bool fe_z_weno_w(const Dims& g, const FacePre& p, float weps, int i, int j, int k, float& out) {
+  if (i > g.nx || j > g.ny || k > g.nz) return false;
   if (k > g.nz - 1) { out = 0.0f; return true; }
-  const Win7 W = vwin(s, c, sk);
+  const Win7& W = p.W;
 #define L(n) W(n)
-  float wbar = 0.5f * (rrw[c] + rrw[c + sk]);
+  float wbar = p.a;
   const int nz = g.nz;
   float f;
   if (k == 0) f = (wbar >= 0.0f) ? 0.5f * (L(0) + L(1)) : upstrpd(L(2), L(1), L(0), weps);
@@ -274,6 +335,9 @@ TE_DEV bool fv_z_weno_w(const Dims& g, const float* s, const float* rrw, float w
 #undef L
   out = wbar * f;
   return true;
+}
+TE_DEV bool fv_z_weno_w(const Dims& g, const float* s, const float* rrw, float weps, int i, int j, int k, int64_t c, float& out) {
+  return fe_z_weno_w(g, fp_z_weno_w(g, s, rrw, c), weps, i, j, k, out);
 }
 __global__ __launch_bounds__(256) void k_adv_flux_z_weno_w(uint32_t* M, KArg a) {
   TE_TID3
@@ -433,24 +497,37 @@ struct FusedFlux {
   int stag, pdef, pdw;
   const float *s, *rru, *rrv, *rrw, *c1, *c2;
   float weps;
-  // face flux of direction D at (i, j, k); NaN where no flux dispatch writes
+  // loads of the face of direction D at index c (level k: the face's)
   template <int D>
-  __device__ __forceinline__ float at(int i, int j, int k, int64_t c) const {
+  __device__ __forceinline__ FacePre pre(int k, int64_t c) const {
+    if (KIND == 0)
+      return D == 0 ? fp_h5<0>(g, s, rru, c) : D == 1 ? fp_h5<1>(g, s, rrv, c) : fp_z5(g, s, rrw, c1, c2, k, c);
+    if (KIND == 1)
+      return D == 0 ? fp_h_weno<0>(g, s, rru, c)
+                    : D == 1 ? fp_h_weno<1>(g, s, rrv, c) : fp_z_weno(g, s, rrw, c1, c2, k, c);
+    if (KIND == 2)
+      return D == 0 ? fp_h_stag<0>(g, stag, s, rru, c2, k, c)
+                    : D == 1 ? fp_h_stag<1>(g, stag, s, rrv, c2, k, c) : fp_z_stag(g, stag, s, rrw, c1, c2, k, c);
+    return D == 0 ? fp_h_weno_w<0>(g, s, rru, c2, k, c)
+                  : D == 1 ? fp_h_weno_w<1>(g, s, rrv, c2, k, c) : fp_z_weno_w(g, s, rrw, c);
+  }
+  // face flux of direction D at (i, j, k) from its loads; NaN where no flux
+  // dispatch writes
+  template <int D>
+  __device__ __forceinline__ float eval(const FacePre& p, int i, int j, int k) const {
     float f = __builtin_nanf("");
     bool ok;
     if (KIND == 0)
-      ok = D == 0 ? fv_h5<0>(g, s, rru, i, j, k, c, f) : D == 1 ? fv_h5<1>(g, s, rrv, i, j, k, c, f)
-                                                             : fv_z5(g, s, rrw, c1, c2, i, j, k, c, f);
+      ok = D == 0 ? fe_h5<0>(g, p, i, j, k, f) : D == 1 ? fe_h5<1>(g, p, i, j, k, f) : fe_z5(g, p, i, j, k, f);
     else if (KIND == 1)
-      ok = D == 0 ? fv_h_weno<0>(g, s, rru, weps, pdef == 1, i, j, k, c, f)
-                  : D == 1 ? fv_h_weno<1>(g, s, rrv, weps, pdef == 1, i, j, k, c, f)
-                           : fv_z_weno(g, s, rrw, c1, c2, weps, pdw, i, j, k, c, f);
+      ok = D == 0 ? fe_h_weno<0>(g, p, weps, pdef == 1, i, j, k, f)
+                  : D == 1 ? fe_h_weno<1>(g, p, weps, pdef == 1, i, j, k, f) : fe_z_weno(g, p, weps, pdw, i, j, k, f);
     else if (KIND == 2)
-      ok = D == 0 ? fv_h_stag<0>(g, stag, s, rru, c2, i, j, k, c, f)
-                  : D == 1 ? fv_h_stag<1>(g, stag, s, rrv, c2, i, j, k, c, f) : fv_z_stag(g, stag, s, rrw, c1, c2, i, j, k, c, f);
+      ok = D == 0 ? fe_h_stag<0>(g, stag, p, i, j, k, f)
+                  : D == 1 ? fe_h_stag<1>(g, stag, p, i, j, k, f) : fe_z_stag(g, stag, p, i, j, k, f);
     else
-      ok = D == 0 ? fv_h_weno_w<0>(g, s, rru, c2, weps, i, j, k, c, f)
-                  : D == 1 ? fv_h_weno_w<1>(g, s, rrv, c2, weps, i, j, k, c, f) : fv_z_weno_w(g, s, rrw, weps, i, j, k, c, f);
+      ok = D == 0 ? fe_h_weno_w<0>(g, p, weps, i, j, k, f)
+                  : D == 1 ? fe_h_weno_w<1>(g, p, weps, i, j, k, f) : fe_z_weno_w(g, p, weps, i, j, k, f);
     return ok ? f : __builtin_nanf("");  // a face no flux dispatch writes: poisons the result visibly
   }
 };
@@ -474,9 +551,13 @@ __device__ __forceinline__ void adv_fused_body(uint32_t* M, const KArg& a) {
   const int64_t c = g.idx(i, j, k), sj = g.sJ, sk = g.sk();
   const int xlo = st == 2 ? i - 1 : i, ylo = st == 3 ? j - 1 : j, zlo = st == 4 ? k - 1 : k;
   const int64_t cx = st == 2 ? c - 1 : c, cy = st == 3 ? c - sj : c, cz = st == 4 ? c - sk : c;
-  const float fxl = F.template at<0>(xlo, j, k, cx), fxh = F.template at<0>(xlo + 1, j, k, cx + 1);
-  const float fyl = F.template at<1>(i, ylo, k, cy), fyh = F.template at<1>(i, ylo + 1, k, cy + sj);
-  const float fzl = F.template at<2>(i, j, zlo, cz), fzh = F.template at<2>(i, j, zlo + 1, cz + sk);
+  // every face's loads first, then the faces (one wait for memory)
+  const FacePre pxl = F.template pre<0>(k, cx), pxh = F.template pre<0>(k, cx + 1);
+  const FacePre pyl = F.template pre<1>(k, cy), pyh = F.template pre<1>(k, cy + sj);
+  const FacePre pzl = F.template pre<2>(zlo, cz), pzh = F.template pre<2>(zlo + 1, cz + sk);
+  const float fxl = F.template eval<0>(pxl, xlo, j, k), fxh = F.template eval<0>(pxh, xlo + 1, j, k);
+  const float fyl = F.template eval<1>(pyl, i, ylo, k), fyh = F.template eval<1>(pyh, i, ylo + 1, k);
+  const float fzl = F.template eval<2>(pzl, i, j, zlo), fzh = F.template eval<2>(pzh, i, j, zlo + 1);
   // p[32] -> chain flags: the pointwise dispatches around a field's assembly
   // in the reference, applied to the tendency in registers:
   // 16 turb_zero of the accumulating sink before it (tend = 0 + t),
