@@ -392,9 +392,22 @@ __global__ __launch_bounds__(256) void k_adv_flux_z_weno_uv(uint32_t* M, KArg a)
 // by the reference density; open-boundary inflow corrections included.
 // STC != 0: the stagger is known at compile time (specialised launches).
 // c: index of (i, j, k).
+// The field at the cell and the two mass-divergence values the assembly
+// averages (divx at c and at the stagger's other point), loaded ahead of the
+// assembly's branches.
+struct AsmPre {
+  float s, d0, d1;
+};
+template <int STC = 0>
+TE_DEV AsmPre asm_pre(const KArg& a, const Dims& g, int64_t c) {
+  const int stag = STC != 0 ? STC : ldi(a.q[1], 0), st = stag < 0 ? -stag : stag;
+  const float *s = fq(a.q[2]), *divx = fq(a.q[6]);
+  const int64_t nb = st == 2 ? c - 1 : st == 3 ? c - g.sJ : st == 4 ? c - g.sk() : c;
+  return {s[c], divx[c], divx[nb]};
+}
 template <int STC = 0, class FLUX, class STORE>
-TE_DEV void assemble_at(uint32_t* M, const KArg& a, const Dims& g, int i, int j, int k, int64_t c, FLUX F,
-                        STORE store) {
+TE_DEV void assemble_at(uint32_t* M, const KArg& a, const Dims& g, int i, int j, int k, int64_t c, const AsmPre& P,
+                        FLUX F, STORE store) {
   const int stag = STC != 0 ? STC : ldi(a.q[1], 0);
   int st = stag < 0 ? -stag : stag;
   int iMax = st == 2 ? g.nx : g.nx - 1, jMax = st == 3 ? g.ny : g.ny - 1, kLo = st == 4 ? 1 : 0, kMax = g.nz - 1;
@@ -402,8 +415,7 @@ TE_DEV void assemble_at(uint32_t* M, const KArg& a, const Dims& g, int i, int j,
   int openx = ldi(a.q[17], 0), openy = ldi(a.q[17], 1);
   if (openx != 0 && st == 2 && (i == 0 || i == g.nx)) return;
   if (openy != 0 && st == 3 && (j == 0 || j == g.ny)) return;
-  const float *s = fq(a.q[2]), *divx = fq(a.q[6]),
-              *rho0 = fq(a.q[7]), *rf0 = fq(a.q[8]), *mh = fq(a.q[9]), *mf = fq(a.q[10]),
+  const float *rho0 = fq(a.q[7]), *rf0 = fq(a.q[8]), *mh = fq(a.q[9]), *mf = fq(a.q[10]),
               *rru = fq(a.q[13]), *rrv = fq(a.q[14]), *c1 = fq(a.q[15]), *c2 = fq(a.q[16]),
               *hm = fq(a.q[30]);
   float* tend = fq(a.q[11]);
@@ -413,11 +425,10 @@ TE_DEV void assemble_at(uint32_t* M, const KArg& a, const Dims& g, int i, int j,
   float dFy = st == 3 ? F(1, i, j, k) - F(1, i, j - 1, k) : F(1, i, j + 1, k) - F(1, i, j, k);
   float dFz = st == 4 ? F(2, i, j, k) - F(2, i, j, k - 1) : F(2, i, j, k + 1) - F(2, i, j, k);
   float mz = stag < 0 ? 1.0f : (stag == 4 ? mf[k] : mh[k]);
-  float dv;
-  if (st == 2) dv = 0.5f * (divx[c] + divx[c - 1]);
-  else if (st == 3) dv = 0.5f * (divx[c] + divx[c - sj]);
-  else if (st == 4) dv = fma_(c2[k], divx[c], c1[k] * divx[c - sk]);
-  else dv = divx[c];
+  float dv;  // divx at c and at c - 1, c - sj or c - sk
+  if (st == 2 || st == 3) dv = 0.5f * (P.d0 + P.d1);
+  else if (st == 4) dv = fma_(c2[k], P.d0, c1[k] * P.d1);
+  else dv = P.d0;
   if (openx != 0 && st != 2) {
     bool west = i == 0, east = i == g.nx - 1;
     if (west || east) {
@@ -462,7 +473,7 @@ TE_DEV void assemble_at(uint32_t* M, const KArg& a, const Dims& g, int i, int j,
   float mym = st == 3 ? hm_vf(hm, g, j) : hm_vh(hm, g, j);
   float divF = fma_(dFz * g.rdz, mz, fma_(dFx * g.rdx, mxm, (dFy * g.rdy) * mym));
   float rho = st == 4 ? rf0[k] : rho0[k];
-  float t = fma_(s[c], dv, -divF) / rho;
+  float t = fma_(P.s, dv, -divF) / rho;
   store(c, t, accumulate);
 }
 
@@ -471,8 +482,9 @@ __global__ __launch_bounds__(256) void k_adv_assemble(uint32_t* M, KArg a) {
   Dims g = dims(a.q[0], 0);
   const float *fx = fq(a.q[3]), *fy = fq(a.q[4]), *fz = fq(a.q[5]);
   float* tend = fq(a.q[11]);
+  const int64_t c0 = g.idx(gx, gy, gz);
   assemble_at(
-      M, a, g, gx, gy, gz, g.idx(gx, gy, gz),
+      M, a, g, gx, gy, gz, c0, asm_pre(a, g, c0),
       [&](int d, int i, int j, int k) {
         const int64_t c = g.idx(i, j, k);
         return d == 0 ? fx[c] : (d == 1 ? fy[c] : fz[c]);
@@ -551,10 +563,18 @@ __device__ __forceinline__ void adv_fused_body(uint32_t* M, const KArg& a) {
   const int64_t c = g.idx(i, j, k), sj = g.sJ, sk = g.sk();
   const int xlo = st == 2 ? i - 1 : i, ylo = st == 3 ? j - 1 : j, zlo = st == 4 ? k - 1 : k;
   const int64_t cx = st == 2 ? c - 1 : c, cy = st == 3 ? c - sj : c, cz = st == 4 ? c - sk : c;
-  // every face's loads first, then the faces (one wait for memory)
+  // every face's loads first, with the assembly's and the chain's operands
+  // (below), then the faces (one wait for memory)
   const FacePre pxl = F.template pre<0>(k, cx), pxh = F.template pre<0>(k, cx + 1);
   const FacePre pyl = F.template pre<1>(k, cy), pyh = F.template pre<1>(k, cy + sj);
   const FacePre pzl = F.template pre<2>(zlo, cz), pzh = F.template pre<2>(zlo + 1, cz + sk);
+  const AsmPre P = asm_pre<STC>(a, g, c);
+  const int chain = ldi(a.q[32], 0), acc = ldi(a.q[12], 0);
+  float* tend = fq(a.q[11]);
+  const float t_old = (chain & 16) == 0 && acc != 0 ? tend[c] : 0.0f;
+  const float d3 = (chain & 1) ? fq(a.q[33])[c] : 0.0f, th3 = (chain & 1) ? fq(a.q[34])[c] : 0.0f;
+  const float add1 = (chain & 2) ? fq(a.q[35])[c] : 0.0f, add2 = (chain & 4) ? fq(a.q[37])[c] : 0.0f;
+  const float phi_a = (chain & 8) ? fq(a.q[39])[c] : 0.0f;
   const float fxl = F.template eval<0>(pxl, xlo, j, k), fxh = F.template eval<0>(pxh, xlo + 1, j, k);
   const float fyl = F.template eval<1>(pyl, i, ylo, k), fyh = F.template eval<1>(pyh, i, ylo + 1, k);
   const float fzl = F.template eval<2>(pzl, i, j, zlo), fzh = F.template eval<2>(pzh, i, j, zlo + 1);
@@ -566,22 +586,20 @@ __device__ __forceinline__ void adv_fused_body(uint32_t* M, const KArg& a) {
   // 8 rk_update (phi = fma(dttmp, tend, phi_a); p39 phi_a, p40 phi, p41 dttmp;
   // phi must not be the advected field, which other workgroups still read).
   // With an update in the chain the tendency itself is not stored.
-  const int chain = ldi(a.q[32], 0);
-  float* tend = fq(a.q[11]);
   assemble_at<STC>(
-      M, a, g, i, j, k, c,
+      M, a, g, i, j, k, c, P,
       [&](int d, int fi, int fj, int fk) {
         if (d == 0) return fi == xlo ? fxl : fxh;
         if (d == 1) return fj == ylo ? fyl : fyh;
         return fk == zlo ? fzl : fzh;
       },
       [&](int64_t c, float t, int acc) {
-        float v = (chain & 16) ? 0.0f + t : (acc != 0 ? tend[c] + t : t);
-        if (chain & 1) v = fma_(-fq(a.q[33])[c], fq(a.q[34])[c], v);
-        if (chain & 2) v = fma_(ldf(a.q[36], 0), fq(a.q[35])[c], v);
-        if (chain & 4) v = fma_(ldf(a.q[38], 0), fq(a.q[37])[c], v);
+        float v = (chain & 16) ? 0.0f + t : (acc != 0 ? t_old + t : t);
+        if (chain & 1) v = fma_(-d3, th3, v);
+        if (chain & 2) v = fma_(ldf(a.q[36], 0), add1, v);
+        if (chain & 4) v = fma_(ldf(a.q[38], 0), add2, v);
         if (chain & 8) {
-          fq(a.q[40])[c] = fma_(ldf(a.q[41], 0), v, fq(a.q[39])[c]);
+          fq(a.q[40])[c] = fma_(ldf(a.q[41], 0), v, phi_a);
         } else {
           tend[c] = v;
         }
