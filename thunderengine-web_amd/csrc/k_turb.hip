@@ -24,21 +24,23 @@ __global__ __launch_bounds__(256) void k_rayleigh_damping(uint32_t* M, KArg a) {
   float rdalpha = ldf(a.q[9], 0);
   int lateral = ldi(a.q[16], 0), toAverage = ldi(a.q[17], 0);
   const int64_t c = g.idx(i, j, k);
+  // the four fields loaded before the first store (the loads go out together)
+  const float uc = u[c], vc = v[c], thc = th[c], wc = w[c];
   if (k < g.nz) {
     float t = tauh[k];
     float ur = toAverage != 0 ? avg[k] : u0[k];
     float vr = toAverage != 0 ? avg[g.nz + k] : v0[k];
-    if (j < g.ny) uten[c] = ((-rdalpha) * t) * (u[c] - ur);
-    if (i < g.nx) vten[c] = ((-rdalpha) * t) * (v[c] - vr);
+    if (j < g.ny) uten[c] = ((-rdalpha) * t) * (uc - ur);
+    if (i < g.nx) vten[c] = ((-rdalpha) * t) * (vc - vr);
     if (i < g.nx && j < g.ny) {
-      float d = toAverage != 0 ? (th0[k] - avg[2 * g.nz + k]) + th[c] : th[c];
+      float d = toAverage != 0 ? (th0[k] - avg[2 * g.nz + k]) + thc : thc;
       thten[c] = ((-rdalpha) * t) * d;
     }
   }
   if (i < g.nx && j < g.ny) {
     float tf = tauf[k];
     if (lateral != 0 && k < g.nz) tf = mx(mx(tf, hdx[i]), hdy[j]);
-    wten[c] = ((-rdalpha) * tf) * w[c];
+    wten[c] = ((-rdalpha) * tf) * wc;
   }
 }
 
@@ -75,12 +77,18 @@ __global__ __launch_bounds__(256) void k_turb_def_centers(uint32_t* M, KArg a) {
   const float *u = fq(a.q[1]), *v = fq(a.q[2]), *w = fq(a.q[3]), *rho = fq(a.q[4]), *mh = fq(a.q[5]),
               *hm = fq(a.q[30]);
   const int64_t sj = g.sJ, sk = g.sk(), c = g.idx(i, j, k);
-  fq(a.q[6])[c] = ((rho[c] * (u[c + 1] - u[c])) * g.rdx) * hm_uh(hm, g, i);
-  fq(a.q[7])[c] = ((rho[c] * (v[c + sj] - v[c])) * g.rdy) * hm_vh(hm, g, j);
-  fq(a.q[8])[c] = ((rho[c] * (w[c + sk] - w[c])) * g.rdz) * mh[k];
-  float rcorner = 0.25f * ((rho[c - 1 - sj] + rho[c]) + (rho[c - 1] + rho[c - sj]));
-  fq(a.q[9])[c] =
+  // every value formed before the first store (the loads go out together)
+  const float r0 = rho[c];
+  const float s11 = ((r0 * (u[c + 1] - u[c])) * g.rdx) * hm_uh(hm, g, i);
+  const float s22 = ((r0 * (v[c + sj] - v[c])) * g.rdy) * hm_vh(hm, g, j);
+  const float s33 = ((r0 * (w[c + sk] - w[c])) * g.rdz) * mh[k];
+  const float rcorner = 0.25f * ((rho[c - 1 - sj] + r0) + (rho[c - 1] + rho[c - sj]));
+  const float s12 =
       (0.5f * fma_((u[c] - u[c - sj]) * g.rdy, hm_vf(hm, g, j), ((v[c] - v[c - 1]) * g.rdx) * hm_uf(hm, g, i))) * rcorner;
+  fq(a.q[6])[c] = s11;
+  fq(a.q[7])[c] = s22;
+  fq(a.q[8])[c] = s33;
+  fq(a.q[9])[c] = s12;
 }
 
 // turb_def_faces.wgsl:125-151
@@ -94,10 +102,13 @@ __global__ __launch_bounds__(256) void k_turb_def_faces(uint32_t* M, KArg a) {
   const int64_t sj = g.sJ, sk = g.sk(), c = g.idx(i, j, k);
   float mfu = 0.5f * (mf[k] + mf[k]);
   float mfv = 0.5f * (mf[k] + mf[k]);
-  fq(a.q[6])[c] =
+  // both values formed before the first store
+  const float s13 =
       ((0.5f * fma_((w[c] - w[c - 1]) * g.rdx, hm_uf(hm, g, i), ((u[c] - u[c - sk]) * g.rdz) * mfu)) * 0.5f) * (rf[c - 1] + rf[c]);
-  fq(a.q[7])[c] =
+  const float s23 =
       ((0.5f * fma_((w[c] - w[c - sj]) * g.rdy, hm_vf(hm, g, j), ((v[c] - v[c - sk]) * g.rdz) * mfv)) * 0.5f) * (rf[c - sj] + rf[c]);
+  fq(a.q[6])[c] = s13;
+  fq(a.q[7])[c] = s23;
 }
 
 // turb_def_assemble.wgsl:111-149
@@ -308,16 +319,20 @@ __global__ __launch_bounds__(256) void k_turb_gettau(uint32_t* M, KArg a) {
   float *t11 = fq(a.q[3]), *t22 = fq(a.q[4]), *t33 = fq(a.q[5]), *t12 = fq(a.q[6]), *t13 = fq(a.q[7]),
         *t23 = fq(a.q[8]);
   const int64_t sj = g.sJ, sk = g.sk(), c = g.idx(i, j, k);
-  float tem = kmh[c] + kmh[c + sk];
-  t11[c] = t11[c] * tem;
-  t22[c] = t22[c] * tem;
-  t33[c] = t33[c] * tem;
-  float q = ((kmh[c - 1 - sj] + kmh[c]) + (kmh[c - 1] + kmh[c - sj])) +
-            ((kmh[c - 1 - sj + sk] + kmh[c + sk]) + (kmh[c - 1 + sk] + kmh[c - sj + sk]));
-  t12[c] = t12[c] * (0.25f * q);
+  // every operand loaded before the first store (the loads go out together;
+  // t13 / t23 and kmv at the column's first level are read but not used)
+  const float tem = kmh[c] + kmh[c + sk];
+  const float q = ((kmh[c - 1 - sj] + kmh[c]) + (kmh[c - 1] + kmh[c - sj])) +
+                  ((kmh[c - 1 - sj + sk] + kmh[c + sk]) + (kmh[c - 1 + sk] + kmh[c - sj + sk]));
+  const float a11 = t11[c], a22 = t22[c], a33 = t33[c], a12 = t12[c], a13 = t13[c], a23 = t23[c];
+  const float kv = kmv[c], kvx = kmv[c - 1], kvy = kmv[c - sj];
+  t11[c] = a11 * tem;
+  t22[c] = a22 * tem;
+  t33[c] = a33 * tem;
+  t12[c] = a12 * (0.25f * q);
   if (k >= 1 && k <= g.nz - 1 && i >= 0 && j >= 0) {
-    t13[c] = t13[c] * (kmv[c - 1] + kmv[c]);
-    t23[c] = t23[c] * (kmv[c - sj] + kmv[c]);
+    t13[c] = a13 * (kvx + kv);
+    t23[c] = a23 * (kvy + kv);
   }
 }
 
