@@ -317,38 +317,41 @@ __global__ __launch_bounds__(256) void k_pdef_apply(uint32_t* M, KArg a) {
   float base = rho0[k] * s0[c];
   float delta = 0.0f;
   bool flag;
+  // the three directions' dum windows and the tendency loaded together
+  // (what a flagged direction reads besides comes after)
+  const float x0 = dx[c - 2], x1 = dx[c - 1], x2 = dx[c], x3 = dx[c + 1], x4 = dx[c + 2];
+  const float y0 = dy[c - 2 * sj], y1 = dy[c - sj], y2 = dy[c], y3 = dy[c + sj], y4 = dy[c + 2 * sj];
+  const float zm2 = k >= 2 ? dz[c - 2 * sk] : 0.0f, zm1 = k >= 1 ? dz[c - sk] : 0.0f, z0 = dz[c];
+  const float zp1 = k + 1 < g.nz ? dz[c + sk] : 0.0f, zp2 = k + 2 < g.nz ? dz[c + 2 * sk] : 0.0f;
+  const float t0 = tend[c];
   {
-    float ms = pdef_mass(dx[c - 2], dx[c - 1], dx[c], dx[c + 1], dx[c + 2], flag);
+    float ms = pdef_mass(x0, x1, x2, x3, x4, flag);
     if (flag) {
       float sdiv = ((s[c] * (rru[c + 1] - rru[c])) * g.rdx) * hm_uh(hm, g, i);
       float advOld = pdef_inflow(g, 0, i, j, c, 1, rru, openx, openy) ? -sdiv : ((-(fx[c + 1] - fx[c])) * g.rdx) * hm_uh(hm, g, i);
-      float advNew = fma_((dx[c] + ms) - base, rdt, -sdiv);
+      float advNew = fma_((x2 + ms) - base, rdt, -sdiv);
       delta = delta + (advNew - advOld);
     }
   }
   {
-    float ms = pdef_mass(dy[c - 2 * sj], dy[c - sj], dy[c], dy[c + sj], dy[c + 2 * sj], flag);
+    float ms = pdef_mass(y0, y1, y2, y3, y4, flag);
     if (flag) {
       float sdiv = ((s[c] * (rrv[c + sj] - rrv[c])) * g.rdy) * hm_vh(hm, g, j);
       float advOld = pdef_inflow(g, 1, i, j, c, sj, rrv, openx, openy) ? -sdiv : ((-(fy[c + sj] - fy[c])) * g.rdy) * hm_vh(hm, g, j);
-      float advNew = fma_((dy[c] + ms) - base, rdt, -sdiv);
+      float advNew = fma_((y2 + ms) - base, rdt, -sdiv);
       delta = delta + (advNew - advOld);
     }
   }
   {
-    float zm2 = k >= 2 ? dz[c - 2 * sk] : 0.0f;
-    float zm1 = k >= 1 ? dz[c - sk] : 0.0f;
-    float zp1 = k + 1 < g.nz ? dz[c + sk] : 0.0f;
-    float zp2 = k + 2 < g.nz ? dz[c + 2 * sk] : 0.0f;
-    float ms = pdef_mass(zm2, zm1, dz[c], zp1, zp2, flag);
+    float ms = pdef_mass(zm2, zm1, z0, zp1, zp2, flag);
     if (flag) {
       float sdiv = ((s[c] * (rrw[c + sk] - rrw[c])) * g.rdz) * mh[k];
       float advOld = ((-(fz[c + sk] - fz[c])) * g.rdz) * mh[k];
-      float advNew = fma_((dz[c] + ms) - base, rdt, -sdiv);
+      float advNew = fma_((z0 + ms) - base, rdt, -sdiv);
       delta = delta + (advNew - advOld);
     }
   }
-  tend[c] = tend[c] + delta / rho0[k];
+  tend[c] = t0 + delta / rho0[k];
 }
 
 }  // namespace te

@@ -54,18 +54,19 @@ __global__ __launch_bounds__(256) void k_turb_rho_faces(uint32_t* M, KArg a) {
   float *rr = fq(a.q[4]), *rf = fq(a.q[5]);
   const uint32_t *os = a.q[6], *ot = a.q[7];
   const int64_t sk = g.sk(), c = g.idx(i, j, k);
-  if (k < g.nz) rr[c] = 1.0f / rho[c];
+  // rf formed before either store (the loads go out together)
+  float r;
   if (k == 0) {
-    float r = fma_(ldf(os, 0), rho[c], ldf(os, 1) * rho[c + sk]);
+    r = fma_(ldf(os, 0), rho[c], ldf(os, 1) * rho[c + sk]);
     if (g.nz >= 3) r = fma_(ldf(os, 2), rho[c + 2 * sk], r);
-    rf[c] = r;
   } else if (k == g.nz) {
-    float r = fma_(ldf(ot, 0), rho[c - sk], ldf(ot, 1) * rho[c - 2 * sk]);
+    r = fma_(ldf(ot, 0), rho[c - sk], ldf(ot, 1) * rho[c - 2 * sk]);
     if (g.nz >= 3) r = fma_(ldf(ot, 2), rho[c - 3 * sk], r);
-    rf[c] = r;
   } else {
-    rf[c] = fma_(c1[k], rho[c - sk], c2[k] * rho[c]);
+    r = fma_(c1[k], rho[c - sk], c2[k] * rho[c]);
   }
+  if (k < g.nz) rr[c] = 1.0f / rho[c];
+  rf[c] = r;
 }
 
 // turb_def_centers.wgsl:139-168
@@ -251,26 +252,28 @@ __global__ __launch_bounds__(256) void k_turb_km_kh(uint32_t* M, KArg a) {
   float dzk = (1.0f / g.rdz) / mf[k];
   float grdscl = fast_pow((((1.0f / g.rdx) / hm_uh(hm, g, i)) * ((1.0f / g.rdy) / hm_vh(hm, g, j))) * dzk, 0.33333334f);
   float rgrdscl = 1.0f / grdscl;
+  // the cell's operands, loaded once before the stores below
+  const float tkc = tkea[c], nmc = nm[c], dissc = dissten[c];
   float tk, len;
-  if (tkea[c] <= tke_min) {
+  if (tkc <= tke_min) {
     tk = tke_min;
     len = small_len_frac * grdscl;
   } else {
-    tk = tkea[c];
+    tk = tkc;
     len = grdscl;
-    if (nm[c] > nm_min) {
-      len = c_l * sqrtf(tk / nm[c]);
+    if (nmc > nm_min) {
+      len = c_l * sqrtf(tk / nmc);
       len = mn(len, grdscl);
       len = mx(len, small_len_frac * grdscl);
     }
   }
   lenscl[c] = len;
-  dissten[c] = dissten[c] + ((fma_(ce2z * len, rgrdscl, ce1z) * tk) * sqrtf(tk)) / len;
+  dissten[c] = dissc + ((fma_(ce2z * len, rgrdscl, ce1z) * tk) * sqrtf(tk)) / len;
   float km = kmFixed > 0.0f ? kmFixed : ((cmz * sqrtf(tk)) * len) * kmScale;
   kmh[c] = km;
   kmv[c] = km;
   float prinv = 3.0f;
-  if (nm[c] > nm_min) prinv = mn(fma_(2.0f * len, rgrdscl, 1.0f), 3.0f);
+  if (nmc > nm_min) prinv = mn(fma_(2.0f * len, rgrdscl, 1.0f), 3.0f);
   khh[c] = km * prinv;
   khv[c] = km * prinv;
 }
@@ -1230,10 +1233,13 @@ __global__ __launch_bounds__(256) void k_turb_scalar_flux_h(uint32_t* M, KArg a)
   const float *s = fq(a.q[1]), *rho = fq(a.q[2]), *khh = fq(a.q[3]), *hm = fq(a.q[30]);
   float leNum = ldf(a.q[4], 0);
   const int64_t sj = g.sJ, sk = g.sk(), c = g.idx(i, j, k);
-  fq(a.q[5])[c] = (((((-0.125f * (rho[c] + rho[c - 1])) * leNum) * ((khh[c] + khh[c - 1]) + (khh[c + sk] + khh[c - 1 + sk]))) *
-                       (s[c] - s[c - 1])) * g.rdx) * hm_uf(hm, g, i);
-  fq(a.q[6])[c] = (((((-0.125f * (rho[c] + rho[c - sj])) * leNum) * ((khh[c] + khh[c - sj]) + (khh[c + sk] + khh[c - sj + sk]))) *
-                       (s[c] - s[c - sj])) * g.rdy) * hm_vf(hm, g, j);
+  // both fluxes formed before the first store (the loads go out together)
+  const float fx = (((((-0.125f * (rho[c] + rho[c - 1])) * leNum) * ((khh[c] + khh[c - 1]) + (khh[c + sk] + khh[c - 1 + sk]))) *
+                     (s[c] - s[c - 1])) * g.rdx) * hm_uf(hm, g, i);
+  const float fy = (((((-0.125f * (rho[c] + rho[c - sj])) * leNum) * ((khh[c] + khh[c - sj]) + (khh[c + sk] + khh[c - sj + sk]))) *
+                     (s[c] - s[c - sj])) * g.rdy) * hm_vf(hm, g, j);
+  fq(a.q[5])[c] = fx;
+  fq(a.q[6])[c] = fy;
 }
 
 // turb_scalar_vert_implicit.wgsl:210-327
@@ -1609,10 +1615,13 @@ __global__ __launch_bounds__(256) void k_tke_flux_h(uint32_t* M, KArg a) {
   if (i > g.nx || j > g.ny || k > g.nz - 1) return;
   const float *t = fq(a.q[1]), *rf = fq(a.q[2]), *kmh = fq(a.q[3]), *hm = fq(a.q[30]);
   const int64_t sj = g.sJ, c = g.idx(i, j, k);
-  fq(a.q[4])[c] =
+  // both fluxes formed before the first store
+  const float fx =
       (((((-0.25f * (rf[c] + rf[c - 1])) * 2.0f) * (kmh[c] + kmh[c - 1])) * (t[c] - t[c - 1])) * g.rdx) * hm_uf(hm, g, i);
-  fq(a.q[5])[c] =
+  const float fy =
       (((((-0.25f * (rf[c] + rf[c - sj])) * 2.0f) * (kmh[c] + kmh[c - sj])) * (t[c] - t[c - sj])) * g.rdy) * hm_vf(hm, g, j);
+  fq(a.q[4])[c] = fx;
+  fq(a.q[5])[c] = fy;
 }
 
 // tke_transport.wgsl:158-301
