@@ -66,12 +66,37 @@ inline void call(const KernelEntry* e, const K<TL>& k) {
   e->ftl(k);
 }
 
+// The periodic halo fill of field f (timestep.js:1593 haloFill):
+// bc_periodic_x then bc_periodic_y, with the dims and metric slots of A.
+template <class T>
+static void fill_periodic(uint32_t* w, T* v, const uint32_t* A, uint32_t f) {
+  const uint32_t* dm = w + A[4];
+  const uint32_t nx = dm[0], ny = dm[1], nz = dm[2];
+  uint32_t Ax[64] = {0}, Ay[64] = {0};
+  Ax[0] = 4; Ax[1] = ny + 7; Ax[2] = nz + 1;
+  Ay[0] = 4; Ay[1] = nx + 7; Ay[2] = nz + 1;
+  Ax[4] = Ay[4] = A[4];
+  Ax[5] = Ay[5] = f;
+  Ax[34] = Ay[34] = A[34];
+  K<T> kx{w, v, Ax}, ky{w, v, Ay};
+  call(find_kernel("bc_periodic_x"), kx);
+  call(find_kernel("bc_periodic_y"), ky);
+}
+
 template <class T>
 static int run_program(uint32_t* w, T* v, const char* const* names, const uint32_t* args, int nops, int first,
                        int last, char* err, int errlen) {
   for (int o = first; o < nops && o <= last; ++o) {
     const uint32_t* A = args + (size_t)o * 64;
     std::string nm(names[o]);
+    if (nm == "rk_update_xy" || nm == "tke_advance_xy") {
+      // the library's update with the halo fill of its output folded in: the
+      // update, then bc_periodic_x and bc_periodic_y of the field it writes
+      K<T> k1{w, v, A};
+      call(find_kernel(nm.substr(0, nm.size() - 3)), k1);
+      fill_periodic(w, v, A, A[4 + 4]);
+      continue;
+    }
     if (nm == "_copy") {
       const uint32_t n = A[0], src = A[4], dst = A[5];
 #pragma omp parallel for schedule(static)
@@ -137,6 +162,7 @@ static int run_program(uint32_t* w, T* v, const char* const* names, const uint32
         run1("bc_periodic_x", {4, ny + 7, nz + 1}, {{1, f}});
         run1("bc_periodic_y", {4, nx + 7, nz + 1}, {{1, f}});
       }
+      if (P(34) != 0 && w[P(34)] != 0u) fill_periodic(w, v, A, PP3D);  // the stage's pp3d fill, folded in
       continue;
     }
     if (nm == "ac_pp_accumulate") {
@@ -227,6 +253,7 @@ static int run_program(uint32_t* w, T* v, const char* const* names, const uint32
       if (chain & 2) run1("turb_add_tendency", whole, {{1, A[4 + 35]}, {2, TEND}, {3, A[4 + 36]}});
       if (chain & 4) run1("turb_add_tendency", whole, {{1, A[4 + 37]}, {2, TEND}, {3, A[4 + 38]}});
       if (chain & 8) run1("rk_update", asz, {{1, A[4 + 1]}, {2, A[4 + 39]}, {3, TEND}, {4, A[4 + 40]}, {5, A[4 + 41]}});
+      if (chain & 32) fill_periodic(w, v, A, A[4 + 40]);  // the updated field's halo fill, folded in
       continue;
     }
     if (nm == "tendency_reset_add") {  // stage_tendency_reset, then turb_add_tendency x3

@@ -584,7 +584,10 @@ __device__ __forceinline__ void adv_fused_body(uint32_t* M, const KArg& a) {
   // 1 apply_thterm (tend = fma(-div3d, thterm, tend); p33 div3d, p34 thterm),
   // 2 / 4 turb_add_tendency (tend = fma(scale, src, tend); p35/p36, p37/p38),
   // 8 rk_update (phi = fma(dttmp, tend, phi_a); p39 phi_a, p40 phi, p41 dttmp;
-  // phi must not be the advected field, which other workgroups still read).
+  // phi must not be the advected field, which other workgroups still read),
+  // 32 with 8 on a periodic domain (nx, ny >= 4, cell-centred field): the
+  // halo fill of phi after it, each cell's periodic images written by the
+  // cell, level nz's by the cell below it (no thread of the launch reads phi).
   // With an update in the chain the tendency itself is not stored.
   assemble_at<STC>(
       M, a, g, i, j, k, c, P,
@@ -599,7 +602,17 @@ __device__ __forceinline__ void adv_fused_body(uint32_t* M, const KArg& a) {
         if (chain & 2) v = fma_(ldf(a.q[36], 0), add1, v);
         if (chain & 4) v = fma_(ldf(a.q[38], 0), add2, v);
         if (chain & 8) {
-          fq(a.q[40])[c] = fma_(ldf(a.q[41], 0), v, phi_a);
+          float* phi = fq(a.q[40]);
+          const float nv = fma_(ldf(a.q[41], 0), v, phi_a);
+          phi[c] = nv;
+          if (chain & 32) {
+            const bool top = k == g.nz - 1;
+            const float ptop = top ? phi[c + sk] : 0.0f;
+            periodic_images(g, i, j, [&](int64_t o) {
+              phi[c + o] = nv;
+              if (top) phi[c + sk + o] = ptop;
+            });
+          }
         } else {
           tend[c] = v;
         }

@@ -45,6 +45,55 @@ __global__ __launch_bounds__(256) void k_bc_periodic_xy(uint32_t* M, KArg a) {
   }
 }
 
+// A pointwise update of levels [k0, k1) of a cell-centred field on a
+// periodic domain (nx, ny >= 4) followed by the field's halo fill
+// (bc_periodic_x, bc_periodic_y): each cell writes its new value and its
+// periodic images, the cells next to levels 0 and nz also the images of
+// those levels (copied as they are: the update does not write them, and no
+// thread of the launch reads the field's halo).
+template <class F>
+__device__ __forceinline__ void update_with_images(const Dims& g, float* f, int i, int j, int k, int k0, int k1,
+                                                   F value) {
+  const int64_t c = g.idx(i, j, k), sk = g.sk();
+  const bool lo = k == k0 && k0 == 1, hi = k == k1 - 1 && k1 == g.nz;
+  const float flo = lo ? f[c - sk] : 0.0f, fhi = hi ? f[c + sk] : 0.0f;
+  const float v = value(c);
+  f[c] = v;
+  periodic_images(g, i, j, [&](int64_t o) {
+    f[c + o] = v;
+    if (lo) f[c - sk + o] = flo;
+    if (hi) f[c + sk + o] = fhi;
+  });
+}
+
+// rk_update of a scalar (stagger 1) then its halo fill (rk_update.wgsl:111-132,
+// timestep.js:1593 haloFill); arguments as rk_update's.
+__global__ __launch_bounds__(256) void k_rk_update_xy(uint32_t* M, KArg a) {
+  TE_TID3
+  const Dims g = dims(a.q[0], 0);
+  const int i = gx, j = gy, k = gz;
+  if (i >= g.nx || j >= g.ny || k >= g.nz) return;
+  const float *phiA = fq(a.q[2]), *tend = fq(a.q[3]);
+  const float dttmp = ldf(a.q[5], 0);
+  update_with_images(g, fq(a.q[4]), i, j, k, 0, g.nz, [&](int64_t c) { return fma_(dttmp, tend[c], phiA[c]); });
+}
+
+// tke_advance (levels 1 .. nz-1) then the halo fill of tke
+// (tke_advance.wgsl:111-129, timestep.js:1593 haloFill); arguments as
+// tke_advance's.
+__global__ __launch_bounds__(256) void k_tke_advance_xy(uint32_t* M, KArg a) {
+  TE_TID3
+  const Dims g = dims(a.q[0], 0);
+  const int i = gx, j = gy, k = gz + 1;
+  if (i >= g.nx || j >= g.ny || k > g.nz - 1) return;
+  const float *tkea = fq(a.q[1]), *tketen = fq(a.q[2]);
+  const float dttmp = ldf(a.q[3], 0);
+  update_with_images(g, fq(a.q[4]), i, j, k, 1, g.nz, [&](int64_t c) {
+    const float v = fma_(dttmp, tketen[c], tkea[c]);
+    return (v < 1e-10f) ? 0.0f : v;
+  });
+}
+
 // turb_zero (turb_zero.wgsl, timestep.js:1070-1072) of up to 8 fields in one
 // launch over the whole grid [strideJ, strideK, nz+1].  p[1..8] fields,
 // p[9] -> field count.
@@ -188,7 +237,8 @@ namespace te {
 
 // One acoustic substep after acoustic_uv, on a periodic domain, in one launch:
 // acoustic_fpk, acoustic_w_solve, acoustic_pp and acoustic_accumulate, then
-// the periodic halo fill of ppd, u, v and w (timestep.js:2023-2054,
+// the periodic halo fill of ppd, u, v and w, and on the stage's last substep
+// the one of pp3d that follows the substeps (timestep.js:2023-2054,
 // acoustic_fpk.wgsl:167-207, acoustic_w_solve.wgsl:124-212,
 // acoustic_pp.wgsl:131-167, acoustic_accumulate.wgsl:129-158); on the first
 // substep of a stage also acoustic_begin (acoustic_begin.wgsl:129-149).
@@ -218,7 +268,8 @@ namespace te {
 // p: 0 dims, 1 acoustic, 2 u, 3 v, 4 w, 5 piadv, 6 ppten, 7 ppterm, 8 thv,
 //    9 wten, 10 pp3d, 11 ppd, 12 ppx, 13 rru, 14 rrv, 15 rrw, 16 -> save ppx,
 //    17 -> tavg, 18 rr0, 19 rf0, 20 mh, 21 mf, 22 c1, 23 c2, 30 metric,
-//    33 -> first substep of the stage; the fields the replaced dispatches
+//    33 -> first substep of the stage, 34 -> (if bound) fill pp3d's halo
+//    too (with ppd's, group 4 below); the fields the replaced dispatches
 //    write as scratch (for expansions of this launch into them): 24 qk,
 //    25 pk1, 26 pk2, 27 mm, 28 fpk, 29 cA, 31 cB, and 32 rho0.
 // Grid: 64 x AC_WAVES threads per workgroup, ceil(nx / 64) x ny workgroups.
@@ -250,6 +301,7 @@ __device__ __forceinline__ void ac_column(uint32_t* M, const KArg& a) {
   // fields as 32-bit word offsets into M (fewer scalar registers than pointers)
   auto F = [&](int q) { return reinterpret_cast<float*>(M) + a.p[q]; };
   const bool saveX = ldi(a.q[16], 0) != 0, first = ldi(a.q[33], 0) != 0;
+  const bool imgP = a.p[34] != 0 && ldi(a.q[34], 0) != 0;
   const float tavg = ldf(a.q[17], 0);
   if (threadIdx.x == 0 && threadIdx.y == 0) {
     hcnt = 0;
@@ -313,7 +365,8 @@ __device__ __forceinline__ void ac_column(uint32_t* M, const KArg& a) {
   // the wave's own acoustic_fpk reads of u at i = nx and v at j = ny came
   // first.  w, ppd (group 4): rrw accumulates the old w there, then the new w
   // and ppd of the interior cell are copied in; the interior cell is in this
-  // workgroup's row at the same level, written by this wave.
+  // workgroup's row at the same level, written by this wave.  pp3d (with
+  // group 4 when imgP) likewise: no thread of the launch reads its halo.
   // halo(): the cells of the list, 2 (cell, level) items per lane per round
   // with their loads before their stores.
   // The row's own images, lane = column, 2 levels of loads before their
@@ -322,7 +375,7 @@ __device__ __forceinline__ void ac_column(uint32_t* M, const KArg& a) {
   auto row_images = [&](auto G) {
     constexpr int groups = decltype(G)::value;
     if (!active) return;
-    float *u = F(2), *v = F(3), *w = F(4), *ppd = F(11), *rru = F(13), *rrv = F(14), *rrw = F(15);
+    float *u = F(2), *v = F(3), *w = F(4), *pp3d = F(10), *ppd = F(11), *rru = F(13), *rrv = F(14), *rrw = F(15);
     const int nl = Lq + (top ? 1 : 0);
     for (int r = 0; r < 3; ++r) {
       int hy, m;
@@ -339,7 +392,7 @@ __device__ __forceinline__ void ac_column(uint32_t* M, const KArg& a) {
       if (m == 0) continue;
       const int sy = hy < 0 ? hy + ny : (hy >= ny ? hy - ny : hy);
       for (int t0 = 0; t0 < nl; t0 += 2) {
-        float x[2][10];
+        float x[2][11];
 #pragma unroll
         for (int d = 0; d < 2; ++d)
           if (t0 + d < nl) {
@@ -347,7 +400,10 @@ __device__ __forceinline__ void ac_column(uint32_t* M, const KArg& a) {
             const int64_t ch = g.idx(i, hy, k), cs = g.idx(i, sy, k);
             if (m & 1) { x[d][0] = u[ch]; x[d][1] = u[cs]; x[d][2] = rru[ch]; }
             if (m & 2) { x[d][3] = v[ch]; x[d][4] = v[cs]; x[d][5] = rrv[ch]; }
-            if (m & 4) { x[d][6] = w[ch]; x[d][7] = rrw[ch]; x[d][8] = w[cs]; x[d][9] = ppd[cs]; }
+            if (m & 4) {
+              x[d][6] = w[ch]; x[d][7] = rrw[ch]; x[d][8] = w[cs]; x[d][9] = ppd[cs];
+              if (imgP) x[d][10] = pp3d[cs];
+            }
           }
 #pragma unroll
         for (int d = 0; d < 2; ++d)
@@ -366,6 +422,7 @@ __device__ __forceinline__ void ac_column(uint32_t* M, const KArg& a) {
               rrw[ch] = accw(first ? x[d][6] * ac.beta : x[d][7], x[d][6]);
               w[ch] = x[d][8];
               ppd[ch] = x[d][9];
+              if (imgP) pp3d[ch] = x[d][10];
             }
           }
       }
@@ -373,11 +430,11 @@ __device__ __forceinline__ void ac_column(uint32_t* M, const KArg& a) {
   };
   auto halo = [&](auto G) {
     constexpr int groups = decltype(G)::value;
-    float *u = F(2), *v = F(3), *w = F(4), *ppd = F(11), *rru = F(13), *rrv = F(14), *rrw = F(15);
+    float *u = F(2), *v = F(3), *w = F(4), *pp3d = F(10), *ppd = F(11), *rru = F(13), *rrv = F(14), *rrw = F(15);
     const int nc = hcnt, nl = Lq + (top ? 1 : 0), items = nc * nl;
     constexpr int B = 2;
     for (int it0 = lane; it0 < items; it0 += B * 64) {
-      float x[B][10];
+      float x[B][11];
       int64_t hc[B];
       int mask[B];
 #pragma unroll
@@ -393,7 +450,10 @@ __device__ __forceinline__ void ac_column(uint32_t* M, const KArg& a) {
           mask[d] = hcell[q][2] & groups;
           if (mask[d] & 1) { x[d][0] = u[ch]; x[d][1] = u[cs]; x[d][2] = rru[ch]; }
           if (mask[d] & 2) { x[d][3] = v[ch]; x[d][4] = v[cs]; x[d][5] = rrv[ch]; }
-          if (mask[d] & 4) { x[d][6] = w[ch]; x[d][7] = rrw[ch]; x[d][8] = w[cs]; x[d][9] = ppd[cs]; }
+          if (mask[d] & 4) {
+            x[d][6] = w[ch]; x[d][7] = rrw[ch]; x[d][8] = w[cs]; x[d][9] = ppd[cs];
+            if (imgP) x[d][10] = pp3d[cs];
+          }
         }
       }
 #pragma unroll
@@ -411,6 +471,7 @@ __device__ __forceinline__ void ac_column(uint32_t* M, const KArg& a) {
           rrw[ch] = accw(first ? x[d][6] * ac.beta : x[d][7], x[d][6]);
           w[ch] = x[d][8];
           ppd[ch] = x[d][9];
+          if (imgP) pp3d[ch] = x[d][10];
         }
       }
     }

@@ -160,12 +160,12 @@ __global__ __launch_bounds__(256) void k_turb_nm(uint32_t* M, KArg a) {
   float *nm = fq(a.q[15]), *sat = fq(a.q[17]);
   int moist = ldi(a.q[16], 0), wantSat = ldi(a.q[18], 0);
   const int64_t sk = g.sk(), c = g.idx(i, j, k);
-  if (wantSat != 0) sat[c] = 0.0f;
   const float nmsat = 1e-10f, eps = RD / RV, reps = RV / RD, gAcc = 9.81f;
   const float LV1 = 3134708.0f, LV2 = 2.32e+03f, LS1 = 2898463.5f, LS2 = 236.0f;
   float thA = th0[k - 1] + thP[c - sk];
   float thB = th0[k] + thP[c];
   if (moist == 0) {
+    if (wantSat != 0) sat[c] = 0.0f;
     nm[c] = ((fast_log(thB / thA) * gAcc) * g.rdz) * mf[k];
     return;
   }
@@ -178,58 +178,69 @@ __global__ __launch_bounds__(256) void k_turb_nm(uint32_t* M, KArg a) {
   float thvA = (thA * fma_(reps, qv[c - sk], 1.0f)) / (1.0f + qtA);
   float thvB = (thB * fma_(reps, qv[c], 1.0f)) / (1.0f + qtB);
   float rmf = g.rdz * mf[k];
-  float dlnT = fast_log(tB / tA) * rmf;
-  float dlnQt = fast_log((1.0f + qtB) / (1.0f + qtA)) * rmf;
-  float out = (gAcc * fast_log(thvB / thvA)) * rmf;
   float A = c1[k], B = c2[k];
   float pavg = fma_(A, prs[c - sk], B * prs[c]);
   float tavg = fma_(A, tA, B * tB);
   float qtavg = fma_(A, qtA, B * qtB);
   float qciavg = fma_(A, qciA, B * qciB);
-  auto lam = [&](float qvs, float lh, float drdt, float cpml) {
-    float gamma = ((gAcc * (1.0f + qtavg)) * (1.0f + (lh * qvs) / (RD * tavg))) / fma_(lh, drdt, cpml);
-    return gAcc * fma_(dlnT + gamma / tavg, 1.0f + (tavg * drdt) / (eps + qvs), -dlnQt);
-  };
+  // The three temperature ranges only choose lam's operands (liquid, ice, or
+  // both blended by ff); lam and the logarithms it needs are evaluated once,
+  // below, for the saturated points alone.
+  bool satd = false, blend = false;
+  float qvs1 = 0.0f, lh1 = 0.0f, drdt1 = 0.0f, qvs2 = 0.0f, lh2 = 0.0f, drdt2 = 0.0f, cpml = 0.0f, ff = 0.0f;
   if (tavg >= 273.15f) {
     float qvs = mp_rslf(pavg, tavg);
     float ql = mx(qciavg - qvs, 0.0f);
     if (ql > nmsat) {
-      if (wantSat != 0) sat[c] = 1.0f;
+      satd = true;
       float qvl = mx(qciavg - ql, 0.0f);
-      float cpml = fma_(CPL, ql, fma_(CPV, qvl, CP));
-      float lh = fma_(-LV2, tavg, LV1);
-      float drdt = ((17.67f * (273.15f - 29.65f)) * qvs) / ((tavg - 29.65f) * (tavg - 29.65f));
-      out = lam(qvs, lh, drdt, cpml);
+      cpml = fma_(CPL, ql, fma_(CPV, qvl, CP));
+      qvs1 = qvs;
+      lh1 = fma_(-LV2, tavg, LV1);
+      drdt1 = ((17.67f * (273.15f - 29.65f)) * qvs) / ((tavg - 29.65f) * (tavg - 29.65f));
     }
   } else if (tavg <= 233.15f) {
     float qvs = mp_rsif(pavg, tavg);
     float qice = mx(qciavg - qvs, 0.0f);
     if (qice > nmsat) {
-      if (wantSat != 0) sat[c] = 1.0f;
+      satd = true;
       float qvl = mx(qciavg - qice, 0.0f);
-      float cpml = fma_(CPI, qice, fma_(CPV, qvl, CP));
-      float lh = fma_(-LS2, tavg, LS1);
-      float drdt = ((21.874557f * (273.15f - 7.66f)) * qvs) / ((tavg - 7.66f) * (tavg - 7.66f));
-      out = lam(qvs, lh, drdt, cpml);
+      cpml = fma_(CPI, qice, fma_(CPV, qvl, CP));
+      qvs1 = qvs;
+      lh1 = fma_(-LS2, tavg, LS1);
+      drdt1 = ((21.874557f * (273.15f - 7.66f)) * qvs) / ((tavg - 7.66f) * (tavg - 7.66f));
     }
   } else {
-    float ff = (tavg - 233.15f) / (273.15f - 233.15f);
+    ff = (tavg - 233.15f) / (273.15f - 233.15f);
     float qvsl = mp_rslf(pavg, tavg), qvsi = mp_rsif(pavg, tavg);
     float qvs = fma_(ff, qvsl, (1.0f - ff) * qvsi);
     float qice = mx(0.0f, (1.0f - ff) * (qciavg - qvs));
     float ql = mx(0.0f, ff * (qciavg - qvs));
     if (ql + qice > nmsat) {
-      if (wantSat != 0) sat[c] = 1.0f;
+      satd = blend = true;
       float qvl = mx(0.0f, (qciavg - ql) - qice);
-      float cpml = fma_(CPI, qice, fma_(CPL, ql, fma_(CPV, qvl, CP)));
-      float lhl = fma_(-LV2, tavg, LV1);
-      float drdtl = ((17.67f * (273.15f - 29.65f)) * qvsl) / ((tavg - 29.65f) * (tavg - 29.65f));
-      float nml = lam(qvsl, lhl, drdtl, cpml);
-      float lhi = fma_(-LS2, tavg, LS1);
-      float drdti = ((21.874557f * (273.15f - 7.66f)) * qvsi) / ((tavg - 7.66f) * (tavg - 7.66f));
-      float nmi = lam(qvsi, lhi, drdti, cpml);
-      out = fma_(ff, nml, (1.0f - ff) * nmi);
+      cpml = fma_(CPI, qice, fma_(CPL, ql, fma_(CPV, qvl, CP)));
+      qvs1 = qvsl;
+      lh1 = fma_(-LV2, tavg, LV1);
+      drdt1 = ((17.67f * (273.15f - 29.65f)) * qvsl) / ((tavg - 29.65f) * (tavg - 29.65f));
+      qvs2 = qvsi;
+      lh2 = fma_(-LS2, tavg, LS1);
+      drdt2 = ((21.874557f * (273.15f - 7.66f)) * qvsi) / ((tavg - 7.66f) * (tavg - 7.66f));
     }
+  }
+  if (wantSat != 0) sat[c] = satd ? 1.0f : 0.0f;
+  float out;
+  if (satd) {
+    const float dlnT = fast_log(tB / tA) * rmf;
+    const float dlnQt = fast_log((1.0f + qtB) / (1.0f + qtA)) * rmf;
+    auto lam = [&](float qvs, float lh, float drdt) {
+      float gamma = ((gAcc * (1.0f + qtavg)) * (1.0f + (lh * qvs) / (RD * tavg))) / fma_(lh, drdt, cpml);
+      return gAcc * fma_(dlnT + gamma / tavg, 1.0f + (tavg * drdt) / (eps + qvs), -dlnQt);
+    };
+    const float n1 = lam(qvs1, lh1, drdt1);
+    out = blend ? fma_(ff, n1, (1.0f - ff) * lam(qvs2, lh2, drdt2)) : n1;
+  } else {
+    out = (gAcc * fast_log(thvB / thvA)) * rmf;
   }
   nm[c] = out;
 }

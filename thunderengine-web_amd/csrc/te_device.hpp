@@ -62,6 +62,23 @@ struct Dims {
   TE_DEV int64_t sk() const { return (int64_t)sJ * sK; }
   TE_DEV int64_t idx(int i, int j, int k) const { return (int64_t)(i + 3) + (int64_t)sJ * ((j + 3) + (int64_t)sK * k); }
 };
+// The periodic halo cells whose image is interior cell (i, j) of a level:
+// hx in {i, i - nx (i >= nx - 3), i + nx (i <= 3)}, hy likewise with ny, all
+// pairs but (i, j) itself -- the cells bc_periodic_x then bc_periodic_y
+// (bc_periodic_x.wgsl:111-133, bc_periodic_y.wgsl:111-133) copy from it,
+// for nx, ny >= 4.  put(o) gets each one's word offset from (i, j).
+template <class F>
+TE_DEV void periodic_images(const Dims& g, int i, int j, F&& put) {
+  int64_t dx[3] = {0, 0, 0}, dy[3] = {0, 0, 0};
+  int nxo = 1, nyo = 1;
+  if (i >= g.nx - 3) dx[nxo++] = -g.nx;
+  if (i <= 3) dx[nxo++] = g.nx;
+  if (j >= g.ny - 3) dy[nyo++] = -(int64_t)g.ny * g.sJ;
+  if (j <= 3) dy[nyo++] = (int64_t)g.ny * g.sJ;
+  for (int b = 0; b < nyo; ++b)
+    for (int a = 0; a < nxo; ++a)
+      if (a != 0 || b != 0) put(dx[a] + dy[b]);
+}
 // GridDims bytes: nx ny nz strideJ strideK 1/dx 1/dy 1/dz (timestep.js:88-90).
 TE_DEV Dims dims(const uint32_t* M, uint32_t o) {
   Dims g;

@@ -121,6 +121,11 @@ class Encoder {
   // fusion is on; they are written by one bc_periodic_xy launch (up to 8
   // fields) just before the next op that is not a halo fill.
   bool mergeHalo = false;
+  // fuse = 1 on a periodic domain (nx, ny >= 4): the halo fill after a
+  // scalar's RK update, TKE's advance and the stage's pressure is written by
+  // the launch that produces the field (each cell writes its periodic images)
+  // instead of by a bc_periodic_xy launch
+  bool foldHalo = false;
   std::vector<std::string> pendingHalo, pendingZero;
   // turb_zero runs queue likewise (fusion on): one zero_fields launch per 8.
   void flushZero() {
@@ -275,6 +280,7 @@ void Encoder::setup() {
   openX = cfg.open_x != 0; openY = cfg.open_y != 0;
   periodicX = !openX; periodicY = !openY;
   mergeHalo = cfg.fuse != 0 && periodicX && periodicY;
+  foldHalo = mergeHalo && cfg.fuse == 1 && cfg.nx >= 4 && cfg.ny >= 4;
   wenoS = cfg.weno_scalars != 0;
   wenoM = cfg.weno_momentum != 0;
   explicitZ = cfg.explicit_vdiff != 0;
@@ -592,7 +598,7 @@ void Encoder::advect(int nrk, const std::string& s, int stagger, const std::stri
         bs.push_back(bb(36, F1(1.0f)));
       }
       if (update) {
-        flags |= 8;
+        flags |= 8 | (foldHalo ? 32 : 0);
         bs.push_back(fb(39, target + "_a"));
         bs.push_back(fb(40, target));
         bs.push_back(bb(41, dttmp));
@@ -605,8 +611,9 @@ void Encoder::advect(int nrk, const std::string& s, int stagger, const std::stri
     for (const Bind& bd : bs) E.ops.back().A[4 + bd.idx] = (uint32_t)bd.off;
     if (chained) {
       if (!update)
-        op("rk_update", asmSize, {bb(1, I1(stagger)), fb(2, target + "_a"), fb(3, "tend"), fb(4, target), bb(5, dttmp)});
-      haloFill(target);
+        op(foldHalo ? "rk_update_xy" : "rk_update", asmSize,
+           {bb(1, I1(stagger)), fb(2, target + "_a"), fb(3, "tend"), fb(4, target), bb(5, dttmp)});
+      if (!foldHalo) haloFill(target);
       return;
     }
     if (tkeChained) return;
@@ -717,8 +724,10 @@ void Encoder::afterAssemble(int nrk, const std::string& s, int stagger, const st
     const char* src = target == "th" ? "qtten" : (target == "qv" ? "qvten" : (target == "qc" ? "qcten" : nullptr));
     if (src) op("turb_add_tendency", whole, {fb(1, src), fb(2, "tend"), bb(3, F1(1.0f))});
   }
-  op("rk_update", asmSize, {bb(1, I1(stagger)), fb(2, target + "_a"), fb(3, "tend"), fb(4, target), bb(5, dttmp)});
-  haloFill(target);
+  const bool fold = foldHalo && stagger == 1;
+  op(fold ? "rk_update_xy" : "rk_update", asmSize,
+     {bb(1, I1(stagger)), fb(2, target + "_a"), fb(3, "tend"), fb(4, target), bb(5, dttmp)});
+  if (!fold) haloFill(target);
 }
 
 void Encoder::stages() {
@@ -766,14 +775,15 @@ void Encoder::stages() {
     if (tke) {
       if (cfg.fuse != 0) {  // zero, assembly and add in one launch
         advect(nrk, "tke", 4, "tketen_s", true, "");
-        op("tke_advance", full, {fb(1, "tke_a"), fb(2, "tketen_s"), bb(3, dttmp), fb(4, "tke")});
+        op(foldHalo ? "tke_advance_xy" : "tke_advance", full,
+           {fb(1, "tke_a"), fb(2, "tketen_s"), bb(3, dttmp), fb(4, "tke")});
       } else {
         op("turb_zero", whole, {fb(1, "tketen_s")});
         advect(nrk, "tke", 4, "tketen_s", true, "");
         op("turb_add_tendency", whole, {fb(1, "tketen"), fb(2, "tketen_s"), bb(3, F1(1.0f))});
         op("tke_advance", full, {fb(1, "tke_a"), fb(2, "tketen_s"), bb(3, dttmp), fb(4, "tke")});
       }
-      haloFill("tke");
+      if (!foldHalo) haloFill("tke");
     }
     op("buoyancy_apply", {nx, ny, std::max(nz - 1, 1)}, {fb(1, "buoy"), rb(2, "c1"), rb(3, "c2"), fb(4, "wten")});
     op("stage_state_reset", whole,
@@ -807,7 +817,8 @@ void Encoder::stages() {
             fb(9, "wten"), fb(10, "pp3d"), fb(11, "ppd"), fb(12, "ppx"), fb(13, "rru"), fb(14, "rrv"),
             fb(15, "rrw"), bb(16, I1(nrk == 2 ? 1 : 0)), bb(17, F1(tavg)), rb(18, "rr0"), rb(19, "rf0"),
             rb(20, "mh"), rb(21, "mf"), rb(22, "c1"), rb(23, "c2"), fb(24, "qk"), fb(25, "pk1"), fb(26, "pk2"),
-            fb(27, "mm"), fb(28, "fpk"), fb(29, "cA"), fb(31, "cB"), rb(32, "rho0"), bb(33, I1(n == 0 ? 1 : 0))},
+            fb(27, "mm"), fb(28, "fpk"), fb(29, "cA"), fb(31, "cB"), rb(32, "rho0"), bb(33, I1(n == 0 ? 1 : 0)),
+            bb(34, I1(foldHalo && n == nloop - 1 ? 1 : 0))},
            true);
         continue;
       }
@@ -850,7 +861,7 @@ void Encoder::stages() {
       haloFill("v");
       haloFill("w");
     }
-    haloFill("pp3d");
+    if (!(column && foldHalo)) haloFill("pp3d");
     if (openX)
       op("radbc_ew", {ny, 1, 1}, {fb(1, "u_a"), fb(2, "u"), bb(3, dttmp), rb(4, "radbcw"), rb(5, "radbce")});
     if (openY)
