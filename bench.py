@@ -130,7 +130,7 @@ def alg_bytes_per_launch(run, names, A, op):
         io = [(1 + q, False, True) for q in range(3)] + [(4 + q, True, False) for q in range(3)]
     elif name == "adv_fused":
         io = adv_fused_io(run, A, op)
-    elif name in ("rk_update_xy", "tke_advance_xy"):  # the update, then its output's periodic halo cells
+    elif name.endswith("_xy") and name[:-3] in kernel_io():  # an update, then its output's periodic halo cells
         halo = ((run.nx + 7) * (run.ny + 7) - run.nx * run.ny) * (run.nz + 1)
         return 4.0 * (count(kernel_io()[name[:-3]]) * pts + halo)
     else:

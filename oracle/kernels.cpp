@@ -89,13 +89,21 @@ static int run_program(uint32_t* w, T* v, const char* const* names, const uint32
   for (int o = first; o < nops && o <= last; ++o) {
     const uint32_t* A = args + (size_t)o * 64;
     std::string nm(names[o]);
-    if (nm == "rk_update_xy" || nm == "tke_advance_xy") {
-      // the library's update with the halo fill of its output folded in: the
-      // update, then bc_periodic_x and bc_periodic_y of the field it writes
-      K<T> k1{w, v, A};
-      call(find_kernel(nm.substr(0, nm.size() - 3)), k1);
-      fill_periodic(w, v, A, A[4 + 4]);
-      continue;
+    {
+      // the library's pointwise updates with the halo fill of their output
+      // folded in: the update, then bc_periodic_x and bc_periodic_y of the
+      // field it writes (parameter slot of that field)
+      static const std::pair<const char*, int> kFolded[] = {
+          {"rk_update_xy", 4}, {"tke_advance_xy", 4}, {"pdef_dum_xy", 10}, {"ppx_finish_xy", 2}};
+      int out = -1;
+      for (const auto& fo : kFolded)
+        if (nm == fo.first) out = fo.second;
+      if (out >= 0) {
+        K<T> k1{w, v, A};
+        call(find_kernel(nm.substr(0, nm.size() - 3)), k1);
+        fill_periodic(w, v, A, A[4 + out]);
+        continue;
+      }
     }
     if (nm == "_copy") {
       const uint32_t n = A[0], src = A[4], dst = A[5];

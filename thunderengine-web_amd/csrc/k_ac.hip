@@ -252,29 +252,41 @@ TE_DEV bool pdef_inflow(const Dims& g, int dir, int i, int j, int64_t c, int64_t
   return false;
 }
 
-// pdef_dum.wgsl:177-231
-__global__ __launch_bounds__(256) void k_pdef_dum(uint32_t* M, KArg a) {
-  TE_TID3
-  Dims g = dims(a.q[0], 0);
-  int i = gx, j = gy, k = gz;
-  if (i >= g.nx || j >= g.ny || k >= g.nz) return;
+// pdef_dum.wgsl:177-231: the value at cell (i, j, k), index c
+TE_DEV float pdef_dum_at(uint32_t* M, const KArg& a, const Dims g, int i, int j, int k, int64_t c) {
   int dir = ldi(a.q[1], 0);
   const float *s = fq(a.q[2]), *s0 = fq(a.q[3]), *f = fq(a.q[4]), *rr = fq(a.q[5]), *rho0 = fq(a.q[6]),
               *mh = fq(a.q[7]), *hm = fq(a.q[30]);
   float dt = ldf(a.q[8], 0);
   int openx = ldi(a.q[9], 0), openy = ldi(a.q[9], 1);
-  float* dum = fq(a.q[10]);
-  const int64_t c = g.idx(i, j, k);
   const int64_t step = dir == 0 ? 1 : (dir == 1 ? (int64_t)g.sJ : g.sk());
   float rd = dir == 0 ? g.rdx : (dir == 1 ? g.rdy : g.rdz);
   float base = rho0[k] * s0[c];
-  if (pdef_inflow(g, dir, i, j, c, step, rr, openx, openy)) { dum[c] = base; return; }
+  if (pdef_inflow(g, dir, i, j, c, step, rr, openx, openy)) return base;
   float adv = (-(f[c + step] - f[c])) * rd;
   float sdiv = (s[c] * (rr[c + step] - rr[c])) * rd;
   float m = dir == 0 ? hm_uh(hm, g, i) : (dir == 1 ? hm_vh(hm, g, j) : mh[k]);
   adv = adv * m;
   sdiv = sdiv * m;
-  dum[c] = fma_(dt, adv + sdiv, base);
+  return fma_(dt, adv + sdiv, base);
+}
+__global__ __launch_bounds__(256) void k_pdef_dum(uint32_t* M, KArg a) {
+  TE_TID3
+  const Dims g = dims(a.q[0], 0);
+  const int i = gx, j = gy, k = gz;
+  if (i >= g.nx || j >= g.ny || k >= g.nz) return;
+  const int64_t c = g.idx(i, j, k);
+  fq(a.q[10])[c] = pdef_dum_at(M, a, g, i, j, k, c);
+}
+// pdef_dum, then the halo fill of its output on a periodic domain (nx, ny >=
+// 4; timestep.js:1593 haloFill): each cell writes its periodic images.
+// Arguments as pdef_dum's.
+__global__ __launch_bounds__(256) void k_pdef_dum_xy(uint32_t* M, KArg a) {
+  TE_TID3
+  const Dims g = dims(a.q[0], 0);
+  const int i = gx, j = gy, k = gz;
+  if (i >= g.nx || j >= g.ny || k >= g.nz) return;
+  update_with_images(g, fq(a.q[10]), i, j, k, 0, g.nz, [&](int64_t c) { return pdef_dum_at(M, a, g, i, j, k, c); });
 }
 
 struct PdefCell { bool neg; float f1, f2, foo3; };

@@ -324,6 +324,16 @@ __global__ __launch_bounds__(256) void k_ppx_finish(uint32_t* M, KArg a) {
   float* ppx = fq(a.q[2]);
   ppx[c] = fq(a.q[1])[c] + ppx[c];
 }
+// ppx_finish, then the halo fill of ppx on a periodic domain (nx, ny >= 4;
+// timestep.js:1593 haloFill): each cell writes its periodic images.
+__global__ __launch_bounds__(256) void k_ppx_finish_xy(uint32_t* M, KArg a) {
+  TE_TID3
+  const Dims g = dims(a.q[0], 0);
+  if (gx >= g.nx || gy >= g.ny || gz >= g.nz) return;
+  const float* pp = fq(a.q[1]);
+  float* ppx = fq(a.q[2]);
+  update_with_images(g, ppx, gx, gy, gz, 0, g.nz, [&](int64_t c) { return pp[c] + ppx[c]; });
+}
 
 // diabatic_unapply.wgsl:111-140
 __global__ __launch_bounds__(256) void k_diabatic_unapply(uint32_t* M, KArg a) {

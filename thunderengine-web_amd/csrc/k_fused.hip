@@ -45,27 +45,6 @@ __global__ __launch_bounds__(256) void k_bc_periodic_xy(uint32_t* M, KArg a) {
   }
 }
 
-// A pointwise update of levels [k0, k1) of a cell-centred field on a
-// periodic domain (nx, ny >= 4) followed by the field's halo fill
-// (bc_periodic_x, bc_periodic_y): each cell writes its new value and its
-// periodic images, the cells next to levels 0 and nz also the images of
-// those levels (copied as they are: the update does not write them, and no
-// thread of the launch reads the field's halo).
-template <class F>
-__device__ __forceinline__ void update_with_images(const Dims& g, float* f, int i, int j, int k, int k0, int k1,
-                                                   F value) {
-  const int64_t c = g.idx(i, j, k), sk = g.sk();
-  const bool lo = k == k0 && k0 == 1, hi = k == k1 - 1 && k1 == g.nz;
-  const float flo = lo ? f[c - sk] : 0.0f, fhi = hi ? f[c + sk] : 0.0f;
-  const float v = value(c);
-  f[c] = v;
-  periodic_images(g, i, j, [&](int64_t o) {
-    f[c + o] = v;
-    if (lo) f[c - sk + o] = flo;
-    if (hi) f[c + sk + o] = fhi;
-  });
-}
-
 // rk_update of a scalar (stagger 1) then its halo fill (rk_update.wgsl:111-132,
 // timestep.js:1593 haloFill); arguments as rk_update's.
 __global__ __launch_bounds__(256) void k_rk_update_xy(uint32_t* M, KArg a) {

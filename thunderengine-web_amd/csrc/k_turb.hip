@@ -386,36 +386,54 @@ __global__ __launch_bounds__(256) void k_turb_sfcstress(uint32_t* M, KArg a) {
   t23[c] = t23[c + plane];
 }
 
-// tke_epst.wgsl:170-206 (shear production)
+// tke_epst.wgsl:170-206 (shear production).  Two levels per thread (the
+// runtime halves the launch's z extent): k0 = 2 gz + 1 and k0 + 1.  The
+// terms of the face between them (L1, the four-point L2 sum, L5 at k0) enter
+// level k0 as its upper terms and level k0 + 1 as its lower ones, formed once.
 __global__ __launch_bounds__(256) void k_tke_epst(uint32_t* M, KArg a) {
   TE_TID3
   Dims g = dims(a.q[0], 0);
-  int i = gx, j = gy, k = gz + 1;
-  if (i >= g.nx || j >= g.ny || k > g.nz - 1) return;
+  const int i = gx, j = gy, k0 = 2 * gz + 1;
+  if (i >= g.nx || j >= g.ny || k0 > g.nz - 1) return;
   const float *u = fq(a.q[1]), *v = fq(a.q[2]), *w = fq(a.q[3]), *t11 = fq(a.q[4]), *t12 = fq(a.q[5]),
               *t13 = fq(a.q[6]), *t22 = fq(a.q[7]), *t23 = fq(a.q[8]), *t33 = fq(a.q[9]), *rf = fq(a.q[10]),
               *mh = fq(a.q[11]), *mf = fq(a.q[12]), *c1 = fq(a.q[13]), *c2 = fq(a.q[14]), *hm = fq(a.q[30]);
-  const int64_t sj = g.sJ, sk = g.sk(), c = g.idx(i, j, k), cd = c - sk;
+  const int64_t sj = g.sJ, sk = g.sk(), c0 = g.idx(i, j, k0);
   auto L1 = [&](int64_t cc) {
     return fma_(((u[cc + 1] - u[cc]) * g.rdx) * hm_uh(hm, g, i), t11[cc], (((v[cc + sj] - v[cc]) * g.rdy) * hm_vh(hm, g, j)) * t22[cc]);
   };
   auto L2 = [&](int64_t cc, int ii, int jj) {
     return t12[cc] * fma_((u[cc] - u[cc - sj]) * g.rdy, hm_vf(hm, g, jj), ((v[cc] - v[cc - 1]) * g.rdx) * hm_uf(hm, g, ii));
   };
-  auto L3 = [&](int64_t cc, int ii) {
+  auto L3 = [&](int64_t cc, int ii, int k) {
     return fma_((w[cc] - w[cc - 1]) * g.rdx, hm_uf(hm, g, ii), ((u[cc] - u[cc - sk]) * g.rdz) * mf[k]) * t13[cc];
   };
-  auto L4 = [&](int64_t cc, int jj) {
+  auto L4 = [&](int64_t cc, int jj, int k) {
     return fma_((w[cc] - w[cc - sj]) * g.rdy, hm_vf(hm, g, jj), ((v[cc] - v[cc - sk]) * g.rdz) * mf[k]) * t23[cc];
   };
   auto L5 = [&](int64_t cc, int lev) { return (((w[cc + sk] - w[cc]) * g.rdz) * mh[lev]) * t33[cc]; };
-  float e = fma_(c1[k], L1(cd), c2[k] * L1(c));
-  float up = (L2(c, i, j) + L2(c + 1 + sj, i + 1, j + 1)) + (L2(c + sj, i, j + 1) + L2(c + 1, i + 1, j));
-  float dn = (L2(cd, i, j) + L2(cd + 1 + sj, i + 1, j + 1)) + (L2(cd + sj, i, j + 1) + L2(cd + 1, i + 1, j));
-  e = fma_(0.25f, fma_(c2[k], up, c1[k] * dn), e);
-  e = fma_(0.5f, (L3(c, i) + L3(c + 1, i + 1)) + (L4(c, j) + L4(c + sj, j + 1)), e);
-  e = e + fma_(c1[k], L5(cd, k - 1), c2[k] * L5(c, k));
-  fq(a.q[15])[c] = e / rf[c];
+  auto L2s = [&](int64_t cc) {
+    return (L2(cc, i, j) + L2(cc + 1 + sj, i + 1, j + 1)) + (L2(cc + sj, i, j + 1) + L2(cc + 1, i + 1, j));
+  };
+  // level k at c from its lower (l1d, dn, l5d) and upper (l1c, up, l5c) terms
+  auto level = [&](int k, int64_t c, float l1d, float l1c, float dn, float up, float l5d, float l5c) {
+    float e = fma_(c1[k], l1d, c2[k] * l1c);
+    e = fma_(0.25f, fma_(c2[k], up, c1[k] * dn), e);
+    e = fma_(0.5f, (L3(c, i, k) + L3(c + 1, i + 1, k)) + (L4(c, j, k) + L4(c + sj, j + 1, k)), e);
+    e = e + fma_(c1[k], l5d, c2[k] * l5c);
+    return e / rf[c];
+  };
+  const int64_t cd = c0 - sk, cu = c0 + sk;
+  const float l1m = L1(cd), l10 = L1(c0), s2m = L2s(cd), s20 = L2s(c0), l5m = L5(cd, k0 - 1), l50 = L5(c0, k0);
+  const float e0 = level(k0, c0, l1m, l10, s2m, s20, l5m, l50);
+  float* epst = fq(a.q[15]);
+  if (k0 + 1 <= g.nz - 1) {
+    const float e1 = level(k0 + 1, cu, l10, L1(cu), s20, L2s(cu), l50, L5(cu, k0 + 1));
+    epst[c0] = e0;
+    epst[cu] = e1;
+  } else {
+    epst[c0] = e0;
+  }
 }
 
 // tke_sources.wgsl:111-129

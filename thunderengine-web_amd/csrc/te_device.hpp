@@ -79,6 +79,25 @@ TE_DEV void periodic_images(const Dims& g, int i, int j, F&& put) {
     for (int a = 0; a < nxo; ++a)
       if (a != 0 || b != 0) put(dx[a] + dy[b]);
 }
+// A pointwise update of levels [k0, k1) of a cell-centred field on a
+// periodic domain (nx, ny >= 4) followed by the field's halo fill
+// (bc_periodic_x, bc_periodic_y): each cell writes its new value and its
+// periodic images, the cells next to levels 0 and nz also the images of
+// those levels (copied as they are: the update does not write them, and no
+// thread of the launch reads the field's halo).
+template <class F>
+TE_DEV void update_with_images(const Dims& g, float* f, int i, int j, int k, int k0, int k1, F value) {
+  const int64_t c = g.idx(i, j, k), sk = g.sk();
+  const bool lo = k == k0 && k0 == 1, hi = k == k1 - 1 && k1 == g.nz;
+  const float flo = lo ? f[c - sk] : 0.0f, fhi = hi ? f[c + sk] : 0.0f;
+  const float v = value(c);
+  f[c] = v;
+  periodic_images(g, i, j, [&](int64_t o) {
+    f[c + o] = v;
+    if (lo) f[c - sk + o] = flo;
+    if (hi) f[c + sk + o] = fhi;
+  });
+}
 // GridDims bytes: nx ny nz strideJ strideK 1/dx 1/dy 1/dz (timestep.js:88-90).
 TE_DEV Dims dims(const uint32_t* M, uint32_t o) {
   Dims g;

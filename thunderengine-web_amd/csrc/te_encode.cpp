@@ -703,11 +703,13 @@ void Encoder::afterAssemble(int nrk, const std::string& s, int stagger, const st
     const std::string s0 = target + "_a";
     const char* names[3][3] = {{"fx", "rru", "pk1"}, {"fy", "rrv", "pk2"}, {"fz", "rrw", "fpk"}};
     for (int dd = 0; dd < 3; ++dd)
-      op("pdef_dum", asmSize,
+      op(foldHalo && dd < 2 ? "pdef_dum_xy" : "pdef_dum", asmSize,
          {bb(1, I1(dd)), fb(2, s), fb(3, s0), fb(4, names[dd][0]), fb(5, names[dd][1]), rb(6, "rho0"), rb(7, "mh"),
           bb(8, dttmp), bb(9, open), fb(10, names[dd][2])});
-    haloFill("pk1");
-    haloFill("pk2");
+    if (!foldHalo) {
+      haloFill("pk1");
+      haloFill("pk2");
+    }
     op("pdef_apply", asmSize,
        {fb(1, s), fb(2, s0), fb(3, "fx"), fb(4, "fy"), fb(5, "fz"), fb(6, "rru"), fb(7, "rrv"), fb(8, "rrw"),
         fb(9, "pk1"), fb(10, "pk2"), fb(11, "fpk"), rb(12, "rho0"), rb(13, "mh"), bb(14, dttmp), bb(15, open),
@@ -915,8 +917,8 @@ void Encoder::end() {
     op("diabatic_rates", full,
        {bb(1, F1(dtf)), fb(2, "pp3d"), fb(3, "th"), fb(4, "qv"), fb(5, "qc"), fb(6, "qpten"), fb(7, "qtten"),
         fb(8, "qvten"), fb(9, "qcten")});
-  op("ppx_finish", full, {fb(1, "pp3d"), fb(2, "ppx")});
-  haloFill("ppx");
+  op(foldHalo ? "ppx_finish_xy" : "ppx_finish", full, {fb(1, "pp3d"), fb(2, "ppx")});
+  if (!foldHalo) haloFill("ppx");
   flushAdd();
   flushHalo();
   flushZero();

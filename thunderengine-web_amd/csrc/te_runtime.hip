@@ -123,8 +123,9 @@ static hipError_t launch(te_ctx* c, const Op& op) {
   for (int q = 0; q < 48; ++q) a.q[q] = c->dM + a.p[q];
   if (a.n[0] == 0 || a.n[1] == 0 || a.n[2] == 0) return hipSuccess;
   dim3 b = block_for(a.n);
-  static const int kAdvFused = kernel_id("adv_fused");
+  static const int kAdvFused = kernel_id("adv_fused"), kTkeEpst = kernel_id("tke_epst");
   if (op.kid == kAdvFused) b = dim3(64, 4, 1);  // one row of cells per wave (k_adv_fused)
+  if (op.kid == kTkeEpst) a.n[2] = (a.n[2] + 1) / 2;  // two levels per thread (k_tke_epst)
   static const int kAcColumn = kernel_id("ac_column");
   if (op.kid == kAcColumn) {  // 64 columns of a row per workgroup, AC_WAVES waves over the levels
     const uint32_t nz = op.nz, lw = (nz + AC_WAVES - 1) / AC_WAVES;
