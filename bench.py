@@ -130,9 +130,12 @@ def alg_bytes_per_launch(run, names, A, op):
         io = [(1 + q, False, True) for q in range(3)] + [(4 + q, True, False) for q in range(3)]
     elif name == "adv_fused":
         io = adv_fused_io(run, A, op)
-    elif name.endswith("_xy") and name[:-3] in kernel_io():  # an update, then its output's periodic halo cells
+    elif name == "massflux_div":
+        io = [(1, True, False), (2, True, False), (3, True, False), (6, False, True), (7, False, True),
+              (8, False, True), (10, False, True)]
+    elif name.endswith("_xy") and name[:-3] in kernel_io():  # an update, then its outputs' periodic halo cells
         halo = ((run.nx + 7) * (run.ny + 7) - run.nx * run.ny) * (run.nz + 1)
-        return 4.0 * (count(kernel_io()[name[:-3]]) * pts + halo)
+        return 4.0 * (count(kernel_io()[name[:-3]]) * pts + halo * (4 if name == "turb_km_kh_xy" else 1))
     else:
         io = kernel_io().get(name)
     if io is None:

@@ -93,15 +93,17 @@ static int run_program(uint32_t* w, T* v, const char* const* names, const uint32
       // the library's pointwise updates with the halo fill of their output
       // folded in: the update, then bc_periodic_x and bc_periodic_y of the
       // field it writes (parameter slot of that field)
-      static const std::pair<const char*, int> kFolded[] = {
-          {"rk_update_xy", 4}, {"tke_advance_xy", 4}, {"pdef_dum_xy", 10}, {"ppx_finish_xy", 2}};
-      int out = -1;
-      for (const auto& fo : kFolded)
-        if (nm == fo.first) out = fo.second;
-      if (out >= 0) {
+      struct Folded { const char* name; int out[4]; int n; };
+      static const Folded kFolded[] = {{"rk_update_xy", {4}, 1},    {"tke_advance_xy", {4}, 1},
+                                       {"pdef_dum_xy", {10}, 1},    {"ppx_finish_xy", {2}, 1},
+                                       {"turb_km_kh_xy", {4, 5, 6, 7}, 4}};
+      const Folded* fo = nullptr;
+      for (const auto& f : kFolded)
+        if (nm == f.name) fo = &f;
+      if (fo) {
         K<T> k1{w, v, A};
         call(find_kernel(nm.substr(0, nm.size() - 3)), k1);
-        fill_periodic(w, v, A, A[4 + out]);
+        for (int q = 0; q < fo->n; ++q) fill_periodic(w, v, A, A[4 + fo->out[q]]);
         continue;
       }
     }
@@ -171,6 +173,18 @@ static int run_program(uint32_t* w, T* v, const char* const* names, const uint32
         run1("bc_periodic_y", {4, nx + 7, nz + 1}, {{1, f}});
       }
       if (P(34) != 0 && w[P(34)] != 0u) fill_periodic(w, v, A, PP3D);  // the stage's pp3d fill, folded in
+      continue;
+    }
+    if (nm == "massflux_div") {
+      // massflux_from_state, then div_massflux of the fluxes it wrote
+      K<T> k1{w, v, A};
+      call(find_kernel("massflux_from_state"), k1);
+      uint32_t X[64] = {0};
+      X[0] = A[0]; X[1] = A[1]; X[2] = A[2];
+      X[4] = A[4]; X[34] = A[34];
+      X[5] = A[4 + 6]; X[6] = A[4 + 7]; X[7] = A[4 + 8]; X[8] = A[4 + 9]; X[9] = A[4 + 10];
+      K<T> k2{w, v, X};
+      call(find_kernel("div_massflux"), k2);
       continue;
     }
     if (nm == "ac_pp_accumulate") {

@@ -217,6 +217,13 @@ def test_fused_launches_write_no_field_their_neighbours_read(kw):
             assert off(5) not in written and not {off(q) for q in (6, 7, 8, 9)} & written, (op, name)
             assert off(2) not in written and off(3) not in written, (op, name)
             seen.add(name)
+        elif name == "massflux_div":  # u, v, w read at i+1, j+1, k+1; the fluxes and their divergence written
+            written = {off(q) for q in (6, 7, 8, 10)}
+            assert not {off(1), off(2), off(3)} & written, (op, name)
+            seen.add(name)
+        elif name == "pdef_dum_xy":  # flux and mass flux read at the next face; the output and its images written
+            assert off(10) not in {off(2), off(3), off(4), off(5)}, (op, name)
+            seen.add(name)
         elif name == "add_fields":  # independent triples
             n = word(op, 13)
             srcs, dsts = {off(1 + q) for q in range(n)}, [off(5 + q) for q in range(n)]

@@ -156,6 +156,39 @@ __global__ __launch_bounds__(256) void k_massflux_from_state(uint32_t* M, KArg a
   rrw[c] = (k == 0 || k == g.nz) ? 0.0f : rf0[k] * w[c];
 }
 
+// massflux_from_state then div_massflux (div_massflux.wgsl:125-147) in one
+// launch: each cell stores its mass fluxes and forms the divergence from its
+// own and its i+1, j+1, k+1 neighbours' fluxes, each the product
+// massflux_from_state stores there (so the same bits), instead of reading them
+// back.  p: 1-8 as massflux_from_state's, 9 mh, 10 divx.  Grid: whole.
+__global__ __launch_bounds__(256) void k_massflux_div(uint32_t* M, KArg a) {
+  TE_TID3
+  const Dims g = dims(a.q[0], 0);
+  const int i = gx - 3, j = gy - 3, k = gz;
+  if (i > g.nx + 3 || j > g.ny + 3 || k > g.nz) return;
+  const float *u = fq(a.q[1]), *v = fq(a.q[2]), *w = fq(a.q[3]), *rho0 = fq(a.q[4]), *rf0 = fq(a.q[5]),
+              *mh = fq(a.q[9]), *hm = fq(a.q[30]);
+  const int64_t sj = g.sJ, sk = g.sk(), c = g.idx(i, j, k);
+  auto fw = [&](int kk, int64_t cc) { return (kk == 0 || kk == g.nz) ? 0.0f : rf0[kk] * w[cc]; };
+  const bool div = i <= g.nx + 2 && j <= g.ny + 2 && k < g.nz;
+  const float r = k < g.nz ? rho0[k] : 0.0f;
+  const float uc = k < g.nz ? u[c] : 0.0f, vc = k < g.nz ? v[c] : 0.0f;
+  const float up = div ? u[c + 1] : 0.0f, vp = div ? v[c + sj] : 0.0f;
+  const float mw = fw(k, c), mwp = div ? fw(k + 1, c + sk) : 0.0f;
+  const float mu = r * uc, mv = r * vc;
+  // the divergence formed before the first store (its profile and metric
+  // loads go out with the others)
+  const float d = div ? fma_((mwp - mw) * g.rdz, mh[k],
+                             fma_((r * up - mu) * g.rdx, hm_uh(hm, g, i), ((r * vp - mv) * g.rdy) * hm_vh(hm, g, j)))
+                      : 0.0f;
+  if (k < g.nz) {
+    fq(a.q[6])[c] = mu;
+    fq(a.q[7])[c] = mv;
+  }
+  fq(a.q[8])[c] = mw;
+  if (div) fq(a.q[10])[c] = d;
+}
+
 // velocity_to_massflux.wgsl:111-134
 __global__ __launch_bounds__(256) void k_velocity_to_massflux(uint32_t* M, KArg a) {
   {
@@ -392,14 +425,17 @@ __global__ __launch_bounds__(256) void k_pressure_tendency(uint32_t* M, KArg a) 
     return;
   }
   float qvten = qvt[c];
+  // every operand loaded with the tendencies (one wait for memory), whether
+  // or not the cell takes the update
+  const float thc = th[c], qvc = qv[c], rhoc = rho[c], ppic = ppi[c], qp = hasDiabatic != 0 ? qpten[c] : 0.0f;
   float p = 0.0f;
   if (ab(dt * tht) > tem || ab(dt * qvten) > 1e-14f) {
-    float thnew = fma_(dt, tht, th[c]);
-    float qvnew = fma_(dt, qvten, qv[c]);
-    float pinew = fast_pow(((rho[c] * (th0[k] + thnew)) * fma_(RV, qvnew, RD)) * rp00, rddcv) - pi0[k];
-    p = (pinew - ppi[c]) / dt;
+    float thnew = fma_(dt, tht, thc);
+    float qvnew = fma_(dt, qvten, qvc);
+    float pinew = fast_pow(((rhoc * (th0[k] + thnew)) * fma_(RV, qvnew, RD)) * rp00, rddcv) - pi0[k];
+    p = (pinew - ppic) / dt;
   }
-  if (hasDiabatic != 0) p = p + qpten[c];
+  if (hasDiabatic != 0) p = p + qp;
   ppten[c] = p;
 }
 

@@ -245,10 +245,14 @@ __global__ __launch_bounds__(256) void k_turb_nm(uint32_t* M, KArg a) {
   nm[c] = out;
 }
 
-// turb_km_kh.wgsl:125-187
-__global__ __launch_bounds__(256) void k_turb_km_kh(uint32_t* M, KArg a) {
+// turb_km_kh.wgsl:125-187; IMG: then the halo fill of kmh, kmv, khh, khv on
+// a periodic domain (nx, ny >= 4; timestep.js:1593 haloFill): each cell
+// writes its periodic images, the cells of levels 1 and nz-1 also the images
+// of levels 0 and nz (not written here; no thread of the launch reads K).
+template <bool IMG>
+__device__ __forceinline__ void turb_km_kh_body(uint32_t* M, const KArg& a) {
   TE_TID3
-  Dims g = dims(a.q[0], 0);
+  const Dims g = dims(a.q[0], 0);
   int i = gx, j = gy, k = gz + 1;
   if (i >= g.nx || j >= g.ny || k > g.nz - 1) return;
   const float *tkea = fq(a.q[1]), *nm = fq(a.q[2]), *mf = fq(a.q[3]), *hm = fq(a.q[30]);
@@ -287,7 +291,27 @@ __global__ __launch_bounds__(256) void k_turb_km_kh(uint32_t* M, KArg a) {
   if (nmc > nm_min) prinv = mn(fma_(2.0f * len, rgrdscl, 1.0f), 3.0f);
   khh[c] = km * prinv;
   khv[c] = km * prinv;
+  if (IMG) {
+    float* F[4] = {kmh, kmv, khh, khv};
+    const float v[4] = {km, km, km * prinv, km * prinv};
+    const int64_t sk = g.sk(), e = k == 1 ? -sk : (k == g.nz - 1 ? sk : 0);
+    const int64_t e2 = k == 1 && k == g.nz - 1 ? sk : 0;  // nz = 2: level 1 is next to both
+    float ev[4], ev2[4];
+    for (int q = 0; q < 4; ++q) {
+      ev[q] = e != 0 ? F[q][c + e] : 0.0f;
+      ev2[q] = e2 != 0 ? F[q][c + e2] : 0.0f;
+    }
+    periodic_images(g, i, j, [&](int64_t o) {
+      for (int q = 0; q < 4; ++q) {
+        F[q][c + o] = v[q];
+        if (e != 0) F[q][c + e + o] = ev[q];
+        if (e2 != 0) F[q][c + e2 + o] = ev2[q];
+      }
+    });
+  }
 }
+__global__ __launch_bounds__(256) void k_turb_km_kh(uint32_t* M, KArg a) { turb_km_kh_body<false>(M, a); }
+__global__ __launch_bounds__(256) void k_turb_km_kh_xy(uint32_t* M, KArg a) { turb_km_kh_body<true>(M, a); }
 
 // turb_dissipative_heating.wgsl:139-194
 __global__ __launch_bounds__(256) void k_turb_dissipative_heating(uint32_t* M, KArg a) {

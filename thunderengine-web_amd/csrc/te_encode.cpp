@@ -462,10 +462,11 @@ void Encoder::turbulence() {
     for (const Bind& bd : bs) o.A[4 + bd.idx] = (uint32_t)bd.off;
     E.ops.back() = o;
   }
-  op("turb_km_kh", full,
+  op(foldHalo ? "turb_km_kh_xy" : "turb_km_kh", full,
      {fb(1, "tke_a"), fb(2, "nm"), rb(3, "mf"), fb(4, "kmh"), fb(5, "kmv"), fb(6, "khh"), fb(7, "khv"), fb(8, "lenscl"),
       fb(9, "dissten"), bb(10, F1(1.0f)), bb(11, F1(0.0f)), bb(12, F1(1.0f))});
-  for (const char* n : {"kmh", "kmv", "khh", "khv"}) haloFill(n);
+  if (!foldHalo)
+    for (const char* n : {"kmh", "kmv", "khh", "khv"}) haloFill(n);
   if (diss) {
     const char* ks[] = {"qv", "qc", "qr", "qi", "qs", "qg"};
     const float dz1 = (float)(v.zf[1] - v.zf[0]);
@@ -765,12 +766,18 @@ void Encoder::stages() {
           op("turb_add_tendency", whole,
              {fb(1, std::string(p) + "ten_t"), fb(2, std::string(p) + "ten"), bb(3, F1(1.0f))});
     }
-    op("massflux_from_state", whole,
-       {fb(1, "u"), fb(2, "v"), fb(3, "w"), rb(4, "rho0"), rb(5, "rf0"), fb(6, "rru"), fb(7, "rrv"), fb(8, "rrw")});
     auto computeDiv = [&] {
       op("div_massflux", whole, {fb(1, "rru"), fb(2, "rrv"), fb(3, "rrw"), rb(4, "mh"), fb(5, "divx")});
     };
-    computeDiv();
+    if (cfg.fuse == 1) {  // the mass fluxes and their divergence in one launch
+      op("massflux_div", whole,
+         {fb(1, "u"), fb(2, "v"), fb(3, "w"), rb(4, "rho0"), rb(5, "rf0"), fb(6, "rru"), fb(7, "rrv"), fb(8, "rrw"),
+          rb(9, "mh"), fb(10, "divx")});
+    } else {
+      op("massflux_from_state", whole,
+         {fb(1, "u"), fb(2, "v"), fb(3, "w"), rb(4, "rho0"), rb(5, "rf0"), fb(6, "rru"), fb(7, "rrv"), fb(8, "rrw")});
+      computeDiv();
+    }
     advect(nrk, "u", 2, "uten", true, "");
     advect(nrk, "v", 3, "vten", true, "");
     advect(nrk, "w", 4, "wten", true, "");

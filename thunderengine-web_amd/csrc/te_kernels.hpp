@@ -27,7 +27,7 @@
 #define TE_FUSED_LIST(X) X(bc_periodic_xy) X(zero_fields) X(copy_fields) X(ac_pp_accumulate) \
   X(adv_fused) X(add_fields) X(tendency_reset_add) X(acoustic_w_solve_lds) X(turb_scalar_vert_implicit_lds) X(turb_u_tendency_lds) \
   X(turb_v_tendency_lds) X(turb_w_tendency_lds) X(tke_transport_lds) X(rk_update_xy) X(tke_advance_xy) \
-  X(pdef_dum_xy) X(ppx_finish_xy)
+  X(pdef_dum_xy) X(ppx_finish_xy) X(turb_km_kh_xy) X(massflux_div)
 
 // Fused launches of 64 x AC_WAVES threads per workgroup (one acoustic
 // substep per launch, k_fused.hip).
