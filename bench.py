@@ -73,6 +73,8 @@ def adv_fused_io(run, A, op):
         io.append((35, True, False))
     if chain & 4:
         io.append((37, True, False))
+    if chain & 64:
+        io.append((42, True, False))
     if chain & (8 | 32):
         io += [(39, True, False), (40, False, True)]
     else:

@@ -276,6 +276,10 @@ static int run_program(uint32_t* w, T* v, const char* const* names, const uint32
       if (chain & 4) run1("turb_add_tendency", whole, {{1, A[4 + 37]}, {2, TEND}, {3, A[4 + 38]}});
       if (chain & 8) run1("rk_update", asz, {{1, A[4 + 1]}, {2, A[4 + 39]}, {3, TEND}, {4, A[4 + 40]}, {5, A[4 + 41]}});
       if (chain & 32) fill_periodic(w, v, A, A[4 + 40]);  // the updated field's halo fill, folded in
+      if (chain & 64) {  // buoyancy_apply on the assembled tendency of w
+        const uint32_t bg[3] = {nx, ny, std::max<uint32_t>(nz - 1, 1)};
+        run1("buoyancy_apply", bg, {{1, A[4 + 42]}, {2, A[4 + 15]}, {3, A[4 + 16]}, {4, TEND}});
+      }
       continue;
     }
     if (nm == "tendency_reset_add") {  // stage_tendency_reset, then turb_add_tendency x3

@@ -575,6 +575,9 @@ __device__ __forceinline__ void adv_fused_body(uint32_t* M, const KArg& a) {
   const float d3 = (chain & 1) ? fq(a.q[33])[c] : 0.0f, th3 = (chain & 1) ? fq(a.q[34])[c] : 0.0f;
   const float add1 = (chain & 2) ? fq(a.q[35])[c] : 0.0f, add2 = (chain & 4) ? fq(a.q[37])[c] : 0.0f;
   const float phi_a = (chain & 8) ? fq(a.q[39])[c] : 0.0f;
+  const bool buoy = (chain & 64) && k >= 1 && k <= g.nz - 1;
+  const float bm = buoy ? fq(a.q[42])[c - sk] : 0.0f, b0 = buoy ? fq(a.q[42])[c] : 0.0f;
+  const float bc1 = buoy ? fq(a.q[15])[k] : 0.0f, bc2 = buoy ? fq(a.q[16])[k] : 0.0f;
   const float fxl = F.template eval<0>(pxl, xlo, j, k), fxh = F.template eval<0>(pxh, xlo + 1, j, k);
   const float fyl = F.template eval<1>(pyl, i, ylo, k), fyh = F.template eval<1>(pyh, i, ylo + 1, k);
   const float fzl = F.template eval<2>(pzl, i, j, zlo), fzh = F.template eval<2>(pzh, i, j, zlo + 1);
@@ -587,7 +590,9 @@ __device__ __forceinline__ void adv_fused_body(uint32_t* M, const KArg& a) {
   // phi must not be the advected field, which other workgroups still read),
   // 32 with 8 on a periodic domain (nx, ny >= 4, cell-centred field): the
   // halo fill of phi after it, each cell's periodic images written by the
-  // cell, level nz's by the cell below it (no thread of the launch reads phi).
+  // cell, level nz's by the cell below it (no thread of the launch reads phi),
+  // 64 buoyancy_apply after the assembly of w (tend = tend + fma(c1[k],
+  // buoy[k-1], c2[k] * buoy[k]) at levels 1 .. nz-1; p42 buoy, p15 c1, p16 c2).
   // With an update in the chain the tendency itself is not stored.
   assemble_at<STC>(
       M, a, g, i, j, k, c, P,
@@ -601,6 +606,7 @@ __device__ __forceinline__ void adv_fused_body(uint32_t* M, const KArg& a) {
         if (chain & 1) v = fma_(-d3, th3, v);
         if (chain & 2) v = fma_(ldf(a.q[36], 0), add1, v);
         if (chain & 4) v = fma_(ldf(a.q[38], 0), add2, v);
+        if (buoy) v = v + fma_(bc1, bm, bc2 * b0);
         if (chain & 8) {
           float* phi = fq(a.q[40]);
           const float nv = fma_(ldf(a.q[41], 0), v, phi_a);

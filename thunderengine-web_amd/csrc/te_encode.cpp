@@ -576,7 +576,12 @@ void Encoder::advect(int nrk, const std::string& s, int stagger, const std::stri
     const bool chained = stagger == 1 && !accumulate && (!moistSgs || cacheMT) && !(moist && hasMicro && usePT);
     const bool update = chained && s != target;
     const bool tkeChained = stagger == 4 && s == "tke" && accumulate && into == "tketen_s";
-    if (tkeChained) {
+    if (cfg.fuse == 1 && stagger == 4 && s == "w" && into == "wten") {
+      // buoyancy_apply (after the advection of TKE, which touches neither
+      // wten nor buoy) joins the assembly of w
+      bs.push_back(bb(32, I1(64)));
+      bs.push_back(fb(42, "buoy"));
+    } else if (tkeChained) {
       // turb_zero(tketen_s), this assembly and turb_add_tendency(tketen) in
       // one launch; tke_advance (which writes the advected tke) follows
       bs.push_back(bb(32, I1(16 | 2)));
@@ -794,7 +799,8 @@ void Encoder::stages() {
       }
       if (!foldHalo) haloFill("tke");
     }
-    op("buoyancy_apply", {nx, ny, std::max(nz - 1, 1)}, {fb(1, "buoy"), rb(2, "c1"), rb(3, "c2"), fb(4, "wten")});
+    if (cfg.fuse != 1)  // fuse = 1: in the assembly of w (advect)
+      op("buoyancy_apply", {nx, ny, std::max(nz - 1, 1)}, {fb(1, "buoy"), rb(2, "c1"), rb(3, "c2"), fb(4, "wten")});
     op("stage_state_reset", whole,
        {fb(1, "u_a"), fb(2, "v_a"), fb(3, "w_a"), fb(4, "ppi"), fb(5, "u"), fb(6, "v"), fb(7, "w"), fb(8, "pp3d"),
         fb(9, "ppd"), fb(10, "ppx")});
